@@ -1,0 +1,175 @@
+/*
+ * ncrystal_amd.h -- C ABI of the MI355X (gfx950) implementation of NCrystal's
+ * per-neutron physics hot path.
+ *
+ * Plain C types only (pointers, sizes, doubles).  Each batched entry point
+ * mirrors the NCrystal v4.4.2 C API function named in its comment
+ * (ncrystal_core/include/NCrystal/cinterface/ncrystal.h, implementation in
+ * ncrystal_core/src/cinterface/ncrystal.cc) with the same argument meaning,
+ * output layout and error convention:
+ *   - outputs are poisoned on error (xs=-1, ekin_final=-1, mu=-999,
+ *     direction=(0,0,0)), ncrystal.cc:1095-1156,1240-1245;
+ *   - the error flag/message are queried with ncamd_error/ncamd_lasterror/
+ *     ncamd_clearerror (ncrystal.h:1031-1045); with haltonerror set (the
+ *     default, like ncrystal.cc:279-323) the process exits on error.
+ *
+ * Handles are opaque, reference counted, and not thread safe (one handle per
+ * thread, as for NCrystal's Scatter objects).  "_dev" entry points take
+ * device pointers and a hipStream_t (passed as void*) and are asynchronous;
+ * all others take host pointers and return after the results are copied back.
+ */
+#ifndef NCRYSTAL_AMD_H
+#define NCRYSTAL_AMD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { void* internal; } ncamd_process_t;
+
+/* ---------- library / errors (ncrystal.h:1031-1045, ncrystal.cc:279-323) ---------- */
+const char* ncamd_version_str(void);
+int ncamd_device_count(void);
+int ncamd_error(void);
+const char* ncamd_lasterror(void);
+const char* ncamd_lasterrortype(void);
+void ncamd_clearerror(void);
+void ncamd_sethaltonerror(int);
+void ncamd_setquietonerror(int);
+
+/* ---------- handle lifetime (ncrystal.h:672-676) ---------- */
+void ncamd_ref(ncamd_process_t);
+void ncamd_unref(ncamd_process_t*);
+int ncamd_refcount(ncamd_process_t);
+int ncamd_valid(ncamd_process_t);
+/* 0 isotropic powder-like process, 1 oriented single crystal */
+int ncamd_isoriented(ncamd_process_t);
+
+/* ---------- material compilation (init path) ---------- */
+
+/* hkl families of a cell with all angles 90 degrees, no space group.
+ * Replaces calculateHKLPlanes (ncrystal_core/src/extd_utils/NCFillHKL.cc:230-450)
+ * followed by the InfoBuilder sort (ncrystal_core/src/infobld/NCInfoBuilder.cc:452).
+ * The |F|^2 of every (h,k,l) is evaluated on the GPU; family grouping and
+ * sorting run on the host.  positions: 3*sum(natoms) fractional coordinates,
+ * species-major; csl in sqrt(barn); msd in Aa^2.  Returns the number of
+ * families (<= maxout written), or -1 (maxout too small; call again with the
+ * returned requirement in *nrequired), or -2 (more than 1e6 families).
+ */
+int64_t ncamd_hkl_orthogonal(double a, double b, double c, int nspecies, const int* natoms,
+                             const double* positions, const double* csl, const double* msd,
+                             double dcutoff, double fsquarecut, double merge_tolerance,
+                             int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp,
+                             double* out_fsq, int64_t* nrequired);
+
+/* Debye-model isotropic mean-squared displacement [Aa^2]
+ * (ncrystal_core/src/phys_utils/NCDebyeMSD.cc:27-36). */
+double ncamd_debye_msd(double debye_temp, double temperature, double mass_amu);
+
+/* PowderBragg from (d-spacing, |F|^2*multiplicity) pairs; replaces
+ * PowderBragg(double v0_times_natoms, VectDFM&&)
+ * (ncrystal_core/src/powderbragg/NCPowderBragg.cc:117-120,68-110).
+ * fp32_tables != 0 additionally builds FP32 Kahan-compensated cumulative sums
+ * used by ncamd_crosssection_nonoriented_many_fp32. */
+ncamd_process_t ncamd_create_powderbragg(double v0_times_natoms, const double* dspacings,
+                                         const double* fsq_x_mult, int64_t n, int fp32_tables);
+
+/* Incoherent elastic, replaces ElIncScatter(msd, bixs, scale)
+ * (ncrystal_core/src/elincscatter/NCElIncScatter.cc:171-180). */
+ncamd_process_t ncamd_create_elincscatter(int n, const double* msd, const double* bound_incoh_xs,
+                                          const double* scale);
+
+/* S(alpha,beta) scatter, replaces SABScatter(SABData&&, const VectD& egrid)
+ * (ncrystal_core/src/sabscatter/NCSABScatter.cc:68-80) with the
+ * SABIntegrator table build (ncrystal_core/src/sab/NCSABIntegrator.cc:283-559)
+ * running on the GPU in FP64.  sab[ibeta*nalpha+ialpha].  egrid: n_egrid==0
+ * automatic; 3 values {emin,emax,npts} with 0 meaning automatic; >3 explicit. */
+ncamd_process_t ncamd_create_sabscatter(int nalpha, int nbeta, const double* alpha,
+                                        const double* beta, const double* sab,
+                                        double temperature, double mass_amu,
+                                        double sigma_bound, double suggested_emax,
+                                        int n_egrid, const double* egrid);
+
+/* Composition of processes with scales, replaces ProcComposition
+ * (ncrystal_core/src/interfaces/NCProcImpl.cc:110-389).  At most one
+ * process of each kind; order is kept (it fixes the component pick). */
+ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, const double* scales);
+
+/* read back tables (for tests and tools) */
+int64_t ncamd_powderbragg_tables(ncamd_process_t, double* e2d, double* fdm, float* fdm32, int64_t maxn);
+int ncamd_sab_egrid(ncamd_process_t, double* egrid, double* xs, int maxn);
+int ncamd_sab_derived(ncamd_process_t, double* logsab, double* cumul);
+/* per energy-grid point: kind, ibeta_low, nvals, first-bin value; arrays of
+ * the beta distribution (x, y, cdf; nvals each) and alpha infos
+ * (8 doubles + 2 ints per beta, nvals-1 entries). */
+int ncamd_sab_sampler(ncamd_process_t, int iegrid, int* kind, int* ibeta_low, int* nvals,
+                      double* firstbin, double* bx, double* by, double* bcdf,
+                      double* info_dbl, int* info_idx);
+
+/* ---------- RNG (ncrystal.h:1067-1087) ----------
+ * Each process handle owns a counter-based stream: (seed, call counter).
+ * Every sampling call consumes one call-counter value; neutron k of a call
+ * uses Philox4x32-10 with key=seed, counter=(draw, call, k). */
+void ncamd_setrngseed(ncamd_process_t, uint64_t seed);
+void ncamd_getrngstate(ncamd_process_t, uint64_t* seed, uint64_t* callcounter);
+void ncamd_setrngstate(ncamd_process_t, uint64_t seed, uint64_t callcounter);
+
+/* ---------- batched per-neutron calls (host pointers) ---------- */
+
+/* ncrystal_crosssection_nonoriented_many (ncrystal.h:1289-1293,
+ * ncrystal.cc:1107-1141): results[irepeat*n_ekin+i] = xs(ekin[i]) [barn]. */
+void ncamd_crosssection_nonoriented_many(ncamd_process_t, const double* ekin,
+                                         unsigned long n_ekin, unsigned long repeat,
+                                         double* results);
+/* same with FP32 PowderBragg cumulative tables (FP64 thresholds/searches) */
+void ncamd_crosssection_nonoriented_many_fp32(ncamd_process_t, const double* ekin,
+                                              unsigned long n_ekin, unsigned long repeat,
+                                              double* results);
+
+/* ncrystal_samplescatterisotropic_many (ncrystal.h:1295-1301,
+ * ncrystal.cc:1201-1246): out index irepeat*n_ekin+i. */
+void ncamd_samplescatterisotropic_many(ncamd_process_t, const double* ekin,
+                                       unsigned long n_ekin, unsigned long repeat,
+                                       double* results_ekin, double* results_mu);
+
+/* ncrystal_samplescatter_many (ncrystal.h:1303-1309, ncrystal.cc:1248-1282):
+ * one energy and direction, repeated. */
+void ncamd_samplescatter_many(ncamd_process_t, double ekin, const double (*direction)[3],
+                              unsigned long repeat, double* results_ekin, double* results_dirx,
+                              double* results_diry, double* results_dirz);
+
+/* New: per-neutron directions (NCrystal's Python vectorises this in Python,
+ * ncrystal_python/src/NCrystal/_chooks.py:579-593).  dirs/out_dirs: 3*n. */
+void ncamd_crosssection_many_dirs(ncamd_process_t, const double* ekin, const double* dirs,
+                                  unsigned long n, double* results);
+void ncamd_samplescatter_many_dirs(ncamd_process_t, const double* ekin, const double* dirs,
+                                   unsigned long n, double* results_ekin, double* results_dirs);
+
+/* ---------- batched per-neutron calls (device pointers, async on stream) ----------
+ * Same semantics; RNG call counter advanced at enqueue time.  Errors raised
+ * inside kernels are counted on the device and reported by
+ * ncamd_dev_check_errors (which synchronises the stream). */
+void ncamd_dev_crosssection_nonoriented_many(ncamd_process_t, const double* d_ekin,
+                                             unsigned long n_ekin, unsigned long repeat,
+                                             double* d_results, void* stream);
+void ncamd_dev_crosssection_nonoriented_many_fp32(ncamd_process_t, const double* d_ekin,
+                                                  unsigned long n_ekin, unsigned long repeat,
+                                                  double* d_results, void* stream);
+void ncamd_dev_samplescatterisotropic_many(ncamd_process_t, const double* d_ekin,
+                                           unsigned long n_ekin, unsigned long repeat,
+                                           double* d_results_ekin, double* d_results_mu,
+                                           void* stream);
+void ncamd_dev_samplescatter_many_dirs(ncamd_process_t, const double* d_ekin,
+                                       const double* d_dirs, unsigned long n,
+                                       double* d_results_ekin, double* d_results_dirs,
+                                       void* stream);
+/* S(alpha,beta) only: raw (alpha,beta) samples of the plain sampler */
+void ncamd_dev_sab_sample_alphabeta(ncamd_process_t, const double* d_ekin, unsigned long n,
+                                    double* d_alpha, double* d_beta, void* stream);
+/* number of neutrons whose sampling failed since the last check (syncs stream) */
+int64_t ncamd_dev_check_errors(ncamd_process_t, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

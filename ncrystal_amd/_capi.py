@@ -1,0 +1,149 @@
+"""ctypes bindings of libncrystal_amd.so (C ABI in include/ncrystal_amd.h).
+
+Mirrors how NCrystal's Python layer binds its C API
+(ncrystal_python/src/NCrystal/_chooks.py:65-78 load, :134-191 error
+handling): the library is told not to halt or print on errors, and every
+call is followed by an error check that raises a Python exception.
+The library is always the in-tree build; a missing library is an error
+(there is no CPU substitute).
+"""
+import ctypes
+import pathlib
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIBPATH = _HERE / "libncrystal_amd.so"
+
+
+class NCAMDError(RuntimeError):
+    pass
+
+
+class NCBadInput(NCAMDError):
+    pass
+
+
+class NCCalcError(NCAMDError):
+    pass
+
+
+class NCLogicError(NCAMDError):
+    pass
+
+
+class NCNoDevice(NCAMDError):
+    pass
+
+
+_ERRMAP = {"BadInput": NCBadInput, "CalcError": NCCalcError, "LogicError": NCLogicError,
+           "NoDevice": NCNoDevice}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_ul = ctypes.c_ulong
+_vp = ctypes.c_void_p
+
+
+class Handle(ctypes.Structure):
+    _fields_ = [("internal", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBPATH.exists():
+        raise NCAMDError(f"{LIBPATH} is not built (run ncrystal_amd.build.build())")
+    L = ctypes.CDLL(str(LIBPATH))
+    sig = {
+        "ncamd_version_str": (ctypes.c_char_p, []),
+        "ncamd_device_count": (ctypes.c_int, []),
+        "ncamd_error": (ctypes.c_int, []),
+        "ncamd_lasterror": (ctypes.c_char_p, []),
+        "ncamd_lasterrortype": (ctypes.c_char_p, []),
+        "ncamd_clearerror": (None, []),
+        "ncamd_sethaltonerror": (None, [ctypes.c_int]),
+        "ncamd_setquietonerror": (None, [ctypes.c_int]),
+        "ncamd_ref": (None, [Handle]),
+        "ncamd_unref": (None, [ctypes.POINTER(Handle)]),
+        "ncamd_refcount": (ctypes.c_int, [Handle]),
+        "ncamd_valid": (ctypes.c_int, [Handle]),
+        "ncamd_isoriented": (ctypes.c_int, [Handle]),
+        "ncamd_debye_msd": (ctypes.c_double, [ctypes.c_double] * 3),
+        "ncamd_hkl_orthogonal": (ctypes.c_int64, [ctypes.c_double] * 3 + [ctypes.c_int, _ip, _dp, _dp, _dp,
+                                                                         ctypes.c_double, ctypes.c_double,
+                                                                         ctypes.c_double, ctypes.c_int64, _ip,
+                                                                         _ip, _dp, _dp, _i64p]),
+        "ncamd_create_powderbragg": (Handle, [ctypes.c_double, _dp, _dp, ctypes.c_int64, ctypes.c_int]),
+        "ncamd_create_elincscatter": (Handle, [ctypes.c_int, _dp, _dp, _dp]),
+        "ncamd_create_sabscatter": (Handle, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                             _dp]),
+        "ncamd_create_composition": (Handle, [ctypes.c_int, ctypes.POINTER(Handle), _dp]),
+        "ncamd_powderbragg_tables": (ctypes.c_int64, [Handle, _dp, _dp, _fp, ctypes.c_int64]),
+        "ncamd_sab_egrid": (ctypes.c_int, [Handle, _dp, _dp, ctypes.c_int]),
+        "ncamd_sab_derived": (ctypes.c_int, [Handle, _dp, _dp]),
+        "ncamd_sab_sampler": (ctypes.c_int, [Handle, ctypes.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _ip]),
+        "ncamd_setrngseed": (None, [Handle, ctypes.c_uint64]),
+        "ncamd_getrngstate": (None, [Handle, _u64p, _u64p]),
+        "ncamd_setrngstate": (None, [Handle, ctypes.c_uint64, ctypes.c_uint64]),
+        "ncamd_crosssection_nonoriented_many": (None, [Handle, _dp, _ul, _ul, _dp]),
+        "ncamd_crosssection_nonoriented_many_fp32": (None, [Handle, _dp, _ul, _ul, _dp]),
+        "ncamd_samplescatterisotropic_many": (None, [Handle, _dp, _ul, _ul, _dp, _dp]),
+        "ncamd_samplescatter_many": (None, [Handle, ctypes.c_double, ctypes.POINTER(ctypes.c_double * 3), _ul,
+                                            _dp, _dp, _dp, _dp]),
+        "ncamd_crosssection_many_dirs": (None, [Handle, _dp, _dp, _ul, _dp]),
+        "ncamd_samplescatter_many_dirs": (None, [Handle, _dp, _dp, _ul, _dp, _dp]),
+        "ncamd_dev_crosssection_nonoriented_many": (None, [Handle, _vp, _ul, _ul, _vp, _vp]),
+        "ncamd_dev_crosssection_nonoriented_many_fp32": (None, [Handle, _vp, _ul, _ul, _vp, _vp]),
+        "ncamd_dev_samplescatterisotropic_many": (None, [Handle, _vp, _ul, _ul, _vp, _vp, _vp]),
+        "ncamd_dev_samplescatter_many_dirs": (None, [Handle, _vp, _vp, _ul, _vp, _vp, _vp]),
+        "ncamd_dev_sab_sample_alphabeta": (None, [Handle, _vp, _ul, _vp, _vp, _vp]),
+        "ncamd_dev_check_errors": (ctypes.c_int64, [Handle, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    L.ncamd_sethaltonerror(0)
+    L.ncamd_setquietonerror(1)
+    _lib = L
+    return L
+
+
+def check():
+    L = lib()
+    if L.ncamd_error():
+        msg = L.ncamd_lasterror().decode()
+        typ = L.ncamd_lasterrortype().decode()
+        L.ncamd_clearerror()
+        raise _ERRMAP.get(typ, NCAMDError)(f"[{typ}] {msg}")
+
+
+def call(name, *args):
+    r = getattr(lib(), name)(*args)
+    check()
+    return r
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def fptr(a):
+    return a.ctypes.data_as(_fp)
+
+
+def as_f64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))

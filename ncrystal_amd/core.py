@@ -1,0 +1,234 @@
+"""Python mirror of NCrystal's Process/Scatter interface for the GPU path.
+
+Method names, argument meaning and output shapes follow
+ncrystal_python/src/NCrystal/core.py (Process.crossSectionIsotropic /
+xsect :1277-1310, Scatter.sampleScatterIsotropic :1492-1504,
+Scatter.sampleScatter :1478-1489); the heavy lifting is one batched call
+into libncrystal_amd per method call.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi as C
+from .synth import WL2EKIN
+
+
+def wl2ekin(wl):
+    wl = np.asarray(wl, dtype=np.float64)
+    return WL2EKIN / (wl * wl)
+
+
+class Process:
+    """Owns one ncamd_process_t handle (reference counted on the C side)."""
+
+    def __init__(self, handle, name="process", keep=()):
+        if not handle.internal:
+            C.check()
+            raise C.NCAMDError("process creation failed")
+        self._h = handle
+        self.name = name
+        self._keep = keep
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        try:
+            if self._h is not None and self._h.internal:
+                C.lib().ncamd_unref(ctypes.byref(self._h))
+        except Exception:
+            pass
+
+    def refCount(self):
+        return C.call("ncamd_refcount", self._h)
+
+    # --- cross sections
+    def crossSectionIsotropic(self, ekin, repeat=None):
+        scalar = np.ndim(ekin) == 0 and repeat is None
+        e = C.as_f64(np.atleast_1d(ekin))
+        rep = 1 if repeat is None else int(repeat)
+        out = np.empty(e.size * rep)
+        C.call("ncamd_crosssection_nonoriented_many", self._h, C.dptr(e), e.size, rep, C.dptr(out))
+        if scalar:
+            return float(out[0])
+        return out if repeat is not None else out.reshape(np.shape(ekin))
+
+    def crossSectionIsotropic_fp32(self, ekin):
+        e = C.as_f64(np.atleast_1d(ekin))
+        out = np.empty(e.size)
+        C.call("ncamd_crosssection_nonoriented_many_fp32", self._h, C.dptr(e), e.size, 1, C.dptr(out))
+        return out
+
+    def crossSection(self, ekin, direction):
+        e = C.as_f64(np.atleast_1d(ekin))
+        d = C.as_f64(np.broadcast_to(np.asarray(direction, dtype=np.float64), (e.size, 3)))
+        out = np.empty(e.size)
+        C.call("ncamd_crosssection_many_dirs", self._h, C.dptr(e), C.dptr(d), e.size, C.dptr(out))
+        return out
+
+    def xsect(self, ekin=None, wl=None, direction=None, repeat=None):
+        if (ekin is None) == (wl is None):
+            raise C.NCBadInput("specify exactly one of ekin and wl")
+        e = wl2ekin(wl) if wl is not None else np.asarray(ekin, dtype=np.float64)
+        if direction is None:
+            return self.crossSectionIsotropic(e, repeat=repeat)
+        return self.crossSection(e, direction)
+
+    # --- device-resident batched calls (pointers to device memory, stream)
+    def dev_crossSectionIsotropic(self, d_ekin, n, d_out, stream=0, repeat=1, fp32=False):
+        fn = "ncamd_dev_crosssection_nonoriented_many_fp32" if fp32 else "ncamd_dev_crosssection_nonoriented_many"
+        C.call(fn, self._h, d_ekin, n, repeat, d_out, stream)
+
+
+class Scatter(Process):
+    def setRandomSeed(self, seed):
+        C.call("ncamd_setrngseed", self._h, int(seed))
+
+    def getRNGState(self):
+        s = ctypes.c_uint64()
+        c = ctypes.c_uint64()
+        C.call("ncamd_getrngstate", self._h, ctypes.byref(s), ctypes.byref(c))
+        return s.value, c.value
+
+    def setRNGState(self, seed, callcounter):
+        C.call("ncamd_setrngstate", self._h, int(seed), int(callcounter))
+
+    def sampleScatterIsotropic(self, ekin, repeat=None):
+        e = C.as_f64(np.atleast_1d(ekin))
+        rep = 1 if repeat is None else int(repeat)
+        oe = np.empty(e.size * rep)
+        om = np.empty(e.size * rep)
+        C.call("ncamd_samplescatterisotropic_many", self._h, C.dptr(e), e.size, rep, C.dptr(oe), C.dptr(om))
+        if np.ndim(ekin) == 0 and repeat is None:
+            return float(oe[0]), float(om[0])
+        return oe, om
+
+    def sampleScatter(self, ekin, direction, repeat=None):
+        if np.ndim(ekin) == 0 and np.ndim(direction) == 1:
+            rep = 1 if repeat is None else int(repeat)
+            oe, ox, oy, oz = (np.empty(rep) for _ in range(4))
+            d = (ctypes.c_double * 3)(*[float(x) for x in direction])
+            C.call("ncamd_samplescatter_many", self._h, float(ekin), ctypes.byref(d), rep, C.dptr(oe),
+                   C.dptr(ox), C.dptr(oy), C.dptr(oz))
+            if repeat is None:
+                return float(oe[0]), (float(ox[0]), float(oy[0]), float(oz[0]))
+            return oe, np.stack([ox, oy, oz], axis=1)
+        e = C.as_f64(np.atleast_1d(ekin))
+        d = C.as_f64(np.broadcast_to(np.asarray(direction, dtype=np.float64), (e.size, 3)))
+        oe = np.empty(e.size)
+        od = np.empty((e.size, 3))
+        C.call("ncamd_samplescatter_many_dirs", self._h, C.dptr(e), C.dptr(d), e.size, C.dptr(oe), C.dptr(od))
+        return oe, od
+
+    def dev_sampleScatterIsotropic(self, d_ekin, n, d_oe, d_omu, stream=0, repeat=1):
+        C.call("ncamd_dev_samplescatterisotropic_many", self._h, d_ekin, n, repeat, d_oe, d_omu, stream)
+
+    def dev_check_errors(self, stream=0):
+        return C.call("ncamd_dev_check_errors", self._h, stream)
+
+
+# ---------------------------------------------------------------- factories
+
+def debye_msd(debye_temp, temperature, mass):
+    return C.call("ncamd_debye_msd", float(debye_temp), float(temperature), float(mass))
+
+
+def hkl_orthogonal(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tolerance=1e-6):
+    """species: list of (csl [sqrt(barn)], msd [Aa^2], positions (n,3)).
+    Returns (hkl[n,3], mult[n], dspacing[n], fsquared[n]) sorted like
+    NCrystal's InfoBuilder."""
+    natoms = np.array([len(s[2]) for s in species], dtype=np.int32)
+    pos = C.as_f64(np.concatenate([np.asarray(s[2], dtype=np.float64).reshape(-1, 3) for s in species]).ravel())
+    csl = C.as_f64([s[0] for s in species])
+    msd = C.as_f64([s[1] for s in species])
+    maxout = 4096
+    req = ctypes.c_int64()
+    while True:
+        hkl = np.empty(3 * maxout, dtype=np.int32)
+        mult = np.empty(maxout, dtype=np.int32)
+        dsp = np.empty(maxout)
+        fsq = np.empty(maxout)
+        n = C.call("ncamd_hkl_orthogonal", a, b, c, len(species), C.iptr(natoms), C.dptr(pos), C.dptr(csl),
+                   C.dptr(msd), dcutoff, fsquarecut, merge_tolerance, maxout, C.iptr(hkl), C.iptr(mult),
+                   C.dptr(dsp), C.dptr(fsq), ctypes.byref(req))
+        if n == -1:
+            maxout = int(req.value)
+            continue
+        if n == -2:
+            raise C.NCCalcError(f"Combinatorics too great to reach dcutoff = {dcutoff} Aa")
+        if n < 0:
+            raise C.NCAMDError("hkl enumeration failed")
+        return hkl[:3 * n].reshape(n, 3).copy(), mult[:n].copy(), dsp[:n].copy(), fsq[:n].copy()
+
+
+def create_powderbragg(v0_times_natoms, dspacings, fsq_x_mult, fp32_tables=True):
+    d = C.as_f64(dspacings)
+    f = C.as_f64(fsq_x_mult)
+    h = C.call("ncamd_create_powderbragg", float(v0_times_natoms), C.dptr(d), C.dptr(f), d.size,
+               1 if fp32_tables else 0)
+    return Scatter(h, name="PowderBragg")
+
+
+def powderbragg_tables(proc):
+    n = C.call("ncamd_powderbragg_tables", proc.handle, None, None, None, 0)
+    e2d, fdm, f32 = np.empty(n), np.empty(n), np.empty(n, dtype=np.float32)
+    C.call("ncamd_powderbragg_tables", proc.handle, C.dptr(e2d), C.dptr(fdm), C.fptr(f32), n)
+    return e2d, fdm, f32
+
+
+def create_elincscatter(msd, bound_incoh_xs, scale):
+    m, b, s = C.as_f64(msd), C.as_f64(bound_incoh_xs), C.as_f64(scale)
+    h = C.call("ncamd_create_elincscatter", m.size, C.dptr(m), C.dptr(b), C.dptr(s))
+    return Scatter(h, name="ElIncScatter")
+
+
+class SABScatter(Scatter):
+    def egrid(self):
+        n = C.call("ncamd_sab_egrid", self._h, None, None, 0)
+        eg, xs = np.empty(n), np.empty(n)
+        C.call("ncamd_sab_egrid", self._h, C.dptr(eg), C.dptr(xs), n)
+        return eg, xs
+
+    def derived(self, na, nb):
+        ls = np.empty(na * nb)
+        cu = np.empty(na * nb)
+        C.call("ncamd_sab_derived", self._h, C.dptr(ls), C.dptr(cu))
+        return ls.reshape(nb, na), cu.reshape(nb, na)
+
+    def sampler(self, ie, nb):
+        kind, ibl, nv = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        fb = ctypes.c_double()
+        n = nb + 1
+        bx, by, bc = np.empty(n), np.empty(n), np.empty(n)
+        idbl = np.empty(8 * nb)
+        iidx = np.empty(2 * nb, dtype=np.int32)
+        C.call("ncamd_sab_sampler", self._h, ie, ctypes.byref(kind), ctypes.byref(ibl), ctypes.byref(nv),
+               ctypes.byref(fb), C.dptr(bx), C.dptr(by), C.dptr(bc), C.dptr(idbl), C.iptr(iidx))
+        res = dict(kind=kind.value, ibeta_low=ibl.value, nvals=nv.value, firstbin=fb.value)
+        if kind.value == 1:
+            k = nv.value
+            res.update(bx=bx[:k], by=by[:k], bcdf=bc[:k], info=idbl.reshape(-1, 8)[:k - 1],
+                       infoidx=iidx.reshape(-1, 2)[:k - 1])
+        return res
+
+    def dev_sample_alphabeta(self, d_ekin, n, d_a, d_b, stream=0):
+        C.call("ncamd_dev_sab_sample_alphabeta", self._h, d_ekin, n, d_a, d_b, stream)
+
+
+def create_sabscatter(alpha, beta, sab, temperature, mass_amu, sigma_bound, suggested_emax=0.0, egrid=None):
+    a, b = C.as_f64(alpha), C.as_f64(beta)
+    s = C.as_f64(sab).reshape(b.size, a.size)
+    eg = C.as_f64(egrid if egrid is not None else np.zeros(0))
+    h = C.call("ncamd_create_sabscatter", a.size, b.size, C.dptr(a), C.dptr(b), C.dptr(s), float(temperature),
+               float(mass_amu), float(sigma_bound), float(suggested_emax), eg.size,
+               C.dptr(eg) if eg.size else None)
+    return SABScatter(h, name="SABScatter")
+
+
+def create_composition(procs, scales):
+    arr = (C.Handle * len(procs))(*[p.handle for p in procs])
+    sc = C.as_f64(scales)
+    h = C.call("ncamd_create_composition", len(procs), arr, C.dptr(sc))
+    return Scatter(h, name="ProcComposition", keep=tuple(procs))

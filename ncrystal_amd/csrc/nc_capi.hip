@@ -1,0 +1,1102 @@
+// Host layer and C ABI of libncrystal_amd (declared in include/ncrystal_amd.h).
+//
+// Process handles own their device tables.  Material compilation (hkl
+// families, PowderBragg cumulative tables, S(alpha,beta) integration) runs
+// once per handle; every per-neutron evaluation runs in the HIP kernels of
+// nc_kernels.hip.  There is no CPU evaluation path: without a usable GPU every
+// entry point raises an error.
+//
+// Error convention and batched-call semantics follow
+// ncrystal_core/src/cinterface/ncrystal.cc:275-323 (setError/haltonerror) and
+// :1107-1282 (_many functions, poisoned outputs on error).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ncrystal_amd.h"
+#include "nc_kernels.hh"
+
+namespace ncamd {
+namespace {
+
+// ---------------------------------------------------------------- errors ---
+int g_quiet = 0, g_halt = 1, g_waserror = 0;
+char g_errmsg[512];
+char g_errtype[64];
+
+void set_error(const char* msg, const char* etype) noexcept {
+  strncpy(g_errmsg, msg, sizeof(g_errmsg) - 1);
+  strncpy(g_errtype, etype, sizeof(g_errtype) - 1);
+  g_errmsg[sizeof(g_errmsg) - 1] = '\0';
+  g_errtype[sizeof(g_errtype) - 1] = '\0';
+  g_waserror = 1;
+  if (!g_quiet) fprintf(stderr, "ncrystal_amd ERROR [%s]: %s\n", g_errtype, g_errmsg);
+  if (g_halt) {
+    fprintf(stderr, "ncrystal_amd terminating due to ERROR\n");
+    exit(1);
+  }
+}
+
+struct Error : std::runtime_error {
+  std::string type;
+  Error(const std::string& t, const std::string& m) : std::runtime_error(m), type(t) {}
+};
+[[noreturn]] void raise(const char* type, const std::string& msg) { throw Error(type, msg); }
+
+#define NCAMD_HIP(call)                                                                       \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) raise("HIPError", std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define NCAMD_TRY try {
+#define NCAMD_CATCH                                             \
+  }                                                             \
+  catch (const Error& e) { set_error(e.what(), e.type.c_str()); } \
+  catch (const std::exception& e) { set_error(e.what(), "std::exception"); }
+
+void ensure_device() {
+  static int ok = -1;
+  static std::mutex mtx;
+  std::lock_guard<std::mutex> lk(mtx);
+  if (ok == 1) return;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n < 1) {
+    ok = 0;
+    raise("NoDevice", "ncrystal_amd requires a gfx950 GPU; none is available (no CPU fallback exists)");
+  }
+  ok = 1;
+}
+
+// -------------------------------------------------------- device buffers ---
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  void alloc(size_t b) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    bytes = b;
+    if (b) NCAMD_HIP(hipMalloc(&p, b));
+  }
+  void ensure(size_t b) { if (b > bytes) alloc(b); }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+  template <class T> void upload(const T* h, size_t n) {
+    alloc(n * sizeof(T));
+    if (n) NCAMD_HIP(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  }
+};
+
+// -------------------------------------------------------------- numerics ---
+// Romberg + Debye MSD (ncrystal_core/src/utils/NCRomberg.cc:58-140,
+// ncrystal_core/src/phys_utils/NCDebyeMSD.cc:27-86); init-time scalar.
+template <class F, class A>
+double romberg(const F& f, const A& acc, double a, double b) {
+  double h = (b - a);
+  double fv[17];
+  const double delta = h * 0.0625;
+  for (int i = 0; i < 17; ++i) fv[i] = f(a + delta * (double)i);
+  h *= 0.5;
+  const double R00 = (fv[0] + fv[16]) * h;
+  const double R10 = h * fv[8] + 0.5 * R00;
+  const double R11 = (4. / 3.) * R10 + (-1. / 3.) * R00;
+  h *= 0.5;
+  const double R20 = h * (fv[4] + fv[12]) + 0.5 * R10;
+  const double R21 = (4. / 3.) * R20 + (-1. / 3.) * R10;
+  const double R22 = (16. / 15.) * R21 + (-1. / 15.) * R11;
+  h *= 0.5;
+  const double R30 = h * ((fv[2] + fv[6]) + (fv[10] + fv[14])) + 0.5 * R20;
+  const double R31 = (4. / 3.) * R30 + (-1. / 3.) * R20;
+  const double R32 = (16. / 15.) * R31 + (-1. / 15.) * R21;
+  const double R33 = (64. / 63.) * R32 + (-1. / 63.) * R22;
+  h *= 0.5;
+  const double R40 = h * (((fv[1] + fv[3]) + (fv[5] + fv[7])) + ((fv[9] + fv[11]) + (fv[13] + fv[15]))) + 0.5 * R30;
+  const double R41 = (4. / 3.) * R40 + (-1. / 3.) * R30;
+  const double R42 = (16. / 15.) * R41 + (-1. / 15.) * R31;
+  const double R43 = (64. / 63.) * R42 + (-1. / 63.) * R32;
+  const double R44 = (256. / 255.) * R43 + (-1. / 255.) * R33;
+  if (acc(4, R33, R44)) return R44;
+  double c5 = 0.0;
+  {
+    const double off = a + h * 0.5;
+    for (double i = 0; i < 16.0; ++i) c5 += f(off + h * i);
+  }
+  h *= 0.5;
+  const double R50 = h * c5 + 0.5 * R40;
+  const double R51 = (4. / 3.) * R50 + (-1. / 3.) * R40;
+  const double R52 = (16. / 15.) * R51 + (-1. / 15.) * R41;
+  const double R53 = (64. / 63.) * R52 + (-1. / 63.) * R42;
+  const double R54 = (256. / 255.) * R53 + (-1. / 255.) * R43;
+  const double R55 = (1024. / 1023.) * R54 + (-1. / 1023.) * R44;
+  if (acc(5, R44, R55)) return R55;
+  constexpr unsigned maxlevel = 16;
+  double cache1[maxlevel], cache2[maxlevel];
+  double *row_prev = cache1, *row = cache2;
+  row_prev[0] = R50; row_prev[1] = R51; row_prev[2] = R52;
+  row_prev[3] = R53; row_prev[4] = R54; row_prev[5] = R55;
+  unsigned nj = 16;
+  for (unsigned i = 6; i < maxlevel; ++i) {
+    const double hh = h;
+    h *= 0.5;
+    nj *= 2;
+    double c = 0.0;
+    {
+      const double off = a + h, nn = nj;
+      for (double j = 0; j < nn; ++j) c += f(off + hh * j);
+    }
+    row[0] = h * c + 0.5 * row_prev[0];
+    double n_k = 1.;
+    for (unsigned j = 0; j < i; ++j) {
+      n_k *= 4.0;
+      row[j + 1] = (n_k * row[j] - row_prev[j]) / (n_k - 1.0);
+    }
+    if (acc(i, row_prev[i - 1], row[i])) return row[i];
+    std::swap(row_prev, row);
+  }
+  raise("CalcError", "Romberg integration did not converge");
+}
+
+double debye_msd(double dt, double t, double am) {
+  if (!(dt > 0.0 && dt < 1e5) || !(t >= 0.0) || !(am >= 1.007 && am < 500))
+    raise("BadInput", "debye_msd: parameters out of range");
+  constexpr double hbar = 4.135667662e-15 * 0.159154943091895335768883763372514362034459646;
+  constexpr double c = 299792458e10;
+  const double kk = 3.0 * hbar * hbar * c * c / (931494095.17 * kBoltzmann);
+  const double scale = kk / (am * dt);
+  const double x = t / dt;
+  if (x < 1e-50) return scale * 0.25;
+  auto f = [](double xx) {
+    if (fabs(xx) < 1e-4) {
+      const double x2 = xx * xx;
+      return 1 - xx * 0.5 + x2 * 0.08333333333333333333333333333333 - x2 * x2 * 0.00138888888888888888888888888888888889;
+    }
+    if (xx <= 700) return xx / expm1(xx);
+    return (xx > 760 ? 0.0 : exp(-xx + log(-xx / expm1(-xx))));
+  };
+  auto acc = [](unsigned lvl, double prev, double est) { return lvl > 11 || (lvl > 7 && fabs(est - prev) < 1e-10); };
+  return scale * (0.25 + x * x * romberg(f, acc, 0.0, 1.0 / x));
+}
+
+// ------------------------------------------------------------- processes ---
+constexpr uint32_t kMagic = 0x6e63616dU;
+
+struct Proc {
+  uint32_t magic = kMagic;
+  int refcount = 1;
+  int kind = kKindNone;  // kKindPB, kKindElInc, kKindSAB or 0 for composition
+  uint64_t seed = 0;
+  uint64_t callctr = 0;
+  // owned device memory
+  std::vector<std::unique_ptr<DevBuf>> bufs;
+  DevBuf d_err;
+  // staging for the host-pointer API
+  DevBuf st_in, st_in2, st_out1, st_out2;
+  // component data
+  PBView pb{};
+  ElIncView ei{};
+  SABView sab{};
+  double pb_v0n = 0.0;
+  std::vector<double> pb_e2d_h, pb_fdm_h;
+  std::vector<float> pb_fdm32_h;
+  bool has_fp32 = false;
+  // S(alpha,beta) host copies for read-back
+  int sab_na = 0, sab_nb = 0;
+  std::vector<double> sab_egrid_h, sab_xs_h;
+  // composition
+  std::vector<Proc*> comps;
+  std::vector<double> scales;
+
+  DevBuf* newbuf() {
+    bufs.emplace_back(new DevBuf());
+    return bufs.back().get();
+  }
+  ~Proc() {
+    magic = 0;
+    for (auto* c : comps) {
+      if (c && --c->refcount == 0) delete c;
+    }
+  }
+};
+
+Proc* extract(ncamd_process_t h) {
+  Proc* p = static_cast<Proc*>(h.internal);
+  if (!p || p->magic != kMagic) raise("LogicError", "invalid ncamd_process_t handle");
+  return p;
+}
+
+ncamd_process_t wrap(Proc* p) {
+  ncamd_process_t h;
+  h.internal = p;
+  return h;
+}
+
+void init_errbuf(Proc* p) {
+  p->d_err.alloc(sizeof(unsigned long long));
+  NCAMD_HIP(hipMemset(p->d_err.p, 0, sizeof(unsigned long long)));
+}
+
+CompView make_view(Proc* p) {
+  CompView v{};
+  std::vector<std::pair<Proc*, double>> list;
+  if (p->kind == kKindNone) {
+    for (size_t i = 0; i < p->comps.size(); ++i) list.emplace_back(p->comps[i], p->scales[i]);
+  } else {
+    list.emplace_back(p, 1.0);
+  }
+  v.n = (int)list.size();
+  double lo = __builtin_huge_val(), hi = -__builtin_huge_val();
+  for (int i = 0; i < v.n; ++i) {
+    Proc* c = list[i].first;
+    v.kind[i] = c->kind;
+    v.scale[i] = list[i].second;
+    double dlo = 0.0, dhi = __builtin_huge_val();
+    switch (c->kind) {
+      case kKindPB:
+        v.pb = c->pb;
+        if (c->pb.n > 0) dlo = c->pb.threshold; else { dlo = __builtin_huge_val(); dhi = __builtin_huge_val(); }
+        break;
+      case kKindElInc: v.ei = c->ei; break;
+      case kKindSAB: v.sab = c->sab; break;
+      default: raise("LogicError", "bad component kind");
+    }
+    if (dlo < dhi) { lo = std::min(lo, dlo); hi = std::max(hi, dhi); }
+  }
+  v.domain_lo = lo;
+  v.domain_hi = hi;
+  return v;
+}
+
+// ------------------------------------------------------------ PowderBragg ---
+// init: ncrystal_core/src/powderbragg/NCPowderBragg.cc:68-110 (+ the exact-d
+// merge of :122-145)
+__global__ void k_fp32_kahan_cumul(const double* __restrict__ contrib, int n, float* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  float s = 0.0f, c = 0.0f;
+  for (int i = 0; i < n; ++i) {
+    const float x = (float)contrib[i];
+    const float t = s + x;
+    c += fabsf(s) >= fabsf(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+    out[i] = s + c;
+  }
+}
+
+Proc* create_pb(double v0n, const double* dsp, const double* fsqm, int64_t n, bool fp32) {
+  ensure_device();
+  if (!(v0n > 0)) raise("BadInput", "v0_times_natoms is not a positive number.");
+  std::vector<std::pair<double, double>> data;
+  data.reserve((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (fsqm[i] < 0) raise("CalcError", "Inconsistent data implies negative |F|^2*multiplicity.");
+    if (data.empty() || data.back().first != dsp[i]) data.emplace_back(dsp[i], fsqm[i]);
+    else data.back().second += fsqm[i];
+  }
+  double xsectfact = 0.5 / v0n;
+  xsectfact *= kWl2Ekin / (1.0 * 1.0);
+  std::sort(data.begin(), data.end(), std::greater<std::pair<double, double>>());
+  std::unique_ptr<Proc> p(new Proc());
+  p->kind = kKindPB;
+  p->pb_v0n = v0n;
+  std::vector<double> e2d, fdm, contrib;
+  StableSum s;
+  double prev = -__builtin_huge_val();
+  for (auto& it : data) {
+    if (!(it.first > 0.0)) raise("CalcError", "Inconsistent plane data implies non-positive (or NaN) d_spacing.");
+    const double c = it.first * it.second * xsectfact;
+    if (fabs(prev - it.first) < 1e-11) {
+      s.add(c);
+      fdm.back() = s.sum();
+      contrib.back() += c;
+    } else {
+      prev = it.first;
+      s.add(c);
+      fdm.push_back(s.sum());
+      contrib.push_back(c);
+      const double wl = 2.0 * it.first;
+      e2d.push_back(kWl2Ekin / (wl * wl));
+    }
+  }
+  if (fdm.empty() || fdm.back() <= 0.0) { fdm.clear(); e2d.clear(); contrib.clear(); }
+  p->pb.n = (int)fdm.size();
+  p->pb.threshold = fdm.empty() ? __builtin_huge_val() : e2d.front();
+  DevBuf* be = p->newbuf();
+  be->upload(e2d.data(), e2d.size());
+  DevBuf* bf = p->newbuf();
+  bf->upload(fdm.data(), fdm.size());
+  p->pb.e2d = be->as<double>();
+  p->pb.fdm = bf->as<double>();
+  p->pb.fdm32 = nullptr;
+  p->pb_e2d_h = e2d;
+  p->pb_fdm_h = fdm;
+  if (fp32 && !fdm.empty()) {
+    DevBuf* bc = p->newbuf();
+    bc->upload(contrib.data(), contrib.size());
+    DevBuf* b32 = p->newbuf();
+    b32->alloc(sizeof(float) * contrib.size());
+    hipLaunchKernelGGL(k_fp32_kahan_cumul, dim3(1), dim3(64), 0, 0, bc->as<double>(), (int)contrib.size(),
+                       b32->as<float>());
+    NCAMD_HIP(hipGetLastError());
+    p->pb_fdm32_h.resize(contrib.size());
+    NCAMD_HIP(hipMemcpy(p->pb_fdm32_h.data(), b32->p, sizeof(float) * contrib.size(), hipMemcpyDeviceToHost));
+    p->pb.fdm32 = b32->as<float>();
+    p->has_fp32 = true;
+  }
+  init_errbuf(p.get());
+  return p.release();
+}
+
+// ------------------------------------------------------------------ ElInc ---
+Proc* create_elinc(int n, const double* msd, const double* bixs, const double* scale) {
+  ensure_device();
+  if (n < 1 || n > kMaxElInc) raise("BadInput", "ElIncScatter supports 1..16 elements");
+  std::unique_ptr<Proc> p(new Proc());
+  p->kind = kKindElInc;
+  p->ei.n = n;
+  for (int i = 0; i < n; ++i) {
+    if (!(msd[i] >= 0.0 && msd[i] < 1e6) || !(bixs[i] >= 0.0 && bixs[i] < 1e6) || !(scale[i] >= 0.0 && scale[i] <= 1e6))
+      raise("BadInput", "ElIncScatter element data out of range");
+    p->ei.msd[i] = msd[i];
+    p->ei.bixs[i] = bixs[i] * scale[i];
+  }
+  init_errbuf(p.get());
+  return p.release();
+}
+
+// ------------------------------------------------------------------- SAB ---
+// egrid: ncrystal_core/src/sab/NCSABIntegrator.cc:147-281,
+// geomspace/logspace: ncrystal_core/src/utils/NCMath.cc:43-66
+std::vector<double> geomspace(double start, double stop, unsigned num) {
+  std::vector<double> v(num);
+  double ls = log10(start);
+  const double lst = log10(stop);
+  const double interval = (lst - ls) / (num - 1);
+  for (unsigned i = 0; i < num; ++i) { v[i] = pow(10.0, ls); ls += interval; }
+  v[num - 1] = pow(10.0, lst);
+  v[0] = start;
+  v[num - 1] = stop;
+  return v;
+}
+
+std::vector<double> erfc_table() {
+  std::vector<double> t(kErfcTabLen);
+  const int nedges = kErfcTabLen - 2;
+  t[0] = 2.0;
+  const double start = -2.0, stop = 9.0;
+  const unsigned nm1 = (unsigned)nedges - 1;
+  const double interval = (stop - start) / nm1;
+  for (unsigned i = 0; i < nm1; ++i) t[1 + i] = erfc(start + i * interval);
+  t[1 + nm1] = erfc(stop);
+  t[kErfcTabLen - 1] = 0.0;
+  return t;
+}
+
+Proc* create_sab(int na, int nb, const double* alpha, const double* beta, const double* sab, double T,
+                 double mass, double sigma_bound, double suggested_emax, int n_egrid, const double* egrid_in) {
+  ensure_device();
+  if (na < 2 || nb < 2) raise("BadInput", "S(alpha,beta) grids need at least 2 points");
+  if (na >= 65535) raise("BadInput", "alpha grid too large");
+  for (int i = 1; i < na; ++i) if (!(alpha[i] > alpha[i - 1])) raise("BadInput", "alpha grid not increasing");
+  for (int i = 1; i < nb; ++i) if (!(beta[i] > beta[i - 1])) raise("BadInput", "beta grid not increasing");
+  for (size_t i = 0; i < (size_t)na * nb; ++i)
+    if (!(sab[i] >= 0.0) || std::isinf(sab[i])) raise("BadInput", "S(alpha,beta) must be finite and non-negative");
+  if (!(T > 0.0) || !(mass > 0.0) || !(sigma_bound > 0.0)) raise("BadInput", "invalid T, mass or sigma");
+  std::unique_ptr<Proc> p(new Proc());
+  p->kind = kKindSAB;
+  p->sab_na = na;
+  p->sab_nb = nb;
+  SABView& s = p->sab;
+  s.na = na;
+  s.nb = nb;
+  s.T = T;
+  s.kT = kBoltzmann * T;
+  s.mass = mass;
+  {
+    const double r = mass / (kNeutronMassAmu + mass);
+    s.sigma_free = (r * r) * sigma_bound;
+    const double A = (1.0 / kNeutronMassAmu) * mass;
+    s.fg_ca = A / s.kT;
+  }
+  s.egrid_margin = 1.05;
+  DevBuf* ba = p->newbuf(); ba->upload(alpha, (size_t)na);
+  DevBuf* bb = p->newbuf(); bb->upload(beta, (size_t)nb);
+  DevBuf* bs = p->newbuf(); bs->upload(sab, (size_t)na * nb);
+  DevBuf* bsl = p->newbuf(); bsl->alloc(sizeof(double2) * (size_t)na * nb);
+  DevBuf* bcu = p->newbuf(); bcu->alloc(sizeof(double) * (size_t)na * nb);
+  SABBuildBufs B{};
+  B.na = na; B.nb = nb;
+  B.alpha = ba->as<double>(); B.beta = bb->as<double>(); B.sab = bs->as<double>();
+  B.sl = bsl->as<double2>(); B.cumul = bcu->as<double>();
+  B.kT = s.kT; B.sigma_bound = sigma_bound;
+  launch_sab_derive(B, 0);
+  NCAMD_HIP(hipGetLastError());
+
+  DevBuf tmpE, tmpX, tmpF;
+  auto xs_points = [&](const std::vector<double>& es, std::vector<double>& xs, std::vector<double>* fg) {
+    tmpE.upload(es.data(), es.size());
+    tmpX.alloc(sizeof(double) * es.size());
+    launch_sab_xs_points(B, tmpE.as<double>(), (int)es.size(), tmpX.as<double>(), 0);
+    NCAMD_HIP(hipGetLastError());
+    xs.resize(es.size());
+    NCAMD_HIP(hipMemcpy(xs.data(), tmpX.p, sizeof(double) * es.size(), hipMemcpyDeviceToHost));
+    if (fg) {
+      tmpF.alloc(sizeof(double) * es.size());
+      launch_fg_xs(s.sigma_free, s.fg_ca, tmpE.as<double>(), (int)es.size(), tmpF.as<double>(), 0);
+      NCAMD_HIP(hipGetLastError());
+      fg->resize(es.size());
+      NCAMD_HIP(hipMemcpy(fg->data(), tmpF.p, sizeof(double) * es.size(), hipMemcpyDeviceToHost));
+    }
+  };
+
+  std::vector<double> egrid;
+  if (n_egrid > 3) {
+    egrid.assign(egrid_in, egrid_in + n_egrid);
+  } else {
+    if (n_egrid != 0 && n_egrid != 3)
+      raise("BadInput", "SABIntegrator invalid energy grid. It must either be a complete array, empty, or {emin, emax, npts}");
+    double emin = 0, emax = 0;
+    unsigned npts = 0;
+    if (n_egrid == 3) {
+      emin = egrid_in[0]; emax = egrid_in[1];
+      npts = (unsigned)egrid_in[2];
+      if ((double)npts != egrid_in[2]) raise("BadInput", "egrid npts must be integral");
+    }
+    if (!(emin >= 0.0) || !(emax >= 0.0) || !(emax == 0.0 || emin == 0.0 || emax > emin))
+      raise("BadInput", "SABIntegrator invalid energy grid values");
+    if (npts == 0) npts = 300;
+    if (emax == 0.0 && suggested_emax > 0.0) emax = suggested_emax;
+    if (emax == 0.0) {
+      // determineEMax: candidates eul*0.95^k down to eul*1e-4
+      const double bmin = beta[0], amax = alpha[na - 1];
+      const double eul = s.kT * (bmin - amax) * (bmin - amax) / (4 * amax);
+      std::vector<double> cand;
+      for (double e = eul, elow = eul * 1e-4; e > elow; e *= 0.95) cand.push_back(e);
+      std::vector<double> xs, fg;
+      xs_points(cand, xs, &fg);
+      double prev = __builtin_huge_val();
+      emax = 0.0;
+      for (size_t k = 0; k < cand.size(); ++k) {
+        const double dist = fabs(xs[k] - fg[k]);
+        if (dist > prev) { emax = 0.95 * cand[k]; break; }
+        prev = dist;
+      }
+      if (!(emax > 0.0)) emax = 0.5 * eul;
+    }
+    if (emin == 0.0) {
+      // determineEMin: candidates e_upp*0.9 halved down to 1e-30*e_upp
+      const double e_upp = std::min(emax * 0.01, 0.01 * s.kT);
+      std::vector<double> cand;
+      double e = e_upp * 0.9;
+      cand.push_back(e);
+      while (true) {
+        const double en = 0.5 * e;
+        if (en <= 1e-30 * e_upp) break;
+        cand.push_back(en);
+        e = en;
+      }
+      std::vector<double> xs;
+      xs_points(cand, xs, nullptr);
+      double e_old = cand[0], f_old = sqrt(e_old) * xs[0];
+      emin = -1.0;
+      for (size_t k = 1; k < cand.size(); ++k) {
+        const double f_new = sqrt(cand[k]) * xs[k];
+        if (f_new == 0.0) { emin = 0.001 * e_upp; break; }
+        if (fabs(f_old / f_new - 1.0) < 1e-3) { emin = e_old; break; }
+        e_old = cand[k];
+        f_old = f_new;
+      }
+      if (emin < 0.0) emin = std::min(e_old, e_upp * 0.01);
+    } else if (!(emax > emin)) {
+      raise("BadInput", "energy grid does not have emax>emin");
+    }
+    egrid = geomspace(emin, emax, npts);
+  }
+  if (egrid.size() < 10) raise("BadInput", "SABIntegrator invalid energy grid - must have at least 10 points.");
+  for (size_t i = 1; i < egrid.size(); ++i)
+    if (!(egrid[i] > egrid[i - 1])) raise("BadInput", "SABIntegrator invalid energy grid - must be sorted");
+  if (!(egrid.front() > 0.0)) raise("BadInput", "SABIntegrator invalid energy grid - must be positive");
+
+  const int ne = (int)egrid.size();
+  s.ne = ne;
+  DevBuf* beg = p->newbuf(); beg->upload(egrid.data(), egrid.size());
+  DevBuf* bxs = p->newbuf(); bxs->alloc(sizeof(double) * ne);
+  DevBuf* bhdr = p->newbuf(); bhdr->alloc(sizeof(int4) * ne);
+  DevBuf* bfst = p->newbuf(); bfst->alloc(sizeof(double) * ne);
+  DevBuf* bcdf = p->newbuf(); bcdf->alloc(sizeof(double) * (size_t)ne * (nb + 1));
+  DevBuf* bxy = p->newbuf(); bxy->alloc(sizeof(double2) * (size_t)ne * (nb + 1));
+  DevBuf* binf = p->newbuf(); binf->alloc(sizeof(AInfo) * (size_t)ne * nb);
+  NCAMD_HIP(hipMemset(bcdf->p, 0, bcdf->bytes));
+  NCAMD_HIP(hipMemset(bxy->p, 0, bxy->bytes));
+  NCAMD_HIP(hipMemset(binf->p, 0, binf->bytes));
+  launch_sab_analyse(B, beg->as<double>(), ne, bxs->as<double>(), bhdr->as<int4>(), bfst->as<double>(),
+                     bcdf->as<double>(), bxy->as<double2>(), binf->as<AInfo>(), 0);
+  NCAMD_HIP(hipGetLastError());
+  std::vector<double> xs(ne);
+  NCAMD_HIP(hipMemcpy(xs.data(), bxs->p, sizeof(double) * ne, hipMemcpyDeviceToHost));
+  std::vector<int4> hdr(ne);
+  NCAMD_HIP(hipMemcpy(hdr.data(), bhdr->p, sizeof(int4) * ne, hipMemcpyDeviceToHost));
+  for (int i = 0; i < ne; ++i)
+    if (hdr[i].x < 0) raise("CalcError", "S(alpha,beta) beta distribution has no area");
+  // high-E constants (NCSABXSProvider.cc:35-51, NCSABSampler.cc:40-57)
+  {
+    std::vector<double> em{egrid.back()}, dummy, fg;
+    tmpE.upload(em.data(), 1);
+    tmpF.alloc(sizeof(double));
+    launch_fg_xs(s.sigma_free, s.fg_ca, tmpE.as<double>(), 1, tmpF.as<double>(), 0);
+    NCAMD_HIP(hipGetLastError());
+    double extxs = 0.0;
+    NCAMD_HIP(hipMemcpy(&extxs, tmpF.p, sizeof(double), hipMemcpyDeviceToHost));
+    const double emax = egrid.back();
+    s.k_ext = (xs.back() - extxs) * emax;
+    s.k1 = xs.back() * emax;
+    s.k2 = extxs * emax;
+  }
+  std::vector<double> et = erfc_table();
+  DevBuf* bet = p->newbuf(); bet->upload(et.data(), et.size());
+  s.alpha = ba->as<double>();
+  s.beta = bb->as<double>();
+  s.sl = bsl->as<double2>();
+  s.cumul = bcu->as<double>();
+  s.egrid = beg->as<double>();
+  s.xs = bxs->as<double>();
+  s.shdr = bhdr->as<int4>();
+  s.sfirst = bfst->as<double>();
+  s.scdf = bcdf->as<double>();
+  s.sxy = bxy->as<double2>();
+  s.sinfo = binf->as<AInfo>();
+  s.erfc_tab = bet->as<double>();
+  p->sab_egrid_h = egrid;
+  p->sab_xs_h = xs;
+  init_errbuf(p.get());
+  return p.release();
+}
+
+// ------------------------------------------------------------------- hkl ---
+// ncrystal_core/src/extd_utils/NCFillHKL.cc:34-49 (keygen), :230-450 (loop and
+// family grouping); ncrystal_core/src/utils/NCLatticeUtils.cc:177-216
+// (estimateHKLRange); sort ncrystal_core/src/infobld/NCInfoBuilder.cc:98-117,452;
+// HKL ordering ncrystal_core/include/NCrystal/core/NCTypes.hh:978-981.
+unsigned famkey(double fsq, double dsp) {
+  const int ef = (int)std::ceil(std::log10(fsq));
+  const double mf = fsq * std::pow(10.0, (double)(-ef));
+  const int ed = (int)std::ceil(std::log10(dsp));
+  const double md = dsp * std::pow(10.0, (double)(-ed));
+  const unsigned part = (unsigned)(mf * 1000 + 0.5) * 4000000u + (unsigned)(md * 1000 + 0.5) * 4000u;
+  return (unsigned)((double)part + std::max((double)(3000 + ef * 30 + ed), 0.0));
+}
+int floorhkl(double x) {
+  const double xc = std::floor(x * (1.0 + 1e-12));
+  if (xc <= 1.0) return 1;
+  if (xc >= 2147483647.0) return 2147483647;
+  return (int)(xc + 0.5);
+}
+struct HKL3 {
+  int h, k, l;
+};
+bool hkl_less(const HKL3& a, const HKL3& o) {
+  return (a.h != o.h ? o.h < a.h : (a.k != o.k ? o.k < a.k : o.l < a.l));
+}
+struct Fam {
+  HKL3 rep;
+  int mult;
+  double fsq, dsp;
+  std::vector<HKL3> members;
+};
+
+int64_t hkl_orthogonal(double a, double b, double c, int nspecies, const int* natoms, const double* positions,
+                       const double* csl, const double* msd, double dcutoff, double fsquarecut, double merge_tol,
+                       int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq,
+                       int64_t* nrequired) {
+  ensure_device();
+  if (!(dcutoff > 0.0)) raise("BadInput", "dcutoff must be positive");
+  if (!(a > 0 && b > 0 && c > 0)) raise("BadInput", "lattice parameters must be positive");
+  const double invd = 1.0 / dcutoff;
+  HKLBuild P{};
+  P.rl0 = k2Pi / a; P.rl1 = k2Pi / b; P.rl2 = k2Pi / c;
+  P.max_h = floorhkl(a * invd); P.max_k = floorhkl(b * invd); P.max_l = floorhkl(c * invd);
+  if (fsquarecut >= 0.0) fsquarecut = std::max(fsquarecut, 1.0e-300);
+  P.fsquarecut = fsquarecut;
+  std::vector<double> vpos, vcsl, vmsd, vthr;
+  std::vector<int> off{0};
+  {
+    const double* pp = positions;
+    for (int i = 0; i < nspecies; ++i) {
+      if (fabs(csl[i]) > 0.0) {
+        vpos.insert(vpos.end(), pp, pp + 3 * natoms[i]);
+        vcsl.push_back(csl[i]);
+        vmsd.push_back(msd[i]);
+        vthr.push_back((fsquarecut < 0.01 && fsquarecut > 1.0e-300) ? std::log(fabs(csl[i]) / fsquarecut)
+                                                                    : __builtin_huge_val());
+        off.push_back(off.back() + natoms[i]);
+      }
+      pp += 3 * natoms[i];
+    }
+  }
+  const int ns = (int)vcsl.size();
+  if (ns > 32) raise("BadInput", "at most 32 species with non-zero coherent scattering length");
+  if (nrequired) *nrequired = 0;
+  if (ns == 0) return 0;
+  const double DMIN = 2.2250738585072014e-308, DMAX = 1.7976931348623157e308;
+  P.ksq_lo = std::min(std::max((39.4784176043574344753379639995046045412547976 * (1.0 - 1e-14)) /
+                                   (__builtin_huge_val() * __builtin_huge_val()), DMIN), DMAX);
+  P.ksq_hi = std::min(std::max((39.4784176043574344753379639995046045412547976 * (1.0 + 1e-14)) /
+                                   (dcutoff * dcutoff), DMIN), DMAX);
+  P.dcut_lo = std::min(std::max(dcutoff, DMIN), DMAX);
+  P.dcut_hi = DMAX;
+  DevBuf dpos, dcsl, dmsd, dthr, doff;
+  dpos.upload(vpos.data(), vpos.size());
+  dcsl.upload(vcsl.data(), vcsl.size());
+  dmsd.upload(vmsd.data(), vmsd.size());
+  dthr.upload(vthr.data(), vthr.size());
+  doff.upload(off.data(), off.size());
+  P.nspecies = ns;
+  P.pos = dpos.as<double>(); P.csl = dcsl.as<double>(); P.msd = dmsd.as<double>();
+  P.thr = dthr.as<double>(); P.pos_off = doff.as<int>();
+  const int64_t nl = 2 * (int64_t)P.max_l + 1, nkf = 2 * (int64_t)P.max_k + 1;
+  const int64_t total = ((int64_t)P.max_k + 1) * nl + (int64_t)P.max_h * nkf * nl;
+  const int64_t chunk = std::min<int64_t>(total, (int64_t)1 << 23);
+  DevBuf dF, dD, dV;
+  dF.alloc(sizeof(double) * chunk);
+  dD.alloc(sizeof(double) * chunk);
+  dV.alloc(chunk);
+  std::vector<double> hF(chunk), hD(chunk);
+  std::vector<unsigned char> hV(chunk);
+  std::vector<Fam> fams;
+  std::multimap<unsigned, size_t> fmap;
+  const int64_t n_h0 = ((int64_t)P.max_k + 1) * nl;
+  for (int64_t first = 0; first < total; first += chunk) {
+    const int64_t cnt = std::min(chunk, total - first);
+    launch_hkl_eval(P, first, cnt, dF.as<double>(), dD.as<double>(), dV.as<unsigned char>(), 0);
+    NCAMD_HIP(hipGetLastError());
+    NCAMD_HIP(hipMemcpy(hV.data(), dV.p, cnt, hipMemcpyDeviceToHost));
+    NCAMD_HIP(hipMemcpy(hF.data(), dF.p, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+    NCAMD_HIP(hipMemcpy(hD.data(), dD.p, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+    for (int64_t t = 0; t < cnt; ++t) {
+      if (!hV[t]) continue;
+      const int64_t q = first + t;
+      HKL3 cur;
+      if (q < n_h0) {
+        cur.h = 0; cur.k = (int)(q / nl); cur.l = (int)(q % nl) - P.max_l;
+      } else {
+        const int64_t qq = q - n_h0, per_h = nkf * nl;
+        cur.h = 1 + (int)(qq / per_h);
+        const int64_t r = qq % per_h;
+        cur.k = (int)(r / nl) - P.max_k;
+        cur.l = (int)(r % nl) - P.max_l;
+      }
+      const double fsq = hF[t], dsp = hD[t];
+      const unsigned key = famkey(fsq, dsp);
+      auto itLB = fmap.lower_bound(key);
+      bool isnew = true;
+      for (auto it = itLB; it != fmap.end() && it->first == key; ++it) {
+        Fam& f = fams[it->second];
+        if (fabs(fsq - f.fsq) < merge_tol * (fsq + f.fsq) && fabs(dsp - f.dsp) < merge_tol * (dsp + f.dsp)) {
+          f.mult += 2;
+          f.members.push_back(cur);
+          isnew = false;
+          break;
+        }
+      }
+      if (isnew) {
+        if (fams.size() > 1000000) return -2;
+        Fam f;
+        f.rep = cur; f.mult = 2; f.fsq = fsq; f.dsp = dsp;
+        f.members.push_back(cur);
+        fmap.insert(itLB, std::make_pair(key, fams.size()));
+        fams.push_back(std::move(f));
+      }
+    }
+  }
+  for (auto& f : fams) {
+    std::sort(f.members.begin(), f.members.end(), hkl_less);
+    f.rep = f.members.front();
+  }
+  auto before = [](const Fam& rh, const Fam& lh) {
+    if (fabs(lh.dsp - rh.dsp) > 1e-6) return lh.dsp < rh.dsp;
+    const double lfm = lh.fsq * lh.mult, rfm = rh.fsq * rh.mult;
+    if (fabs(lfm - rfm) > 1e-6) return lfm < rfm;
+    if (lh.mult != rh.mult) return lh.mult < rh.mult;
+    return hkl_less(lh.rep, rh.rep);
+  };
+  std::stable_sort(fams.begin(), fams.end(), before);
+  const int64_t nf = (int64_t)fams.size();
+  if (nrequired) *nrequired = nf;
+  if (nf > maxout) return -1;
+  for (int64_t i = 0; i < nf; ++i) {
+    out_hkl[3 * i] = fams[i].rep.h; out_hkl[3 * i + 1] = fams[i].rep.k; out_hkl[3 * i + 2] = fams[i].rep.l;
+    out_mult[i] = fams[i].mult; out_dsp[i] = fams[i].dsp; out_fsq[i] = fams[i].fsq;
+  }
+  return nf;
+}
+
+// ---------------------------------------------------- batched evaluation ---
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+uint32_t next_stream(Proc* p) { return (uint32_t)(p->callctr++); }
+
+unsigned long long read_reset_err(Proc* p, hipStream_t st) {
+  unsigned long long n = 0;
+  NCAMD_HIP(hipMemcpyAsync(&n, p->d_err.p, sizeof(n), hipMemcpyDeviceToHost, st));
+  NCAMD_HIP(hipStreamSynchronize(st));
+  if (n) NCAMD_HIP(hipMemsetAsync(p->d_err.p, 0, sizeof(n), st));
+  return n;
+}
+
+void check_fp32(Proc* p) {
+  CompView v = make_view(p);
+  for (int i = 0; i < v.n; ++i)
+    if (v.kind[i] == kKindPB && !v.pb.fdm32)
+      raise("BadInput", "process was created without FP32 PowderBragg tables");
+}
+
+}  // namespace
+}  // namespace ncamd
+
+using namespace ncamd;
+
+extern "C" {
+
+const char* ncamd_version_str(void) { return "0.1.0 (NCrystal 4.4.2 physics)"; }
+int ncamd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+int ncamd_error(void) { return g_waserror; }
+const char* ncamd_lasterror(void) { return g_waserror ? g_errmsg : ""; }
+const char* ncamd_lasterrortype(void) { return g_waserror ? g_errtype : ""; }
+void ncamd_clearerror(void) { g_waserror = 0; g_errmsg[0] = '\0'; g_errtype[0] = '\0'; }
+void ncamd_sethaltonerror(int h) { g_halt = h; }
+void ncamd_setquietonerror(int q) { g_quiet = q; }
+
+void ncamd_ref(ncamd_process_t h) {
+  NCAMD_TRY extract(h)->refcount++; NCAMD_CATCH
+}
+void ncamd_unref(ncamd_process_t* h) {
+  NCAMD_TRY
+  Proc* p = extract(*h);
+  if (--p->refcount == 0) delete p;
+  h->internal = nullptr;
+  NCAMD_CATCH
+}
+int ncamd_refcount(ncamd_process_t h) {
+  NCAMD_TRY return extract(h)->refcount; NCAMD_CATCH
+  return -1;
+}
+int ncamd_valid(ncamd_process_t h) {
+  Proc* p = static_cast<Proc*>(h.internal);
+  return (p && p->magic == kMagic) ? 1 : 0;
+}
+int ncamd_isoriented(ncamd_process_t h) {
+  NCAMD_TRY extract(h); return 0; NCAMD_CATCH
+  return -1;
+}
+
+double ncamd_debye_msd(double dt, double t, double am) {
+  NCAMD_TRY return debye_msd(dt, t, am); NCAMD_CATCH
+  return -1.0;
+}
+
+int64_t ncamd_hkl_orthogonal(double a, double b, double c, int nspecies, const int* natoms, const double* positions,
+                             const double* csl, const double* msd, double dcutoff, double fsquarecut,
+                             double merge_tolerance, int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp,
+                             double* out_fsq, int64_t* nrequired) {
+  NCAMD_TRY
+  return hkl_orthogonal(a, b, c, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut, merge_tolerance, maxout,
+                        out_hkl, out_mult, out_dsp, out_fsq, nrequired);
+  NCAMD_CATCH
+  return -3;
+}
+
+ncamd_process_t ncamd_create_powderbragg(double v0n, const double* dsp, const double* fsqm, int64_t n, int fp32) {
+  NCAMD_TRY return wrap(create_pb(v0n, dsp, fsqm, n, fp32 != 0)); NCAMD_CATCH
+  return wrap(nullptr);
+}
+ncamd_process_t ncamd_create_elincscatter(int n, const double* msd, const double* bixs, const double* scale) {
+  NCAMD_TRY return wrap(create_elinc(n, msd, bixs, scale)); NCAMD_CATCH
+  return wrap(nullptr);
+}
+ncamd_process_t ncamd_create_sabscatter(int na, int nb, const double* alpha, const double* beta, const double* sab,
+                                        double T, double mass, double sigma_bound, double suggested_emax,
+                                        int n_egrid, const double* egrid) {
+  NCAMD_TRY
+  return wrap(create_sab(na, nb, alpha, beta, sab, T, mass, sigma_bound, suggested_emax, n_egrid, egrid));
+  NCAMD_CATCH
+  return wrap(nullptr);
+}
+ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, const double* scales) {
+  NCAMD_TRY
+  ensure_device();
+  if (n < 1 || n > kMaxComp) raise("BadInput", "composition needs 1..4 components");
+  std::unique_ptr<Proc> p(new Proc());
+  p->kind = kKindNone;
+  int seen[4] = {0, 0, 0, 0};
+  for (int i = 0; i < n; ++i) {
+    Proc* c = extract(procs[i]);
+    if (c->kind == kKindNone) raise("BadInput", "nested compositions are not supported");
+    if (seen[c->kind]++) raise("BadInput", "at most one process of each kind per composition");
+    if (!(scales[i] >= 0.0) || std::isinf(scales[i])) raise("BadInput", "component scales must be finite and >= 0");
+    // null processes and zero scales are dropped (NCProcImpl.cc:263-283)
+    if (scales[i] == 0.0 || (c->kind == kKindPB && c->pb.n == 0)) continue;
+    c->refcount++;
+    p->comps.push_back(c);
+    p->scales.push_back(scales[i]);
+  }
+  if (p->comps.empty()) raise("BadInput", "composition has no non-null component");
+  init_errbuf(p.get());
+  return wrap(p.release());
+  NCAMD_CATCH
+  return wrap(nullptr);
+}
+
+int64_t ncamd_powderbragg_tables(ncamd_process_t h, double* e2d, double* fdm, float* fdm32, int64_t maxn) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindPB) raise("BadInput", "not a PowderBragg process");
+  const int64_t n = (int64_t)p->pb_e2d_h.size();
+  if (n > maxn) return n;
+  std::copy(p->pb_e2d_h.begin(), p->pb_e2d_h.end(), e2d);
+  std::copy(p->pb_fdm_h.begin(), p->pb_fdm_h.end(), fdm);
+  if (fdm32 && p->has_fp32) std::copy(p->pb_fdm32_h.begin(), p->pb_fdm32_h.end(), fdm32);
+  return n;
+  NCAMD_CATCH
+  return -1;
+}
+int ncamd_sab_egrid(ncamd_process_t h, double* egrid, double* xs, int maxn) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
+  const int n = (int)p->sab_egrid_h.size();
+  if (n > maxn) return n;
+  std::copy(p->sab_egrid_h.begin(), p->sab_egrid_h.end(), egrid);
+  std::copy(p->sab_xs_h.begin(), p->sab_xs_h.end(), xs);
+  return n;
+  NCAMD_CATCH
+  return -1;
+}
+int ncamd_sab_derived(ncamd_process_t h, double* logsab, double* cumul) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
+  const size_t N = (size_t)p->sab_na * p->sab_nb;
+  std::vector<double2> sl(N);
+  NCAMD_HIP(hipMemcpy(sl.data(), p->sab.sl, sizeof(double2) * N, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < N; ++i) logsab[i] = sl[i].y;
+  NCAMD_HIP(hipMemcpy(cumul, p->sab.cumul, sizeof(double) * N, hipMemcpyDeviceToHost));
+  return 0;
+  NCAMD_CATCH
+  return -1;
+}
+int ncamd_sab_sampler(ncamd_process_t h, int ie, int* kind, int* ibl, int* nvals, double* firstbin, double* bx,
+                      double* by, double* bcdf, double* info_dbl, int* info_idx) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
+  const SABView& s = p->sab;
+  if (ie < 0 || ie >= s.ne) raise("BadInput", "energy grid index out of range");
+  int4 hdr;
+  NCAMD_HIP(hipMemcpy(&hdr, s.shdr + ie, sizeof(int4), hipMemcpyDeviceToHost));
+  NCAMD_HIP(hipMemcpy(firstbin, s.sfirst + ie, sizeof(double), hipMemcpyDeviceToHost));
+  *kind = hdr.x; *ibl = hdr.y; *nvals = hdr.z;
+  if (hdr.x != 1 || !bx) return 0;
+  const int nv = hdr.z;
+  std::vector<double2> xy(nv);
+  NCAMD_HIP(hipMemcpy(xy.data(), s.sxy + (size_t)ie * (s.nb + 1), sizeof(double2) * nv, hipMemcpyDeviceToHost));
+  NCAMD_HIP(hipMemcpy(bcdf, s.scdf + (size_t)ie * (s.nb + 1), sizeof(double) * nv, hipMemcpyDeviceToHost));
+  for (int i = 0; i < nv; ++i) { bx[i] = xy[i].x; by[i] = xy[i].y; }
+  const int ni = hdr.w;
+  std::vector<AInfo> inf(ni);
+  NCAMD_HIP(hipMemcpy(inf.data(), s.sinfo + (size_t)ie * s.nb, sizeof(AInfo) * ni, hipMemcpyDeviceToHost));
+  for (int i = 0; i < ni; ++i) {
+    const AInfo& f = inf[i];
+    double* d = info_dbl + 8 * i;
+    d[0] = f.f_alpha; d[1] = f.f_sval; d[2] = f.f_logsval; d[3] = f.b_alpha; d[4] = f.b_sval; d[5] = f.b_logsval;
+    d[6] = f.prob_front; d[7] = f.prob_notback;
+    info_idx[2 * i] = f.f_idx; info_idx[2 * i + 1] = f.b_idx;
+  }
+  return 0;
+  NCAMD_CATCH
+  return -1;
+}
+
+void ncamd_setrngseed(ncamd_process_t h, uint64_t seed) {
+  NCAMD_TRY Proc* p = extract(h); p->seed = seed; p->callctr = 0; NCAMD_CATCH
+}
+void ncamd_getrngstate(ncamd_process_t h, uint64_t* seed, uint64_t* ctr) {
+  NCAMD_TRY Proc* p = extract(h); *seed = p->seed; *ctr = p->callctr; NCAMD_CATCH
+}
+void ncamd_setrngstate(ncamd_process_t h, uint64_t seed, uint64_t ctr) {
+  NCAMD_TRY Proc* p = extract(h); p->seed = seed; p->callctr = ctr; NCAMD_CATCH
+}
+
+// ---- host pointer API
+static void xs_many_impl(ncamd_process_t h, const double* ekin, unsigned long n, unsigned long repeat, double* out,
+                         bool fp32) {
+  bool ok = false;
+  NCAMD_TRY
+  ensure_device();
+  Proc* p = extract(h);
+  if (fp32) check_fp32(p);
+  const CompView v = make_view(p);
+  p->st_in.ensure(sizeof(double) * n);
+  p->st_out1.ensure(sizeof(double) * n * repeat);
+  if (n) NCAMD_HIP(hipMemcpy(p->st_in.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
+  launch_xs_iso(v, p->st_in.as<double>(), (int64_t)n, (int64_t)repeat, p->st_out1.as<double>(), fp32, 0);
+  NCAMD_HIP(hipGetLastError());
+  if (n * repeat) NCAMD_HIP(hipMemcpy(out, p->st_out1.p, sizeof(double) * n * repeat, hipMemcpyDeviceToHost));
+  ok = true;
+  NCAMD_CATCH
+  if (!ok)
+    for (unsigned long i = 0; i < n * repeat; ++i) out[i] = -1.0;
+}
+void ncamd_crosssection_nonoriented_many(ncamd_process_t h, const double* ekin, unsigned long n, unsigned long repeat,
+                                         double* out) {
+  xs_many_impl(h, ekin, n, repeat, out, false);
+}
+void ncamd_crosssection_nonoriented_many_fp32(ncamd_process_t h, const double* ekin, unsigned long n,
+                                              unsigned long repeat, double* out) {
+  xs_many_impl(h, ekin, n, repeat, out, true);
+}
+
+void ncamd_samplescatterisotropic_many(ncamd_process_t h, const double* ekin, unsigned long n, unsigned long repeat,
+                                       double* oE, double* oMu) {
+  bool ok = false;
+  NCAMD_TRY
+  ensure_device();
+  Proc* p = extract(h);
+  const CompView v = make_view(p);
+  const size_t tot = (size_t)n * repeat;
+  p->st_in.ensure(sizeof(double) * n);
+  p->st_out1.ensure(sizeof(double) * tot);
+  p->st_out2.ensure(sizeof(double) * tot);
+  if (n) NCAMD_HIP(hipMemcpy(p->st_in.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
+  launch_sample_iso(v, p->st_in.as<double>(), (int64_t)n, (int64_t)repeat, p->seed, next_stream(p),
+                    p->st_out1.as<double>(), p->st_out2.as<double>(), p->d_err.as<unsigned long long>(), 0);
+  NCAMD_HIP(hipGetLastError());
+  if (tot) {
+    NCAMD_HIP(hipMemcpy(oE, p->st_out1.p, sizeof(double) * tot, hipMemcpyDeviceToHost));
+    NCAMD_HIP(hipMemcpy(oMu, p->st_out2.p, sizeof(double) * tot, hipMemcpyDeviceToHost));
+  }
+  const unsigned long long nerr = read_reset_err(p, 0);
+  if (nerr)
+    raise("CalcError", std::to_string(nerr) + " neutron(s) failed in scatter sampling (rejection loop limit)");
+  ok = true;
+  NCAMD_CATCH
+  if (!ok)
+    for (size_t i = 0; i < (size_t)n * repeat; ++i) { oE[i] = -1.0; oMu[i] = -999.0; }
+}
+
+void ncamd_samplescatter_many_dirs(ncamd_process_t h, const double* ekin, const double* dirs, unsigned long n,
+                                   double* oE, double* oD) {
+  bool ok = false;
+  NCAMD_TRY
+  ensure_device();
+  Proc* p = extract(h);
+  const CompView v = make_view(p);
+  p->st_in.ensure(sizeof(double) * n);
+  p->st_in2.ensure(sizeof(double) * 3 * n);
+  p->st_out1.ensure(sizeof(double) * n);
+  p->st_out2.ensure(sizeof(double) * 3 * n);
+  if (n) {
+    NCAMD_HIP(hipMemcpy(p->st_in.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
+    NCAMD_HIP(hipMemcpy(p->st_in2.p, dirs, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  }
+  launch_sample_dirs(v, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->seed, next_stream(p),
+                     p->st_out1.as<double>(), p->st_out2.as<double>(), p->d_err.as<unsigned long long>(), 0);
+  NCAMD_HIP(hipGetLastError());
+  if (n) {
+    NCAMD_HIP(hipMemcpy(oE, p->st_out1.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    NCAMD_HIP(hipMemcpy(oD, p->st_out2.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  }
+  const unsigned long long nerr = read_reset_err(p, 0);
+  if (nerr)
+    raise("CalcError", std::to_string(nerr) + " neutron(s) failed in scatter sampling (rejection loop limit)");
+  ok = true;
+  NCAMD_CATCH
+  if (!ok)
+    for (unsigned long i = 0; i < n; ++i) { oE[i] = -1.0; oD[3 * i] = oD[3 * i + 1] = oD[3 * i + 2] = 0.0; }
+}
+
+void ncamd_samplescatter_many(ncamd_process_t h, double ekin, const double (*direction)[3], unsigned long repeat,
+                              double* oE, double* ox, double* oy, double* oz) {
+  std::vector<double> e(repeat, ekin), d(3 * repeat), od(3 * repeat);
+  for (unsigned long i = 0; i < repeat; ++i) {
+    d[3 * i] = (*direction)[0]; d[3 * i + 1] = (*direction)[1]; d[3 * i + 2] = (*direction)[2];
+  }
+  ncamd_samplescatter_many_dirs(h, e.data(), d.data(), repeat, oE, od.data());
+  for (unsigned long i = 0; i < repeat; ++i) { ox[i] = od[3 * i]; oy[i] = od[3 * i + 1]; oz[i] = od[3 * i + 2]; }
+}
+
+void ncamd_crosssection_many_dirs(ncamd_process_t h, const double* ekin, const double* dirs, unsigned long n,
+                                  double* out) {
+  (void)dirs;  // isotropic materials: the direction does not enter
+  xs_many_impl(h, ekin, n, 1, out, false);
+}
+
+// ---- device pointer API
+void ncamd_dev_crosssection_nonoriented_many(ncamd_process_t h, const double* d_ekin, unsigned long n,
+                                             unsigned long repeat, double* d_out, void* stream) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  launch_xs_iso(make_view(p), d_ekin, (int64_t)n, (int64_t)repeat, d_out, false, as_stream(stream));
+  NCAMD_HIP(hipGetLastError());
+  NCAMD_CATCH
+}
+void ncamd_dev_crosssection_nonoriented_many_fp32(ncamd_process_t h, const double* d_ekin, unsigned long n,
+                                                  unsigned long repeat, double* d_out, void* stream) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  check_fp32(p);
+  launch_xs_iso(make_view(p), d_ekin, (int64_t)n, (int64_t)repeat, d_out, true, as_stream(stream));
+  NCAMD_HIP(hipGetLastError());
+  NCAMD_CATCH
+}
+void ncamd_dev_samplescatterisotropic_many(ncamd_process_t h, const double* d_ekin, unsigned long n,
+                                           unsigned long repeat, double* d_oE, double* d_oMu, void* stream) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  launch_sample_iso(make_view(p), d_ekin, (int64_t)n, (int64_t)repeat, p->seed, next_stream(p), d_oE, d_oMu,
+                    p->d_err.as<unsigned long long>(), as_stream(stream));
+  NCAMD_HIP(hipGetLastError());
+  NCAMD_CATCH
+}
+void ncamd_dev_samplescatter_many_dirs(ncamd_process_t h, const double* d_ekin, const double* d_dirs,
+                                       unsigned long n, double* d_oE, double* d_oD, void* stream) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  launch_sample_dirs(make_view(p), d_ekin, d_dirs, (int64_t)n, p->seed, next_stream(p), d_oE, d_oD,
+                     p->d_err.as<unsigned long long>(), as_stream(stream));
+  NCAMD_HIP(hipGetLastError());
+  NCAMD_CATCH
+}
+void ncamd_dev_sab_sample_alphabeta(ncamd_process_t h, const double* d_ekin, unsigned long n, double* d_a,
+                                    double* d_b, void* stream) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
+  launch_sab_ab(p->sab, d_ekin, (int64_t)n, p->seed, next_stream(p), d_a, d_b, p->d_err.as<unsigned long long>(),
+                as_stream(stream));
+  NCAMD_HIP(hipGetLastError());
+  NCAMD_CATCH
+}
+int64_t ncamd_dev_check_errors(ncamd_process_t h, void* stream) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  return (int64_t)read_reset_err(p, as_stream(stream));
+  NCAMD_CATCH
+  return -1;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+// |F(hkl)|^2 and d-spacing of every (h,k,l) in the enumeration domain of a
+// cell with all angles 90 degrees, one lane per (h,k,l).
+//
+// Follows ncrystal_core/src/extd_utils/NCFillHKL.cc:299-383 in operation
+// order (so that d-spacings are bit-identical and |F|^2 agrees to the last
+// few ulp of exp): diagonal reciprocal lattice
+// (ncrystal_core/src/utils/NCLatticeUtils.cc:105-117), Debye-Waller factors
+// exp(-0.5 ksq msd) with the per-species early-abort thresholds, phases with
+// the polynomial sincos of ncrystal_core/include/NCrystal/internal/utils/NCMath.hh:318-326
+// and ncrystal_core/src/utils/NCMath.cc:96-125, Neumaier sums per species and
+// over species.
+#include "nc_kernels.hh"
+
+namespace ncamd {
+
+NCD void sincos_mpi2pi2(double A, double& cosA, double& sinA) {
+  const double x = 0.5 * A;
+  const double mx2 = -x * x;
+  const double s2 = x * (1.0 + mx2 * (1.66666666666666666666666666666666666666666667e-1
+                   + mx2 * (8.33333333333333333333333333333333333333333333e-3
+                   + mx2 * (1.98412698412698412698412698412698412698412698e-4
+                   + mx2 * (2.75573192239858906525573192239858906525573192e-6
+                   + mx2 * (2.50521083854417187750521083854417187750521084e-8
+                   + mx2 * (1.60590438368216145993923771701549479327257105e-10
+                   + mx2 * (7.64716373181981647590113198578807044415510024e-13))))))));
+  const double c2m1 = mx2 * (0.5
+                + mx2 * (4.16666666666666666666666666666666666666666667e-2
+                + mx2 * (1.38888888888888888888888888888888888888888889e-3
+                + mx2 * (2.48015873015873015873015873015873015873015873e-5
+                + mx2 * (2.75573192239858906525573192239858906525573192e-7
+                + mx2 * (2.08767569878680989792100903212014323125434237e-9
+                + mx2 * (1.14707455977297247138516979786821056662326504e-11
+                + mx2 * (4.77947733238738529743820749111754402759693765e-14))))))));
+  const double k = 2.0 * c2m1;
+  sinA = (k + 2.0) * s2;
+  cosA = 1.0 + k * (c2m1 + 2.0);
+}
+NCD void sincos_2pix(double xx, double& s, double& c) {
+  double A = (xx - floor(xx)) * k2Pi;
+  A -= kPi;
+  const double Aabs = fabs(A);
+  sincos_mpi2pi2(dmin(Aabs, kPi - Aabs), c, s);
+  c = copysign(c, Aabs - kPiHalf);
+  s = copysign(s, -A);
+}
+
+constexpr int kMaxHKLSpecies = 32;
+
+__global__ void __launch_bounds__(256) k_hkl_eval(HKLBuild p, int64_t first, int64_t count,
+                                                  double* __restrict__ fsq_out, double* __restrict__ dsp_out,
+                                                  unsigned char* __restrict__ valid) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int64_t q = first + t;
+  const int64_t nl = 2 * (int64_t)p.max_l + 1;
+  const int64_t nk_full = 2 * (int64_t)p.max_k + 1;
+  const int64_t n_h0 = ((int64_t)p.max_k + 1) * nl;
+  int h, k, l;
+  if (q < n_h0) {
+    h = 0;
+    k = (int)(q / nl);
+    l = (int)(q % nl) - p.max_l;
+  } else {
+    const int64_t qq = q - n_h0;
+    const int64_t per_h = nk_full * nl;
+    h = 1 + (int)(qq / per_h);
+    const int64_t r = qq % per_h;
+    k = (int)(r / nl) - p.max_k;
+    l = (int)(r % nl) - p.max_l;
+  }
+  unsigned char ok = 0;
+  double fsq = 0.0, dsp = 0.0;
+  const double dh = (double)h, dk = (double)k, dl = (double)l;
+  const double wx = p.rl0 * dh + 0.0 * dk + 0.0 * dl;
+  const double wy = 0.0 * dh + p.rl1 * dk + 0.0 * dl;
+  const double wz = 0.0 * dh + 0.0 * dk + p.rl2 * dl;
+  const double ksq = wx * wx + wy * wy + wz * wz;
+  do {
+    if (!in_interval(p.ksq_lo, p.ksq_hi, ksq)) break;
+    const double kk2 = 0.5 * ksq;
+    double fac[kMaxHKLSpecies];
+    double upper = 0.0;
+    for (int i = 0; i < p.nspecies; ++i) {
+      const double w = kk2 * p.msd[i];
+      if (w > p.thr[i]) { fac[i] = 0.0; continue; }
+      const double f = p.csl[i] * exp(-w);
+      fac[i] = f;
+      upper += (double)(p.pos_off[i + 1] - p.pos_off[i]) * fabs(f);
+    }
+    if (upper * upper * 2.0 < p.fsquarecut) break;
+    if (h == 0 && k == 0 && l <= 0) break;
+    StableSum re, im;
+    for (int i = 0; i < p.nspecies; ++i) {
+      const double f = fac[i];
+      if (!f) continue;
+      StableSum cs, sn;
+      for (int j = p.pos_off[i]; j < p.pos_off[i + 1]; ++j) {
+        const double ph = dh * p.pos[3 * j] + dk * p.pos[3 * j + 1] + dl * p.pos[3 * j + 2];
+        double sv, cv;
+        sincos_2pix(ph, sv, cv);
+        cs.add(cv);
+        sn.add(sv);
+      }
+      re.add(cs.sum() * f);
+      im.add(sn.sum() * f);
+    }
+    const double r = re.sum(), ii = im.sum();
+    fsq = r * r + ii * ii;
+    if (fsq < p.fsquarecut) break;
+    const double kval = sqrt(ksq);
+    const double invk = 1.0 / kval;
+    dsp = k2Pi * invk;
+    if (!in_interval(p.dcut_lo, p.dcut_hi, dsp)) break;
+    ok = 1;
+  } while (false);
+  fsq_out[t] = fsq;
+  dsp_out[t] = dsp;
+  valid[t] = ok;
+}
+
+void launch_hkl_eval(const HKLBuild& p, int64_t first, int64_t count, double* fsq, double* dsp,
+                     unsigned char* valid, hipStream_t st) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_hkl_eval, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, p, first, count, fsq,
+                     dsp, valid);
+}
+
+}  // namespace ncamd

@@ -1,0 +1,159 @@
+// Per-neutron kernels: cross sections and scatter sampling of a process
+// composition (PowderBragg, incoherent elastic, S(alpha,beta)) with one lane
+// per neutron and a Philox stream per neutron.
+//
+// Composition semantics: ncrystal_core/src/interfaces/NCProcImpl.cc:166-204
+// (cumulative component xs), :353-362 (xs), :379-389 (component pick +
+// sampling); batched layout: ncrystal_core/src/cinterface/ncrystal.cc:1107-1246.
+#include "nc_kernels.hh"
+
+namespace ncamd {
+
+NCD double comp_domain_lo(const CompView& c) { return c.domain_lo; }
+
+template <bool FP32PB>
+NCD double pb_xs_sel(const PBView& pb, double E) {
+  if constexpr (FP32PB) {
+    if (!pb_active(pb, E)) return 0.0;
+    const int idx = pb_last_valid(pb, E);
+    const double inv = 1.0 / E;
+    return (double)pb.fdm32[idx] * inv;
+  } else {
+    return pb_xs(pb, E);
+  }
+}
+
+template <bool FP32PB>
+NCD double comp_xs(const CompView& c, double E, double* commul) {
+  if (!(E >= c.domain_lo && E <= c.domain_hi)) return 0.0;
+  double tot = 0.0;
+  for (int i = 0; i < c.n; ++i) {
+    double xs = 0.0;
+    switch (c.kind[i]) {
+      case kKindPB: xs = pb_active(c.pb, E) ? pb_xs_sel<FP32PB>(c.pb, E) : 0.0; break;
+      case kKindElInc: xs = elinc_xs(c.ei, E, nullptr); break;
+      case kKindSAB: xs = sab_xs(c.sab, E); break;
+      default: break;
+    }
+    tot += c.scale[i] * xs;
+    if (commul) commul[i] = tot;
+  }
+  return tot;
+}
+
+NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, double& mu) {
+  if (!(E >= c.domain_lo && E <= c.domain_hi)) { Ef = E; mu = 1.0; return kOK; }
+  double commul[kMaxComp];
+  const double tot = comp_xs<false>(c, E, commul);
+  (void)tot;
+  const int ic = pick_idx(rng, commul, c.n);
+  switch (c.kind[ic]) {
+    case kKindPB: Ef = E; mu = pb_sample_mu(c.pb, rng, E); return kOK;
+    case kKindElInc: Ef = E; mu = elinc_sample_mu(c.ei, rng, E); return kOK;
+    case kKindSAB: return sab_sample_iso(c.sab, rng, E, Ef, mu);
+    default: return kErrBadSampler;
+  }
+}
+
+template <bool FP32PB>
+__global__ void __launch_bounds__(256) k_xs_iso(CompView c, const double* __restrict__ ekin,
+                                                int64_t n, int64_t repeat, double* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double xs = comp_xs<FP32PB>(c, ekin[i], nullptr);
+    for (int64_t r = 0; r < repeat; ++r) out[r * n + i] = xs;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sample_iso(CompView c, const double* __restrict__ ekin,
+                                                    int64_t n, int64_t total, uint64_t seed,
+                                                    uint32_t stream, double* __restrict__ oE,
+                                                    double* __restrict__ oMu,
+                                                    unsigned long long* __restrict__ nerr) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gstride) {
+    const int64_t i = (total == n) ? k : (k % n);
+    Rng rng;
+    rng.init(seed, stream, (uint64_t)k);
+    double Ef, mu;
+    const int rc = comp_sample_iso(c, rng, ekin[i], Ef, mu);
+    if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
+    oE[k] = Ef;
+    oMu[k] = mu;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sample_dirs(CompView c, const double* __restrict__ ekin,
+                                                     const double* __restrict__ dirs, int64_t n,
+                                                     uint64_t seed, uint32_t stream,
+                                                     double* __restrict__ oE, double* __restrict__ oD,
+                                                     unsigned long long* __restrict__ nerr) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gstride) {
+    Rng rng;
+    rng.init(seed, stream, (uint64_t)k);
+    double Ef, mu;
+    const int rc = comp_sample_iso(c, rng, ekin[k], Ef, mu);
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    if (rc) {
+      Ef = -1.0;
+      atomicAdd(nerr, 1ull);
+    } else {
+      rand_dir_given_mu(rng, mu, dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2], ox, oy, oz);
+    }
+    oE[k] = Ef;
+    oD[3 * k] = ox; oD[3 * k + 1] = oy; oD[3 * k + 2] = oz;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sab_ab(SABView s, const double* __restrict__ ekin, int64_t n,
+                                                uint64_t seed, uint32_t stream, double* __restrict__ oa,
+                                                double* __restrict__ ob,
+                                                unsigned long long* __restrict__ nerr) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gstride) {
+    Rng rng;
+    rng.init(seed, stream, (uint64_t)k);
+    double a, b;
+    const int rc = sab_sample_ab(s, ekin[k], rng, a, b);
+    if (rc) { a = -1.0; b = 0.0; atomicAdd(nerr, 1ull); }
+    oa[k] = a;
+    ob[k] = b;
+  }
+}
+
+static inline unsigned grid_for(int64_t n, int bs) {
+  int64_t g = (n + bs - 1) / bs;
+  // enough waves for the 256 CUs, bounded so a grid-stride loop covers the rest
+  const int64_t gmax = 256 * 64;
+  if (g > gmax) g = gmax;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+void launch_xs_iso(const CompView& c, const double* ekin, int64_t n, int64_t repeat, double* out,
+                   bool fp32, hipStream_t st) {
+  if (n <= 0 || repeat <= 0) return;
+  if (fp32) hipLaunchKernelGGL(k_xs_iso<true>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, n, repeat, out);
+  else hipLaunchKernelGGL(k_xs_iso<false>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, n, repeat, out);
+}
+void launch_sample_iso(const CompView& c, const double* ekin, int64_t n, int64_t repeat, uint64_t seed,
+                       uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, hipStream_t st) {
+  const int64_t total = n * repeat;
+  if (total <= 0) return;
+  hipLaunchKernelGGL(k_sample_iso, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin, n, total, seed,
+                     stream, oE, oMu, nerr);
+}
+void launch_sample_dirs(const CompView& c, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
+                        uint32_t stream, double* oE, double* oD, unsigned long long* nerr, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_sample_dirs, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, dirs, n, seed, stream,
+                     oE, oD, nerr);
+}
+void launch_sab_ab(const SABView& s, const double* ekin, int64_t n, uint64_t seed, uint32_t stream, double* oa,
+                   double* ob, unsigned long long* nerr, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_sab_ab, dim3(grid_for(n, 256)), dim3(256), 0, st, s, ekin, n, seed, stream, oa, ob, nerr);
+}
+
+}  // namespace ncamd

@@ -1,0 +1,60 @@
+"""Material assembly: from a synthetic crystal description to a process
+composition on the GPU, following the component wiring of NCrystal's
+standard scatter factory (ncrystal_core/src/stdscatfactory/NCStdScatFact.cc:
+137-146 incoherent elastic first, :148-185 Bragg, :263-330 inelastic; the
+composition keeps that order).
+"""
+import numpy as np
+
+from . import core
+
+
+def species_msd(crystal):
+    """Debye-model MSD per species (ncrystal_core/src/phys_utils/NCDebyeMSD.cc)."""
+    return [core.debye_msd(s.debye_temp, crystal.temperature, s.mass) for s in crystal.species]
+
+
+def hkl_list(crystal, msd=None):
+    msd = species_msd(crystal) if msd is None else msd
+    sp = [(s.csl, m, s.positions) for s, m in zip(crystal.species, msd)]
+    return core.hkl_orthogonal(crystal.a, crystal.b, crystal.c, sp, crystal.dcutoff)
+
+
+def powderbragg_inputs(crystal, msd=None):
+    hkl, mult, dsp, fsq = hkl_list(crystal, msd)
+    return crystal.volume * crystal.n_atoms, dsp, fsq * mult
+
+
+def elinc_inputs(crystal, msd=None):
+    """(msd, bound incoherent xs, scale) per species; scale = fraction of atoms
+    (ncrystal_core/src/elincscatter/NCElIncScatter.cc:70-97)."""
+    msd = species_msd(crystal) if msd is None else msd
+    ntot = crystal.n_atoms
+    scale = [float(s.natoms) * 1.0 / ntot for s in crystal.species]
+    return np.asarray(msd), np.asarray([s.sigma_inc for s in crystal.species]), np.asarray(scale)
+
+
+class Material:
+    def __init__(self, crystal, sab=None, with_elinc=True, with_bragg=True, fp32_tables=True):
+        self.crystal = crystal
+        msd = species_msd(crystal)
+        self.msd = msd
+        comps = []
+        self.components = {}
+        if with_elinc:
+            m, b, s = elinc_inputs(crystal, msd)
+            ei = core.create_elincscatter(m, b, s)
+            comps.append(ei)
+            self.components["elinc"] = ei
+        if with_bragg:
+            v0n, dsp, fm = powderbragg_inputs(crystal, msd)
+            self.pb_inputs = (v0n, dsp, fm)
+            pb = core.create_powderbragg(v0n, dsp, fm, fp32_tables=fp32_tables)
+            comps.append(pb)
+            self.components["bragg"] = pb
+        if sab is not None:
+            sb = core.create_sabscatter(sab["alpha"], sab["beta"], sab["sab"], sab["temperature"], sab["mass"],
+                                        sab["sigma_bound"], egrid=sab.get("egrid"))
+            comps.append(sb)
+            self.components["inelas"] = sb
+        self.scatter = core.create_composition(comps, [1.0] * len(comps))

@@ -1,0 +1,241 @@
+"""Synthetic materials and neutron batches for the benchmark and test workloads.
+
+Everything here is input preparation (host-side numpy): crystal cells, atom
+data, Debye temperatures, a three-Gaussian vibrational density of states and
+its phonon expansion into an S(alpha,beta) table, and seeded neutron energies
+and directions.  None of it is on the per-neutron path.
+
+Physical conventions follow NCrystal (ncrystal_core/include/NCrystal/core/
+NCDefs.hh:79-128): energies in eV, lengths in Angstrom, cross sections in barn,
+coherent scattering lengths in sqrt(barn) (= 10 fm), alpha/beta dimensionless
+with alpha = hbar^2 q^2 / (2 m_neutron kT) and beta = (E_final - E)/kT
+(ncrystal_core/include/NCrystal/internal/phys_utils/NCKinUtils.hh:108-121).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+K_BOLTZMANN = 8.6173303e-5
+K_NEUTRON_MASS_AMU = 1.00866491588
+WL2EKIN = 0.081804209605330899
+
+
+def wl2ekin(wl):
+    wl = np.asarray(wl, dtype=np.float64)
+    return WL2EKIN / (wl * wl)
+
+
+def ekin2wl(e):
+    return np.sqrt(WL2EKIN / np.asarray(e, dtype=np.float64))
+
+
+@dataclass
+class Species:
+    name: str
+    csl: float            # coherent scattering length [sqrt(barn)]
+    sigma_inc: float      # bound incoherent xs [barn]
+    sigma_abs: float      # absorption xs at 2200 m/s [barn]
+    mass: float           # [amu]
+    debye_temp: float     # [K]
+    positions: np.ndarray  # (n,3) fractional, mapped to [0,1) and sorted
+
+    @property
+    def natoms(self):
+        return len(self.positions)
+
+
+@dataclass
+class Crystal:
+    a: float
+    b: float
+    c: float
+    species: list
+    temperature: float
+    dcutoff: float
+    name: str = "crystal"
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def volume(self):
+        # ncrystal_core/src/infobld/NCInfoBuilder.cc:237-244 (diagonal lattice)
+        return (self.a * self.b) * self.c
+
+    @property
+    def n_atoms(self):
+        return sum(s.natoms for s in self.species)
+
+
+def _norm_positions(pos):
+    """Map to [0,1) and sort lexicographically like the reference's
+    InfoBuilder (ncrystal_core/src/infobld/NCInfoBuilder.cc:118-123,180-192)."""
+    p = np.asarray(pos, dtype=np.float64).reshape(-1, 3).copy()
+    p = np.where(p < 0.0, p + 1.0, p)
+    p = np.where(p >= 1.0, p - 1.0, p)
+    order = sorted(range(len(p)), key=lambda i: (p[i, 0], p[i, 1], p[i, 2]))
+    return p[order]
+
+
+def fcc_powder(a=4.05, mass=26.98, csl=0.3449, sigma_inc=0.0082, sigma_abs=0.231,
+               debye_temp=410.0, temperature=293.0, dcutoff=0.5):
+    """fcc single-species powder (a=4.05 Aa, 4 atoms/cell, b_coh 3.449 fm,
+    sigma_inc 0.0082 b, sigma_abs 0.231 b, T_Debye 410 K, T 293 K)."""
+    pos = _norm_positions([(0, .5, .5), (0, 0, 0), (.5, .5, 0), (.5, 0, .5)])
+    sp = Species("Al", csl, sigma_inc, sigma_abs, mass, debye_temp, pos)
+    return Crystal(a, a, a, [sp], temperature, dcutoff, name="fcc_a4.05")
+
+
+def ortho_random(seed=1, dcutoff=0.1, temperature=293.0, nspecies=3, atoms_per_species=8):
+    """Orthorhombic cell a,b,c ~ U(5,15) Aa with 3 species x 8 atoms at
+    seeded fractional positions; b_coh ~ U(2,8) fm; T_Debye ~ U(200,600) K.
+    Masses (not fixed by the workload definition) ~ U(20,60) amu, sigma_inc ~
+    U(0,2) b, sigma_abs ~ U(0,1) b."""
+    rng = np.random.default_rng(seed)
+    a, b, c = rng.uniform(5.0, 15.0, size=3)
+    species = []
+    for s in range(nspecies):
+        csl = rng.uniform(2.0, 8.0) / 10.0
+        pos = _norm_positions(rng.uniform(0.0, 1.0, size=(atoms_per_species, 3)))
+        species.append(Species(f"X{s}", csl, rng.uniform(0.0, 2.0), rng.uniform(0.0, 1.0),
+                               rng.uniform(20.0, 60.0), rng.uniform(200.0, 600.0), pos))
+    return Crystal(float(a), float(b), float(c), species, temperature, dcutoff, name=f"ortho_seed{seed}")
+
+
+# ----------------------------------------------------------------------------
+# S(alpha,beta) from a vibrational density of states (incoherent Gaussian
+# approximation, phonon expansion; Sjolander 1958).  Standard textbook method,
+# written independently of the reference's VDOS expansion code.
+
+def vdos_3gauss(seed=1, emax=0.15, npts=500):
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0.01, 0.12, size=3)
+    widths = rng.uniform(0.003, 0.02, size=3)
+    amps = rng.uniform(0.5, 1.5, size=3)
+    e = np.linspace(0.0, emax, npts)
+    rho = np.zeros_like(e)
+    for c0, w, A in zip(centres, widths, amps):
+        rho += A * np.exp(-0.5 * ((e - c0) / w) ** 2)
+    rho[0] = 0.0
+    # Debye-like low-energy behaviour rho ~ E^2 below the first grid point region
+    rho *= np.clip(e / 0.004, 0.0, 1.0) ** 2
+    return e, rho
+
+
+def phonon_expand_sab(vdos_e, vdos_rho, temperature, mass_amu, alpha, beta, nmax=None):
+    """S(alpha,beta) (non-symmetric, S(a,-b) = e^b S(a,b) detailed balance)
+    for an incoherent harmonic solid with the given VDOS, excluding the
+    elastic (n=0) term.  alpha uses the neutron mass (NCrystal convention);
+    the phonon expansion uses alpha/A with A = M/m_n."""
+    kT = K_BOLTZMANN * temperature
+    A = mass_amu / K_NEUTRON_MASS_AMU
+    alpha = np.asarray(alpha, dtype=np.float64)
+    beta = np.asarray(beta, dtype=np.float64)
+    amax = alpha[-1] / A
+    # Debye-Waller exponent lam from the VDOS (needed for the number of terms)
+    eb = vdos_e / kT
+    bpos_grid = np.linspace(1e-6, eb[-1], 4000)
+    rho_pos = np.interp(bpos_grid, eb, vdos_rho)
+    norm = np.trapezoid(rho_pos, bpos_grid)
+    lam_est = np.trapezoid(rho_pos / norm / (bpos_grid * np.tanh(bpos_grid / 2.0)), bpos_grid)
+    if nmax is None:
+        x = amax * lam_est
+        nmax = int(min(600, max(30, x + 12.0 * np.sqrt(x + 1.0) + 10)))
+    # fine uniform beta grid; the margin keeps the truncated linear convolutions
+    # exact inside the window that is finally used
+    db = 0.02
+    bneed = max(abs(beta[0]), abs(beta[-1])) + 2.0
+    bmax_fine = bneed + 60.0
+    nfine = int(np.ceil(bmax_fine / db))
+    bf = np.arange(-nfine, nfine + 1) * db
+    rho_b = np.interp(np.abs(bf), eb, vdos_rho, right=0.0) / norm
+    T1 = np.zeros_like(bf)
+    m = (rho_b > 0.0) & (bf != 0.0)
+    # P(b) e^{-b/2} with P = rho/(2 b sinh(b/2)); e^{-b/2}/sinh(b/2) = 2/(e^b - 1)
+    T1[m] = rho_b[m] / (bf[m] * np.expm1(bf[m]))
+    i0 = nfine
+    T1[i0] = 0.5 * (T1[i0 - 1] + T1[i0 + 1])
+    lam = np.trapezoid(T1, bf)
+    T1 = T1 / lam
+    L = len(bf)
+    nfft = 1 << int(np.ceil(np.log2(2 * L)))
+    F1 = np.fft.rfft(T1 * db, nfft)
+    Tn = T1.copy()
+    Tlist = [Tn]
+    for n in range(2, nmax + 1):
+        full = np.fft.irfft(np.fft.rfft(Tn, nfft) * F1, nfft)
+        Tn = np.maximum(full[nfine:nfine + L], 0.0)
+        Tlist.append(Tn)
+    Tarr = np.array(Tlist)  # (nmax, L)
+    # interpolate each T_n onto the requested beta grid
+    Tb = np.array([np.interp(beta, bf, t) for t in Tarr])  # (nmax, nb)
+    sab = np.zeros((beta.size, alpha.size))
+    from math import lgamma
+    for ia, al in enumerate(alpha):
+        x = al * lam / A
+        ln = np.array([n * np.log(x) - lgamma(n + 1) for n in range(1, nmax + 1)]) - x
+        w = np.exp(ln)
+        sab[:, ia] = w @ Tb
+    sab[sab < 1e-300] = 0.0
+    return sab
+
+
+def sab_table_c3(seed=1, temperature=293.0, mass_amu=26.98, nalpha=200, nbeta=800):
+    """200 x 800 S(alpha,beta) table from the 3-Gaussian VDOS (500 points up
+    to 0.15 eV) at T=293 K."""
+    e, rho = vdos_3gauss(seed)
+    kT = K_BOLTZMANN * temperature
+    emax_cover = 1.2  # eV, kinematic coverage of the alpha/beta grids
+    amax = 4.6 * emax_cover / kT
+    alpha = np.geomspace(1e-3, amax, nalpha)
+    bneg = -(emax_cover / kT) * 1.02
+    bpos = 25.0
+    u = np.linspace(-1.0, 1.0, nbeta)
+    # denser near beta=0
+    beta = np.where(u < 0, -np.abs(bneg) * np.abs(u) ** 1.5, bpos * np.abs(u) ** 1.5)
+    beta = np.unique(beta)
+    assert beta.size == nbeta
+    sab = phonon_expand_sab(e, rho, temperature, mass_amu, alpha, beta)
+    return dict(alpha=alpha, beta=beta, sab=sab, temperature=temperature, mass=mass_amu,
+                sigma_bound=1.5, vdos=(e, rho))
+
+
+def small_sab_table(seed=3, temperature=293.0, mass_amu=12.0, nalpha=40, nbeta=60, amax=40.0,
+                    bmin=-20.0, bmax=10.0):
+    """Small table for fast tests (high-E extender path is reached above Emax)."""
+    e, rho = vdos_3gauss(seed)
+    alpha = np.geomspace(2e-3, amax, nalpha)
+    beta = np.linspace(bmin, bmax, nbeta)
+    sab = phonon_expand_sab(e, rho, temperature, mass_amu, alpha, beta)
+    return dict(alpha=alpha, beta=beta, sab=sab, temperature=temperature, mass=mass_amu,
+                sigma_bound=4.0)
+
+
+# ----------------------------------------------------------------------------
+# neutron batches
+
+def energies_logu(n, emin, emax, seed=1):
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(emin), np.log(emax), size=n))
+
+
+def isotropic_dirs(n, seed=2):
+    rng = np.random.default_rng(seed)
+    uz = rng.uniform(-1.0, 1.0, size=n)
+    ph = rng.uniform(0.0, 2 * np.pi, size=n)
+    s = np.sqrt(1.0 - uz * uz)
+    return np.stack([s * np.cos(ph), s * np.sin(ph), uz], axis=1)
+
+
+def cone_dirs(n, axis, half_angle_rad, seed=3):
+    rng = np.random.default_rng(seed)
+    axis = np.asarray(axis, dtype=np.float64)
+    axis = axis / np.linalg.norm(axis)
+    cmin = np.cos(half_angle_rad)
+    ct = rng.uniform(cmin, 1.0, size=n)
+    ph = rng.uniform(0.0, 2 * np.pi, size=n)
+    st = np.sqrt(1.0 - ct * ct)
+    tmp = np.array([1.0, 0.0, 0.0]) if abs(axis[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    e1 = np.cross(axis, tmp)
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(axis, e1)
+    return (ct[:, None] * axis[None, :] + (st * np.cos(ph))[:, None] * e1[None, :]
+            + (st * np.sin(ph))[:, None] * e2[None, :])

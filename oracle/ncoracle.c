@@ -1,0 +1,1902 @@
+/*
+ * ncoracle.c -- TEST INFRASTRUCTURE ONLY (the CPU checker; never linked by the
+ * product library).  Plain-C restatement of NCrystal v4.4.2's per-neutron
+ * physics.  Citations are reference file:line (relative to the reference repo).
+ *
+ * How each part is pinned (see tests/test_oracle_*.py):
+ *  - Philox4x32-10: Random123 published known-answer vectors.
+ *  - sincos_2pix: libm sin/cos to 1e-15 (closed form).
+ *  - integrateAlphaInterval_fast: mpmath known answers held by the reference
+ *    test tests/src/app_sabutils/main.cc:88-135 (1e-14).
+ *  - debyeIsotropicMSD: golden table tests/src/app_debyemsd/test.log.
+ *  - hkl enumeration: golden hkl list tests/scripts/nosg.log (Al, dcutoff 0.4),
+ *    plus brute-force multiplicity counting and the survey-recorded values for
+ *    the fcc a=4.05 material (first family d, |F|^2 to 17 digits).
+ *  - PowderBragg xs: survey-recorded xs(4 Aa) = 1.6308398365212686 b, closed
+ *    form sum_{2d>=lambda} c_i/E, threshold behaviour.
+ *  - PowderBragg mu: single-family closed form mu = 1-2(lambda/2d)^2 and the
+ *    exact family probabilities (chi-square).
+ *  - ElInc xs / mu: closed forms (1-exp(-t))/t and the exp(a mu) CDF.
+ *  - SAB integrator / xs: brute-force trapezoid/loglin integration of the
+ *    cell-interpolated table on tiny grids, 1/E free-gas limits.
+ *  - SAB Alg1 sampler: statistical agreement with the xs-weighted kinematic
+ *    integral of the interpolated table (KS), kinematic validity invariants.
+ *  - free-gas extender: xs shape closed form (erf), sampler vs analytic
+ *    free-gas S(alpha,beta) marginals.
+ */
+#include "ncoracle.h"
+#include <math.h>
+#include <float.h>
+#include <stdlib.h>
+#include <string.h>
+#include <pthread.h>
+
+/* ------------------------------------------------------------------ */
+/* constants: ncrystal_core/include/NCrystal/core/NCDefs.hh:79-128,834-898 and
+   ncrystal_core/include/NCrystal/internal/utils/NCMath.hh:34-47 */
+#define K_BOLTZMANN 8.6173303e-5
+#define K_NEUTRON_MASS_AMU 1.00866491588
+#define K_PI 3.1415926535897932384626433832795028841971694
+#define K_2PI 6.2831853071795864769252867665590057683943388
+#define K_PIHALF 1.5707963267948966192313216916397514420985847
+#define K_PISQ 9.86960440108935861883449099987615113531369941
+#define K_4PISQ 39.4784176043574344753379639995046045412547976
+#define K_INVSQRTPI 0.564189583547756286948079451560772585844050629
+#define K_WL2EKIN 0.081804209605330899
+#define K_EKIN2WLSQINV 12.22430978582345950656
+#define K_C_AAS 299792458e10
+#define K_DALTON2EVC2 931494095.17
+#define K_PLANCK 4.135667662e-15
+#define K_INV2PI 0.159154943091895335768883763372514362034459646
+
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dclamp(double v, double lo, double hi) { return dmin(dmax(v, lo), hi); }
+static inline int in_interval(double a, double b, double x) { return (a <= x) & (x <= b); }
+static inline double wl2ekin(double wl) { double w2 = wl * wl; return w2 ? K_WL2EKIN / w2 : INFINITY; }
+
+/* StableSum (Neumaier): NCMath.hh:526-536 */
+typedef struct { double s, c; } ssum_t;
+static inline void ss_add(ssum_t* ss, double x) {
+  double t = ss->s + x;
+  ss->c += fabs(ss->s) >= fabs(x) ? (ss->s - t) + x : (x - t) + ss->s;
+  ss->s = t;
+}
+static inline double ss_sum(const ssum_t* ss) { return ss->s + ss->c; }
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (Random123, Salmon, Moraes, Dror, Shaw, SC'11). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0; c1 = lo1; c2 = hi0 ^ c3 ^ k1; c3 = lo0;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* randUInt64ToFP01: NCDefs.hh:1308-1329 (maps to (0,1]) */
+double orc_u64_to_fp01(uint64_t x) {
+  const double r1 = (double)(x >> 11) * 0x1.0p-53;
+  const double r2 = (double)(x & 0x7FF) * 0x1.0p-64;
+  return (1.0 - r1) - r2;
+}
+
+/* Per-neutron stream: key = seed, counter = (block, stream, idx_lo, idx_hi).
+   Draw 2j uses the low 64 bits of block j, draw 2j+1 the high 64 bits. */
+typedef struct {
+  uint32_t key[2];
+  uint32_t ctr[4];
+  uint64_t buf[2];
+  int nleft;
+  int64_t ndraws;
+} rng_t;
+static void rng_init(rng_t* r, uint64_t seed, uint32_t stream, uint64_t idx) {
+  r->key[0] = (uint32_t)seed; r->key[1] = (uint32_t)(seed >> 32);
+  r->ctr[0] = 0; r->ctr[1] = stream; r->ctr[2] = (uint32_t)idx; r->ctr[3] = (uint32_t)(idx >> 32);
+  r->nleft = 0; r->ndraws = 0;
+}
+static double rng_gen(rng_t* r) {
+  if (!r->nleft) {
+    uint32_t o[4];
+    orc_philox4x32_10(r->ctr, r->key, o);
+    r->ctr[0]++;
+    r->buf[0] = ((uint64_t)o[1] << 32) | o[0];
+    r->buf[1] = ((uint64_t)o[3] << 32) | o[2];
+    r->nleft = 2;
+  }
+  r->ndraws++;
+  return orc_u64_to_fp01(r->buf[2 - r->nleft--]);
+}
+void orc_rng_fill(uint64_t seed, uint32_t stream, uint64_t idx, int n, double* out) {
+  rng_t r; rng_init(&r, seed, stream, idx);
+  for (int i = 0; i < n; ++i) out[i] = rng_gen(&r);
+}
+
+/* ------------------------------------------------------------------ */
+/* sincos_mpi2pi2: ncrystal_core/src/utils/NCMath.cc:96-125 (Taylor at A/2 +
+   double angle); sincos_02pi: NCMath.hh:382-392; sincos_2pix: NCMath.hh:318-326 */
+static void sincos_mpi2pi2(double A, double* cosA, double* sinA) {
+  double x = 0.5 * A;
+  double mx2 = -x * x;
+  double s2 = x * (1.0 + mx2 * (1.66666666666666666666666666666666666666666667e-1
+                   + mx2 * (8.33333333333333333333333333333333333333333333e-3
+                   + mx2 * (1.98412698412698412698412698412698412698412698e-4
+                   + mx2 * (2.75573192239858906525573192239858906525573192e-6
+                   + mx2 * (2.50521083854417187750521083854417187750521084e-8
+                   + mx2 * (1.60590438368216145993923771701549479327257105e-10
+                   + mx2 * (7.64716373181981647590113198578807044415510024e-13))))))));
+  double c2m1 = mx2 * (0.5
+                + mx2 * (4.16666666666666666666666666666666666666666667e-2
+                + mx2 * (1.38888888888888888888888888888888888888888889e-3
+                + mx2 * (2.48015873015873015873015873015873015873015873e-5
+                + mx2 * (2.75573192239858906525573192239858906525573192e-7
+                + mx2 * (2.08767569878680989792100903212014323125434237e-9
+                + mx2 * (1.14707455977297247138516979786821056662326504e-11
+                + mx2 * (4.77947733238738529743820749111754402759693765e-14))))))));
+  double k = 2.0 * c2m1;
+  *sinA = (k + 2.0) * s2;
+  *cosA = 1.0 + k * (c2m1 + 2.0);
+}
+static void sincos_02pi(double A, double* cosA, double* sinA) {
+  A -= K_PI;
+  double Aabs = fabs(A);
+  sincos_mpi2pi2(dmin(Aabs, K_PI - Aabs), cosA, sinA);
+  *cosA = copysign(*cosA, Aabs - K_PIHALF);
+  *sinA = copysign(*sinA, -A);
+}
+void orc_sincos_2pix(double x, double* s, double* c) {
+  sincos_02pi((x - floor(x)) * K_2PI, c, s);
+}
+
+/* ------------------------------------------------------------------ */
+/* Romberg integration: ncrystal_core/src/utils/NCRomberg.cc:58-140 */
+typedef double (*rfunc_t)(double, void*);
+typedef int (*raccept_t)(unsigned, double, double, void*);
+static double romberg(rfunc_t f, raccept_t acc, void* ctx, double a, double b, int* ok) {
+  double h = (b - a);
+  double fv[17];
+  {
+    double delta = h * 0.0625;
+    for (int i = 0; i < 17; ++i) fv[i] = f(a + delta * (double)i, ctx);
+  }
+  h *= 0.5;
+  const double R00 = (fv[0] + fv[16]) * h;
+  const double R10 = h * fv[8] + 0.5 * R00;
+  const double R11 = (4. / 3.) * R10 + (-1. / 3.) * R00;
+  h *= 0.5;
+  const double R20 = h * (fv[4] + fv[12]) + 0.5 * R10;
+  const double R21 = (4. / 3.) * R20 + (-1. / 3.) * R10;
+  const double R22 = (16. / 15.) * R21 + (-1. / 15.) * R11;
+  h *= 0.5;
+  const double R30 = h * ((fv[2] + fv[6]) + (fv[10] + fv[14])) + 0.5 * R20;
+  const double R31 = (4. / 3.) * R30 + (-1. / 3.) * R20;
+  const double R32 = (16. / 15.) * R31 + (-1. / 15.) * R21;
+  const double R33 = (64. / 63.) * R32 + (-1. / 63.) * R22;
+  h *= 0.5;
+  const double R40 = h * (((fv[1] + fv[3]) + (fv[5] + fv[7])) + ((fv[9] + fv[11]) + (fv[13] + fv[15]))) + 0.5 * R30;
+  const double R41 = (4. / 3.) * R40 + (-1. / 3.) * R30;
+  const double R42 = (16. / 15.) * R41 + (-1. / 15.) * R31;
+  const double R43 = (64. / 63.) * R42 + (-1. / 63.) * R32;
+  const double R44 = (256. / 255.) * R43 + (-1. / 255.) * R33;
+  *ok = 1;
+  if (acc(4, R33, R44, ctx)) return R44;
+  double c5 = 0.0;
+  {
+    double off = a + h * 0.5;
+    for (double i = 0; i < 16.0; ++i) c5 += f(off + h * i, ctx);
+  }
+  h *= 0.5;
+  const double R50 = h * c5 + 0.5 * R40;
+  const double R51 = (4. / 3.) * R50 + (-1. / 3.) * R40;
+  const double R52 = (16. / 15.) * R51 + (-1. / 15.) * R41;
+  const double R53 = (64. / 63.) * R52 + (-1. / 63.) * R42;
+  const double R54 = (256. / 255.) * R53 + (-1. / 255.) * R43;
+  const double R55 = (1024. / 1023.) * R54 + (-1. / 1023.) * R44;
+  if (acc(5, R44, R55, ctx)) return R55;
+  enum { maxlevel = 16 };
+  double cache1[maxlevel], cache2[maxlevel];
+  double *row_prev = cache1, *row = cache2;
+  row_prev[0] = R50; row_prev[1] = R51; row_prev[2] = R52;
+  row_prev[3] = R53; row_prev[4] = R54; row_prev[5] = R55;
+  unsigned nj = 16;
+  for (unsigned i = 6; i < maxlevel; ++i) {
+    double hh = h;
+    h *= 0.5;
+    nj *= 2;
+    double c = 0.0;
+    {
+      double off = a + h, nn = nj;
+      for (double j = 0; j < nn; ++j) c += f(off + hh * j, ctx);
+    }
+    row[0] = h * c + 0.5 * row_prev[0];
+    double n_k = 1.;
+    for (unsigned j = 0; j < i; ++j) {
+      n_k *= 4.0;
+      row[j + 1] = (n_k * row[j] - row_prev[j]) / (n_k - 1.0);
+    }
+    if (acc(i, row_prev[i - 1], row[i], ctx)) return row[i];
+    double* tmp = row_prev; row_prev = row; row = tmp;
+  }
+  *ok = 0;
+  return row_prev[maxlevel - 1];
+}
+
+/* ------------------------------------------------------------------ */
+/* Debye MSD: ncrystal_core/src/phys_utils/NCDebyeMSD.cc:27-86 */
+static double debye_shape_f(double x, void* ctx) {
+  (void)ctx;
+  if (fabs(x) < 1e-4) {
+    double x2 = x * x;
+    return 1 - x * 0.5 + x2 * 0.08333333333333333333333333333333 - x2 * x2 * 0.00138888888888888888888888888888888889;
+  }
+  if (x <= 700) {
+    const double em1 = expm1(x);
+    return x / em1;
+  }
+  return (x > 760 ? 0.0 : exp(-x + log(-x / expm1(-x))));
+}
+static int debye_shape_acc(unsigned lvl, double prev, double est, void* ctx) {
+  (void)ctx;
+  return lvl > 11 || (lvl > 7 && fabs(est - prev) < 1e-10);
+}
+double orc_debye_msd(double dt, double t, double am) {
+  const double hbar = K_PLANCK * K_INV2PI;
+  const double kk = 3.0 * hbar * hbar * K_C_AAS * K_C_AAS / (K_DALTON2EVC2 * K_BOLTZMANN);
+  const double scale = kk / (am * dt);
+  const double x = t / dt;
+  double shape;
+  if (x < 1e-50) {
+    shape = 0.25;
+  } else {
+    int ok;
+    shape = 0.25 + x * x * romberg(debye_shape_f, debye_shape_acc, NULL, 0.0, 1.0 / x, &ok);
+  }
+  return scale * shape;
+}
+
+/* ------------------------------------------------------------------ */
+/* hkl enumeration, orthogonal cells, no space group:
+   ncrystal_core/src/extd_utils/NCFillHKL.cc:34-49 (keygen), :129-144 (W_hkl),
+   :170-226 (precalc), :230-450 (loop + family grouping);
+   ncrystal_core/src/utils/NCLatticeUtils.cc:95-117 (exact diagonal reciprocal
+   lattice), :177-216 (estimateHKLRange); sort:
+   ncrystal_core/src/infobld/NCInfoBuilder.cc:98-117,452 (stable, dhkl_compare);
+   HKL ordering ncrystal_core/include/NCrystal/core/NCTypes.hh:978-981. */
+static unsigned famkey(double fsq, double dsp) {
+  const int ef = (int)ceil(log10(fsq));
+  const double mf = fsq * pow(10.0, (double)(-ef));
+  const int ed = (int)ceil(log10(dsp));
+  const double md = dsp * pow(10.0, (double)(-ed));
+  unsigned part = (unsigned)(mf * 1000 + 0.5) * 4000000u + (unsigned)(md * 1000 + 0.5) * 4000u;
+  return (unsigned)((double)part + dmax((double)(3000 + ef * 30 + ed), 0.0));
+}
+static int floorhkl(double x) {
+  double xc = floor(x * (1.0 + 1e-12));
+  if (xc <= 1.0) return 1;
+  if (xc >= (double)2147483647) return 2147483647;
+  return (int)(xc + 0.5);
+}
+typedef struct { int h, k, l; } hkl3;
+static int hkl_less(hkl3 a, hkl3 o) { /* NCTypes.hh:978-981 */
+  return (a.h != o.h ? o.h < a.h : (a.k != o.k ? o.k < a.k : o.l < a.l));
+}
+typedef struct {
+  hkl3 rep; int mult; double fsq, dsp;
+  hkl3* members; int nmem, cap;
+} fam_t;
+/* key -> list of family indices, newest first (std::multimap insert with the
+   lower_bound hint puts the new element before existing equal keys) */
+typedef struct { unsigned key; int* idx; int n, cap; int used; } kslot_t;
+typedef struct { kslot_t* slots; size_t cap, n; } kmap_t;
+static kslot_t* kmap_get(kmap_t* m, unsigned key, int create) {
+  if (create && (m->n + 1) * 2 > m->cap) {
+    size_t ncap = m->cap ? m->cap * 2 : 1024;
+    kslot_t* ns = (kslot_t*)calloc(ncap, sizeof(kslot_t));
+    for (size_t i = 0; i < m->cap; ++i)
+      if (m->slots[i].used) {
+        size_t j = (m->slots[i].key * 2654435761u) & (ncap - 1);
+        while (ns[j].used) j = (j + 1) & (ncap - 1);
+        ns[j] = m->slots[i];
+      }
+    free(m->slots); m->slots = ns; m->cap = ncap;
+  }
+  if (!m->cap) return NULL;
+  size_t j = (key * 2654435761u) & (m->cap - 1);
+  while (m->slots[j].used) {
+    if (m->slots[j].key == key) return &m->slots[j];
+    j = (j + 1) & (m->cap - 1);
+  }
+  if (!create) return NULL;
+  m->slots[j].used = 1; m->slots[j].key = key; m->slots[j].n = 0; m->slots[j].cap = 0; m->slots[j].idx = NULL;
+  m->n++;
+  return &m->slots[j];
+}
+static int cmp_hkl_qsort(const void* a, const void* b) {
+  hkl3 x = *(const hkl3*)a, y = *(const hkl3*)b;
+  if (hkl_less(x, y)) return -1;
+  if (hkl_less(y, x)) return 1;
+  return 0;
+}
+/* dhkl_compare(rh,lh): NCInfoBuilder.cc:98-117 (true if rh sorts before lh) */
+static int dhkl_before(const fam_t* rh, const fam_t* lh) {
+  if (fabs(lh->dsp - rh->dsp) > 1e-6) return lh->dsp < rh->dsp;
+  double lfm = lh->fsq * lh->mult, rfm = rh->fsq * rh->mult;
+  if (fabs(lfm - rfm) > 1e-6) return lfm < rfm;
+  if (lh->mult != rh->mult) return lh->mult < rh->mult;
+  return hkl_less(lh->rep, rh->rep);
+}
+static void stable_sort_fams(fam_t* v, int n) {
+  /* merge sort (stable) */
+  if (n < 2) return;
+  fam_t* tmp = (fam_t*)malloc(sizeof(fam_t) * (size_t)n);
+  for (int w = 1; w < n; w *= 2) {
+    for (int lo = 0; lo < n; lo += 2 * w) {
+      int mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+      int i = lo, j = mid, k = lo;
+      while (i < mid && j < hi) {
+        if (dhkl_before(&v[j], &v[i])) tmp[k++] = v[j++];
+        else tmp[k++] = v[i++];
+      }
+      while (i < mid) tmp[k++] = v[i++];
+      while (j < hi) tmp[k++] = v[j++];
+    }
+    memcpy(v, tmp, sizeof(fam_t) * (size_t)n);
+  }
+  free(tmp);
+}
+
+int orc_hkl_orthogonal(double a, double b, double c, int nspecies, const int* natoms,
+                       const double* positions, const double* csl, const double* msd,
+                       double dcutoff, double fsquarecut, double merge_tol, int maxout,
+                       int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq) {
+  const double rl0 = K_2PI / a, rl1 = K_2PI / b, rl2 = K_2PI / c;
+  const double invd = 1.0 / dcutoff;
+  const int max_h = floorhkl(a * invd), max_k = floorhkl(b * invd), max_l = floorhkl(c * invd);
+  if (fsquarecut >= 0.0) fsquarecut = dmax(fsquarecut, 1.0e-300);
+  /* species with bcoh==0 are ignored (NCFillHKL.cc:180-182) */
+  int ns = 0;
+  const double** pos = (const double**)malloc(sizeof(double*) * (size_t)(nspecies + 1));
+  int* np = (int*)malloc(sizeof(int) * (size_t)(nspecies + 1));
+  double* vcsl = (double*)malloc(sizeof(double) * (size_t)(nspecies + 1));
+  double* vmsd = (double*)malloc(sizeof(double) * (size_t)(nspecies + 1));
+  double* thr = (double*)malloc(sizeof(double) * (size_t)(nspecies + 1));
+  double* whkl = (double*)malloc(sizeof(double) * (size_t)(nspecies + 1));
+  double* fac = (double*)malloc(sizeof(double) * (size_t)(nspecies + 1));
+  {
+    const double* p = positions;
+    for (int i = 0; i < nspecies; ++i) {
+      if (fabs(csl[i]) > 0.0) {
+        pos[ns] = p; np[ns] = natoms[i]; vcsl[ns] = csl[i]; vmsd[ns] = msd[i];
+        if (fsquarecut < 0.01 && fsquarecut > 1.0e-300) thr[ns] = log(fabs(csl[i]) / fsquarecut);
+        else thr[ns] = INFINITY;
+        ++ns;
+      }
+      p += 3 * natoms[i];
+    }
+  }
+  const double ksq_lo = dclamp((K_4PISQ * (1.0 - 1e-14)) / (INFINITY * INFINITY), DBL_MIN, DBL_MAX);
+  const double ksq_hi = dclamp((K_4PISQ * (1.0 + 1e-14)) / (dcutoff * dcutoff), DBL_MIN, DBL_MAX);
+  const double dcut_lo = dclamp(dcutoff, DBL_MIN, DBL_MAX), dcut_hi = DBL_MAX;
+
+  int nf = 0, capf = 0; fam_t* fams = NULL;
+  kmap_t km = {0};
+  int err = 0;
+  for (int h = 0; h <= max_h && !err; ++h) {
+    for (int k = (h ? -max_k : 0); k <= max_k && !err; ++k) {
+      for (int l = -max_l; l <= max_l; ++l) {
+        const double wx = rl0 * (double)h + 0.0 * (double)k + 0.0 * (double)l;
+        const double wy = 0.0 * (double)h + rl1 * (double)k + 0.0 * (double)l;
+        const double wz = 0.0 * (double)h + 0.0 * (double)k + rl2 * (double)l;
+        const double ksq = wx * wx + wy * wy + wz * wz;
+        if (!in_interval(ksq_lo, ksq_hi, ksq)) continue;
+        const double kk2 = 0.5 * ksq;
+        for (int i = 0; i < ns; ++i) whkl[i] = kk2 * vmsd[i];
+        double upper = 0.0;
+        for (int i = 0; i < ns; ++i) {
+          if (whkl[i] > thr[i]) { fac[i] = 0.0; continue; }
+          double f = vcsl[i] * exp(-whkl[i]);
+          fac[i] = f;
+          upper += np[i] * fabs(f);
+        }
+        if (upper * upper * 2.0 < fsquarecut) continue;
+        if (h == 0 && k == 0 && l <= 0) continue;
+        ssum_t re = {0, 0}, im = {0, 0};
+        for (int i = 0; i < ns; ++i) {
+          double f = fac[i];
+          if (!f) continue;
+          ssum_t cs = {0, 0}, sn = {0, 0};
+          const double* p = pos[i];
+          for (int j = 0; j < np[i]; ++j) {
+            const double ph = (double)h * p[3 * j] + (double)k * p[3 * j + 1] + (double)l * p[3 * j + 2];
+            double sv, cv;
+            orc_sincos_2pix(ph, &sv, &cv);
+            ss_add(&cs, cv); ss_add(&sn, sv);
+          }
+          ss_add(&re, ss_sum(&cs) * f);
+          ss_add(&im, ss_sum(&sn) * f);
+        }
+        const double r = ss_sum(&re), im2 = ss_sum(&im);
+        const double fsq = r * r + im2 * im2;
+        if (fsq < fsquarecut) continue;
+        const double kval = sqrt(ksq);
+        const double invk = 1.0 / kval;
+        const double dsp = K_2PI * invk;
+        if (!in_interval(dcut_lo, dcut_hi, dsp)) continue;
+        unsigned key = famkey(fsq, dsp);
+        kslot_t* sl = kmap_get(&km, key, 1);
+        int found = -1;
+        for (int q = sl->n - 1; q >= 0; --q) { /* newest first */
+          fam_t* f = &fams[sl->idx[q]];
+          if (fabs(fsq - f->fsq) < merge_tol * (fsq + f->fsq) &&
+              fabs(dsp - f->dsp) < merge_tol * (dsp + f->dsp)) { found = sl->idx[q]; break; }
+        }
+        hkl3 cur = {h, k, l};
+        if (found >= 0) {
+          fam_t* f = &fams[found];
+          f->mult += 2;
+          if (f->nmem == f->cap) { f->cap *= 2; f->members = (hkl3*)realloc(f->members, sizeof(hkl3) * (size_t)f->cap); }
+          f->members[f->nmem++] = cur;
+        } else {
+          if (nf > 1000000) { err = -2; break; }
+          if (nf == capf) { capf = capf ? capf * 2 : 1024; fams = (fam_t*)realloc(fams, sizeof(fam_t) * (size_t)capf); }
+          fam_t* f = &fams[nf];
+          f->rep = cur; f->mult = 2; f->fsq = fsq; f->dsp = dsp;
+          f->cap = 8; f->nmem = 1; f->members = (hkl3*)malloc(sizeof(hkl3) * 8); f->members[0] = cur;
+          if (sl->n == sl->cap) { sl->cap = sl->cap ? sl->cap * 2 : 4; sl->idx = (int*)realloc(sl->idx, sizeof(int) * (size_t)sl->cap); }
+          sl->idx[sl->n++] = nf;
+          ++nf;
+        }
+      }
+    }
+  }
+  int result = err;
+  if (!err) {
+    for (int i = 0; i < nf; ++i) {
+      qsort(fams[i].members, (size_t)fams[i].nmem, sizeof(hkl3), cmp_hkl_qsort);
+      fams[i].rep = fams[i].members[0];
+    }
+    stable_sort_fams(fams, nf);
+    if (nf > maxout) {
+      result = -1;
+    } else {
+      for (int i = 0; i < nf; ++i) {
+        out_hkl[3 * i] = fams[i].rep.h; out_hkl[3 * i + 1] = fams[i].rep.k; out_hkl[3 * i + 2] = fams[i].rep.l;
+        out_mult[i] = fams[i].mult; out_dsp[i] = fams[i].dsp; out_fsq[i] = fams[i].fsq;
+      }
+      result = nf;
+    }
+  }
+  for (int i = 0; i < nf; ++i) free(fams[i].members);
+  free(fams);
+  for (size_t i = 0; i < km.cap; ++i) if (km.slots[i].used) free(km.slots[i].idx);
+  free(km.slots);
+  free(pos); free(np); free(vcsl); free(vmsd); free(thr); free(whkl); free(fac);
+  return result;
+}
+
+/* ------------------------------------------------------------------ */
+/* process objects */
+enum { P_PB = 1, P_ELINC = 2, P_SAB = 3, P_COMP = 4 };
+
+typedef struct {
+  int n; double* e2d; double* fdm; double threshold;
+} pb_t;
+
+typedef struct { int n; double* msd; double* bixs; } elinc_t;
+
+typedef struct {
+  double alpha, sval, logsval; unsigned alpha_idx;
+} sapoint_t;
+typedef struct { sapoint_t front, back; double prob_front, prob_notback; } ainfo_t;
+
+typedef struct {
+  int kind; /* 0 no-scatter, 1 Alg1 */
+  int ibeta_low;
+  double firstbin;
+  int nvals;
+  double *bx, *by, *bcdf;
+  ainfo_t* infos;
+  int ninfos;
+} sampE_t;
+
+typedef struct {
+  int na, nb;
+  double *alpha, *beta, *sab, *logsab, *cumul;
+  double T, kT, mass, sigma_bound, sigma_free, suggested_emax;
+  double fg_ca; /* free-gas xs: a^2 = ca*E */
+  int ne; double* egrid; double* xs;
+  sampE_t* samplers;
+  double k_ext; /* XSProvider high-E constant */
+  double k1, k2; /* sampler high-E constants */
+} sab_t;
+
+typedef struct { int n; orc_proc** comps; double* scales; } comp_t;
+
+struct orc_proc {
+  int type;
+  union { pb_t pb; elinc_t ei; sab_t sab; comp_t comp; } u;
+};
+
+/* ------------------------------------------------------------------ */
+/* PowderBragg: ncrystal_core/src/powderbragg/NCPowderBragg.cc:68-110 (init),
+   :152-176 (xs), :178-216 (sampling) */
+typedef struct { double d, f; } pairdd;
+static int cmp_pair_desc(const void* a, const void* b) {
+  const pairdd* x = (const pairdd*)a; const pairdd* y = (const pairdd*)b;
+  /* std::greater<pair>: descending by first, then by second */
+  if (x->d != y->d) return x->d > y->d ? -1 : 1;
+  if (x->f != y->f) return x->f > y->f ? -1 : 1;
+  return 0;
+}
+orc_proc* orc_create_powderbragg(double v0n, const double* dsp, const double* fsqm, int n) {
+  if (!(v0n > 0)) return NULL;
+  /* PowderBragg(const Info&) merges entries with identical dspacing first
+     (NCPowderBragg.cc:122-145) */
+  pairdd* data = (pairdd*)malloc(sizeof(pairdd) * (size_t)(n > 0 ? n : 1));
+  int nd = 0;
+  for (int i = 0; i < n; ++i) {
+    if (nd == 0 || data[nd - 1].d != dsp[i]) { data[nd].d = dsp[i]; data[nd].f = fsqm[i]; ++nd; }
+    else data[nd - 1].f += fsqm[i];
+  }
+  double xsectfact = 0.5 / v0n;
+  xsectfact *= wl2ekin(1.0);
+  qsort(data, (size_t)nd, sizeof(pairdd), cmp_pair_desc);
+  orc_proc* p = (orc_proc*)calloc(1, sizeof(orc_proc));
+  p->type = P_PB;
+  pb_t* pb = &p->u.pb;
+  pb->e2d = (double*)malloc(sizeof(double) * (size_t)(nd > 0 ? nd : 1));
+  pb->fdm = (double*)malloc(sizeof(double) * (size_t)(nd > 0 ? nd : 1));
+  ssum_t s = {0, 0};
+  double prev = -INFINITY;
+  int m = 0;
+  for (int i = 0; i < nd; ++i) {
+    if (!(data[i].d > 0.0)) { free(data); free(pb->e2d); free(pb->fdm); free(p); return NULL; }
+    if (fabs(prev - data[i].d) < 1e-11) {
+      double cc = data[i].d * data[i].f * xsectfact;
+      ss_add(&s, cc);
+      pb->fdm[m - 1] = ss_sum(&s);
+    } else {
+      prev = data[i].d;
+      double cc = data[i].d * data[i].f * xsectfact;
+      ss_add(&s, cc);
+      pb->fdm[m] = ss_sum(&s);
+      pb->e2d[m] = wl2ekin(2.0 * data[i].d);
+      ++m;
+    }
+  }
+  free(data);
+  if (m == 0 || pb->fdm[m - 1] <= 0.0) { m = 0; pb->threshold = INFINITY; }
+  else pb->threshold = pb->e2d[0];
+  pb->n = m;
+  return p;
+}
+int orc_pb_ntables(const orc_proc* p) { return p->type == P_PB ? p->u.pb.n : -1; }
+void orc_pb_tables(const orc_proc* p, double* e2d, double* fdm) {
+  memcpy(e2d, p->u.pb.e2d, sizeof(double) * (size_t)p->u.pb.n);
+  memcpy(fdm, p->u.pb.fdm, sizeof(double) * (size_t)p->u.pb.n);
+}
+static size_t upper_bound_d(const double* v, size_t lo, size_t hi, double x) {
+  /* first index in [lo,hi) with v[i] > x */
+  size_t n = hi - lo;
+  while (n > 0) {
+    size_t h2 = n / 2, m = lo + h2;
+    if (!(x < v[m])) { lo = m + 1; n -= h2 + 1; } else n = h2;
+  }
+  return lo;
+}
+static size_t lower_bound_d(const double* v, size_t lo, size_t hi, double x) {
+  /* first index in [lo,hi) with v[i] >= x */
+  size_t n = hi - lo;
+  while (n > 0) {
+    size_t h2 = n / 2, m = lo + h2;
+    if (v[m] < x) { lo = m + 1; n -= h2 + 1; } else n = h2;
+  }
+  return lo;
+}
+static double pb_xs(const pb_t* pb, double E) {
+  if (!pb->n || E < pb->threshold || !isfinite(E)) return 0.0;
+  size_t idx = upper_bound_d(pb->e2d, 1, (size_t)pb->n, E) - 1;
+  double inv = 1.0 / E;
+  return pb->fdm[idx] * inv;
+}
+static double pb_sample_mu(const pb_t* pb, rng_t* rng, double E) {
+  if (!pb->n || E < pb->threshold || !isfinite(E)) return 1.0;
+  size_t idx = upper_bound_d(pb->e2d, 1, (size_t)pb->n, E) - 1;
+  double u = rng_gen(rng);
+  size_t i = lower_bound_d(pb->fdm, 0, idx, u * pb->fdm[idx]);
+  double s2 = pb->e2d[i] / E;
+  return 1.0 - 2.0 * s2;
+}
+
+/* ------------------------------------------------------------------ */
+/* pickRandIdxByWeight: ncrystal_core/src/utils/NCRandUtils.cc:198-222 and
+   NCRandUtils.hh:130-133 (no draw when n==1) */
+static size_t pick_idx(rng_t* rng, const double* commul, size_t n) {
+  if (n == 1) return 0;
+  double r = rng_gen(rng);
+  if (n < 5) {
+    const double rc = commul[n - 1] * r;
+    for (size_t i = 0; i < n; ++i) if (commul[i] > rc) return i;
+    return n - 1;
+  }
+  size_t it = lower_bound_d(commul, 0, n, commul[n - 1] * r);
+  return it < n - 1 ? it : n - 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* ElInc: ncrystal_core/src/phys_utils/NCElIncXS.cc:34-51 (1-exp(-t))/t,
+   :58-79 (set), :81-115 (sampleMuMonoAtomic), :130-141 (commulative xs),
+   :178-190 (sampleMu); NCMath.hh:434-440 (exp_smallarg_approx) */
+orc_proc* orc_create_elinc(int n, const double* msd, const double* bixs, const double* scale) {
+  orc_proc* p = (orc_proc*)calloc(1, sizeof(orc_proc));
+  p->type = P_ELINC;
+  p->u.ei.n = n;
+  p->u.ei.msd = (double*)malloc(sizeof(double) * (size_t)n);
+  p->u.ei.bixs = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; ++i) { p->u.ei.msd[i] = msd[i]; p->u.ei.bixs[i] = bixs[i] * scale[i]; }
+  return p;
+}
+static double eval_1mexpmtdivt(double t) {
+  if (t < 0.01) return (1 + t * (-0.5 + t * 0.16666666666666666666666666666666666666666667 * (1. - 0.25 * t)));
+  if (t > 24.0) return 1.0 / t;
+  t = -t;
+  return expm1(t) / t;
+}
+static double exp_smallarg_approx(double x) {
+  return 1.0 + x * (1 + x * (0.5 + x * (0.16666666666666666666666666666666666667 + x * (0.04166666666666666666666666666666666667
+         + x * (0.00833333333333333333333333333333333333 + x * (0.00138888888888888888888888888888888889
+         + x * 0.00019841269841269841269841269841269841))))));
+}
+static double elinc_xs_commul(const elinc_t* ei, double E, double* commul) {
+  const double kkk = 16.0 * K_PISQ * K_EKIN2WLSQINV;
+  double e = kkk * E;
+  double xs = 0.0;
+  for (int i = 0; i < ei->n; ++i) {
+    xs += ei->bixs[i] * eval_1mexpmtdivt(ei->msd[i] * e);
+    if (commul) commul[i] = xs;
+  }
+  return xs;
+}
+static double elinc_mu_mono(rng_t* rng, double E, double msd) {
+  const double kkk = 8.0 * K_PISQ * K_EKIN2WLSQINV;
+  double twoksq = kkk * E;
+  double a = twoksq * msd;
+  if (a < 0.01) {
+    double maxval = exp_smallarg_approx(a);
+    while (1) {
+      double mu = rng_gen(rng) * 2.0 - 1.0;
+      if (rng_gen(rng) * maxval < exp_smallarg_approx(a * mu)) return mu;
+    }
+  }
+  return dclamp(log1p(rng_gen(rng) * expm1(2.0 * a)) / a - 1.0, -1.0, 1.0);
+}
+static double elinc_sample_mu(const elinc_t* ei, rng_t* rng, double E) {
+  if (ei->n == 1) return elinc_mu_mono(rng, E, ei->msd[0]);
+  double cm[64];
+  double* commul = ei->n <= 64 ? cm : (double*)malloc(sizeof(double) * (size_t)ei->n);
+  elinc_xs_commul(ei, E, commul);
+  size_t i = pick_idx(rng, commul, (size_t)ei->n);
+  if (commul != cm) free(commul);
+  return elinc_mu_mono(rng, E, ei->msd[i]);
+}
+
+/* ------------------------------------------------------------------ */
+/* Kinematics: ncrystal_core/include/NCrystal/internal/phys_utils/NCKinUtils.hh:64-121,
+   ncrystal_core/src/phys_utils/NCKinUtils.cc:25-58 */
+static void alpha_limits(double e, double beta, double* lo, double* hi) {
+  const double kk = e + beta;
+  if (!(kk >= 0.0)) { *lo = 1.0; *hi = -1.0; return; }
+  const double a = kk + e;
+  const double b = 2.0 * sqrt(e * kk);
+  double am;
+  if (fabs(beta) < 0.01 * e) {
+    const double x = beta / e;
+    const double c9 = -715. / 32768., c8 = 429. / 16384., c7 = -33. / 1024., c6 = 21. / 512.,
+                 c5 = -7. / 128., c4 = 5. / 64., c3 = -1. / 8., c2 = 1. / 4.;
+    am = beta * x * (c2 + x * (c3 + x * (c4 + x * (c5 + x * (c6 + x * (c7 + x * (c8 + x * c9)))))));
+  } else {
+    am = dmax(0.0, a - b);
+  }
+  *lo = am; *hi = a + b;
+}
+void orc_alpha_limits(double e, double beta, double* lo, double* hi) { alpha_limits(e, beta, lo, hi); }
+static int mu_iso_at_beta(double beta, double e) { return beta <= e * (-1.0 + 1e-14); }
+static int ab_to_demu(double alpha, double beta, double E, double kT, double* dE, double* mu) {
+  *dE = beta * kT;
+  const double ef = E + *dE;
+  const double denom = 2.0 * sqrt(E * ef);
+  if (!denom) return -1;
+  ssum_t s = {0, 0};
+  ss_add(&s, E); ss_add(&s, ef); ss_add(&s, -alpha * kT);
+  *mu = dclamp(ss_sum(&s) / denom, -1.0, 1.0);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* SAB utilities: ncrystal_core/include/NCrystal/internal/sab/NCSABUtils.hh:176-300 */
+static double interp_loglin_fast(double a, double fa, double b, double fb, double x,
+                                 double logfa, double logfb) {
+  const double bma = b - a;
+  const double mid = 0.5 * (b + a);
+  const int linlin = (fa * fb == 0.0);
+  if (x < mid) {
+    const double r = (x - a) / bma;
+    return linlin ? (fa + (fb - fa) * r) : exp(logfa + (logfb - logfa) * r);
+  } else {
+    const double s = (b - x) / bma;
+    return linlin ? (fb + (fa - fb) * s) : exp(logfb + (logfa - logfb) * s);
+  }
+}
+double orc_integrate_alpha_interval_fast(double a1, double s1, double a2, double s2,
+                                         double logs1, double logs2) {
+  const double da = a2 - a1;
+  const double ps = s1 + s2;
+  const double ds = s2 - s1;
+  if (dmin(s1, s2) < 1e-300) return 0.5 * da * ps;
+  if (fabs(ds) > 0.006 * ps) return da * ds / (logs2 - logs1);
+  const double y = ds / ps;
+  const double c1 = 0.166666666666666666666666666666666666666666667;
+  const double c2 = 0.0444444444444444444444444444444444444444444444;
+  const double c3 = 0.0232804232804232804232804232804232804232804233;
+  const double ysq = y * y;
+  return da * ps * (0.5 - ysq * (c1 + ysq * (c2 + ysq * c3)));
+}
+static double sample_loglin_fast(double a, double fa, double b, double fb, double r,
+                                 double logfa, double logfb) {
+  double df = fb - fa;
+  if (fa * fb * df != 0.0) {
+    double a_sub_b = a - b;
+    double lfab = logfb - logfa;
+    if (a_sub_b * lfab != 0.0)
+      return a_sub_b * log(fa * exp(a * lfab / a_sub_b) / (fa + r * df)) / lfab;
+    df = 0.0;
+  }
+  if (!df) return a + r * (b - a);
+  double x = (b - a) * sqrt(r);
+  return fa ? b - x : a + x;
+}
+
+/* SABData -> derived data (logS, cumulative alpha integrals):
+   ncrystal_core/src/sab/NCSABIntegrator.cc:105-140 */
+static void sab_derive(sab_t* s) {
+  const size_t N = (size_t)s->na * (size_t)s->nb;
+  s->logsab = (double*)malloc(sizeof(double) * N);
+  s->cumul = (double*)calloc(N, sizeof(double));
+  for (size_t i = 0; i < N; ++i) s->logsab[i] = s->sab[i] > 0.0 ? log(s->sab[i]) : -INFINITY;
+  size_t g = 0;
+  for (int ib = 0; ib < s->nb; ++ib, ++g) {
+    double cumul = 0.0;
+    for (int ai = 0; ai < s->na - 1; ++ai, ++g) {
+      double integ = orc_integrate_alpha_interval_fast(s->alpha[ai], s->sab[g], s->alpha[ai + 1], s->sab[g + 1],
+                                                       s->logsab[g], s->logsab[g + 1]);
+      s->cumul[g + 1] = (cumul += integ);
+    }
+  }
+}
+
+/* activeGridRanges: ncrystal_core/src/sab/NCSABUtils.cc:460-536 (restated with
+   the same incremental iterator walk) */
+static void active_grid_ranges(const sab_t* s, double ediv, int* ranges_lo, int* ranges_hi,
+                               int* nranges, int* ibeta_low) {
+  const double* ag = s->alpha;
+  const int na = s->na;
+  *ibeta_low = 0;
+  int nr = 0;
+  const double af = ag[0], ab = ag[na - 1];
+  int itLow = 0, itUpp = na - 1;
+  const int itLast = na - 1;
+  for (int ib = 0; ib < s->nb; ++ib) {
+    const double beta = s->beta[ib];
+    double alow = -1.0, aupp = -2.0;
+    if (beta > -ediv) alpha_limits(ediv, beta, &alow, &aupp);
+    if (ab <= alow || af >= aupp || aupp < alow) {
+      if (nr == 0) *ibeta_low = ib + 1;
+      else { ranges_lo[nr] = na; ranges_hi[nr] = na; ++nr; }
+      continue;
+    }
+    while (ag[itLow] > alow && itLow > 0) --itLow;
+    while (itLow < itLast && ag[itLow + 1] <= alow) ++itLow;
+    if (itUpp < itLow) itUpp = itLow;
+    while (ag[itUpp] < aupp && itUpp < itLast) ++itUpp;
+    while (itUpp > 0 && ag[itUpp - 1] >= aupp) --itUpp;
+    ranges_lo[nr] = itLow; ranges_hi[nr] = itUpp; ++nr;
+  }
+  *nranges = nr;
+}
+
+/* createTailedBreakdown: ncrystal_core/src/sab/NCSABUtils.cc:538-633 */
+typedef struct {
+  double xs_front, xs_middle, xs_back;
+  unsigned imiddle_low, imiddle_upp;
+  double f_alpha, f_sval, f_logsval, b_alpha, b_sval, b_logsval;
+  int narrow;
+} tb_t;
+static void set_tail(const double* ag, const double* sab, const double* lsab, unsigned aidx,
+                     double alpha, double* a_out, double* s_out, double* ls_out) {
+  *a_out = alpha;
+  *s_out = interp_loglin_fast(ag[aidx], sab[aidx], ag[aidx + 1], sab[aidx + 1], alpha, lsab[aidx], lsab[aidx + 1]);
+  *ls_out = log(dmax(*s_out, DBL_MIN));
+}
+static tb_t tailed_breakdown(const double* ag, int na, const double* sab, const double* lsab,
+                             const double* cumul, double alow, double aupp,
+                             unsigned ilo, unsigned iup) {
+  tb_t tb; memset(&tb, 0, sizeof(tb));
+  alow = dclamp(alow, ag[0], ag[na - 1]);
+  aupp = dclamp(aupp, ag[0], ag[na - 1]);
+  if (ilo == iup || alow == aupp) return tb;
+  if (ilo + 1 == iup) {
+    tb.narrow = 1;
+    set_tail(ag, sab, lsab, ilo, alow, &tb.f_alpha, &tb.f_sval, &tb.f_logsval);
+    set_tail(ag, sab, lsab, ilo, aupp, &tb.b_alpha, &tb.b_sval, &tb.b_logsval);
+    tb.xs_front = orc_integrate_alpha_interval_fast(tb.f_alpha, tb.f_sval, tb.b_alpha, tb.b_sval,
+                                                    tb.f_logsval, tb.b_logsval);
+    return tb;
+  }
+  tb.imiddle_low = ilo; tb.imiddle_upp = iup;
+  if (alow >= ag[ilo]) {
+    set_tail(ag, sab, lsab, ilo, alow, &tb.f_alpha, &tb.f_sval, &tb.f_logsval);
+    tb.xs_front = orc_integrate_alpha_interval_fast(tb.f_alpha, tb.f_sval, ag[ilo + 1], sab[ilo + 1],
+                                                    tb.f_logsval, lsab[ilo + 1]);
+    ++tb.imiddle_low;
+  }
+  if (aupp <= ag[iup]) {
+    set_tail(ag, sab, lsab, iup - 1, aupp, &tb.b_alpha, &tb.b_sval, &tb.b_logsval);
+    tb.xs_back = orc_integrate_alpha_interval_fast(ag[iup - 1], sab[iup - 1], tb.b_alpha, tb.b_sval,
+                                                   lsab[iup - 1], tb.b_logsval);
+    --tb.imiddle_upp;
+  }
+  tb.xs_middle = (tb.imiddle_upp > tb.imiddle_low ? cumul[tb.imiddle_upp] - cumul[tb.imiddle_low] : 0.0);
+  return tb;
+}
+
+/* PointwiseDist ctor + percentileWithIndex:
+   ncrystal_core/src/utils/NCPointwiseDist.cc:32-107 */
+static int pwdist_init(int n, double* x, double* y, double* cdf) {
+  if (n < 2) return -1;
+  for (int i = 1; i < n; ++i) if (x[i] < x[i - 1]) return -1;
+  for (int i = 0; i < n; ++i) if (y[i] < 0. || isinf(y[i])) return -1;
+  ssum_t tot = {0, 0};
+  cdf[0] = 0.;
+  for (int i = 1; i < n; ++i) {
+    double area = (x[i] - x[i - 1]) * 0.5 * (y[i] + y[i - 1]);
+    if (area < 0) return -1;
+    ss_add(&tot, area);
+    cdf[i] = ss_sum(&tot);
+  }
+  double tv = ss_sum(&tot);
+  if (!(tv > 0.0)) return -1;
+  double nf = 1.0 / tv;
+  for (int i = 0; i < n; ++i) cdf[i] *= nf;
+  for (int i = 0; i < n; ++i) y[i] *= nf;
+  cdf[n - 1] = 1.0;
+  return 0;
+}
+static double pwdist_percentile(int n, const double* x, const double* y, const double* cdf,
+                                double p, unsigned* idx_out) {
+  if (p == 1.) { *idx_out = (unsigned)(n - 2); return x[n - 1]; }
+  size_t lb = lower_bound_d(cdf, 0, (size_t)n, p);
+  size_t i = lb < (size_t)(n - 1) ? lb : (size_t)(n - 1);
+  if (i < 1) i = 1;
+  double dx = x[i] - x[i - 1];
+  double c = (p - cdf[i - 1]);
+  double a = y[i - 1];
+  double d = y[i] - a;
+  double zdx;
+  if (!a) {
+    zdx = d > 0.0 ? sqrt((2.0 * c * dx) / d) : 0.5 * dx;
+  } else {
+    double e = d * c / (dx * a * a);
+    if (fabs(e) > 1e-7) zdx = (sqrt(1.0 + 2.0 * e) - 1.0) * dx * a / d;
+    else zdx = (1 + 0.5 * e * (e - 1.0)) * c / a;
+  }
+  *idx_out = (unsigned)(i - 1);
+  return dclamp(x[i - 1] + zdx, x[i - 1], x[i]);
+}
+
+/* analyseEnergyPoint: ncrystal_core/src/sab/NCSABIntegrator.cc:352-559 */
+static double sab_analyse(const sab_t* s, double ekin, int doSampler, sampE_t* out) {
+  const double* bg = s->beta;
+  const int nb = s->nb, na = s->na;
+  const double kT = s->kT;
+  const double ediv = ekin / kT;
+  double bll = -ediv;
+  int starts_kin = 1;
+  if (bll < bg[0]) {
+    double c1 = bg[0] - (bg[1] - bg[0]) * 1e-6;
+    double c2 = bg[0] - fabs(bg[0]) * 1e-13;
+    double c3 = nexttoward(bg[0], (long double)bll);
+    bll = dmin(dmin(c1, c2), c3);
+    starts_kin = 0;
+  }
+  int* rlo = (int*)malloc(sizeof(int) * (size_t)nb);
+  int* rhi = (int*)malloc(sizeof(int) * (size_t)nb);
+  int nr, ibl;
+  active_grid_ranges(s, ediv, rlo, rhi, &nr, &ibl);
+  if (ibl >= nb) {
+    free(rlo); free(rhi);
+    if (doSampler) { memset(out, 0, sizeof(*out)); out->kind = 0; }
+    return 0.0;
+  }
+  if (ibl > 0 && bll < bg[ibl - 1]) { bll = bg[ibl - 1]; starts_kin = 0; }
+  const int nrel = nb - ibl;
+  double prev_b = bll, prev_xs = 0.0;
+  double *bx = NULL, *by = NULL;
+  ainfo_t* infos = NULL;
+  int nv = 0, ni = 0;
+  if (doSampler) {
+    bx = (double*)malloc(sizeof(double) * (size_t)(nrel + 1));
+    by = (double*)malloc(sizeof(double) * (size_t)(nrel + 1));
+    infos = (ainfo_t*)calloc((size_t)nrel, sizeof(ainfo_t));
+    bx[nv] = prev_b; by[nv] = prev_xs; ++nv;
+  }
+  ssum_t tot = {0, 0};
+  int next_kin = starts_kin;
+  for (int j = 0; j < nrel; ++j) {
+    const double beta = bg[ibl + j];
+    if (beta == bll) continue;
+    double alow, aupp;
+    alpha_limits(ediv, beta, &alow, &aupp);
+    double xsb = 0.0;
+    unsigned ilo = (unsigned)rlo[j], iup = (unsigned)rhi[j];
+    tb_t tb; memset(&tb, 0, sizeof(tb));
+    if (iup > ilo && aupp > alow) {
+      size_t off = (size_t)na * (size_t)(j + ibl);
+      tb = tailed_breakdown(s->alpha, na, s->sab + off, s->logsab + off, s->cumul + off, alow, aupp, ilo, iup);
+      xsb = tb.xs_front + tb.xs_back + tb.xs_middle;
+    }
+    if (doSampler) {
+      ainfo_t* inf = &infos[ni++];
+      memset(inf, 0, sizeof(*inf));
+      if (xsb > 0.0) {
+        inf->front.alpha = tb.f_alpha; inf->front.sval = tb.f_sval; inf->front.logsval = tb.f_logsval;
+        inf->back.alpha = tb.b_alpha; inf->back.sval = tb.b_sval; inf->back.logsval = tb.b_logsval;
+        if (tb.narrow) {
+          inf->prob_front = 1.0;
+        } else {
+          inf->prob_front = tb.xs_front / xsb;
+          inf->prob_notback = 1.0 - tb.xs_back / xsb;
+          inf->front.alpha_idx = tb.imiddle_low;
+          inf->back.alpha_idx = tb.imiddle_upp;
+        }
+      } else {
+        inf->prob_front = 2.0;
+        inf->front.alpha = alow;
+        inf->back.alpha = aupp;
+      }
+    }
+    if (next_kin) {
+      next_kin = 0;
+      double db_real = (beta - prev_b);
+      prev_b -= db_real * (1.0 / 3.0);
+      if (doSampler) bx[0] = prev_b;
+    }
+    ss_add(&tot, 0.5 * (beta - prev_b) * (xsb + prev_xs));
+    prev_b = beta; prev_xs = xsb;
+    if (doSampler) { bx[nv] = prev_b; by[nv] = prev_xs; ++nv; }
+  }
+  free(rlo); free(rhi);
+  double xs_total = ss_sum(&tot) * s->sigma_bound / (4 * ediv);
+  if (!(xs_total >= 0.0)) xs_total = 0.0;
+  if (!doSampler) return xs_total;
+  memset(out, 0, sizeof(*out));
+  if (xs_total == 0.0) { free(bx); free(by); free(infos); out->kind = 0; return xs_total; }
+  out->kind = 1;
+  out->ibeta_low = ibl;
+  out->firstbin = starts_kin ? bll : 1.0;
+  out->nvals = nv;
+  out->bx = bx; out->by = by;
+  out->bcdf = (double*)malloc(sizeof(double) * (size_t)nv);
+  out->infos = infos; out->ninfos = ni;
+  if (pwdist_init(nv, bx, by, out->bcdf) != 0) out->kind = -1;
+  return xs_total;
+}
+
+/* free-gas xs shape: ncrystal_core/src/phys_utils/NCFreeGasUtils.cc:64-84 */
+double orc_freegas_xs_shape(double a2) {
+  if (a2 > 36.0) return 1.0 + 0.5 / a2;
+  const double a = sqrt(a2);
+  if (a < 0.1) {
+    if (a == 0.0) return INFINITY;
+    const double c1 = 2.0 / 3.0, c2 = 1.0 / 15.0, c3 = 1.0 / 105.0, c4 = 1.0 / 756.0, c5 = 1.0 / 5940.0;
+    return K_INVSQRTPI * (2.0 / a + a * (c1 - a2 * (c2 - a2 * (c3 - a2 * (c4 - a2 * c5)))));
+  }
+  const double inva = 1.0 / a;
+  return (1.0 + 0.5 * inva * inva) * erf(a) + K_INVSQRTPI * exp(-a2) * inva;
+}
+static double fg_xs(const sab_t* s, double E) { return s->sigma_free * orc_freegas_xs_shape(s->fg_ca * E); }
+
+/* egrid: ncrystal_core/src/sab/NCSABIntegrator.cc:147-281;
+   logspace/geomspace ncrystal_core/src/utils/NCMath.cc:43-66 */
+static double sab_xs_at(const sab_t* s, double e) { return sab_analyse(s, e, 0, NULL); }
+static double determine_emin(const sab_t* s, double e_upp) {
+  const double tol = 1e-3;
+  double e_old = e_upp * 0.9;
+  double f_old = sqrt(e_old) * sab_xs_at(s, e_old);
+  while (1) {
+    const double e_new = 0.5 * e_old;
+    if (e_new <= 1e-30 * e_upp) return dmin(e_old, e_upp * 0.01);
+    const double f_new = sqrt(e_new) * sab_xs_at(s, e_new);
+    if (f_new == 0.0) return 0.001 * e_upp;
+    if (fabs(f_old / f_new - 1.0) < tol) return e_old;
+    e_old = e_new; f_old = f_new;
+  }
+}
+static double determine_emax(const sab_t* s, double eul) {
+  const double step = 0.95;
+  double prev = INFINITY, ekin = eul;
+  const double elow = eul * 1e-4;
+  while (ekin > elow) {
+    double dist = fabs(sab_xs_at(s, ekin) - fg_xs(s, ekin));
+    if (dist > prev) return step * ekin;
+    prev = dist;
+    ekin *= step;
+  }
+  return 0.0;
+}
+static double* geomspace(double start, double stop, unsigned num) {
+  double* v = (double*)malloc(sizeof(double) * num);
+  double lstart = log10(start), lstop = log10(stop);
+  double interval = (lstop - lstart) / (num - 1);
+  for (unsigned i = 0; i < num; ++i) { v[i] = pow(10.0, lstart); lstart += interval; }
+  v[num - 1] = pow(10.0, lstop);
+  v[0] = start; v[num - 1] = stop;
+  return v;
+}
+
+orc_proc* orc_create_sab(int na, int nb, const double* alpha, const double* beta,
+                         const double* sab, double temperature, double mass_amu,
+                         double sigma_bound, double suggested_emax, int negrid_in,
+                         const double* egrid_in) {
+  orc_proc* p = (orc_proc*)calloc(1, sizeof(orc_proc));
+  p->type = P_SAB;
+  sab_t* s = &p->u.sab;
+  s->na = na; s->nb = nb;
+  s->alpha = (double*)malloc(sizeof(double) * (size_t)na);
+  s->beta = (double*)malloc(sizeof(double) * (size_t)nb);
+  s->sab = (double*)malloc(sizeof(double) * (size_t)na * (size_t)nb);
+  memcpy(s->alpha, alpha, sizeof(double) * (size_t)na);
+  memcpy(s->beta, beta, sizeof(double) * (size_t)nb);
+  memcpy(s->sab, sab, sizeof(double) * (size_t)na * (size_t)nb);
+  s->T = temperature; s->kT = K_BOLTZMANN * temperature;
+  s->mass = mass_amu; s->sigma_bound = sigma_bound; s->suggested_emax = suggested_emax;
+  {
+    double r = mass_amu / (K_NEUTRON_MASS_AMU + mass_amu);
+    s->sigma_free = (r * r) * sigma_bound;
+    double A = (1.0 / K_NEUTRON_MASS_AMU) * mass_amu;
+    s->fg_ca = A / s->kT;
+  }
+  sab_derive(s);
+  /* setupEnergyGrid */
+  if (negrid_in > 3) {
+    s->ne = negrid_in;
+    s->egrid = (double*)malloc(sizeof(double) * (size_t)negrid_in);
+    memcpy(s->egrid, egrid_in, sizeof(double) * (size_t)negrid_in);
+  } else {
+    double emin = 0, emax = 0; unsigned npts = 0;
+    if (negrid_in == 3) { emin = egrid_in[0]; emax = egrid_in[1]; npts = (unsigned)egrid_in[2]; }
+    if (npts == 0) npts = 300;
+    if (emax == 0.0 && suggested_emax > 0.0) emax = suggested_emax;
+    if (emax == 0.0) {
+      const double bmin = beta[0], amax = alpha[na - 1];
+      const double eul = s->kT * (bmin - amax) * (bmin - amax) / (4 * amax);
+      emax = determine_emax(s, eul);
+      if (!(emax > 0.0)) emax = 0.5 * eul;
+    }
+    if (emin == 0.0) emin = determine_emin(s, dmin(emax * 0.01, 0.01 * s->kT));
+    s->ne = (int)npts;
+    s->egrid = geomspace(emin, emax, npts);
+  }
+  s->xs = (double*)malloc(sizeof(double) * (size_t)s->ne);
+  s->samplers = (sampE_t*)calloc((size_t)s->ne, sizeof(sampE_t));
+  for (int i = 0; i < s->ne; ++i) s->xs[i] = sab_analyse(s, s->egrid[i], 1, &s->samplers[i]);
+  /* SABXSProvider::setData NCSABXSProvider.cc:35-51 and SABSampler::setData
+     NCSABSampler.cc:40-57 */
+  const double emax = s->egrid[s->ne - 1];
+  const double extxs = fg_xs(s, emax);
+  s->k_ext = (s->xs[s->ne - 1] - extxs) * emax;
+  s->k1 = s->xs[s->ne - 1] * emax;
+  s->k2 = extxs * emax;
+  return p;
+}
+int orc_sab_negrid(const orc_proc* p) { return p->u.sab.ne; }
+void orc_sab_egrid(const orc_proc* p, double* eg, double* xs) {
+  memcpy(eg, p->u.sab.egrid, sizeof(double) * (size_t)p->u.sab.ne);
+  memcpy(xs, p->u.sab.xs, sizeof(double) * (size_t)p->u.sab.ne);
+}
+void orc_sab_sampler_info(const orc_proc* p, int ie, int* ibl, int* nv, int* kind, double* fb) {
+  const sampE_t* S = &p->u.sab.samplers[ie];
+  *ibl = S->ibeta_low; *nv = S->nvals; *kind = S->kind; *fb = S->firstbin;
+}
+void orc_sab_sampler_arrays(const orc_proc* p, int ie, double* bx, double* by, double* bcdf,
+                            double* idbl, int* iidx) {
+  const sampE_t* S = &p->u.sab.samplers[ie];
+  if (S->kind != 1) return;
+  memcpy(bx, S->bx, sizeof(double) * (size_t)S->nvals);
+  memcpy(by, S->by, sizeof(double) * (size_t)S->nvals);
+  memcpy(bcdf, S->bcdf, sizeof(double) * (size_t)S->nvals);
+  for (int i = 0; i < S->ninfos; ++i) {
+    const ainfo_t* f = &S->infos[i];
+    idbl[8 * i + 0] = f->front.alpha; idbl[8 * i + 1] = f->front.sval; idbl[8 * i + 2] = f->front.logsval;
+    idbl[8 * i + 3] = f->back.alpha; idbl[8 * i + 4] = f->back.sval; idbl[8 * i + 5] = f->back.logsval;
+    idbl[8 * i + 6] = f->prob_front; idbl[8 * i + 7] = f->prob_notback;
+    iidx[2 * i] = (int)f->front.alpha_idx; iidx[2 * i + 1] = (int)f->back.alpha_idx;
+  }
+}
+void orc_sab_derived(const orc_proc* p, double* logsab, double* cumul) {
+  size_t N = (size_t)p->u.sab.na * (size_t)p->u.sab.nb;
+  memcpy(logsab, p->u.sab.logsab, sizeof(double) * N);
+  memcpy(cumul, p->u.sab.cumul, sizeof(double) * N);
+}
+double orc_sab_analyse_xs(const orc_proc* p, double e) { return sab_xs_at(&p->u.sab, e); }
+
+/* SABXSProvider::crossSection: ncrystal_core/src/sab/NCSABXSProvider.cc:54-95 */
+static double sab_xs(const sab_t* s, double E) {
+  size_t up = upper_bound_d(s->egrid, 0, (size_t)s->ne, E);
+  if (up == (size_t)s->ne) return s->k_ext / E + fg_xs(s, E);
+  if (up == 0) return E > 0.0 ? sqrt(s->egrid[0] / E) * s->xs[0] : INFINITY;
+  size_t lo = up - 1;
+  const double dXS = s->xs[up] - s->xs[lo];
+  const double dE = s->egrid[up] - s->egrid[lo];
+  return s->xs[lo] + dXS * (E - s->egrid[lo]) / dE;
+}
+
+/* SABSamplerAtE_Alg1::sampleAlpha: ncrystal_core/src/sab/NCSABSamplerModels.cc:157-234 */
+static double alg1_sample_alpha(const sab_t* s, const sampE_t* S, size_t ibeta, double rp) {
+  const ainfo_t* info = &S->infos[ibeta - (size_t)S->ibeta_low];
+  const int na = s->na;
+  const double* cumul = s->cumul + (size_t)na * ibeta;
+  const double* sab = s->sab + (size_t)na * ibeta;
+  const double* lsab = s->logsab + (size_t)na * ibeta;
+  const double* ag = s->alpha;
+  if (rp <= info->prob_front) {
+    if (info->prob_front == 2.0) {
+      double da = info->back.alpha - info->front.alpha;
+      return info->front.alpha + rp * da;
+    } else if (info->prob_front == 1.0) {
+      return sample_loglin_fast(info->front.alpha, info->front.sval, info->back.alpha, info->back.sval,
+                                rp, info->front.logsval, info->back.logsval);
+    } else {
+      double p2 = dclamp(rp / info->prob_front, DBL_MIN, 1.0);
+      unsigned ia = info->front.alpha_idx;
+      return sample_loglin_fast(info->front.alpha, info->front.sval, ag[ia], sab[ia], p2,
+                                info->front.logsval, lsab[ia]);
+    }
+  } else if (rp <= info->prob_notback) {
+    double p2 = dclamp((rp - info->prob_front) / (info->prob_notback - info->prob_front), 0.0, 1.0);
+    unsigned il = info->front.alpha_idx, iu = info->back.alpha_idx;
+    double sel = cumul[il] + p2 * (cumul[iu] - cumul[il]);
+    size_t e = upper_bound_d(cumul, il, (size_t)iu + 1, sel);
+    if (e > iu) return ag[iu];
+    if (e <= il) return ag[il];
+    size_t el = e - 1;
+    double binArea = cumul[e] - cumul[el];
+    double rr = dclamp((sel - cumul[el]) / binArea, DBL_MIN, 1.0);
+    size_t a0 = el, a1 = el + 1;
+    return sample_loglin_fast(ag[a0], sab[a0], ag[a1], sab[a1], rr, lsab[a0], lsab[a1]);
+  } else {
+    double p2 = dclamp((rp - info->prob_notback) / (1.0 - info->prob_notback), DBL_MIN, 1.0);
+    unsigned ib = info->back.alpha_idx;
+    return sample_loglin_fast(ag[ib], sab[ib], info->back.alpha, info->back.sval, p2,
+                              lsab[ib], info->back.logsval);
+  }
+}
+
+/* SABSamplerAtE_Alg1::sampleAlphaBeta: NCSABSamplerModels.cc:48-155.
+   Returns 0 on success, -1 when the 100-try loop is exhausted.
+   Deviation (documented): when ibetaSampled==0 without a kinematic endpoint and
+   ibeta_low>0, the reference reads alpha-sampler info at index -1 (undefined);
+   here that draw is rejected. */
+static int alg1_sample(const sab_t* s, const sampE_t* S, double ediv, rng_t* rng,
+                       double* a_out, double* b_out) {
+  const double* bg = s->beta;
+  unsigned it = 100 + 1;
+  while (--it) {
+    unsigned ibs;
+    double beta = pwdist_percentile(S->nvals, S->bx, S->by, S->bcdf, rng_gen(rng), &ibs);
+    if (ibs == 0 && S->firstbin <= 0.0) {
+      const double b0 = S->firstbin;
+      const double b1 = S->bx[1];
+      if (b1 < -ediv) continue;
+      const double db = b1 - b0;
+      double alphaval = 0.0;
+      for (int iii = 0; iii < 30; ++iii) {
+        beta = dmax(S->firstbin, b0 + db * rng_gen(rng));
+        if (beta < -ediv) break;
+        alphaval = alg1_sample_alpha(s, S, (size_t)S->ibeta_low, rng_gen(rng));
+        double l1, l2;
+        alpha_limits(-S->firstbin, beta, &l1, &l2);
+        if (in_interval(l1, l2, alphaval)) break;
+        if (iii == 29) { alphaval = 0.5 * (l1 + l2); break; }
+      }
+      if (beta < -ediv) continue;
+      double l1, l2;
+      alpha_limits(ediv, beta, &l1, &l2);
+      if (in_interval(l1, l2, alphaval)) { *a_out = alphaval; *b_out = beta; return 0; }
+      continue;
+    }
+    if (beta <= dmax(-ediv, bg[0])) continue;
+    double rp = rng_gen(rng);
+    size_t ibeta = (size_t)S->ibeta_low + ibs;
+    if (ibeta < (size_t)S->ibeta_low + 1) continue; /* see deviation note above */
+    double bl = bg[ibeta - 1];
+    double al = alg1_sample_alpha(s, S, ibeta - 1, rp);
+    double bh = bg[ibeta];
+    double ah = alg1_sample_alpha(s, S, ibeta, rp);
+    double alpha = al + (ah - al) * (beta - bl) / (bh - bl);
+    double l1, l2;
+    alpha_limits(ediv, beta, &l1, &l2);
+    if (in_interval(l1, l2, alpha)) { *a_out = alpha; *b_out = beta; return 0; }
+  }
+  return -1;
+}
+
+/* SABSampler::sampleAlphaBeta: ncrystal_core/src/sab/NCSABSampler.cc:158-227.
+   High-E extension (E >= Emax) is handled by sab_sample_highe. */
+static int sab_sample_highe(const sab_t* s, double E, rng_t* rng, double* a, double* b);
+static int sab_sample_ab(const sab_t* s, double E, rng_t* rng, double* a_out, double* b_out) {
+  size_t up = upper_bound_d(s->egrid, 0, (size_t)s->ne, E);
+  int ultra = 0;
+  const double ultra_e = s->egrid[0];
+  size_t isamp;
+  if (up == (size_t)s->ne) {
+    double a, b;
+    int rc = sab_sample_highe(s, E, rng, &a, &b);
+    if (rc) return rc;
+    if (a >= 0.0) { *a_out = a; *b_out = b; return 0; }
+    E = s->egrid[s->ne - 1];
+    isamp = (size_t)s->ne - 1;
+  } else if (up == 0) {
+    isamp = 0;
+    ultra = (E < ultra_e);
+  } else {
+    while (up + 1 != (size_t)s->ne && E * 1.05 > s->egrid[up]) ++up;
+    isamp = up;
+  }
+  const sampE_t* S = &s->samplers[isamp];
+  const double ediv = E / s->kT;
+  const double sdiv = ultra ? ultra_e / s->kT : ediv;
+  int loopmax = 100;
+  while (loopmax--) {
+    double alpha, beta;
+    if (S->kind == 0) { alpha = 0.0; beta = 0.0; }
+    else if (S->kind != 1) return -3;
+    else if (alg1_sample(s, S, sdiv, rng, &alpha, &beta)) return -1;
+    if (beta < -ediv) continue;
+    double l1, l2;
+    alpha_limits(ediv, beta, &l1, &l2);
+    if (in_interval(l1, l2, alpha)) { *a_out = alpha; *b_out = beta; return 0; }
+    if (ultra) { *a_out = l1 + rng_gen(rng) * (l2 - l1); *b_out = beta; return 0; }
+  }
+  return -2;
+}
+
+/* ------------------------------------------------------------------ */
+/* Free-gas sampler (SAB high-E extender):
+   ncrystal_core/src/phys_utils/NCFreeGasUtils.cc:86-498 (helpers, alpha
+   sampling) and :492-930 (FreeGasSampler), NCFreeGasUtils.hh:132-146;
+   ncrystal_core/src/sab/NCSABExtender.cc:25-50;
+   ncrystal_core/src/utils/NCRandUtils.cc:224-... (randExpDivSqrt,
+   randExpInterval).  Implemented in ncoracle_freegas.c. */
+static int orc_fg_sample_alpha_beta(double E, double T, double mass, rng_t* rng, double* alpha, double* beta);
+
+/* SABSampler::sampleHighE: ncrystal_core/src/sab/NCSABSampler.cc:59-156.
+   Sets *a = -1 when the table must be sampled at Emax. */
+static int sab_sample_highe(const sab_t* s, double E, rng_t* rng, double* a, double* b) {
+  const double emax = s->egrid[s->ne - 1];
+  const double exm = E * fg_xs(s, E);
+  const double P_inside = s->k1 / ((s->k1 - s->k2) + exm);
+  const double P_ext_inside = s->k2 / exm;
+  const double P_discard = (P_ext_inside >= P_inside ? (1.0 - P_inside / P_ext_inside) : 0.0);
+  if (P_discard > 0.95) return -4;
+  if (P_ext_inside < P_inside) {
+    double aa = 1.0 - P_ext_inside;
+    const double P_extra = aa > 1e-10 ? (P_inside - P_ext_inside) / aa : 1.0;
+    if (rng_gen(rng) < P_extra) { *a = -1.0; *b = 0.0; return 0; }
+  }
+  const double emax_div_kt = emax / s->kT;
+  while (1) {
+    double al, be;
+    int rc = orc_fg_sample_alpha_beta(E, s->T, s->mass, rng, &al, &be);
+    if (rc) return rc;
+    if (be <= -emax_div_kt) { *a = al; *b = be; return 0; }
+    double l1, l2;
+    alpha_limits(emax_div_kt, be, &l1, &l2);
+    if (!in_interval(l1, l2, al)) { *a = al; *b = be; return 0; }
+    if (P_discard && rng_gen(rng) < P_discard) continue;
+    *a = -1.0; *b = 0.0; return 0;
+  }
+}
+
+/* SABSampler::sampleDeltaEMu NCSABSampler.cc:229-236 + SABScatter
+   ncrystal_core/src/sabscatter/NCSABScatter.cc:93-100 */
+static int sab_sample_iso(const sab_t* s, rng_t* rng, double E, double* Ef, double* mu) {
+  double a, b;
+  int rc = sab_sample_ab(s, E, rng, &a, &b);
+  if (rc) return rc;
+  double dE, m;
+  if (mu_iso_at_beta(b, E / s->kT)) {
+    dE = b * s->kT;
+    m = rng_gen(rng) * 2.0 - 1.0;
+  } else {
+    if (ab_to_demu(a, b, E, s->kT, &dE, &m)) return -5;
+  }
+  *Ef = dmax(0.0, E + dE);
+  *mu = m;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Composition: ncrystal_core/src/interfaces/NCProcImpl.cc:166-204,353-389 */
+orc_proc* orc_create_composition(int n, orc_proc* const* comps, const double* scales) {
+  /* null processes and zero scales are dropped (NCProcImpl.cc:263-283) */
+  orc_proc* p = (orc_proc*)calloc(1, sizeof(orc_proc));
+  p->type = P_COMP;
+  p->u.comp.comps = (orc_proc**)malloc(sizeof(orc_proc*) * (size_t)(n > 0 ? n : 1));
+  p->u.comp.scales = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  int m = 0;
+  for (int i = 0; i < n; ++i) {
+    if (scales[i] == 0.0 || (comps[i]->type == P_PB && comps[i]->u.pb.n == 0)) continue;
+    p->u.comp.comps[m] = comps[i]; p->u.comp.scales[m] = scales[i]; ++m;
+  }
+  p->u.comp.n = m;
+  return p;
+}
+static double proc_xs(const orc_proc* p, double E);
+static int proc_sample_iso(const orc_proc* p, rng_t* rng, double E, double* Ef, double* mu);
+/* component domain: PowderBragg [threshold,inf), others [0,inf) */
+static int comp_domain_ok(const orc_proc* p, double E) {
+  if (p->type == P_PB) return p->u.pb.n > 0 && E >= p->u.pb.threshold;
+  return 1;
+}
+/* composition energy domain: union of component domains
+   (NCProcImpl.cc:285-296, EnergyDomain::contains NCTypes.hh:833-839) */
+static int comp_in_domain(const orc_proc* p, double E) {
+  double lo = INFINITY, hi = -INFINITY;
+  for (int i = 0; i < p->u.comp.n; ++i) {
+    const orc_proc* c = p->u.comp.comps[i];
+    double dlo = 0.0, dhi = INFINITY;
+    if (c->type == P_PB) dlo = c->u.pb.threshold;
+    if (dlo < dhi) { lo = dmin(lo, dlo); hi = dmax(hi, dhi); }
+  }
+  return (lo < hi) && (E >= lo) && (E <= hi);
+}
+static double proc_xs(const orc_proc* p, double E) {
+  switch (p->type) {
+    case P_PB: return pb_xs(&p->u.pb, E);
+    case P_ELINC: return elinc_xs_commul(&p->u.ei, E, NULL);
+    case P_SAB: return sab_xs(&p->u.sab, E);
+    case P_COMP: {
+      if (!comp_in_domain(p, E)) return 0.0;
+      double tot = 0.0;
+      for (int i = 0; i < p->u.comp.n; ++i) {
+        const orc_proc* c = p->u.comp.comps[i];
+        double xs = comp_domain_ok(c, E) ? proc_xs(c, E) : 0.0;
+        tot += p->u.comp.scales[i] * xs;
+      }
+      return tot;
+    }
+  }
+  return 0.0;
+}
+static int proc_sample_iso(const orc_proc* p, rng_t* rng, double E, double* Ef, double* mu) {
+  switch (p->type) {
+    case P_PB: *Ef = E; *mu = pb_sample_mu(&p->u.pb, rng, E); return 0;
+    case P_ELINC: *Ef = E; *mu = elinc_sample_mu(&p->u.ei, rng, E); return 0;
+    case P_SAB: return sab_sample_iso(&p->u.sab, rng, E, Ef, mu);
+    case P_COMP: {
+      if (!comp_in_domain(p, E)) { *Ef = E; *mu = 1.0; return 0; }
+      const int n = p->u.comp.n;
+      double cm[16];
+      double tot = 0.0;
+      for (int i = 0; i < n; ++i) {
+        const orc_proc* c = p->u.comp.comps[i];
+        double xs = comp_domain_ok(c, E) ? proc_xs(c, E) : 0.0;
+        cm[i] = (tot += p->u.comp.scales[i] * xs);
+      }
+      if (!(tot > 0.0)) { *Ef = E; *mu = 1.0; return 0; }
+      size_t ic = pick_idx(rng, cm, (size_t)n);
+      return proc_sample_iso(p->u.comp.comps[ic], rng, E, Ef, mu);
+    }
+  }
+  return -9;
+}
+
+void orc_free(orc_proc* p) {
+  if (!p) return;
+  switch (p->type) {
+    case P_PB: free(p->u.pb.e2d); free(p->u.pb.fdm); break;
+    case P_ELINC: free(p->u.ei.msd); free(p->u.ei.bixs); break;
+    case P_SAB: {
+      sab_t* s = &p->u.sab;
+      for (int i = 0; i < s->ne; ++i) {
+        free(s->samplers[i].bx); free(s->samplers[i].by); free(s->samplers[i].bcdf); free(s->samplers[i].infos);
+      }
+      free(s->samplers); free(s->egrid); free(s->xs); free(s->alpha); free(s->beta);
+      free(s->sab); free(s->logsab); free(s->cumul);
+      break;
+    }
+    case P_COMP: free(p->u.comp.comps); free(p->u.comp.scales); break;
+  }
+  free(p);
+}
+
+/* ------------------------------------------------------------------ */
+/* randDirectionGivenScatterMu: ncrystal_core/src/utils/NCRandUtils.cc:51-96 */
+static void rand_dir_given_mu(rng_t* rng, double mu, const double* in, double* out) {
+  const double m2 = in[0] * in[0] + in[1] * in[1] + in[2] * in[2];
+  const double invm = (fabs(m2 - 1.0) < 1e-14 ? 1.0 : 1.0 / sqrt(m2));
+  double u0 = in[0] * invm, u1 = in[1] * invm, u2 = in[2] * invm;
+  double t0 = 0, t1 = 0, t2 = 0, tm2 = 0.0;
+  do {
+    double x0 = 2.0 * rng_gen(rng) - 1.0;
+    double x1 = 2.0 * rng_gen(rng) - 1.0;
+    double s = x0 * x0 + x1 * x1;
+    if (s < 1.0) {
+      double t = 2.0 * sqrt(1.0 - s);
+      double a0 = x0 * t, a1 = x1 * t, a2 = 1.0 - 2.0 * s;
+      /* cross_inplace: (y*o.z-z*o.y, z*o.x-x*o.z, x*o.y-y*o.x) */
+      t0 = a1 * u2 - a2 * u1; t1 = a2 * u0 - a0 * u2; t2 = a0 * u1 - a1 * u0;
+      tm2 = t0 * t0 + t1 * t1 + t2 * t2;
+    }
+  } while (tm2 < 0.001);
+  u0 *= mu; u1 *= mu; u2 *= mu;
+  double f = sqrt((1 - mu * mu) / tm2);
+  t0 *= f; t1 *= f; t2 *= f;
+  out[0] = u0 + t0; out[1] = u1 + t1; out[2] = u2 + t2;
+}
+
+/* ------------------------------------------------------------------ */
+/* batched drivers (the loops of ncrystal_core/src/cinterface/ncrystal.cc:1107-1246) */
+typedef struct {
+  const orc_proc* p; const double* ekin; const double* dirs; int64_t n, repeat;
+  double *o1, *o2; uint64_t seed, idxoff; uint32_t stream; int64_t lo, hi; int64_t nerr; int mode;
+} job_t;
+static void* job_run(void* arg) {
+  job_t* j = (job_t*)arg;
+  for (int64_t k = j->lo; k < j->hi; ++k) {
+    if (j->mode == 0) {
+      j->o1[k] = proc_xs(j->p, j->ekin[k]);
+      continue;
+    }
+    int64_t i = k % j->n;
+    rng_t rng; rng_init(&rng, j->seed, j->stream, j->idxoff + (uint64_t)k);
+    if (j->mode == 1) {
+      double Ef, mu;
+      if (proc_sample_iso(j->p, &rng, j->ekin[i], &Ef, &mu)) { Ef = -1.0; mu = -999.0; j->nerr++; }
+      j->o1[k] = Ef; j->o2[k] = mu;
+    } else if (j->mode == 2) {
+      double a, b;
+      if (sab_sample_ab(&j->p->u.sab, j->ekin[i], &rng, &a, &b)) { a = -1.0; b = 0.0; j->nerr++; }
+      j->o1[k] = a; j->o2[k] = b;
+    } else {
+      double Ef, mu, od[3];
+      if (proc_sample_iso(j->p, &rng, j->ekin[i], &Ef, &mu)) {
+        Ef = -1.0; od[0] = od[1] = od[2] = 0.0; j->nerr++;
+      } else {
+        rand_dir_given_mu(&rng, mu, j->dirs + 3 * i, od);
+      }
+      j->o1[k] = Ef; j->o2[3 * k] = od[0]; j->o2[3 * k + 1] = od[1]; j->o2[3 * k + 2] = od[2];
+    }
+  }
+  return NULL;
+}
+static int64_t run_jobs(job_t proto, int64_t total, int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  if (total < nthreads) nthreads = total > 0 ? (int)total : 1;
+  job_t jobs[256];
+  pthread_t th[256];
+  for (int t = 0; t < nthreads; ++t) {
+    jobs[t] = proto;
+    jobs[t].lo = total * t / nthreads;
+    jobs[t].hi = total * (t + 1) / nthreads;
+    jobs[t].nerr = 0;
+  }
+  if (nthreads == 1) { job_run(&jobs[0]); return jobs[0].nerr; }
+  for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, job_run, &jobs[t]);
+  int64_t nerr = 0;
+  for (int t = 0; t < nthreads; ++t) { pthread_join(th[t], NULL); nerr += jobs[t].nerr; }
+  return nerr;
+}
+void orc_xs_many(const orc_proc* p, const double* ekin, int64_t n, double* out, int nthreads) {
+  job_t j; memset(&j, 0, sizeof(j));
+  j.p = p; j.ekin = ekin; j.n = n; j.repeat = 1; j.o1 = out; j.mode = 0;
+  run_jobs(j, n, nthreads);
+}
+int64_t orc_sample_iso_many(const orc_proc* p, uint64_t seed, uint32_t stream, uint64_t idxoff,
+                            const double* ekin, int64_t n, int64_t repeat, double* oe,
+                            double* om, int nthreads) {
+  job_t j; memset(&j, 0, sizeof(j));
+  j.p = p; j.ekin = ekin; j.n = n; j.repeat = repeat; j.o1 = oe; j.o2 = om; j.seed = seed;
+  j.stream = stream; j.idxoff = idxoff; j.mode = 1;
+  return run_jobs(j, n * repeat, nthreads);
+}
+int64_t orc_sab_sample_ab_many(const orc_proc* p, uint64_t seed, uint32_t stream, const double* ekin,
+                               int64_t n, double* oa, double* ob, int nthreads) {
+  job_t j; memset(&j, 0, sizeof(j));
+  j.p = p; j.ekin = ekin; j.n = n; j.repeat = 1; j.o1 = oa; j.o2 = ob; j.seed = seed;
+  j.stream = stream; j.mode = 2;
+  return run_jobs(j, n, nthreads);
+}
+int64_t orc_sample_dir_many(const orc_proc* p, uint64_t seed, uint32_t stream, uint64_t idxoff,
+                            const double* ekin, const double* dirs, int64_t n, double* oe,
+                            double* od, int nthreads) {
+  job_t j; memset(&j, 0, sizeof(j));
+  j.p = p; j.ekin = ekin; j.dirs = dirs; j.n = n; j.repeat = 1; j.o1 = oe; j.o2 = od;
+  j.seed = seed; j.stream = stream; j.idxoff = idxoff; j.mode = 3;
+  return run_jobs(j, n, nthreads);
+}
+
+/* ================================================================== */
+/* Free-gas sampling.                                                  */
+/* erfcdiff / erfc_rescaled: ncrystal_core/src/utils/NCMath.cc:338-417 */
+static double erfcdiff_notaylor(double a, double b) {
+  if (b < 0) { double t = -b; b = -a; a = t; }
+  const double erfca = a > 27.3 ? 0.0 : erfc(a);
+  if (b > a + 4.0 && (a >= 4 || (a < 0.0 && b > 6.0))) return erfca;
+  const double erfcb = b > 27.3 ? 0.0 : erfc(b);
+  return erfca - erfcb;
+}
+static double erfcdiff(double a, double b) {
+  if (dmax(fabs(a), fabs(b)) < 0.32) {
+    const double c1 = -2.0 * K_INVSQRTPI, c3 = 2.0 * K_INVSQRTPI / 3.0, c5 = -0.2 * K_INVSQRTPI,
+                 c7 = K_INVSQRTPI / 21.0, c9 = -K_INVSQRTPI / 108.0, c11 = K_INVSQRTPI / 660.0,
+                 c13 = -K_INVSQRTPI / 4680.0, c15 = K_INVSQRTPI / 37800.0;
+    const double a2 = a * a, b2 = b * b;
+    const double a3 = a * a2 * (c3 + a2 * (c5 + a2 * (c7 + (a2 * (c9 + a2 * (c11 + a2 * (c13 + a2 * c15)))))));
+    const double b3 = b * b2 * (c3 + b2 * (c5 + b2 * (c7 + (b2 * (c9 + b2 * (c11 + b2 * (c13 + b2 * c15)))))));
+    return c1 * (a - b) + (a3 - b3);
+  }
+  return a > b ? -erfcdiff_notaylor(b, a) : erfcdiff_notaylor(a, b);
+}
+static double erfc_rescaled(double x, double b) {
+  if (b < -745.1) return 0.0;
+  if ((x < 23.0 && fabs(b) < 700) || x < 5) return exp(b) * erfc(x);
+  const double bxx = b - x * x;
+  if (bxx < -745.1) return 0.0;
+  const double c3 = -0.5, c5 = 0.75, c7 = -1.875, c9 = 6.5625, c11 = -29.53125;
+  const double y = 1 / x, y2 = y * y;
+  return K_INVSQRTPI * exp(bxx) * (y + y2 * (c3 + y2 * (c5 + y2 * (c7 + y2 * (c9 + y2 * c11)))));
+}
+
+/* erfc bounds lookup table: NCFreeGasUtils.cc:97-145 (linspace NCMath.cc:68-80) */
+#define EB_LEN 1103
+static double eb_cache[EB_LEN];
+static pthread_once_t eb_once = PTHREAD_ONCE_INIT;
+static void eb_init(void) {
+  const int nedges = EB_LEN - 2;
+  eb_cache[0] = 2.0;
+  const double start = -2.0, stop = 9.0;
+  const unsigned nm1 = (unsigned)nedges - 1;
+  const double interval = (stop - start) / nm1;
+  for (unsigned i = 0; i < nm1; ++i) eb_cache[1 + i] = erfc(start + i * interval);
+  eb_cache[1 + nm1] = erfc(stop);
+  eb_cache[EB_LEN - 1] = 0.0;
+}
+static void erfc_bounds(double x, double* lb, double* ub) {
+  const double bw = (9.0 - (-2.0)) / (EB_LEN - 3);
+  const double ibw = (EB_LEN - 3) / (9.0 - (-2.0));
+  const int nb_inf = EB_LEN - 1;
+  const double vlow = -2.0 - 0.5 * bw, vhigh = 9.0 + 0.5 * bw;
+  const double xs = dclamp(x, vlow, vhigh);
+  const double xr = (xs - (-2.0));
+  int bi = (int)(1.0 + xr * ibw);
+  if (bi > nb_inf) bi = nb_inf;
+  if (bi < 0) bi = 0;
+  *lb = eb_cache[bi + 1] * 0.99999999;
+  *ub = eb_cache[bi] * 1.00000001;
+}
+
+/* FGEvalBetaDistHelper: NCFreeGasUtils.cc:151-232 */
+typedef struct { double beta, normfact, expmbeta, k11, k12, k21, k22; } fgeval_t;
+static void fgeval_init(fgeval_t* h, double c, double invA, double sqrtAc, double beta, double normfact) {
+  h->beta = beta; h->normfact = normfact; h->expmbeta = -1.0;
+  const double eps = beta / c;
+  const double sq1pe = sqrt(1 + eps);
+  const double S = (beta < 0.0 ? -1.0 : 1.0);
+  const double sqrtepsprime = (eps >= 0.0 ? 1.0 : sq1pe);
+  const double sqrtgp = sqrt(2.0 + eps + 2.0 * sq1pe);
+  const double SP = 0.5 * (S + invA), SM = 0.5 * (S - invA);
+  const double ia_se = invA * sqrtepsprime, mS_se = -S * sqrtepsprime;
+  const double SPsgp = sqrtgp * SP, SMsgp = sqrtgp * SM;
+  h->k11 = sqrtAc * (-ia_se + SPsgp);
+  h->k12 = sqrtAc * (mS_se + SPsgp);
+  h->k21 = sqrtAc * (mS_se + SMsgp);
+  h->k22 = sqrtAc * (ia_se + SMsgp);
+}
+static void fgeval_expmbeta(fgeval_t* h) {
+  if (h->expmbeta < 0) h->expmbeta = h->beta < -700.0 ? 0.0 : exp(-h->beta);
+}
+static double fgeval_exact(fgeval_t* h) {
+  double t1 = erfcdiff(h->k11, h->k12);
+  fgeval_expmbeta(h);
+  if (!h->expmbeta) return h->normfact * t1;
+  double t2 = erfcdiff(h->k21, h->k22);
+  return h->normfact * (t1 + t2 * h->expmbeta);
+}
+static void fgeval_bounds(fgeval_t* h, double* lb, double* ub) {
+  double l11, u11, l12, u12, l21, u21, l22, u22;
+  erfc_bounds(h->k11, &l11, &u11); erfc_bounds(h->k12, &l12, &u12);
+  double t1l = l11 - u12, t1u = u11 - l12;
+  erfc_bounds(h->k21, &l21, &u21); erfc_bounds(h->k22, &l22, &u22);
+  double t2l = l21 - u22, t2u = u21 - l22;
+  if (t2u > 0.0) {
+    fgeval_expmbeta(h);
+    *lb = h->normfact * (t1l + t2l * h->expmbeta);
+    *ub = h->normfact * (t1u + t2u * h->expmbeta);
+    return;
+  }
+  *lb = h->normfact * t1l; *ub = h->normfact * t1u;
+}
+
+/* randExpDivSqrt: ncrystal_core/src/utils/NCRandUtils.cc:224-380 */
+static double rand_exp_div_sqrt(rng_t* rng, double c, double a, double b) {
+  const double A = c * a;
+  if (A > 0.1) {
+    const double U = c * (b - a);
+    const double invA = 1.0 / A;
+    const double m_c1 = -1.0 / 1.0, m_c2 = expm1(-1.0 * (U - 0));
+    while (1) {
+      double ug = 0 + m_c1 * log(1.0 + rng_gen(rng) * m_c2);
+      double R = rng_gen(rng);
+      if ((1.0 + ug * invA) * R * R < 1.0) return dclamp((ug + A) / c, a, b);
+    }
+  }
+  const double Ulim = 16.1180956509583;
+  const double U = dmin(c * (b - a), Ulim);
+  const double B = U + A;
+  if (!(B > A)) return a;
+  const double sA = sqrt(A), sB = sqrt(B), d = sB - sA, tsA = 2 * sA;
+  double ug;
+  while (1) {
+    const double T = rng_gen(rng) * d;
+    ug = T * (T + tsA);
+    const double Ra = rng_gen(rng);
+    if (ug < 2.0) {
+      const double c1 = -1.0, c2 = 1.0 / 2.0, c3 = -1.0 / 6.0, c4 = 1.0 / 24.0, c5 = -1.0 / 120.0, c6 = 1.0 / 720.0;
+      const double t6 = 1.0 + ug * (c1 + ug * (c2 + ug * (c3 + ug * (c4 + ug * (c5 + ug * c6)))));
+      if (Ra > t6) continue;
+      if (Ra + 0.020221 < t6) break;
+    } else {
+      if (Ra > 0.135335283236614) continue;
+      if (ug > 4.0 && Ra > 0.0183156388887343) continue;
+    }
+    if (Ra < exp(-ug)) break;
+  }
+  return dclamp((ug + A) / c, a, b);
+}
+
+/* randExpMInvXMCXDivSqrtX: NCFreeGasUtils.cc:240-490 */
+static double fe_eval(double x, double xmax, double c) {
+  const double ea = (x - xmax) / (x * xmax) - c * (x - xmax);
+  if (ea >= 706.0) return 1.0;
+  return ea < -745.1 ? 0.0 : exp(ea) * sqrt(xmax / x);
+}
+static double rand_expminvx(rng_t* rng, double c, double xm, double xp) {
+  if (xp == xm) return xm;
+  const double sqrtc = sqrt(c);
+  const double invsqrtc = 1 / sqrtc;
+  const double xpeak = (c > 1e-5 ? (c > 1e200 ? invsqrtc : (sqrt(16.0 * c + 1.0) - 1.0) / (4.0 * c))
+                                 : (2.0 - c * (8.0 - c * (64.0 - c * (640.0 - c * 7168.0)))));
+  if (xpeak == 0.0) return xm > 0.0 ? xm : dmin(DBL_MIN, xp);
+  const double xmax = (xm > xpeak ? xm : dmin(xp, xpeak));
+  if (!(xmax > 0.0)) return xm;
+  double xlarge = dmax(5.0 / sqrt(c), 2 * xpeak);
+  double xsmall = dmin(0.2 / sqrt(c), 0.5 * xpeak);
+  if (xp > xlarge) xp = dmin(xp, dmax(xm, xlarge) + 15.0 / c);
+  if (xm < xsmall) {
+    double xsm = dmin(xp, xsmall);
+    xm = dmax(xm, xsm / (1 + 30.0 * xsm));
+  }
+  if ((xm = dmax(DBL_MIN, dmax(DBL_MIN / xp, xm))) >= xp) return xp;
+  const double fcut = 1e-9;
+  if (xp < xpeak) {
+    while (1) {
+      double xm_new = xp - 0.01 * (xp - xm);
+      double fv = fe_eval(xm_new, xmax, c);
+      if (fv >= fcut) break;
+      xm = xm_new;
+    }
+  }
+  double pflat = -1.0, xswitch = -1.0, area_right = -1.0;
+  if (xm >= xlarge) { pflat = 0.0; xswitch = xm; }
+  else if (c > 25 || xp <= xlarge) { pflat = 1.0; xswitch = xp; }
+  else {
+    xswitch = xlarge;
+    const double area_left = (xswitch - xm);
+    const double B = c * xmax + 1 / xmax - 1 / xp;
+    area_right = (erfc_rescaled(sqrtc * sqrt(xswitch), B) - erfc_rescaled(sqrtc * sqrt(xp), B)) * sqrt(K_PI * (xmax / c));
+    pflat = area_left / (area_left + area_right);
+  }
+  int single_side, always_left;
+#define EVALP(pl) do { always_left = ((pl) > 1.0 - fcut); int ar_ = ((pl) < fcut); single_side = (always_left || ar_); } while (0)
+  EVALP(pflat);
+  if (!single_side && fe_eval(xswitch, xmax, c) < fcut * 1.1) {
+    pflat = 1.0; area_right = 0.0; xp = xswitch; always_left = 1; single_side = 0;
+  }
+  while (1) {
+    int do_flat = single_side ? always_left : (rng_gen(rng) < pflat);
+    if (do_flat) {
+      double dx = xswitch - xm;
+      const double xthr_up = xswitch, xthr_low = xm;
+      const double fth = 0.05;
+      double xgen = xm + rng_gen(rng) * dx;
+      double Ra = rng_gen(rng);
+      if (!in_interval(xthr_low, xthr_up, xgen) && Ra > fth) continue;
+      const double fv = fe_eval(xgen, xmax, c);
+      if (fv < fth) {
+        if (fv < fcut) {
+          if (xgen < xmax) xm = xgen;
+          else xswitch = xgen;
+          dx = xswitch - xm;
+          if (!single_side) {
+            const double area_left = dx;
+            pflat = area_left / (area_left + area_right);
+            EVALP(pflat);
+          }
+          continue;
+        }
+      }
+      if (Ra <= fv) return xgen;
+    } else {
+      double xgen = rand_exp_div_sqrt(rng, c, xswitch, xp);
+      if (rng_gen(rng) < exp((xgen - xp) / (xgen * xp))) return xgen;
+    }
+  }
+#undef EVALP
+}
+
+typedef struct { double c, kT, sqrtAc, invA, Adiv4, normfact, c_real; } fgs_t;
+static void fgs_init(fgs_t* f, double E, double T, double mass) {
+  const double kT = K_BOLTZMANN * T;
+  f->c = dmin(1e14, dmax(1e-10, E / kT));
+  f->kT = kT;
+  f->sqrtAc = sqrt(mass * f->c / K_NEUTRON_MASS_AMU);
+  const double A = (1.0 / K_NEUTRON_MASS_AMU) * mass;
+  f->invA = 1.0 / A;
+  f->Adiv4 = 0.25 * A;
+  f->normfact = 0.5 / erf(sqrt(f->c * f->invA));
+  f->c_real = E / kT;
+}
+/* FreeGasSampler::sampleBeta: NCFreeGasUtils.cc:530-848 */
+static double fgs_sample_beta(const fgs_t* f, rng_t* rng) {
+  if (f->c_real > 1e4) {
+    const double A = 1.0 / f->invA, A2 = A * A;
+    const double thr = 1e4 * dmin(1000.0 * A, A2 * A2 * A2);
+    if (f->c_real > thr) {
+      double q = (1.0 - f->invA) / (1.0 + f->invA);
+      double elossmax = f->c_real * (1.0 - q * q);
+      return -elossmax * rng_gen(rng);
+    }
+  }
+  const double Tlim = 2.0, Tlim_k1 = 0.135335283236612691893999494972484403407, Tlim_k2 = 274. / 315.;
+  const double fcut = 1e-6;
+  double aa = dmax(-f->c_real, -f->c), bb = 13.815510557964274;
+  if (f->invA <= 1.0 / 10.0) {
+    if (f->c > 10.1) {
+      while (1) {
+        double an = aa * 0.2;
+        if (an > -1e-99) break;
+        fgeval_t h; fgeval_init(&h, f->c, f->invA, f->sqrtAc, an, f->normfact);
+        if (fgeval_exact(&h) > fcut) break;
+        aa = an;
+      }
+    }
+    while (1) {
+      double bn = bb * 0.25;
+      if (bn < 1e-99) break;
+      fgeval_t h; fgeval_init(&h, f->c, f->invA, f->sqrtAc, bn, f->normfact);
+      double lb, ub; fgeval_bounds(&h, &lb, &ub);
+      if (ub > fcut) break;
+      bb = bn;
+    }
+  }
+  if (!(bb > aa)) return aa;
+  double a, b, pdown, pnotclose;
+  double es_a = 0, es_c1 = 0, es_c2 = 0; /* RandExpIntervalSampler */
+#define SETAB(AAA, BBB) do { \
+    a = (AAA); b = (BBB); double adown = -a, afar, aclose; \
+    if (b <= Tlim) { afar = 0.0; \
+      aclose = b * (1.0 + b * (-1. / 2. + b * (1. / 6. + b * (-1. / 24. + b * (1. / 120. + b * (-1. / 720. + b * (1. / 5040.))))))); } \
+    else { afar = Tlim_k1 - exp(-b); aclose = Tlim_k2; } \
+    double iat = 1.0 / (adown + aclose + afar); \
+    pdown = adown * iat; pnotclose = (afar + adown) * iat; es_a = es_c1 = es_c2 = 0.0; } while (0)
+  SETAB(aa, bb);
+  const double fth = 0.1;
+  double afth = a, bfth = b;
+  while (1) {
+    double beta, fov;
+    double Rs = rng_gen(rng);
+    if (Rs < pdown) {
+      beta = rng_gen(rng) * a;
+      fov = 1.0;
+    } else if (Rs < pnotclose) {
+      if (!(es_c1 < 0.0)) { es_a = Tlim; es_c1 = -1.0 / 1.0; es_c2 = expm1(-1.0 * (b - Tlim)); }
+      beta = es_a + es_c1 * log(1.0 + rng_gen(rng) * es_c2);
+      fov = exp(-beta);
+    } else {
+      double bmax = dmin(b, Tlim);
+      while (1) {
+        beta = rng_gen(rng) * bmax;
+        double R0 = rng_gen(rng);
+        const double kcheap = 19. / 45.;
+        if (R0 > 1.0 - kcheap * beta) continue;
+        fov = 1.0 + beta * (-1. + beta * (1. / 2. + beta * (-1. / 6. + beta * (1. / 24. + beta * (-1. / 120. + beta * (1. / 720.))))));
+        if (R0 < fov) break;
+      }
+    }
+    const double facc = rng_gen(rng) * fov;
+    if (facc > fth && !in_interval(afth, bfth, beta)) continue;
+    fgeval_t h; fgeval_init(&h, f->c, f->invA, f->sqrtAc, beta, f->normfact);
+    int need_exact = 1;
+    double fval = 0.0;
+    if (beta > 0) {
+      double lb, ub; fgeval_bounds(&h, &lb, &ub);
+      if (facc <= lb) return beta;
+      fval = ub;
+      if (facc > ub) need_exact = 0;
+    }
+    if (need_exact) {
+      fval = fgeval_exact(&h);
+      if (facc < fval) return beta;
+    }
+    if (fval < fcut) {
+      if (beta < 0) SETAB(beta, b); else SETAB(a, beta);
+      continue;
+    }
+    if (fval < fth) {
+      if (beta < 0) afth = dmax(afth, beta);
+      else bfth = dmin(bfth, beta);
+    }
+  }
+#undef SETAB
+}
+/* FreeGasSampler::sampleAlpha: NCFreeGasUtils.cc:851-930 */
+static double fgs_sample_alpha(const fgs_t* f, double beta, rng_t* rng) {
+  if (f->c_real < f->c || mu_iso_at_beta(beta, f->c)) {
+    double l1, l2; alpha_limits(f->c_real, beta, &l1, &l2);
+    double al = l1 + rng_gen(rng) * (l2 - l1);
+    return dclamp(al, l1, l2);
+  }
+  beta = dmax(-f->c, beta);
+  double am, ap; alpha_limits(f->c, beta, &am, &ap);
+  if (am == ap) return am;
+  const double bsq = beta * beta;
+  const double t = bsq * f->Adiv4;
+  const double c = 0.0625 * bsq;
+  if (dmin(t, c) < 1e-5) {
+    const double fourA = f->Adiv4 * 16.0;
+    const double inv4A = 1.0 / fourA;
+    const double xxm = am * inv4A, xxp = ap * inv4A;
+    while (1) {
+      const double xx = rand_exp_div_sqrt(rng, 1.0, xxm, xxp);
+      double al = xx * fourA;
+      if (al < am || al > ap) continue;
+      if (al * ap * (-log(rng_gen(rng))) >= t * (ap - al)) return al;
+    }
+  }
+  double invt = 1.0 / t;
+  double x = rand_expminvx(rng, c, am * invt, ap * invt);
+  return dclamp(x * t, am, ap);
+}
+/* FreeGasSampler::sampleAlphaBeta: NCFreeGasUtils.hh:132-146 */
+static int orc_fg_sample_alpha_beta(double E, double T, double mass, rng_t* rng, double* alpha, double* beta) {
+  pthread_once(&eb_once, eb_init);
+  fgs_t f; fgs_init(&f, E, T, mass);
+  double b = fgs_sample_beta(&f, rng);
+  if (b < -f.c || mu_iso_at_beta(b, f.c)) {
+    double l1, l2; alpha_limits(f.c_real, b, &l1, &l2);
+    double al = l1 + rng_gen(rng) * (l2 - l1);
+    *alpha = dclamp(al, l1, l2); *beta = b;
+    return 0;
+  }
+  *alpha = fgs_sample_alpha(&f, b, rng);
+  *beta = b;
+  return 0;
+}

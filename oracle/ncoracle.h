@@ -1,0 +1,99 @@
+/*
+ * ncoracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Plain-C restatement of the per-neutron physics of NCrystal v4.4.2
+ * (PowderBragg, Debye-model incoherent elastic, S(alpha,beta) inelastic with
+ * its free-gas high-energy extender, hkl enumeration) used as the CPU checker
+ * for the HIP implementation in ncrystal_amd/.  Only tests/, the smoke() entry
+ * and the CPU-baseline leg of bench.py may load this library.  The product
+ * never links or calls it.
+ *
+ * Every function cites the reference file:line whose algorithm it restates
+ * (paths relative to the reference repository root).
+ *
+ * Parity status of each function (how it is pinned) is listed in the header
+ * comment of ncoracle.c.
+ */
+#ifndef NCORACLE_H
+#define NCORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- counter-based RNG (Philox4x32-10, Salmon et al. SC'11) ---- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double orc_u64_to_fp01(uint64_t x);
+void orc_rng_fill(uint64_t seed, uint32_t stream, uint64_t idx, int n, double* out);
+
+/* ---- numerics helpers exposed for known-answer tests ---- */
+double orc_integrate_alpha_interval_fast(double a1, double s1, double a2, double s2,
+                                         double logs1, double logs2);
+void orc_sincos_2pix(double x, double* s, double* c);
+double orc_debye_msd(double debye_temp, double temp, double mass_amu);
+void orc_alpha_limits(double ekin_div_kT, double beta, double* alow, double* aupp);
+double orc_freegas_xs_shape(double a_squared);
+
+/* ---- hkl enumeration for cells with all angles 90 degrees, no space group ---- */
+/* positions: 3*sum(natoms) fractional coords, species-major. csl in sqrt(barn)
+   (10 fm).  Returns number of families written (sorted like the reference's
+   InfoBuilder), -1 if maxout too small, -2 if the 1e6-family guard fires. */
+int orc_hkl_orthogonal(double a, double b, double c,
+                       int nspecies, const int* natoms, const double* positions,
+                       const double* csl, const double* msd,
+                       double dcutoff, double fsquarecut, double merge_tolerance,
+                       int maxout, int* out_hkl, int* out_mult,
+                       double* out_dsp, double* out_fsq);
+
+/* ---- processes ---- */
+typedef struct orc_proc orc_proc;
+orc_proc* orc_create_powderbragg(double v0_times_natoms, const double* dsp,
+                                 const double* fsq_x_mult, int n);
+int orc_pb_ntables(const orc_proc*);
+void orc_pb_tables(const orc_proc*, double* e2d, double* fdm_commul);
+
+orc_proc* orc_create_elinc(int n, const double* msd, const double* bixs,
+                           const double* scale);
+
+/* S(alpha,beta) table: sab[ibeta*nalpha+ialpha].  egrid: NULL/0 => automatic,
+   3 values => {emin,emax,npts} (0 = auto), >3 => explicit grid. */
+orc_proc* orc_create_sab(int nalpha, int nbeta, const double* alpha,
+                         const double* beta, const double* sab, double temperature,
+                         double mass_amu, double sigma_bound, double suggested_emax,
+                         int negrid_in, const double* egrid_in);
+int orc_sab_negrid(const orc_proc*);
+void orc_sab_egrid(const orc_proc*, double* egrid, double* xs);
+/* per egrid point sampler summary: ibeta_low, nvals, kind, first-bin value */
+void orc_sab_sampler_info(const orc_proc*, int iegrid, int* ibeta_low, int* nvals,
+                          int* kind, double* firstbin);
+void orc_sab_sampler_arrays(const orc_proc*, int iegrid, double* bx, double* by,
+                            double* bcdf, double* info_dbl /*8 per beta*/,
+                            int* info_idx /*2 per beta*/);
+void orc_sab_derived(const orc_proc*, double* logsab, double* cumul);
+double orc_sab_analyse_xs(const orc_proc*, double ekin);
+
+orc_proc* orc_create_composition(int n, orc_proc* const* comps, const double* scales);
+void orc_free(orc_proc*);
+
+/* batched evaluation (nthreads<=1: single thread) */
+void orc_xs_many(const orc_proc*, const double* ekin, int64_t n, double* out, int nthreads);
+/* out index = irepeat*n + i; RNG stream = (seed, stream, idx_offset + out index)
+   returns number of neutrons whose sampling raised an error (poisoned outputs) */
+int64_t orc_sample_iso_many(const orc_proc*, uint64_t seed, uint32_t stream,
+                            uint64_t idx_offset, const double* ekin, int64_t n,
+                            int64_t repeat, double* out_ekin, double* out_mu,
+                            int nthreads);
+/* sample (alpha,beta) from the SAB sampler only (no conversion) */
+int64_t orc_sab_sample_ab_many(const orc_proc*, uint64_t seed, uint32_t stream,
+                               const double* ekin, int64_t n, double* out_alpha,
+                               double* out_beta, int nthreads);
+/* oriented: per-neutron directions (isotropic materials: mu + azimuth) */
+int64_t orc_sample_dir_many(const orc_proc*, uint64_t seed, uint32_t stream,
+                            uint64_t idx_offset, const double* ekin,
+                            const double* dirs, int64_t n, double* out_ekin,
+                            double* out_dirs, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,248 @@
+"""ctypes front end of the CPU checker (oracle/ncoracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py``.  The product package never imports it.
+"""
+import ctypes
+import os
+import pathlib
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+_LIBPATH = _HERE / "build" / "libncoracle.so"
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not _LIBPATH.exists():
+            build()
+        L = ctypes.CDLL(str(_LIBPATH))
+        L.orc_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint32)] * 3
+        L.orc_u64_to_fp01.argtypes = [ctypes.c_uint64]
+        L.orc_u64_to_fp01.restype = ctypes.c_double
+        L.orc_rng_fill.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, _dp]
+        L.orc_integrate_alpha_interval_fast.argtypes = [ctypes.c_double] * 6
+        L.orc_integrate_alpha_interval_fast.restype = ctypes.c_double
+        L.orc_sincos_2pix.argtypes = [ctypes.c_double, _dp, _dp]
+        L.orc_debye_msd.argtypes = [ctypes.c_double] * 3
+        L.orc_debye_msd.restype = ctypes.c_double
+        L.orc_alpha_limits.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.orc_freegas_xs_shape.argtypes = [ctypes.c_double]
+        L.orc_freegas_xs_shape.restype = ctypes.c_double
+        L.orc_hkl_orthogonal.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _ip, _dp, _dp, _dp,
+                                                                  ctypes.c_double, ctypes.c_double,
+                                                                  ctypes.c_double, ctypes.c_int, _ip, _ip,
+                                                                  _dp, _dp]
+        L.orc_hkl_orthogonal.restype = ctypes.c_int
+        L.orc_create_powderbragg.argtypes = [ctypes.c_double, _dp, _dp, ctypes.c_int]
+        L.orc_create_powderbragg.restype = _vp
+        L.orc_pb_ntables.argtypes = [_vp]
+        L.orc_pb_tables.argtypes = [_vp, _dp, _dp]
+        L.orc_create_elinc.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_create_elinc.restype = _vp
+        L.orc_create_sab.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_create_sab.restype = _vp
+        L.orc_sab_negrid.argtypes = [_vp]
+        L.orc_sab_egrid.argtypes = [_vp, _dp, _dp]
+        L.orc_sab_sampler_info.argtypes = [_vp, ctypes.c_int, _ip, _ip, _ip, _dp]
+        L.orc_sab_sampler_arrays.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]
+        L.orc_sab_derived.argtypes = [_vp, _dp, _dp]
+        L.orc_sab_analyse_xs.argtypes = [_vp, ctypes.c_double]
+        L.orc_sab_analyse_xs.restype = ctypes.c_double
+        L.orc_create_composition.argtypes = [ctypes.c_int, ctypes.POINTER(_vp), _dp]
+        L.orc_create_composition.restype = _vp
+        L.orc_free.argtypes = [_vp]
+        L.orc_xs_many.argtypes = [_vp, _dp, ctypes.c_int64, _dp, ctypes.c_int]
+        L.orc_sample_iso_many.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _dp,
+                                          ctypes.c_int64, ctypes.c_int64, _dp, _dp, ctypes.c_int]
+        L.orc_sample_iso_many.restype = ctypes.c_int64
+        L.orc_sab_sample_ab_many.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint32, _dp, ctypes.c_int64,
+                                             _dp, _dp, ctypes.c_int]
+        L.orc_sab_sample_ab_many.restype = ctypes.c_int64
+        L.orc_sample_dir_many.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _dp, _dp,
+                                          ctypes.c_int64, _dp, _dp, ctypes.c_int]
+        L.orc_sample_dir_many.restype = ctypes.c_int64
+        _lib = L
+    return _lib
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def rng_fill(seed, stream, idx, n):
+    out = np.empty(n)
+    lib().orc_rng_fill(seed, stream, idx, n, _d(out))
+    return out
+
+
+def debye_msd(debye_temp, temp, mass):
+    return lib().orc_debye_msd(debye_temp, temp, mass)
+
+
+def alpha_limits(ediv, beta):
+    a = ctypes.c_double()
+    b = ctypes.c_double()
+    lib().orc_alpha_limits(ediv, beta, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def hkl_orthogonal(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tolerance=1e-6, maxout=2000000):
+    """species: list of (csl_sqrtbarn, msd, positions[n,3])."""
+    natoms = np.array([len(s[2]) for s in species], dtype=np.int32)
+    pos = np.ascontiguousarray(np.concatenate([np.asarray(s[2], dtype=np.float64).reshape(-1, 3)
+                                               for s in species]).ravel())
+    csl = np.array([s[0] for s in species], dtype=np.float64)
+    msd = np.array([s[1] for s in species], dtype=np.float64)
+    hkl = np.empty(3 * maxout, dtype=np.int32)
+    mult = np.empty(maxout, dtype=np.int32)
+    dsp = np.empty(maxout)
+    fsq = np.empty(maxout)
+    n = lib().orc_hkl_orthogonal(a, b, c, len(species), _i(natoms), _d(pos), _d(csl), _d(msd), dcutoff,
+                                 fsquarecut, merge_tolerance, maxout, _i(hkl), _i(mult), _d(dsp), _d(fsq))
+    if n < 0:
+        raise RuntimeError(f"oracle hkl enumeration failed ({n})")
+    return hkl[:3 * n].reshape(n, 3).copy(), mult[:n].copy(), dsp[:n].copy(), fsq[:n].copy()
+
+
+class Proc:
+    def __init__(self, ptr, keep=()):
+        if not ptr:
+            raise RuntimeError("oracle process creation failed")
+        self.ptr = ptr
+        self._keep = keep
+
+    def __del__(self):
+        try:
+            if self.ptr and not getattr(self, "_borrowed", False):
+                lib().orc_free(self.ptr)
+        except Exception:
+            pass
+
+    def xs(self, ekin, nthreads=1):
+        e = np.ascontiguousarray(np.atleast_1d(np.asarray(ekin, dtype=np.float64)))
+        out = np.empty_like(e)
+        lib().orc_xs_many(self.ptr, _d(e), e.size, _d(out), nthreads)
+        return out
+
+    def sample_iso(self, ekin, repeat=1, seed=1, stream=0, idx_offset=0, nthreads=1):
+        e = np.ascontiguousarray(np.atleast_1d(np.asarray(ekin, dtype=np.float64)))
+        oe = np.empty(e.size * repeat)
+        om = np.empty(e.size * repeat)
+        nerr = lib().orc_sample_iso_many(self.ptr, seed, stream, idx_offset, _d(e), e.size, repeat,
+                                         _d(oe), _d(om), nthreads)
+        return oe, om, nerr
+
+    def sample_dir(self, ekin, dirs, seed=1, stream=0, idx_offset=0, nthreads=1):
+        e = np.ascontiguousarray(np.atleast_1d(np.asarray(ekin, dtype=np.float64)))
+        d = np.ascontiguousarray(np.asarray(dirs, dtype=np.float64).reshape(-1, 3))
+        oe = np.empty(e.size)
+        od = np.empty((e.size, 3))
+        nerr = lib().orc_sample_dir_many(self.ptr, seed, stream, idx_offset, _d(e), _d(d), e.size,
+                                         _d(oe), _d(od), nthreads)
+        return oe, od, nerr
+
+
+def powderbragg(v0_times_natoms, dsp, fsq_x_mult):
+    d = np.ascontiguousarray(dsp, dtype=np.float64)
+    f = np.ascontiguousarray(fsq_x_mult, dtype=np.float64)
+    p = Proc(lib().orc_create_powderbragg(v0_times_natoms, _d(d), _d(f), d.size))
+    return p
+
+
+def pb_tables(p):
+    n = lib().orc_pb_ntables(p.ptr)
+    e2d = np.empty(n)
+    fdm = np.empty(n)
+    lib().orc_pb_tables(p.ptr, _d(e2d), _d(fdm))
+    return e2d, fdm
+
+
+def elinc(msd, bixs, scale):
+    m = np.ascontiguousarray(msd, dtype=np.float64)
+    b = np.ascontiguousarray(bixs, dtype=np.float64)
+    s = np.ascontiguousarray(scale, dtype=np.float64)
+    return Proc(lib().orc_create_elinc(m.size, _d(m), _d(b), _d(s)))
+
+
+class SAB(Proc):
+    def egrid(self):
+        n = lib().orc_sab_negrid(self.ptr)
+        eg = np.empty(n)
+        xs = np.empty(n)
+        lib().orc_sab_egrid(self.ptr, _d(eg), _d(xs))
+        return eg, xs
+
+    def analyse_xs(self, e):
+        return lib().orc_sab_analyse_xs(self.ptr, e)
+
+    def sampler(self, ie):
+        ibl, nv, kind, fb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        lib().orc_sab_sampler_info(self.ptr, ie, ctypes.byref(ibl), ctypes.byref(nv), ctypes.byref(kind),
+                                   ctypes.byref(fb))
+        res = dict(ibeta_low=ibl.value, nvals=nv.value, kind=kind.value, firstbin=fb.value)
+        if kind.value == 1:
+            n = nv.value
+            bx, by, bc = np.empty(n), np.empty(n), np.empty(n)
+            idbl = np.empty(8 * max(n - 1, 1))
+            iidx = np.empty(2 * max(n - 1, 1), dtype=np.int32)
+            lib().orc_sab_sampler_arrays(self.ptr, ie, _d(bx), _d(by), _d(bc), _d(idbl), _i(iidx))
+            res.update(bx=bx, by=by, bcdf=bc, info=idbl.reshape(-1, 8)[:n - 1], infoidx=iidx.reshape(-1, 2)[:n - 1])
+        return res
+
+    def derived(self, na, nb):
+        ls = np.empty(na * nb)
+        cu = np.empty(na * nb)
+        lib().orc_sab_derived(self.ptr, _d(ls), _d(cu))
+        return ls.reshape(nb, na), cu.reshape(nb, na)
+
+    def sample_ab(self, ekin, seed=1, stream=0, nthreads=1):
+        e = np.ascontiguousarray(np.atleast_1d(np.asarray(ekin, dtype=np.float64)))
+        oa = np.empty(e.size)
+        ob = np.empty(e.size)
+        nerr = lib().orc_sab_sample_ab_many(self.ptr, seed, stream, _d(e), e.size, _d(oa), _d(ob), nthreads)
+        return oa, ob, nerr
+
+
+def sab(alpha, beta, sab_table, temperature, mass_amu, sigma_bound, suggested_emax=0.0, egrid=None):
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    s = np.ascontiguousarray(sab_table, dtype=np.float64).reshape(b.size, a.size)
+    eg = np.ascontiguousarray(egrid if egrid is not None else np.zeros(0), dtype=np.float64)
+    ptr = lib().orc_create_sab(a.size, b.size, _d(a), _d(b), _d(s), temperature, mass_amu, sigma_bound,
+                               suggested_emax, eg.size, _d(eg) if eg.size else None)
+    return SAB(ptr)
+
+
+def composition(procs, scales):
+    arr = (_vp * len(procs))(*[p.ptr for p in procs])
+    sc = np.ascontiguousarray(scales, dtype=np.float64)
+    p = Proc(lib().orc_create_composition(len(procs), arr, _d(sc)), keep=tuple(procs))
+    return p
