@@ -62,50 +62,46 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
     import numpy as np
-    import torch
+    from ncrystal_amd import _capi as C
     from ncrystal_amd import core, synth
 
     table = synth.sab_table_c3(seed=1)
     sab = core.create_sabscatter(table["alpha"], table["beta"], table["sab"], table["temperature"],
                                  table["mass"], table["sigma_bound"])
     n = args.n
-    dev = torch.device("cuda:0")
-    g = torch.Generator(device=dev)
-    g.manual_seed(1)
-    u = torch.rand(n, dtype=torch.float64, device=dev, generator=g)
-    ekin = torch.exp(math.log(1e-3) + u * (math.log(1.0) - math.log(1e-3)))
-    del u
-    oe = torch.empty_like(ekin)
-    om = torch.empty_like(ekin)
-    stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
+    stream = C.Stream()
+    ekin = C.DeviceArray(n)
+    oe = C.DeviceArray(n)
+    om = C.DeviceArray(n)
+    # inputs resident in HBM before timing: E log-uniform in [1e-3, 1] eV
+    C.call("ncamd_dev_fill_loguniform", ekin.ptr, n, 1e-3, 1.0, 1, stream.ptr)
+    stream.sync()
     sab.setRandomSeed(1234)
 
     def step():
-        sab.dev_sampleScatterIsotropic(ekin.data_ptr(), n, oe.data_ptr(), om.data_ptr(), stream=sp)
+        sab.dev_sampleScatterIsotropic(ekin.ptr, n, oe.ptr, om.ptr, stream=stream.ptr)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    nerr = sab.dev_check_errors(stream=sp)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
+    stream.sync()
+    nerr = sab.dev_check_errors(stream=stream.ptr)
+    ev0, ev1 = C.Event(), C.Event()
+    stream.sync()
     t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(args.steps):
         step()
     ev1.record(stream)
-    torch.cuda.synchronize()
+    stream.sync()
     t1 = time.perf_counter()
-    nerr += sab.dev_check_errors(stream=sp)
+    nerr += sab.dev_check_errors(stream=stream.ptr)
     wall = t1 - t0
-    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    kernel_ms = ev0.elapsed_ms(ev1) / args.steps
     value = n * args.steps / wall
     achieved = BYTES_PER_SAMPLE * n / (kernel_ms * 1e-3) / 1e9
     # sanity of the outputs of the last step
-    mu = om[:1000000].cpu().numpy()
-    ef = oe[:1000000].cpu().numpy()
+    mu = om.numpy(1000000)
+    ef = oe.numpy(1000000)
     assert np.all(np.abs(mu) <= 1.0) and np.all(ef >= 0.0) and nerr == 0, "invalid samples"
 
     cpu = None

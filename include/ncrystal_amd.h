@@ -169,6 +169,22 @@ void ncamd_dev_sab_sample_alphabeta(ncamd_process_t, const double* d_ekin, unsig
 /* number of neutrons whose sampling failed since the last check (syncs stream) */
 int64_t ncamd_dev_check_errors(ncamd_process_t, void* stream);
 
+/* ---------- device memory, streams and events (plumbing for callers that
+ * do not bring their own HIP runtime objects; no counterpart in NCrystal) ---- */
+void* ncamd_dev_malloc(uint64_t bytes);
+void ncamd_dev_free(void* p);
+void ncamd_memcpy_htod(void* dst, const void* src, uint64_t bytes);
+void ncamd_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
+void* ncamd_stream_create(void);
+void ncamd_stream_destroy(void* stream);
+void ncamd_stream_sync(void* stream);
+void* ncamd_event_create(void);
+void ncamd_event_destroy(void* ev);
+void ncamd_event_record(void* ev, void* stream);
+float ncamd_event_elapsed_ms(void* ev_start, void* ev_stop);
+/* fill d_out[0..n) with E = exp(U(log emin, log emax)) from Philox(seed) */
+void ncamd_dev_fill_loguniform(double* d_out, uint64_t n, double emin, double emax, uint64_t seed, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

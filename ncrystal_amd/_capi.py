@@ -107,6 +107,19 @@ def lib():
         "ncamd_dev_samplescatter_many_dirs": (None, [Handle, _vp, _vp, _ul, _vp, _vp, _vp]),
         "ncamd_dev_sab_sample_alphabeta": (None, [Handle, _vp, _ul, _vp, _vp, _vp]),
         "ncamd_dev_check_errors": (ctypes.c_int64, [Handle, _vp]),
+        "ncamd_dev_malloc": (_vp, [ctypes.c_uint64]),
+        "ncamd_dev_free": (None, [_vp]),
+        "ncamd_memcpy_htod": (None, [_vp, _vp, ctypes.c_uint64]),
+        "ncamd_memcpy_dtoh": (None, [_vp, _vp, ctypes.c_uint64]),
+        "ncamd_stream_create": (_vp, []),
+        "ncamd_stream_destroy": (None, [_vp]),
+        "ncamd_stream_sync": (None, [_vp]),
+        "ncamd_event_create": (_vp, []),
+        "ncamd_event_destroy": (None, [_vp]),
+        "ncamd_event_record": (None, [_vp, _vp]),
+        "ncamd_event_elapsed_ms": (ctypes.c_float, [_vp, _vp]),
+        "ncamd_dev_fill_loguniform": (None, [_vp, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_uint64, _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -147,3 +160,66 @@ def fptr(a):
 
 def as_f64(x):
     return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+class DeviceArray:
+    """A float64 device buffer owned by libncrystal_amd (hipMalloc/hipFree)."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.ptr = call("ncamd_dev_malloc", 8 * max(self.n, 1))
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = as_f64(a)
+        d = cls(a.size)
+        call("ncamd_memcpy_htod", d.ptr, a.ctypes.data, 8 * a.size)
+        return d
+
+    def numpy(self, count=None):
+        count = self.n if count is None else int(count)
+        out = np.empty(count)
+        call("ncamd_memcpy_dtoh", out.ctypes.data, self.ptr, 8 * count)
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().ncamd_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        self.ptr = call("ncamd_stream_create")
+
+    def sync(self):
+        call("ncamd_stream_sync", self.ptr)
+
+    def __del__(self):
+        try:
+            lib().ncamd_stream_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self.ptr = call("ncamd_event_create")
+
+    def record(self, stream):
+        call("ncamd_event_record", self.ptr, stream.ptr if isinstance(stream, Stream) else stream)
+
+    def elapsed_ms(self, end):
+        return call("ncamd_event_elapsed_ms", self.ptr, end.ptr)
+
+    def __del__(self):
+        try:
+            lib().ncamd_event_destroy(self.ptr)
+        except Exception:
+            pass

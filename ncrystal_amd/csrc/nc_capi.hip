@@ -1099,4 +1099,69 @@ int64_t ncamd_dev_check_errors(ncamd_process_t h, void* stream) {
   return -1;
 }
 
+void* ncamd_dev_malloc(uint64_t bytes) {
+  NCAMD_TRY
+  ensure_device();
+  void* p = nullptr;
+  NCAMD_HIP(hipMalloc(&p, bytes ? bytes : 1));
+  return p;
+  NCAMD_CATCH
+  return nullptr;
+}
+void ncamd_dev_free(void* p) {
+  NCAMD_TRY if (p) NCAMD_HIP(hipFree(p)); NCAMD_CATCH
+}
+void ncamd_memcpy_htod(void* dst, const void* src, uint64_t bytes) {
+  NCAMD_TRY if (bytes) NCAMD_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); NCAMD_CATCH
+}
+void ncamd_memcpy_dtoh(void* dst, const void* src, uint64_t bytes) {
+  NCAMD_TRY if (bytes) NCAMD_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); NCAMD_CATCH
+}
+void* ncamd_stream_create(void) {
+  NCAMD_TRY
+  ensure_device();
+  hipStream_t s;
+  NCAMD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return (void*)s;
+  NCAMD_CATCH
+  return nullptr;
+}
+void ncamd_stream_destroy(void* s) {
+  NCAMD_TRY if (s) NCAMD_HIP(hipStreamDestroy((hipStream_t)s)); NCAMD_CATCH
+}
+void ncamd_stream_sync(void* s) {
+  NCAMD_TRY NCAMD_HIP(hipStreamSynchronize((hipStream_t)s)); NCAMD_CATCH
+}
+void* ncamd_event_create(void) {
+  NCAMD_TRY
+  ensure_device();
+  hipEvent_t e;
+  NCAMD_HIP(hipEventCreate(&e));
+  return (void*)e;
+  NCAMD_CATCH
+  return nullptr;
+}
+void ncamd_event_destroy(void* e) {
+  NCAMD_TRY if (e) NCAMD_HIP(hipEventDestroy((hipEvent_t)e)); NCAMD_CATCH
+}
+void ncamd_event_record(void* e, void* s) {
+  NCAMD_TRY NCAMD_HIP(hipEventRecord((hipEvent_t)e, (hipStream_t)s)); NCAMD_CATCH
+}
+float ncamd_event_elapsed_ms(void* a, void* b) {
+  NCAMD_TRY
+  float ms = 0.0f;
+  NCAMD_HIP(hipEventSynchronize((hipEvent_t)b));
+  NCAMD_HIP(hipEventElapsedTime(&ms, (hipEvent_t)a, (hipEvent_t)b));
+  return ms;
+  NCAMD_CATCH
+  return -1.0f;
+}
+void ncamd_dev_fill_loguniform(double* d, uint64_t n, double emin, double emax, uint64_t seed, void* s) {
+  NCAMD_TRY
+  if (!(emin > 0.0 && emax > emin)) raise("BadInput", "need 0 < emin < emax");
+  launch_fill_loguniform(d, (int64_t)n, log(emin), log(emax), seed, (hipStream_t)s);
+  NCAMD_HIP(hipGetLastError());
+  NCAMD_CATCH
+}
+
 }  // extern "C"

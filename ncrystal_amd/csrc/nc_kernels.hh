@@ -28,6 +28,7 @@ void launch_sample_dirs(const CompView&, const double* ekin, const double* dirs,
                         uint32_t stream, double* oE, double* oD, unsigned long long* nerr, hipStream_t);
 void launch_sab_ab(const SABView&, const double* ekin, int64_t n, uint64_t seed, uint32_t stream, double* oa,
                    double* ob, unsigned long long* nerr, hipStream_t);
+void launch_fill_loguniform(double* out, int64_t n, double lemin, double lemax, uint64_t seed, hipStream_t);
 
 // S(alpha,beta) table construction (nc_sab_build.hip)
 struct SABBuildBufs {

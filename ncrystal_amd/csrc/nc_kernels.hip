@@ -122,6 +122,18 @@ __global__ void __launch_bounds__(256) k_sab_ab(SABView s, const double* __restr
   }
 }
 
+// synthetic neutron energies, log-uniform, from a dedicated Philox stream
+__global__ void __launch_bounds__(256) k_fill_loguniform(double* __restrict__ out, int64_t n, double lemin,
+                                                         double lemax, uint64_t seed) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gstride) {
+    Rng rng;
+    rng.init(seed, 0xFFFFFFFFu, (uint64_t)k);
+    const double u = rng.gen();
+    out[k] = exp(lemin + u * (lemax - lemin));
+  }
+}
+
 static inline unsigned grid_for(int64_t n, int bs) {
   int64_t g = (n + bs - 1) / bs;
   // enough waves for the 256 CUs, bounded so a grid-stride loop covers the rest
@@ -154,6 +166,11 @@ void launch_sab_ab(const SABView& s, const double* ekin, int64_t n, uint64_t see
                    double* ob, unsigned long long* nerr, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_sab_ab, dim3(grid_for(n, 256)), dim3(256), 0, st, s, ekin, n, seed, stream, oa, ob, nerr);
+}
+
+void launch_fill_loguniform(double* out, int64_t n, double lemin, double lemax, uint64_t seed, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_loguniform, dim3(grid_for(n, 256)), dim3(256), 0, st, out, n, lemin, lemax, seed);
 }
 
 }  // namespace ncamd

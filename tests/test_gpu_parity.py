@@ -267,20 +267,24 @@ def test_rng_state_roundtrip(amd, oracle):
 
 
 def test_dev_pointer_api(amd, oracle):
-    import torch
+    from ncrystal_amd import _capi as C
     core, _ = amd
     d, g, o = _small_sab(core, oracle)
     e = synth.energies_logu(100000, 1e-4, 1.0, seed=12)
-    de = torch.from_numpy(e).cuda()
-    xs = torch.empty_like(de)
-    oe = torch.empty_like(de)
-    om = torch.empty_like(de)
-    st = torch.cuda.current_stream().cuda_stream
-    g.dev_crossSectionIsotropic(de.data_ptr(), len(e), xs.data_ptr(), stream=st)
+    st = C.Stream()
+    de = C.DeviceArray.from_numpy(e)
+    xs, oe, om = C.DeviceArray(len(e)), C.DeviceArray(len(e)), C.DeviceArray(len(e))
+    g.dev_crossSectionIsotropic(de.ptr, len(e), xs.ptr, stream=st.ptr)
     g.setRandomSeed(8)
-    g.dev_sampleScatterIsotropic(de.data_ptr(), len(e), oe.data_ptr(), om.data_ptr(), stream=st)
-    assert g.dev_check_errors(stream=st) == 0
-    torch.cuda.synchronize()
-    assert rel_err(xs.cpu().numpy(), o.xs(e)).max() <= XS_RTOL
+    g.dev_sampleScatterIsotropic(de.ptr, len(e), oe.ptr, om.ptr, stream=st.ptr)
+    assert g.dev_check_errors(stream=st.ptr) == 0
+    st.sync()
+    assert rel_err(xs.numpy(), o.xs(e)).max() <= XS_RTOL
     re_, rm, _ = o.sample_iso(e, seed=8)
-    assert _compare_samples(oe.cpu().numpy(), om.cpu().numpy(), re_, rm, "dev") < 2e-3
+    assert _compare_samples(oe.numpy(), om.numpy(), re_, rm, "dev") < 2e-3
+    # device-side energy generator: log-uniform in range
+    C.call("ncamd_dev_fill_loguniform", de.ptr, len(e), 1e-3, 1.0, 5, st.ptr)
+    st.sync()
+    v = de.numpy()
+    assert v.min() >= 1e-3 and v.max() <= 1.0
+    assert stats.kstest(np.log(v), "uniform", args=(math.log(1e-3), -math.log(1e-3))).pvalue > 1e-4
