@@ -175,6 +175,7 @@ void* ncamd_dev_malloc(uint64_t bytes);
 void ncamd_dev_free(void* p);
 void ncamd_memcpy_htod(void* dst, const void* src, uint64_t bytes);
 void ncamd_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
+void ncamd_memcpy_dtod(void* dst, const void* src, uint64_t bytes, void* stream);
 void* ncamd_stream_create(void);
 void ncamd_stream_destroy(void* stream);
 void ncamd_stream_sync(void* stream);

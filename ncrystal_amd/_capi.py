@@ -111,6 +111,7 @@ def lib():
         "ncamd_dev_free": (None, [_vp]),
         "ncamd_memcpy_htod": (None, [_vp, _vp, ctypes.c_uint64]),
         "ncamd_memcpy_dtoh": (None, [_vp, _vp, ctypes.c_uint64]),
+        "ncamd_memcpy_dtod": (None, [_vp, _vp, ctypes.c_uint64, _vp]),
         "ncamd_stream_create": (_vp, []),
         "ncamd_stream_destroy": (None, [_vp]),
         "ncamd_stream_sync": (None, [_vp]),

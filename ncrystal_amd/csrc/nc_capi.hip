@@ -1117,6 +1117,11 @@ void ncamd_memcpy_htod(void* dst, const void* src, uint64_t bytes) {
 void ncamd_memcpy_dtoh(void* dst, const void* src, uint64_t bytes) {
   NCAMD_TRY if (bytes) NCAMD_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); NCAMD_CATCH
 }
+void ncamd_memcpy_dtod(void* dst, const void* src, uint64_t bytes, void* s) {
+  NCAMD_TRY
+  if (bytes) NCAMD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)s));
+  NCAMD_CATCH
+}
 void* ncamd_stream_create(void) {
   NCAMD_TRY
   ensure_device();
