@@ -204,6 +204,8 @@ struct Proc {
   DevBuf d_err;
   // staging for the host-pointer API
   DevBuf st_in, st_in2, st_out1, st_out2;
+  // high-energy queue of the S(alpha,beta)-only sampling path
+  DevBuf q_idx, q_count;
   // component data
   PBView pb{};
   ElIncView ei{};
@@ -565,6 +567,36 @@ Proc* create_sab(int na, int nb, const double* alpha, const double* beta, const 
   }
   std::vector<double> et = erfc_table();
   DevBuf* bet = p->newbuf(); bet->upload(et.data(), et.size());
+  // egrid IEEE-bit bucket table: buckets narrower than half the smallest
+  // relative egrid spacing so a bracket holds at most ~2 grid points
+  {
+    double rmin = 1e300;
+    for (size_t i = 1; i < egrid.size(); ++i) rmin = std::min(rmin, egrid[i] / egrid[i - 1] - 1.0);
+    int m = (int)std::ceil(std::log2(2.0 / std::max(rmin, 1e-12)));
+    m = std::max(0, std::min(m, 20));
+    auto bits = [](double x) { unsigned long long u; std::memcpy(&u, &x, 8); return u; };
+    auto fromb = [](unsigned long long u) { double x; std::memcpy(&x, &u, 8); return x; };
+    int shift;
+    long long base, nbkt;
+    while (true) {
+      shift = 52 - m;
+      base = (long long)(bits(egrid.front()) >> shift);
+      nbkt = (long long)(bits(egrid.back()) >> shift) - base + 1;
+      if (nbkt <= (1 << 20) || m == 0) break;
+      --m;
+    }
+    std::vector<int> tab((size_t)nbkt + 1);
+    for (long long b = 0; b <= nbkt; ++b) {
+      const double edge = fromb((unsigned long long)(base + b) << shift);
+      tab[(size_t)b] = (int)(std::upper_bound(egrid.begin(), egrid.end(), edge) - egrid.begin());
+    }
+    DevBuf* btab = p->newbuf();
+    btab->upload(tab.data(), tab.size());
+    s.eg_bucket = btab->as<int>();
+    s.eg_nbkt = (int)nbkt;
+    s.eg_shift = shift;
+    s.eg_base = base;
+  }
   s.alpha = ba->as<double>();
   s.beta = bb->as<double>();
   s.sl = bsl->as<double2>();
@@ -577,6 +609,18 @@ Proc* create_sab(int na, int nb, const double* alpha, const double* beta, const 
   s.sxy = bxy->as<double2>();
   s.sinfo = binf->as<AInfo>();
   s.erfc_tab = bet->as<double>();
+  {
+    DevBuf* bcg = p->newbuf(); bcg->alloc(sizeof(unsigned short) * (size_t)ne * (kCdfGuide + 1));
+    DevBuf* brg = p->newbuf(); brg->alloc(sizeof(unsigned short) * (size_t)nb * (kRowGuide + 1));
+    DevBuf* brs = p->newbuf(); brs->alloc(sizeof(double) * (size_t)nb);
+    if (nb + 1 > 65535) raise("BadInput", "beta grid too large (at most 65534 points)");
+    launch_sab_guides(s, bcg->as<unsigned short>(), brg->as<unsigned short>(), brs->as<double>(), 0);
+    NCAMD_HIP(hipGetLastError());
+    NCAMD_HIP(hipDeviceSynchronize());
+    s.cdf_guide = bcg->as<unsigned short>();
+    s.row_guide = brg->as<unsigned short>();
+    s.row_scale = brs->as<double>();
+  }
   p->sab_egrid_h = egrid;
   p->sab_xs_h = xs;
   init_errbuf(p.get());
@@ -752,6 +796,31 @@ unsigned long long read_reset_err(Proc* p, hipStream_t st) {
   NCAMD_HIP(hipStreamSynchronize(st));
   if (n) NCAMD_HIP(hipMemsetAsync(p->d_err.p, 0, sizeof(n), st));
   return n;
+}
+
+// isotropic sampling dispatch: S(alpha,beta)-only handles take the split
+// main/high-energy kernels, everything else the composition kernel
+void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat, double* oE, double* oMu,
+                         hipStream_t st) {
+  const uint32_t stream = next_stream(p);
+  Proc* single = p->kind == kKindSAB ? p : nullptr;
+  if (!single && p->kind == kKindNone && p->comps.size() == 1 && p->comps[0]->kind == kKindSAB &&
+      p->scales[0] == 1.0)
+    single = p->comps[0];
+  if (single) {
+    const size_t need = sizeof(int64_t) * (size_t)std::max<int64_t>(n * repeat, 1);
+    if (p->q_idx.bytes < need) {
+      NCAMD_HIP(hipStreamSynchronize(st));  // the old queue may still be in use on this stream
+      p->q_idx.alloc(need);
+    }
+    if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
+    launch_sab_sample_iso(single->sab, ekin, n, repeat, p->seed, stream, oE, oMu,
+                          p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
+                          p->q_count.as<unsigned long long>(), st);
+  } else {
+    launch_sample_iso(make_view(p), ekin, n, repeat, p->seed, stream, oE, oMu, p->d_err.as<unsigned long long>(),
+                      st);
+  }
 }
 
 void check_fp32(Proc* p) {
@@ -982,8 +1051,9 @@ void ncamd_samplescatterisotropic_many(ncamd_process_t h, const double* ekin, un
   p->st_out1.ensure(sizeof(double) * tot);
   p->st_out2.ensure(sizeof(double) * tot);
   if (n) NCAMD_HIP(hipMemcpy(p->st_in.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
-  launch_sample_iso(v, p->st_in.as<double>(), (int64_t)n, (int64_t)repeat, p->seed, next_stream(p),
-                    p->st_out1.as<double>(), p->st_out2.as<double>(), p->d_err.as<unsigned long long>(), 0);
+  (void)v;
+  sample_iso_dispatch(p, p->st_in.as<double>(), (int64_t)n, (int64_t)repeat, p->st_out1.as<double>(),
+                      p->st_out2.as<double>(), 0);
   NCAMD_HIP(hipGetLastError());
   if (tot) {
     NCAMD_HIP(hipMemcpy(oE, p->st_out1.p, sizeof(double) * tot, hipMemcpyDeviceToHost));
@@ -1067,8 +1137,7 @@ void ncamd_dev_samplescatterisotropic_many(ncamd_process_t h, const double* d_ek
                                            unsigned long repeat, double* d_oE, double* d_oMu, void* stream) {
   NCAMD_TRY
   Proc* p = extract(h);
-  launch_sample_iso(make_view(p), d_ekin, (int64_t)n, (int64_t)repeat, p->seed, next_stream(p), d_oE, d_oMu,
-                    p->d_err.as<unsigned long long>(), as_stream(stream));
+  sample_iso_dispatch(p, d_ekin, (int64_t)n, (int64_t)repeat, d_oE, d_oMu, as_stream(stream));
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH
 }

@@ -311,12 +311,33 @@ struct SABView {
   const double* erfc_tab; // [1103] erfc lookup for the free-gas sampler
   double kT, T, mass, sigma_free, fg_ca, k_ext, k1, k2;
   double egrid_margin;
+  // Search acceleration.  Every table stores, per bucket, the search result
+  // at the bucket's lower edge, so the bracket [table[b], table[b+1]] always
+  // contains the result of the full binary search: identical answers, fewer
+  // dependent loads.
+  const int* eg_bucket;            // [eg_nbkt+1]: upper_bound(egrid, lower edge of IEEE-bit bucket b)
+  int eg_nbkt, eg_shift;
+  long long eg_base;               // bucket b = (bits(E) >> eg_shift) - eg_base
+  const unsigned short* cdf_guide; // [ne][kCdfGuide+1]: lower_bound(cdf, g/kCdfGuide)
+  const unsigned short* row_guide; // [nb][kRowGuide+1]: upper_bound(cumul row, g/row_scale)
+  const double* row_scale;         // [nb]: kRowGuide / cumul[row][na-1] (0: row without area)
 };
+constexpr int kCdfGuide = 256;  // power of two: p*kCdfGuide is exact
+constexpr int kRowGuide = 64;
 
 NCD double sab_fg_xs(const SABView& s, double E) { return s.sigma_free * freegas_xs_shape(s.fg_ca * E); }
 
+// upper_bound(egrid, E) through the IEEE-bit bucket table
+NCD int egrid_upper_bound(const SABView& s, double E) {
+  if (!(E >= s.egrid[0])) return (E < s.egrid[0]) ? 0 : s.ne;  // below grid / NaN
+  if (E >= s.egrid[s.ne - 1]) return s.ne;
+  const long long b = (long long)((unsigned long long)__double_as_longlong(E) >> s.eg_shift) - s.eg_base;
+  const int lo = s.eg_bucket[b], hi = s.eg_bucket[b + 1];
+  return upper_bound_d(s.egrid, lo, hi, E);
+}
+
 NCD double sab_xs(const SABView& s, double E) {
-  const int up = upper_bound_d(s.egrid, 0, s.ne, E);
+  const int up = egrid_upper_bound(s, E);
   if (up == s.ne) return s.k_ext / E + sab_fg_xs(s, E);
   if (up == 0) return E > 0.0 ? sqrt(s.egrid[0] / E) * s.xs[0] : __builtin_huge_val();
   const int lo = up - 1;
@@ -326,10 +347,11 @@ NCD double sab_xs(const SABView& s, double E) {
 }
 
 // PointwiseDist::percentileWithIndex
-NCD double pw_percentile(const double* __restrict__ cdf, const double2* __restrict__ xy, int n,
-                         double p, int& idx) {
+NCD double pw_percentile(const double* __restrict__ cdf, const double2* __restrict__ xy,
+                         const unsigned short* __restrict__ guide, int n, double p, int& idx) {
   if (p == 1.) { idx = n - 2; return xy[n - 1].x; }
-  int lb = lower_bound_d(cdf, 0, n, p);
+  const int g = (int)(p * kCdfGuide);  // exact scaling; p in (0,1) -> g in [0, kCdfGuide-1]
+  int lb = lower_bound_d(cdf, guide[g], guide[g + 1], p);
   int i = lb < n - 1 ? lb : n - 1;
   if (i < 1) i = 1;
   const double2 p0 = xy[i - 1], p1 = xy[i];
@@ -373,7 +395,22 @@ NCD double alg1_sample_alpha(const SABView& s, int ie, int ibl, int ibeta, doubl
     const int il = info.f_idx, iu = info.b_idx;
     const double cl = cumul[il], cu = cumul[iu];
     const double sel = cl + p2 * (cu - cl);
-    const int e = upper_bound_d(cumul, il, iu + 1, sel);
+    int e;
+    const double rs = s.row_scale[ibeta];
+    if (rs > 0.0) {
+      // the global upper_bound of sel lies in the guide bracket (one bucket of
+      // margin either side absorbs rounding of sel*rs); clamping it to iu+1
+      // equals the bounded search of the reference (cumul[il] <= sel)
+      int g = (int)(sel * rs);
+      g = g < 0 ? 0 : (g > kRowGuide ? kRowGuide : g);
+      const unsigned short* rg = s.row_guide + (size_t)ibeta * (kRowGuide + 1);
+      const int lo = rg[g > 0 ? g - 1 : 0];
+      const int hi = rg[g + 2 <= kRowGuide ? g + 2 : kRowGuide];
+      e = upper_bound_d(cumul, lo, hi, sel);
+      if (e > iu + 1) e = iu + 1;
+    } else {
+      e = upper_bound_d(cumul, il, iu + 1, sel);
+    }
     if (e > iu) return ag[iu];
     if (e <= il) return ag[il];
     const int el = e - 1;
@@ -402,10 +439,11 @@ NCD int alg1_sample(const SABView& s, int ie, double ediv, Rng& rng, double& a_o
   const double firstbin = s.sfirst[ie];
   const double* __restrict__ cdf = s.scdf + (size_t)ie * (s.nb + 1);
   const double2* __restrict__ xy = s.sxy + (size_t)ie * (s.nb + 1);
+  const unsigned short* __restrict__ guide = s.cdf_guide + (size_t)ie * (kCdfGuide + 1);
   const double* __restrict__ bg = s.beta;
   for (int it = 0; it < 100; ++it) {
     int ibs;
-    double beta = pw_percentile(cdf, xy, nv, rng.gen(), ibs);
+    double beta = pw_percentile(cdf, xy, guide, nv, rng.gen(), ibs);
     if (ibs == 0 && firstbin <= 0.0) {
       const double b0 = firstbin;
       const double b1 = xy[1].x;
@@ -473,12 +511,14 @@ NCD int sab_sample_highe(const SABView& s, double E, Rng& rng, double& a_out, do
 }
 
 // SABSampler::sampleAlphaBeta (NCSABSampler.cc:158-227)
+template <bool WithHighE = true>
 NCD int sab_sample_ab(const SABView& s, double E, Rng& rng, double& a_out, double& b_out) {
-  int up = upper_bound_d(s.egrid, 0, s.ne, E);
+  int up = egrid_upper_bound(s, E);
   bool ultra = false;
   const double ultra_e = s.egrid[0];
   int isamp;
   if (up == s.ne) {
+    if constexpr (!WithHighE) return kErrBadSampler;  // caller routes E >= Emax elsewhere
     double a, b;
     const int rc = sab_sample_highe(s, E, rng, a, b);
     if (rc) return rc;
@@ -512,9 +552,10 @@ NCD int sab_sample_ab(const SABView& s, double E, Rng& rng, double& a_out, doubl
 }
 
 // SABSampler::sampleDeltaEMu + SABScatter::sampleScatterIsotropic
+template <bool WithHighE = true>
 NCD int sab_sample_iso(const SABView& s, Rng& rng, double E, double& Ef, double& mu) {
   double a, b;
-  const int rc = sab_sample_ab(s, E, rng, a, b);
+  const int rc = sab_sample_ab<WithHighE>(s, E, rng, a, b);
   if (rc) return rc;
   const double kT = s.kT;
   double dE, m;

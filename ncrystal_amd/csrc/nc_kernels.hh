@@ -29,6 +29,12 @@ void launch_sample_dirs(const CompView&, const double* ekin, const double* dirs,
 void launch_sab_ab(const SABView&, const double* ekin, int64_t n, uint64_t seed, uint32_t stream, double* oa,
                    double* ob, unsigned long long* nerr, hipStream_t);
 void launch_fill_loguniform(double* out, int64_t n, double lemin, double lemax, uint64_t seed, hipStream_t);
+// S(alpha,beta)-only sampling: main kernel + high-energy queue kernel
+void launch_sab_sample_iso(const SABView&, const double* ekin, int64_t n, int64_t repeat, uint64_t seed,
+                           uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, int64_t* queue,
+                           unsigned long long* qcount, hipStream_t);
+void launch_sab_guides(const SABView&, unsigned short* cdf_guide, unsigned short* row_guide, double* row_scale,
+                       hipStream_t);
 
 // S(alpha,beta) table construction (nc_sab_build.hip)
 struct SABBuildBufs {

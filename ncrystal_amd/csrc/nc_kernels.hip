@@ -122,6 +122,93 @@ __global__ void __launch_bounds__(256) k_sab_ab(SABView s, const double* __restr
   }
 }
 
+// S(alpha,beta)-only materials: the free-gas extender (E >= Emax) is rare and
+// register-hungry, so the main kernel leaves those neutrons in a queue that a
+// second kernel drains.  Both draw from the neutron's own Philox stream from
+// its first draw, so the split does not change any result.
+__global__ void __launch_bounds__(256) k_sab_sample_iso_main(SABView s, const double* __restrict__ ekin,
+                                                             int64_t n, int64_t total, uint64_t seed,
+                                                             uint32_t stream, double* __restrict__ oE,
+                                                             double* __restrict__ oMu,
+                                                             unsigned long long* __restrict__ nerr,
+                                                             int64_t* __restrict__ queue,
+                                                             unsigned long long* __restrict__ qcount) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  const double emax = s.egrid[s.ne - 1];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gstride) {
+    const int64_t i = (total == n) ? k : (k % n);
+    const double E = ekin[i];
+    double Ef, mu;
+    if (!(E >= 0.0 && E <= __builtin_huge_val())) {
+      Ef = E; mu = 1.0;
+    } else if (E >= emax) {
+      const unsigned long long slot = atomicAdd(qcount, 1ull);
+      queue[slot] = k;
+      continue;
+    } else {
+      Rng rng;
+      rng.init(seed, stream, (uint64_t)k);
+      const int rc = sab_sample_iso<false>(s, rng, E, Ef, mu);
+      if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
+    }
+    oE[k] = Ef;
+    oMu[k] = mu;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sab_sample_iso_highe(SABView s, const double* __restrict__ ekin,
+                                                              int64_t n, int64_t total, uint64_t seed,
+                                                              uint32_t stream, double* __restrict__ oE,
+                                                              double* __restrict__ oMu,
+                                                              unsigned long long* __restrict__ nerr,
+                                                              const int64_t* __restrict__ queue,
+                                                              const unsigned long long* __restrict__ qcount) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t nq = (int64_t)*qcount;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gstride) {
+    const int64_t k = queue[q];
+    const int64_t i = (total == n) ? k : (k % n);
+    Rng rng;
+    rng.init(seed, stream, (uint64_t)k);
+    double Ef, mu;
+    const int rc = sab_sample_iso<true>(s, rng, ekin[i], Ef, mu);
+    if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
+    oE[k] = Ef;
+    oMu[k] = mu;
+  }
+}
+
+// search-acceleration tables of an S(alpha,beta) process (built once)
+__global__ void k_sab_cdf_guides(SABView s, unsigned short* __restrict__ guide) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = kCdfGuide + 1;
+  if (t >= (int64_t)s.ne * per) return;
+  const int ie = (int)(t / per), g = (int)(t % per);
+  const int4 hdr = s.shdr[ie];
+  unsigned short r = 0;
+  if (hdr.x == 1) {
+    const double* cdf = s.scdf + (size_t)ie * (s.nb + 1);
+    r = (unsigned short)lower_bound_d(cdf, 0, hdr.z, (double)g / (double)kCdfGuide);
+  }
+  guide[t] = r;
+}
+__global__ void k_sab_row_guides(SABView s, unsigned short* __restrict__ guide, double* __restrict__ scale) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = kRowGuide + 1;
+  if (t >= (int64_t)s.nb * per) return;
+  const int ib = (int)(t / per), g = (int)(t % per);
+  const double* row = s.cumul + (size_t)ib * s.na;
+  const double rmax = row[s.na - 1];
+  const double rs = rmax > 0.0 ? (double)kRowGuide / rmax : 0.0;
+  if (g == 0) scale[ib] = rs;
+  // the last bucket is closed by na itself so that values at or above the
+  // row maximum always have their answer inside the bracket
+  const int r = (g == kRowGuide) ? s.na : upper_bound_d(row, 0, s.na, (double)g / rs);
+  guide[t] = rs > 0.0 ? (unsigned short)r : (unsigned short)0;
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
 // synthetic neutron energies, log-uniform, from a dedicated Philox stream
 __global__ void __launch_bounds__(256) k_fill_loguniform(double* __restrict__ out, int64_t n, double lemin,
                                                          double lemax, uint64_t seed) {
@@ -168,6 +255,26 @@ void launch_sab_ab(const SABView& s, const double* ekin, int64_t n, uint64_t see
   hipLaunchKernelGGL(k_sab_ab, dim3(grid_for(n, 256)), dim3(256), 0, st, s, ekin, n, seed, stream, oa, ob, nerr);
 }
 
+void launch_sab_sample_iso(const SABView& s, const double* ekin, int64_t n, int64_t repeat, uint64_t seed,
+                           uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, int64_t* queue,
+                           unsigned long long* qcount, hipStream_t st) {
+  const int64_t total = n * repeat;
+  if (total <= 0) return;
+  // kernel instead of a memset node, so that callers may capture this in a graph
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount);
+  hipLaunchKernelGGL(k_sab_sample_iso_main, dim3(grid_for(total, 256)), dim3(256), 0, st, s, ekin, n, total, seed,
+                     stream, oE, oMu, nerr, queue, qcount);
+  hipLaunchKernelGGL(k_sab_sample_iso_highe, dim3(grid_for(total, 256)), dim3(256), 0, st, s, ekin, n, total, seed,
+                     stream, oE, oMu, nerr, queue, qcount);
+}
+void launch_sab_guides(const SABView& s, unsigned short* cdf_guide, unsigned short* row_guide, double* row_scale,
+                       hipStream_t st) {
+  const int64_t n1 = (int64_t)s.ne * (kCdfGuide + 1);
+  hipLaunchKernelGGL(k_sab_cdf_guides, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, s, cdf_guide);
+  const int64_t n2 = (int64_t)s.nb * (kRowGuide + 1);
+  hipLaunchKernelGGL(k_sab_row_guides, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, s, row_guide,
+                     row_scale);
+}
 void launch_fill_loguniform(double* out, int64_t n, double lemin, double lemax, uint64_t seed, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill_loguniform, dim3(grid_for(n, 256)), dim3(256), 0, st, out, n, lemin, lemax, seed);

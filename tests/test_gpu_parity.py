@@ -200,6 +200,12 @@ def test_sab_sampling(amd, oracle):
     assert _compare_samples(oe, om, re_, rm, "sab") < 2e-3
     for x, y in ((oe, re_), (om, rm)):
         assert stats.ks_2samp(x, y).statistic < 0.01
+    # repeat layout through the split main/high-energy kernels
+    g.setRandomSeed(7)
+    oe2, om2 = g.sampleScatterIsotropic(e[:20000], repeat=3)
+    re2, rm2, nerr = o.sample_iso(e[:20000], repeat=3, seed=7)
+    assert nerr == 0
+    assert _compare_samples(oe2, om2, re2, rm2, "sab repeat") < 2e-3
 
 
 def test_sab_c3_table(amd, oracle):
