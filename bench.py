@@ -143,7 +143,7 @@ def main():
                    "neutrons_per_step": n, "sampler": "plain", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sample_iso", "kernel_ms": kernel_ms,
+                     "kernel": "k_sab_sample_iso_main", "kernel_ms": kernel_ms,
                      "bytes_per_sample": BYTES_PER_SAMPLE},
         "cpu_baseline": cpu,
     }

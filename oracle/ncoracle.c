@@ -355,6 +355,14 @@ int orc_hkl_orthogonal(double a, double b, double c, int nspecies, const int* na
                        const double* positions, const double* csl, const double* msd,
                        double dcutoff, double fsquarecut, double merge_tol, int maxout,
                        int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq) {
+  return orc_hkl_orthogonal_members(a, b, c, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut,
+                                    merge_tol, maxout, out_hkl, out_mult, out_dsp, out_fsq, NULL, NULL, 0);
+}
+int orc_hkl_orthogonal_members(double a, double b, double c, int nspecies, const int* natoms,
+                               const double* positions, const double* csl, const double* msd,
+                               double dcutoff, double fsquarecut, double merge_tol, int maxout,
+                               int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq,
+                               int64_t* out_mem_off, int* out_mem_hkl, int64_t maxmem) {
   const double rl0 = K_2PI / a, rl1 = K_2PI / b, rl2 = K_2PI / c;
   const double invd = 1.0 / dcutoff;
   const int max_h = floorhkl(a * invd), max_k = floorhkl(b * invd), max_l = floorhkl(c * invd);
@@ -470,6 +478,24 @@ int orc_hkl_orthogonal(double a, double b, double c, int nspecies, const int* na
         out_mult[i] = fams[i].mult; out_dsp[i] = fams[i].dsp; out_fsq[i] = fams[i].fsq;
       }
       result = nf;
+      if (out_mem_off) {
+        int64_t tot = 0;
+        for (int i = 0; i < nf; ++i) tot += fams[i].nmem;
+        if (tot > maxmem) {
+          result = -1;
+        } else {
+          int64_t o = 0;
+          for (int i = 0; i < nf; ++i) {
+            out_mem_off[i] = o;
+            for (int j = 0; j < fams[i].nmem; ++j, ++o) {
+              out_mem_hkl[3 * o] = fams[i].members[j].h;
+              out_mem_hkl[3 * o + 1] = fams[i].members[j].k;
+              out_mem_hkl[3 * o + 2] = fams[i].members[j].l;
+            }
+          }
+          out_mem_off[nf] = o;
+        }
+      }
     }
   }
   for (int i = 0; i < nf; ++i) free(fams[i].members);
@@ -482,7 +508,7 @@ int orc_hkl_orthogonal(double a, double b, double c, int nspecies, const int* na
 
 /* ------------------------------------------------------------------ */
 /* process objects */
-enum { P_PB = 1, P_ELINC = 2, P_SAB = 3, P_COMP = 4 };
+enum { P_PB = 1, P_ELINC = 2, P_SAB = 3, P_COMP = 4, P_SCB = 5 };
 
 typedef struct {
   int n; double* e2d; double* fdm; double threshold;
@@ -518,9 +544,10 @@ typedef struct {
 
 typedef struct { int n; orc_proc** comps; double* scales; } comp_t;
 
+struct scb_s;
 struct orc_proc {
   int type;
-  union { pb_t pb; elinc_t ei; sab_t sab; comp_t comp; } u;
+  union { pb_t pb; elinc_t ei; sab_t sab; comp_t comp; struct scb_s* scb; } u;
 };
 
 /* ------------------------------------------------------------------ */
@@ -1407,6 +1434,7 @@ static int proc_sample_iso(const orc_proc* p, rng_t* rng, double E, double* Ef, 
   return -9;
 }
 
+static void scb_free(orc_proc* p);
 void orc_free(orc_proc* p) {
   if (!p) return;
   switch (p->type) {
@@ -1422,6 +1450,7 @@ void orc_free(orc_proc* p) {
       break;
     }
     case P_COMP: free(p->u.comp.comps); free(p->u.comp.scales); break;
+    case P_SCB: scb_free(p); break;
   }
   free(p);
 }
@@ -1457,11 +1486,19 @@ typedef struct {
   const orc_proc* p; const double* ekin; const double* dirs; int64_t n, repeat;
   double *o1, *o2; uint64_t seed, idxoff; uint32_t stream; int64_t lo, hi; int64_t nerr; int mode;
 } job_t;
+static double scb_xs_p(const orc_proc* p, double E, const double* d);
+static void scb_sample_p(const orc_proc* p, rng_t* rng, double E, const double* d, double* od);
 static void* job_run(void* arg) {
   job_t* j = (job_t*)arg;
   for (int64_t k = j->lo; k < j->hi; ++k) {
     if (j->mode == 0) {
-      j->o1[k] = proc_xs(j->p, j->ekin[k]);
+      j->o1[k] = j->p->type == P_SCB ? scb_xs_p(j->p, j->ekin[k], j->dirs + 3 * k) : proc_xs(j->p, j->ekin[k]);
+      continue;
+    }
+    if (j->mode == 3 && j->p->type == P_SCB) {
+      rng_t rng; rng_init(&rng, j->seed, j->stream, j->idxoff + (uint64_t)k);
+      j->o1[k] = j->ekin[k];
+      scb_sample_p(j->p, &rng, j->ekin[k], j->dirs + 3 * k, j->o2 + 3 * k);
       continue;
     }
     int64_t i = k % j->n;
@@ -1507,6 +1544,12 @@ static int64_t run_jobs(job_t proto, int64_t total, int nthreads) {
 void orc_xs_many(const orc_proc* p, const double* ekin, int64_t n, double* out, int nthreads) {
   job_t j; memset(&j, 0, sizeof(j));
   j.p = p; j.ekin = ekin; j.n = n; j.repeat = 1; j.o1 = out; j.mode = 0;
+  run_jobs(j, n, nthreads);
+}
+void orc_xs_dir_many(const orc_proc* p, const double* ekin, const double* dirs, int64_t n, double* out,
+                     int nthreads) {
+  job_t j; memset(&j, 0, sizeof(j));
+  j.p = p; j.ekin = ekin; j.dirs = dirs; j.n = n; j.repeat = 1; j.o1 = out; j.mode = 0;
   run_jobs(j, n, nthreads);
 }
 int64_t orc_sample_iso_many(const orc_proc* p, uint64_t seed, uint32_t stream, uint64_t idxoff,
@@ -1899,4 +1942,750 @@ static int orc_fg_sample_alpha_beta(double E, double T, double mass, rng_t* rng,
   *alpha = fgs_sample_alpha(&f, b, rng);
   *beta = b;
   return 0;
+}
+
+/* ================================================================== */
+/* SCBragg: mosaic single crystal.                                     */
+/* ncrystal_core/src/scbragg/NCSCBragg.cc:125-323 (families, cache,    */
+/* sampling); ncrystal_core/src/phys_utils/NCGaussMos.cc:29-280;       */
+/* ncrystal_core/src/phys_utils/NCGaussOnSphere.cc:21-547 and          */
+/* ncrystal_core/include/NCrystal/internal/phys_utils/NCGaussOnSphere.hh:150-211; */
+/* splines ncrystal_core/src/utils/NCSpline.cc:31-108 +                */
+/* ncrystal_core/include/NCrystal/internal/utils/NCSpline.hh:118-160;  */
+/* numerics ncrystal_core/src/utils/NCMath.cc:96-336, NCMath.hh:339-524; */
+/* rotations ncrystal_core/src/utils/NCRotMatrix.cc:84-147,            */
+/* ncrystal_core/include/NCrystal/internal/utils/NCRotMatrix.hh:169-191; */
+/* demi-normals ncrystal_core/src/extd_utils/NCPlaneProvider.cc:111-175. */
+
+#define K_FWHM2SIGMA 0.424660900144009521360751417051444809857570547
+#define K_ARCSEC 0.00000484813681109535993589914102357947975956353302
+
+static double cos_mpi2pi2_t(double x) {
+  double mx2 = -x * x;
+  return 1.0 + mx2 * (0.5
+         + mx2 * (4.16666666666666666666666666666666666666666667e-2
+         + mx2 * (1.38888888888888888888888888888888888888888889e-3
+         + mx2 * (2.48015873015873015873015873015873015873015873e-5
+         + mx2 * (2.75573192239858906525573192239858906525573192e-7
+         + mx2 * (2.08767569878680989792100903212014323125434237e-9
+         + mx2 * (1.14707455977297247138516979786821056662326504e-11
+         + mx2 * (4.77947733238738529743820749111754402759693765e-14
+         + mx2 * (1.56192069685862264622163643500573334235194041e-16
+         + mx2 * (4.1103176233121648584779906184361403746103695e-19
+         + mx2 * (8.89679139245057328674889744250246834331248809e-22)))))))))));
+}
+static double cos_mpipi_t(double A) {
+  double Aabs = fabs(A);
+  double x = dmin(Aabs, K_PI - Aabs);
+  return copysign(cos_mpi2pi2_t(x), K_PIHALF - Aabs);
+}
+static void sincos_mpi8pi8_t(double A, double* cosA, double* sinA) {
+  double x = 0.5 * A;
+  double mx2 = -x * x;
+  double s2 = x * (1.0 + mx2 * (1.66666666666666666666666666666666666666666667e-1
+                   + mx2 * (8.33333333333333333333333333333333333333333333e-3
+                   + mx2 * (1.98412698412698412698412698412698412698412698e-4
+                   + mx2 * (2.75573192239858906525573192239858906525573192e-6
+                   + mx2 * (2.50521083854417187750521083854417187750521084e-8))))));
+  double c2m1 = mx2 * (0.5 + mx2 * (4.16666666666666666666666666666666666666666667e-2
+                + mx2 * (1.38888888888888888888888888888888888888888889e-3
+                + mx2 * (2.48015873015873015873015873015873015873015873e-5
+                + mx2 * (2.75573192239858906525573192239858906525573192e-7)))));
+  double k = 2.0 * c2m1;
+  *sinA = (k + 2.0) * s2;
+  *cosA = 1.0 + k * (c2m1 + 2.0);
+}
+static void sincos_0pi32_t(double A, double* c, double* s) {
+  sincos_mpi2pi2(dmin(A, K_PI - A), c, s);
+  *c = copysign(*c, K_PIHALF - A);
+}
+
+/* CosSinGridGen (NCMath.hh:489-524), small_angles=true */
+typedef struct { double c, s, cd, sd; unsigned left, recalc; double phimax, negdelta; } csgrid_t;
+static void csgrid_init(csgrid_t* g, unsigned n, double offset, double delta) {
+  g->left = n - 1;
+  g->recalc = ((128u - 1) + (n / 128u) * 128u) - n;
+  g->phimax = offset + (n - 1) * delta;
+  g->negdelta = -delta;
+  sincos_0pi32_t(offset, &g->c, &g->s);
+  sincos_mpi8pi8_t(delta, &g->cd, &g->sd);
+}
+static int csgrid_step(csgrid_t* g) {
+  if (!g->left) return 0;
+  --g->left;
+  if ((g->left + g->recalc) % 128u) {
+    double c = g->c * g->cd - g->s * g->sd;
+    g->s = g->c * g->sd + g->s * g->cd;
+    g->c = c;
+  } else {
+    double v = g->phimax + g->negdelta * g->left;
+    g->c = cos(v);
+    g->s = sin(v);
+  }
+  return 1;
+}
+
+/* CubicSpline + SplinedLookupTable */
+typedef struct { double a, b, invdelta; int n; double* y; double* y2; } slt_t;
+typedef double (*f1d_t)(double, const void*);
+static void spline_set(slt_t* t, const double* y, int n, double dl, double dr) {
+  const int nm1 = n - 1;
+  double* u = (double*)calloc((size_t)nm1, sizeof(double));
+  double* y2 = (double*)calloc((size_t)n, sizeof(double));
+  y2[0] = -0.5;
+  double dy_left = y[1] - y[0];
+  u[0] = 3.0 * (dy_left - dl);
+  for (int i = 1; i < nm1; ++i) {
+    double p = 2.0 + 0.5 * y2[i - 1];
+    p = 1.0 / p;
+    y2[i] = -0.5 * p;
+    double uu = y[i + 1] - 2.0 * y[i] + y[i - 1];
+    u[i] = (3.0 * uu - 0.5 * u[i - 1]) * p;
+  }
+  double qn = 0.5;
+  double un = 3.0 * (dr - (y[nm1] - y[n - 2]));
+  y2[nm1] = (un - qn * u[n - 2]) / (qn * y2[n - 2] + 1.0);
+  for (int k = nm1; k; --k) {
+    int km1 = k - 1;
+    y2[km1] *= y2[k];
+    y2[km1] += u[km1];
+  }
+  free(u);
+  t->n = n;
+  t->y = (double*)malloc(sizeof(double) * (size_t)n);
+  memcpy(t->y, y, sizeof(double) * (size_t)n);
+  t->y2 = y2;
+}
+static void slt_set(slt_t* t, f1d_t f, const void* ctx, double a, double b, double fpa, double fpb, unsigned npts) {
+  t->a = a; t->b = b;
+  double* fv = (double*)malloc(sizeof(double) * npts);
+  double delta = (b - a) / (npts - 1);
+  unsigned nm1 = npts - 1;
+  for (unsigned i = 0; i < nm1; ++i) fv[i] = f(a + i * delta, ctx);
+  fv[nm1] = f(b, ctx);
+  spline_set(t, fv, (int)npts, fpa * delta, fpb * delta);
+  t->invdelta = 1.0 / delta;
+  free(fv);
+}
+static double slt_eval(const slt_t* t, double x) {
+  double xx = (x - t->a) * t->invdelta;
+  size_t nm2 = (size_t)(t->n - 2);
+  size_t idx = (size_t)xx;
+  if (xx < 0) idx = 0; /* conversion of negative values: the reference casts */
+  if (idx > nm2) idx = nm2;
+  double b = xx - (double)idx;
+  double a = 1.0 - b;
+  double tmp = a * t->y[idx];
+  double tmp2 = (a * a * a - a) * t->y2[idx];
+  tmp += b * t->y[idx + 1];
+  tmp2 += (b * b * b - b) * t->y2[idx + 1];
+  return tmp + 0.166666666666666666666666666666666666666666666666666667 * tmp2;
+}
+/* estimateSingleSidedDerivative, order 4 (NCMath.cc:271-290) */
+static double ssd4(f1d_t f, const void* ctx, double x, double h) {
+  return -(2 / 3.) * (31.5 * f(x, ctx) - 64. * f(x + 0.125 * h, ctx) + 40. * f(x + 0.25 * h, ctx)
+                      - 8. * f(x + 0.5 * h, ctx) + 0.5 * f(x + h, ctx)) / h;
+}
+/* findRoot (NCMath.cc:292-334) */
+static double find_root(f1d_t f, const void* ctx, double a, double b, double acc) {
+  double fa = f(a, ctx), fb = f(b, ctx);
+  acc *= 0.5;
+  unsigned i = 60;
+  while (--i) {
+    double dfba = fb - fa;
+    double c = (a * fb - b * fa) / dfba;
+    if (b - a < acc) return c;
+    double k = 0.15 * (b - a);
+    c = dmax(a + k, dmin(b - k, c));
+    double fc = f(c, ctx);
+    if (!fc) return c;
+    if (fa * fc < 0) { b = c; fb = fc; } else { a = c; fa = fc; }
+  }
+  return NAN;
+}
+
+typedef struct { double norm, expval; } evcx_t;
+static double f_evalcosx(double cosx, const void* v) {
+  const evcx_t* e = (const evcx_t*)v;
+  double x = acos(dmin(1.0, dmax(-1.0, cosx)));
+  return e->norm * exp(e->expval * x * x);
+}
+typedef struct { double k, expfact, tasq; } sofcosd_t;
+static double f_sofcosd(double cosd, const void* v) {
+  const sofcosd_t* e = (const sofcosd_t*)v;
+  double d = acos(dmin(1.0, dmax(-1.0, cosd)));
+  double d2 = d * d;
+  return e->k * exp(e->expfact * d2) * erf(sqrt(dmax(0.0, -e->expfact * (e->tasq - d2))));
+}
+static double f_cosexp(double x, const void* v) {
+  double prec = *(const double*)v;
+  return (1.0 - (1.0 - 0.5 * x * x) / cos_mpi2pi2_t(x)) - prec;
+}
+typedef struct { double c; } gnorm_t;
+static double f_gnorm(double x, void* v) {
+  const gnorm_t* g = (const gnorm_t*)v;
+  return dmax(0.0, sin(x) * exp(g->c * x * x));
+}
+static int acc_gnorm(unsigned lvl, double prev, double est, void* v) {
+  (void)v;
+  return lvl > 12 || fabs(prev - est) < 1e-12 * fabs(est);
+}
+
+typedef struct {
+  double sigma, truncangle, prec, cta, sta, expfact, norm, k1, k2, numacc;
+  unsigned nlt;
+  slt_t lt_evalcosx, lt_sofcosd;
+} gos_t;
+
+static void gos_set(gos_t* g, double sigma, double trunc, double prec) {
+  if (prec <= 1) prec = dclamp(prec, 1e-7, 0.1); else prec = dclamp(prec, 1.0, 10000.0);
+  g->truncangle = trunc; g->sigma = sigma; g->prec = prec;
+  g->cta = cos(trunc); g->sta = sin(trunc);
+  g->expfact = -0.5 / (sigma * sigma);
+  {
+    gnorm_t gn = { -0.5 / (sigma * sigma) };
+    int ok;
+    double integral = K_2PI * romberg(f_gnorm, acc_gnorm, &gn, 0.0, dmin(trunc, 20.0 * sigma), &ok);
+    g->norm = 1.0 / integral;
+  }
+  unsigned nlt;
+  if (prec < 1.0) {
+    double tp = dmin(0.999999, prec * 0.5);
+    double amax = find_root(f_cosexp, &tp, 0.0, 0.999999 * K_PIHALF, 1e-13);
+    if (g->truncangle > amax || amax <= 1e-10) { g->k1 = INFINITY; g->k2 = 0.0; }
+    else { g->k1 = cos_mpi2pi2_t(amax); g->k2 = g->cta - 1e-5; }
+    g->numacc = dclamp(prec * 0.1, 1e-7, 1e-4);
+    unsigned v = (unsigned)(10.0 / sqrt(prec) + 0.5);
+    nlt = v < 10000 ? v : 10000;
+    if (nlt < 40) nlt = 40;
+  } else {
+    unsigned v = (unsigned)(prec + 0.5);
+    nlt = v > 20 ? v : 20;
+    g->numacc = 1e-4;
+    g->k1 = 0.0; g->k2 = INFINITY;
+  }
+  if (g->sigma < 10 * K_ARCSEC) {
+    g->numacc = dmax(g->numacc, 1e-6);
+    if (nlt > 1000) nlt = 1000;
+    if (g->sigma < K_ARCSEC) {
+      if (nlt > 500) nlt = 500;
+      g->numacc = dmax(g->numacc, 1e-5);
+      if (g->sigma < 0.1 * K_ARCSEC) { if (nlt > 200) nlt = 200; g->numacc = dmax(g->numacc, 1e-4); }
+    }
+  }
+  g->nlt = nlt;
+  {
+    double nltm1 = nlt - 1.0;
+    double ltmin = cos_mpi2pi2_t(dmin(K_PIHALF, trunc));
+    double ltinvdelta = nltm1 / (1.0 - ltmin);
+    evcx_t f = { g->norm, g->expfact };
+    double h = 0.01 / ltinvdelta;
+    double minh = dmin(1e-4, (1.0 - ltmin) * 0.1);
+    h = dmax(h, minh);
+    double fpb = ssd4(f_evalcosx, &f, 1.0, -h);
+    double fpa = ssd4(f_evalcosx, &f, ltmin, h);
+    slt_set(&g->lt_evalcosx, f_evalcosx, &f, ltmin, 1.0, fpa, fpb, nlt);
+  }
+  {
+    const double cta = cos_mpi2pi2_t(trunc);
+    sofcosd_t f = { g->norm * 2.506628274631000502415765284811045253006986740609938316629923576 * sigma,
+                    g->expfact, trunc * trunc };
+    double h = 0.01 * (1.0 - cta) / nlt;
+    double minh = dmin(1e-4, (1.0 - cta) * 0.1);
+    h = dmax(h, minh);
+    double fpb = ssd4(f_sofcosd, &f, 1.0, -h);
+    double fpa = f_sofcosd(cta + (1.0 - cta) / nlt, &f) / ((1.0 - cta) / nlt);
+    slt_set(&g->lt_sofcosd, f_sofcosd, &f, cta, 1.0, fpa, fpb, nlt);
+  }
+}
+static double gos_evalcosx_inrange(const gos_t* g, double cosx) {
+  return dmax(0.0, slt_eval(&g->lt_evalcosx, cosx));
+}
+static double gos_evalcosx(const gos_t* g, double cosx) {
+  return cosx >= g->lt_evalcosx.a ? gos_evalcosx_inrange(g, cosx) : 0.0;
+}
+
+/* Romberg with grid-generator evaluation (GOSCircleInt, NCGaussOnSphere.cc:58-140) */
+typedef struct { const gos_t* g; double sasg, cacg, acc; } gci_t;
+static void gci_many(const gci_t* c, double* fv, unsigned n, double off, double delta) {
+  csgrid_t gr; csgrid_init(&gr, n, off, delta);
+  unsigned i = 0;
+  do { fv[i++] = gos_evalcosx_inrange(c->g, c->sasg * gr.c + c->cacg); } while (csgrid_step(&gr));
+}
+static double gci_sum(const gci_t* c, unsigned n, double off, double delta) {
+  csgrid_t gr; csgrid_init(&gr, n, off, delta);
+  double sum = 0.;
+  do { sum += gos_evalcosx_inrange(c->g, c->sasg * gr.c + c->cacg); } while (csgrid_step(&gr));
+  return sum;
+}
+static int gci_accept(const gci_t* c, unsigned level, double prev, double est) {
+  if (fabs(prev - est) <= c->acc * fabs(est)) return 1;
+  if (level < 11) return 0;
+  if (c->acc < 5e-4 && fabs(prev - est) <= 10.0 * c->acc * fabs(est)) return 1;
+  return 1;
+}
+static double gci_integrate(const gci_t* c, double a, double b) {
+  double h = (b - a);
+  double fv[17];
+  gci_many(c, fv, 17, a, h * 0.0625);
+  h *= 0.5;
+  const double R00 = (fv[0] + fv[16]) * h;
+  const double R10 = h * fv[8] + 0.5 * R00;
+  const double R11 = (4. / 3.) * R10 + (-1. / 3.) * R00;
+  h *= 0.5;
+  const double R20 = h * (fv[4] + fv[12]) + 0.5 * R10;
+  const double R21 = (4. / 3.) * R20 + (-1. / 3.) * R10;
+  const double R22 = (16. / 15.) * R21 + (-1. / 15.) * R11;
+  h *= 0.5;
+  const double R30 = h * ((fv[2] + fv[6]) + (fv[10] + fv[14])) + 0.5 * R20;
+  const double R31 = (4. / 3.) * R30 + (-1. / 3.) * R20;
+  const double R32 = (16. / 15.) * R31 + (-1. / 15.) * R21;
+  const double R33 = (64. / 63.) * R32 + (-1. / 63.) * R22;
+  h *= 0.5;
+  const double R40 = h * (((fv[1] + fv[3]) + (fv[5] + fv[7])) + ((fv[9] + fv[11]) + (fv[13] + fv[15]))) + 0.5 * R30;
+  const double R41 = (4. / 3.) * R40 + (-1. / 3.) * R30;
+  const double R42 = (16. / 15.) * R41 + (-1. / 15.) * R31;
+  const double R43 = (64. / 63.) * R42 + (-1. / 63.) * R32;
+  const double R44 = (256. / 255.) * R43 + (-1. / 255.) * R33;
+  if (gci_accept(c, 4, R33, R44)) return R44;
+  const double c5 = gci_sum(c, 16, a + h * 0.5, h);
+  h *= 0.5;
+  const double R50 = h * c5 + 0.5 * R40;
+  const double R51 = (4. / 3.) * R50 + (-1. / 3.) * R40;
+  const double R52 = (16. / 15.) * R51 + (-1. / 15.) * R41;
+  const double R53 = (64. / 63.) * R52 + (-1. / 63.) * R42;
+  const double R54 = (256. / 255.) * R53 + (-1. / 255.) * R43;
+  const double R55 = (1024. / 1023.) * R54 + (-1. / 1023.) * R44;
+  if (gci_accept(c, 5, R44, R55)) return R55;
+  enum { maxlevel = 16 };
+  double cache1[maxlevel], cache2[maxlevel];
+  double *row_prev = cache1, *row = cache2;
+  row_prev[0] = R50; row_prev[1] = R51; row_prev[2] = R52;
+  row_prev[3] = R53; row_prev[4] = R54; row_prev[5] = R55;
+  unsigned nj = 16;
+  for (unsigned i = 6; i < maxlevel; ++i) {
+    double hh = h;
+    h *= 0.5;
+    nj *= 2;
+    double cc = gci_sum(c, nj, a + h, hh);
+    row[0] = h * cc + 0.5 * row_prev[0];
+    double n_k = 1.;
+    for (unsigned j = 0; j < i; ++j) {
+      n_k *= 4.0;
+      row[j + 1] = (n_k * row[j] - row_prev[j]) / (n_k - 1.0);
+    }
+    if (gci_accept(c, i, row_prev[i - 1], row[i])) return row[i];
+    double* tmp = row_prev; row_prev = row; row = tmp;
+  }
+  return row_prev[maxlevel - 1];
+}
+static double gos_circle_slow(const gos_t* g, double cg, double sg, double ca, double sa) {
+  const double sasg = sa * sg;
+  const double cacg = ca * cg;
+  const double cd = cacg + sasg;
+  if (cd <= g->cta) return 0.0;
+  if (sasg < 1e-14) return K_2PI * sa * gos_evalcosx(g, ca);
+  double cos_tmax = (g->cta - cacg) / sasg;
+  double tmax = (cos_tmax <= -1.0 ? K_PI : acos(dmin(1.0, cos_tmax)));
+  if (tmax <= 1e-12) return 0.0;
+  double intacc = g->numacc;
+  if (tmax < 10 * K_ARCSEC) {
+    intacc = dmax(intacc, 1e-6);
+    if (tmax < K_ARCSEC) {
+      intacc = dmax(intacc, 1e-5);
+      if (tmax < 0.1 * K_ARCSEC) intacc = dmax(intacc, 1e-4);
+    }
+  }
+  gci_t c = { g, sasg, cacg, intacc };
+  return 2.0 * sa * gci_integrate(&c, 0, tmax);
+}
+static double gos_circle(const gos_t* g, double cg, double sg, double ca, double sa) {
+  const double sasg = sa * sg;
+  const double cacg = ca * cg;
+  const double cd = cacg + sasg;
+  if (cd > g->cta && sasg >= 1e-14 && g->k2 > g->k1 * sasg + cacg)
+    return slt_eval(&g->lt_sofcosd, cd) * sqrt(sa / sg);
+  return gos_circle_slow(g, cg, sg, ca, sa);
+}
+/* randPointOnUnitCircle (NCRandUtils.cc:98-111) */
+static void rand_point_on_circle(rng_t* rng, double* c, double* s) {
+  double a, b, m2;
+  do {
+    a = -1.0 + rng_gen(rng) * 2.0;
+    b = -1.0 + rng_gen(rng) * 2.0;
+    m2 = a * a + b * b;
+  } while (!in_interval(0.001, 1.0, m2));
+  double m = 1.0 / sqrt(m2);
+  *c = a * m; *s = b * m;
+}
+static int gos_gen_point(const gos_t* g, rng_t* rng, double cg, double sg, double ca, double sa, double* ct,
+                         double* st) {
+  const double sasg = sa * sg;
+  const double cacg = ca * cg;
+  const double cd = cacg + sasg;
+  if (cd <= g->cta) return 0;
+  if (sasg < 1e-14) {
+    if (sa < 1e-7) return 0;
+    rand_point_on_circle(rng, ct, st);
+    return 1;
+  }
+  double cos_tmax = (g->cta - cacg) / sasg;
+  if (cos_tmax >= 1.0) return 0;
+  double tmax = (cos_tmax <= -1.0 ? K_PI : acos(cos_tmax));
+  double densitymax = gos_evalcosx_inrange(g, cd) * 1.00000001;
+  int triesleft = 1001;
+  while (--triesleft) {
+    *ct = cos_mpipi_t(rng_gen(rng) * tmax);
+    double cd_at_t = sasg * (*ct) + cacg;
+    double d = gos_evalcosx_inrange(g, cd_at_t);
+    if (d > densitymax * rng_gen(rng)) break;
+  }
+  if (triesleft <= 0) return 0;
+  *st = sqrt(1.0 - (*ct) * (*ct));
+  *st = (rng_gen(rng) > 0.5 ? *st : -*st);
+  return 1;
+}
+
+typedef struct { double x, y, z; } v3_t;
+static double v3dot(v3_t a, v3_t b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static v3_t v3cross(v3_t a, v3_t o) {
+  v3_t r = { a.y * o.z - a.z * o.y, a.z * o.x - a.x * o.z, a.x * o.y - a.y * o.x };
+  return r;
+}
+static v3_t v3normalise(v3_t v) { /* Vector::normalise, NCVector.hh:196-209 */
+  double m2 = v.x * v.x + v.y * v.y + v.z * v.z;
+  const double lo = 1.0 - 2.0 * DBL_EPSILON, hi = 1.0 + 2.0 * DBL_EPSILON;
+  if (m2 >= lo && m2 <= hi) return v;
+  double ff = 1.0 / sqrt(m2);
+  v.x *= ff; v.y *= ff; v.z *= ff;
+  return v;
+}
+static v3_t phirot(double cp, double sp, v3_t v, v3_t axis) {
+  v3_t ac = v3cross(axis, v);
+  double ad = v3dot(axis, v);
+  v3_t r = { v.x * cp, v.y * cp, v.z * cp };
+  double f = sp;
+  r.x += ac.x * f; r.y += ac.y * f; r.z += ac.z * f;
+  double g2 = ad * (1.0 - cp);
+  r.x += axis.x * g2; r.y += axis.y * g2; r.z += axis.z * g2;
+  return r;
+}
+static v3_t rotate_to_frame(double sinab, double cosab, v3_t a, v3_t b, v3_t v, rng_t* rng) {
+  if (fabs(sinab) < 1e-10) {
+    double bz = b.z;
+    double cp = bz, sp = -sqrt(1.0 - bz * bz);
+    v3_t axis = { b.y, -b.x, 0. };
+    double m2 = axis.x * axis.x + axis.y * axis.y + axis.z * axis.z;
+    if (m2 > 1e-12) {
+      double f = 1.0 / sqrt(m2);
+      axis.x *= f; axis.y *= f; axis.z *= f;
+      v = phirot(cp, sp, v, axis);
+    } else if (b.z < 0.0) {
+      v.z *= -1.0;
+    }
+    double rc, rs;
+    rand_point_on_circle(rng, &rc, &rs);
+    v = phirot(rc, rs, v, b);
+    return v3normalise(v);
+  }
+  double s = 1.0 / sinab;
+  v3_t c1 = b;
+  c1.x *= (-cosab); c1.y *= (-cosab); c1.z *= (-cosab);
+  c1.x += a.x; c1.y += a.y; c1.z += a.z;
+  c1.x *= s; c1.y *= s; c1.z *= s;
+  v3_t c2 = v3cross(b, a);
+  c2.x *= s; c2.y *= s; c2.z *= s;
+  v3_t r = { v.x * c1.x + v.y * c2.x + v.z * b.x, v.x * c1.y + v.y * c2.y + v.z * b.y,
+             v.x * c1.z + v.y * c2.z + v.z * b.z };
+  return v3normalise(r);
+}
+
+typedef struct {
+  int nfam;
+  double* inv2d;  /* [nfam] */
+  double* xsfact; /* [nfam] */
+  int64_t* off;   /* [nfam+1] */
+  v3_t* normals;  /* lab-frame demi-normals */
+  double threshold;
+  gos_t gos;
+} scb_t;
+
+static double gm_round(double x) { return floor(dmax(x, 1e-15) * 1e15 + 0.5) * 1e-15; }
+static double scb_round(double x) { return floor(x * 1e15 + 0.5) * 1e-15; }
+
+/* per-(E,dir) evaluation: walks families/normals; mode 0 = xs only,
+   mode 1 = find the entry with the cumulative index target (second pass) */
+typedef struct {
+  int count; double total;
+  /* selection */
+  int sel_found; v3_t sel_normal; double sel_inv2d;
+} scbacc_t;
+static void scb_walk(const scb_t* sc, double wl, v3_t dir, int mode, int n_select, double target, scbacc_t* acc) {
+  acc->count = 0; acc->total = 0.0; acc->sel_found = 0;
+  const double inv2dcut = (1.0 - 2 * DBL_EPSILON) / wl;
+  const double cta = sc->gos.cta;
+  double commul_last = 0.0;
+  for (int f = 0; f < sc->nfam; ++f) {
+    if (sc->inv2d[f] >= inv2dcut) break;
+    /* InteractionPars::set (NCGaussMos.cc:252-280) */
+    const double xsf = sc->xsfact[f] * 0.5;
+    const double w = gm_round(wl), i2 = gm_round(sc->inv2d[f]);
+    const double wl3 = w * w * w;
+    const double sinpt = w * i2;
+    const double cptsq = 1 - sinpt * sinpt;
+    double Q = -1.0, Qprime = -1.0, cospt = -1.0;
+    const double xsoffset = acc->count ? commul_last : 0.0;
+    double xssum = 0.0;
+    for (int64_t j = sc->off[f]; j < sc->off[f + 1]; ++j) {
+      const v3_t nrm = sc->normals[j];
+      const double dot = v3dot(nrm, dir);
+      const double sdotcptsq = (1.0 - dot * dot) * cptsq;
+      const double ds = dot * sinpt;
+      const double A0 = dmax(0.0, cta - fabs(ds));
+      if (sdotcptsq <= A0 * A0) continue;
+      for (int side = 0; side < 2; ++side) {
+        /* side 0: anti-normal (uses dot), side 1: normal (uses -dot) */
+        const double Aside = side == 0 ? dmax(0.0, cta - ds) : dmax(0.0, cta + ds);
+        if (!(sdotcptsq > Aside * Aside)) continue;
+        double cc = side == 0 ? dot : -dot;
+        /* calcRawCrossSectionValue (NCGaussMos.hh:204-214, NCGaussMos.cc:147-188) */
+        cc = dclamp(cc, -1.0, 1.0);
+        double xs;
+        if (!(Q > 0.)) {
+          if (Qprime == -1) {
+            cospt = sqrt(cptsq);
+            double tmp2 = cospt * sinpt;
+            if (tmp2 > 0) Qprime = wl3 / tmp2;
+            else Qprime = (sinpt > 0.5 && xsf) ? -2.0 : 0.0;
+          }
+          if (Qprime > 0.) Q = Qprime * xsf;
+        }
+        if (Q > 0.) {
+          const double sc2 = sqrt(1.0 - cc * cc);
+          xs = Q * gos_circle(&sc->gos, cc, sc2, sinpt, cospt);
+        } else {
+          xs = Qprime ? INFINITY : 0.0;
+        }
+        if (xs) {
+          const double cm = xsoffset + (xssum += xs);
+          commul_last = cm;
+          if (mode == 1 && !acc->sel_found) {
+            int hit;
+            if (n_select < 5) hit = cm > target;
+            else hit = !(cm < target);
+            if (hit || acc->count == n_select - 1) {
+              acc->sel_found = 1;
+              acc->sel_inv2d = sc->inv2d[f];
+              v3_t s2 = nrm;
+              if (side == 0) { s2.x = -s2.x; s2.y = -s2.y; s2.z = -s2.z; }
+              acc->sel_normal = s2;
+            }
+          }
+          acc->count++;
+        }
+      }
+    }
+  }
+  acc->total = commul_last;
+}
+
+/* SCBragg::crossSection / sampleScatter (NCSCBragg.cc:233-323) */
+static double scb_xs(const scb_t* sc, double E, const double* d) {
+  if (!(E > sc->threshold)) return 0.0;
+  const double ekin = scb_round(E);
+  const double wl = ekin ? sqrt(K_WL2EKIN / ekin) : INFINITY;
+  if (wl == 0) return 0.0;
+  v3_t dir = { d[0], d[1], d[2] };
+  dir = v3normalise(dir);
+  scbacc_t acc;
+  scb_walk(sc, wl, dir, 0, 0, 0.0, &acc);
+  return acc.count ? acc.total : 0.0;
+}
+/* GaussMos::genScat (NCGaussMos.cc:196-250) */
+static void gm_genscat(const scb_t* sc, rng_t* rng, v3_t normal, double inv2d_raw, double wl_raw, v3_t indir,
+                       v3_t* out) {
+  double wl = gm_round(wl_raw);
+  double inv2d = gm_round(inv2d_raw);
+  double stb = wl * inv2d;
+  if (stb == 0.) { *out = indir; return; }
+  double ca = stb;
+  double sa = sqrt(1.0 - ca * ca);
+  double cg = dclamp(-(v3dot(indir, normal)), -1.0, 1.0);
+  double sg = sqrt(1.0 - cg * cg);
+  double ct, st;
+  if (!gos_gen_point(&sc->gos, rng, cg, sg, ca, sa, &ct, &st)) { *out = indir; return; }
+  double s2a = 2 * sa * ca;
+  double c2a = ca * ca - sa * sa;
+  v3_t v = { s2a * ct, s2a * st, c2a };
+  v3_t mb = { -indir.x, -indir.y, -indir.z };
+  *out = rotate_to_frame(sg, cg, normal, mb, v, rng);
+}
+static void scb_sample(const scb_t* sc, rng_t* rng, double E, const double* d, double* od) {
+  od[0] = d[0]; od[1] = d[1]; od[2] = d[2];
+  if (!(E > sc->threshold)) return;
+  const double ekin = scb_round(E);
+  const double wl = ekin ? sqrt(K_WL2EKIN / ekin) : INFINITY;
+  if (wl == 0) return;
+  v3_t dir = { d[0], d[1], d[2] };
+  dir = v3normalise(dir);
+  scbacc_t acc;
+  scb_walk(sc, wl, dir, 0, 0, 0.0, &acc);
+  if (!acc.count || !(acc.total > 0.0)) return;
+  double target = 0.0;
+  if (acc.count > 1) target = acc.total * rng_gen(rng);
+  scbacc_t sel;
+  scb_walk(sc, wl, dir, 1, acc.count, target, &sel);
+  v3_t o;
+  gm_genscat(sc, rng, sel.sel_normal, sel.sel_inv2d, wl, dir, &o);
+  od[0] = o.x; od[1] = o.y; od[2] = o.z;
+}
+
+/* setup: families from (d, |F|^2, explicit hkl) in hkl-list order
+   (NCSCBragg.cc:125-231, NCPlaneProvider.cc:111-175) */
+typedef struct { uint64_t kd, kf; int64_t src; } scbkey_t;
+static int cmp_scbkey_desc(const void* a, const void* b) {
+  const scbkey_t* x = (const scbkey_t*)a; const scbkey_t* y = (const scbkey_t*)b;
+  if (x->kd != y->kd) return x->kd > y->kd ? -1 : 1;
+  if (x->kf != y->kf) return x->kf > y->kf ? -1 : 1;
+  return x->src < y->src ? -1 : (x->src > y->src ? 1 : 0); /* insertion order within a key */
+}
+typedef struct { uint64_t key; double val; int64_t src; } origv_t;
+static int cmp_origv(const void* a, const void* b) {
+  const origv_t* x = (const origv_t*)a; const origv_t* y = (const origv_t*)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->src < y->src ? -1 : (x->src > y->src ? 1 : 0);
+}
+/* sort by (key, insertion order) and collapse to one entry per key */
+static void origv_reduce(origv_t* v, int64_t* n) {
+  qsort(v, (size_t)*n, sizeof(origv_t), cmp_origv);
+  int64_t m = 0;
+  for (int64_t i = 0; i < *n;) {
+    int64_t j = i;
+    double first = v[i].val, val = first;
+    while (j < *n && v[j].key == v[i].key) {
+      if (val > 0 && fabs(v[j].val - first) > 1e-12) val = -1;
+      ++j;
+    }
+    v[m].key = v[i].key; v[m].val = val; ++m;
+    i = j;
+  }
+  *n = m;
+}
+static double origv_find(const origv_t* v, int64_t n, uint64_t key) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) { int64_t mid = (lo + hi) / 2; if (v[mid].key < key) lo = mid + 1; else hi = mid; }
+  return (lo < n && v[lo].key == key) ? v[lo].val : -1.0;
+}
+
+typedef struct { int dummy; } scb_dummy_t;
+
+orc_proc* orc_create_scbragg(double a, double b, double c, double v0n, int nfam, const double* dsp,
+                             const double* fsq, const int64_t* mem_off, const int* mem_hkl,
+                             const double* cry2lab, double mos_fwhm, double prec);
+int orc_scb_info(const orc_proc* p, int* nfam, int64_t* nnormals, double* threshold, double* gosdbl);
+void orc_scb_tables(const orc_proc* p, double* inv2d, double* xsfact, int64_t* off, double* normals,
+                    double* lt1, double* lt2);
+
+struct scb_s { scb_t s; };
+static double scb_xs_p(const orc_proc* p, double E, const double* d) { return scb_xs(&p->u.scb->s, E, d); }
+static void scb_sample_p(const orc_proc* p, rng_t* rng, double E, const double* d, double* od) {
+  scb_sample(&p->u.scb->s, rng, E, d, od);
+}
+static void scb_free(orc_proc* p) {
+  scb_t* sc = &p->u.scb->s;
+  free(sc->inv2d); free(sc->xsfact); free(sc->off); free(sc->normals);
+  free(sc->gos.lt_evalcosx.y); free(sc->gos.lt_evalcosx.y2);
+  free(sc->gos.lt_sofcosd.y); free(sc->gos.lt_sofcosd.y2);
+  free(p->u.scb);
+}
+
+orc_proc* orc_create_scbragg(double a, double b, double c, double v0n, int nfam, const double* dsp,
+                             const double* fsq, const int64_t* mem_off, const int* mem_hkl,
+                             const double* R, double mos_fwhm, double prec) {
+  const double two30 = 1073741824.0;
+  const double rl0 = K_2PI / a, rl1 = K_2PI / b, rl2 = K_2PI / c;
+  const int64_t nplanes = mem_off[nfam];
+  scbkey_t* keys = (scbkey_t*)malloc(sizeof(scbkey_t) * (size_t)(nplanes > 0 ? nplanes : 1));
+  v3_t* pn = (v3_t*)malloc(sizeof(v3_t) * (size_t)(nplanes > 0 ? nplanes : 1));
+  origv_t* od = (origv_t*)malloc(sizeof(origv_t) * (size_t)(nplanes > 0 ? nplanes : 1));
+  origv_t* of = (origv_t*)malloc(sizeof(origv_t) * (size_t)(nplanes > 0 ? nplanes : 1));
+  int64_t nod = 0, nof = 0;
+  double maxd = 0.0;
+  int64_t q = 0;
+  for (int f = 0; f < nfam; ++f) {
+    for (int64_t m = mem_off[f]; m < mem_off[f + 1]; ++m, ++q) {
+      const double h = mem_hkl[3 * m], k = mem_hkl[3 * m + 1], l = mem_hkl[3 * m + 2];
+      v3_t v = { rl0 * h + 0.0 * k + 0.0 * l, 0.0 * h + rl1 * k + 0.0 * l, 0.0 * h + 0.0 * k + rl2 * l };
+      pn[q] = v3normalise(v);
+      if (dsp[f] > maxd) maxd = dsp[f];
+      const uint64_t kd = (uint64_t)(dsp[f] * two30 + 0.5), kf = (uint64_t)(fsq[f] * two30 + 0.5);
+      keys[q].kd = kd; keys[q].kf = kf; keys[q].src = q;
+      /* origvals (NCSCBragg.cc:164-176): per integer key the first value
+         seen, or -1 if another value further than 1e-12 away was seen */
+      od[nod].key = kd; od[nod].val = dsp[f]; od[nod].src = nod; ++nod;
+      of[nof].key = kf; of[nof].val = fsq[f]; of[nof].src = nof; ++nof;
+    }
+  }
+  origv_reduce(od, &nod);
+  origv_reduce(of, &nof);
+  qsort(keys, (size_t)nplanes, sizeof(scbkey_t), cmp_scbkey_desc);
+  orc_proc* p = (orc_proc*)calloc(1, sizeof(orc_proc));
+  p->type = P_SCB;
+  p->u.scb = (struct scb_s*)calloc(1, sizeof(struct scb_s));
+  scb_t* sc = &p->u.scb->s;
+  sc->inv2d = (double*)malloc(sizeof(double) * (size_t)(nplanes + 1));
+  sc->xsfact = (double*)malloc(sizeof(double) * (size_t)(nplanes + 1));
+  sc->off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nplanes + 2));
+  sc->normals = (v3_t*)malloc(sizeof(v3_t) * (size_t)(nplanes > 0 ? nplanes : 1));
+  int nf = 0;
+  for (int64_t i = 0; i < nplanes; ++i) {
+    if (i == 0 || keys[i].kd != keys[i - 1].kd || keys[i].kf != keys[i - 1].kf) {
+      const double dd = origv_find(od, nod, keys[i].kd);
+      const double ff = origv_find(of, nof, keys[i].kf);
+      const double dspv = dd > 0 ? dd : keys[i].kd / two30;
+      const double fsqv = ff > 0 ? ff : keys[i].kf / two30;
+      sc->xsfact[nf] = fsqv / v0n;
+      sc->inv2d[nf] = 0.5 / dspv;
+      sc->off[nf] = i;
+      ++nf;
+    }
+    const v3_t n = pn[keys[i].src];
+    v3_t r = { R[0] * n.x + R[1] * n.y + R[2] * n.z, R[3] * n.x + R[4] * n.y + R[5] * n.z,
+               R[6] * n.x + R[7] * n.y + R[8] * n.z };
+    sc->normals[i] = r;
+  }
+  sc->off[nf] = nplanes;
+  sc->nfam = nf;
+  sc->threshold = wl2ekin(maxd * 2.0);
+  free(keys); free(pn); free(od); free(of);
+  /* GaussMos (NCGaussMos.cc:38-75), estimateNTruncFromPrec (NCGaussOnSphere.cc:510-547) */
+  double ntrunc;
+  if (prec == 0) ntrunc = 8.0;
+  else if (prec >= 1.0) ntrunc = 3.0;
+  else ntrunc = dclamp(1.1 * sqrt(-2.0 * log(dmax(1e-300, prec))), 3.0, 8.0);
+  const double sigma = mos_fwhm * K_FWHM2SIGMA;
+  const double trunc = ntrunc * sigma;
+  if (!(trunc < K_PIHALF)) { scb_free(p); free(p); return NULL; }
+  gos_set(&sc->gos, sigma, trunc, prec);
+  return p;
+}
+int orc_scb_info(const orc_proc* p, int* nfam, int64_t* nnormals, double* threshold, double* gd) {
+  const scb_t* sc = &p->u.scb->s;
+  *nfam = sc->nfam; *nnormals = sc->off[sc->nfam]; *threshold = sc->threshold;
+  const gos_t* g = &sc->gos;
+  gd[0] = g->sigma; gd[1] = g->truncangle; gd[2] = g->prec; gd[3] = g->cta; gd[4] = g->sta;
+  gd[5] = g->expfact; gd[6] = g->norm; gd[7] = g->k1; gd[8] = g->k2; gd[9] = g->numacc; gd[10] = g->nlt;
+  gd[11] = g->lt_evalcosx.a; gd[12] = g->lt_evalcosx.invdelta; gd[13] = g->lt_sofcosd.a; gd[14] = g->lt_sofcosd.invdelta;
+  return 0;
+}
+void orc_scb_tables(const orc_proc* p, double* inv2d, double* xsfact, int64_t* off, double* normals, double* lt1,
+                    double* lt2) {
+  const scb_t* sc = &p->u.scb->s;
+  memcpy(inv2d, sc->inv2d, sizeof(double) * (size_t)sc->nfam);
+  memcpy(xsfact, sc->xsfact, sizeof(double) * (size_t)sc->nfam);
+  memcpy(off, sc->off, sizeof(int64_t) * (size_t)(sc->nfam + 1));
+  memcpy(normals, sc->normals, sizeof(double) * 3 * (size_t)sc->off[sc->nfam]);
+  const gos_t* g = &sc->gos;
+  for (int i = 0; i < g->lt_evalcosx.n; ++i) { lt1[2 * i] = g->lt_evalcosx.y[i]; lt1[2 * i + 1] = g->lt_evalcosx.y2[i]; }
+  for (int i = 0; i < g->lt_sofcosd.n; ++i) { lt2[2 * i] = g->lt_sofcosd.y[i]; lt2[2 * i + 1] = g->lt_sofcosd.y2[i]; }
 }

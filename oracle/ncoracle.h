@@ -44,6 +44,15 @@ int orc_hkl_orthogonal(double a, double b, double c,
                        double dcutoff, double fsquarecut, double merge_tolerance,
                        int maxout, int* out_hkl, int* out_mult,
                        double* out_dsp, double* out_fsq);
+/* same, plus the explicit hkl list of every family (members sorted like the
+   reference, first = representative): out_mem_off[nfam+1], out_mem_hkl[3*nmem] */
+int orc_hkl_orthogonal_members(double a, double b, double c,
+                               int nspecies, const int* natoms, const double* positions,
+                               const double* csl, const double* msd,
+                               double dcutoff, double fsquarecut, double merge_tolerance,
+                               int maxout, int* out_hkl, int* out_mult,
+                               double* out_dsp, double* out_fsq,
+                               int64_t* out_mem_off, int* out_mem_hkl, int64_t maxmem);
 
 /* ---- processes ---- */
 typedef struct orc_proc orc_proc;
@@ -92,6 +101,18 @@ int64_t orc_sample_dir_many(const orc_proc*, uint64_t seed, uint32_t stream,
                             uint64_t idx_offset, const double* ekin,
                             const double* dirs, int64_t n, double* out_ekin,
                             double* out_dirs, int nthreads);
+
+/* SCBragg: families (d, |F|^2 and explicit hkl members, in hkl-list order) of
+   a cell with 90-degree angles; cry2lab row-major 3x3; mosaicity FWHM [rad];
+   prec as NCrystal's mosprec */
+orc_proc* orc_create_scbragg(double a, double b, double c, double v0_times_natoms, int nfam,
+                             const double* dsp, const double* fsq, const int64_t* mem_off,
+                             const int* mem_hkl, const double* cry2lab, double mos_fwhm, double prec);
+int orc_scb_info(const orc_proc*, int* nfam, int64_t* nnormals, double* threshold, double* gos15);
+void orc_scb_tables(const orc_proc*, double* inv2d, double* xsfact, int64_t* off, double* normals,
+                    double* lt_evalcosx /*2*nlt*/, double* lt_sofcosd /*2*nlt*/);
+void orc_xs_dir_many(const orc_proc*, const double* ekin, const double* dirs, int64_t n, double* out,
+                     int nthreads);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,7 @@ _LIBPATH = _HERE / "build" / "libncoracle.so"
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+_i64p = ctypes.POINTER(ctypes.c_int64)
 _vp = ctypes.c_void_p
 
 
@@ -56,6 +57,17 @@ def lib():
                                                                   ctypes.c_double, ctypes.c_int, _ip, _ip,
                                                                   _dp, _dp]
         L.orc_hkl_orthogonal.restype = ctypes.c_int
+        L.orc_hkl_orthogonal_members.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _ip, _dp, _dp, _dp,
+                                                                          ctypes.c_double, ctypes.c_double,
+                                                                          ctypes.c_double, ctypes.c_int, _ip, _ip,
+                                                                          _dp, _dp, _i64p, _ip, ctypes.c_int64]
+        L.orc_hkl_orthogonal_members.restype = ctypes.c_int
+        L.orc_create_scbragg.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int, _dp, _dp, _i64p, _ip, _dp,
+                                                                  ctypes.c_double, ctypes.c_double]
+        L.orc_create_scbragg.restype = _vp
+        L.orc_scb_info.argtypes = [_vp, _ip, _i64p, _dp, _dp]
+        L.orc_scb_tables.argtypes = [_vp, _dp, _dp, _i64p, _dp, _dp, _dp]
+        L.orc_xs_dir_many.argtypes = [_vp, _dp, _dp, ctypes.c_int64, _dp, ctypes.c_int]
         L.orc_create_powderbragg.argtypes = [ctypes.c_double, _dp, _dp, ctypes.c_int]
         L.orc_create_powderbragg.restype = _vp
         L.orc_pb_ntables.argtypes = [_vp]
@@ -246,3 +258,63 @@ def composition(procs, scales):
     sc = np.ascontiguousarray(scales, dtype=np.float64)
     p = Proc(lib().orc_create_composition(len(procs), arr, _d(sc)), keep=tuple(procs))
     return p
+
+
+def hkl_orthogonal_members(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tolerance=1e-6, maxout=2000000,
+                           maxmem=8000000):
+    natoms = np.array([len(s[2]) for s in species], dtype=np.int32)
+    pos = np.ascontiguousarray(np.concatenate([np.asarray(s[2], dtype=np.float64).reshape(-1, 3)
+                                               for s in species]).ravel())
+    csl = np.array([s[0] for s in species], dtype=np.float64)
+    msd = np.array([s[1] for s in species], dtype=np.float64)
+    hkl = np.empty(3 * maxout, dtype=np.int32)
+    mult = np.empty(maxout, dtype=np.int32)
+    dsp = np.empty(maxout)
+    fsq = np.empty(maxout)
+    moff = np.empty(maxout + 1, dtype=np.int64)
+    mhkl = np.empty(3 * maxmem, dtype=np.int32)
+    n = lib().orc_hkl_orthogonal_members(a, b, c, len(species), _i(natoms), _d(pos), _d(csl), _d(msd), dcutoff,
+                                         fsquarecut, merge_tolerance, maxout, _i(hkl), _i(mult), _d(dsp), _d(fsq),
+                                         moff.ctypes.data_as(_i64p), _i(mhkl), maxmem)
+    if n < 0:
+        raise RuntimeError(f"oracle hkl enumeration failed ({n})")
+    nm = int(moff[n])
+    return (hkl[:3 * n].reshape(n, 3).copy(), mult[:n].copy(), dsp[:n].copy(), fsq[:n].copy(),
+            moff[:n + 1].copy(), mhkl[:3 * nm].reshape(nm, 3).copy())
+
+
+class SCB(Proc):
+    def xs_dir(self, ekin, dirs, nthreads=1):
+        e = np.ascontiguousarray(np.atleast_1d(np.asarray(ekin, dtype=np.float64)))
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dirs, dtype=np.float64), (e.size, 3)))
+        out = np.empty(e.size)
+        lib().orc_xs_dir_many(self.ptr, _d(e), _d(d), e.size, _d(out), nthreads)
+        return out
+
+    def info(self):
+        nf, nn, thr = ctypes.c_int(), ctypes.c_int64(), ctypes.c_double()
+        gd = np.empty(15)
+        lib().orc_scb_info(self.ptr, ctypes.byref(nf), ctypes.byref(nn), ctypes.byref(thr), _d(gd))
+        return dict(nfam=nf.value, nnormals=nn.value, threshold=thr.value, gos=gd)
+
+    def tables(self):
+        inf = self.info()
+        nf, nn, nlt = inf["nfam"], inf["nnormals"], int(inf["gos"][10])
+        inv2d, xsf = np.empty(nf), np.empty(nf)
+        off = np.empty(nf + 1, dtype=np.int64)
+        nrm = np.empty((nn, 3))
+        lt1, lt2 = np.empty(2 * nlt), np.empty(2 * nlt)
+        lib().orc_scb_tables(self.ptr, _d(inv2d), _d(xsf), off.ctypes.data_as(_i64p), _d(nrm), _d(lt1), _d(lt2))
+        return dict(inv2d=inv2d, xsfact=xsf, off=off, normals=nrm, lt_evalcosx=lt1.reshape(-1, 2),
+                    lt_sofcosd=lt2.reshape(-1, 2))
+
+
+def scbragg(a, b, c, v0_times_natoms, dsp, fsq, mem_off, mem_hkl, cry2lab, mos_fwhm, prec=1e-3):
+    d = np.ascontiguousarray(dsp, dtype=np.float64)
+    f = np.ascontiguousarray(fsq, dtype=np.float64)
+    mo = np.ascontiguousarray(mem_off, dtype=np.int64)
+    mh = np.ascontiguousarray(mem_hkl, dtype=np.int32).ravel()
+    R = np.ascontiguousarray(cry2lab, dtype=np.float64).ravel()
+    ptr = lib().orc_create_scbragg(a, b, c, v0_times_natoms, d.size, _d(d), _d(f), mo.ctypes.data_as(_i64p),
+                                   _i(mh), _d(R), mos_fwhm, prec)
+    return SCB(ptr)

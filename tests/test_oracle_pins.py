@@ -320,3 +320,110 @@ def test_sab_highe_extender_runs(oracle):
     oe, om, nerr = s.sample_iso(e, seed=5)
     assert nerr == 0
     assert np.all(np.abs(om) <= 1.0) and np.all(oe >= 0)
+
+
+# ---------------------------------------------------------------- SCBragg --
+GE_A = 5.65735
+GE_D220 = GE_A / math.sqrt(8.0)
+GE_220_DEMI = np.array([[2, 2, 0], [2, -2, 0], [2, 0, 2], [2, 0, -2], [0, 2, 2], [0, 2, -2]], dtype=np.int32)
+
+
+def _ge220(oracle, fsq, mos, prec=1e-3, R=None):
+    v0n = GE_A ** 3 * 8
+    return oracle.scbragg(GE_A, GE_A, GE_A, v0n, [GE_D220], [fsq], [0, 6], GE_220_DEMI,
+                          np.eye(3) if R is None else R, mos, prec)
+
+
+def test_scbragg_ge220_reference_log(oracle):
+    # tests/src/app_backscat/main.cc + test.log: Ge (0 2 -2) family only, mosaicity 1 deg,
+    # identity orientation, isotropic average of the single-crystal xs over 1e5 directions
+    # ("sgl_cry") and the powder xs ("pc").  |F|^2*mult is fixed by the log's powder value.
+    log = [(1.91137, 1.91584), (4.30058, 4.22342), (4.39668, 4.38903), (4.49384, 4.52423),
+           (4.59206, 4.53223), (4.69135, 4.66975), (4.79169, 4.91714), (4.8931, 4.97866),
+           (4.99557, 5.00076), (5.0991, 5.05824), (5.2037, 5.10551), (5.30925, 4.51393), (0.0, 0.0)]
+    thr = 2 * GE_D220
+    wls = [thr * 0.6] + list(np.linspace(thr * 0.9, thr * 0.99, 10)) + [thr * 0.99999, thr * 1.00001]
+    v0n = GE_A ** 3 * 8
+    fm = log[0][0] * 2 * v0n / (wls[0] ** 2 * GE_D220)
+    sc = _ge220(oracle, fm / 12, math.pi / 180)
+    n = 400000
+    dirs = synth.isotropic_dirs(n, seed=11)
+    for wl, (pc, sgl) in zip(wls, log):
+        if pc:  # the powder value scales as wl^2 for one family
+            assert fm * wl * wl * GE_D220 / (2 * v0n) == pytest.approx(pc, rel=5e-6)
+        x = sc.xs_dir(np.full(n, WL2EKIN / wl ** 2), dirs, nthreads=8)
+        if not sgl:
+            assert not x.any()
+            continue
+        s = x.std()
+        pull = (x.mean() - sgl) / math.sqrt(s * s / 1e5 + s * s / n)
+        assert abs(pull) < 4.0, (wl, x.mean(), sgl)
+
+
+def test_scbragg_rocking_curve_closed_form(oracle):
+    # one demi-normal in reflection: xs(delta) = Q/(sigma sqrt(2pi)) exp(-delta^2/2sigma^2)
+    # with Q = (|F|^2/(V0 n)) * wl^3 / (2 sin(thB) cos(thB)), delta the rocking offset
+    v0n = GE_A ** 3 * 8
+    fsq = 40.0
+    n = np.array([1.0, 1.0, 0.0]) / math.sqrt(2)
+    t = np.array([1.0, -1.0, 0.0]) / math.sqrt(2)
+    for mos, prec in [(math.pi / 180, 1e-3), (0.1 * math.pi / 180, 1e-4), (20 / 60 * math.pi / 180, 1e-2)]:
+        sc = _ge220(oracle, fsq, mos, prec)
+        sig = mos * 0.42466090014400952
+        for wl in (1.0, 2.0, 3.0):
+            tb = math.asin(wl / (2 * GE_D220))
+            ds = np.linspace(-3.5, 3.5, 15) * sig
+            dirs = np.array([-math.sin(tb + x) * n + math.cos(tb + x) * t for x in ds])
+            x = sc.xs_dir(np.full(len(ds), WL2EKIN / wl ** 2), dirs)
+            Q = fsq / v0n * wl ** 3 / (2 * math.sin(tb) * math.cos(tb))
+            assert x[7] == pytest.approx(Q / (sig * math.sqrt(2 * math.pi)), rel=4e-3)
+            np.testing.assert_allclose(x / x[7], np.exp(-ds ** 2 / (2 * sig ** 2)), atol=3e-3)
+            # far off the Bragg condition (beyond the truncation) the normal does not reflect
+            one = oracle.scbragg(GE_A, GE_A, GE_A, v0n, [GE_D220], [fsq], [0, 1], GE_220_DEMI[:1],
+                                 np.eye(3), mos, prec)
+            far = -math.sin(tb + 20 * sig) * n + math.cos(tb + 20 * sig) * t
+            assert one.xs_dir([WL2EKIN / wl ** 2], far)[0] == 0.0
+            assert one.xs_dir([WL2EKIN / wl ** 2], dirs[7])[0] == pytest.approx(x[7], rel=1e-12)
+
+
+def test_scbragg_isotropic_average_is_powder(oracle):
+    # orientation-averaged mosaic crystal = powder (same families, |F|^2 and multiplicities)
+    cr = synth.fcc_powder(dcutoff=0.2)
+    msd = oracle.debye_msd(410.0, 293.0, 26.98)
+    sp = [(cr.species[0].csl, msd, cr.species[0].positions)]
+    hkl, mult, dsp, fsq, moff, mh = oracle.hkl_orthogonal_members(cr.a, cr.b, cr.c, sp, cr.dcutoff)
+    assert np.array_equal(np.diff(moff) * 2, mult)  # members are demi-normals
+    v0n = cr.volume * cr.n_atoms
+    th = 0.3
+    R = np.array([[1, 0, 0], [0, math.cos(th), -math.sin(th)], [0, math.sin(th), math.cos(th)]])
+    sc = oracle.scbragg(cr.a, cr.b, cr.c, v0n, dsp, fsq, moff, mh, R, 30 * math.pi / (180 * 60))
+    pb = oracle.powderbragg(v0n, dsp, fsq * mult)
+    n = 200000
+    dirs = synth.isotropic_dirs(n, seed=3)
+    for wl in (1.0, 2.5, 4.0):
+        e = WL2EKIN / wl ** 2
+        x = sc.xs_dir(np.full(n, e), dirs, nthreads=8)
+        assert abs(x.mean() - pb.xs([e])[0]) < 4 * x.std() / math.sqrt(n) + 2e-3 * pb.xs([e])[0]
+
+
+def test_scbragg_sampling_is_bragg_elastic(oracle):
+    cr = synth.fcc_powder(dcutoff=0.3)
+    msd = oracle.debye_msd(410.0, 293.0, 26.98)
+    sp = [(cr.species[0].csl, msd, cr.species[0].positions)]
+    hkl, mult, dsp, fsq, moff, mh = oracle.hkl_orthogonal_members(cr.a, cr.b, cr.c, sp, cr.dcutoff)
+    v0n = cr.volume * cr.n_atoms
+    sc = oracle.scbragg(cr.a, cr.b, cr.c, v0n, dsp, fsq, moff, mh, np.eye(3), 30 * math.pi / (180 * 60))
+    n = 100000
+    wl = 2.5
+    dirs = synth.isotropic_dirs(n, seed=5)
+    e = np.full(n, WL2EKIN / wl ** 2)
+    x = sc.xs_dir(e, dirs, nthreads=8)
+    oe, od, nerr = sc.sample_dir(e, dirs, seed=7, nthreads=8)
+    assert nerr == 0 and np.array_equal(oe, e)
+    hit = x > 0
+    assert 0.05 < hit.mean() < 0.95
+    assert np.array_equal(od[~hit], dirs[~hit])
+    assert np.abs(np.linalg.norm(od, axis=1) - 1).max() < 1e-14
+    mu = np.sum(od * dirs, axis=1)[hit]
+    cand = 1 - wl ** 2 / (2 * dsp[dsp > wl / 2] ** 2)  # cos(2 thB) of the reachable families
+    assert np.min(np.abs(mu[:, None] - cand[None, :]), axis=1).max() < 1e-12
