@@ -1,14 +1,20 @@
 """Benchmark: S(alpha,beta) scatter sampling throughput on one MI355X.
 
-Workload (one "step" = one pass of the sampler over one batch):
+Default workload (one "step" = one pass of the sampler over one batch):
   S(alpha,beta) table 200 (alpha) x 800 (beta) FP64 from a three-Gaussian
   VDOS (centres 0.01-0.12 eV, widths 0.003-0.02 eV, seeded) on 500 points up
   to 0.15 eV, T = 293 K; batch of 1e8 neutrons with E log-uniform in
   [1e-3, 1] eV resident in HBM; outputs E' and mu (FP64) in HBM.
 
-Prints one JSON line (metric, value = samples/s, roofline of the sampling
-kernel from HIP events on the launch stream, CPU baseline of the CPU checker
-timed on a bounded sample on this host).
+--workload scbragg: mosaic single crystal, the seeded orthorhombic cell
+  (a,b,c ~ U(5,15) Aa, 3 species x 8 atoms), dcutoff 0.2 Aa, mosaicity FWHM
+  30 arcmin, seeded random orientation; 1e7 neutrons with wavelength
+  ~ U(1,6) Aa and directions in a 2-degree cone about a seeded axis; one
+  step = cross sections + scatter sampling of the whole batch.
+
+Prints one JSON line (metric, value, roofline of the dominant kernel from
+HIP events on the launch stream, CPU baseline of the CPU checker timed on a
+bounded sample on this host).
 """
 import argparse
 import json
@@ -37,6 +43,7 @@ def parse():
                     help="also run one device-to-device copy of the output size (counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["sab", "scbragg"], default="sab")
     return ap.parse_args()
 
 
@@ -67,11 +74,127 @@ def cpu_baseline(table, batch, seconds, threads):
                        f"(E log-uniform 1e-3..1 eV, same table), {threads} threads, {dt:.1f} s")
 
 
+FP64_PEAK_TFLOPS = 78.6
+SCB_FLOPS_PER_NORMAL = 11  # dot (5) + (1-dot^2)*cos^2 (3) + dot*sin (1) + cta-|ds| (1) + square (1)
+
+
+def scb_setup(n):
+    import numpy as np
+    from ncrystal_amd import core, synth
+    cr = synth.ortho_random(seed=1, dcutoff=0.2)
+    sp = [(s.csl, core.debye_msd(s.debye_temp, cr.temperature, s.mass), s.positions) for s in cr.species]
+    hkl, mult, dsp, fsq, moff, mh = core.hkl_orthogonal_members(cr.a, cr.b, cr.c, sp, cr.dcutoff)
+    R = synth.random_rotation(4)
+    args = (cr.volume * cr.n_atoms, dsp, fsq, moff, mh, core.reci_lattice_orthogonal(cr.a, cr.b, cr.c), R,
+            30 * math.pi / (180 * 60))
+    wl = synth.wavelengths_uniform(n, 1.0, 6.0, seed=6)
+    e = synth.WL2EKIN / (wl * wl)
+    dirs = synth.cone_dirs(n, synth.random_unit_vector(5), 2 * math.pi / 180, seed=3)
+    return cr, args, e, np.ascontiguousarray(dirs)
+
+
+def cpu_baseline_scb(args_scb, e, dirs, seconds, threads):
+    """CPU checker timed on a bounded sample of the single-crystal workload:
+    cross section + sampling of the first neutrons of the batch, repeated
+    for about `seconds`."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    pyoracle.build()
+    v0n, dsp, fsq, moff, mh, L, R, mos = args_scb
+    o = pyoracle.scbragg(0, 0, 0, v0n, dsp, fsq, moff, mh, R, mos, reci=L)
+    batch = 20000
+    eb, db = e[:batch], dirs[:batch]
+    done, k = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        o.xs_dir(eb, db, nthreads=threads)
+        o.sample_dir(eb, db, seed=5, stream=k, nthreads=threads)
+        done += batch
+        k += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    return dict(value=done / dt, unit="neutrons/s (xs + sample)", cores=threads, kind="port",
+                sample=f"{done} neutrons ({k} passes over the first {batch} neutrons of the batch, "
+                       f"cross section + sampling), {threads} threads, {dt:.1f} s")
+
+
+def run_scbragg(args):
+    import numpy as np
+    from ncrystal_amd import _capi as C
+    from ncrystal_amd import core
+    n = args.n if args.n != 100_000_000 else 10_000_000
+    cr, a, e, dirs = scb_setup(n)
+    sc = core.create_scbragg(*a)
+    t = sc.tables()
+    # demi-normals with 2d > wl scanned per (E, dir) evaluation
+    wl = np.sqrt(0.081804209605330899 / e)
+    nf = np.searchsorted(t["inv2d"], (1.0 - 2 * 2.220446049250313e-16) / wl, side="left")
+    scanned = float(t["off"][nf].sum())
+    stream = C.Stream()
+    de, dd = C.DeviceArray.from_numpy(e), C.DeviceArray.from_numpy(dirs.ravel())
+    xs, oe, od = C.DeviceArray(n), C.DeviceArray(n), C.DeviceArray(3 * n)
+    sc.setRandomSeed(77)
+    ev = [C.Event() for _ in range(3)]
+
+    def step():
+        sc.dev_crossSection(de.ptr, dd.ptr, n, xs.ptr, stream=stream.ptr)
+        sc.dev_sampleScatter(de.ptr, dd.ptr, n, oe.ptr, od.ptr, stream=stream.ptr)
+
+    for _ in range(args.warmup):
+        step()
+    stream.sync()
+    xs_ms = samp_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ev[0].record(stream)
+        sc.dev_crossSection(de.ptr, dd.ptr, n, xs.ptr, stream=stream.ptr)
+        ev[1].record(stream)
+        sc.dev_sampleScatter(de.ptr, dd.ptr, n, oe.ptr, od.ptr, stream=stream.ptr)
+        ev[2].record(stream)
+        stream.sync()
+        xs_ms += ev[0].elapsed_ms(ev[1])
+        samp_ms += ev[1].elapsed_ms(ev[2])
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    xs_ms /= args.steps
+    samp_ms /= args.steps
+    x = xs.numpy()
+    o = od.numpy().reshape(n, 3)
+    assert np.all(x >= 0.0) and np.abs(np.linalg.norm(o[:100000], axis=1) - 1).max() < 1e-12, "invalid output"
+    flops = SCB_FLOPS_PER_NORMAL * scanned
+    achieved = flops / (xs_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline_scb(a, e, dirs, args.cpu_seconds, threads)
+    out = {
+        "metric": "SCBragg oriented neutrons/s on 1 MI355X (cross section + scatter sampling)",
+        "value": n * args.steps / wall,
+        "unit": "neutrons/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "SCBragg, orthorhombic cell seed 1 (a,b,c=%.3f,%.3f,%.3f Aa, 24 atoms), dcutoff 0.2 Aa, "
+                               "mosaicity 30 arcmin, random orientation, wavelength U(1,6) Aa, 2-degree cone"
+                               % (cr.a, cr.b, cr.c),
+                   "neutrons_per_step": n, "families": int(len(t["inv2d"])),
+                   "demi_normals": int(t["off"][-1]), "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_scb_xs_main",
+                     "kernel_ms": xs_ms, "sample_ms": samp_ms,
+                     "flops_per_launch": flops, "normals_scanned_per_neutron": scanned / n},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
+    if args.workload == "scbragg":
+        return run_scbragg(args)
     import numpy as np
     from ncrystal_amd import _capi as C
     from ncrystal_amd import core, synth

@@ -62,6 +62,20 @@ int64_t ncamd_hkl_orthogonal(double a, double b, double c, int nspecies, const i
                              int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp,
                              double* out_fsq, int64_t* nrequired);
 
+/* Same, plus the explicit hkl list of every family (the members that
+ * NCrystal's HKLInfo carries for ExplicitHKLs lists and that the standard
+ * plane provider expands, ncrystal_core/src/extd_utils/NCPlaneProvider.cc:111-175):
+ * family i owns members [out_mem_off[i], out_mem_off[i+1]) of out_mem_hkl
+ * (3 ints each, one per +-pair, sorted so that the first is the
+ * representative hkl).  Returns -1 if maxout or maxmem is too small
+ * (*nrequired / *nmem_required say how much is needed). */
+int64_t ncamd_hkl_orthogonal_members(double a, double b, double c, int nspecies, const int* natoms,
+                                     const double* positions, const double* csl, const double* msd,
+                                     double dcutoff, double fsquarecut, double merge_tolerance,
+                                     int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp,
+                                     double* out_fsq, int64_t* nrequired, int64_t maxmem,
+                                     int64_t* out_mem_off, int* out_mem_hkl, int64_t* nmem_required);
+
 /* Debye-model isotropic mean-squared displacement [Aa^2]
  * (ncrystal_core/src/phys_utils/NCDebyeMSD.cc:27-36). */
 double ncamd_debye_msd(double debye_temp, double temperature, double mass_amu);
@@ -105,6 +119,33 @@ int ncamd_sab_derived(ncamd_process_t, double* logsab, double* cumul);
 int ncamd_sab_sampler(ncamd_process_t, int iegrid, int* kind, int* ibeta_low, int* nvals,
                       double* firstbin, double* bx, double* by, double* bcdf,
                       double* info_dbl, int* info_idx);
+
+/* Mosaic single crystal (oriented), replaces
+ * SCBragg(const Info&, const SCOrientation&, MosaicityFWHM, 0.0, nullptr, prec, ntrunc)
+ * (ncrystal_core/src/scbragg/NCSCBragg.cc:90-231) with the standard plane
+ * provider: families (d-spacing [Aa], |F|^2 [barn]) with explicit hkl members
+ * (mem_off[nfam+1], mem_hkl[3*nmem], as ncamd_hkl_orthogonal_members returns),
+ * the reciprocal lattice (row-major 3x3, getReciprocalLatticeRot of
+ * ncrystal_core/src/utils/NCLatticeUtils.cc:95-117) and the crystal-to-lab
+ * rotation (row-major 3x3, getCrystal2LabRot of
+ * ncrystal_core/src/extd_utils/NCOrientUtils.cc:28-35).  Mosaicity is the
+ * Gaussian FWHM in radians; prec and ntrunc as NCrystal's mosprec and
+ * truncation (ntrunc 0 = derived from prec).  Evaluated with
+ * ncamd_crosssection_many_dirs / ncamd_samplescatter_many(_dirs) and their
+ * device-pointer forms; the non-oriented calls raise BadInput. */
+ncamd_process_t ncamd_create_scbragg(double v0_times_natoms, int64_t nfam, const double* dspacing,
+                                     const double* fsquared, const int64_t* mem_off, const int* mem_hkl,
+                                     const double* reci_lattice, const double* cry2lab,
+                                     double mosaicity_fwhm, double prec, double ntrunc);
+/* read-back: family count, demi-normal count, threshold energy [eV] and 15
+ * GaussOnSphere parameters (sigma, truncangle, prec, cos/sin truncangle,
+ * expfact, norm, k1, k2, numacc, table size, first argument and inverse step
+ * of both spline tables) */
+int ncamd_scbragg_info(ncamd_process_t, int64_t* nfam, int64_t* nnormals, double* threshold, double* gos);
+/* families (1/(2d), |F|^2/(V0 n), demi-normal offsets[nfam+1]), lab-frame
+ * demi-normals (3 per) and the two spline tables ((y, y'') pairs) */
+int ncamd_scbragg_tables(ncamd_process_t, double* inv2d, double* xsfact, int64_t* off, double* normals,
+                         double* lt_evalcosx, double* lt_sofcosd);
 
 /* ---------- RNG (ncrystal.h:1067-1087) ----------
  * Each process handle owns a counter-based stream: (seed, call counter).
@@ -163,6 +204,9 @@ void ncamd_dev_samplescatter_many_dirs(ncamd_process_t, const double* d_ekin,
                                        const double* d_dirs, unsigned long n,
                                        double* d_results_ekin, double* d_results_dirs,
                                        void* stream);
+/* per-neutron directions (oriented processes; isotropic ones ignore d_dirs) */
+void ncamd_dev_crosssection_many_dirs(ncamd_process_t, const double* d_ekin, const double* d_dirs,
+                                      unsigned long n, double* d_results, void* stream);
 /* S(alpha,beta) only: raw (alpha,beta) samples of the plain sampler */
 void ncamd_dev_sab_sample_alphabeta(ncamd_process_t, const double* d_ekin, unsigned long n,
                                     double* d_alpha, double* d_beta, void* stream);

@@ -81,6 +81,15 @@ def lib():
                                                                          ctypes.c_double, ctypes.c_double,
                                                                          ctypes.c_double, ctypes.c_int64, _ip,
                                                                          _ip, _dp, _dp, _i64p]),
+        "ncamd_hkl_orthogonal_members": (ctypes.c_int64, [ctypes.c_double] * 3 + [ctypes.c_int, _ip, _dp, _dp, _dp,
+                                                                                 ctypes.c_double, ctypes.c_double,
+                                                                                 ctypes.c_double, ctypes.c_int64,
+                                                                                 _ip, _ip, _dp, _dp, _i64p,
+                                                                                 ctypes.c_int64, _i64p, _ip, _i64p]),
+        "ncamd_create_scbragg": (Handle, [ctypes.c_double, ctypes.c_int64, _dp, _dp, _i64p, _ip, _dp, _dp,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+        "ncamd_scbragg_info": (ctypes.c_int, [Handle, _i64p, _i64p, _dp, _dp]),
+        "ncamd_scbragg_tables": (ctypes.c_int, [Handle, _dp, _dp, _i64p, _dp, _dp, _dp]),
         "ncamd_create_powderbragg": (Handle, [ctypes.c_double, _dp, _dp, ctypes.c_int64, ctypes.c_int]),
         "ncamd_create_elincscatter": (Handle, [ctypes.c_int, _dp, _dp, _dp]),
         "ncamd_create_sabscatter": (Handle, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
@@ -105,6 +114,7 @@ def lib():
         "ncamd_dev_crosssection_nonoriented_many_fp32": (None, [Handle, _vp, _ul, _ul, _vp, _vp]),
         "ncamd_dev_samplescatterisotropic_many": (None, [Handle, _vp, _ul, _ul, _vp, _vp, _vp]),
         "ncamd_dev_samplescatter_many_dirs": (None, [Handle, _vp, _vp, _ul, _vp, _vp, _vp]),
+        "ncamd_dev_crosssection_many_dirs": (None, [Handle, _vp, _vp, _ul, _vp, _vp]),
         "ncamd_dev_sab_sample_alphabeta": (None, [Handle, _vp, _ul, _vp, _vp, _vp]),
         "ncamd_dev_check_errors": (ctypes.c_int64, [Handle, _vp]),
         "ncamd_dev_malloc": (_vp, [ctypes.c_uint64]),
@@ -153,6 +163,11 @@ def dptr(a):
 
 def iptr(a):
     return a.ctypes.data_as(_ip)
+
+
+def i64ptr(a):
+    assert a.dtype == np.int64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_i64p)
 
 
 def fptr(a):

@@ -163,6 +163,90 @@ def hkl_orthogonal(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tolerance=1
         return hkl[:3 * n].reshape(n, 3).copy(), mult[:n].copy(), dsp[:n].copy(), fsq[:n].copy()
 
 
+def hkl_orthogonal_members(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tolerance=1e-6):
+    """As hkl_orthogonal, plus the explicit hkl members of every family:
+    returns (hkl, mult, dspacing, fsquared, mem_off[n+1], mem_hkl[m,3])."""
+    natoms = np.array([len(s[2]) for s in species], dtype=np.int32)
+    pos = C.as_f64(np.concatenate([np.asarray(s[2], dtype=np.float64).reshape(-1, 3) for s in species]).ravel())
+    csl = C.as_f64([s[0] for s in species])
+    msd = C.as_f64([s[1] for s in species])
+    maxout, maxmem = 4096, 65536
+    req, mreq = ctypes.c_int64(), ctypes.c_int64()
+    while True:
+        hkl = np.empty(3 * maxout, dtype=np.int32)
+        mult = np.empty(maxout, dtype=np.int32)
+        dsp = np.empty(maxout)
+        fsq = np.empty(maxout)
+        moff = np.empty(maxout + 1, dtype=np.int64)
+        mhkl = np.empty(3 * maxmem, dtype=np.int32)
+        n = C.call("ncamd_hkl_orthogonal_members", a, b, c, len(species), C.iptr(natoms), C.dptr(pos),
+                   C.dptr(csl), C.dptr(msd), dcutoff, fsquarecut, merge_tolerance, maxout, C.iptr(hkl),
+                   C.iptr(mult), C.dptr(dsp), C.dptr(fsq), ctypes.byref(req), maxmem, C.i64ptr(moff),
+                   C.iptr(mhkl), ctypes.byref(mreq))
+        if n == -1:
+            maxout = max(maxout, int(req.value))
+            maxmem = max(maxmem, int(mreq.value))
+            continue
+        if n == -2:
+            raise C.NCCalcError(f"Combinatorics too great to reach dcutoff = {dcutoff} Aa")
+        if n < 0:
+            raise C.NCAMDError("hkl enumeration failed")
+        nm = int(moff[n])
+        return (hkl[:3 * n].reshape(n, 3).copy(), mult[:n].copy(), dsp[:n].copy(), fsq[:n].copy(),
+                moff[:n + 1].copy(), mhkl[:3 * nm].reshape(nm, 3).copy())
+
+
+def reci_lattice_orthogonal(a, b, c):
+    """Reciprocal lattice of a cell with 90-degree angles, row-major 3x3
+    (diag(2pi/a, 2pi/b, 2pi/c), ncrystal_core/src/utils/NCLatticeUtils.cc:95-117)."""
+    k = 2.0 * np.pi
+    return np.array([k / a, 0.0, 0.0, 0.0, k / b, 0.0, 0.0, 0.0, k / c])
+
+
+class SCBragg(Scatter):
+    """Oriented mosaic single crystal (NCrystal's SCBragg).  Use
+    crossSection / sampleScatter / xsect(direction=...)."""
+
+    def info(self):
+        nf, nn, thr = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+        gos = np.empty(15)
+        C.call("ncamd_scbragg_info", self._h, ctypes.byref(nf), ctypes.byref(nn), ctypes.byref(thr), C.dptr(gos))
+        return dict(nfam=nf.value, nnormals=nn.value, threshold=thr.value, gos=gos)
+
+    def tables(self):
+        inf = self.info()
+        nf, nn, nlt = inf["nfam"], inf["nnormals"], int(inf["gos"][10])
+        inv2d, xsf = np.empty(nf), np.empty(nf)
+        off = np.empty(nf + 1, dtype=np.int64)
+        nrm = np.empty((nn, 3))
+        lt1, lt2 = np.empty(2 * nlt), np.empty(2 * nlt)
+        C.call("ncamd_scbragg_tables", self._h, C.dptr(inv2d), C.dptr(xsf), C.i64ptr(off), C.dptr(nrm),
+               C.dptr(lt1), C.dptr(lt2))
+        return dict(inv2d=inv2d, xsfact=xsf, off=off, normals=nrm, lt_evalcosx=lt1.reshape(-1, 2),
+                    lt_sofcosd=lt2.reshape(-1, 2))
+
+    def dev_crossSection(self, d_ekin, d_dirs, n, d_out, stream=0):
+        C.call("ncamd_dev_crosssection_many_dirs", self._h, d_ekin, d_dirs, n, d_out, stream)
+
+    def dev_sampleScatter(self, d_ekin, d_dirs, n, d_oe, d_od, stream=0):
+        C.call("ncamd_dev_samplescatter_many_dirs", self._h, d_ekin, d_dirs, n, d_oe, d_od, stream)
+
+
+def create_scbragg(v0_times_natoms, dspacing, fsquared, mem_off, mem_hkl, reci_lattice, cry2lab,
+                   mosaicity_fwhm, prec=1e-3, ntrunc=0.0):
+    """SCBragg from hkl families with explicit members (see
+    hkl_orthogonal_members); cry2lab: 3x3 crystal-to-lab rotation;
+    mosaicity_fwhm in radians."""
+    d, f = C.as_f64(dspacing), C.as_f64(fsquared)
+    mo = np.ascontiguousarray(mem_off, dtype=np.int64)
+    mh = np.ascontiguousarray(mem_hkl, dtype=np.int32).ravel()
+    L = C.as_f64(np.asarray(reci_lattice, dtype=np.float64).ravel())
+    R = C.as_f64(np.asarray(cry2lab, dtype=np.float64).ravel())
+    h = C.call("ncamd_create_scbragg", float(v0_times_natoms), d.size, C.dptr(d), C.dptr(f), C.i64ptr(mo),
+               C.iptr(mh), C.dptr(L), C.dptr(R), float(mosaicity_fwhm), float(prec), float(ntrunc))
+    return SCBragg(h, name="SCBragg")
+
+
 def create_powderbragg(v0_times_natoms, dspacings, fsq_x_mult, fp32_tables=True):
     d = C.as_f64(dspacings)
     f = C.as_f64(fsq_x_mult)

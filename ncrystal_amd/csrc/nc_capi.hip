@@ -25,6 +25,7 @@
 
 #include "../../include/ncrystal_amd.h"
 #include "nc_kernels.hh"
+#include "nc_math.hh"
 
 namespace ncamd {
 namespace {
@@ -217,6 +218,11 @@ struct Proc {
   // S(alpha,beta) host copies for read-back
   int sab_na = 0, sab_nb = 0;
   std::vector<double> sab_egrid_h, sab_xs_h;
+  // single-crystal Bragg (host copies of the device tables for read-back)
+  SCBView scb{};
+  std::vector<double> scb_inv2d_h, scb_xsfact_h, scb_normals_h, scb_lt1_h, scb_lt2_h;
+  std::vector<int64_t> scb_off_h;
+  double scb_gos[15] = {};
   // composition
   std::vector<Proc*> comps;
   std::vector<double> scales;
@@ -272,6 +278,7 @@ CompView make_view(Proc* p) {
         break;
       case kKindElInc: v.ei = c->ei; break;
       case kKindSAB: v.sab = c->sab; break;
+      case kKindSCB: raise("BadInput", "oriented process: use the calls with neutron directions");
       default: raise("LogicError", "bad component kind");
     }
     if (dlo < dhi) { lo = std::min(lo, dlo); hi = std::max(hi, dhi); }
@@ -662,7 +669,8 @@ struct Fam {
 int64_t hkl_orthogonal(double a, double b, double c, int nspecies, const int* natoms, const double* positions,
                        const double* csl, const double* msd, double dcutoff, double fsquarecut, double merge_tol,
                        int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq,
-                       int64_t* nrequired) {
+                       int64_t* nrequired, int64_t maxmem = 0, int64_t* out_mem_off = nullptr,
+                       int* out_mem_hkl = nullptr, int64_t* nmem_required = nullptr) {
   ensure_device();
   if (!(dcutoff > 0.0)) raise("BadInput", "dcutoff must be positive");
   if (!(a > 0 && b > 0 && c > 0)) raise("BadInput", "lattice parameters must be positive");
@@ -778,11 +786,309 @@ int64_t hkl_orthogonal(double a, double b, double c, int nspecies, const int* na
   const int64_t nf = (int64_t)fams.size();
   if (nrequired) *nrequired = nf;
   if (nf > maxout) return -1;
+  int64_t nmem = 0;
+  for (const auto& f : fams) nmem += (int64_t)f.members.size();
+  if (nmem_required) *nmem_required = nmem;
+  if (out_mem_off && nmem > maxmem) return -1;
   for (int64_t i = 0; i < nf; ++i) {
     out_hkl[3 * i] = fams[i].rep.h; out_hkl[3 * i + 1] = fams[i].rep.k; out_hkl[3 * i + 2] = fams[i].rep.l;
     out_mult[i] = fams[i].mult; out_dsp[i] = fams[i].dsp; out_fsq[i] = fams[i].fsq;
   }
+  if (out_mem_off) {
+    int64_t m = 0;
+    for (int64_t i = 0; i < nf; ++i) {
+      out_mem_off[i] = m;
+      for (const auto& x : fams[i].members) {
+        out_mem_hkl[3 * m] = x.h; out_mem_hkl[3 * m + 1] = x.k; out_mem_hkl[3 * m + 2] = x.l;
+        ++m;
+      }
+    }
+    out_mem_off[nf] = m;
+  }
   return nf;
+}
+
+// ------------------------------------------------- single-crystal Bragg ---
+// Construction of an SCBragg process (init, host FP64, same operation order
+// as the reference so that every table is bit-identical):
+//  - families: ncrystal_core/src/scbragg/NCSCBragg.cc:125-231 (planes keyed
+//    by (d, |F|^2) rounded to 2^-30, std::greater order, original values kept
+//    when unambiguous, demi-normals rotated to the lab frame)
+//  - demi-normals from explicit hkl: ncrystal_core/src/extd_utils/NCPlaneProvider.cc:111-175
+//  - GaussMos / truncation: ncrystal_core/src/phys_utils/NCGaussMos.cc:38-75,
+//    ncrystal_core/src/phys_utils/NCGaussOnSphere.cc:510-547
+//  - GaussOnSphere::set (normalisation, fast-path bounds, spline tables):
+//    ncrystal_core/src/phys_utils/NCGaussOnSphere.cc:181-378
+//  - cubic spline with end-point derivatives: ncrystal_core/src/utils/NCSpline.cc:31-108
+//  - estimateSingleSidedDerivative / findRoot: ncrystal_core/src/utils/NCMath.cc:271-334
+constexpr double kFWHM2Sigma = 0.424660900144009521360751417051444809857570547;
+constexpr double kArcSecH = 0.00000484813681109535993589914102357947975956353302;
+
+struct HostSpline {
+  double a = 0, b = 0, invdelta = 0;
+  std::vector<double> y, y2;
+};
+
+void spline_fit(HostSpline& t, const std::vector<double>& y, double dleft, double dright) {
+  const int n = (int)y.size(), nm1 = n - 1;
+  std::vector<double> u(nm1), y2(n);
+  y2[0] = -0.5;
+  u[0] = 3.0 * ((y[1] - y[0]) - dleft);
+  for (int i = 1; i < nm1; ++i) {
+    const double p = 1.0 / (2.0 + 0.5 * y2[i - 1]);
+    y2[i] = -0.5 * p;
+    const double uu = y[i + 1] - 2.0 * y[i] + y[i - 1];
+    u[i] = (3.0 * uu - 0.5 * u[i - 1]) * p;
+  }
+  const double qn = 0.5;
+  const double un = 3.0 * (dright - (y[nm1] - y[n - 2]));
+  y2[nm1] = (un - qn * u[n - 2]) / (qn * y2[n - 2] + 1.0);
+  for (int k = nm1; k; --k) {
+    y2[k - 1] *= y2[k];
+    y2[k - 1] += u[k - 1];
+  }
+  t.y = y;
+  t.y2 = std::move(y2);
+}
+
+template <class F>
+void spline_table(HostSpline& t, const F& f, double a, double b, double fpa, double fpb, unsigned npts) {
+  t.a = a;
+  t.b = b;
+  std::vector<double> fv(npts);
+  const double delta = (b - a) / (npts - 1);
+  for (unsigned i = 0; i + 1 < npts; ++i) fv[i] = f(a + i * delta);
+  fv[npts - 1] = f(b);
+  spline_fit(t, fv, fpa * delta, fpb * delta);
+  t.invdelta = 1.0 / delta;
+}
+
+template <class F>
+double deriv_single_sided4(const F& f, double x, double h) {
+  return -(2 / 3.) * (31.5 * f(x) - 64. * f(x + 0.125 * h) + 40. * f(x + 0.25 * h) - 8. * f(x + 0.5 * h) +
+                      0.5 * f(x + h)) / h;
+}
+
+template <class F>
+double find_root(const F& f, double a, double b, double acc) {
+  double fa = f(a), fb = f(b);
+  acc *= 0.5;
+  unsigned i = 60;
+  while (--i) {
+    double c = (a * fb - b * fa) / (fb - fa);
+    if (b - a < acc) return c;
+    const double k = 0.15 * (b - a);
+    c = dmax(a + k, dmin(b - k, c));
+    const double fc = f(c);
+    if (!fc) return c;
+    if (fa * fc < 0) { b = c; fb = fc; } else { a = c; fa = fc; }
+  }
+  raise("CalcError", "root search did not converge");
+}
+
+struct GosHost {
+  double sigma, trunc, prec, cta, sta, expfact, norm, k1, k2, numacc;
+  unsigned nlt;
+  HostSpline evalcosx, sofcosd;
+};
+
+void gos_setup(GosHost& g, double sigma, double trunc, double prec) {
+  prec = prec <= 1 ? dclamp(prec, 1e-7, 0.1) : dclamp(prec, 1.0, 10000.0);
+  g.trunc = trunc; g.sigma = sigma; g.prec = prec;
+  g.cta = std::cos(trunc);
+  g.sta = std::sin(trunc);
+  g.expfact = -0.5 / (sigma * sigma);
+  {
+    const double c = -0.5 / (sigma * sigma);
+    auto f = [c](double x) { return dmax(0.0, std::sin(x) * std::exp(c * x * x)); };
+    auto acc = [](unsigned lvl, double prev, double est) { return lvl > 12 || fabs(prev - est) < 1e-12 * fabs(est); };
+    const double integral = k2Pi * romberg(f, acc, 0.0, dmin(trunc, 20.0 * sigma));
+    g.norm = 1.0 / integral;
+  }
+  unsigned nlt;
+  if (prec < 1.0) {
+    const double tp = dmin(0.999999, prec * 0.5);
+    auto fce = [tp](double x) { return (1.0 - (1.0 - 0.5 * x * x) / cos_mpi2pi2(x)) - tp; };
+    const double amax = find_root(fce, 0.0, 0.999999 * kPiHalf, 1e-13);
+    if (g.trunc > amax || amax <= 1e-10) {
+      g.k1 = __builtin_huge_val(); g.k2 = 0.0;
+    } else {
+      g.k1 = cos_mpi2pi2(amax); g.k2 = g.cta - 1e-5;
+    }
+    g.numacc = dclamp(prec * 0.1, 1e-7, 1e-4);
+    const unsigned v = (unsigned)(10.0 / std::sqrt(prec) + 0.5);
+    nlt = std::max(40u, std::min(v, 10000u));
+  } else {
+    const unsigned v = (unsigned)(prec + 0.5);
+    nlt = std::max(v, 20u);
+    g.numacc = 1e-4;
+    g.k1 = 0.0; g.k2 = __builtin_huge_val();
+  }
+  if (g.sigma < 10 * kArcSecH) {
+    g.numacc = dmax(g.numacc, 1e-6);
+    nlt = std::min(nlt, 1000u);
+    if (g.sigma < kArcSecH) {
+      nlt = std::min(nlt, 500u);
+      g.numacc = dmax(g.numacc, 1e-5);
+      if (g.sigma < 0.1 * kArcSecH) { nlt = std::min(nlt, 200u); g.numacc = dmax(g.numacc, 1e-4); }
+    }
+  }
+  g.nlt = nlt;
+  {
+    const double ltmin = cos_mpi2pi2(dmin(kPiHalf, trunc));
+    const double ltinvdelta = (nlt - 1.0) / (1.0 - ltmin);
+    const double norm = g.norm, ef = g.expfact;
+    auto f = [norm, ef](double cosx) {
+      const double x = std::acos(dmin(1.0, dmax(-1.0, cosx)));
+      return norm * std::exp(ef * x * x);
+    };
+    double h = 0.01 / ltinvdelta;
+    h = dmax(h, dmin(1e-4, (1.0 - ltmin) * 0.1));
+    const double fpb = deriv_single_sided4(f, 1.0, -h);
+    const double fpa = deriv_single_sided4(f, ltmin, h);
+    spline_table(g.evalcosx, f, ltmin, 1.0, fpa, fpb, nlt);
+  }
+  {
+    const double cta = cos_mpi2pi2(trunc);
+    const double kk = g.norm * 2.506628274631000502415765284811045253006986740609938316629923576 * sigma;
+    const double ef = g.expfact, tasq = trunc * trunc;
+    auto f = [kk, ef, tasq](double cosd) {
+      const double d = std::acos(dmin(1.0, dmax(-1.0, cosd)));
+      const double d2 = d * d;
+      return kk * std::exp(ef * d2) * std::erf(std::sqrt(dmax(0.0, -ef * (tasq - d2))));
+    };
+    double h = 0.01 * (1.0 - cta) / nlt;
+    h = dmax(h, dmin(1e-4, (1.0 - cta) * 0.1));
+    const double fpb = deriv_single_sided4(f, 1.0, -h);
+    const double fpa = f(cta + (1.0 - cta) / nlt) / ((1.0 - cta) / nlt);
+    spline_table(g.sofcosd, f, cta, 1.0, fpa, fpb, nlt);
+  }
+}
+
+Proc* create_scb(double v0n, int64_t nfam, const double* dsp, const double* fsq, const int64_t* mem_off,
+                 const int* mem_hkl, const double* L, const double* R, double mos_fwhm, double prec,
+                 double ntrunc) {
+  ensure_device();
+  if (!(v0n > 0)) raise("BadInput", "v0_times_natoms is not a positive number.");
+  if (nfam < 1) raise("BadInput", "SCBragg needs at least one reflection family");
+  if (!(mos_fwhm > 0.0) || !(prec >= 0.0) || !(ntrunc >= 0.0)) raise("BadInput", "invalid mosaicity parameters");
+  if (mem_off[0] != 0) raise("BadInput", "member offsets must start at 0");
+  const int64_t nplanes = mem_off[nfam];
+  if (nplanes < 1 || nplanes >= (int64_t)1 << 31) raise("BadInput", "number of demi-normals out of range");
+  constexpr double two30 = 1073741824.0;
+  // planes in provider order, keyed by rounded (d, |F|^2)
+  struct Key { uint64_t kd, kf; int64_t src; };
+  std::vector<Key> keys((size_t)nplanes);
+  std::vector<double> pn(3 * (size_t)nplanes);
+  std::map<uint64_t, double> orig_d, orig_f;
+  double maxd = 0.0;
+  int64_t q = 0;
+  for (int64_t f = 0; f < nfam; ++f) {
+    if (mem_off[f + 1] < mem_off[f]) raise("BadInput", "member offsets must be non-decreasing");
+    if (!(dsp[f] > 0.0 && fsq[f] > 0.0 && dsp[f] < 1e7 && fsq[f] < 1e7))
+      raise("BadInput", "d-spacing and |F|^2 must be in (0, 1e7)");
+    for (int64_t m = mem_off[f]; m < mem_off[f + 1]; ++m, ++q) {
+      const double h = mem_hkl[3 * m], k = mem_hkl[3 * m + 1], l = mem_hkl[3 * m + 2];
+      double vx = L[0] * h + L[1] * k + L[2] * l;
+      double vy = L[3] * h + L[4] * k + L[5] * l;
+      double vz = L[6] * h + L[7] * k + L[8] * l;
+      const double m2 = vx * vx + vy * vy + vz * vz;
+      constexpr double eps = 2.220446049250313080847263336181640625e-16;
+      if (!(m2 >= 1.0 - 2.0 * eps && m2 <= 1.0 + 2.0 * eps)) {
+        if (!(m2 > 0.0)) raise("BadInput", "hkl (0,0,0) in a family member list");
+        const double ff = 1.0 / std::sqrt(m2);
+        vx *= ff; vy *= ff; vz *= ff;
+      }
+      pn[3 * q] = vx; pn[3 * q + 1] = vy; pn[3 * q + 2] = vz;
+      if (dsp[f] > maxd) maxd = dsp[f];
+      const uint64_t kd = (uint64_t)(dsp[f] * two30 + 0.5), kf = (uint64_t)(fsq[f] * two30 + 0.5);
+      keys[q] = Key{kd, kf, q};
+      auto itd = orig_d.find(kd);
+      if (itd == orig_d.end()) orig_d[kd] = dsp[f];
+      else if (fabs(dsp[f] - itd->second) > 1e-12) itd->second = -1;
+      auto itf = orig_f.find(kf);
+      if (itf == orig_f.end()) orig_f[kf] = fsq[f];
+      else if (fabs(fsq[f] - itf->second) > 1e-12) itf->second = -1;
+    }
+  }
+  // std::greater on (kd, kf); planes of one key stay in provider order
+  std::stable_sort(keys.begin(), keys.end(), [](const Key& x, const Key& y) {
+    return x.kd != y.kd ? x.kd > y.kd : x.kf > y.kf;
+  });
+  std::vector<double> inv2d, inv2d_r, xsfact, xsf, normals(3 * (size_t)nplanes);
+  std::vector<int64_t> off;
+  for (int64_t i = 0; i < nplanes; ++i) {
+    if (i == 0 || keys[i].kd != keys[i - 1].kd || keys[i].kf != keys[i - 1].kf) {
+      const double dd = orig_d[keys[i].kd], fv = orig_f[keys[i].kf];
+      const double dspv = dd > 0 ? dd : keys[i].kd / two30;
+      const double fsqv = fv > 0 ? fv : keys[i].kf / two30;
+      xsfact.push_back(fsqv / v0n);
+      inv2d.push_back(0.5 / dspv);
+      off.push_back(i);
+    }
+    const double* n = &pn[3 * keys[i].src];
+    normals[3 * i] = R[0] * n[0] + R[1] * n[1] + R[2] * n[2];
+    normals[3 * i + 1] = R[3] * n[0] + R[4] * n[1] + R[5] * n[2];
+    normals[3 * i + 2] = R[6] * n[0] + R[7] * n[1] + R[8] * n[2];
+  }
+  off.push_back(nplanes);
+  const int nf = (int)inv2d.size();
+  for (int f = 0; f < nf; ++f) {
+    inv2d_r.push_back(std::floor(dmax(inv2d[f], 1e-15) * 1e15 + 0.5) * 1e-15);
+    xsf.push_back(xsfact[f] * 0.5);
+  }
+  // GaussMos truncation and GaussOnSphere tables
+  double nt = ntrunc;
+  if (nt == 0) {
+    if (prec == 0) nt = 8.0;
+    else if (prec >= 1.0) nt = 3.0;
+    else nt = dclamp(1.1 * std::sqrt(-2.0 * std::log(dmax(1e-300, prec))), 3.0, 8.0);
+  }
+  const double sigma = mos_fwhm * kFWHM2Sigma;
+  const double trunc = nt * sigma;
+  if (!(trunc < kPiHalf))
+    raise("BadInput", "Mosaicity too large, truncation angle (sigma*Ntrunc) must be less than pi/2");
+  GosHost g;
+  gos_setup(g, sigma, trunc, prec);
+
+  std::unique_ptr<Proc> p(new Proc());
+  p->kind = kKindSCB;
+  std::vector<int> off32(off.begin(), off.end());
+  std::vector<double> lt1, lt2;
+  for (size_t i = 0; i < g.evalcosx.y.size(); ++i) { lt1.push_back(g.evalcosx.y[i]); lt1.push_back(g.evalcosx.y2[i]); }
+  for (size_t i = 0; i < g.sofcosd.y.size(); ++i) { lt2.push_back(g.sofcosd.y[i]); lt2.push_back(g.sofcosd.y2[i]); }
+  DevBuf* b_inv2d = p->newbuf(); b_inv2d->upload(inv2d.data(), inv2d.size());
+  DevBuf* b_inv2dr = p->newbuf(); b_inv2dr->upload(inv2d_r.data(), inv2d_r.size());
+  DevBuf* b_xsf = p->newbuf(); b_xsf->upload(xsf.data(), xsf.size());
+  DevBuf* b_off = p->newbuf(); b_off->upload(off32.data(), off32.size());
+  DevBuf* b_nrm = p->newbuf(); b_nrm->upload(normals.data(), normals.size());
+  DevBuf* b_lt1 = p->newbuf(); b_lt1->upload(lt1.data(), lt1.size());
+  DevBuf* b_lt2 = p->newbuf(); b_lt2->upload(lt2.data(), lt2.size());
+  SCBView& v = p->scb;
+  v.nfam = nf;
+  v.inv2d = b_inv2d->as<double>();
+  v.inv2d_r = b_inv2dr->as<double>();
+  v.xsf = b_xsf->as<double>();
+  v.off = b_off->as<int>();
+  v.normals = b_nrm->as<double>();
+  {
+    const double w = maxd * 2.0, w2 = w * w;
+    v.threshold = w2 ? kWl2Ekin / w2 : __builtin_huge_val();
+  }
+  v.cta = g.cta; v.k1 = g.k1; v.k2 = g.k2; v.numacc = g.numacc;
+  v.lt_evalcosx = SplineView{g.evalcosx.a, g.evalcosx.invdelta, (int)g.evalcosx.y.size(), b_lt1->as<double2>()};
+  v.lt_sofcosd = SplineView{g.sofcosd.a, g.sofcosd.invdelta, (int)g.sofcosd.y.size(), b_lt2->as<double2>()};
+  p->scb_inv2d_h = inv2d;
+  p->scb_xsfact_h = xsfact;
+  p->scb_off_h = off;
+  p->scb_normals_h = normals;
+  p->scb_lt1_h = lt1;
+  p->scb_lt2_h = lt2;
+  const double gd[15] = {g.sigma, g.trunc, g.prec, g.cta, g.sta, g.expfact, g.norm, g.k1, g.k2, g.numacc,
+                         (double)g.nlt, g.evalcosx.a, g.evalcosx.invdelta, g.sofcosd.a, g.sofcosd.invdelta};
+  std::copy(gd, gd + 15, p->scb_gos);
+  init_errbuf(p.get());
+  return p.release();
 }
 
 // ---------------------------------------------------- batched evaluation ---
@@ -821,6 +1127,16 @@ void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat,
     launch_sample_iso(make_view(p), ekin, n, repeat, p->seed, stream, oE, oMu, p->d_err.as<unsigned long long>(),
                       st);
   }
+}
+
+// queue of the SCBragg Romberg kernels, sized for n neutrons
+void scb_queue(Proc* p, int64_t n, hipStream_t st) {
+  const size_t need = sizeof(int64_t) * (size_t)std::max<int64_t>(n, 1);
+  if (p->q_idx.bytes < need) {
+    NCAMD_HIP(hipStreamSynchronize(st));  // the old queue may still be in use on this stream
+    p->q_idx.alloc(need);
+  }
+  if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
 }
 
 void check_fp32(Proc* p) {
@@ -869,7 +1185,7 @@ int ncamd_valid(ncamd_process_t h) {
   return (p && p->magic == kMagic) ? 1 : 0;
 }
 int ncamd_isoriented(ncamd_process_t h) {
-  NCAMD_TRY extract(h); return 0; NCAMD_CATCH
+  NCAMD_TRY return extract(h)->kind == kKindSCB ? 1 : 0; NCAMD_CATCH
   return -1;
 }
 
@@ -887,6 +1203,56 @@ int64_t ncamd_hkl_orthogonal(double a, double b, double c, int nspecies, const i
                         out_hkl, out_mult, out_dsp, out_fsq, nrequired);
   NCAMD_CATCH
   return -3;
+}
+int64_t ncamd_hkl_orthogonal_members(double a, double b, double c, int nspecies, const int* natoms,
+                                     const double* positions, const double* csl, const double* msd, double dcutoff,
+                                     double fsquarecut, double merge_tolerance, int64_t maxout, int* out_hkl,
+                                     int* out_mult, double* out_dsp, double* out_fsq, int64_t* nrequired,
+                                     int64_t maxmem, int64_t* out_mem_off, int* out_mem_hkl,
+                                     int64_t* nmem_required) {
+  NCAMD_TRY
+  if (!out_mem_off || !out_mem_hkl) raise("BadInput", "member output arrays are required");
+  return hkl_orthogonal(a, b, c, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut, merge_tolerance, maxout,
+                        out_hkl, out_mult, out_dsp, out_fsq, nrequired, maxmem, out_mem_off, out_mem_hkl,
+                        nmem_required);
+  NCAMD_CATCH
+  return -3;
+}
+
+ncamd_process_t ncamd_create_scbragg(double v0n, int64_t nfam, const double* dsp, const double* fsq,
+                                     const int64_t* mem_off, const int* mem_hkl, const double* reci_lattice,
+                                     const double* cry2lab, double mos_fwhm, double prec, double ntrunc) {
+  NCAMD_TRY
+  return wrap(create_scb(v0n, nfam, dsp, fsq, mem_off, mem_hkl, reci_lattice, cry2lab, mos_fwhm, prec, ntrunc));
+  NCAMD_CATCH
+  return wrap(nullptr);
+}
+int ncamd_scbragg_info(ncamd_process_t h, int64_t* nfam, int64_t* nnormals, double* threshold, double* gos) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindSCB) raise("BadInput", "not an SCBragg process");
+  *nfam = p->scb.nfam;
+  *nnormals = p->scb_off_h.back();
+  *threshold = p->scb.threshold;
+  std::copy(p->scb_gos, p->scb_gos + 15, gos);
+  return 0;
+  NCAMD_CATCH
+  return -1;
+}
+int ncamd_scbragg_tables(ncamd_process_t h, double* inv2d, double* xsfact, int64_t* off, double* normals,
+                         double* lt_evalcosx, double* lt_sofcosd) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindSCB) raise("BadInput", "not an SCBragg process");
+  std::copy(p->scb_inv2d_h.begin(), p->scb_inv2d_h.end(), inv2d);
+  std::copy(p->scb_xsfact_h.begin(), p->scb_xsfact_h.end(), xsfact);
+  std::copy(p->scb_off_h.begin(), p->scb_off_h.end(), off);
+  std::copy(p->scb_normals_h.begin(), p->scb_normals_h.end(), normals);
+  std::copy(p->scb_lt1_h.begin(), p->scb_lt1_h.end(), lt_evalcosx);
+  std::copy(p->scb_lt2_h.begin(), p->scb_lt2_h.end(), lt_sofcosd);
+  return 0;
+  NCAMD_CATCH
+  return -1;
 }
 
 ncamd_process_t ncamd_create_powderbragg(double v0n, const double* dsp, const double* fsqm, int64_t n, int fp32) {
@@ -915,6 +1281,8 @@ ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, co
   for (int i = 0; i < n; ++i) {
     Proc* c = extract(procs[i]);
     if (c->kind == kKindNone) raise("BadInput", "nested compositions are not supported");
+    if (c->kind == kKindSCB)
+      raise("BadInput", "an SCBragg process is evaluated on its own (oriented), not inside a composition");
     if (seen[c->kind]++) raise("BadInput", "at most one process of each kind per composition");
     if (!(scales[i] >= 0.0) || std::isinf(scales[i])) raise("BadInput", "component scales must be finite and >= 0");
     // null processes and zero scales are dropped (NCProcImpl.cc:263-283)
@@ -1074,7 +1442,9 @@ void ncamd_samplescatter_many_dirs(ncamd_process_t h, const double* ekin, const 
   NCAMD_TRY
   ensure_device();
   Proc* p = extract(h);
-  const CompView v = make_view(p);
+  const bool scb = p->kind == kKindSCB;
+  CompView v{};
+  if (!scb) v = make_view(p);
   p->st_in.ensure(sizeof(double) * n);
   p->st_in2.ensure(sizeof(double) * 3 * n);
   p->st_out1.ensure(sizeof(double) * n);
@@ -1083,8 +1453,15 @@ void ncamd_samplescatter_many_dirs(ncamd_process_t h, const double* ekin, const 
     NCAMD_HIP(hipMemcpy(p->st_in.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
     NCAMD_HIP(hipMemcpy(p->st_in2.p, dirs, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
   }
-  launch_sample_dirs(v, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->seed, next_stream(p),
-                     p->st_out1.as<double>(), p->st_out2.as<double>(), p->d_err.as<unsigned long long>(), 0);
+  if (scb) {
+    scb_queue(p, (int64_t)n, 0);
+    launch_scb_sample(p->scb, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->seed, next_stream(p),
+                      p->st_out1.as<double>(), p->st_out2.as<double>(), p->q_idx.as<int64_t>(),
+                      p->q_count.as<unsigned long long>(), 0);
+  } else {
+    launch_sample_dirs(v, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->seed, next_stream(p),
+                       p->st_out1.as<double>(), p->st_out2.as<double>(), p->d_err.as<unsigned long long>(), 0);
+  }
   NCAMD_HIP(hipGetLastError());
   if (n) {
     NCAMD_HIP(hipMemcpy(oE, p->st_out1.p, sizeof(double) * n, hipMemcpyDeviceToHost));
@@ -1111,8 +1488,32 @@ void ncamd_samplescatter_many(ncamd_process_t h, double ekin, const double (*dir
 
 void ncamd_crosssection_many_dirs(ncamd_process_t h, const double* ekin, const double* dirs, unsigned long n,
                                   double* out) {
-  (void)dirs;  // isotropic materials: the direction does not enter
-  xs_many_impl(h, ekin, n, 1, out, false);
+  bool scb = false;
+  NCAMD_TRY scb = extract(h)->kind == kKindSCB; NCAMD_CATCH
+  if (!scb) {
+    xs_many_impl(h, ekin, n, 1, out, false);  // isotropic materials: the direction does not enter
+    return;
+  }
+  bool ok = false;
+  NCAMD_TRY
+  ensure_device();
+  Proc* p = extract(h);
+  p->st_in.ensure(sizeof(double) * n);
+  p->st_in2.ensure(sizeof(double) * 3 * n);
+  p->st_out1.ensure(sizeof(double) * n);
+  if (n) {
+    NCAMD_HIP(hipMemcpy(p->st_in.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
+    NCAMD_HIP(hipMemcpy(p->st_in2.p, dirs, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  }
+  scb_queue(p, (int64_t)n, 0);
+  launch_scb_xs(p->scb, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->st_out1.as<double>(),
+                p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), 0);
+  NCAMD_HIP(hipGetLastError());
+  if (n) NCAMD_HIP(hipMemcpy(out, p->st_out1.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  ok = true;
+  NCAMD_CATCH
+  if (!ok)
+    for (unsigned long i = 0; i < n; ++i) out[i] = -1.0;
 }
 
 // ---- device pointer API
@@ -1145,8 +1546,28 @@ void ncamd_dev_samplescatter_many_dirs(ncamd_process_t h, const double* d_ekin, 
                                        unsigned long n, double* d_oE, double* d_oD, void* stream) {
   NCAMD_TRY
   Proc* p = extract(h);
-  launch_sample_dirs(make_view(p), d_ekin, d_dirs, (int64_t)n, p->seed, next_stream(p), d_oE, d_oD,
-                     p->d_err.as<unsigned long long>(), as_stream(stream));
+  if (p->kind == kKindSCB) {
+    scb_queue(p, (int64_t)n, as_stream(stream));
+    launch_scb_sample(p->scb, d_ekin, d_dirs, (int64_t)n, p->seed, next_stream(p), d_oE, d_oD,
+                      p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), as_stream(stream));
+  } else {
+    launch_sample_dirs(make_view(p), d_ekin, d_dirs, (int64_t)n, p->seed, next_stream(p), d_oE, d_oD,
+                       p->d_err.as<unsigned long long>(), as_stream(stream));
+  }
+  NCAMD_HIP(hipGetLastError());
+  NCAMD_CATCH
+}
+void ncamd_dev_crosssection_many_dirs(ncamd_process_t h, const double* d_ekin, const double* d_dirs,
+                                      unsigned long n, double* d_out, void* stream) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind == kKindSCB) {
+    scb_queue(p, (int64_t)n, as_stream(stream));
+    launch_scb_xs(p->scb, d_ekin, d_dirs, (int64_t)n, d_out, p->q_idx.as<int64_t>(),
+                  p->q_count.as<unsigned long long>(), as_stream(stream));
+  } else {
+    launch_xs_iso(make_view(p), d_ekin, (int64_t)n, 1, d_out, false, as_stream(stream));
+  }
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH
 }

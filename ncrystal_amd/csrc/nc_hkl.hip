@@ -10,31 +10,10 @@
 // and ncrystal_core/src/utils/NCMath.cc:96-125, Neumaier sums per species and
 // over species.
 #include "nc_kernels.hh"
+#include "nc_math.hh"
 
 namespace ncamd {
 
-NCD void sincos_mpi2pi2(double A, double& cosA, double& sinA) {
-  const double x = 0.5 * A;
-  const double mx2 = -x * x;
-  const double s2 = x * (1.0 + mx2 * (1.66666666666666666666666666666666666666666667e-1
-                   + mx2 * (8.33333333333333333333333333333333333333333333e-3
-                   + mx2 * (1.98412698412698412698412698412698412698412698e-4
-                   + mx2 * (2.75573192239858906525573192239858906525573192e-6
-                   + mx2 * (2.50521083854417187750521083854417187750521084e-8
-                   + mx2 * (1.60590438368216145993923771701549479327257105e-10
-                   + mx2 * (7.64716373181981647590113198578807044415510024e-13))))))));
-  const double c2m1 = mx2 * (0.5
-                + mx2 * (4.16666666666666666666666666666666666666666667e-2
-                + mx2 * (1.38888888888888888888888888888888888888888889e-3
-                + mx2 * (2.48015873015873015873015873015873015873015873e-5
-                + mx2 * (2.75573192239858906525573192239858906525573192e-7
-                + mx2 * (2.08767569878680989792100903212014323125434237e-9
-                + mx2 * (1.14707455977297247138516979786821056662326504e-11
-                + mx2 * (4.77947733238738529743820749111754402759693765e-14))))))));
-  const double k = 2.0 * c2m1;
-  sinA = (k + 2.0) * s2;
-  cosA = 1.0 + k * (c2m1 + 2.0);
-}
 NCD void sincos_2pix(double xx, double& s, double& c) {
   double A = (xx - floor(xx)) * k2Pi;
   A -= kPi;

@@ -2,10 +2,11 @@
 // layer of libncrystal_amd.
 #pragma once
 #include "nc_dev.hh"
+#include "nc_scb.hh"
 
 namespace ncamd {
 
-enum : int { kKindNone = 0, kKindPB = 1, kKindElInc = 2, kKindSAB = 3 };
+enum : int { kKindNone = 0, kKindPB = 1, kKindElInc = 2, kKindSAB = 3, kKindSCB = 4 };
 constexpr int kMaxComp = 4;
 
 // Device view of a process composition (passed by value as kernel argument).
@@ -34,6 +35,13 @@ void launch_sab_sample_iso(const SABView&, const double* ekin, int64_t n, int64_
                            uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, int64_t* queue,
                            unsigned long long* qcount, hipStream_t);
 void launch_sab_guides(const SABView&, unsigned short* cdf_guide, unsigned short* row_guide, double* row_scale,
+                       hipStream_t);
+
+// single-crystal Bragg, oriented (nc_scb.hip): main kernel + Romberg queue kernel
+void launch_scb_xs(const SCBView&, const double* ekin, const double* dirs, int64_t n, double* out,
+                   int64_t* queue, unsigned long long* qcount, hipStream_t);
+void launch_scb_sample(const SCBView&, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
+                       uint32_t stream, double* oE, double* oD, int64_t* queue, unsigned long long* qcount,
                        hipStream_t);
 
 // S(alpha,beta) table construction (nc_sab_build.hip)

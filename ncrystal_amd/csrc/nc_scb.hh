@@ -1,0 +1,464 @@
+// Device side of the mosaic single-crystal Bragg process (SCBragg): one lane
+// per neutron scans the lab-frame demi-normals of every reflection family
+// with 2d > lambda, evaluates the Gaussian-mosaic circle integral for the
+// normals within the truncated mosaic cone, and for sampling picks one
+// contribution by its cumulative weight and draws the outgoing direction on
+// the circle of valid mosaic normals.
+//
+// Reference algorithms (NCrystal v4.4.2):
+//  - cache/families/sampling: ncrystal_core/src/scbragg/NCSCBragg.cc:225-323
+//  - GaussMos::calcCrossSections, genScat, InteractionPars::set:
+//    ncrystal_core/src/phys_utils/NCGaussMos.cc:29-36,147-280 and the inline
+//    calcRawCrossSectionValue of ncrystal_core/include/NCrystal/internal/phys_utils/NCGaussMos.hh
+//  - GaussOnSphere::circleIntegral (fast spline path) and circleIntegralSlow,
+//    genPointOnCircle: ncrystal_core/include/NCrystal/internal/phys_utils/NCGaussOnSphere.hh:194-211,
+//    ncrystal_core/src/phys_utils/NCGaussOnSphere.cc:58-140,380-508
+//  - cubic spline lookup: ncrystal_core/include/NCrystal/internal/utils/NCSpline.hh:118-160
+//  - CosSinGridGen: ncrystal_core/include/NCrystal/internal/utils/NCMath.hh:489-524
+//  - randPointOnUnitCircle: ncrystal_core/src/utils/NCRandUtils.cc:98-111
+//  - rotateToFrame: ncrystal_core/src/utils/NCRotMatrix.cc:84-147
+//  - pickRandIdxByWeight: ncrystal_core/src/utils/NCRandUtils.cc:198-222
+//
+// Lanes of a wavefront walk the families together: the family and normal
+// indices are wave-uniform (the loads of family data and normals are scalar
+// loads shared by the 64 lanes) and a lane whose families are exhausted is
+// masked.  The rare backscattering geometry that needs the Romberg circle
+// integral is left to a second kernel (AllowSlow=true) through a queue, so
+// the main kernel carries no Romberg state.
+#pragma once
+#include "nc_dev.hh"
+#include "nc_math.hh"
+
+namespace ncamd {
+
+struct SplineView {
+  double a, invdelta;
+  int n;
+  const double2* __restrict__ yy2;  // (y_i, y''_i) pairs
+};
+
+struct SCBView {
+  int nfam;
+  const double* __restrict__ inv2d;     // [nfam] 1/(2d), non-decreasing
+  const double* __restrict__ inv2d_r;   // [nfam] inv2d rounded to 1e-15 (GaussMos cache rounding)
+  const double* __restrict__ xsf;       // [nfam] |F|^2/(V0 n) * 0.5
+  const int* __restrict__ off;          // [nfam+1] demi-normal offsets
+  const double* __restrict__ normals;   // [3*n] lab-frame unit demi-normals
+  double threshold;                     // ekin below which nothing reflects
+  double cta, k1, k2, numacc;           // GaussOnSphere: cos(truncation), fast-path bounds, Romberg accuracy
+  SplineView lt_evalcosx, lt_sofcosd;
+};
+
+enum : int { kScbOK = 0, kScbSlow = 1 };
+
+NCD double scb_round(double x) { return floor(x * 1e15 + 0.5) * 1e-15; }
+NCD double gm_round(double x) { return floor(dmax(x, 1e-15) * 1e15 + 0.5) * 1e-15; }
+
+NCD double slt_eval(const SplineView& t, double x) {
+  const double xx = (x - t.a) * t.invdelta;
+  const int nm2 = t.n - 2;
+  const int idx = xx < 0.0 ? 0 : (xx < (double)nm2 ? (int)xx : nm2);
+  const double b = xx - (double)idx;
+  const double a = 1.0 - b;
+  const double2 p0 = t.yy2[idx], p1 = t.yy2[idx + 1];
+  double tmp = a * p0.x;
+  double tmp2 = (a * a * a - a) * p0.y;
+  tmp += b * p1.x;
+  tmp2 += (b * b * b - b) * p1.y;
+  return tmp + 0.166666666666666666666666666666666666666666666666666667 * tmp2;
+}
+NCD double gos_evalcosx_inrange(const SCBView& v, double cosx) {
+  return dmax(0.0, slt_eval(v.lt_evalcosx, cosx));
+}
+NCD double gos_evalcosx(const SCBView& v, double cosx) {
+  return cosx >= v.lt_evalcosx.a ? gos_evalcosx_inrange(v, cosx) : 0.0;
+}
+
+// cos/sin of offset + i*delta, i = 0..n-1, by rotation with periodic exact
+// recomputation (every 128 steps)
+struct CSGrid {
+  double c, s, cd, sd, phimax, negdelta;
+  unsigned left, recalc;
+  NCD void init(unsigned n, double offset, double delta) {
+    left = n - 1;
+    recalc = ((128u - 1) + (n / 128u) * 128u) - n;
+    phimax = offset + (n - 1) * delta;
+    negdelta = -delta;
+    sincos_0pi32(offset, c, s);
+    sincos_mpi8pi8(delta, cd, sd);
+  }
+  NCD bool step() {
+    if (!left) return false;
+    --left;
+    if ((left + recalc) % 128u) {
+      const double cn = c * cd - s * sd;
+      s = c * sd + s * cd;
+      c = cn;
+    } else {
+      const double ang = phimax + negdelta * left;
+      c = cos(ang);
+      s = sin(ang);
+    }
+    return true;
+  }
+};
+
+struct GCI {
+  double sasg, cacg, acc;
+};
+NCD double gci_f(const SCBView& v, const GCI& g, double c) { return gos_evalcosx_inrange(v, g.sasg * c + g.cacg); }
+NCD bool gci_accept(const GCI& g, unsigned level, double prev, double est) {
+  if (fabs(prev - est) <= g.acc * fabs(est)) return true;
+  if (level < 11) return false;
+  return true;
+}
+NCD double gci_sum(const SCBView& v, const GCI& g, unsigned n, double off, double delta) {
+  CSGrid gr;
+  gr.init(n, off, delta);
+  double sum = 0.0;
+  do { sum += gci_f(v, g, gr.c); } while (gr.step());
+  return sum;
+}
+// Romberg integration of the mosaic density along the circle, t in [a,b]
+__device__ __noinline__ double gci_integrate(const SCBView& v, const GCI& g, double a, double b) {
+  double h = (b - a);
+  double fv[17];
+  {
+    CSGrid gr;
+    gr.init(17, a, h * 0.0625);
+    unsigned i = 0;
+    do { fv[i++] = gci_f(v, g, gr.c); } while (gr.step());
+  }
+  h *= 0.5;
+  const double R00 = (fv[0] + fv[16]) * h;
+  const double R10 = h * fv[8] + 0.5 * R00;
+  const double R11 = (4. / 3.) * R10 + (-1. / 3.) * R00;
+  h *= 0.5;
+  const double R20 = h * (fv[4] + fv[12]) + 0.5 * R10;
+  const double R21 = (4. / 3.) * R20 + (-1. / 3.) * R10;
+  const double R22 = (16. / 15.) * R21 + (-1. / 15.) * R11;
+  h *= 0.5;
+  const double R30 = h * ((fv[2] + fv[6]) + (fv[10] + fv[14])) + 0.5 * R20;
+  const double R31 = (4. / 3.) * R30 + (-1. / 3.) * R20;
+  const double R32 = (16. / 15.) * R31 + (-1. / 15.) * R21;
+  const double R33 = (64. / 63.) * R32 + (-1. / 63.) * R22;
+  h *= 0.5;
+  const double R40 = h * (((fv[1] + fv[3]) + (fv[5] + fv[7])) + ((fv[9] + fv[11]) + (fv[13] + fv[15]))) + 0.5 * R30;
+  const double R41 = (4. / 3.) * R40 + (-1. / 3.) * R30;
+  const double R42 = (16. / 15.) * R41 + (-1. / 15.) * R31;
+  const double R43 = (64. / 63.) * R42 + (-1. / 63.) * R32;
+  const double R44 = (256. / 255.) * R43 + (-1. / 255.) * R33;
+  if (gci_accept(g, 4, R33, R44)) return R44;
+  const double c5 = gci_sum(v, g, 16, a + h * 0.5, h);
+  h *= 0.5;
+  const double R50 = h * c5 + 0.5 * R40;
+  const double R51 = (4. / 3.) * R50 + (-1. / 3.) * R40;
+  const double R52 = (16. / 15.) * R51 + (-1. / 15.) * R41;
+  const double R53 = (64. / 63.) * R52 + (-1. / 63.) * R42;
+  const double R54 = (256. / 255.) * R53 + (-1. / 255.) * R43;
+  const double R55 = (1024. / 1023.) * R54 + (-1. / 1023.) * R44;
+  if (gci_accept(g, 5, R44, R55)) return R55;
+  constexpr unsigned maxlevel = 16;
+  double rows[2][maxlevel];
+  int ip = 0;
+  rows[0][0] = R50; rows[0][1] = R51; rows[0][2] = R52;
+  rows[0][3] = R53; rows[0][4] = R54; rows[0][5] = R55;
+  unsigned nj = 16;
+  for (unsigned i = 6; i < maxlevel; ++i) {
+    const double hh = h;
+    h *= 0.5;
+    nj *= 2;
+    const double cc = gci_sum(v, g, nj, a + h, hh);
+    double* row_prev = rows[ip];
+    double* row = rows[ip ^ 1];
+    row[0] = h * cc + 0.5 * row_prev[0];
+    double n_k = 1.;
+    for (unsigned j = 0; j < i; ++j) {
+      n_k *= 4.0;
+      row[j + 1] = (n_k * row[j] - row_prev[j]) / (n_k - 1.0);
+    }
+    if (gci_accept(g, i, row_prev[i - 1], row[i])) return row[i];
+    ip ^= 1;
+  }
+  return rows[ip][maxlevel - 1];
+}
+
+NCD double gos_circle_slow(const SCBView& v, double cg, double sg, double ca, double sa) {
+  const double sasg = sa * sg;
+  const double cacg = ca * cg;
+  const double cd = cacg + sasg;
+  if (cd <= v.cta) return 0.0;
+  if (sasg < 1e-14) return k2Pi * sa * gos_evalcosx(v, ca);
+  const double cos_tmax = (v.cta - cacg) / sasg;
+  const double tmax = (cos_tmax <= -1.0 ? kPi : acos(dmin(1.0, cos_tmax)));
+  if (tmax <= 1e-12) return 0.0;
+  constexpr double kArcSec = 0.00000484813681109535993589914102357947975956353302;
+  double intacc = v.numacc;
+  if (tmax < 10 * kArcSec) {
+    intacc = dmax(intacc, 1e-6);
+    if (tmax < kArcSec) {
+      intacc = dmax(intacc, 1e-5);
+      if (tmax < 0.1 * kArcSec) intacc = dmax(intacc, 1e-4);
+    }
+  }
+  GCI g{sasg, cacg, intacc};
+  return 2.0 * sa * gci_integrate(v, g, 0.0, tmax);
+}
+
+// true if the fast spline evaluation applies (NCGaussOnSphere.hh:194-211)
+NCD bool gos_circle_fast_ok(const SCBView& v, double cg, double sg, double ca, double sa, double& val) {
+  const double sasg = sa * sg;
+  const double cacg = ca * cg;
+  const double cd = cacg + sasg;
+  if (cd > v.cta && sasg >= 1e-14 && v.k2 > v.k1 * sasg + cacg) {
+    val = slt_eval(v.lt_sofcosd, cd) * sqrt(sa / sg);
+    return true;
+  }
+  return false;
+}
+
+// randPointOnUnitCircle: rejection in the square, 2 draws per try
+NCD void rand_point_on_circle(Rng& rng, double& c, double& s) {
+  double a, b, m2;
+  do {
+    a = -1.0 + rng.gen() * 2.0;
+    b = -1.0 + rng.gen() * 2.0;
+    m2 = a * a + b * b;
+  } while (!in_interval(0.001, 1.0, m2));
+  const double m = 1.0 / sqrt(m2);
+  c = a * m;
+  s = b * m;
+}
+
+NCD bool gos_gen_point(const SCBView& v, Rng& rng, double cg, double sg, double ca, double sa, double& ct,
+                       double& st) {
+  const double sasg = sa * sg;
+  const double cacg = ca * cg;
+  const double cd = cacg + sasg;
+  if (cd <= v.cta) return false;
+  if (sasg < 1e-14) {
+    if (sa < 1e-7) return false;
+    rand_point_on_circle(rng, ct, st);
+    return true;
+  }
+  const double cos_tmax = (v.cta - cacg) / sasg;
+  if (cos_tmax >= 1.0) return false;
+  const double tmax = (cos_tmax <= -1.0 ? kPi : acos(cos_tmax));
+  const double densitymax = gos_evalcosx_inrange(v, cd) * 1.00000001;
+  int triesleft = 1001;
+  while (--triesleft) {
+    ct = cos_mpipi(rng.gen() * tmax);
+    const double cd_at_t = sasg * ct + cacg;
+    const double d = gos_evalcosx_inrange(v, cd_at_t);
+    if (d > densitymax * rng.gen()) break;
+  }
+  if (triesleft <= 0) return false;
+  st = sqrt(1.0 - ct * ct);
+  st = (rng.gen() > 0.5 ? st : -st);
+  return true;
+}
+
+struct V3 {
+  double x, y, z;
+};
+NCD double v3dot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+NCD V3 v3cross(const V3& a, const V3& o) {
+  return V3{a.y * o.z - a.z * o.y, a.z * o.x - a.x * o.z, a.x * o.y - a.y * o.x};
+}
+// Vector::normalise: unchanged when |v|^2 is within 2 ulp of 1
+NCD V3 v3normalise(V3 v) {
+  const double m2 = v.x * v.x + v.y * v.y + v.z * v.z;
+  constexpr double eps = 2.220446049250313080847263336181640625e-16;
+  if (m2 >= 1.0 - 2.0 * eps && m2 <= 1.0 + 2.0 * eps) return v;
+  const double ff = 1.0 / sqrt(m2);
+  v.x *= ff; v.y *= ff; v.z *= ff;
+  return v;
+}
+NCD V3 phirot(double cp, double sp, const V3& v, const V3& axis) {
+  const V3 ac = v3cross(axis, v);
+  const double ad = v3dot(axis, v);
+  V3 r{v.x * cp, v.y * cp, v.z * cp};
+  r.x += ac.x * sp; r.y += ac.y * sp; r.z += ac.z * sp;
+  const double g2 = ad * (1.0 - cp);
+  r.x += axis.x * g2; r.y += axis.y * g2; r.z += axis.z * g2;
+  return r;
+}
+// maps v from the frame (c1, c2, b) built from unit vectors a, b with
+// sin/cos of their angle to the lab frame
+NCD V3 rotate_to_frame(double sinab, double cosab, const V3& a, const V3& b, V3 v, Rng& rng) {
+  if (fabs(sinab) < 1e-10) {
+    const double bz = b.z;
+    const double cp = bz, sp = -sqrt(1.0 - bz * bz);
+    V3 axis{b.y, -b.x, 0.};
+    const double m2 = axis.x * axis.x + axis.y * axis.y + axis.z * axis.z;
+    if (m2 > 1e-12) {
+      const double f = 1.0 / sqrt(m2);
+      axis.x *= f; axis.y *= f; axis.z *= f;
+      v = phirot(cp, sp, v, axis);
+    } else if (b.z < 0.0) {
+      v.z *= -1.0;
+    }
+    double rc, rs;
+    rand_point_on_circle(rng, rc, rs);
+    v = phirot(rc, rs, v, b);
+    return v3normalise(v);
+  }
+  const double s = 1.0 / sinab;
+  V3 c1 = b;
+  c1.x *= (-cosab); c1.y *= (-cosab); c1.z *= (-cosab);
+  c1.x += a.x; c1.y += a.y; c1.z += a.z;
+  c1.x *= s; c1.y *= s; c1.z *= s;
+  V3 c2 = v3cross(b, a);
+  c2.x *= s; c2.y *= s; c2.z *= s;
+  const V3 r{v.x * c1.x + v.y * c2.x + v.z * b.x, v.x * c1.y + v.y * c2.y + v.z * b.y,
+             v.x * c1.z + v.y * c2.z + v.z * b.z};
+  return v3normalise(r);
+}
+
+// GaussMos::genScat for the chosen (normal, inv2d) contribution
+NCD V3 gm_genscat(const SCBView& v, Rng& rng, const V3& normal, double inv2d_rounded, double wl_raw, const V3& indir) {
+  const double wl = gm_round(wl_raw);
+  const double stb = wl * inv2d_rounded;
+  if (stb == 0.) return indir;
+  const double ca = stb;
+  const double sa = sqrt(1.0 - ca * ca);
+  const double cg = dclamp(-(v3dot(indir, normal)), -1.0, 1.0);
+  const double sg = sqrt(1.0 - cg * cg);
+  double ct, st;
+  if (!gos_gen_point(v, rng, cg, sg, ca, sa, ct, st)) return indir;
+  const double s2a = 2 * sa * ca;
+  const double c2a = ca * ca - sa * sa;
+  const V3 w{s2a * ct, s2a * st, c2a};
+  const V3 mb{-indir.x, -indir.y, -indir.z};
+  return rotate_to_frame(sg, cg, normal, mb, w, rng);
+}
+
+// ---------------------------------------------------------------------------
+// wave-uniform helpers
+NCD int wave_max_i(int x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int y = __shfl_xor(x, o, 64);
+    x = x > y ? x : y;
+  }
+  return __builtin_amdgcn_readfirstlane(x);
+}
+
+// One lane's state during a scan of the reflection families.
+struct ScbLane {
+  double wl;       // wavelength (from the rounded energy)
+  V3 u;            // unit direction
+  int nf;          // number of families with 2d > wl (0: lane takes no part)
+  int count;       // number of non-zero contributions
+  double total;    // cumulative xs after the last contribution
+  bool slow;       // a contribution needs the Romberg circle integral
+  // selection result (Select=true)
+  bool found;
+  V3 sel_normal;
+  double sel_inv2d_r;
+};
+
+// Scan of all (family, demi-normal, side) contributions in the reference's
+// order (families by decreasing d, normals in family order, anti-normal side
+// first).  Select=false: count and cumulative total.  Select=true: find the
+// contribution picked by pickRandIdxByWeight for (n_select, target).
+// All lanes of the wavefront must call this together.
+template <bool Select, bool AllowSlow>
+NCD void scb_walk(const SCBView& v, ScbLane& L, int n_select, double target) {
+  L.count = 0;
+  L.total = 0.0;
+  L.slow = false;
+  L.found = false;
+  bool alive = L.nf > 0;
+  const int nfmax = wave_max_i(L.nf);
+  const double cta = v.cta;
+  const double w = gm_round(L.wl);
+  const double wl3 = w * w * w;
+  double commul_last = 0.0;
+  for (int f = 0; f < nfmax; ++f) {
+    if (!__any(alive)) break;
+    const bool act = alive && f < L.nf;
+    const double i2 = v.inv2d_r[f];
+    const double xsf = v.xsf[f];
+    const int j0 = v.off[f], j1 = v.off[f + 1];
+    const double sinpt = w * i2;
+    const double cptsq = 1 - sinpt * sinpt;
+    double Q = -1.0, Qprime = -1.0, cospt = -1.0;
+    const double xsoffset = L.count ? commul_last : 0.0;
+    double xssum = 0.0;
+    bool fam_alive = act;
+    for (int j = j0; j < j1; ++j) {
+      const V3 nrm{v.normals[3 * j], v.normals[3 * j + 1], v.normals[3 * j + 2]};
+      const double dot = v3dot(nrm, L.u);
+      const double sdotcptsq = (1.0 - dot * dot) * cptsq;
+      const double ds = dot * sinpt;
+      const double A0 = dmax(0.0, cta - fabs(ds));
+      if (!fam_alive || sdotcptsq <= A0 * A0) continue;
+      for (int side = 0; side < 2; ++side) {
+        const double Aside = side == 0 ? dmax(0.0, cta - ds) : dmax(0.0, cta + ds);
+        if (!(sdotcptsq > Aside * Aside)) continue;
+        const double cc = dclamp(side == 0 ? dot : -dot, -1.0, 1.0);
+        if (!(Q > 0.)) {
+          if (Qprime == -1) {
+            cospt = sqrt(cptsq);
+            const double tmp2 = cospt * sinpt;
+            if (tmp2 > 0) Qprime = wl3 / tmp2;
+            else Qprime = (sinpt > 0.5 && xsf) ? -2.0 : 0.0;
+          }
+          if (Qprime > 0.) Q = Qprime * xsf;
+        }
+        double xs;
+        if (Q > 0.) {
+          const double sc2 = sqrt(1.0 - cc * cc);
+          double ci;
+          if (!gos_circle_fast_ok(v, cc, sc2, sinpt, cospt, ci)) {
+            if constexpr (AllowSlow) {
+              ci = gos_circle_slow(v, cc, sc2, sinpt, cospt);
+            } else {
+              L.slow = true;
+              fam_alive = false;
+              break;
+            }
+          }
+          xs = Q * ci;
+        } else {
+          xs = Qprime ? __builtin_huge_val() : 0.0;
+        }
+        if (xs) {
+          const double cm = xsoffset + (xssum += xs);
+          commul_last = cm;
+          if constexpr (Select) {
+            const bool hit = n_select < 5 ? (cm > target) : !(cm < target);
+            if (hit || L.count == n_select - 1) {
+              L.found = true;
+              L.sel_inv2d_r = i2;
+              L.sel_normal = side == 0 ? V3{-nrm.x, -nrm.y, -nrm.z} : nrm;
+              fam_alive = false;
+              break;
+            }
+          }
+          L.count++;
+        }
+      }
+    }
+    if (act && !fam_alive) alive = false;
+  }
+  L.total = commul_last;
+}
+
+// per-lane set-up of a neutron (NCSCBragg.cc:233-259): returns false when
+// the neutron cannot reflect (below threshold, zero wavelength)
+NCD bool scb_lane_setup(const SCBView& v, double E, double dx, double dy, double dz, ScbLane& L) {
+  L.nf = 0;
+  if (!(E > v.threshold)) return false;
+  const double ekin = scb_round(E);
+  const double wl = ekin ? sqrt(kWl2Ekin / ekin) : __builtin_huge_val();
+  if (wl == 0) return false;
+  L.wl = wl;
+  L.u = v3normalise(V3{dx, dy, dz});
+  const double inv2dcut = (1.0 - 2 * 2.220446049250313080847263336181640625e-16) / wl;
+  L.nf = lower_bound_d(v.inv2d, 0, v.nfam, inv2dcut);
+  return true;
+}
+
+}  // namespace ncamd

@@ -239,3 +239,26 @@ def cone_dirs(n, axis, half_angle_rad, seed=3):
     e2 = np.cross(axis, e1)
     return (ct[:, None] * axis[None, :] + (st * np.cos(ph))[:, None] * e1[None, :]
             + (st * np.sin(ph))[:, None] * e2[None, :])
+
+
+def random_rotation(seed=4):
+    """Uniformly random proper rotation (3x3, det +1) from a seeded Gaussian
+    matrix (QR with sign fix)."""
+    rng = np.random.default_rng(seed)
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))[None, :]
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def random_unit_vector(seed=5):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=3)
+    return v / np.linalg.norm(v)
+
+
+def wavelengths_uniform(n, wl_lo, wl_hi, seed=6):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(wl_lo, wl_hi, size=n)
+

@@ -2579,9 +2579,9 @@ static double origv_find(const origv_t* v, int64_t n, uint64_t key) {
 
 typedef struct { int dummy; } scb_dummy_t;
 
-orc_proc* orc_create_scbragg(double a, double b, double c, double v0n, int nfam, const double* dsp,
+orc_proc* orc_create_scbragg(const double* reci, double v0n, int nfam, const double* dsp,
                              const double* fsq, const int64_t* mem_off, const int* mem_hkl,
-                             const double* cry2lab, double mos_fwhm, double prec);
+                             const double* cry2lab, double mos_fwhm, double prec, double ntrunc);
 int orc_scb_info(const orc_proc* p, int* nfam, int64_t* nnormals, double* threshold, double* gosdbl);
 void orc_scb_tables(const orc_proc* p, double* inv2d, double* xsfact, int64_t* off, double* normals,
                     double* lt1, double* lt2);
@@ -2599,11 +2599,10 @@ static void scb_free(orc_proc* p) {
   free(p->u.scb);
 }
 
-orc_proc* orc_create_scbragg(double a, double b, double c, double v0n, int nfam, const double* dsp,
+orc_proc* orc_create_scbragg(const double* L, double v0n, int nfam, const double* dsp,
                              const double* fsq, const int64_t* mem_off, const int* mem_hkl,
-                             const double* R, double mos_fwhm, double prec) {
+                             const double* R, double mos_fwhm, double prec, double ntrunc_in) {
   const double two30 = 1073741824.0;
-  const double rl0 = K_2PI / a, rl1 = K_2PI / b, rl2 = K_2PI / c;
   const int64_t nplanes = mem_off[nfam];
   scbkey_t* keys = (scbkey_t*)malloc(sizeof(scbkey_t) * (size_t)(nplanes > 0 ? nplanes : 1));
   v3_t* pn = (v3_t*)malloc(sizeof(v3_t) * (size_t)(nplanes > 0 ? nplanes : 1));
@@ -2615,7 +2614,8 @@ orc_proc* orc_create_scbragg(double a, double b, double c, double v0n, int nfam,
   for (int f = 0; f < nfam; ++f) {
     for (int64_t m = mem_off[f]; m < mem_off[f + 1]; ++m, ++q) {
       const double h = mem_hkl[3 * m], k = mem_hkl[3 * m + 1], l = mem_hkl[3 * m + 2];
-      v3_t v = { rl0 * h + 0.0 * k + 0.0 * l, 0.0 * h + rl1 * k + 0.0 * l, 0.0 * h + 0.0 * k + rl2 * l };
+      /* reciprocal lattice (row-major) times (h,k,l), RotMatrix::operator* order */
+      v3_t v = { L[0] * h + L[1] * k + L[2] * l, L[3] * h + L[4] * k + L[5] * l, L[6] * h + L[7] * k + L[8] * l };
       pn[q] = v3normalise(v);
       if (dsp[f] > maxd) maxd = dsp[f];
       const uint64_t kd = (uint64_t)(dsp[f] * two30 + 0.5), kf = (uint64_t)(fsq[f] * two30 + 0.5);
@@ -2660,7 +2660,8 @@ orc_proc* orc_create_scbragg(double a, double b, double c, double v0n, int nfam,
   free(keys); free(pn); free(od); free(of);
   /* GaussMos (NCGaussMos.cc:38-75), estimateNTruncFromPrec (NCGaussOnSphere.cc:510-547) */
   double ntrunc;
-  if (prec == 0) ntrunc = 8.0;
+  if (ntrunc_in != 0) ntrunc = ntrunc_in;
+  else if (prec == 0) ntrunc = 8.0;
   else if (prec >= 1.0) ntrunc = 3.0;
   else ntrunc = dclamp(1.1 * sqrt(-2.0 * log(dmax(1e-300, prec))), 3.0, 8.0);
   const double sigma = mos_fwhm * K_FWHM2SIGMA;

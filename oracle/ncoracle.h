@@ -102,12 +102,13 @@ int64_t orc_sample_dir_many(const orc_proc*, uint64_t seed, uint32_t stream,
                             const double* dirs, int64_t n, double* out_ekin,
                             double* out_dirs, int nthreads);
 
-/* SCBragg: families (d, |F|^2 and explicit hkl members, in hkl-list order) of
-   a cell with 90-degree angles; cry2lab row-major 3x3; mosaicity FWHM [rad];
-   prec as NCrystal's mosprec */
-orc_proc* orc_create_scbragg(double a, double b, double c, double v0_times_natoms, int nfam,
+/* SCBragg: families (d, |F|^2 and explicit hkl members, in hkl-list order);
+   reciprocal lattice and cry2lab row-major 3x3; mosaicity FWHM [rad]; prec as
+   NCrystal's mosprec; ntrunc 0 = derived from prec */
+orc_proc* orc_create_scbragg(const double* reci_lattice, double v0_times_natoms, int nfam,
                              const double* dsp, const double* fsq, const int64_t* mem_off,
-                             const int* mem_hkl, const double* cry2lab, double mos_fwhm, double prec);
+                             const int* mem_hkl, const double* cry2lab, double mos_fwhm, double prec,
+                             double ntrunc);
 int orc_scb_info(const orc_proc*, int* nfam, int64_t* nnormals, double* threshold, double* gos15);
 void orc_scb_tables(const orc_proc*, double* inv2d, double* xsfact, int64_t* off, double* normals,
                     double* lt_evalcosx /*2*nlt*/, double* lt_sofcosd /*2*nlt*/);

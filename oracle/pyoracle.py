@@ -62,8 +62,8 @@ def lib():
                                                                           ctypes.c_double, ctypes.c_int, _ip, _ip,
                                                                           _dp, _dp, _i64p, _ip, ctypes.c_int64]
         L.orc_hkl_orthogonal_members.restype = ctypes.c_int
-        L.orc_create_scbragg.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int, _dp, _dp, _i64p, _ip, _dp,
-                                                                  ctypes.c_double, ctypes.c_double]
+        L.orc_create_scbragg.argtypes = [_dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _i64p, _ip, _dp,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_create_scbragg.restype = _vp
         L.orc_scb_info.argtypes = [_vp, _ip, _i64p, _dp, _dp]
         L.orc_scb_tables.argtypes = [_vp, _dp, _dp, _i64p, _dp, _dp, _dp]
@@ -309,12 +309,20 @@ class SCB(Proc):
                     lt_sofcosd=lt2.reshape(-1, 2))
 
 
-def scbragg(a, b, c, v0_times_natoms, dsp, fsq, mem_off, mem_hkl, cry2lab, mos_fwhm, prec=1e-3):
+def reci_lattice_orthogonal(a, b, c):
+    """diag(2pi/a, 2pi/b, 2pi/c), row-major"""
+    k = 6.283185307179586
+    return np.array([k / a, 0.0, 0.0, 0.0, k / b, 0.0, 0.0, 0.0, k / c])
+
+
+def scbragg(a, b, c, v0_times_natoms, dsp, fsq, mem_off, mem_hkl, cry2lab, mos_fwhm, prec=1e-3, ntrunc=0.0,
+            reci=None):
     d = np.ascontiguousarray(dsp, dtype=np.float64)
     f = np.ascontiguousarray(fsq, dtype=np.float64)
     mo = np.ascontiguousarray(mem_off, dtype=np.int64)
     mh = np.ascontiguousarray(mem_hkl, dtype=np.int32).ravel()
     R = np.ascontiguousarray(cry2lab, dtype=np.float64).ravel()
-    ptr = lib().orc_create_scbragg(a, b, c, v0_times_natoms, d.size, _d(d), _d(f), mo.ctypes.data_as(_i64p),
-                                   _i(mh), _d(R), mos_fwhm, prec)
-    return SCB(ptr)
+    L = np.ascontiguousarray(reci_lattice_orthogonal(a, b, c) if reci is None else reci, dtype=np.float64).ravel()
+    ptr = lib().orc_create_scbragg(_d(L), v0_times_natoms, d.size, _d(d), _d(f), mo.ctypes.data_as(_i64p),
+                                   _i(mh), _d(R), mos_fwhm, prec, ntrunc)
+    return SCB(ptr, keep=(L,))
