@@ -43,7 +43,7 @@ def parse():
                     help="also run one device-to-device copy of the output size (counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["sab", "scbragg"], default="sab")
+    ap.add_argument("--workload", choices=["sab", "scbragg", "powderbragg"], default="sab")
     return ap.parse_args()
 
 
@@ -188,6 +188,94 @@ def run_scbragg(args):
     print(json.dumps(out))
 
 
+def cpu_baseline_pb(v0n, dsp, fm, seconds, threads):
+    """CPU checker timed on a bounded sample of the PowderBragg workload:
+    cross section + sampling of 2e6 neutrons, repeated for about `seconds`."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    from ncrystal_amd import synth
+    pyoracle.build()
+    o = pyoracle.powderbragg(v0n, dsp, fm)
+    batch = 2_000_000
+    e = synth.energies_logu(batch, 1e-4, 1.0, seed=2)
+    done, k = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        o.xs(e, nthreads=threads)
+        o.sample_iso(e, seed=5, stream=k, nthreads=threads)
+        done += batch
+        k += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    return dict(value=done / dt, unit="neutrons/s (xs + sample)", cores=threads, kind="port",
+                sample=f"{done} neutrons ({k} batches of {batch}, E log-uniform 1e-4..1 eV), "
+                       f"cross section + sampling, {threads} threads, {dt:.1f} s")
+
+
+def run_powderbragg(args):
+    import numpy as np
+    from ncrystal_amd import _capi as C
+    from ncrystal_amd import core, synth
+    n = args.n
+    cr = synth.ortho_random(seed=1, dcutoff=0.1)
+    sp = [(s.csl, core.debye_msd(s.debye_temp, cr.temperature, s.mass), s.positions) for s in cr.species]
+    hkl, mult, dsp, fsq = core.hkl_orthogonal(cr.a, cr.b, cr.c, sp, cr.dcutoff)
+    v0n = cr.volume * cr.n_atoms
+    pb = core.create_powderbragg(v0n, dsp, fsq * mult)
+    nplanes = len(core.powderbragg_tables(pb)[0])
+    stream = C.Stream()
+    ekin, xs, oe, om = C.DeviceArray(n), C.DeviceArray(n), C.DeviceArray(n), C.DeviceArray(n)
+    C.call("ncamd_dev_fill_loguniform", ekin.ptr, n, 1e-4, 1.0, 1, stream.ptr)
+    stream.sync()
+    pb.setRandomSeed(9)
+    ev = [C.Event() for _ in range(4)]
+    for _ in range(args.warmup):
+        pb.dev_crossSectionIsotropic(ekin.ptr, n, xs.ptr, stream=stream.ptr)
+        pb.dev_sampleScatterIsotropic(ekin.ptr, n, oe.ptr, om.ptr, stream=stream.ptr)
+    stream.sync()
+    t64 = t32 = ts = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ev[0].record(stream)
+        pb.dev_crossSectionIsotropic(ekin.ptr, n, xs.ptr, stream=stream.ptr)
+        ev[1].record(stream)
+        pb.dev_crossSectionIsotropic(ekin.ptr, n, xs.ptr, stream=stream.ptr, fp32=True)
+        ev[2].record(stream)
+        pb.dev_sampleScatterIsotropic(ekin.ptr, n, oe.ptr, om.ptr, stream=stream.ptr)
+        ev[3].record(stream)
+        stream.sync()
+        t64 += ev[0].elapsed_ms(ev[1])
+        t32 += ev[1].elapsed_ms(ev[2])
+        ts += ev[2].elapsed_ms(ev[3])
+    wall = time.perf_counter() - t0
+    t64, t32, ts = t64 / args.steps, t32 / args.steps, ts / args.steps
+    assert pb.dev_check_errors(stream=stream.ptr) == 0
+    mu = om.numpy(1000000)
+    assert np.all(np.abs(mu) <= 1.0), "invalid output"
+    ach = 24 * n / (ts * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline_pb(v0n, dsp, fsq * mult, args.cpu_seconds, threads)
+    out = {
+        "metric": "PowderBragg neutrons/s on 1 MI355X (FP64 xs + FP32-table xs + scatter sampling)",
+        "value": n * args.steps / wall, "unit": "neutrons/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "PowderBragg, orthorhombic cell seed 1 (a,b,c=%.3f,%.3f,%.3f Aa, 24 atoms), "
+                               "dcutoff 0.1 Aa, E log-uniform 1e-4..1 eV" % (cr.a, cr.b, cr.c),
+                   "neutrons_per_step": n, "families": int(len(dsp)), "planes": nplanes,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_sample_iso",
+                     "kernel_ms": ts, "xs_fp64_ms": t64, "xs_fp32_ms": t32,
+                     "xs_fp64_GBs": 16 * n / (t64 * 1e-3) / 1e9, "xs_fp32_GBs": 16 * n / (t32 * 1e-3) / 1e9},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -195,6 +283,8 @@ def main():
         sys.exit(2)
     if args.workload == "scbragg":
         return run_scbragg(args)
+    if args.workload == "powderbragg":
+        return run_powderbragg(args)
     import numpy as np
     from ncrystal_amd import _capi as C
     from ncrystal_amd import core, synth

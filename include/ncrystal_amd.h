@@ -147,6 +147,27 @@ int ncamd_scbragg_info(ncamd_process_t, int64_t* nfam, int64_t* nnormals, double
 int ncamd_scbragg_tables(ncamd_process_t, double* inv2d, double* xsfact, int64_t* off, double* normals,
                          double* lt_evalcosx, double* lt_sofcosd);
 
+/* ---------- exact envelope sampler of S(alpha,beta) (no counterpart in
+ * NCrystal; an alternative to the plain sampler of NCSABSampler.cc that
+ * samples the interpolated table density itself) ----------
+ * Switches an S(alpha,beta) process to rejection sampling in kinematic
+ * coordinates (w = sqrt(beta+eps)/wmax, kappa = position between the
+ * kinematic alpha limits) with a box envelope per energy interval:
+ * nintervals log-spaced intervals over [emin, emax) (0 = the process's
+ * energy grid range), nbox_w x nbox_k boxes, at most maxtries proposals per
+ * neutron.  Neutrons outside [emin, emax) or without an accepted proposal
+ * use the plain sampler (from the first draw of their stream).  The
+ * accepted (alpha, beta) follow the interpolated table density exactly. */
+int ncamd_sab_use_envelope_sampler(ncamd_process_t, int nintervals, double emin, double emax, int nbox_w,
+                                   int nbox_k, int maxtries);
+int ncamd_sab_use_plain_sampler(ncamd_process_t);
+/* interval edges [nintervals+1] and envelope values [nintervals*nbox_w*nbox_k] */
+int ncamd_sab_envelope_tables(ncamd_process_t, double* node_e, double* env);
+/* raw (alpha, beta) of the envelope sampler; returns the number of neutrons
+ * that fell back to the plain sampler (synchronises the stream) */
+int64_t ncamd_dev_sab_sample_alphabeta_envelope(ncamd_process_t, const double* d_ekin, unsigned long n,
+                                                double* d_alpha, double* d_beta, void* stream);
+
 /* ---------- RNG (ncrystal.h:1067-1087) ----------
  * Each process handle owns a counter-based stream: (seed, call counter).
  * Every sampling call consumes one call-counter value; neutron k of a call
@@ -212,6 +233,32 @@ void ncamd_dev_sab_sample_alphabeta(ncamd_process_t, const double* d_ekin, unsig
                                     double* d_alpha, double* d_beta, void* stream);
 /* number of neutrons whose sampling failed since the last check (syncs stream) */
 int64_t ncamd_dev_check_errors(ncamd_process_t, void* stream);
+
+/* ---------- many materials in one launch (no counterpart in NCrystal, whose
+ * C API evaluates one material per call) ----------
+ * A batch references nmat processes (isotropic: PowderBragg, ElInc,
+ * S(alpha,beta) or compositions of them) and the number of neutrons of each.
+ * Inputs/outputs are concatenated per material in batch order (material i
+ * owns entries [sum(counts[0..i)), +counts[i])).  One call runs all materials
+ * in one persistent launch (replayed from a captured HIP graph) and returns
+ * exactly what one ncamd_dev_* call per material would return: every
+ * material keeps its own seed and call counter, and a sampling call advances
+ * each counter by one. */
+typedef struct { void* internal; } ncamd_batch_t;
+ncamd_batch_t ncamd_batch_create(int nmat, const ncamd_process_t* procs, const int64_t* counts);
+void ncamd_batch_destroy(ncamd_batch_t*);
+int64_t ncamd_batch_total(ncamd_batch_t);
+void ncamd_dev_batch_crosssection(ncamd_batch_t, const double* d_ekin, double* d_xs, void* stream);
+void ncamd_dev_batch_samplescatterisotropic(ncamd_batch_t, const double* d_ekin, double* d_results_ekin,
+                                            double* d_results_mu, void* stream);
+int64_t ncamd_dev_batch_check_errors(ncamd_batch_t, void* stream);
+/* launch parameters: neutrons per tile, resident workgroups, tile order
+ * (0: material order, 1: most expensive materials first) */
+void ncamd_batch_get_tuning(ncamd_batch_t, int* tile, int* nblocks, int* ordering);
+void ncamd_batch_set_tuning(ncamd_batch_t, int tile, int nblocks, int ordering);
+/* times the candidate tunings on these inputs (outputs are scratch, RNG
+ * states unchanged), keeps the fastest and returns its time [ms] */
+double ncamd_dev_batch_autotune(ncamd_batch_t, const double* d_ekin, double* d_o1, double* d_o2, void* stream);
 
 /* ---------- device memory, streams and events (plumbing for callers that
  * do not bring their own HIP runtime objects; no counterpart in NCrystal) ---- */

@@ -52,6 +52,10 @@ class Handle(ctypes.Structure):
     _fields_ = [("internal", ctypes.c_void_p)]
 
 
+class BatchHandle(ctypes.Structure):
+    _fields_ = [("internal", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -117,6 +121,20 @@ def lib():
         "ncamd_dev_crosssection_many_dirs": (None, [Handle, _vp, _vp, _ul, _vp, _vp]),
         "ncamd_dev_sab_sample_alphabeta": (None, [Handle, _vp, _ul, _vp, _vp, _vp]),
         "ncamd_dev_check_errors": (ctypes.c_int64, [Handle, _vp]),
+        "ncamd_sab_use_envelope_sampler": (ctypes.c_int, [Handle, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                          ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "ncamd_sab_use_plain_sampler": (ctypes.c_int, [Handle]),
+        "ncamd_sab_envelope_tables": (ctypes.c_int, [Handle, _dp, _dp]),
+        "ncamd_dev_sab_sample_alphabeta_envelope": (ctypes.c_int64, [Handle, _vp, _ul, _vp, _vp, _vp]),
+        "ncamd_batch_create": (BatchHandle, [ctypes.c_int, ctypes.POINTER(Handle), _i64p]),
+        "ncamd_batch_destroy": (None, [ctypes.POINTER(BatchHandle)]),
+        "ncamd_batch_total": (ctypes.c_int64, [BatchHandle]),
+        "ncamd_dev_batch_crosssection": (None, [BatchHandle, _vp, _vp, _vp]),
+        "ncamd_dev_batch_samplescatterisotropic": (None, [BatchHandle, _vp, _vp, _vp, _vp]),
+        "ncamd_dev_batch_check_errors": (ctypes.c_int64, [BatchHandle, _vp]),
+        "ncamd_batch_get_tuning": (None, [BatchHandle, _ip, _ip, _ip]),
+        "ncamd_batch_set_tuning": (None, [BatchHandle, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "ncamd_dev_batch_autotune": (ctypes.c_double, [BatchHandle, _vp, _vp, _vp, _vp]),
         "ncamd_dev_malloc": (_vp, [ctypes.c_uint64]),
         "ncamd_dev_free": (None, [_vp]),
         "ncamd_memcpy_htod": (None, [_vp, _vp, ctypes.c_uint64]),

@@ -300,6 +300,25 @@ class SABScatter(Scatter):
     def dev_sample_alphabeta(self, d_ekin, n, d_a, d_b, stream=0):
         C.call("ncamd_dev_sab_sample_alphabeta", self._h, d_ekin, n, d_a, d_b, stream)
 
+    # exact envelope sampler (kinematic-box rejection) as the sampling mode
+    def use_envelope_sampler(self, nintervals=512, emin=0.0, emax=0.0, nbox_w=64, nbox_k=32, maxtries=64):
+        C.call("ncamd_sab_use_envelope_sampler", self._h, int(nintervals), float(emin), float(emax), int(nbox_w),
+               int(nbox_k), int(maxtries))
+        self._kb = (int(nintervals), int(nbox_w) * int(nbox_k))
+
+    def use_plain_sampler(self):
+        C.call("ncamd_sab_use_plain_sampler", self._h)
+
+    def envelope_tables(self):
+        nn, nbox = self._kb
+        node = np.empty(nn + 1)
+        env = np.empty(nn * nbox)
+        C.call("ncamd_sab_envelope_tables", self._h, C.dptr(node), C.dptr(env))
+        return node, env.reshape(nn, nbox)
+
+    def dev_sample_alphabeta_envelope(self, d_ekin, n, d_a, d_b, stream=0):
+        return C.call("ncamd_dev_sab_sample_alphabeta_envelope", self._h, d_ekin, n, d_a, d_b, stream)
+
 
 def create_sabscatter(alpha, beta, sab, temperature, mass_amu, sigma_bound, suggested_emax=0.0, egrid=None):
     a, b = C.as_f64(alpha), C.as_f64(beta)
@@ -316,3 +335,51 @@ def create_composition(procs, scales):
     sc = C.as_f64(scales)
     h = C.call("ncamd_create_composition", len(procs), arr, C.dptr(sc))
     return Scatter(h, name="ProcComposition", keep=tuple(procs))
+
+
+class Batch:
+    """Many materials evaluated in one persistent launch (device pointers).
+    Inputs and outputs are concatenated per material in the given order."""
+
+    def __init__(self, procs, counts):
+        arr = (C.Handle * len(procs))(*[p.handle for p in procs])
+        cnt = np.ascontiguousarray(counts, dtype=np.int64)
+        h = C.call("ncamd_batch_create", len(procs), arr, C.i64ptr(cnt))
+        if not h.internal:
+            C.check()
+            raise C.NCAMDError("batch creation failed")
+        self._h = h
+        self._procs = tuple(procs)
+        self.counts = cnt.copy()
+        self.offsets = np.concatenate([[0], np.cumsum(cnt)])
+
+    def __del__(self):
+        try:
+            if self._h is not None and self._h.internal:
+                C.lib().ncamd_batch_destroy(ctypes.byref(self._h))
+        except Exception:
+            pass
+
+    @property
+    def total(self):
+        return C.call("ncamd_batch_total", self._h)
+
+    def dev_crossSection(self, d_ekin, d_xs, stream=0):
+        C.call("ncamd_dev_batch_crosssection", self._h, d_ekin, d_xs, stream)
+
+    def dev_sampleScatterIsotropic(self, d_ekin, d_oe, d_omu, stream=0):
+        C.call("ncamd_dev_batch_samplescatterisotropic", self._h, d_ekin, d_oe, d_omu, stream)
+
+    def dev_check_errors(self, stream=0):
+        return C.call("ncamd_dev_batch_check_errors", self._h, stream)
+
+    def tuning(self):
+        t, b, o = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        C.call("ncamd_batch_get_tuning", self._h, ctypes.byref(t), ctypes.byref(b), ctypes.byref(o))
+        return dict(tile=t.value, nblocks=b.value, ordering=o.value)
+
+    def set_tuning(self, tile, nblocks, ordering):
+        C.call("ncamd_batch_set_tuning", self._h, int(tile), int(nblocks), int(ordering))
+
+    def dev_autotune(self, d_ekin, d_o1, d_o2, stream=0):
+        return C.call("ncamd_dev_batch_autotune", self._h, d_ekin, d_o1, d_o2, stream)

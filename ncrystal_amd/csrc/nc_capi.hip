@@ -205,8 +205,11 @@ struct Proc {
   DevBuf d_err;
   // staging for the host-pointer API
   DevBuf st_in, st_in2, st_out1, st_out2;
-  // high-energy queue of the S(alpha,beta)-only sampling path
+  // high-energy queue of the S(alpha,beta)-only sampling path (also the
+  // Romberg queue of SCBragg)
   DevBuf q_idx, q_count;
+  // SCBragg batch ordering
+  DevBuf scb_keys, scb_perm, scb_hist, scb_cursor;
   // component data
   PBView pb{};
   ElIncView ei{};
@@ -217,6 +220,13 @@ struct Proc {
   bool has_fp32 = false;
   // S(alpha,beta) host copies for read-back
   int sab_na = 0, sab_nb = 0;
+  std::vector<double> sab_alpha_h, sab_beta_h;
+  // kinematic-box envelope sampler (optional sampling mode of an S(alpha,beta) process)
+  bool use_kb = false;
+  int kb_maxtries = 64;
+  KBView kb{};
+  std::vector<std::unique_ptr<DevBuf>> kb_bufs;
+  std::vector<double> kb_node_h, kb_env_h;
   std::vector<double> sab_egrid_h, sab_xs_h;
   // single-crystal Bragg (host copies of the device tables for read-back)
   SCBView scb{};
@@ -303,6 +313,49 @@ __global__ void k_fp32_kahan_cumul(const double* __restrict__ contrib, int n, fl
   }
 }
 
+// search guides of a PowderBragg table: IEEE-bit buckets of the energy
+// axis for the threshold search and equal-value buckets of the cumulative
+// sum for the plane pick.  Each bucket stores the search answer at its lower
+// edge, so a bracket [bucket, next bucket] (widened by one bucket for the
+// rounding of target*scale) always contains the plain binary-search answer.
+void pb_guides(Proc* p, const std::vector<double>& e2d, const std::vector<double>& fdm) {
+  const int n = (int)e2d.size();
+  auto bits = [](double x) { unsigned long long u; std::memcpy(&u, &x, 8); return u; };
+  auto fromb = [](unsigned long long u) { double x; std::memcpy(&x, &u, 8); return x; };
+  // about 4 buckets per plane over the [e2d[0], e2d[n-1]] range, at most 2^20
+  int shift = 52, m = 0;
+  long long base = 0, nbkt = 1;
+  for (m = 20; m >= 0; --m) {
+    shift = 52 - m;
+    base = (long long)(bits(e2d.front()) >> shift);
+    nbkt = (long long)(bits(e2d.back()) >> shift) - base + 1;
+    if (nbkt <= std::max<long long>(64, 4LL * n) && nbkt <= (1 << 20)) break;
+  }
+  if (m < 0) { m = 0; shift = 52; base = (long long)(bits(e2d.front()) >> shift);
+               nbkt = (long long)(bits(e2d.back()) >> shift) - base + 1; }
+  std::vector<int> eb((size_t)nbkt + 1);
+  for (long long b = 0; b <= nbkt; ++b) {
+    const double edge = fromb((unsigned long long)(base + b) << shift);
+    eb[(size_t)b] = (int)(std::upper_bound(e2d.begin(), e2d.end(), edge) - e2d.begin());
+  }
+  eb[(size_t)nbkt] = n;
+  const int cn = (int)std::min<long long>(1 << 20, std::max<long long>(64, 4LL * n));
+  const double scale = (double)cn / fdm.back();
+  std::vector<int> cg((size_t)cn + 1);
+  for (int g = 0; g < cn; ++g)
+    cg[(size_t)g] = (int)(std::lower_bound(fdm.begin(), fdm.end(), (double)g / scale) - fdm.begin());
+  cg[(size_t)cn] = n;
+  DevBuf* b1 = p->newbuf(); b1->upload(eb.data(), eb.size());
+  DevBuf* b2 = p->newbuf(); b2->upload(cg.data(), cg.size());
+  p->pb.e_bucket = b1->as<int>();
+  p->pb.e_base = base;
+  p->pb.e_shift = shift;
+  p->pb.e_nbkt = (int)nbkt;
+  p->pb.c_guide = b2->as<int>();
+  p->pb.c_n = cn;
+  p->pb.c_scale = scale;
+}
+
 Proc* create_pb(double v0n, const double* dsp, const double* fsqm, int64_t n, bool fp32) {
   ensure_device();
   if (!(v0n > 0)) raise("BadInput", "v0_times_natoms is not a positive number.");
@@ -348,6 +401,7 @@ Proc* create_pb(double v0n, const double* dsp, const double* fsqm, int64_t n, bo
   p->pb.e2d = be->as<double>();
   p->pb.fdm = bf->as<double>();
   p->pb.fdm32 = nullptr;
+  if (!fdm.empty()) pb_guides(p.get(), e2d, fdm);
   p->pb_e2d_h = e2d;
   p->pb_fdm_h = fdm;
   if (fp32 && !fdm.empty()) {
@@ -439,6 +493,8 @@ Proc* create_sab(int na, int nb, const double* alpha, const double* beta, const 
     s.fg_ca = A / s.kT;
   }
   s.egrid_margin = 1.05;
+  p->sab_alpha_h.assign(alpha, alpha + na);
+  p->sab_beta_h.assign(beta, beta + nb);
   DevBuf* ba = p->newbuf(); ba->upload(alpha, (size_t)na);
   DevBuf* bb = p->newbuf(); bb->upload(beta, (size_t)nb);
   DevBuf* bs = p->newbuf(); bs->upload(sab, (size_t)na * nb);
@@ -1054,23 +1110,47 @@ Proc* create_scb(double v0n, int64_t nfam, const double* dsp, const double* fsq,
   std::unique_ptr<Proc> p(new Proc());
   p->kind = kKindSCB;
   std::vector<int> off32(off.begin(), off.end());
+  // per demi-normal scan records (normal, rounded 1/(2d) of its family) and family index
+  std::vector<double> nrec(4 * (size_t)(nplanes + kScbPad), 0.0);
+  std::vector<float> nrec32(4 * (size_t)(nplanes + kScbPad), 0.0f);
+  std::vector<int> famidx((size_t)nplanes);
+  for (int f = 0; f < nf; ++f)
+    for (int64_t j = off[f]; j < off[f + 1]; ++j) {
+      nrec[4 * j] = normals[3 * j];
+      nrec[4 * j + 1] = normals[3 * j + 1];
+      nrec[4 * j + 2] = normals[3 * j + 2];
+      nrec[4 * j + 3] = inv2d_r[f];
+      famidx[j] = f;
+    }
+  for (size_t j = 0; j < nrec.size(); ++j) nrec32[j] = (float)nrec[j];
+  for (int64_t j = nplanes; j < nplanes + kScbPad; ++j) {  // padding records never pass the pre-filter
+    nrec[4 * j + 3] = 3.0e38;
+    nrec32[4 * j + 3] = 3.0e38f;
+  }
   std::vector<double> lt1, lt2;
   for (size_t i = 0; i < g.evalcosx.y.size(); ++i) { lt1.push_back(g.evalcosx.y[i]); lt1.push_back(g.evalcosx.y2[i]); }
   for (size_t i = 0; i < g.sofcosd.y.size(); ++i) { lt2.push_back(g.sofcosd.y[i]); lt2.push_back(g.sofcosd.y2[i]); }
   DevBuf* b_inv2d = p->newbuf(); b_inv2d->upload(inv2d.data(), inv2d.size());
-  DevBuf* b_inv2dr = p->newbuf(); b_inv2dr->upload(inv2d_r.data(), inv2d_r.size());
   DevBuf* b_xsf = p->newbuf(); b_xsf->upload(xsf.data(), xsf.size());
   DevBuf* b_off = p->newbuf(); b_off->upload(off32.data(), off32.size());
-  DevBuf* b_nrm = p->newbuf(); b_nrm->upload(normals.data(), normals.size());
+  DevBuf* b_nrec = p->newbuf(); b_nrec->upload(nrec.data(), nrec.size());
+  DevBuf* b_nrec32 = p->newbuf(); b_nrec32->upload(nrec32.data(), nrec32.size());
+  DevBuf* b_fam = p->newbuf(); b_fam->upload(famidx.data(), famidx.size());
   DevBuf* b_lt1 = p->newbuf(); b_lt1->upload(lt1.data(), lt1.size());
   DevBuf* b_lt2 = p->newbuf(); b_lt2->upload(lt2.data(), lt2.size());
   SCBView& v = p->scb;
   v.nfam = nf;
+  v.nnormals = (int)nplanes;
   v.inv2d = b_inv2d->as<double>();
-  v.inv2d_r = b_inv2dr->as<double>();
   v.xsf = b_xsf->as<double>();
   v.off = b_off->as<int>();
-  v.normals = b_nrm->as<double>();
+  v.nrec = b_nrec->as<double4>();
+  v.nrec32 = b_nrec32->as<float4>();
+  v.famidx = b_fam->as<int>();
+  // |sin(phi) - sin(theta)| <= |phi - theta| < truncation angle for every
+  // contributing normal; the margins cover the rounding of cos(truncation)
+  // (~1e-14) and of the FP32 records and products of the pre-filter (< 3e-6)
+  v.band32 = (float)(trunc * (1.0 + 1e-6) + 1e-5);
   {
     const double w = maxd * 2.0, w2 = w * w;
     v.threshold = w2 ? kWl2Ekin / w2 : __builtin_huge_val();
@@ -1089,6 +1169,238 @@ Proc* create_scb(double v0n, int64_t nfam, const double* dsp, const double* fsq,
   std::copy(gd, gd + 15, p->scb_gos);
   init_errbuf(p.get());
   return p.release();
+}
+
+// ------------------------------------------------------- material batches ---
+// Many materials in one persistent launch (nc_batch.hip).  The batch holds
+// references to its materials, their device views, the tile list and the
+// tuned launch parameters; each call uploads the materials' (seed, call
+// counter) pairs and replays a captured graph of the reset + persistent
+// kernels.
+constexpr uint32_t kBatchMagic = 0x6e636262U;
+
+struct Batch {
+  uint32_t magic = kBatchMagic;
+  std::vector<Proc*> mats;
+  std::vector<int64_t> counts, offsets;
+  std::vector<double> cost;  // estimated relative cost per neutron
+  int64_t total = 0;
+  DevBuf d_mats, d_rngs, d_tiles, d_next, d_err;
+  std::vector<BatchRng> h_rngs;
+  int tile = 4096, nblocks = 1024, ordering = 1;
+  int ntiles = 0;
+  bool tiles_valid = false;
+  // captured graph of the last (mode, pointers, tuning) combination
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int g_mode = -1, g_tile = 0, g_nblocks = 0, g_ordering = -1;
+  const void *g_in = nullptr, *g_o1 = nullptr, *g_o2 = nullptr;
+  ~Batch() {
+    magic = 0;
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    for (auto* m : mats)
+      if (m && --m->refcount == 0) delete m;
+  }
+};
+
+Batch* extract_batch(ncamd_batch_t h) {
+  Batch* b = static_cast<Batch*>(h.internal);
+  if (!b || b->magic != kBatchMagic) raise("LogicError", "invalid ncamd_batch_t handle");
+  return b;
+}
+
+// per-neutron cost estimate of a material (relative units): sampling an
+// S(alpha,beta) component dominates, plane searches grow with log2(planes)
+double batch_cost(const CompView& v) {
+  double c = 1.0;
+  for (int i = 0; i < v.n; ++i) {
+    if (v.kind[i] == kKindSAB) c += 8.0;
+    if (v.kind[i] == kKindPB) c += 0.1 * std::log2(2.0 + v.pb.n);
+    if (v.kind[i] == kKindElInc) c += 1.0;
+  }
+  return c;
+}
+
+void batch_build_tiles(Batch* b) {
+  std::vector<BatchTile> tiles;
+  std::vector<int> order(b->mats.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  if (b->ordering == 1)  // most expensive materials first (longest-processing-time order)
+    std::stable_sort(order.begin(), order.end(), [b](int x, int y) {
+      return b->cost[x] * (double)b->counts[x] > b->cost[y] * (double)b->counts[y];
+    });
+  for (int m : order)
+    for (int64_t k = 0; k < b->counts[m]; k += b->tile)
+      tiles.push_back(BatchTile{m, k, std::min<int64_t>(b->counts[m], k + b->tile)});
+  if (tiles.size() > (size_t)INT32_MAX) raise("BadInput", "too many tiles");
+  // the captured graph refers to the old tile list
+  if (b->gexec) { NCAMD_HIP(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
+  b->ntiles = (int)tiles.size();
+  b->d_tiles.upload(tiles.data(), tiles.size());
+  b->tiles_valid = true;
+}
+
+void batch_upload_rngs(Batch* b, bool advance, hipStream_t st) {
+  for (size_t i = 0; i < b->mats.size(); ++i) {
+    b->h_rngs[i].seed = b->mats[i]->seed;
+    b->h_rngs[i].stream = advance ? b->mats[i]->callctr++ : b->mats[i]->callctr;
+  }
+  NCAMD_HIP(hipMemcpyAsync(b->d_rngs.p, b->h_rngs.data(), sizeof(BatchRng) * b->h_rngs.size(),
+                           hipMemcpyHostToDevice, st));
+}
+
+// launch through a captured graph (re-captured when the mode, the pointers or
+// the tuning change)
+void batch_run(Batch* b, int mode, const double* in, double* o1, double* o2, hipStream_t st) {
+  if (!b->tiles_valid) batch_build_tiles(b);
+  if (!b->cap_stream) NCAMD_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+  const bool same = b->gexec && b->g_mode == mode && b->g_in == in && b->g_o1 == o1 && b->g_o2 == o2 &&
+                    b->g_tile == b->tile && b->g_nblocks == b->nblocks && b->g_ordering == b->ordering;
+  if (!same) {
+    if (b->gexec) { NCAMD_HIP(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
+    hipGraph_t g = nullptr;
+    NCAMD_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    launch_batch(b->d_mats.as<BatchMat>(), b->d_rngs.as<BatchRng>(), b->d_tiles.as<BatchTile>(), b->ntiles,
+                 b->nblocks, mode, in, o1, o2, b->d_next.as<unsigned>(), b->d_err.as<unsigned long long>(),
+                 b->cap_stream);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipStreamEndCapture(b->cap_stream, &g);
+    NCAMD_HIP(e1);
+    NCAMD_HIP(e2);
+    const hipError_t e3 = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    NCAMD_HIP(e3);
+    b->g_mode = mode; b->g_in = in; b->g_o1 = o1; b->g_o2 = o2;
+    b->g_tile = b->tile; b->g_nblocks = b->nblocks; b->g_ordering = b->ordering;
+  }
+  NCAMD_HIP(hipGraphLaunch(b->gexec, st));
+}
+
+
+// ------------------------------------------- S(alpha,beta) envelope sampler ---
+// Tables of the kinematic-box envelope sampler (device side in nc_kb.hh):
+// interval edges log-spaced in [emin, emax), per interval the coordinate
+// parameters, the envelope of every box (GPU) and its alias table (host,
+// Walker/Vose), plus exact guides for the alpha/beta cell searches.
+DevBuf* kb_newbuf(Proc* p) {
+  p->kb_bufs.emplace_back(new DevBuf());
+  return p->kb_bufs.back().get();
+}
+
+void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int maxtries) {
+  if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
+  const SABView& s = p->sab;
+  if (emax <= 0.0) emax = p->sab_egrid_h.back();
+  if (emin <= 0.0) emin = p->sab_egrid_h.front();
+  if (!(emin > 0.0 && emax > emin)) raise("BadInput", "envelope sampler: need 0 < emin < emax");
+  if (nn < 1 || nn > 1 << 16 || bw < 1 || bk < 1 || (int64_t)bw * bk > 1 << 16 || maxtries < 1)
+    raise("BadInput", "envelope sampler: invalid grid sizes");
+  if (!(p->sab_alpha_h.front() > 0.0)) raise("BadInput", "envelope sampler: alpha grid must be positive");
+  p->kb_bufs.clear();
+  KBView k{};
+  k.nn = nn; k.bw = bw; k.bk = bk; k.nbox = bw * bk;
+  const double lr = std::log(emax / emin) / nn;
+  k.elo = emin; k.ehi = emax; k.lr_inv = 1.0 / lr;
+  std::vector<double> node((size_t)nn + 1);
+  for (int i = 0; i <= nn; ++i) node[(size_t)i] = emin * std::exp(lr * i);
+  node[0] = emin; node[(size_t)nn] = emax;
+  const double bmax = p->sab_beta_h.back();
+  std::vector<double> par(4 * (size_t)nn);
+  for (int i = 0; i < nn; ++i) {
+    const double eps = node[(size_t)i + 1] / s.kT;
+    const double seps = std::sqrt(eps), wmax = std::sqrt(bmax + eps);
+    par[4 * i] = eps; par[4 * i + 1] = seps; par[4 * i + 2] = wmax; par[4 * i + 3] = 8.0 * seps * wmax * wmax * wmax;
+  }
+  DevBuf* bn = kb_newbuf(p); bn->upload(node.data(), node.size());
+  DevBuf* bp = kb_newbuf(p); bp->upload(par.data(), par.size());
+  k.node_e = bn->as<double>();
+  k.node_par = bp->as<double4>();
+  // alpha cell search: IEEE-bit buckets (a few per grid point, at most 2^20)
+  {
+    const auto& a = p->sab_alpha_h;
+    auto bits = [](double x) { unsigned long long u; std::memcpy(&u, &x, 8); return u; };
+    auto fromb = [](unsigned long long u) { double x; std::memcpy(&x, &u, 8); return x; };
+    int shift = 52, m;
+    long long base = 0, nbkt = 1;
+    for (m = 20; m >= 0; --m) {
+      shift = 52 - m;
+      base = (long long)(bits(a.front()) >> shift);
+      nbkt = (long long)(bits(a.back()) >> shift) - base + 1;
+      if (nbkt <= std::max<long long>(64, 8LL * (long long)a.size()) && nbkt <= (1 << 20)) break;
+    }
+    if (m < 0) { shift = 52; base = (long long)(bits(a.front()) >> shift);
+                 nbkt = (long long)(bits(a.back()) >> shift) - base + 1; }
+    std::vector<int> tab((size_t)nbkt + 1);
+    for (long long b = 0; b <= nbkt; ++b)
+      tab[(size_t)b] = (int)(std::upper_bound(a.begin(), a.end(), fromb((unsigned long long)(base + b) << shift)) -
+                             a.begin());
+    tab[(size_t)nbkt] = (int)a.size();
+    DevBuf* bt = kb_newbuf(p); bt->upload(tab.data(), tab.size());
+    k.a_bucket = bt->as<int>(); k.a_base = base; k.a_shift = shift; k.a_nbkt = (int)nbkt;
+  }
+  // beta cell search: equal-width buckets, one bucket of margin on each side
+  {
+    const auto& b = p->sab_beta_h;
+    const int ng = (int)std::min<long long>(1 << 20, std::max<long long>(64, 4LL * (long long)b.size()));
+    const double scale = ng / (b.back() - b.front());
+    std::vector<int> g((size_t)ng + 1);
+    for (int i = 0; i < ng; ++i)
+      g[(size_t)i] = (int)(std::upper_bound(b.begin(), b.end(), b.front() + (double)i / scale) - b.begin());
+    g[(size_t)ng] = (int)b.size();
+    DevBuf* bg = kb_newbuf(p); bg->upload(g.data(), g.size());
+    k.b_guide = bg->as<int>(); k.b_ng = ng; k.b_scale = scale;
+  }
+  // envelopes on the GPU
+  const size_t ntot = (size_t)nn * k.nbox;
+  DevBuf cmax, benv_tmp;
+  cmax.alloc(sizeof(double) * (size_t)(s.na - 1) * (s.nb - 1));
+  DevBuf* benv = kb_newbuf(p); benv->alloc(sizeof(double) * ntot);
+  k.env = benv->as<double>();
+  launch_kb_build(s, k, cmax.as<double>(), benv->as<double>(), 0);
+  NCAMD_HIP(hipGetLastError());
+  std::vector<double> env(ntot);
+  NCAMD_HIP(hipMemcpy(env.data(), benv->p, sizeof(double) * ntot, hipMemcpyDeviceToHost));
+  // alias tables (Walker/Vose) per interval; weights = envelope (equal box areas)
+  std::vector<double> prob(ntot);
+  std::vector<int> alias(ntot);
+  std::vector<double> q((size_t)k.nbox);
+  std::vector<int> small, large;
+  for (int i = 0; i < nn; ++i) {
+    const double* e = env.data() + (size_t)i * k.nbox;
+    double tot = 0.0;
+    for (int j = 0; j < k.nbox; ++j) tot += e[j];
+    double* pr = prob.data() + (size_t)i * k.nbox;
+    int* al = alias.data() + (size_t)i * k.nbox;
+    if (!(tot > 0.0) || !std::isfinite(tot)) {  // nothing to sample: every proposal is rejected -> fallback
+      for (int j = 0; j < k.nbox; ++j) { pr[j] = 1.0; al[j] = j; }
+      continue;
+    }
+    small.clear(); large.clear();
+    for (int j = 0; j < k.nbox; ++j) {
+      q[(size_t)j] = e[j] * k.nbox / tot;
+      (q[(size_t)j] < 1.0 ? small : large).push_back(j);
+    }
+    while (!small.empty() && !large.empty()) {
+      const int sj = small.back(); small.pop_back();
+      const int lj = large.back(); large.pop_back();
+      pr[sj] = q[(size_t)sj];
+      al[sj] = lj;
+      q[(size_t)lj] = (q[(size_t)lj] + q[(size_t)sj]) - 1.0;
+      (q[(size_t)lj] < 1.0 ? small : large).push_back(lj);
+    }
+    for (int j : large) { pr[j] = 1.0; al[j] = j; }
+    for (int j : small) { pr[j] = 1.0; al[j] = j; }
+  }
+  DevBuf* bpr = kb_newbuf(p); bpr->upload(prob.data(), prob.size());
+  DevBuf* bal = kb_newbuf(p); bal->upload(alias.data(), alias.size());
+  k.prob = bpr->as<double>();
+  k.alias = bal->as<int>();
+  p->kb = k;
+  p->kb_maxtries = maxtries;
+  p->kb_node_h = node;
+  p->kb_env_h = std::move(env);
+  p->use_kb = true;
 }
 
 // ---------------------------------------------------- batched evaluation ---
@@ -1120,23 +1432,44 @@ void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat,
       p->q_idx.alloc(need);
     }
     if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
-    launch_sab_sample_iso(single->sab, ekin, n, repeat, p->seed, stream, oE, oMu,
-                          p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
-                          p->q_count.as<unsigned long long>(), st);
+    if (single->use_kb)
+      launch_kb_sample_iso(single->sab, single->kb, ekin, n, repeat, p->seed, stream, oE, oMu,
+                           p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
+                           p->q_count.as<unsigned long long>(), single->kb_maxtries, st);
+    else
+      launch_sab_sample_iso(single->sab, ekin, n, repeat, p->seed, stream, oE, oMu,
+                            p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
+                            p->q_count.as<unsigned long long>(), st);
   } else {
     launch_sample_iso(make_view(p), ekin, n, repeat, p->seed, stream, oE, oMu, p->d_err.as<unsigned long long>(),
                       st);
   }
 }
 
-// queue of the SCBragg Romberg kernels, sized for n neutrons
-void scb_queue(Proc* p, int64_t n, hipStream_t st) {
-  const size_t need = sizeof(int64_t) * (size_t)std::max<int64_t>(n, 1);
-  if (p->q_idx.bytes < need) {
-    NCAMD_HIP(hipStreamSynchronize(st));  // the old queue may still be in use on this stream
-    p->q_idx.alloc(need);
+// work buffers of the SCBragg kernels (batch ordering, Romberg queue), sized
+// for n neutrons
+ScbWork scb_work(Proc* p, int64_t n, hipStream_t st) {
+  const size_t m = (size_t)std::max<int64_t>(n, 1);
+  if (p->q_idx.bytes < sizeof(int64_t) * m || p->scb_keys.bytes < sizeof(unsigned) * m) {
+    NCAMD_HIP(hipStreamSynchronize(st));  // the old buffers may still be in use on this stream
+    p->q_idx.ensure(sizeof(int64_t) * m);
+    p->scb_perm.ensure(sizeof(int64_t) * m);
+    p->scb_keys.ensure(sizeof(unsigned) * m);
   }
   if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
+  if (!p->scb_hist.p) {
+    p->scb_hist.alloc(sizeof(unsigned) * kScbBuckets);
+    NCAMD_HIP(hipMemset(p->scb_hist.p, 0, sizeof(unsigned) * kScbBuckets));
+    p->scb_cursor.alloc(sizeof(unsigned long long) * kScbBuckets);
+  }
+  ScbWork w;
+  w.keys = p->scb_keys.as<unsigned>();
+  w.perm = p->scb_perm.as<int64_t>();
+  w.hist = p->scb_hist.as<unsigned>();
+  w.cursor = p->scb_cursor.as<unsigned long long>();
+  w.queue = p->q_idx.as<int64_t>();
+  w.qcount = p->q_count.as<unsigned long long>();
+  return w;
 }
 
 void check_fp32(Proc* p) {
@@ -1454,10 +1787,9 @@ void ncamd_samplescatter_many_dirs(ncamd_process_t h, const double* ekin, const 
     NCAMD_HIP(hipMemcpy(p->st_in2.p, dirs, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
   }
   if (scb) {
-    scb_queue(p, (int64_t)n, 0);
+    const ScbWork w = scb_work(p, (int64_t)n, 0);
     launch_scb_sample(p->scb, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->seed, next_stream(p),
-                      p->st_out1.as<double>(), p->st_out2.as<double>(), p->q_idx.as<int64_t>(),
-                      p->q_count.as<unsigned long long>(), 0);
+                      p->st_out1.as<double>(), p->st_out2.as<double>(), w, 0);
   } else {
     launch_sample_dirs(v, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->seed, next_stream(p),
                        p->st_out1.as<double>(), p->st_out2.as<double>(), p->d_err.as<unsigned long long>(), 0);
@@ -1505,9 +1837,8 @@ void ncamd_crosssection_many_dirs(ncamd_process_t h, const double* ekin, const d
     NCAMD_HIP(hipMemcpy(p->st_in.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
     NCAMD_HIP(hipMemcpy(p->st_in2.p, dirs, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
   }
-  scb_queue(p, (int64_t)n, 0);
-  launch_scb_xs(p->scb, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->st_out1.as<double>(),
-                p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), 0);
+  const ScbWork w = scb_work(p, (int64_t)n, 0);
+  launch_scb_xs(p->scb, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->st_out1.as<double>(), w, 0);
   NCAMD_HIP(hipGetLastError());
   if (n) NCAMD_HIP(hipMemcpy(out, p->st_out1.p, sizeof(double) * n, hipMemcpyDeviceToHost));
   ok = true;
@@ -1547,9 +1878,9 @@ void ncamd_dev_samplescatter_many_dirs(ncamd_process_t h, const double* d_ekin, 
   NCAMD_TRY
   Proc* p = extract(h);
   if (p->kind == kKindSCB) {
-    scb_queue(p, (int64_t)n, as_stream(stream));
-    launch_scb_sample(p->scb, d_ekin, d_dirs, (int64_t)n, p->seed, next_stream(p), d_oE, d_oD,
-                      p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), as_stream(stream));
+    const ScbWork w = scb_work(p, (int64_t)n, as_stream(stream));
+    launch_scb_sample(p->scb, d_ekin, d_dirs, (int64_t)n, p->seed, next_stream(p), d_oE, d_oD, w,
+                      as_stream(stream));
   } else {
     launch_sample_dirs(make_view(p), d_ekin, d_dirs, (int64_t)n, p->seed, next_stream(p), d_oE, d_oD,
                        p->d_err.as<unsigned long long>(), as_stream(stream));
@@ -1562,9 +1893,8 @@ void ncamd_dev_crosssection_many_dirs(ncamd_process_t h, const double* d_ekin, c
   NCAMD_TRY
   Proc* p = extract(h);
   if (p->kind == kKindSCB) {
-    scb_queue(p, (int64_t)n, as_stream(stream));
-    launch_scb_xs(p->scb, d_ekin, d_dirs, (int64_t)n, d_out, p->q_idx.as<int64_t>(),
-                  p->q_count.as<unsigned long long>(), as_stream(stream));
+    const ScbWork w = scb_work(p, (int64_t)n, as_stream(stream));
+    launch_scb_xs(p->scb, d_ekin, d_dirs, (int64_t)n, d_out, w, as_stream(stream));
   } else {
     launch_xs_iso(make_view(p), d_ekin, (int64_t)n, 1, d_out, false, as_stream(stream));
   }
@@ -1581,6 +1911,181 @@ void ncamd_dev_sab_sample_alphabeta(ncamd_process_t h, const double* d_ekin, uns
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH
 }
+ncamd_batch_t ncamd_batch_create(int nmat, const ncamd_process_t* procs, const int64_t* counts) {
+  ncamd_batch_t out;
+  out.internal = nullptr;
+  NCAMD_TRY
+  ensure_device();
+  if (nmat < 1) raise("BadInput", "a batch needs at least one material");
+  std::unique_ptr<Batch> b(new Batch());
+  std::vector<BatchMat> hm((size_t)nmat);
+  int64_t off = 0;
+  for (int i = 0; i < nmat; ++i) {
+    Proc* p = extract(procs[i]);
+    if (counts[i] < 0) raise("BadInput", "negative neutron count");
+    const CompView v = make_view(p);  // raises for oriented processes
+    p->refcount++;
+    b->mats.push_back(p);
+    b->counts.push_back(counts[i]);
+    b->offsets.push_back(off);
+    b->cost.push_back(batch_cost(v));
+    hm[(size_t)i].view = v;
+    hm[(size_t)i].offset = off;
+    off += counts[i];
+  }
+  b->total = off;
+  b->h_rngs.resize((size_t)nmat);
+  b->d_mats.upload(hm.data(), hm.size());
+  b->d_rngs.alloc(sizeof(BatchRng) * (size_t)nmat);
+  b->d_next.alloc(sizeof(unsigned));
+  b->d_err.alloc(sizeof(unsigned long long));
+  NCAMD_HIP(hipMemset(b->d_err.p, 0, sizeof(unsigned long long)));
+  out.internal = b.release();
+  NCAMD_CATCH
+  return out;
+}
+void ncamd_batch_destroy(ncamd_batch_t* h) {
+  NCAMD_TRY
+  delete extract_batch(*h);
+  h->internal = nullptr;
+  NCAMD_CATCH
+}
+int64_t ncamd_batch_total(ncamd_batch_t h) {
+  NCAMD_TRY return extract_batch(h)->total; NCAMD_CATCH
+  return -1;
+}
+void ncamd_batch_get_tuning(ncamd_batch_t h, int* tile, int* nblocks, int* ordering) {
+  NCAMD_TRY
+  Batch* b = extract_batch(h);
+  *tile = b->tile; *nblocks = b->nblocks; *ordering = b->ordering;
+  NCAMD_CATCH
+}
+void ncamd_batch_set_tuning(ncamd_batch_t h, int tile, int nblocks, int ordering) {
+  NCAMD_TRY
+  Batch* b = extract_batch(h);
+  if (tile < 64 || tile > (1 << 24) || nblocks < 1 || nblocks > 65536 || (ordering != 0 && ordering != 1))
+    raise("BadInput", "invalid batch tuning parameters");
+  if (tile != b->tile || ordering != b->ordering) b->tiles_valid = false;
+  b->tile = tile; b->nblocks = nblocks; b->ordering = ordering;
+  NCAMD_CATCH
+}
+void ncamd_dev_batch_crosssection(ncamd_batch_t h, const double* d_ekin, double* d_xs, void* stream) {
+  NCAMD_TRY
+  Batch* b = extract_batch(h);
+  batch_upload_rngs(b, false, as_stream(stream));
+  batch_run(b, 0, d_ekin, d_xs, nullptr, as_stream(stream));
+  NCAMD_CATCH
+}
+void ncamd_dev_batch_samplescatterisotropic(ncamd_batch_t h, const double* d_ekin, double* d_oE, double* d_oMu,
+                                            void* stream) {
+  NCAMD_TRY
+  Batch* b = extract_batch(h);
+  batch_upload_rngs(b, true, as_stream(stream));
+  batch_run(b, 1, d_ekin, d_oE, d_oMu, as_stream(stream));
+  NCAMD_CATCH
+}
+// Times every candidate (tile size x resident workgroups x tile order) on
+// the given inputs with sampling (outputs are scratch; no material's RNG
+// state advances) and keeps the fastest.  Returns its time in ms.
+double ncamd_dev_batch_autotune(ncamd_batch_t h, const double* d_ekin, double* d_o1, double* d_o2, void* stream) {
+  NCAMD_TRY
+  Batch* b = extract_batch(h);
+  hipStream_t st = as_stream(stream);
+  batch_upload_rngs(b, false, st);
+  int cu = 256;
+  {
+    int dev = 0;
+    NCAMD_HIP(hipGetDevice(&dev));
+    NCAMD_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  hipEvent_t e0, e1;
+  NCAMD_HIP(hipEventCreate(&e0));
+  NCAMD_HIP(hipEventCreate(&e1));
+  double best = __builtin_huge_val();
+  int bt = b->tile, bn = b->nblocks, bo = b->ordering;
+  const int tiles[] = {512, 1024, 2048, 4096, 8192, 16384};
+  const int per_cu[] = {2, 4, 8};
+  for (int ord = 0; ord < 2; ++ord)
+    for (int t : tiles)
+      for (int pc : per_cu) {
+        b->tile = t; b->nblocks = pc * cu; b->ordering = ord; b->tiles_valid = false;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {  // first run warms the graph and caches
+          NCAMD_HIP(hipEventRecord(e0, st));
+          batch_run(b, 1, d_ekin, d_o1, d_o2, st);
+          NCAMD_HIP(hipEventRecord(e1, st));
+          NCAMD_HIP(hipEventSynchronize(e1));
+          NCAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        }
+        if (ms < best) { best = ms; bt = t; bn = pc * cu; bo = ord; }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->tile = bt; b->nblocks = bn; b->ordering = bo; b->tiles_valid = false;
+  NCAMD_HIP(hipMemsetAsync(b->d_err.p, 0, sizeof(unsigned long long), st));
+  return best;
+  NCAMD_CATCH
+  return -1.0;
+}
+int64_t ncamd_dev_batch_check_errors(ncamd_batch_t h, void* stream) {
+  NCAMD_TRY
+  Batch* b = extract_batch(h);
+  unsigned long long n = 0;
+  NCAMD_HIP(hipMemcpyAsync(&n, b->d_err.p, sizeof(n), hipMemcpyDeviceToHost, as_stream(stream)));
+  NCAMD_HIP(hipStreamSynchronize(as_stream(stream)));
+  if (n) NCAMD_HIP(hipMemset(b->d_err.p, 0, sizeof(n)));
+  return (int64_t)n;
+  NCAMD_CATCH
+  return -1;
+}
+
+int ncamd_sab_use_envelope_sampler(ncamd_process_t h, int nintervals, double emin, double emax, int nbox_w,
+                                   int nbox_k, int maxtries) {
+  NCAMD_TRY
+  ensure_device();
+  build_kb(extract(h), nintervals, emin, emax, nbox_w, nbox_k, maxtries);
+  return 0;
+  NCAMD_CATCH
+  return -1;
+}
+int ncamd_sab_use_plain_sampler(ncamd_process_t h) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
+  p->use_kb = false;
+  return 0;
+  NCAMD_CATCH
+  return -1;
+}
+int ncamd_sab_envelope_tables(ncamd_process_t h, double* node_e, double* env) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindSAB || p->kb_node_h.empty()) raise("BadInput", "no envelope sampler tables");
+  std::copy(p->kb_node_h.begin(), p->kb_node_h.end(), node_e);
+  std::copy(p->kb_env_h.begin(), p->kb_env_h.end(), env);
+  return 0;
+  NCAMD_CATCH
+  return -1;
+}
+int64_t ncamd_dev_sab_sample_alphabeta_envelope(ncamd_process_t h, const double* d_ekin, unsigned long n,
+                                                double* d_a, double* d_b, void* stream) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindSAB || p->kb_node_h.empty()) raise("BadInput", "no envelope sampler tables");
+  if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
+  hipStream_t st = as_stream(stream);
+  launch_zero_u64(p->q_count.as<unsigned long long>(), st);
+  launch_kb_sample_ab(p->sab, p->kb, d_ekin, (int64_t)n, p->seed, next_stream(p), d_a, d_b,
+                      p->q_count.as<unsigned long long>(), p->kb_maxtries, st);
+  NCAMD_HIP(hipGetLastError());
+  unsigned long long nf = 0;
+  NCAMD_HIP(hipMemcpyAsync(&nf, p->q_count.p, sizeof(nf), hipMemcpyDeviceToHost, st));
+  NCAMD_HIP(hipStreamSynchronize(st));
+  return (int64_t)nf;
+  NCAMD_CATCH
+  return -1;
+}
+
 int64_t ncamd_dev_check_errors(ncamd_process_t h, void* stream) {
   NCAMD_TRY
   Proc* p = extract(h);

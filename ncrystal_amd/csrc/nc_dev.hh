@@ -134,9 +134,29 @@ struct PBView {
   const float* fdm32;   // [n] same, FP32 (Kahan-compensated build), may be null
   int n;
   double threshold;     // e2d[0]
+  // search guides (exact: every bracket contains the plain binary-search answer)
+  const int* e_bucket;  // [e_nbkt+1] upper_bound(e2d, bucket lower edge), IEEE-bit buckets of E
+  long long e_base;     // bits(e2d[0]) >> e_shift
+  int e_shift, e_nbkt;
+  const int* c_guide;   // [c_n+1] lower_bound(fdm, g / c_scale), last entry n
+  int c_n;
+  double c_scale;       // c_n / fdm[n-1]
 };
+// index of the last plane with 2d >= wavelength (E >= threshold assumed)
 NCD int pb_last_valid(const PBView& pb, double E) {
-  return upper_bound_d(pb.e2d, 1, pb.n, E) - 1;
+  if (!(E < pb.e2d[pb.n - 1])) return pb.n - 1;
+  const long long b = (long long)((unsigned long long)__double_as_longlong(E) >> pb.e_shift) - pb.e_base;
+  const int lo = max(pb.e_bucket[b], 1), hi = pb.e_bucket[b + 1];
+  return upper_bound_d(pb.e2d, lo, hi, E) - 1;
+}
+// lower_bound over fdm[0..idx) of a target in [0, fdm[idx]]
+NCD int pb_pick_plane(const PBView& pb, int idx, double target) {
+  long long g = (long long)(target * pb.c_scale);
+  g = g < 0 ? 0 : (g >= pb.c_n ? pb.c_n - 1 : g);
+  const int lo = pb.c_guide[g > 0 ? g - 1 : 0];
+  int hi = pb.c_guide[g + 2 <= pb.c_n ? g + 2 : pb.c_n];
+  hi = hi < idx ? hi : idx;
+  return lower_bound_d(pb.fdm, lo < hi ? lo : hi, hi, target);
 }
 NCD bool pb_active(const PBView& pb, double E) {
   return pb.n > 0 && !(E < pb.threshold) && isfinite(E);
@@ -151,7 +171,7 @@ NCD double pb_sample_mu(const PBView& pb, Rng& rng, double E) {
   if (!pb_active(pb, E)) return 1.0;
   const int idx = pb_last_valid(pb, E);
   const double u = rng.gen();
-  const int i = lower_bound_d(pb.fdm, 0, idx, u * pb.fdm[idx]);
+  const int i = pb_pick_plane(pb, idx, u * pb.fdm[idx]);
   const double s2 = pb.e2d[i] / E;
   return 1.0 - 2.0 * s2;
 }
@@ -551,12 +571,9 @@ NCD int sab_sample_ab(const SABView& s, double E, Rng& rng, double& a_out, doubl
   return kErrSamplerLoop;
 }
 
-// SABSampler::sampleDeltaEMu + SABScatter::sampleScatterIsotropic
-template <bool WithHighE = true>
-NCD int sab_sample_iso(const SABView& s, Rng& rng, double E, double& Ef, double& mu) {
-  double a, b;
-  const int rc = sab_sample_ab<WithHighE>(s, E, rng, a, b);
-  if (rc) return rc;
+// SABSampler::sampleDeltaEMu + SABScatter::sampleScatterIsotropic:
+// (alpha, beta) -> (E', mu) (ncrystal_core/src/phys_utils/NCKinUtils.cc:25-58)
+NCD int sab_ab_to_emu(const SABView& s, Rng& rng, double E, double a, double b, double& Ef, double& mu) {
   const double kT = s.kT;
   double dE, m;
   if (mu_iso_at_beta(b, E / kT)) {
@@ -574,6 +591,13 @@ NCD int sab_sample_iso(const SABView& s, Rng& rng, double E, double& Ef, double&
   Ef = dmax(0.0, E + dE);
   mu = m;
   return kOK;
+}
+template <bool WithHighE = true>
+NCD int sab_sample_iso(const SABView& s, Rng& rng, double E, double& Ef, double& mu) {
+  double a, b;
+  const int rc = sab_sample_ab<WithHighE>(s, E, rng, a, b);
+  if (rc) return rc;
+  return sab_ab_to_emu(s, rng, E, a, b, Ef, mu);
 }
 
 // randDirectionGivenScatterMu (NCRandUtils.cc:51-96)

@@ -3,6 +3,7 @@
 #pragma once
 #include "nc_dev.hh"
 #include "nc_scb.hh"
+#include "nc_kb.hh"
 
 namespace ncamd {
 
@@ -19,6 +20,55 @@ struct CompView {
   ElIncView ei;
   SABView sab;
 };
+
+// Composition semantics: ncrystal_core/src/interfaces/NCProcImpl.cc:166-204
+// (cumulative component xs), :353-362 (xs), :379-389 (component pick +
+// sampling).
+NCD double comp_domain_lo(const CompView& c) { return c.domain_lo; }
+
+template <bool FP32PB>
+NCD double pb_xs_sel(const PBView& pb, double E) {
+  if constexpr (FP32PB) {
+    if (!pb_active(pb, E)) return 0.0;
+    const int idx = pb_last_valid(pb, E);
+    const double inv = 1.0 / E;
+    return (double)pb.fdm32[idx] * inv;
+  } else {
+    return pb_xs(pb, E);
+  }
+}
+
+template <bool FP32PB>
+NCD double comp_xs(const CompView& c, double E, double* commul) {
+  if (!(E >= c.domain_lo && E <= c.domain_hi)) return 0.0;
+  double tot = 0.0;
+  for (int i = 0; i < c.n; ++i) {
+    double xs = 0.0;
+    switch (c.kind[i]) {
+      case kKindPB: xs = pb_active(c.pb, E) ? pb_xs_sel<FP32PB>(c.pb, E) : 0.0; break;
+      case kKindElInc: xs = elinc_xs(c.ei, E, nullptr); break;
+      case kKindSAB: xs = sab_xs(c.sab, E); break;
+      default: break;
+    }
+    tot += c.scale[i] * xs;
+    if (commul) commul[i] = tot;
+  }
+  return tot;
+}
+
+NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, double& mu) {
+  if (!(E >= c.domain_lo && E <= c.domain_hi)) { Ef = E; mu = 1.0; return kOK; }
+  double commul[kMaxComp];
+  const double tot = comp_xs<false>(c, E, commul);
+  (void)tot;
+  const int ic = pick_idx(rng, commul, c.n);
+  switch (c.kind[ic]) {
+    case kKindPB: Ef = E; mu = pb_sample_mu(c.pb, rng, E); return kOK;
+    case kKindElInc: Ef = E; mu = elinc_sample_mu(c.ei, rng, E); return kOK;
+    case kKindSAB: return sab_sample_iso(c.sab, rng, E, Ef, mu);
+    default: return kErrBadSampler;
+  }
+}
 
 // per-neutron kernels (nc_kernels.hip)
 void launch_xs_iso(const CompView&, const double* ekin, int64_t n, int64_t repeat, double* out, bool fp32,
@@ -37,12 +87,54 @@ void launch_sab_sample_iso(const SABView&, const double* ekin, int64_t n, int64_
 void launch_sab_guides(const SABView&, unsigned short* cdf_guide, unsigned short* row_guide, double* row_scale,
                        hipStream_t);
 
-// single-crystal Bragg, oriented (nc_scb.hip): main kernel + Romberg queue kernel
+// kinematic-box envelope sampler of S(alpha,beta) (nc_kb.hip)
+void launch_kb_build(const SABView&, const KBView&, double* cmax, double* env, hipStream_t);
+void launch_kb_sample_iso(const SABView&, const KBView&, const double* ekin, int64_t n, int64_t repeat,
+                          uint64_t seed, uint32_t stream, double* oE, double* oMu, unsigned long long* nerr,
+                          int64_t* queue, unsigned long long* qcount, int maxtries, hipStream_t);
+void launch_kb_sample_ab(const SABView&, const KBView&, const double* ekin, int64_t n, uint64_t seed,
+                         uint32_t stream, double* oa, double* ob, unsigned long long* nfallback, int maxtries,
+                         hipStream_t);
+// helpers shared with the S(alpha,beta) kernels (nc_kernels.hip)
+void launch_zero_u64(unsigned long long* p, hipStream_t);
+// drains a queue of neutron indices with the plain sampler (from draw 0)
+void launch_sab_queue(const SABView&, const double* ekin, int64_t n, int64_t total, uint64_t seed, uint32_t stream,
+                      double* oE, double* oMu, unsigned long long* nerr, const int64_t* queue,
+                      const unsigned long long* qcount, hipStream_t);
+
+// many materials in one persistent launch (nc_batch.hip)
+struct BatchMat {
+  CompView view;
+  int64_t offset;     // first neutron of this material in the concatenated arrays
+};
+struct BatchRng {
+  uint64_t seed;      // the material's RNG seed and call counter value for this call
+  uint64_t stream;
+};
+struct BatchTile {
+  int mat;
+  int64_t begin, end;  // neutron range within the material
+};
+// mode 0: cross sections into o1; mode 1: isotropic sampling into (o1=E', o2=mu)
+void launch_batch(const BatchMat* mats, const BatchRng* rngs, const BatchTile* tiles, int ntiles, int nblocks,
+                  int mode, const double* ekin, double* o1, double* o2, unsigned* next, unsigned long long* nerr,
+                  hipStream_t);
+
+// single-crystal Bragg, oriented (nc_scb.hip): counting sort of the batch by
+// scan length, main kernel, Romberg queue kernel
+constexpr int kScbBuckets = 1024;
+struct ScbWork {
+  unsigned* keys;              // [n] bucket of each neutron
+  int64_t* perm;               // [n] neutrons in bucket order
+  unsigned* hist;              // [kScbBuckets]
+  unsigned long long* cursor;  // [kScbBuckets]
+  int64_t* queue;              // [n] neutrons for the Romberg kernels
+  unsigned long long* qcount;  // [1]
+};
 void launch_scb_xs(const SCBView&, const double* ekin, const double* dirs, int64_t n, double* out,
-                   int64_t* queue, unsigned long long* qcount, hipStream_t);
+                   const ScbWork&, hipStream_t);
 void launch_scb_sample(const SCBView&, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
-                       uint32_t stream, double* oE, double* oD, int64_t* queue, unsigned long long* qcount,
-                       hipStream_t);
+                       uint32_t stream, double* oE, double* oD, const ScbWork&, hipStream_t);
 
 // S(alpha,beta) table construction (nc_sab_build.hip)
 struct SABBuildBufs {

@@ -9,52 +9,6 @@
 
 namespace ncamd {
 
-NCD double comp_domain_lo(const CompView& c) { return c.domain_lo; }
-
-template <bool FP32PB>
-NCD double pb_xs_sel(const PBView& pb, double E) {
-  if constexpr (FP32PB) {
-    if (!pb_active(pb, E)) return 0.0;
-    const int idx = pb_last_valid(pb, E);
-    const double inv = 1.0 / E;
-    return (double)pb.fdm32[idx] * inv;
-  } else {
-    return pb_xs(pb, E);
-  }
-}
-
-template <bool FP32PB>
-NCD double comp_xs(const CompView& c, double E, double* commul) {
-  if (!(E >= c.domain_lo && E <= c.domain_hi)) return 0.0;
-  double tot = 0.0;
-  for (int i = 0; i < c.n; ++i) {
-    double xs = 0.0;
-    switch (c.kind[i]) {
-      case kKindPB: xs = pb_active(c.pb, E) ? pb_xs_sel<FP32PB>(c.pb, E) : 0.0; break;
-      case kKindElInc: xs = elinc_xs(c.ei, E, nullptr); break;
-      case kKindSAB: xs = sab_xs(c.sab, E); break;
-      default: break;
-    }
-    tot += c.scale[i] * xs;
-    if (commul) commul[i] = tot;
-  }
-  return tot;
-}
-
-NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, double& mu) {
-  if (!(E >= c.domain_lo && E <= c.domain_hi)) { Ef = E; mu = 1.0; return kOK; }
-  double commul[kMaxComp];
-  const double tot = comp_xs<false>(c, E, commul);
-  (void)tot;
-  const int ic = pick_idx(rng, commul, c.n);
-  switch (c.kind[ic]) {
-    case kKindPB: Ef = E; mu = pb_sample_mu(c.pb, rng, E); return kOK;
-    case kKindElInc: Ef = E; mu = elinc_sample_mu(c.ei, rng, E); return kOK;
-    case kKindSAB: return sab_sample_iso(c.sab, rng, E, Ef, mu);
-    default: return kErrBadSampler;
-  }
-}
-
 template <bool FP32PB>
 __global__ void __launch_bounds__(256) k_xs_iso(CompView c, const double* __restrict__ ekin,
                                                 int64_t n, int64_t repeat, double* __restrict__ out) {
@@ -264,6 +218,15 @@ void launch_sab_sample_iso(const SABView& s, const double* ekin, int64_t n, int6
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount);
   hipLaunchKernelGGL(k_sab_sample_iso_main, dim3(grid_for(total, 256)), dim3(256), 0, st, s, ekin, n, total, seed,
                      stream, oE, oMu, nerr, queue, qcount);
+  hipLaunchKernelGGL(k_sab_sample_iso_highe, dim3(grid_for(total, 256)), dim3(256), 0, st, s, ekin, n, total, seed,
+                     stream, oE, oMu, nerr, queue, qcount);
+}
+void launch_zero_u64(unsigned long long* p, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, p);
+}
+void launch_sab_queue(const SABView& s, const double* ekin, int64_t n, int64_t total, uint64_t seed, uint32_t stream,
+                      double* oE, double* oMu, unsigned long long* nerr, const int64_t* queue,
+                      const unsigned long long* qcount, hipStream_t st) {
   hipLaunchKernelGGL(k_sab_sample_iso_highe, dim3(grid_for(total, 256)), dim3(256), 0, st, s, ekin, n, total, seed,
                      stream, oE, oMu, nerr, queue, qcount);
 }

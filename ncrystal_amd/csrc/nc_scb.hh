@@ -19,12 +19,12 @@
 //  - rotateToFrame: ncrystal_core/src/utils/NCRotMatrix.cc:84-147
 //  - pickRandIdxByWeight: ncrystal_core/src/utils/NCRandUtils.cc:198-222
 //
-// Lanes of a wavefront walk the families together: the family and normal
-// indices are wave-uniform (the loads of family data and normals are scalar
-// loads shared by the 64 lanes) and a lane whose families are exhausted is
-// masked.  The rare backscattering geometry that needs the Romberg circle
-// integral is left to a second kernel (AllowSlow=true) through a queue, so
-// the main kernel carries no Romberg state.
+// Lanes of a wavefront scan the demi-normals together: the normal index is
+// wave-uniform (the records are scalar loads shared by the 64 lanes) and a
+// lane whose families are exhausted is masked.  The rare backscattering
+// geometry that needs the Romberg circle integral is left to a second kernel
+// (AllowSlow=true) through a queue, so the main kernel carries no Romberg
+// state.
 #pragma once
 #include "nc_dev.hh"
 #include "nc_math.hh"
@@ -39,17 +39,26 @@ struct SplineView {
 
 struct SCBView {
   int nfam;
+  int nnormals;
   const double* __restrict__ inv2d;     // [nfam] 1/(2d), non-decreasing
-  const double* __restrict__ inv2d_r;   // [nfam] inv2d rounded to 1e-15 (GaussMos cache rounding)
   const double* __restrict__ xsf;       // [nfam] |F|^2/(V0 n) * 0.5
   const int* __restrict__ off;          // [nfam+1] demi-normal offsets
-  const double* __restrict__ normals;   // [3*n] lab-frame unit demi-normals
+  // one record per demi-normal in scan order: lab-frame unit normal and the
+  // family's 1/(2d) rounded to 1e-15 (GaussMos cache rounding)
+  const double4* __restrict__ nrec;     // [nnormals + kScbPad]
+  // the same records in FP32 for the pre-filter of the scan
+  const float4* __restrict__ nrec32;    // [nnormals + kScbPad]
+  const int* __restrict__ famidx;       // [nnormals] family of each demi-normal
   double threshold;                     // ekin below which nothing reflects
   double cta, k1, k2, numacc;           // GaussOnSphere: cos(truncation), fast-path bounds, Romberg accuracy
+  float band32;                         // pre-filter half-width: truncation angle + rounding margins
   SplineView lt_evalcosx, lt_sofcosd;
 };
 
-enum : int { kScbOK = 0, kScbSlow = 1 };
+// scan group size; the record arrays carry this many padding entries so
+// that a group never reads past the end
+constexpr int kScbGroup = 8;
+constexpr int kScbPad = kScbGroup;
 
 NCD double scb_round(double x) { return floor(x * 1e15 + 0.5) * 1e-15; }
 NCD double gm_round(double x) { return floor(dmax(x, 1e-15) * 1e15 + 0.5) * 1e-15; }
@@ -348,7 +357,7 @@ NCD int wave_max_i(int x) {
 struct ScbLane {
   double wl;       // wavelength (from the rounded energy)
   V3 u;            // unit direction
-  int nf;          // number of families with 2d > wl (0: lane takes no part)
+  int jlim;        // demi-normals of the families with 2d > wl (0: lane takes no part)
   int count;       // number of non-zero contributions
   double total;    // cumulative xs after the last contribution
   bool slow;       // a contribution needs the Romberg circle integral
@@ -362,38 +371,69 @@ struct ScbLane {
 // order (families by decreasing d, normals in family order, anti-normal side
 // first).  Select=false: count and cumulative total.  Select=true: find the
 // contribution picked by pickRandIdxByWeight for (n_select, target).
-// All lanes of the wavefront must call this together.
+//
+// The demi-normals are scanned as one flat list in groups of kScbGroup with a
+// wave-uniform index (scalar loads of FP32 records).  A normal can only
+// contribute if the angle phi between the neutron direction and its plane is
+// within the truncation angle of the Bragg angle theta; since
+// |sin(phi) - sin(theta)| <= |phi - theta|, the FP32 test
+// | |n.u| - sin(theta) | < band32 (band32 = truncation angle plus margins
+// for FP32 rounding, about 1e-5) is a superset of the reference's cone test.
+// Candidates are then processed in order with the FP64 record and the
+// reference's test and arithmetic unchanged.  Per-family state (cumulative
+// offset, lazily computed Q) is (re)initialised at the first candidate of
+// each family, which leaves every value as the reference's family loop
+// computes it.  All lanes of the wavefront must call this together.
 template <bool Select, bool AllowSlow>
 NCD void scb_walk(const SCBView& v, ScbLane& L, int n_select, double target) {
   L.count = 0;
   L.total = 0.0;
   L.slow = false;
   L.found = false;
-  bool alive = L.nf > 0;
-  const int nfmax = wave_max_i(L.nf);
+  bool alive = L.jlim > 0;
+  const int jmax = wave_max_i(L.jlim);
   const double cta = v.cta;
   const double w = gm_round(L.wl);
   const double wl3 = w * w * w;
+  const double ux = L.u.x, uy = L.u.y, uz = L.u.z;
+  const float fx = (float)ux, fy = (float)uy, fz = (float)uz, fw = (float)w;
+  const float band32 = v.band32;
   double commul_last = 0.0;
-  for (int f = 0; f < nfmax; ++f) {
-    if (!__any(alive)) break;
-    const bool act = alive && f < L.nf;
-    const double i2 = v.inv2d_r[f];
-    const double xsf = v.xsf[f];
-    const int j0 = v.off[f], j1 = v.off[f + 1];
-    const double sinpt = w * i2;
-    const double cptsq = 1 - sinpt * sinpt;
-    double Q = -1.0, Qprime = -1.0, cospt = -1.0;
-    const double xsoffset = L.count ? commul_last : 0.0;
-    double xssum = 0.0;
-    bool fam_alive = act;
-    for (int j = j0; j < j1; ++j) {
-      const V3 nrm{v.normals[3 * j], v.normals[3 * j + 1], v.normals[3 * j + 2]};
-      const double dot = v3dot(nrm, L.u);
+  int cur_f = -1;
+  double xsf = 0.0, Q = -1.0, Qprime = -1.0, cospt = -1.0, xsoffset = 0.0, xssum = 0.0;
+  for (int jb = 0; jb < jmax; jb += kScbGroup) {
+    unsigned cand = 0u;
+#pragma unroll
+    for (int q = 0; q < kScbGroup; ++q) {
+      const float4 r = v.nrec32[jb + q];
+      const float d32 = r.x * fx + r.y * fy + r.z * fz;
+      const float s32 = fw * r.w;
+      cand |= (fabsf(fabsf(d32) - s32) < band32 ? 1u : 0u) << q;
+    }
+    const int left = L.jlim - jb;
+    if (left < kScbGroup) cand &= left > 0 ? (1u << left) - 1u : 0u;
+    if (!alive) cand = 0u;
+    if (!__any(cand != 0u)) continue;
+    while (cand) {
+      const int q = __builtin_ctz(cand);
+      cand &= cand - 1u;
+      const int j = jb + q;
+      const double4 r = v.nrec[j];
+      const double dot = r.x * ux + r.y * uy + r.z * uz;
+      const double sinpt = w * r.w;
+      const int f = v.famidx[j];
+      if (f != cur_f) {
+        cur_f = f;
+        xsf = v.xsf[f];
+        Q = -1.0; Qprime = -1.0; cospt = -1.0;
+        xsoffset = commul_last;
+        xssum = 0.0;
+      }
+      const double cptsq = 1 - sinpt * sinpt;
       const double sdotcptsq = (1.0 - dot * dot) * cptsq;
       const double ds = dot * sinpt;
       const double A0 = dmax(0.0, cta - fabs(ds));
-      if (!fam_alive || sdotcptsq <= A0 * A0) continue;
+      if (sdotcptsq <= A0 * A0) continue;
       for (int side = 0; side < 2; ++side) {
         const double Aside = side == 0 ? dmax(0.0, cta - ds) : dmax(0.0, cta + ds);
         if (!(sdotcptsq > Aside * Aside)) continue;
@@ -416,7 +456,7 @@ NCD void scb_walk(const SCBView& v, ScbLane& L, int n_select, double target) {
               ci = gos_circle_slow(v, cc, sc2, sinpt, cospt);
             } else {
               L.slow = true;
-              fam_alive = false;
+              alive = false;
               break;
             }
           }
@@ -431,25 +471,28 @@ NCD void scb_walk(const SCBView& v, ScbLane& L, int n_select, double target) {
             const bool hit = n_select < 5 ? (cm > target) : !(cm < target);
             if (hit || L.count == n_select - 1) {
               L.found = true;
-              L.sel_inv2d_r = i2;
+              L.sel_inv2d_r = r.w;
+              const V3 nrm{r.x, r.y, r.z};
               L.sel_normal = side == 0 ? V3{-nrm.x, -nrm.y, -nrm.z} : nrm;
-              fam_alive = false;
+              alive = false;
               break;
             }
           }
           L.count++;
         }
       }
+      if (!alive) break;
     }
-    if (act && !fam_alive) alive = false;
+    if (!__any(alive)) break;
   }
   L.total = commul_last;
 }
 
 // per-lane set-up of a neutron (NCSCBragg.cc:233-259): returns false when
-// the neutron cannot reflect (below threshold, zero wavelength)
+// the neutron cannot reflect (below threshold, zero wavelength).  jlim =
+// demi-normals of the families with 1/(2d) below (1-2eps)/wl.
 NCD bool scb_lane_setup(const SCBView& v, double E, double dx, double dy, double dz, ScbLane& L) {
-  L.nf = 0;
+  L.jlim = 0;
   if (!(E > v.threshold)) return false;
   const double ekin = scb_round(E);
   const double wl = ekin ? sqrt(kWl2Ekin / ekin) : __builtin_huge_val();
@@ -457,7 +500,7 @@ NCD bool scb_lane_setup(const SCBView& v, double E, double dx, double dy, double
   L.wl = wl;
   L.u = v3normalise(V3{dx, dy, dz});
   const double inv2dcut = (1.0 - 2 * 2.220446049250313080847263336181640625e-16) / wl;
-  L.nf = lower_bound_d(v.inv2d, 0, v.nfam, inv2dcut);
+  L.jlim = v.off[lower_bound_d(v.inv2d, 0, v.nfam, inv2dcut)];
   return true;
 }
 

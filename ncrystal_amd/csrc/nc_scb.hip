@@ -1,12 +1,20 @@
 // Per-neutron kernels of the mosaic single-crystal Bragg process: oriented
 // cross sections and scatter sampling, one lane per neutron.
 //
+// Scheduling: the cost of a neutron is the number of demi-normals of the
+// families with 2d > lambda, which varies as lambda^-3 across a batch.  A
+// counting sort by that number (k_scb_keys / k_scb_offsets / k_scb_scatter)
+// gives every wavefront neutrons of nearly equal cost; the kernels then take
+// neutron perm[k] in lane k.  Results are written at the neutron's own index
+// and every neutron draws from its own Philox stream, so the order changes
+// no result.
+//
 // Each kernel pair is a main kernel without the Romberg circle integral and
 // a queue kernel that redoes, with it, the neutrons whose scan met a
 // contribution outside the spline's validity (backscattering within the
 // mosaic spread).  Cross sections involve no random numbers and sampling
 // draws from the neutron's own Philox stream from its first draw, so the
-// split does not change any result.
+// split does not change any result either.
 //
 // Reference: ncrystal_core/src/scbragg/NCSCBragg.cc:233-323 (crossSection,
 // sampleScatter), ncrystal_core/src/cinterface/ncrystal.cc:1248-1282 (batched
@@ -24,7 +32,7 @@ NCD void scb_xs_one(const SCBView& v, bool valid, int64_t i, const double* __res
   ScbLane L;
   L.wl = 1.0;
   L.u = V3{0.0, 0.0, 1.0};
-  L.nf = 0;
+  L.jlim = 0;
   bool can = false;
   if (valid) can = scb_lane_setup(v, ekin[i], dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2], L);
   scb_walk<false, AllowSlow>(v, L, 0, 0.0);
@@ -47,7 +55,7 @@ NCD void scb_sample_one(const SCBView& v, bool valid, int64_t i, const double* _
   ScbLane L;
   L.wl = 1.0;
   L.u = din;
-  L.nf = 0;
+  L.jlim = 0;
   bool can = false;
   if (valid) {
     E = ekin[i];
@@ -69,7 +77,7 @@ NCD void scb_sample_one(const SCBView& v, bool valid, int64_t i, const double* _
   if (scatter && L.count > 1) target = L.total * rng.gen();
   // pass 2: the contribution whose cumulative value passes the target
   ScbLane S = L;
-  if (!scatter) S.nf = 0;
+  if (!scatter) S.jlim = 0;
   scb_walk<true, AllowSlow>(v, S, L.count, target);
   if (!valid || queued) return;
   V3 o = din;
@@ -82,16 +90,72 @@ NCD void scb_sample_one(const SCBView& v, bool valid, int64_t i, const double* _
 
 }  // namespace
 
-// Grid-stride loops step over whole blocks so that every lane of a
-// wavefront reaches the wave-uniform scan together.
+// ---- counting sort of the batch by scan length
+__global__ void __launch_bounds__(256) k_scb_keys(SCBView v, const double* __restrict__ ekin,
+                                                  const double* __restrict__ dirs, int64_t n,
+                                                  unsigned* __restrict__ keys, unsigned* __restrict__ hist) {
+  __shared__ unsigned lh[kScbBuckets];
+  for (int b = threadIdx.x; b < kScbBuckets; b += blockDim.x) lh[b] = 0u;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    ScbLane L;
+    L.jlim = 0;
+    scb_lane_setup(v, ekin[i], dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2], L);
+    const unsigned b = (unsigned)(((int64_t)L.jlim * kScbBuckets) / ((int64_t)v.nnormals + 1));
+    keys[i] = b;
+    atomicAdd(&lh[b], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kScbBuckets; b += blockDim.x)
+    if (lh[b]) atomicAdd(&hist[b], lh[b]);
+}
+
+// exclusive prefix sum of the bucket counts (one block); cursors start at
+// the bucket offsets, the histogram is cleared for the next call
+__global__ void __launch_bounds__(kScbBuckets) k_scb_offsets(unsigned* __restrict__ hist,
+                                                              unsigned long long* __restrict__ cursor) {
+  __shared__ unsigned long long s[kScbBuckets];
+  const int t = threadIdx.x;
+  s[t] = hist[t];
+  __syncthreads();
+  for (int o = 1; o < kScbBuckets; o <<= 1) {
+    const unsigned long long x = t >= o ? s[t - o] : 0ull;
+    __syncthreads();
+    s[t] += x;
+    __syncthreads();
+  }
+  cursor[t] = s[t] - hist[t];
+  hist[t] = 0u;
+}
+
+__global__ void __launch_bounds__(256) k_scb_scatter(const unsigned* __restrict__ keys, int64_t n,
+                                                     unsigned long long* __restrict__ cursor,
+                                                     int64_t* __restrict__ perm) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const unsigned long long slot = atomicAdd(&cursor[keys[i]], 1ull);
+    perm[slot] = i;
+  }
+}
+
+__global__ void k_scb_clear(unsigned* __restrict__ hist, unsigned long long* __restrict__ qcount) {
+  for (int b = threadIdx.x; b < kScbBuckets; b += blockDim.x) hist[b] = 0u;
+  if (threadIdx.x == 0) *qcount = 0ull;
+}
+
+// ---- main and queue kernels.  Grid-stride loops step over whole blocks so
+// that every lane of a wavefront reaches the wave-uniform scan together.
 __global__ void __launch_bounds__(256) k_scb_xs_main(SCBView v, const double* __restrict__ ekin,
                                                      const double* __restrict__ dirs, int64_t n,
-                                                     double* __restrict__ out, int64_t* __restrict__ queue,
+                                                     const int64_t* __restrict__ perm, double* __restrict__ out,
+                                                     int64_t* __restrict__ queue,
                                                      unsigned long long* __restrict__ qcount) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
-    const int64_t i = base + threadIdx.x;
-    scb_xs_one<false>(v, i < n, i, ekin, dirs, out, queue, qcount);
+    const int64_t k = base + threadIdx.x;
+    const bool valid = k < n;
+    scb_xs_one<false>(v, valid, valid ? perm[k] : 0, ekin, dirs, out, queue, qcount);
   }
 }
 __global__ void __launch_bounds__(256) k_scb_xs_slow(SCBView v, const double* __restrict__ ekin,
@@ -103,19 +167,20 @@ __global__ void __launch_bounds__(256) k_scb_xs_slow(SCBView v, const double* __
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nq; base += stride) {
     const int64_t q = base + threadIdx.x;
     const bool valid = q < nq;
-    const int64_t i = valid ? queue[q] : 0;
-    scb_xs_one<true>(v, valid, i, ekin, dirs, out, nullptr, nullptr);
+    scb_xs_one<true>(v, valid, valid ? queue[q] : 0, ekin, dirs, out, nullptr, nullptr);
   }
 }
 __global__ void __launch_bounds__(256) k_scb_sample_main(SCBView v, const double* __restrict__ ekin,
-                                                         const double* __restrict__ dirs, int64_t n, uint64_t seed,
+                                                         const double* __restrict__ dirs, int64_t n,
+                                                         const int64_t* __restrict__ perm, uint64_t seed,
                                                          uint32_t stream, double* __restrict__ oE,
                                                          double* __restrict__ oD, int64_t* __restrict__ queue,
                                                          unsigned long long* __restrict__ qcount) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
-    const int64_t i = base + threadIdx.x;
-    scb_sample_one<false>(v, i < n, i, ekin, dirs, seed, stream, oE, oD, queue, qcount);
+    const int64_t k = base + threadIdx.x;
+    const bool valid = k < n;
+    scb_sample_one<false>(v, valid, valid ? perm[k] : 0, ekin, dirs, seed, stream, oE, oD, queue, qcount);
   }
 }
 __global__ void __launch_bounds__(256) k_scb_sample_slow(SCBView v, const double* __restrict__ ekin,
@@ -128,12 +193,9 @@ __global__ void __launch_bounds__(256) k_scb_sample_slow(SCBView v, const double
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nq; base += stride) {
     const int64_t q = base + threadIdx.x;
     const bool valid = q < nq;
-    const int64_t i = valid ? queue[q] : 0;
-    scb_sample_one<true>(v, valid, i, ekin, dirs, seed, stream, oE, oD, nullptr, nullptr);
+    scb_sample_one<true>(v, valid, valid ? queue[q] : 0, ekin, dirs, seed, stream, oE, oD, nullptr, nullptr);
   }
 }
-
-__global__ void k_zero_u64_scb(unsigned long long* p) { *p = 0ull; }
 
 static inline unsigned scb_grid(int64_t n) {
   int64_t g = (n + 255) / 256;
@@ -143,23 +205,32 @@ static inline unsigned scb_grid(int64_t n) {
   return (unsigned)g;
 }
 
+// sorts the batch by scan length into w.perm and clears the queue counter
+static void scb_order(const SCBView& v, const double* ekin, const double* dirs, int64_t n, const ScbWork& w,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_scb_clear, dim3(1), dim3(256), 0, st, w.hist, w.qcount);
+  hipLaunchKernelGGL(k_scb_keys, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, w.keys, w.hist);
+  hipLaunchKernelGGL(k_scb_offsets, dim3(1), dim3(kScbBuckets), 0, st, w.hist, w.cursor);
+  hipLaunchKernelGGL(k_scb_scatter, dim3(scb_grid(n)), dim3(256), 0, st, w.keys, n, w.cursor, w.perm);
+}
+
 void launch_scb_xs(const SCBView& v, const double* ekin, const double* dirs, int64_t n, double* out,
-                   int64_t* queue, unsigned long long* qcount, hipStream_t st) {
+                   const ScbWork& w, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_zero_u64_scb, dim3(1), dim3(1), 0, st, qcount);
-  hipLaunchKernelGGL(k_scb_xs_main, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, out, queue, qcount);
-  hipLaunchKernelGGL(k_scb_xs_slow, dim3(256), dim3(256), 0, st, v, ekin, dirs, out, queue, qcount);
+  scb_order(v, ekin, dirs, n, w, st);
+  hipLaunchKernelGGL(k_scb_xs_main, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, w.perm, out, w.queue,
+                     w.qcount);
+  hipLaunchKernelGGL(k_scb_xs_slow, dim3(256), dim3(256), 0, st, v, ekin, dirs, out, w.queue, w.qcount);
 }
 
 void launch_scb_sample(const SCBView& v, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
-                       uint32_t stream, double* oE, double* oD, int64_t* queue, unsigned long long* qcount,
-                       hipStream_t st) {
+                       uint32_t stream, double* oE, double* oD, const ScbWork& w, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_zero_u64_scb, dim3(1), dim3(1), 0, st, qcount);
-  hipLaunchKernelGGL(k_scb_sample_main, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, seed, stream, oE, oD,
-                     queue, qcount);
-  hipLaunchKernelGGL(k_scb_sample_slow, dim3(256), dim3(256), 0, st, v, ekin, dirs, seed, stream, oE, oD, queue,
-                     qcount);
+  scb_order(v, ekin, dirs, n, w, st);
+  hipLaunchKernelGGL(k_scb_sample_main, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, w.perm, seed, stream,
+                     oE, oD, w.queue, w.qcount);
+  hipLaunchKernelGGL(k_scb_sample_slow, dim3(256), dim3(256), 0, st, v, ekin, dirs, seed, stream, oE, oD, w.queue,
+                     w.qcount);
 }
 
 }  // namespace ncamd

@@ -262,3 +262,53 @@ def wavelengths_uniform(n, wl_lo, wl_hi, seed=6):
     rng = np.random.default_rng(seed)
     return rng.uniform(wl_lo, wl_hi, size=n)
 
+
+
+def freegas_like_sab(nalpha, nbeta, mass_amu, temperature=293.0, alpha_max=None, beta_min=None, beta_max=None):
+    """S(alpha,beta) of a free gas of mass mass_amu (non-symmetric form,
+    S(a,-b) = e^b S(a,b)): sqrt(A/(4 pi a)) exp(-(A b + a)^2 / (4 A a)),
+    on a log-spaced alpha grid and a beta grid dense around 0."""
+    A = mass_amu / 1.00866491588
+    amax = alpha_max if alpha_max is not None else 40.0 * A
+    alpha = np.geomspace(1e-3, amax, nalpha)
+    bmin = beta_min if beta_min is not None else -40.0
+    bmax = beta_max if beta_max is not None else 25.0
+    u = np.linspace(-1.0, 1.0, nbeta)
+    beta = np.where(u < 0, -bmin * u ** 3, bmax * u ** 3)
+    beta = np.sort(beta)
+    a = alpha[None, :]
+    b = beta[:, None]
+    sab = np.sqrt(A / (4 * np.pi * a)) * np.exp(-(A * b + a) ** 2 / (4 * A * a))
+    sab = np.maximum(sab, 1e-300)
+    return dict(alpha=alpha, beta=beta, sab=sab, temperature=temperature, mass=mass_amu,
+                sigma_bound=float(4.0 * (A + 1) ** 2 / A ** 2))
+
+
+def batch_materials(nmat=512, seed=11, planes=(1e2, 1e5), grids=((50, 100), (300, 1200)), counts=(1e4, 1e6)):
+    """Inputs of the many-materials workload: per material a synthetic plane
+    list (N_planes log-uniform in `planes`, d with density ~ d^-4 in
+    [0.25, 8] Aa, |F|^2 x multiplicity with a Debye-Waller fall-off), one to
+    three incoherent species, a free-gas-like S(alpha,beta) table with a grid
+    between grids[0] and grids[1] (log-uniform per axis) and a neutron count
+    log-uniform in `counts`."""
+    rng = np.random.default_rng(seed)
+    mats = []
+    for i in range(nmat):
+        npl = int(round(np.exp(rng.uniform(np.log(planes[0]), np.log(planes[1])))))
+        u = rng.uniform(size=npl)
+        dmin, dmax = 0.25, 8.0
+        d = 1.0 / ((dmin ** -3 - u * (dmin ** -3 - dmax ** -3)) ** (1.0 / 3.0))
+        d = np.sort(d)[::-1].copy()
+        msd = rng.uniform(0.005, 0.02)
+        mult = 2 * rng.integers(1, 25, size=npl)
+        fsq = rng.uniform(0.01, 5.0, size=npl) * np.exp(-msd * (2 * np.pi / d) ** 2)
+        v0n = rng.uniform(50.0, 2000.0) * rng.integers(1, 25)
+        ns = int(rng.integers(1, 4))
+        ei = dict(msd=rng.uniform(0.005, 0.03, size=ns), bixs=rng.uniform(0.0, 5.0, size=ns),
+                  scale=rng.dirichlet(np.ones(ns)))
+        na = int(round(np.exp(rng.uniform(np.log(grids[0][0]), np.log(grids[1][0])))))
+        nb = int(round(np.exp(rng.uniform(np.log(grids[0][1]), np.log(grids[1][1])))))
+        sab = freegas_like_sab(na, nb, mass_amu=float(rng.uniform(5.0, 200.0)))
+        n = int(round(np.exp(rng.uniform(np.log(counts[0]), np.log(counts[1])))))
+        mats.append(dict(pb=(v0n, d, fsq * mult), elinc=ei, sab=sab, count=n))
+    return mats

@@ -1,0 +1,166 @@
+"""Exact envelope sampler of S(alpha,beta) (kinematic-box rejection) on the
+GPU.  Its samples follow the interpolated table density exactly, so they are
+checked against (1) the envelope property itself (the stored box bounds
+dominate the target on a fine grid of every box, evaluated here with numpy),
+(2) the exact beta marginal of the interpolated table obtained by numerical
+integration, and (3) the plain sampler of the CPU checker by two-sample KS.
+Neutrons the envelope sampler hands back fall back to the plain sampler on
+their own Philox stream, so they must equal the plain GPU sampler exactly."""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from ncrystal_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+KB = 8.6173303e-5
+
+
+@pytest.fixture(scope="module")
+def env():
+    from ncrystal_amd import _capi, core
+    if _capi.lib().ncamd_device_count() < 1:
+        pytest.fail("no GPU visible to libncrystal_amd")
+    return _capi, core
+
+
+def _table():
+    d = synth.sab_table_c3()
+    return d, d["alpha"], d["beta"], d["sab"].reshape(len(d["beta"]), len(d["alpha"]))
+
+
+def _interp_s(a, b, S, al, be):
+    """SABCellEval (alpha first: log-linear with linear fallback, then linear in beta)"""
+    ia = np.clip(np.searchsorted(a, al, side="right") - 1, 0, len(a) - 2)
+    ib = np.clip(np.searchsorted(b, be, side="right") - 1, 0, len(b) - 2)
+    a0, a1 = a[ia], a[ia + 1]
+
+    def row(j):
+        s0, s1 = S[j, ia], S[j, ia + 1]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            ll = np.exp(np.log(s0) + (np.log(s1) - np.log(s0)) * (al - a0) / (a1 - a0))
+        lin = s0 + (s1 - s0) * (al - a0) / (a1 - a0)
+        return np.where(s0 * s1 == 0.0, lin, ll)
+
+    tb = (be - b[ib]) / (b[ib + 1] - b[ib])
+    v = row(ib) + (row(ib + 1) - row(ib)) * tb
+    inside = (al >= a[0]) & (al <= a[-1]) & (be >= b[0]) & (be <= b[-1])
+    return np.where(inside, v, 0.0)
+
+
+def test_envelope_bounds_target(env):
+    C, core = env
+    d, a, b, S = _table()
+    g = core.create_sabscatter(d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"], d["sigma_bound"])
+    nn, bw, bk = 64, 32, 16
+    g.use_envelope_sampler(nintervals=nn, emin=1e-3, emax=1.0, nbox_w=bw, nbox_k=bk)
+    node, envv = g.envelope_tables()
+    kT = KB * d["temperature"]
+    rng = np.random.default_rng(1)
+    for n in (0, 13, 40, nn - 1):
+        eps = node[n + 1] / kT
+        wmax = math.sqrt(b[-1] + eps)
+        for box in rng.choice(bw * bk, 200, replace=False):
+            iw, ik = divmod(int(box), bk)
+            w = (iw + np.linspace(0, 1, 24)) / bw
+            k = (ik + np.linspace(0, 1, 24)) / bk
+            W, K = np.meshgrid(w, k, indexing="ij")
+            r = W * wmax
+            be = r * r - eps
+            al = r * r + eps - 2 * math.sqrt(eps) * r * (1 - 2 * K)
+            t = _interp_s(a, b, S, al, be) * 8 * math.sqrt(eps) * wmax ** 3 * W * W
+            assert np.all(t <= envv[n, box] * (1 + 1e-12)), (n, box)
+        # the envelope is not loose on average: it integrates to at most ~1.5x
+        # the target over the unit square
+        w = (np.arange(256) + 0.5) / 256
+        k = (np.arange(128) + 0.5) / 128
+        W, K = np.meshgrid(w, k, indexing="ij")
+        r = W * wmax
+        t = _interp_s(a, b, S, r * r + eps - 2 * math.sqrt(eps) * r * (1 - 2 * K), r * r - eps) * \
+            8 * math.sqrt(eps) * wmax ** 3 * W * W
+        print("interval", n, "envelope efficiency", t.mean() / envv[n].mean())
+        assert t.mean() / envv[n].mean() > 0.5
+
+
+def _beta_cdf_exact(a, b, S, E, kT, sub=16):
+    eps = E / kT
+    af = np.concatenate([a[:-1, None] + (a[1:] - a[:-1])[:, None] * (np.arange(32) / 32)[None, :]]).ravel()
+    af = np.append(af, a[-1])
+    bq = np.concatenate([b[:-1, None] + (b[1:] - b[:-1])[:, None] * (np.arange(sub) / sub)[None, :]]).ravel()
+    bq = np.append(bq, b[-1])
+    bq = bq[bq > -eps]
+    # marginal density in beta: integral over the kinematic alpha range of the interpolated S
+    m = np.empty(len(bq))
+    for j0 in range(0, len(bq), 2000):
+        bb = bq[j0:j0 + 2000]
+        lo = (np.sqrt(eps + bb) - math.sqrt(eps)) ** 2
+        hi = (np.sqrt(eps + bb) + math.sqrt(eps)) ** 2
+        AL = np.clip(af[None, :], lo[:, None], hi[:, None])
+        val = _interp_s(a, b, S, AL, np.broadcast_to(bb[:, None], AL.shape))
+        m[j0:j0 + 2000] = np.sum(0.5 * (val[:, 1:] + val[:, :-1]) * np.diff(AL, axis=1), axis=1)
+    F = np.concatenate([[0.0], np.cumsum(0.5 * (m[1:] + m[:-1]) * np.diff(bq))])
+    return bq, F / F[-1]
+
+
+def test_envelope_sampler_distribution(env, oracle):
+    C, core = env
+    d, a, b, S = _table()
+    args = (d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"], d["sigma_bound"])
+    g = core.create_sabscatter(*args)
+    g.use_envelope_sampler()
+    o = oracle.sab(*args)
+    kT = KB * d["temperature"]
+    st = C.Stream()
+    n = 2000000
+    for E in (2e-3, 0.025, 0.2, 0.9):
+        e = np.full(n, E)
+        de = C.DeviceArray.from_numpy(e)
+        da, db = C.DeviceArray(n), C.DeviceArray(n)
+        g.setRandomSeed(3)
+        nfb = g.dev_sample_alphabeta_envelope(de.ptr, n, da.ptr, db.ptr, stream=st.ptr)
+        al, be = da.numpy(), db.numpy()
+        assert nfb < 1e-3 * n
+        # kinematically valid and inside the table
+        lo = (np.sqrt(E / kT + be) - math.sqrt(E / kT)) ** 2
+        hi = (np.sqrt(E / kT + be) + math.sqrt(E / kT)) ** 2
+        assert np.all(be > -E / kT) and np.all(al >= lo * (1 - 1e-9)) and np.all(al <= hi * (1 + 1e-9))
+        # beta marginal vs the exact marginal of the interpolated table
+        bq, F = _beta_cdf_exact(a, b, S, E, kT)
+        bs = np.sort(be)
+        ks_exact = np.max(np.abs(np.searchsorted(bs, bq, side="right") / n - F))
+        # two-sample KS vs the plain sampler of the CPU checker
+        ra, rb, nerr = o.sample_ab(e, seed=5, nthreads=8)
+        ks_b = stats.ks_2samp(be, rb).statistic
+        ks_a = stats.ks_2samp(al, ra).statistic
+        print(f"E={E}: fallback {nfb}, KS(beta vs exact marginal) {ks_exact:.2e}, "
+              f"KS vs plain: beta {ks_b:.2e} alpha {ks_a:.2e}")
+        assert ks_exact < 1.6 / math.sqrt(n) + 2e-4
+        assert ks_b < 2.5 / math.sqrt(n) + 3e-4 and ks_a < 2.5 / math.sqrt(n) + 3e-4
+
+
+def test_envelope_sampler_iso_and_fallback(env):
+    C, core = env
+    d, a, b, S = _table()
+    args = (d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"], d["sigma_bound"])
+    plain = core.create_sabscatter(*args)
+    kbs = core.create_sabscatter(*args)
+    # covered range [1e-3, 0.5) eV: neutrons outside take the plain sampler from their first draw
+    kbs.use_envelope_sampler(nintervals=256, emin=1e-3, emax=0.5)
+    e = synth.energies_logu(400000, 1e-4, 1.0, seed=4)
+    plain.setRandomSeed(21)
+    kbs.setRandomSeed(21)
+    pe, pm = plain.sampleScatterIsotropic(e)
+    ke, km = kbs.sampleScatterIsotropic(e)
+    out = (e < 1e-3) | (e >= 0.5)
+    assert np.array_equal(pe[out], ke[out]) and np.array_equal(pm[out], km[out])
+    assert np.all(np.abs(km) <= 1.0) and np.all(ke >= 0.0)
+    inr = ~out
+    assert stats.ks_2samp(pm[inr], km[inr]).statistic < 0.01
+    assert stats.ks_2samp(pe[inr], ke[inr]).statistic < 0.01
+    kbs.use_plain_sampler()
+    kbs.setRandomSeed(21)
+    ke2, km2 = kbs.sampleScatterIsotropic(e)
+    assert np.array_equal(ke2, pe) and np.array_equal(km2, pm)

@@ -123,6 +123,40 @@ def test_powderbragg_sampling(amd, oracle):
     assert np.array_equal(om, rm)
 
 
+def test_powderbragg_c2_ortho(amd, oracle):
+    # orthorhombic 24-atom cell, dcutoff 0.1 Aa: ~2.6e5 hkl families, ~7.8e4
+    # distinct planes; GPU families/tables vs the oracle, xs bit-exact (FP64)
+    # and within 1e-6 (FP32 Kahan tables), sampling identical per neutron
+    core, _ = amd
+    cr = synth.ortho_random(seed=1, dcutoff=0.1)
+    sp = [(s.csl, oracle.debye_msd(s.debye_temp, cr.temperature, s.mass), s.positions) for s in cr.species]
+    h1, m1, d1, f1 = core.hkl_orthogonal(cr.a, cr.b, cr.c, sp, cr.dcutoff)
+    h2, m2, d2, f2 = oracle.hkl_orthogonal(cr.a, cr.b, cr.c, sp, cr.dcutoff, maxout=3000000)
+    print("c2 families", len(h1))
+    assert np.array_equal(h1, h2) and np.array_equal(m1, m2) and np.array_equal(d1, d2)
+    assert rel_err(f1, f2).max() <= 1e-13
+    v0n = cr.volume * cr.n_atoms
+    g = core.create_powderbragg(v0n, d2, f2 * m2)
+    o = oracle.powderbragg(v0n, d2, f2 * m2)
+    ge, gf, g32 = core.powderbragg_tables(g)
+    oe_, of_ = oracle.pb_tables(o)
+    print("c2 planes", len(ge))
+    assert np.array_equal(ge, oe_) and np.array_equal(gf, of_)
+    e = synth.energies_logu(1000000, 1e-4, 1.0, seed=3)
+    e[:4] = [ge[0], ge[-1], ge[len(ge) // 2], np.nextafter(ge[0], 0)]  # exact thresholds and just below
+    x64 = g.crossSectionIsotropic(e)
+    assert np.array_equal(x64, o.xs(e, nthreads=8))
+    x32 = g.crossSectionIsotropic_fp32(e)
+    nz = x64 > 0
+    err = rel_err(x32[nz], x64[nz]).max()
+    print("c2 fp32 max rel err", err)
+    assert err <= 1e-6 and np.array_equal(x32 > 0, nz)
+    g.setRandomSeed(5)
+    se, sm = g.sampleScatterIsotropic(e)
+    re_, rm, nerr = o.sample_iso(e, seed=5, nthreads=8)
+    assert nerr == 0 and np.array_equal(se, re_) and np.array_equal(sm, rm)
+
+
 def test_elinc(amd, oracle):
     core, _ = amd
     msd = [0.01, 0.02, 0.005]
