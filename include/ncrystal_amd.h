@@ -168,6 +168,31 @@ int ncamd_sab_envelope_tables(ncamd_process_t, double* node_e, double* env);
 int64_t ncamd_dev_sab_sample_alphabeta_envelope(ncamd_process_t, const double* d_ekin, unsigned long n,
                                                 double* d_alpha, double* d_beta, void* stream);
 
+/* ---------- learned proposal model (no counterpart in NCrystal) ----------
+ * MLP: 8 energy features -> 3 x 64 (SiLU) -> 48 outputs = 8 bivariate
+ * Gaussian components over the kinematic coordinates (w, kappa) of an
+ * S(alpha,beta) sample.  BF16 matrix-core products with FP32 accumulation,
+ * FP32 Adam.  Energies are scaled over [emin, emax]. */
+typedef struct { void* internal; } ncamd_mlp_t;
+ncamd_mlp_t ncamd_mlp_create(uint64_t seed, double emin, double emax);
+void ncamd_mlp_destroy(ncamd_mlp_t*);
+int ncamd_mlp_nparams(void);
+int ncamd_mlp_get_params(ncamd_mlp_t, float* out);
+int ncamd_mlp_set_params(ncamd_mlp_t, const float* in);
+/* test hook of the matrix-core product used by the model: C (+)= op(A) op(B)
+ * with BF16 operands and FP32 accumulation; A stored M x K (K x M if ta),
+ * B stored K x N (N x K if tb), C M x N, all row-major FP32 host arrays */
+int ncamd_debug_gemm_bf16(const float* A, const float* B, float* C, int M, int N, int K, int ta, int tb, int accum);
+/* outputs [n*48] at the given energies */
+int ncamd_mlp_forward(ncamd_mlp_t, const double* ekin, int n, float* out);
+/* mean mixture negative log-likelihood of targets (w, kappa) and its
+ * gradient with respect to all parameters [ncamd_mlp_nparams()] */
+double ncamd_mlp_loss_grad(ncamd_mlp_t, const double* ekin, const double* w, const double* kappa, int n,
+                           float* grad);
+/* trains on fresh batches drawn by the plain sampler of an S(alpha,beta)
+ * process; returns the mean loss over the last 10% of the steps */
+double ncamd_mlp_train_sab(ncamd_mlp_t, ncamd_process_t sab, int steps, int batch, double lr, uint64_t seed);
+
 /* ---------- RNG (ncrystal.h:1067-1087) ----------
  * Each process handle owns a counter-based stream: (seed, call counter).
  * Every sampling call consumes one call-counter value; neutron k of a call

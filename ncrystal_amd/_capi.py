@@ -56,6 +56,10 @@ class BatchHandle(ctypes.Structure):
     _fields_ = [("internal", ctypes.c_void_p)]
 
 
+class MlpHandle(ctypes.Structure):
+    _fields_ = [("internal", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -126,6 +130,17 @@ def lib():
         "ncamd_sab_use_plain_sampler": (ctypes.c_int, [Handle]),
         "ncamd_sab_envelope_tables": (ctypes.c_int, [Handle, _dp, _dp]),
         "ncamd_dev_sab_sample_alphabeta_envelope": (ctypes.c_int64, [Handle, _vp, _ul, _vp, _vp, _vp]),
+        "ncamd_mlp_create": (MlpHandle, [ctypes.c_uint64, ctypes.c_double, ctypes.c_double]),
+        "ncamd_mlp_destroy": (None, [ctypes.POINTER(MlpHandle)]),
+        "ncamd_mlp_nparams": (ctypes.c_int, []),
+        "ncamd_mlp_get_params": (ctypes.c_int, [MlpHandle, _fp]),
+        "ncamd_mlp_set_params": (ctypes.c_int, [MlpHandle, _fp]),
+        "ncamd_mlp_forward": (ctypes.c_int, [MlpHandle, _dp, ctypes.c_int, _fp]),
+        "ncamd_mlp_loss_grad": (ctypes.c_double, [MlpHandle, _dp, _dp, _dp, ctypes.c_int, _fp]),
+        "ncamd_mlp_train_sab": (ctypes.c_double, [MlpHandle, Handle, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                  ctypes.c_uint64]),
+        "ncamd_debug_gemm_bf16": (ctypes.c_int, [_fp, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "ncamd_batch_create": (BatchHandle, [ctypes.c_int, ctypes.POINTER(Handle), _i64p]),
         "ncamd_batch_destroy": (None, [ctypes.POINTER(BatchHandle)]),
         "ncamd_batch_total": (ctypes.c_int64, [BatchHandle]),

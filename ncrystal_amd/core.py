@@ -383,3 +383,65 @@ class Batch:
 
     def dev_autotune(self, d_ekin, d_o1, d_o2, stream=0):
         return C.call("ncamd_dev_batch_autotune", self._h, d_ekin, d_o1, d_o2, stream)
+
+
+class MLPProposal:
+    """The learned proposal model (8 energy features -> 3 x 64 -> 8-component
+    bivariate Gaussian mixture over (w, kappa)), trained on the GPU."""
+
+    def __init__(self, emin, emax, seed=1):
+        h = C.call("ncamd_mlp_create", int(seed), float(emin), float(emax))
+        if not h.internal:
+            C.check()
+            raise C.NCAMDError("model creation failed")
+        self._h = h
+        self.emin, self.emax = float(emin), float(emax)
+
+    def __del__(self):
+        try:
+            if self._h is not None and self._h.internal:
+                C.lib().ncamd_mlp_destroy(ctypes.byref(self._h))
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def nparams(self):
+        return C.call("ncamd_mlp_nparams")
+
+    def params(self):
+        out = np.empty(self.nparams(), dtype=np.float32)
+        C.call("ncamd_mlp_get_params", self._h, C.fptr(out))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        C.call("ncamd_mlp_set_params", self._h, C.fptr(p))
+
+    def forward(self, ekin):
+        e = C.as_f64(np.atleast_1d(ekin))
+        out = np.empty((e.size, 48), dtype=np.float32)
+        C.call("ncamd_mlp_forward", self._h, C.dptr(e), e.size, C.fptr(out))
+        return out
+
+    def loss_grad(self, ekin, w, kappa):
+        e, ww, kk = C.as_f64(ekin), C.as_f64(w), C.as_f64(kappa)
+        g = np.empty(self.nparams(), dtype=np.float32)
+        loss = C.call("ncamd_mlp_loss_grad", self._h, C.dptr(e), C.dptr(ww), C.dptr(kk), e.size, C.fptr(g))
+        return loss, g
+
+    def train_sab(self, sab, steps=2000, batch=8192, lr=3e-3, seed=1):
+        return C.call("ncamd_mlp_train_sab", self._h, sab.handle, int(steps), int(batch), float(lr), int(seed))
+
+
+def debug_gemm_bf16(A, B, C0=None, ta=False, tb=False, accum=False):
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    M = A.shape[1] if ta else A.shape[0]
+    K = A.shape[0] if ta else A.shape[1]
+    N = B.shape[0] if tb else B.shape[1]
+    out = np.zeros((M, N), dtype=np.float32) if C0 is None else np.ascontiguousarray(C0, dtype=np.float32).copy()
+    C.call("ncamd_debug_gemm_bf16", C.fptr(A), C.fptr(B), C.fptr(out), M, N, K, int(ta), int(tb), int(accum))
+    return out

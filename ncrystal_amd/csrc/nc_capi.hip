@@ -1403,6 +1403,130 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
   p->use_kb = true;
 }
 
+// --------------------------------------------------------- MLP proposal ---
+// Host side of the MLP (kernels in nc_mlp.hip): parameters, Adam state and
+// activation buffers live on the device; one training step = sample a batch
+// with the plain S(alpha,beta) sampler, forward, mixture NLL, backward, Adam.
+constexpr uint32_t kMlpMagic = 0x6e636d6cU;
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kMlpIn * kMlpHidden, kOffW2 = kOffB1 + kMlpHidden,
+              kOffB2 = kOffW2 + kMlpHidden * kMlpHidden, kOffW3 = kOffB2 + kMlpHidden,
+              kOffB3 = kOffW3 + kMlpHidden * kMlpHidden, kOffW4 = kOffB3 + kMlpHidden,
+              kOffB4 = kOffW4 + kMlpHidden * kMlpOut, kMlpParams = kOffB4 + kMlpOut;
+
+struct Mlp {
+  uint32_t magic = kMlpMagic;
+  DevBuf w, g, m, v, loss;
+  int step = 0;
+  double lelo = 0.0, linv = 1.0;  // feature scaling: x = (log E - lelo) * linv
+  int cap = 0;
+  DevBuf X, T, Z1, H1, Z2, H2, Z3, H3, Y, dY, dA, dB;
+  DevBuf E, a, b;
+  ~Mlp() { magic = 0; }
+  void ensure(int n) {
+    if (n <= cap) return;
+    cap = n;
+    const size_t f = sizeof(float);
+    X.alloc(f * n * kMlpIn); T.alloc(f * n * 2);
+    Z1.alloc(f * n * kMlpHidden); H1.alloc(f * n * kMlpHidden);
+    Z2.alloc(f * n * kMlpHidden); H2.alloc(f * n * kMlpHidden);
+    Z3.alloc(f * n * kMlpHidden); H3.alloc(f * n * kMlpHidden);
+    Y.alloc(f * n * kMlpOut); dY.alloc(f * n * kMlpOut);
+    dA.alloc(f * n * kMlpHidden); dB.alloc(f * n * kMlpHidden);
+    E.alloc(sizeof(double) * n); a.alloc(sizeof(double) * n); b.alloc(sizeof(double) * n);
+  }
+  float* P(int off) { return w.as<float>() + off; }
+  float* G(int off) { return g.as<float>() + off; }
+};
+
+Mlp* extract_mlp(ncamd_mlp_t h) {
+  Mlp* m = static_cast<Mlp*>(h.internal);
+  if (!m || m->magic != kMlpMagic) raise("LogicError", "invalid ncamd_mlp_t handle");
+  return m;
+}
+
+// forward pass of n rows of features X (in m->X) into m->Y
+void mlp_forward(Mlp* m, int n, hipStream_t st) {
+  const int H = kMlpHidden;
+  launch_gemm_bf16(m->X.as<float>(), kMlpIn, false, m->P(kOffW1), H, false, m->Z1.as<float>(), H, m->P(kOffB1), n, H,
+                   kMlpIn, false, st);
+  launch_silu(m->Z1.as<float>(), m->H1.as<float>(), (int64_t)n * H, st);
+  launch_gemm_bf16(m->H1.as<float>(), H, false, m->P(kOffW2), H, false, m->Z2.as<float>(), H, m->P(kOffB2), n, H, H,
+                   false, st);
+  launch_silu(m->Z2.as<float>(), m->H2.as<float>(), (int64_t)n * H, st);
+  launch_gemm_bf16(m->H2.as<float>(), H, false, m->P(kOffW3), H, false, m->Z3.as<float>(), H, m->P(kOffB3), n, H, H,
+                   false, st);
+  launch_silu(m->Z3.as<float>(), m->H3.as<float>(), (int64_t)n * H, st);
+  launch_gemm_bf16(m->H3.as<float>(), H, false, m->P(kOffW4), kMlpOut, false, m->Y.as<float>(), kMlpOut,
+                   m->P(kOffB4), n, kMlpOut, H, false, st);
+}
+
+// loss and gradients (into m->g) for targets m->T
+void mlp_loss_backward(Mlp* m, int n, hipStream_t st) {
+  const int H = kMlpHidden;
+  launch_fill_f32(m->g.as<float>(), kMlpParams, 0.f, st);
+  launch_fill_f32(m->loss.as<float>(), 1, 0.f, st);
+  launch_mix_loss(m->Y.as<float>(), m->T.as<float>(), n, m->dY.as<float>(), m->loss.as<float>(), st);
+  float* dY = m->dY.as<float>();
+  float* dA = m->dA.as<float>();
+  float* dB = m->dB.as<float>();
+  // layer 4
+  launch_gemm_bf16(m->H3.as<float>(), H, true, dY, kMlpOut, false, m->G(kOffW4), kMlpOut, nullptr, H, kMlpOut, n,
+                   true, st);
+  launch_colsum(dY, n, kMlpOut, m->G(kOffB4), st);
+  launch_gemm_bf16(dY, kMlpOut, false, m->P(kOffW4), kMlpOut, true, dA, H, nullptr, n, H, kMlpOut, false, st);
+  launch_silu_back(m->Z3.as<float>(), dA, (int64_t)n * H, st);
+  // layer 3
+  launch_gemm_bf16(m->H2.as<float>(), H, true, dA, H, false, m->G(kOffW3), H, nullptr, H, H, n, true, st);
+  launch_colsum(dA, n, H, m->G(kOffB3), st);
+  launch_gemm_bf16(dA, H, false, m->P(kOffW3), H, true, dB, H, nullptr, n, H, H, false, st);
+  launch_silu_back(m->Z2.as<float>(), dB, (int64_t)n * H, st);
+  // layer 2
+  launch_gemm_bf16(m->H1.as<float>(), H, true, dB, H, false, m->G(kOffW2), H, nullptr, H, H, n, true, st);
+  launch_colsum(dB, n, H, m->G(kOffB2), st);
+  launch_gemm_bf16(dB, H, false, m->P(kOffW2), H, true, dA, H, nullptr, n, H, H, false, st);
+  launch_silu_back(m->Z1.as<float>(), dA, (int64_t)n * H, st);
+  // layer 1
+  launch_gemm_bf16(m->X.as<float>(), kMlpIn, true, dA, H, false, m->G(kOffW1), H, nullptr, kMlpIn, H, n, true, st);
+  launch_colsum(dA, n, H, m->G(kOffB1), st);
+}
+
+Mlp* create_mlp(uint64_t seed, double emin, double emax) {
+  ensure_device();
+  if (!(emin > 0.0 && emax > emin)) raise("BadInput", "MLP: need 0 < emin < emax");
+  std::unique_ptr<Mlp> m(new Mlp());
+  m->lelo = std::log(emin);
+  m->linv = 1.0 / (std::log(emax) - std::log(emin));
+  std::vector<float> w((size_t)kMlpParams, 0.f);
+  uint64_t s = seed * 0x9E3779B97F4A7C15ull + 0x1234567ull;
+  auto unif = [&s]() {  // splitmix64 -> [0,1)
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1.0p-53;
+  };
+  auto init = [&](int off, int fin, int fout, double gain) {
+    const double lim = gain * std::sqrt(6.0 / (fin + fout));
+    for (int i = 0; i < fin * fout; ++i) w[(size_t)off + i] = (float)((2.0 * unif() - 1.0) * lim);
+  };
+  init(kOffW1, kMlpIn, kMlpHidden, 1.0);
+  init(kOffW2, kMlpHidden, kMlpHidden, 1.0);
+  init(kOffW3, kMlpHidden, kMlpHidden, 1.0);
+  init(kOffW4, kMlpHidden, kMlpOut, 0.1);
+  for (int c = 0; c < kMlpComp; ++c) {  // broad initial components spread along w
+    float* b4 = w.data() + kOffB4 + 6 * c;
+    b4[0] = 0.f; b4[1] = (c + 0.5f) / kMlpComp; b4[2] = 0.5f; b4[3] = std::log(0.2f); b4[4] = std::log(0.3f); b4[5] = 0.f;
+  }
+  m->w.upload(w.data(), w.size());
+  m->g.alloc(sizeof(float) * kMlpParams);
+  std::vector<float> z((size_t)kMlpParams, 0.f);
+  m->m.upload(z.data(), z.size());
+  m->v.upload(z.data(), z.size());
+  m->loss.alloc(sizeof(float));
+  return m.release();
+}
+
 // ---------------------------------------------------- batched evaluation ---
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
@@ -2084,6 +2208,127 @@ int64_t ncamd_dev_sab_sample_alphabeta_envelope(ncamd_process_t h, const double*
   return (int64_t)nf;
   NCAMD_CATCH
   return -1;
+}
+
+ncamd_mlp_t ncamd_mlp_create(uint64_t seed, double emin, double emax) {
+  ncamd_mlp_t out;
+  out.internal = nullptr;
+  NCAMD_TRY out.internal = create_mlp(seed, emin, emax); NCAMD_CATCH
+  return out;
+}
+void ncamd_mlp_destroy(ncamd_mlp_t* h) {
+  NCAMD_TRY
+  delete extract_mlp(*h);
+  h->internal = nullptr;
+  NCAMD_CATCH
+}
+int ncamd_mlp_nparams(void) { return kMlpParams; }
+int ncamd_debug_gemm_bf16(const float* A, const float* B, float* C, int M, int N, int K, int ta, int tb, int accum) {
+  NCAMD_TRY
+  ensure_device();
+  if (M < 1 || N < 1 || K < 1) raise("BadInput", "empty matrix");
+  DevBuf dA, dB, dC;
+  dA.upload(A, (size_t)M * K);
+  dB.upload(B, (size_t)K * N);
+  dC.upload(C, (size_t)M * N);
+  launch_gemm_bf16(dA.as<float>(), ta ? M : K, ta != 0, dB.as<float>(), tb ? K : N, tb != 0, dC.as<float>(), N,
+                   nullptr, M, N, K, accum != 0, 0);
+  NCAMD_HIP(hipGetLastError());
+  NCAMD_HIP(hipMemcpy(C, dC.p, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToHost));
+  return 0;
+  NCAMD_CATCH
+  return -1;
+}
+int ncamd_mlp_get_params(ncamd_mlp_t h, float* out) {
+  NCAMD_TRY
+  Mlp* m = extract_mlp(h);
+  NCAMD_HIP(hipMemcpy(out, m->w.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+  return kMlpParams;
+  NCAMD_CATCH
+  return -1;
+}
+int ncamd_mlp_set_params(ncamd_mlp_t h, const float* in) {
+  NCAMD_TRY
+  Mlp* m = extract_mlp(h);
+  NCAMD_HIP(hipMemcpy(m->w.p, in, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+  return 0;
+  NCAMD_CATCH
+  return -1;
+}
+int ncamd_mlp_forward(ncamd_mlp_t h, const double* ekin, int n, float* out) {
+  NCAMD_TRY
+  Mlp* m = extract_mlp(h);
+  if (n < 1) return 0;
+  m->ensure(n);
+  NCAMD_HIP(hipMemcpy(m->E.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
+  launch_mlp_features(m->E.as<double>(), n, m->lelo, m->linv, m->X.as<float>(), 0);
+  mlp_forward(m, n, 0);
+  NCAMD_HIP(hipGetLastError());
+  NCAMD_HIP(hipMemcpy(out, m->Y.p, sizeof(float) * n * kMlpOut, hipMemcpyDeviceToHost));
+  return n;
+  NCAMD_CATCH
+  return -1;
+}
+double ncamd_mlp_loss_grad(ncamd_mlp_t h, const double* ekin, const double* w, const double* kappa, int n,
+                           float* grad) {
+  NCAMD_TRY
+  Mlp* m = extract_mlp(h);
+  if (n < 1) raise("BadInput", "empty batch");
+  m->ensure(n);
+  std::vector<float> t(2 * (size_t)n);
+  for (int i = 0; i < n; ++i) { t[2 * i] = (float)w[i]; t[2 * i + 1] = (float)kappa[i]; }
+  NCAMD_HIP(hipMemcpy(m->E.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
+  NCAMD_HIP(hipMemcpy(m->T.p, t.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+  launch_mlp_features(m->E.as<double>(), n, m->lelo, m->linv, m->X.as<float>(), 0);
+  mlp_forward(m, n, 0);
+  mlp_loss_backward(m, n, 0);
+  NCAMD_HIP(hipGetLastError());
+  float loss = 0.f;
+  NCAMD_HIP(hipMemcpy(&loss, m->loss.p, sizeof(float), hipMemcpyDeviceToHost));
+  if (grad) NCAMD_HIP(hipMemcpy(grad, m->g.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+  return loss;
+  NCAMD_CATCH
+  return -1.0;
+}
+// Trains on fresh batches of the plain S(alpha,beta) sampler (energies
+// log-uniform over the model's range).  Returns the mean loss of the last
+// 10% of the steps.
+double ncamd_mlp_train_sab(ncamd_mlp_t h, ncamd_process_t sab, int steps, int batch, double lr, uint64_t seed) {
+  NCAMD_TRY
+  Mlp* m = extract_mlp(h);
+  Proc* p = extract(sab);
+  if (p->kind != kKindSAB) raise("BadInput", "training needs an S(alpha,beta) process");
+  if (steps < 1 || batch < 64 || !(lr > 0.0)) raise("BadInput", "invalid training parameters");
+  m->ensure(batch);
+  if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
+  const double emin = std::exp(m->lelo), emax = std::exp(m->lelo + 1.0 / m->linv);
+  const double bmax = p->sab_beta_h.back();
+  double acc = 0.0;
+  int nacc = 0;
+  hipStream_t st = 0;
+  for (int it = 0; it < steps; ++it) {
+    launch_fill_loguniform(m->E.as<double>(), batch, std::log(emin), std::log(emax), seed + 7919ull * it, st);
+    launch_sab_ab(p->sab, m->E.as<double>(), batch, seed ^ 0x5ab5ab5abull, (uint32_t)it, m->a.as<double>(),
+                  m->b.as<double>(), p->d_err.as<unsigned long long>(), st);
+    launch_mlp_targets(m->E.as<double>(), m->a.as<double>(), m->b.as<double>(), batch, p->sab.kT, bmax, m->lelo,
+                       m->linv, m->X.as<float>(), m->T.as<float>(), st);
+    mlp_forward(m, batch, st);
+    mlp_loss_backward(m, batch, st);
+    ++m->step;
+    launch_adam(m->w.as<float>(), m->g.as<float>(), m->m.as<float>(), m->v.as<float>(), kMlpParams, (float)lr,
+                m->step, st);
+    if (it >= steps - std::max(1, steps / 10)) {
+      float l = 0.f;
+      NCAMD_HIP(hipMemcpy(&l, m->loss.p, sizeof(float), hipMemcpyDeviceToHost));
+      acc += l;
+      ++nacc;
+    }
+  }
+  NCAMD_HIP(hipGetLastError());
+  NCAMD_HIP(hipStreamSynchronize(st));
+  return acc / nacc;
+  NCAMD_CATCH
+  return -1.0;
 }
 
 int64_t ncamd_dev_check_errors(ncamd_process_t h, void* stream) {

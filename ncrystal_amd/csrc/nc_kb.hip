@@ -22,16 +22,11 @@ NCD int kb_cell_clamped(const double* g, int n, double x) {
   return up - 1 < 0 ? 0 : (up - 1 > n - 2 ? n - 2 : up - 1);
 }
 
-// envelope of t = S * J over each (interval, box): one thread per box
-__global__ void k_kb_envelope(SABView s, KBView k, const double* __restrict__ cmax, double* __restrict__ env) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)k.nn * k.nbox) return;
-  const int n = (int)(t / k.nbox), box = (int)(t % k.nbox);
-  const int iw = box / k.bk, ik = box - iw * k.bk;
-  const double4 par = k.node_par[n];
-  const double eps = par.x, seps = par.y, wmax = par.z, jfac = par.w;
-  const double w0 = (double)iw / k.bw, w1 = (double)(iw + 1) / k.bw;
-  const double k0 = (double)ik / k.bk, k1 = (double)(ik + 1) / k.bk;
+// upper bound of S over the (alpha, beta) image of the sub-box
+// [w0,w1] x [k0,k1] of an interval's unit square: maximum cell-corner S over
+// every table cell touched by the image's bounding rectangle
+NCD double kb_smax_subbox(const SABView& s, const double* __restrict__ cmax, double eps, double seps, double wmax,
+                          double w0, double w1, double k0, double k1) {
   const double r0 = w0 * wmax, r1 = w1 * wmax;
   double blo = r0 * r0 - eps, bhi = r1 * r1 - eps;
   // alpha(r, kappa) = r^2 + eps - 2 sqrt(eps) r (1 - 2 kappa): linear in
@@ -51,17 +46,40 @@ __global__ void k_kb_envelope(SABView s, KBView k, const double* __restrict__ cm
   const double ma = 1e-9 * (1.0 + fabs(alo) + fabs(ahi)) + 1e-9 * eps;
   const double mb = 1e-9 * (1.0 + fabs(blo) + fabs(bhi)) + 1e-9 * eps;
   alo -= ma; ahi += ma; blo -= mb; bhi += mb;
+  if (ahi < s.alpha[0] || alo > s.alpha[s.na - 1] || bhi < s.beta[0] || blo > s.beta[s.nb - 1]) return 0.0;
+  const int ia0 = kb_cell_clamped(s.alpha, s.na, dmax(alo, s.alpha[0]));
+  const int ia1 = kb_cell_clamped(s.alpha, s.na, dmin(ahi, s.alpha[s.na - 1]));
+  const int ib0 = kb_cell_clamped(s.beta, s.nb, dmax(blo, s.beta[0]));
+  const int ib1 = kb_cell_clamped(s.beta, s.nb, dmin(bhi, s.beta[s.nb - 1]));
+  const int nca = s.na - 1;
   double smax = 0.0;
-  if (!(ahi < s.alpha[0] || alo > s.alpha[s.na - 1] || bhi < s.beta[0] || blo > s.beta[s.nb - 1])) {
-    const int ia0 = kb_cell_clamped(s.alpha, s.na, dmax(alo, s.alpha[0]));
-    const int ia1 = kb_cell_clamped(s.alpha, s.na, dmin(ahi, s.alpha[s.na - 1]));
-    const int ib0 = kb_cell_clamped(s.beta, s.nb, dmax(blo, s.beta[0]));
-    const int ib1 = kb_cell_clamped(s.beta, s.nb, dmin(bhi, s.beta[s.nb - 1]));
-    const int nca = s.na - 1;
-    for (int ib = ib0; ib <= ib1; ++ib)
-      for (int ia = ia0; ia <= ia1; ++ia) smax = dmax(smax, cmax[(size_t)ib * nca + ia]);
+  for (int ib = ib0; ib <= ib1; ++ib)
+    for (int ia = ia0; ia <= ia1; ++ia) smax = dmax(smax, cmax[(size_t)ib * nca + ia]);
+  return smax;
+}
+
+// envelope of t = S * J over each (interval, box): one thread per box; the
+// box is split into kSub x kSub sub-boxes, each bounded by its own maximum
+// S times J at its upper w (J grows as w^2), and the box takes the largest
+constexpr int kKBSub = 4;
+__global__ void k_kb_envelope(SABView s, KBView k, const double* __restrict__ cmax, double* __restrict__ env) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)k.nn * k.nbox) return;
+  const int n = (int)(t / k.nbox), box = (int)(t % k.nbox);
+  const int iw = box / k.bk, ik = box - iw * k.bk;
+  const double4 par = k.node_par[n];
+  const double eps = par.x, seps = par.y, wmax = par.z, jfac = par.w;
+  double e = 0.0;
+  for (int sw = 0; sw < kKBSub; ++sw) {
+    const double w0 = (iw + (double)sw / kKBSub) / k.bw;
+    const double w1 = (sw + 1 == kKBSub) ? (double)(iw + 1) / k.bw : (iw + (double)(sw + 1) / kKBSub) / k.bw;
+    for (int sk = 0; sk < kKBSub; ++sk) {
+      const double k0 = (ik + (double)sk / kKBSub) / k.bk;
+      const double k1 = (sk + 1 == kKBSub) ? (double)(ik + 1) / k.bk : (ik + (double)(sk + 1) / kKBSub) / k.bk;
+      e = dmax(e, kb_smax_subbox(s, cmax, eps, seps, wmax, w0, w1, k0, k1) * (jfac * w1 * w1));
+    }
   }
-  env[t] = smax * (jfac * w1 * w1) * (1.0 + 1e-9);
+  env[t] = e * (1.0 + 1e-9);
 }
 
 // S(alpha,beta)-only sampling with the envelope sampler; neutrons outside

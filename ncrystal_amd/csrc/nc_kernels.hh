@@ -102,6 +102,21 @@ void launch_sab_queue(const SABView&, const double* ekin, int64_t n, int64_t tot
                       double* oE, double* oMu, unsigned long long* nerr, const int64_t* queue,
                       const unsigned long long* qcount, hipStream_t);
 
+// MLP proposal model (nc_mlp.hip): 8 input features, 3 hidden layers of 64,
+// 48 outputs = 8 bivariate Gaussian components over (w, kappa)
+constexpr int kMlpIn = 8, kMlpHidden = 64, kMlpComp = 8, kMlpOut = 6 * kMlpComp;
+void launch_gemm_bf16(const float* A, int lda, bool ta, const float* B, int ldb, bool tb, float* C, int ldc,
+                      const float* bias, int M, int N, int K, bool accum, hipStream_t);
+void launch_silu(const float* z, float* h, int64_t n, hipStream_t);
+void launch_silu_back(const float* z, float* d, int64_t n, hipStream_t);
+void launch_fill_f32(float* p, int64_t n, float v, hipStream_t);
+void launch_colsum(const float* D, int M, int N, float* out, hipStream_t);
+void launch_mlp_features(const double* E, int n, double lelo, double linv, float* X, hipStream_t);
+void launch_mix_loss(const float* Y, const float* T, int n, float* dY, float* loss, hipStream_t);
+void launch_adam(float* w, const float* g, float* m, float* v, int n, float lr, int step, hipStream_t);
+void launch_mlp_targets(const double* E, const double* a, const double* b, int n, double kT, double bmax,
+                        double lelo, double linv, float* X, float* T, hipStream_t);
+
 // many materials in one persistent launch (nc_batch.hip)
 struct BatchMat {
   CompView view;

