@@ -331,24 +331,123 @@ static int dhkl_before(const fam_t* rh, const fam_t* lh) {
   if (lh->mult != rh->mult) return lh->mult < rh->mult;
   return hkl_less(lh->rep, rh->rep);
 }
-static void stable_sort_fams(fam_t* v, int n) {
-  /* merge sort (stable) */
-  if (n < 2) return;
-  fam_t* tmp = (fam_t*)malloc(sizeof(fam_t) * (size_t)n);
-  for (int w = 1; w < n; w *= 2) {
-    for (int lo = 0; lo < n; lo += 2 * w) {
-      int mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
-      int i = lo, j = mid, k = lo;
-      while (i < mid && j < hi) {
-        if (dhkl_before(&v[j], &v[i])) tmp[k++] = v[j++];
-        else tmp[k++] = v[i++];
+/* The reference orders the family list with std::stable_sort and the
+   tolerance comparator above, which is not a strict weak ordering (|d1-d2|
+   <= 1e-6 counts as equal without being transitive), so the resulting order
+   depends on the exact merge pattern.  The restatement below follows the
+   algorithm of std::stable_sort in libstdc++ of GCC 11.4.0 (the toolchain of
+   this environment; bits/stl_algo.h): buffer of ceil(n/2) elements, each
+   half sorted by insertion sort of 7-element chunks followed by merge passes
+   that alternate between the range and the buffer (steps 7, 14, 28, ...),
+   then one adaptive merge of the halves (forward through the buffer when
+   both halves have equal length, backward otherwise).  comp(a,b) is
+   dhkl_before(a,b); ties take the left element. */
+static void ss_insertion(fam_t* f, fam_t* l) {
+  if (f == l) return;
+  for (fam_t* i = f + 1; i != l; ++i) {
+    if (dhkl_before(i, f)) {
+      fam_t v = *i;
+      memmove(f + 1, f, sizeof(fam_t) * (size_t)(i - f));
+      *f = v;
+    } else {
+      fam_t v = *i;
+      fam_t* last = i;
+      fam_t* next = i - 1;
+      while (dhkl_before(&v, next)) {
+        *last = *next;
+        last = next;
+        --next;
       }
-      while (i < mid) tmp[k++] = v[i++];
-      while (j < hi) tmp[k++] = v[j++];
+      *last = v;
     }
-    memcpy(v, tmp, sizeof(fam_t) * (size_t)n);
   }
-  free(tmp);
+}
+static fam_t* ss_move_merge(fam_t* f1, fam_t* l1, fam_t* f2, fam_t* l2, fam_t* out) {
+  while (f1 != l1 && f2 != l2) {
+    if (dhkl_before(f2, f1)) *out++ = *f2++;
+    else *out++ = *f1++;
+  }
+  while (f1 != l1) *out++ = *f1++;
+  while (f2 != l2) *out++ = *f2++;
+  return out;
+}
+static void ss_merge_loop(fam_t* f, fam_t* l, fam_t* out, long step) {
+  const long two = 2 * step;
+  while (l - f >= two) {
+    out = ss_move_merge(f, f + step, f + step, f + two, out);
+    f += two;
+  }
+  if (l - f < step) step = l - f;
+  ss_move_merge(f, f + step, f + step, l, out);
+}
+static void ss_merge_sort_with_buffer(fam_t* f, fam_t* l, fam_t* buf) {
+  const long len = l - f;
+  long step = 7;
+  fam_t* c = f;
+  while (l - c >= step) { ss_insertion(c, c + step); c += step; }
+  ss_insertion(c, l);
+  while (step < len) {
+    ss_merge_loop(f, l, buf, step);
+    step *= 2;
+    ss_merge_loop(buf, buf + len, f, step);
+    step *= 2;
+  }
+}
+static void stable_sort_fams(fam_t* v, int n) {
+  if (n < 2) return;
+  const long half = ((long)n + 1) / 2;
+  fam_t* buf = (fam_t*)malloc(sizeof(fam_t) * (size_t)half);
+  fam_t *first = v, *middle = v + half, *last = v + n;
+  ss_merge_sort_with_buffer(first, middle, buf);
+  ss_merge_sort_with_buffer(middle, last, buf);
+  const long len1 = middle - first, len2 = last - middle;
+  if (len1 <= len2) {
+    /* forward: first half to the buffer, merge into [first, last) */
+    memcpy(buf, first, sizeof(fam_t) * (size_t)len1);
+    fam_t *f1 = buf, *l1 = buf + len1, *f2 = middle, *out = first;
+    while (f1 != l1 && f2 != last) {
+      if (dhkl_before(f2, f1)) *out++ = *f2++;
+      else *out++ = *f1++;
+    }
+    while (f1 != l1) *out++ = *f1++;
+  } else {
+    /* backward: second half to the buffer, merge from the end */
+    memcpy(buf, middle, sizeof(fam_t) * (size_t)len2);
+    fam_t *f1 = first, *l1 = middle, *f2 = buf, *l2 = buf + len2, *res = last;
+    --l1;
+    --l2;
+    for (;;) {
+      if (dhkl_before(l2, l1)) {
+        *--res = *l1;
+        if (f1 == l1) {
+          ++l2;
+          while (l2 != f2) *--res = *--l2;
+          break;
+        }
+        --l1;
+      } else {
+        *--res = *l2;
+        if (f2 == l2) break;
+        --l2;
+      }
+    }
+  }
+  free(buf);
+}
+
+/* the family sort on its own (test hook): perm[i] = input index of the i-th
+   family after sorting */
+void orc_debug_sort_fams(int n, const double* dsp, const double* fsq, const int* mult, const int* hkl,
+                         int* perm) {
+  fam_t* v = (fam_t*)calloc((size_t)(n > 0 ? n : 1), sizeof(fam_t));
+  for (int i = 0; i < n; ++i) {
+    v[i].dsp = dsp[i]; v[i].fsq = fsq[i]; v[i].mult = mult[i];
+    v[i].rep.h = hkl[3 * i]; v[i].rep.k = hkl[3 * i + 1]; v[i].rep.l = hkl[3 * i + 2];
+    v[i].nmem = i;  /* carries the input index through the sort */
+  }
+  stable_sort_fams(v, n);
+  for (int i = 0; i < n; ++i) perm[i] = v[i].nmem;
+  free(v);
 }
 
 int orc_hkl_orthogonal(double a, double b, double c, int nspecies, const int* natoms,

@@ -54,6 +54,11 @@ int orc_hkl_orthogonal_members(double a, double b, double c,
                                double* out_dsp, double* out_fsq,
                                int64_t* out_mem_off, int* out_mem_hkl, int64_t maxmem);
 
+/* the hkl family sort alone (test hook): perm[i] = input index of the i-th
+   family in the reference's order */
+void orc_debug_sort_fams(int n, const double* dsp, const double* fsq, const int* mult, const int* hkl,
+                         int* perm);
+
 /* ---- processes ---- */
 typedef struct orc_proc orc_proc;
 orc_proc* orc_create_powderbragg(double v0_times_natoms, const double* dsp,

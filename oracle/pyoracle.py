@@ -62,6 +62,7 @@ def lib():
                                                                           ctypes.c_double, ctypes.c_int, _ip, _ip,
                                                                           _dp, _dp, _i64p, _ip, ctypes.c_int64]
         L.orc_hkl_orthogonal_members.restype = ctypes.c_int
+        L.orc_debug_sort_fams.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip, _ip]
         L.orc_create_scbragg.argtypes = [_dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _i64p, _ip, _dp,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_create_scbragg.restype = _vp
@@ -326,3 +327,13 @@ def scbragg(a, b, c, v0_times_natoms, dsp, fsq, mem_off, mem_hkl, cry2lab, mos_f
     ptr = lib().orc_create_scbragg(_d(L), v0_times_natoms, d.size, _d(d), _d(f), mo.ctypes.data_as(_i64p),
                                    _i(mh), _d(R), mos_fwhm, prec, ntrunc)
     return SCB(ptr, keep=(L,))
+
+
+def debug_sort_fams(dsp, fsq, mult, hkl):
+    d = np.ascontiguousarray(dsp, dtype=np.float64)
+    f = np.ascontiguousarray(fsq, dtype=np.float64)
+    m = np.ascontiguousarray(mult, dtype=np.int32)
+    h = np.ascontiguousarray(hkl, dtype=np.int32).ravel()
+    perm = np.empty(d.size, dtype=np.int32)
+    lib().orc_debug_sort_fams(d.size, _d(d), _d(f), _i(m), _i(h), _i(perm))
+    return perm
