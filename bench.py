@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["sab", "scbragg", "powderbragg"], default="sab")
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--ks-samples", type=int, default=10_000_000,
+                    help="samples per energy for the KS check against the CPU reference sampler (0: skip)")
     return ap.parse_args()
 
 
@@ -74,8 +77,10 @@ def cpu_baseline(table, batch, seconds, threads):
                        f"(E log-uniform 1e-3..1 eV, same table), {threads} threads, {dt:.1f} s")
 
 
-FP64_PEAK_TFLOPS = 78.6
-SCB_FLOPS_PER_NORMAL = 11  # dot (5) + (1-dot^2)*cos^2 (3) + dot*sin (1) + cta-|ds| (1) + square (1)
+FP32_PEAK_TFLOPS = 157.3
+# FP32 pre-filter of the scan per (neutron, demi-normal): dot product (3 mul + 2 add),
+# sin(theta) = w * 1/(2d) (1 mul), | |dot| - sin(theta) | (1 sub), compare (1)
+SCB_FLOPS_PER_NORMAL = 8
 
 
 def scb_setup(n):
@@ -162,8 +167,11 @@ def run_scbragg(args):
     x = xs.numpy()
     o = od.numpy().reshape(n, 3)
     assert np.all(x >= 0.0) and np.abs(np.linalg.norm(o[:100000], axis=1) - 1).max() < 1e-12, "invalid output"
+    # dominant kernel: the sampling scan; counted as one scan per neutron (the
+    # second, partial scan is left out) over the whole call's time (sorting
+    # and queue kernels included), so the figure is a lower bound
     flops = SCB_FLOPS_PER_NORMAL * scanned
-    achieved = flops / (xs_ms * 1e-3) / 1e12
+    achieved = flops / (samp_ms * 1e-3) / 1e12
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -179,9 +187,10 @@ def run_scbragg(args):
                                % (cr.a, cr.b, cr.c),
                    "neutrons_per_step": n, "families": int(len(t["inv2d"])),
                    "demi_normals": int(t["off"][-1]), "parallelism": "single GPU"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_scb_xs_main",
-                     "kernel_ms": xs_ms, "sample_ms": samp_ms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "k_scb_sample_main",
+                     "kernel_ms": samp_ms, "xs_call_ms": xs_ms,
+                     "xs_achieved_TFLOPs": flops / (xs_ms * 1e-3) / 1e12,
                      "flops_per_launch": flops, "normals_scanned_per_neutron": scanned / n},
         "cpu_baseline": cpu,
     }
@@ -276,6 +285,137 @@ def run_powderbragg(args):
     print(json.dumps(out))
 
 
+def ks_sorted(a_sorted, b):
+    """two-sample Kolmogorov-Smirnov statistic; `a_sorted` already sorted"""
+    import numpy as np
+    b = np.sort(b)
+    allv = np.concatenate([a_sorted, b])
+    ca = np.searchsorted(a_sorted, allv, side="right") / a_sorted.size
+    cb = np.searchsorted(b, allv, side="right") / b.size
+    return float(np.abs(ca - cb).max())
+
+
+def ks_check(procs, table, energies, nks, threads):
+    """two-sample KS of E' and of mu at fixed energies: each GPU sampler
+    against the plain sampler of the CPU checker (the reference algorithm)"""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import pyoracle
+    pyoracle.build()
+    o = pyoracle.sab(table["alpha"], table["beta"], table["sab"], table["temperature"], table["mass"],
+                     table["sigma_bound"])
+    out = {k: {} for k in procs}
+    for E in energies:
+        e = np.full(nks, E)
+        re_, rm, _ = o.sample_iso(e, seed=77, nthreads=threads)
+        re_, rm = np.sort(re_), np.sort(rm)
+        for k, p in procs.items():
+            p.setRandomSeed(501)
+            ge, gm = p.sampleScatterIsotropic(e)
+            out[k][str(E)] = max(ks_sorted(re_, ge), ks_sorted(rm, gm))
+    return out
+
+
+def run_sab(args):
+    import numpy as np
+    from ncrystal_amd import _capi as C
+    from ncrystal_amd import core, synth
+
+    table = synth.sab_table_c3(seed=1)
+    targs = (table["alpha"], table["beta"], table["sab"], table["temperature"], table["mass"], table["sigma_bound"])
+    n = args.n
+    stream = C.Stream()
+    ekin = C.DeviceArray(n)
+    oe = C.DeviceArray(n)
+    om = C.DeviceArray(n)
+    # inputs resident in HBM before timing: E log-uniform in [1e-3, 1] eV
+    C.call("ncamd_dev_fill_loguniform", ekin.ptr, n, 1e-3, 1.0, 1, stream.ptr)
+    stream.sync()
+
+    def measure(proc):
+        proc.setRandomSeed(1234)
+        for _ in range(args.warmup):
+            proc.dev_sampleScatterIsotropic(ekin.ptr, n, oe.ptr, om.ptr, stream=stream.ptr)
+        stream.sync()
+        nerr = proc.dev_check_errors(stream=stream.ptr)
+        ev0, ev1 = C.Event(), C.Event()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            proc.dev_sampleScatterIsotropic(ekin.ptr, n, oe.ptr, om.ptr, stream=stream.ptr)
+        ev1.record(stream)
+        stream.sync()
+        wall = time.perf_counter() - t0
+        nerr += proc.dev_check_errors(stream=stream.ptr)
+        mu = om.numpy(1000000)
+        ef = oe.numpy(1000000)
+        assert np.all(np.abs(mu) <= 1.0) and np.all(ef >= 0.0) and nerr == 0, "invalid samples"
+        return dict(wall=wall, ms=ev0.elapsed_ms(ev1) / args.steps, value=n * args.steps / wall)
+
+    plain = core.create_sabscatter(*targs)
+    r_plain = measure(plain)
+    # learned proposal: trained on the plain sampler's samples with the GPU kernels
+    t0 = time.perf_counter()
+    model = core.MLPProposal(1e-3, 1.0, seed=1)
+    loss = model.train_sab(plain, steps=args.train_steps, batch=8192, lr=3e-3, seed=1)
+    learned = core.create_sabscatter(*targs)
+    learned.use_learned_sampler(model, nintervals=512, emin=0.999e-3, emax=1.001)
+    t_setup = time.perf_counter() - t0
+    lst = learned.learned_stats()
+    r_learned = measure(learned)
+    envelope = core.create_sabscatter(*targs)
+    envelope.use_envelope_sampler(nintervals=512, emin=0.999e-3, emax=1.001)
+    r_env = measure(envelope)
+
+    if args.copy_calib:
+        # plain copy of 2*8*n bytes read + written: counter calibration for traffic
+        C.call("ncamd_memcpy_dtod", oe.ptr, ekin.ptr, 8 * n, stream.ptr)
+        stream.sync()
+
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    ks = None
+    if args.ks_samples > 0:
+        ks = ks_check({"plain": plain, "learned": learned, "envelope": envelope}, table,
+                      [0.001, 0.01, 0.025, 0.1, 0.5, 1.0], args.ks_samples, threads)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(table, args.cpu_batch, args.cpu_seconds, threads)
+
+    achieved = BYTES_PER_SAMPLE * n / (r_learned["ms"] * 1e-3) / 1e9
+    M, Z, coarse, fine = lst.T
+    out = {
+        "metric": "scatter samples/s on 1 MI355X: learned vs plain SAB sampler at equal KS error",
+        "value": r_learned["value"],
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r_learned["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "S(alpha,beta) 200x800 from 3-Gaussian VDOS, T=293K, "
+                               "E log-uniform 1e-3..1 eV, learned-proposal sampler (MLP 3x64 BF16) "
+                               "with exact accept/reject",
+                   "neutrons_per_step": n, "sampler": "learned", "parallelism": "single GPU",
+                   "plain_samples_per_s": r_plain["value"], "learned_over_plain": r_learned["value"] / r_plain["value"],
+                   "envelope_samples_per_s": r_env["value"],
+                   "train_steps": args.train_steps, "train_and_setup_s": t_setup, "train_final_loss": loss,
+                   "model_gain_over_coarse_envelope_median": float(np.median(coarse / (M * Z))),
+                   "fine_envelope_over_learned_mass_median": float(np.median(fine / (M * Z))),
+                   "ks_vs_reference": ks, "ks_samples_per_energy": args.ks_samples},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_lp_sample_iso", "kernel_ms": r_learned["ms"],
+                     "plain_kernel": "k_sab_sample_iso_main", "plain_kernel_ms": r_plain["ms"],
+                     "bytes_per_sample": BYTES_PER_SAMPLE},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -285,82 +425,7 @@ def main():
         return run_scbragg(args)
     if args.workload == "powderbragg":
         return run_powderbragg(args)
-    import numpy as np
-    from ncrystal_amd import _capi as C
-    from ncrystal_amd import core, synth
-
-    table = synth.sab_table_c3(seed=1)
-    sab = core.create_sabscatter(table["alpha"], table["beta"], table["sab"], table["temperature"],
-                                 table["mass"], table["sigma_bound"])
-    n = args.n
-    stream = C.Stream()
-    ekin = C.DeviceArray(n)
-    oe = C.DeviceArray(n)
-    om = C.DeviceArray(n)
-    # inputs resident in HBM before timing: E log-uniform in [1e-3, 1] eV
-    C.call("ncamd_dev_fill_loguniform", ekin.ptr, n, 1e-3, 1.0, 1, stream.ptr)
-    stream.sync()
-    sab.setRandomSeed(1234)
-
-    def step():
-        sab.dev_sampleScatterIsotropic(ekin.ptr, n, oe.ptr, om.ptr, stream=stream.ptr)
-
-    for _ in range(args.warmup):
-        step()
-    stream.sync()
-    nerr = sab.dev_check_errors(stream=stream.ptr)
-    ev0, ev1 = C.Event(), C.Event()
-    stream.sync()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        step()
-    ev1.record(stream)
-    stream.sync()
-    t1 = time.perf_counter()
-    nerr += sab.dev_check_errors(stream=stream.ptr)
-    wall = t1 - t0
-    kernel_ms = ev0.elapsed_ms(ev1) / args.steps
-    value = n * args.steps / wall
-    achieved = BYTES_PER_SAMPLE * n / (kernel_ms * 1e-3) / 1e9
-    # sanity of the outputs of the last step
-    mu = om.numpy(1000000)
-    ef = oe.numpy(1000000)
-    assert np.all(np.abs(mu) <= 1.0) and np.all(ef >= 0.0) and nerr == 0, "invalid samples"
-
-    if args.copy_calib:
-        # plain copy of 2*8*n bytes read + written: counter calibration for traffic
-        C.call("ncamd_memcpy_dtod", oe.ptr, ekin.ptr, 8 * n, stream.ptr)
-        stream.sync()
-
-    cpu = None
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(table, args.cpu_batch, args.cpu_seconds, threads)
-
-    out = {
-        "metric": "scatter samples/s on 1 MI355X: learned vs plain SAB sampler at equal KS error",
-        "value": value,
-        "unit": "samples/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "S(alpha,beta) 200x800 from 3-Gaussian VDOS, T=293K, "
-                               "E log-uniform 1e-3..1 eV, plain (Alg1) sampler",
-                   "neutrons_per_step": n, "sampler": "plain", "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sab_sample_iso_main", "kernel_ms": kernel_ms,
-                     "bytes_per_sample": BYTES_PER_SAMPLE},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out))
+    return run_sab(args)
 
 
 if __name__ == "__main__":

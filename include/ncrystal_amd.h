@@ -193,6 +193,24 @@ double ncamd_mlp_loss_grad(ncamd_mlp_t, const double* ekin, const double* w, con
  * process; returns the mean loss over the last 10% of the steps */
 double ncamd_mlp_train_sab(ncamd_mlp_t, ncamd_process_t sab, int steps, int batch, double lr, uint64_t seed);
 
+/* ---------- learned-proposal sampler of S(alpha,beta) (no counterpart in
+ * NCrystal) ----------
+ * Switches an S(alpha,beta) process to exact accept/reject against the
+ * interpolated table with a proposal made of the model's Gaussian mixture
+ * (evaluated at each energy interval's upper edge) plus non-negative patches
+ * on a coarse box grid (coarse_w x coarse_k, dividing nbox_w x nbox_k) that
+ * restore the envelope property; the scale of each interval minimises the
+ * expected number of proposals.  Falls back to the plain sampler like the
+ * envelope sampler.  Stats per interval: scale M, Z (total proposal mass),
+ * envelope mass of the coarse grid without the model, envelope mass of the
+ * fine grid; the expected acceptance is I/(M Z) with the model and I/coarse
+ * without it (I = integral of the target). */
+int ncamd_sab_use_learned_sampler(ncamd_process_t, ncamd_mlp_t, int nintervals, double emin, double emax,
+                                  int nbox_w, int nbox_k, int coarse_w, int coarse_k, int maxtries);
+int ncamd_sab_learned_stats(ncamd_process_t, double* out /* 4 per interval */);
+int64_t ncamd_dev_sab_sample_alphabeta_learned(ncamd_process_t, const double* d_ekin, unsigned long n,
+                                               double* d_alpha, double* d_beta, void* stream);
+
 /* ---------- RNG (ncrystal.h:1067-1087) ----------
  * Each process handle owns a counter-based stream: (seed, call counter).
  * Every sampling call consumes one call-counter value; neutron k of a call

@@ -316,6 +316,21 @@ class SABScatter(Scatter):
         C.call("ncamd_sab_envelope_tables", self._h, C.dptr(node), C.dptr(env))
         return node, env.reshape(nn, nbox)
 
+    def use_learned_sampler(self, model, nintervals=512, emin=0.0, emax=0.0, nbox_w=64, nbox_k=32, coarse_w=16,
+                            coarse_k=8, maxtries=64):
+        C.call("ncamd_sab_use_learned_sampler", self._h, model.handle, int(nintervals), float(emin), float(emax),
+               int(nbox_w), int(nbox_k), int(coarse_w), int(coarse_k), int(maxtries))
+        self._kb = (int(nintervals), int(nbox_w) * int(nbox_k))
+        self._lp_nn = int(nintervals)
+
+    def learned_stats(self):
+        out = np.empty(4 * self._lp_nn)
+        C.call("ncamd_sab_learned_stats", self._h, C.dptr(out))
+        return out.reshape(-1, 4)
+
+    def dev_sample_alphabeta_learned(self, d_ekin, n, d_a, d_b, stream=0):
+        return C.call("ncamd_dev_sab_sample_alphabeta_learned", self._h, d_ekin, n, d_a, d_b, stream)
+
     def dev_sample_alphabeta_envelope(self, d_ekin, n, d_a, d_b, stream=0):
         return C.call("ncamd_dev_sab_sample_alphabeta_envelope", self._h, d_ekin, n, d_a, d_b, stream)
 

@@ -227,6 +227,11 @@ struct Proc {
   KBView kb{};
   std::vector<std::unique_ptr<DevBuf>> kb_bufs;
   std::vector<double> kb_node_h, kb_env_h;
+  // learned proposal (mixture from the MLP + envelope patches)
+  bool use_lp = false;
+  LPView lp{};
+  std::vector<std::unique_ptr<DevBuf>> lp_bufs;
+  std::vector<double> lp_stats_h;  // per interval: M, Z, coarse envelope mass, fine envelope mass
   std::vector<double> sab_egrid_h, sab_xs_h;
   // single-crystal Bragg (host copies of the device tables for read-back)
   SCBView scb{};
@@ -1527,6 +1532,188 @@ Mlp* create_mlp(uint64_t seed, double emin, double emax) {
   return m.release();
 }
 
+
+// ----------------------------------------------- learned proposal sampler ---
+// Per interval: the MLP's mixture at the interval's upper energy, the
+// minimum of the mixture over every fine envelope box (vertex rule: each
+// component is log-concave), and the scale M minimising the expected number
+// of proposals M * Z(M), Z = 1 + sum over coarse boxes of area * patch,
+// patch = max over its fine boxes of max(env/M - qmin, 0).
+void build_lp(Proc* p, Mlp* m, int nn, double emin, double emax, int bw, int bk, int bwc, int bkc, int maxtries) {
+  if (bwc < 1 || bkc < 1 || bw % bwc || bk % bkc) raise("BadInput", "coarse grid must divide the fine grid");
+  build_kb(p, nn, emin, emax, bw, bk, maxtries);
+  p->use_kb = false;
+  const KBView& k = p->kb;
+  // mixture parameters at the interval upper edges (MLP forward on the GPU)
+  std::vector<double> eu((size_t)nn);
+  for (int i = 0; i < nn; ++i) eu[(size_t)i] = p->kb_node_h[(size_t)i + 1];
+  m->ensure(nn);
+  NCAMD_HIP(hipMemcpy(m->E.p, eu.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
+  launch_mlp_features(m->E.as<double>(), nn, m->lelo, m->linv, m->X.as<float>(), 0);
+  mlp_forward(m, nn, 0);
+  NCAMD_HIP(hipGetLastError());
+  std::vector<float> y((size_t)nn * kMlpOut);
+  NCAMD_HIP(hipMemcpy(y.data(), m->Y.p, sizeof(float) * y.size(), hipMemcpyDeviceToHost));
+  const int nbox = bw * bk, ncbox = bwc * bkc;
+  const int fw = bw / bwc, fk = bk / bkc;
+  const double area_c = 1.0 / ncbox, area_f = 1.0 / nbox;
+  std::vector<LPComp> comp((size_t)nn * kLPComp);
+  std::vector<double> mz(2 * (size_t)nn), patch((size_t)nn * ncbox), pprob((size_t)nn * ncbox);
+  std::vector<int> palias((size_t)nn * ncbox);
+  std::vector<double> stats(4 * (size_t)nn);
+  std::vector<double> qmin((size_t)nbox), q((size_t)ncbox);
+  std::vector<int> small, large;
+  for (int n = 0; n < nn; ++n) {
+    const float* yy = y.data() + (size_t)n * kMlpOut;
+    LPComp* c = comp.data() + (size_t)n * kLPComp;
+    double lmax = -1e300;
+    for (int j = 0; j < kLPComp; ++j) lmax = std::max(lmax, (double)yy[6 * j]);
+    double se = 0.0;
+    for (int j = 0; j < kLPComp; ++j) se += std::exp((double)yy[6 * j] - lmax);
+    double cum = 0.0;
+    for (int j = 0; j < kLPComp; ++j) {
+      const double pi = std::exp((double)yy[6 * j] - lmax) / se;
+      cum += pi;
+      const double sw = std::exp(dclamp((double)yy[6 * j + 3], -9.0, 3.0));
+      const double sk = std::exp(dclamp((double)yy[6 * j + 4], -9.0, 3.0));
+      const double r = 0.95 * std::tanh((double)yy[6 * j + 5]);
+      c[j].cpi = j + 1 == kLPComp ? 1.0 : cum;
+      c[j].mw = yy[6 * j + 1];
+      c[j].mk = yy[6 * j + 2];
+      c[j].l11 = sw;
+      c[j].l21 = r * sk;
+      c[j].l22 = sk * std::sqrt(1.0 - r * r);
+      c[j].iw = 1.0 / sw;
+      c[j].ik = 1.0 / sk;
+      c[j].r = r;
+      c[j].om = 1.0 - r * r;
+      c[j].lognorm = std::log(pi) - std::log(2.0 * kPi * sw * sk * std::sqrt(c[j].om));
+      c[j].pad = 0.0;
+    }
+    auto qcomp = [&](int j, double w, double kap) {
+      const double zw = (w - c[j].mw) * c[j].iw, zk = (kap - c[j].mk) * c[j].ik;
+      const double qq = (zw * zw - 2.0 * c[j].r * zw * zk + zk * zk) / c[j].om;
+      return std::exp(c[j].lognorm - 0.5 * qq);
+    };
+    for (int b = 0; b < nbox; ++b) {
+      const int iw = b / bk, ik = b % bk;
+      const double w0 = (double)iw / bw, w1 = (double)(iw + 1) / bw;
+      const double k0 = (double)ik / bk, k1 = (double)(ik + 1) / bk;
+      double qm = 0.0;
+      for (int j = 0; j < kLPComp; ++j)
+        qm += std::min(std::min(qcomp(j, w0, k0), qcomp(j, w0, k1)), std::min(qcomp(j, w1, k0), qcomp(j, w1, k1)));
+      qmin[(size_t)b] = qm * (1.0 - 1e-9);
+    }
+    const double* env = p->kb_env_h.data() + (size_t)n * nbox;
+    auto f = [&](double M) {
+      double tot = M;
+      for (int cb = 0; cb < ncbox; ++cb) {
+        const int cw = cb / bkc, ck = cb % bkc;
+        double mx = 0.0;
+        for (int a = 0; a < fw; ++a)
+          for (int bb = 0; bb < fk; ++bb) {
+            const int j = (cw * fw + a) * bk + (ck * fk + bb);
+            mx = std::max(mx, env[j] - M * qmin[(size_t)j]);
+          }
+        tot += area_c * mx;
+      }
+      return tot;
+    };
+    double fine = 0.0, coarse = f(0.0);
+    double hi = 0.0;
+    for (int j = 0; j < nbox; ++j) {
+      fine += area_f * env[j];
+      if (qmin[(size_t)j] > 0.0) hi = std::max(hi, env[j] / qmin[(size_t)j]);
+    }
+    // convex in M: coarse log scan then golden section around the best point
+    double bestM = 0.0, bestf = coarse;
+    if (hi > 0.0 && coarse > 0.0) {
+      const double lo = std::max(1e-6 * coarse, 1e-300);
+      const int ns = 64;
+      int bi = -1;
+      std::vector<double> grid(ns + 1);
+      for (int i = 0; i <= ns; ++i) {
+        grid[(size_t)i] = lo * std::pow(std::max(hi, 2.0 * lo) / lo, (double)i / ns);
+        const double v = f(grid[(size_t)i]);
+        if (v < bestf) { bestf = v; bestM = grid[(size_t)i]; bi = i; }
+      }
+      if (bi >= 0) {
+        double a = grid[(size_t)std::max(0, bi - 1)], b = grid[(size_t)std::min(ns, bi + 1)];
+        for (int it = 0; it < 60; ++it) {
+          const double m1 = a + 0.381966 * (b - a), m2 = b - 0.381966 * (b - a);
+          if (f(m1) < f(m2)) b = m2; else a = m1;
+        }
+        const double mm = 0.5 * (a + b), fm = f(mm);
+        if (fm < bestf) { bestf = fm; bestM = mm; }
+      }
+    }
+    // an interval with no useful mixture keeps M tiny: the patches then carry
+    // almost all of the proposal mass (the plain coarse envelope)
+    if (!(bestM > 0.0)) bestM = std::max(1e-6 * coarse, 1e-300);
+    double zsum = 0.0;
+    double* pt = patch.data() + (size_t)n * ncbox;
+    for (int cb = 0; cb < ncbox; ++cb) {
+      const int cw = cb / bkc, ck = cb % bkc;
+      double mx = 0.0;
+      for (int a = 0; a < fw; ++a)
+        for (int bb = 0; bb < fk; ++bb) {
+          const int j = (cw * fw + a) * bk + (ck * fk + bb);
+          mx = std::max(mx, env[j] / bestM - qmin[(size_t)j]);
+        }
+      pt[cb] = mx * (1.0 + 1e-12);
+      zsum += area_c * pt[cb];
+    }
+    mz[2 * (size_t)n] = (coarse > 0.0) ? bestM : 0.0;
+    mz[2 * (size_t)n + 1] = 1.0 + zsum;
+    stats[4 * (size_t)n] = bestM;
+    stats[4 * (size_t)n + 1] = 1.0 + zsum;
+    stats[4 * (size_t)n + 2] = coarse;
+    stats[4 * (size_t)n + 3] = fine;
+    // alias table over the patch masses
+    double* pr = pprob.data() + (size_t)n * ncbox;
+    int* al = palias.data() + (size_t)n * ncbox;
+    double tot = 0.0;
+    for (int cb = 0; cb < ncbox; ++cb) tot += pt[cb];
+    if (!(tot > 0.0)) {
+      for (int cb = 0; cb < ncbox; ++cb) { pr[cb] = 1.0; al[cb] = cb; }
+      continue;
+    }
+    small.clear(); large.clear();
+    for (int cb = 0; cb < ncbox; ++cb) {
+      q[(size_t)cb] = pt[cb] * ncbox / tot;
+      (q[(size_t)cb] < 1.0 ? small : large).push_back(cb);
+    }
+    while (!small.empty() && !large.empty()) {
+      const int sj = small.back(); small.pop_back();
+      const int lj = large.back(); large.pop_back();
+      pr[sj] = q[(size_t)sj];
+      al[sj] = lj;
+      q[(size_t)lj] = (q[(size_t)lj] + q[(size_t)sj]) - 1.0;
+      (q[(size_t)lj] < 1.0 ? small : large).push_back(lj);
+    }
+    for (int j : large) { pr[j] = 1.0; al[j] = j; }
+    for (int j : small) { pr[j] = 1.0; al[j] = j; }
+  }
+  p->lp_bufs.clear();
+  auto nb = [p]() { p->lp_bufs.emplace_back(new DevBuf()); return p->lp_bufs.back().get(); };
+  DevBuf* bc = nb(); bc->upload(comp.data(), comp.size());
+  DevBuf* bm = nb(); bm->upload(mz.data(), mz.size());
+  DevBuf* bp = nb(); bp->upload(patch.data(), patch.size());
+  DevBuf* bq = nb(); bq->upload(pprob.data(), pprob.size());
+  DevBuf* ba = nb(); ba->upload(palias.data(), palias.size());
+  LPView v{};
+  v.nn = nn; v.bwc = bwc; v.bkc = bkc; v.ncbox = ncbox;
+  v.comp = bc->as<LPComp>();
+  v.mz = bm->as<double2>();
+  v.patch = bp->as<double>();
+  v.pprob = bq->as<double>();
+  v.palias = ba->as<int>();
+  p->lp = v;
+  p->lp_stats_h = std::move(stats);
+  p->use_lp = true;
+  (void)k;
+}
+
 // ---------------------------------------------------- batched evaluation ---
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
@@ -1556,7 +1743,11 @@ void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat,
       p->q_idx.alloc(need);
     }
     if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
-    if (single->use_kb)
+    if (single->use_lp)
+      launch_lp_sample_iso(single->sab, single->kb, single->lp, ekin, n, repeat, p->seed, stream, oE, oMu,
+                           p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
+                           p->q_count.as<unsigned long long>(), single->kb_maxtries, st);
+    else if (single->use_kb)
       launch_kb_sample_iso(single->sab, single->kb, ekin, n, repeat, p->seed, stream, oE, oMu,
                            p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
                            p->q_count.as<unsigned long long>(), single->kb_maxtries, st);
@@ -2177,7 +2368,44 @@ int ncamd_sab_use_plain_sampler(ncamd_process_t h) {
   Proc* p = extract(h);
   if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
   p->use_kb = false;
+  p->use_lp = false;
   return 0;
+  NCAMD_CATCH
+  return -1;
+}
+int ncamd_sab_use_learned_sampler(ncamd_process_t h, ncamd_mlp_t model, int nintervals, double emin, double emax,
+                                  int nbox_w, int nbox_k, int coarse_w, int coarse_k, int maxtries) {
+  NCAMD_TRY
+  ensure_device();
+  build_lp(extract(h), extract_mlp(model), nintervals, emin, emax, nbox_w, nbox_k, coarse_w, coarse_k, maxtries);
+  return 0;
+  NCAMD_CATCH
+  return -1;
+}
+int ncamd_sab_learned_stats(ncamd_process_t h, double* out) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindSAB || p->lp_stats_h.empty()) raise("BadInput", "no learned sampler tables");
+  std::copy(p->lp_stats_h.begin(), p->lp_stats_h.end(), out);
+  return (int)(p->lp_stats_h.size() / 4);
+  NCAMD_CATCH
+  return -1;
+}
+int64_t ncamd_dev_sab_sample_alphabeta_learned(ncamd_process_t h, const double* d_ekin, unsigned long n,
+                                               double* d_a, double* d_b, void* stream) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindSAB || p->lp_stats_h.empty()) raise("BadInput", "no learned sampler tables");
+  if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
+  hipStream_t st = as_stream(stream);
+  launch_zero_u64(p->q_count.as<unsigned long long>(), st);
+  launch_lp_sample_ab(p->sab, p->kb, p->lp, d_ekin, (int64_t)n, p->seed, next_stream(p), d_a, d_b,
+                      p->q_count.as<unsigned long long>(), p->kb_maxtries, st);
+  NCAMD_HIP(hipGetLastError());
+  unsigned long long nf = 0;
+  NCAMD_HIP(hipMemcpyAsync(&nf, p->q_count.p, sizeof(nf), hipMemcpyDeviceToHost, st));
+  NCAMD_HIP(hipStreamSynchronize(st));
+  return (int64_t)nf;
   NCAMD_CATCH
   return -1;
 }

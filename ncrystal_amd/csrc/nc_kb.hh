@@ -138,4 +138,110 @@ NCD int kb_sample_ab(const SABView& s, const KBView& k, double E, Rng& rng, doub
   return kKBFallback;
 }
 
+// ---------------------------------------------------------------------------
+// Learned proposal: per energy interval a mixture of 8 bivariate Gaussians
+// over (w, kappa) evaluated by the MLP at the interval's upper energy, plus
+// non-negative patches on a coarse box grid that restore the envelope
+// property: with M the interval's scale,
+//   M * (q_mlp(x) + patch(box(x))) >= t(x)  for every x of the unit square,
+// which holds because each patch covers the largest deficit
+// max(env_j / M - min_{box j} q_mlp, 0) over the fine boxes j inside it
+// (env_j bounds t on fine box j; the minimum of a log-concave Gaussian over a
+// box is at a vertex).  Proposals: the mixture with probability 1/Z, else a
+// patch box by its mass (Z = 1 + total patch mass); points outside the unit
+// square are rejected; a point is accepted with probability
+// t / (M (q_mlp + patch)).  Accepted points follow t exactly.
+constexpr int kLPComp = 8;
+struct LPComp {
+  double cpi;              // cumulative weight
+  double mw, mk;           // means
+  double l11, l21, l22;    // Cholesky factor of the covariance
+  double iw, ik, r, lognorm;  // 1/sigma_w, 1/sigma_k, rho, log(pi_k / (2 pi sw sk sqrt(1-rho^2)))
+  double om;               // 1 - rho^2
+  double pad;
+};
+struct LPView {
+  int nn, bwc, bkc, ncbox;
+  const LPComp* comp;      // [nn*kLPComp]
+  const double2* mz;       // [nn]: (M, Z)
+  const double* patch;     // [nn*ncbox] patch density per coarse box
+  const double* pprob;     // [nn*ncbox] alias acceptance over patch masses
+  const int* palias;       // [nn*ncbox]
+};
+
+NCD double lp_qmix(const LPComp* c, double w, double k) {
+  double q = 0.0;
+#pragma unroll
+  for (int j = 0; j < kLPComp; ++j) {
+    const double zw = (w - c[j].mw) * c[j].iw, zk = (k - c[j].mk) * c[j].ik;
+    const double qq = (zw * zw - 2.0 * c[j].r * zw * zk + zk * zk) / c[j].om;
+    q += exp(c[j].lognorm - 0.5 * qq);
+  }
+  return q;
+}
+
+NCD int lp_sample_ab(const SABView& s, const KBView& k, const LPView& lp, double E, Rng& rng, double& a_out,
+                     double& b_out, int maxtries) {
+  if (!(E >= k.elo && E < k.ehi)) return kKBFallback;
+  int n = (int)(log(E / k.elo) * k.lr_inv);
+  n = n < 0 ? 0 : (n > k.nn - 1 ? k.nn - 1 : n);
+  if (E < k.node_e[n]) --n;
+  else if (!(E < k.node_e[n + 1])) ++n;
+  if (n < 0 || n >= k.nn) return kKBFallback;
+  const double2 mz = lp.mz[n];
+  if (!(mz.x > 0.0)) return kKBFallback;
+  const double4 par = k.node_par[n];
+  const double eps = par.x, seps = par.y, wmax = par.z, jfac = par.w;
+  const double ediv = E / s.kT;
+  const double sed = sqrt(ediv);
+  const LPComp* c = lp.comp + (size_t)n * kLPComp;
+  const double* pt = lp.patch + (size_t)n * lp.ncbox;
+  const double amin = s.alpha[0], amax = s.alpha[s.na - 1];
+  const double bmin = s.beta[0], bmax = s.beta[s.nb - 1];
+  for (int t = 0; t < maxtries; ++t) {
+    double w, kap;
+    if (rng.gen() * mz.y <= 1.0) {
+      // mixture component, then a Gaussian point (Box-Muller)
+      const double uc = rng.gen();
+      int j = 0;
+      while (j < kLPComp - 1 && uc > c[j].cpi) ++j;
+      const double u1 = rng.gen(), u2 = rng.gen();
+      const double rad = sqrt(-2.0 * log(u1));
+      double sn, cs;
+      sincos(6.283185307179586 * u2, &sn, &cs);
+      const double z1 = rad * cs, z2 = rad * sn;
+      w = c[j].mw + c[j].l11 * z1;
+      kap = c[j].mk + c[j].l21 * z1 + c[j].l22 * z2;
+      if (!(w > 0.0 && w <= 1.0 && kap >= 0.0 && kap <= 1.0)) continue;
+    } else {
+      const double u0 = rng.gen() * lp.ncbox;
+      int box = (int)u0;
+      box = box >= lp.ncbox ? lp.ncbox - 1 : box;
+      if (lp.pprob[(size_t)n * lp.ncbox + box] < u0 - (double)box) box = lp.palias[(size_t)n * lp.ncbox + box];
+      const int iw = box / lp.bkc, ik = box - iw * lp.bkc;
+      w = (iw + rng.gen()) / lp.bwc;
+      kap = (ik + rng.gen()) / lp.bkc;
+    }
+    const double u = rng.gen();
+    int iw = (int)(w * lp.bwc), ik = (int)(kap * lp.bkc);
+    iw = iw >= lp.bwc ? lp.bwc - 1 : iw;
+    ik = ik >= lp.bkc ? lp.bkc - 1 : ik;
+    const double qv = lp_qmix(c, w, kap) + pt[iw * lp.bkc + ik];
+    const double r = w * wmax;
+    const double beta = r * r - eps;
+    const double alpha = r * r + eps - 2.0 * seps * r * (1.0 - 2.0 * kap);
+    if (!(beta > -ediv) || beta < bmin || beta > bmax || alpha < amin || alpha > amax) continue;
+    const double re = sqrt(ediv + beta);
+    const double alo = (re - sed) * (re - sed), ahi = (re + sed) * (re + sed);
+    if (alpha < alo || alpha > ahi) continue;
+    const double tv = kb_eval_s(s, k, alpha, beta) * (jfac * w * w);
+    if (u * mz.x * qv <= tv) {
+      a_out = alpha;
+      b_out = beta;
+      return kKBAccepted;
+    }
+  }
+  return kKBFallback;
+}
+
 }  // namespace ncamd

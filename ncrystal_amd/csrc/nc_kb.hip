@@ -133,6 +133,52 @@ __global__ void __launch_bounds__(256) k_kb_sample_ab(SABView s, KBView k, const
   }
 }
 
+// the same two kernels for the learned proposal
+__global__ void __launch_bounds__(256) k_lp_sample_iso(SABView s, KBView k, LPView lp, const double* __restrict__ ekin,
+                                                       int64_t n, int64_t total, uint64_t seed, uint32_t stream,
+                                                       double* __restrict__ oE, double* __restrict__ oMu,
+                                                       unsigned long long* __restrict__ nerr,
+                                                       int64_t* __restrict__ queue,
+                                                       unsigned long long* __restrict__ qcount, int maxtries) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; kk < total; kk += gstride) {
+    const int64_t i = (total == n) ? kk : (kk % n);
+    const double E = ekin[i];
+    Rng rng;
+    rng.init(seed, stream, (uint64_t)kk);
+    double a, b;
+    if (lp_sample_ab(s, k, lp, E, rng, a, b, maxtries) != kKBAccepted) {
+      const unsigned long long slot = atomicAdd(qcount, 1ull);
+      queue[slot] = kk;
+      continue;
+    }
+    double Ef, mu;
+    const int rc = sab_ab_to_emu(s, rng, E, a, b, Ef, mu);
+    if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
+    oE[kk] = Ef;
+    oMu[kk] = mu;
+  }
+}
+__global__ void __launch_bounds__(256) k_lp_sample_ab(SABView s, KBView k, LPView lp, const double* __restrict__ ekin,
+                                                      int64_t n, uint64_t seed, uint32_t stream,
+                                                      double* __restrict__ oa, double* __restrict__ ob,
+                                                      unsigned long long* __restrict__ nfallback, int maxtries) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; kk < n; kk += gstride) {
+    Rng rng;
+    rng.init(seed, stream, (uint64_t)kk);
+    double a = -1.0, b = 0.0;
+    if (lp_sample_ab(s, k, lp, ekin[kk], rng, a, b, maxtries) != kKBAccepted) {
+      atomicAdd(nfallback, 1ull);
+      Rng r2;
+      r2.init(seed, stream, (uint64_t)kk);
+      if (sab_sample_ab<true>(s, ekin[kk], r2, a, b)) { a = -1.0; b = 0.0; }
+    }
+    oa[kk] = a;
+    ob[kk] = b;
+  }
+}
+
 static inline unsigned kb_grid(int64_t n) {
   int64_t g = (n + 255) / 256;
   const int64_t gmax = 256 * 64;
@@ -157,6 +203,25 @@ void launch_kb_sample_iso(const SABView& s, const KBView& k, const double* ekin,
   hipLaunchKernelGGL(k_kb_sample_iso, dim3(kb_grid(total)), dim3(256), 0, st, s, k, ekin, n, total, seed, stream, oE,
                      oMu, nerr, queue, qcount, maxtries);
   launch_sab_queue(s, ekin, n, total, seed, stream, oE, oMu, nerr, queue, qcount, st);
+}
+
+void launch_lp_sample_iso(const SABView& s, const KBView& k, const LPView& lp, const double* ekin, int64_t n,
+                          int64_t repeat, uint64_t seed, uint32_t stream, double* oE, double* oMu,
+                          unsigned long long* nerr, int64_t* queue, unsigned long long* qcount, int maxtries,
+                          hipStream_t st) {
+  const int64_t total = n * repeat;
+  if (total <= 0) return;
+  launch_zero_u64(qcount, st);
+  hipLaunchKernelGGL(k_lp_sample_iso, dim3(kb_grid(total)), dim3(256), 0, st, s, k, lp, ekin, n, total, seed, stream,
+                     oE, oMu, nerr, queue, qcount, maxtries);
+  launch_sab_queue(s, ekin, n, total, seed, stream, oE, oMu, nerr, queue, qcount, st);
+}
+void launch_lp_sample_ab(const SABView& s, const KBView& k, const LPView& lp, const double* ekin, int64_t n,
+                         uint64_t seed, uint32_t stream, double* oa, double* ob, unsigned long long* nfallback,
+                         int maxtries, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lp_sample_ab, dim3(kb_grid(n)), dim3(256), 0, st, s, k, lp, ekin, n, seed, stream, oa, ob,
+                     nfallback, maxtries);
 }
 
 void launch_kb_sample_ab(const SABView& s, const KBView& k, const double* ekin, int64_t n, uint64_t seed,

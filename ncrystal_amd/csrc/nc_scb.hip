@@ -129,13 +129,37 @@ __global__ void __launch_bounds__(kScbBuckets) k_scb_offsets(unsigned* __restric
   hist[t] = 0u;
 }
 
+// each block takes kScbScatterItems x 256 consecutive neutrons, ranks them
+// inside the block with LDS atomics and reserves one range per bucket with
+// a single global atomic, so that global atomics do not pile up on the few
+// buckets a batch usually fills
+constexpr int kScbScatterItems = 16;
 __global__ void __launch_bounds__(256) k_scb_scatter(const unsigned* __restrict__ keys, int64_t n,
                                                      unsigned long long* __restrict__ cursor,
                                                      int64_t* __restrict__ perm) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const unsigned long long slot = atomicAdd(&cursor[keys[i]], 1ull);
-    perm[slot] = i;
+  __shared__ unsigned cnt[kScbBuckets];
+  __shared__ unsigned long long base[kScbBuckets];
+  const int64_t chunk = (int64_t)blockDim.x * kScbScatterItems;
+  for (int64_t c0 = (int64_t)blockIdx.x * chunk; c0 < n; c0 += (int64_t)gridDim.x * chunk) {
+    for (int b = threadIdx.x; b < kScbBuckets; b += blockDim.x) cnt[b] = 0u;
+    __syncthreads();
+    unsigned key[kScbScatterItems], rank[kScbScatterItems];
+#pragma unroll
+    for (int j = 0; j < kScbScatterItems; ++j) {
+      const int64_t i = c0 + (int64_t)j * blockDim.x + threadIdx.x;
+      key[j] = i < n ? keys[i] : 0u;
+      rank[j] = i < n ? atomicAdd(&cnt[key[j]], 1u) : 0u;
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kScbBuckets; b += blockDim.x)
+      base[b] = cnt[b] ? atomicAdd(&cursor[b], (unsigned long long)cnt[b]) : 0ull;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kScbScatterItems; ++j) {
+      const int64_t i = c0 + (int64_t)j * blockDim.x + threadIdx.x;
+      if (i < n) perm[base[key[j]] + rank[j]] = i;
+    }
+    __syncthreads();
   }
 }
 
@@ -211,7 +235,8 @@ static void scb_order(const SCBView& v, const double* ekin, const double* dirs, 
   hipLaunchKernelGGL(k_scb_clear, dim3(1), dim3(256), 0, st, w.hist, w.qcount);
   hipLaunchKernelGGL(k_scb_keys, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, w.keys, w.hist);
   hipLaunchKernelGGL(k_scb_offsets, dim3(1), dim3(kScbBuckets), 0, st, w.hist, w.cursor);
-  hipLaunchKernelGGL(k_scb_scatter, dim3(scb_grid(n)), dim3(256), 0, st, w.keys, n, w.cursor, w.perm);
+  hipLaunchKernelGGL(k_scb_scatter, dim3(scb_grid((n + kScbScatterItems - 1) / kScbScatterItems)), dim3(256), 0, st,
+                     w.keys, n, w.cursor, w.perm);
 }
 
 void launch_scb_xs(const SCBView& v, const double* ekin, const double* dirs, int64_t n, double* out,

@@ -163,3 +163,70 @@ def test_envelope_sampler_iso_and_fallback(env):
     kbs.setRandomSeed(21)
     ke2, km2 = kbs.sampleScatterIsotropic(e)
     assert np.array_equal(ke2, pe) and np.array_equal(km2, pm)
+
+
+@pytest.fixture(scope="module")
+def trained(env):
+    C, core = env
+    d, a, b, S = _table()
+    args = (d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"], d["sigma_bound"])
+    trainer = core.create_sabscatter(*args)
+    model = core.MLPProposal(1e-3, 1.0, seed=1)
+    loss = model.train_sab(trainer, steps=1500, batch=8192, lr=3e-3, seed=1)
+    print("trained mean loss", loss)
+    return model
+
+
+def test_learned_sampler_distribution_and_gain(env, oracle, trained):
+    C, core = env
+    d, a, b, S = _table()
+    args = (d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"], d["sigma_bound"])
+    g = core.create_sabscatter(*args)
+    g.use_learned_sampler(trained, nintervals=256, emin=1e-3, emax=1.0)
+    st_ = g.learned_stats()
+    M, Z, coarse, fine = st_.T
+    gain = coarse / (M * Z)
+    print("proposal mass with model / without (coarse boxes): median gain %.3f min %.3f max %.3f; "
+          "fine envelope / learned: median %.3f" % (np.median(gain), gain.min(), gain.max(), np.median(fine / (M * Z))))
+    # the model never makes the proposal worse than the coarse envelope it is combined with
+    assert np.all(gain >= 1.0 - 1e-9)
+    o = oracle.sab(*args)
+    kT = KB * d["temperature"]
+    st = C.Stream()
+    n = 2000000
+    for E in (2e-3, 0.025, 0.2, 0.9):
+        e = np.full(n, E)
+        de = C.DeviceArray.from_numpy(e)
+        da, db = C.DeviceArray(n), C.DeviceArray(n)
+        g.setRandomSeed(4)
+        nfb = g.dev_sample_alphabeta_learned(de.ptr, n, da.ptr, db.ptr, stream=st.ptr)
+        al, be = da.numpy(), db.numpy()
+        assert nfb < 1e-3 * n
+        bq, F = _beta_cdf_exact(a, b, S, E, kT)
+        ks_exact = np.max(np.abs(np.searchsorted(np.sort(be), bq, side="right") / n - F))
+        ra, rb, nerr = o.sample_ab(e, seed=6, nthreads=8)
+        ks_b = stats.ks_2samp(be, rb).statistic
+        ks_a = stats.ks_2samp(al, ra).statistic
+        print(f"learned E={E}: fallback {nfb}, KS(beta vs exact marginal) {ks_exact:.2e}, "
+              f"KS vs plain: beta {ks_b:.2e} alpha {ks_a:.2e}")
+        assert ks_exact < 1.6 / math.sqrt(n) + 2e-4
+        assert ks_b < 2.5 / math.sqrt(n) + 3e-4 and ks_a < 2.5 / math.sqrt(n) + 3e-4
+
+
+def test_learned_sampler_iso_fallback_identity(env, trained):
+    C, core = env
+    d, a, b, S = _table()
+    args = (d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"], d["sigma_bound"])
+    plain = core.create_sabscatter(*args)
+    lps = core.create_sabscatter(*args)
+    lps.use_learned_sampler(trained, nintervals=128, emin=2e-3, emax=0.5)
+    e = synth.energies_logu(400000, 1e-4, 1.0, seed=5)
+    plain.setRandomSeed(22)
+    lps.setRandomSeed(22)
+    pe, pm = plain.sampleScatterIsotropic(e)
+    le, lm = lps.sampleScatterIsotropic(e)
+    out = (e < 2e-3) | (e >= 0.5)
+    assert np.array_equal(pe[out], le[out]) and np.array_equal(pm[out], lm[out])
+    inr = ~out
+    assert stats.ks_2samp(pm[inr], lm[inr]).statistic < 0.01
+    assert stats.ks_2samp(pe[inr], le[inr]).statistic < 0.01
