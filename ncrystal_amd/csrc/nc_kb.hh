@@ -146,8 +146,9 @@ NCD int kb_sample_ab(const SABView& s, const KBView& k, double E, Rng& rng, doub
 //   M * (q_mlp(x) + patch(box(x))) >= t(x)  for every x of the unit square,
 // which holds because each patch covers the largest deficit
 // max(env_j / M - min_{box j} q_mlp, 0) over the fine boxes j inside it
-// (env_j bounds t on fine box j; the minimum of a log-concave Gaussian over a
-// box is at a vertex).  Proposals: the mixture with probability 1/Z, else a
+// (env_j bounds t on fine box j; min_{box j} q_mlp is bounded from below by
+// the sum over components of each component's minimum over the box, which is
+// at a vertex because a Gaussian's superlevel sets are convex).  Proposals: the mixture with probability 1/Z, else a
 // patch box by its mass (Z = 1 + total patch mass); points outside the unit
 // square are rejected; a point is accepted with probability
 // t / (M (q_mlp + patch)).  Accepted points follow t exactly.

@@ -188,8 +188,10 @@ def test_learned_sampler_distribution_and_gain(env, oracle, trained):
     gain = coarse / (M * Z)
     print("proposal mass with model / without (coarse boxes): median gain %.3f min %.3f max %.3f; "
           "fine envelope / learned: median %.3f" % (np.median(gain), gain.min(), gain.max(), np.median(fine / (M * Z))))
-    # the model never makes the proposal worse than the coarse envelope it is combined with
-    assert np.all(gain >= 1.0 - 1e-9)
+    # the model never makes the proposal worse than the coarse envelope it is
+    # combined with: where no scale improves on it, the scale is set to 1e-6
+    # of the coarse mass, which leaves M*Z within 1e-6 of the coarse mass
+    assert np.all(gain >= 1.0 - 2e-6)
     o = oracle.sab(*args)
     kT = KB * d["temperature"]
     st = C.Stream()
