@@ -154,17 +154,28 @@ int ncamd_scbragg_tables(ncamd_process_t, double* inv2d, double* xsfact, int64_t
  * coordinates (w = sqrt(beta+eps)/wmax, kappa = position between the
  * kinematic alpha limits) with a box envelope per energy interval:
  * nintervals log-spaced intervals over [emin, emax) (0 = the process's
- * energy grid range), nbox_w x nbox_k boxes, at most maxtries proposals per
- * neutron.  Neutrons outside [emin, emax) or without an accepted proposal
+ * energy grid range), nbox_w strips in w with nbox_k boxes each, at most
+ * maxtries proposals per neutron.  Every box carries exact upper and lower
+ * bounds of the target; proposals below the lower bound are accepted
+ * without evaluating the table (squeeze), with the same decisions as the
+ * full test.  Neutrons outside [emin, emax) or without an accepted proposal
  * use the plain sampler (from the first draw of their stream).  The
  * accepted (alpha, beta) follow the interpolated table density exactly. */
 int ncamd_sab_use_envelope_sampler(ncamd_process_t, int nintervals, double emin, double emax, int nbox_w,
                                    int nbox_k, int maxtries);
 int ncamd_sab_use_plain_sampler(ncamd_process_t);
-/* interval edges [nintervals+1] and envelope values [nintervals*nbox_w*nbox_k] */
-int ncamd_sab_envelope_tables(ncamd_process_t, double* node_e, double* env);
-/* raw (alpha, beta) of the envelope sampler; returns the number of neutrons
- * that fell back to the plain sampler (synchronises the stream) */
+/* interval edges [nintervals+1] and per box (interval-major, strip-major)
+ * w0, w1, kappa0, kappa1, upper bound, lower bound [6*nintervals*nbox_w*nbox_k] */
+int ncamd_sab_envelope_tables(ncamd_process_t, double* node_e, double* boxes);
+/* per interval: envelope mass of the grid in use, envelope mass of the
+ * uniform grid, blend weight of the uniform grid in the edges of the grid in
+ * use (1: uniform), mass under the lower bounds; the expected acceptance is
+ * (integral of the target) / (envelope mass).  Returns nintervals. */
+int ncamd_sab_envelope_stats(ncamd_process_t, double* out /* 4 per interval */);
+/* test hook: 0 switches the squeeze off (every proposal evaluates the table) */
+int ncamd_sab_envelope_squeeze(ncamd_process_t, int on);
+/* raw (alpha, beta) of the envelope (or learned) sampler; returns the number
+ * of neutrons that fell back to the plain sampler (synchronises the stream) */
 int64_t ncamd_dev_sab_sample_alphabeta_envelope(ncamd_process_t, const double* d_ekin, unsigned long n,
                                                 double* d_alpha, double* d_beta, void* stream);
 
@@ -195,21 +206,16 @@ double ncamd_mlp_train_sab(ncamd_mlp_t, ncamd_process_t sab, int steps, int batc
 
 /* ---------- learned-proposal sampler of S(alpha,beta) (no counterpart in
  * NCrystal) ----------
- * Switches an S(alpha,beta) process to exact accept/reject against the
- * interpolated table with a proposal made of the model's Gaussian mixture
- * (evaluated at each energy interval's upper edge) plus non-negative patches
- * on a coarse box grid (coarse_w x coarse_k, dividing nbox_w x nbox_k) that
- * restore the envelope property; the scale of each interval minimises the
- * expected number of proposals.  Falls back to the plain sampler like the
- * envelope sampler.  Stats per interval: scale M, Z (total proposal mass),
- * envelope mass of the coarse grid without the model, envelope mass of the
- * fine grid; the expected acceptance is I/(M Z) with the model and I/coarse
- * without it (I = integral of the target). */
+ * The envelope sampler above on grids placed by the model: for each energy
+ * interval the model's mixture at the interval's upper energy gives the
+ * w-edges (quantiles of its w-marginal) and the kappa-edges of each strip
+ * (quantiles of its kappa distribution in the strip), blended with the
+ * uniform grid; the blend with the smallest exact envelope mass is kept,
+ * the uniform grid included, so the learned grid is never worse than the
+ * uniform one.  Accept/reject stays exact against the table; statistics
+ * through ncamd_sab_envelope_stats. */
 int ncamd_sab_use_learned_sampler(ncamd_process_t, ncamd_mlp_t, int nintervals, double emin, double emax,
-                                  int nbox_w, int nbox_k, int coarse_w, int coarse_k, int maxtries);
-int ncamd_sab_learned_stats(ncamd_process_t, double* out /* 4 per interval */);
-int64_t ncamd_dev_sab_sample_alphabeta_learned(ncamd_process_t, const double* d_ekin, unsigned long n,
-                                               double* d_alpha, double* d_beta, void* stream);
+                                  int nbox_w, int nbox_k, int maxtries);
 
 /* ---------- RNG (ncrystal.h:1067-1087) ----------
  * Each process handle owns a counter-based stream: (seed, call counter).

@@ -129,6 +129,8 @@ def lib():
                                                           ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "ncamd_sab_use_plain_sampler": (ctypes.c_int, [Handle]),
         "ncamd_sab_envelope_tables": (ctypes.c_int, [Handle, _dp, _dp]),
+        "ncamd_sab_envelope_stats": (ctypes.c_int, [Handle, _dp]),
+        "ncamd_sab_envelope_squeeze": (ctypes.c_int, [Handle, ctypes.c_int]),
         "ncamd_dev_sab_sample_alphabeta_envelope": (ctypes.c_int64, [Handle, _vp, _ul, _vp, _vp, _vp]),
         "ncamd_mlp_create": (MlpHandle, [ctypes.c_uint64, ctypes.c_double, ctypes.c_double]),
         "ncamd_mlp_destroy": (None, [ctypes.POINTER(MlpHandle)]),
@@ -142,10 +144,7 @@ def lib():
         "ncamd_debug_gemm_bf16": (ctypes.c_int, [_fp, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "ncamd_sab_use_learned_sampler": (ctypes.c_int, [Handle, MlpHandle, ctypes.c_int, ctypes.c_double,
-                                                         ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                                         ctypes.c_int, ctypes.c_int]),
-        "ncamd_sab_learned_stats": (ctypes.c_int, [Handle, _dp]),
-        "ncamd_dev_sab_sample_alphabeta_learned": (ctypes.c_int64, [Handle, _vp, _ul, _vp, _vp, _vp]),
+                                                         ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "ncamd_batch_create": (BatchHandle, [ctypes.c_int, ctypes.POINTER(Handle), _i64p]),
         "ncamd_batch_destroy": (None, [ctypes.POINTER(BatchHandle)]),
         "ncamd_batch_total": (ctypes.c_int64, [BatchHandle]),

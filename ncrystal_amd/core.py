@@ -300,36 +300,37 @@ class SABScatter(Scatter):
     def dev_sample_alphabeta(self, d_ekin, n, d_a, d_b, stream=0):
         C.call("ncamd_dev_sab_sample_alphabeta", self._h, d_ekin, n, d_a, d_b, stream)
 
-    # exact envelope sampler (kinematic-box rejection) as the sampling mode
+    # exact envelope sampler (box rejection in kinematic coordinates) as the sampling mode
     def use_envelope_sampler(self, nintervals=512, emin=0.0, emax=0.0, nbox_w=64, nbox_k=32, maxtries=64):
         C.call("ncamd_sab_use_envelope_sampler", self._h, int(nintervals), float(emin), float(emax), int(nbox_w),
                int(nbox_k), int(maxtries))
+        self._kb = (int(nintervals), int(nbox_w) * int(nbox_k))
+
+    def use_learned_sampler(self, model, nintervals=512, emin=0.0, emax=0.0, nbox_w=64, nbox_k=32, maxtries=64):
+        """the envelope sampler on box grids placed by the trained model"""
+        C.call("ncamd_sab_use_learned_sampler", self._h, model.handle, int(nintervals), float(emin), float(emax),
+               int(nbox_w), int(nbox_k), int(maxtries))
         self._kb = (int(nintervals), int(nbox_w) * int(nbox_k))
 
     def use_plain_sampler(self):
         C.call("ncamd_sab_use_plain_sampler", self._h)
 
     def envelope_tables(self):
+        """interval edges [nn+1] and boxes [nn, nbox, 6]: w0, w1, kappa0, kappa1, upper, lower bound"""
         nn, nbox = self._kb
         node = np.empty(nn + 1)
-        env = np.empty(nn * nbox)
-        C.call("ncamd_sab_envelope_tables", self._h, C.dptr(node), C.dptr(env))
-        return node, env.reshape(nn, nbox)
+        boxes = np.empty(nn * nbox * 6)
+        C.call("ncamd_sab_envelope_tables", self._h, C.dptr(node), C.dptr(boxes))
+        return node, boxes.reshape(nn, nbox, 6)
 
-    def use_learned_sampler(self, model, nintervals=512, emin=0.0, emax=0.0, nbox_w=64, nbox_k=32, coarse_w=16,
-                            coarse_k=8, maxtries=64):
-        C.call("ncamd_sab_use_learned_sampler", self._h, model.handle, int(nintervals), float(emin), float(emax),
-               int(nbox_w), int(nbox_k), int(coarse_w), int(coarse_k), int(maxtries))
-        self._kb = (int(nintervals), int(nbox_w) * int(nbox_k))
-        self._lp_nn = int(nintervals)
-
-    def learned_stats(self):
-        out = np.empty(4 * self._lp_nn)
-        C.call("ncamd_sab_learned_stats", self._h, C.dptr(out))
+    def envelope_stats(self):
+        """per interval: envelope mass in use, of the uniform grid, uniform blend weight, squeeze mass"""
+        out = np.empty(4 * self._kb[0])
+        C.call("ncamd_sab_envelope_stats", self._h, C.dptr(out))
         return out.reshape(-1, 4)
 
-    def dev_sample_alphabeta_learned(self, d_ekin, n, d_a, d_b, stream=0):
-        return C.call("ncamd_dev_sab_sample_alphabeta_learned", self._h, d_ekin, n, d_a, d_b, stream)
+    def set_envelope_squeeze(self, on):
+        C.call("ncamd_sab_envelope_squeeze", self._h, int(bool(on)))
 
     def dev_sample_alphabeta_envelope(self, d_ekin, n, d_a, d_b, stream=0):
         return C.call("ncamd_dev_sab_sample_alphabeta_envelope", self._h, d_ekin, n, d_a, d_b, stream)

@@ -22,6 +22,8 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <thread>
+#include <atomic>
 
 #include "../../include/ncrystal_amd.h"
 #include "nc_kernels.hh"
@@ -221,17 +223,17 @@ struct Proc {
   // S(alpha,beta) host copies for read-back
   int sab_na = 0, sab_nb = 0;
   std::vector<double> sab_alpha_h, sab_beta_h;
-  // kinematic-box envelope sampler (optional sampling mode of an S(alpha,beta) process)
+  // box-envelope sampler (optional sampling mode of an S(alpha,beta)
+  // process), on a uniform grid or on a grid placed by the learned model
   bool use_kb = false;
   int kb_maxtries = 64;
   KBView kb{};
   std::vector<std::unique_ptr<DevBuf>> kb_bufs;
-  std::vector<double> kb_node_h, kb_env_h;
-  // learned proposal (mixture from the MLP + envelope patches)
-  bool use_lp = false;
-  LPView lp{};
-  std::vector<std::unique_ptr<DevBuf>> lp_bufs;
-  std::vector<double> lp_stats_h;  // per interval: M, Z, coarse envelope mass, fine envelope mass
+  std::vector<double> kb_node_h;
+  std::vector<KBBox> kb_box_h;
+  // per interval: envelope mass of the grid in use, of the uniform grid,
+  // blend weight of the uniform grid (1: uniform), squeeze mass
+  std::vector<double> kb_stats_h;
   std::vector<double> sab_egrid_h, sab_xs_h;
   // single-crystal Bragg (host copies of the device tables for read-back)
   SCBView scb{};
@@ -1284,16 +1286,113 @@ void batch_run(Batch* b, int mode, const double* in, double* o1, double* o2, hip
 
 
 // ------------------------------------------- S(alpha,beta) envelope sampler ---
-// Tables of the kinematic-box envelope sampler (device side in nc_kb.hh):
-// interval edges log-spaced in [emin, emax), per interval the coordinate
-// parameters, the envelope of every box (GPU) and its alias table (host,
-// Walker/Vose), plus exact guides for the alpha/beta cell searches.
+// Tables of the box-envelope sampler (device side in nc_kb.hh): interval
+// edges log-spaced in [emin, emax), per interval the coordinate parameters,
+// the grid of boxes with the bounds of t over each (GPU) and the alias table
+// over box masses (host, Walker/Vose), plus exact guides for the alpha/beta
+// cell searches.
 DevBuf* kb_newbuf(Proc* p) {
   p->kb_bufs.emplace_back(new DevBuf());
   return p->kb_bufs.back().get();
 }
 
-void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int maxtries) {
+double std_normal_cdf(double z) { return 0.5 * std::erfc(-z * 0.7071067811865476); }
+
+// Edges of a grid of nb cells on [0,1] at the quantiles of
+// H(x) = lam x + (1 - lam) G(x), with G the distribution function `cdf`
+// restricted to [0,1] (G = x where it has no mass there).  Edges are floats
+// shared by neighbouring boxes, strictly increasing, with 0 and 1 at the ends.
+template <class F>
+void kb_edges(int nb, double lam, int ng, F cdf, float* out) {
+  out[0] = 0.0f;
+  out[nb] = 1.0f;
+  const double c0 = lam < 1.0 ? cdf(0.0) : 0.0, c1 = lam < 1.0 ? cdf(1.0) : 1.0;
+  const double mass = c1 - c0;
+  if (lam >= 1.0 || !(mass > 1e-300) || !std::isfinite(mass)) {
+    for (int e = 1; e < nb; ++e) out[e] = (float)((double)e / nb);
+    return;
+  }
+  std::vector<double> H((size_t)ng + 1);
+  for (int i = 0; i <= ng; ++i) {
+    const double x = (double)i / ng;
+    const double g = std::min(1.0, std::max(0.0, (cdf(x) - c0) / mass));
+    H[(size_t)i] = lam * x + (1.0 - lam) * g;
+  }
+  int j = 0;
+  for (int e = 1; e < nb; ++e) {
+    const double target = (double)e / nb;
+    while (j < ng - 1 && H[(size_t)j + 1] < target) ++j;
+    const double dh = H[(size_t)j + 1] - H[(size_t)j];
+    const double f = dh > 0.0 ? std::min(1.0, std::max(0.0, (target - H[(size_t)j]) / dh)) : 0.5;
+    out[e] = (float)((j + f) / ng);
+  }
+  for (int e = 1; e < nb; ++e) out[e] = std::max(out[e], std::nextafter(out[e - 1], 2.0f));
+  for (int e = nb - 1; e >= 1; --e) out[e] = std::min(out[e], std::nextafter(out[e + 1], -1.0f));
+}
+
+// One interval's mixture (from the model's 48 outputs): weights, means,
+// standard deviations and correlations of 8 bivariate Gaussians in (w, kappa).
+struct KBMix {
+  double pi[kMlpComp], mw[kMlpComp], mk[kMlpComp], sw[kMlpComp], sk[kMlpComp], r[kMlpComp];
+  explicit KBMix(const float* y) {
+    double lmax = -1e300, se = 0.0;
+    for (int j = 0; j < kMlpComp; ++j) lmax = std::max(lmax, (double)y[6 * j]);
+    for (int j = 0; j < kMlpComp; ++j) se += std::exp((double)y[6 * j] - lmax);
+    for (int j = 0; j < kMlpComp; ++j) {
+      pi[j] = std::exp((double)y[6 * j] - lmax) / se;
+      mw[j] = y[6 * j + 1];
+      mk[j] = y[6 * j + 2];
+      sw[j] = std::exp(std::min(3.0, std::max(-9.0, (double)y[6 * j + 3])));
+      sk[j] = std::exp(std::min(3.0, std::max(-9.0, (double)y[6 * j + 4])));
+      r[j] = 0.95 * std::tanh((double)y[6 * j + 5]);
+    }
+  }
+};
+
+// grid of one interval: w-edges from the mixture's w-marginal, kappa-edges of
+// each strip from the mixture's kappa distribution given w in the strip
+// (components weighted by their mass in the strip, conditioned at its centre)
+void kb_grid_boxes(const KBMix* mix, double lam, int bw, int bk, KBBox* out) {
+  std::vector<float> we((size_t)bw + 1), ke((size_t)bk + 1);
+  kb_edges(bw, mix ? lam : 1.0, 2048,
+           [mix](double x) {
+             double c = 0.0;
+             for (int j = 0; j < kMlpComp; ++j) c += mix->pi[j] * std_normal_cdf((x - mix->mw[j]) / mix->sw[j]);
+             return c;
+           },
+           we.data());
+  for (int iw = 0; iw < bw; ++iw) {
+    const double wa = we[(size_t)iw], wb = we[(size_t)iw + 1], wc = 0.5 * (wa + wb);
+    double rho[kMlpComp], mu[kMlpComp], sd[kMlpComp];
+    if (mix)
+      for (int j = 0; j < kMlpComp; ++j) {
+        rho[j] = mix->pi[j] * (std_normal_cdf((wb - mix->mw[j]) / mix->sw[j]) -
+                               std_normal_cdf((wa - mix->mw[j]) / mix->sw[j]));
+        mu[j] = mix->mk[j] + mix->r[j] * mix->sk[j] * (wc - mix->mw[j]) / mix->sw[j];
+        sd[j] = mix->sk[j] * std::sqrt(1.0 - mix->r[j] * mix->r[j]);
+      }
+    kb_edges(bk, mix ? lam : 1.0, 256,
+             [&](double x) {
+               double c = 0.0;
+               for (int j = 0; j < kMlpComp; ++j) c += rho[j] * std_normal_cdf((x - mu[j]) / sd[j]);
+               return c;
+             },
+             ke.data());
+    for (int ik = 0; ik < bk; ++ik) {
+      KBBox& b = out[(size_t)iw * bk + ik];
+      b.w0 = we[(size_t)iw]; b.w1 = we[(size_t)iw + 1];
+      b.k0 = ke[(size_t)ik]; b.k1 = ke[(size_t)ik + 1];
+      b.env = 0.0; b.tlow = 0.0;
+    }
+  }
+}
+
+// Builds the sampler; with `mix` (one mixture per interval, at the
+// interval's upper energy) the grid of each interval is the candidate with
+// the smallest envelope mass among the uniform grid and the learned grids
+// blended with it, so it is never worse than the uniform grid.
+void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int maxtries,
+              const std::vector<KBMix>* mix) {
   if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
   const SABView& s = p->sab;
   if (emax <= 0.0) emax = p->sab_egrid_h.back();
@@ -1302,9 +1401,14 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
   if (nn < 1 || nn > 1 << 16 || bw < 1 || bk < 1 || (int64_t)bw * bk > 1 << 16 || maxtries < 1)
     raise("BadInput", "envelope sampler: invalid grid sizes");
   if (!(p->sab_alpha_h.front() > 0.0)) raise("BadInput", "envelope sampler: alpha grid must be positive");
+  const double lams_learned[] = {1.0, 0.6, 0.35, 0.2, 0.1};
+  const int ncand = mix ? 5 : 1;
+  if ((int64_t)ncand * nn * bw * bk > (int64_t)1 << 26) raise("BadInput", "envelope sampler: too many boxes");
+  if (mix && (int)mix->size() != nn) raise("LogicError", "one mixture per interval expected");
+  p->use_kb = false;
   p->kb_bufs.clear();
   KBView k{};
-  k.nn = nn; k.bw = bw; k.bk = bk; k.nbox = bw * bk;
+  k.nn = nn; k.bw = bw; k.bk = bk; k.nbox = bw * bk; k.squeeze = 1;
   const double lr = std::log(emax / emin) / nn;
   k.elo = emin; k.ehi = emax; k.lr_inv = 1.0 / lr;
   std::vector<double> node((size_t)nn + 1);
@@ -1356,55 +1460,91 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
     DevBuf* bg = kb_newbuf(p); bg->upload(g.data(), g.size());
     k.b_guide = bg->as<int>(); k.b_ng = ng; k.b_scale = scale;
   }
-  // envelopes on the GPU
-  const size_t ntot = (size_t)nn * k.nbox;
-  DevBuf cmax, benv_tmp;
-  cmax.alloc(sizeof(double) * (size_t)(s.na - 1) * (s.nb - 1));
-  DevBuf* benv = kb_newbuf(p); benv->alloc(sizeof(double) * ntot);
-  k.env = benv->as<double>();
-  launch_kb_build(s, k, cmax.as<double>(), benv->as<double>(), 0);
-  NCAMD_HIP(hipGetLastError());
-  std::vector<double> env(ntot);
-  NCAMD_HIP(hipMemcpy(env.data(), benv->p, sizeof(double) * ntot, hipMemcpyDeviceToHost));
-  // alias tables (Walker/Vose) per interval; weights = envelope (equal box areas)
-  std::vector<double> prob(ntot);
-  std::vector<int> alias(ntot);
+  // candidate grids (candidate-major, then interval, then box) and their
+  // bounds on the GPU
+  const size_t nper = (size_t)nn * k.nbox;
+  std::vector<KBBox> cand((size_t)ncand * nper);
+  {
+    const int njobs = ncand * nn;
+    const int nth = mix ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1;
+    std::atomic<int> next{0};
+    auto work = [&]() {
+      for (int jb = next++; jb < njobs; jb = next++) {
+        const int c = jb / nn, i = jb % nn;
+        kb_grid_boxes(mix ? &(*mix)[(size_t)i] : nullptr, lams_learned[c], bw, bk,
+                      cand.data() + (size_t)c * nper + (size_t)i * k.nbox);
+      }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+  }
+  {
+    DevBuf dcand;
+    dcand.upload(cand.data(), cand.size());
+    launch_kb_bounds(s, k.node_par, nn, k.nbox, (int64_t)cand.size(), dcand.as<KBBox>(), 0);
+    NCAMD_HIP(hipGetLastError());
+    NCAMD_HIP(hipMemcpy(cand.data(), dcand.p, sizeof(KBBox) * cand.size(), hipMemcpyDeviceToHost));
+  }
+  auto area = [](const KBBox& b) { return ((double)b.w1 - (double)b.w0) * ((double)b.k1 - (double)b.k0); };
+  std::vector<KBBox> boxes(nper);
+  std::vector<double> stats(4 * (size_t)nn);
+  for (int i = 0; i < nn; ++i) {
+    int best = 0;
+    double best_mass = 0.0, uni_mass = 0.0;
+    for (int c = 0; c < ncand; ++c) {
+      const KBBox* bx = cand.data() + (size_t)c * nper + (size_t)i * k.nbox;
+      double m = 0.0;
+      for (int j = 0; j < k.nbox; ++j) m += bx[j].env * area(bx[j]);
+      if (c == 0) uni_mass = m;
+      if (c == 0 || m < best_mass) { best = c; best_mass = m; }
+    }
+    const KBBox* bx = cand.data() + (size_t)best * nper + (size_t)i * k.nbox;
+    double sq = 0.0;
+    for (int j = 0; j < k.nbox; ++j) sq += bx[j].tlow * area(bx[j]);
+    std::copy(bx, bx + k.nbox, boxes.begin() + (ptrdiff_t)((size_t)i * k.nbox));
+    stats[4 * (size_t)i] = best_mass;
+    stats[4 * (size_t)i + 1] = uni_mass;
+    stats[4 * (size_t)i + 2] = lams_learned[best];
+    stats[4 * (size_t)i + 3] = sq;
+  }
+  cand.clear();
+  cand.shrink_to_fit();
+  // alias tables (Walker/Vose) per interval over the box masses env x area
+  std::vector<KBAlias> alias(nper);
   std::vector<double> q((size_t)k.nbox);
   std::vector<int> small, large;
   for (int i = 0; i < nn; ++i) {
-    const double* e = env.data() + (size_t)i * k.nbox;
+    const KBBox* bx = boxes.data() + (size_t)i * k.nbox;
+    KBAlias* al = alias.data() + (size_t)i * k.nbox;
     double tot = 0.0;
-    for (int j = 0; j < k.nbox; ++j) tot += e[j];
-    double* pr = prob.data() + (size_t)i * k.nbox;
-    int* al = alias.data() + (size_t)i * k.nbox;
-    if (!(tot > 0.0) || !std::isfinite(tot)) {  // nothing to sample: every proposal is rejected -> fallback
-      for (int j = 0; j < k.nbox; ++j) { pr[j] = 1.0; al[j] = j; }
-      continue;
-    }
+    for (int j = 0; j < k.nbox; ++j) tot += bx[j].env * area(bx[j]);
+    for (int j = 0; j < k.nbox; ++j) al[j] = KBAlias{1.0, j, 0};
+    if (!(tot > 0.0) || !std::isfinite(tot)) continue;  // nothing to sample: every proposal is rejected -> fallback
     small.clear(); large.clear();
     for (int j = 0; j < k.nbox; ++j) {
-      q[(size_t)j] = e[j] * k.nbox / tot;
+      q[(size_t)j] = bx[j].env * area(bx[j]) * k.nbox / tot;
       (q[(size_t)j] < 1.0 ? small : large).push_back(j);
     }
     while (!small.empty() && !large.empty()) {
       const int sj = small.back(); small.pop_back();
       const int lj = large.back(); large.pop_back();
-      pr[sj] = q[(size_t)sj];
-      al[sj] = lj;
+      al[sj].prob = q[(size_t)sj];
+      al[sj].alias = lj;
       q[(size_t)lj] = (q[(size_t)lj] + q[(size_t)sj]) - 1.0;
       (q[(size_t)lj] < 1.0 ? small : large).push_back(lj);
     }
-    for (int j : large) { pr[j] = 1.0; al[j] = j; }
-    for (int j : small) { pr[j] = 1.0; al[j] = j; }
   }
-  DevBuf* bpr = kb_newbuf(p); bpr->upload(prob.data(), prob.size());
-  DevBuf* bal = kb_newbuf(p); bal->upload(alias.data(), alias.size());
-  k.prob = bpr->as<double>();
-  k.alias = bal->as<int>();
+  DevBuf* bb = kb_newbuf(p); bb->upload(boxes.data(), boxes.size());
+  DevBuf* ba = kb_newbuf(p); ba->upload(alias.data(), alias.size());
+  k.box = bb->as<KBBox>();
+  k.alias = ba->as<KBAlias>();
   p->kb = k;
   p->kb_maxtries = maxtries;
   p->kb_node_h = node;
-  p->kb_env_h = std::move(env);
+  p->kb_box_h = std::move(boxes);
+  p->kb_stats_h = std::move(stats);
   p->use_kb = true;
 }
 
@@ -1534,19 +1674,17 @@ Mlp* create_mlp(uint64_t seed, double emin, double emax) {
 
 
 // ----------------------------------------------- learned proposal sampler ---
-// Per interval: the MLP's mixture at the interval's upper energy, the
-// minimum of the mixture over every fine envelope box (vertex rule: each
-// component is log-concave), and the scale M minimising the expected number
-// of proposals M * Z(M), Z = 1 + sum over coarse boxes of area * patch,
-// patch = max over its fine boxes of max(env/M - qmin, 0).
-void build_lp(Proc* p, Mlp* m, int nn, double emin, double emax, int bw, int bk, int bwc, int bkc, int maxtries) {
-  if (bwc < 1 || bkc < 1 || bw % bwc || bk % bkc) raise("BadInput", "coarse grid must divide the fine grid");
-  build_kb(p, nn, emin, emax, bw, bk, maxtries);
-  p->use_kb = false;
-  const KBView& k = p->kb;
-  // mixture parameters at the interval upper edges (MLP forward on the GPU)
+// The envelope sampler on grids placed by the MLP: its mixture at each
+// interval's upper energy gives the quantiles at which the box edges go.
+void build_learned(Proc* p, Mlp* m, int nn, double emin, double emax, int bw, int bk, int maxtries) {
+  if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
+  if (emax <= 0.0) emax = p->sab_egrid_h.back();
+  if (emin <= 0.0) emin = p->sab_egrid_h.front();
+  if (!(emin > 0.0 && emax > emin)) raise("BadInput", "learned sampler: need 0 < emin < emax");
+  if (nn < 1 || nn > 1 << 16) raise("BadInput", "learned sampler: invalid number of intervals");
+  const double lr = std::log(emax / emin) / nn;
   std::vector<double> eu((size_t)nn);
-  for (int i = 0; i < nn; ++i) eu[(size_t)i] = p->kb_node_h[(size_t)i + 1];
+  for (int i = 0; i < nn; ++i) eu[(size_t)i] = (i + 1 == nn) ? emax : emin * std::exp(lr * (i + 1));
   m->ensure(nn);
   NCAMD_HIP(hipMemcpy(m->E.p, eu.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
   launch_mlp_features(m->E.as<double>(), nn, m->lelo, m->linv, m->X.as<float>(), 0);
@@ -1554,164 +1692,10 @@ void build_lp(Proc* p, Mlp* m, int nn, double emin, double emax, int bw, int bk,
   NCAMD_HIP(hipGetLastError());
   std::vector<float> y((size_t)nn * kMlpOut);
   NCAMD_HIP(hipMemcpy(y.data(), m->Y.p, sizeof(float) * y.size(), hipMemcpyDeviceToHost));
-  const int nbox = bw * bk, ncbox = bwc * bkc;
-  const int fw = bw / bwc, fk = bk / bkc;
-  const double area_c = 1.0 / ncbox, area_f = 1.0 / nbox;
-  std::vector<LPComp> comp((size_t)nn * kLPComp);
-  std::vector<double> mz(2 * (size_t)nn), patch((size_t)nn * ncbox), pprob((size_t)nn * ncbox);
-  std::vector<int> palias((size_t)nn * ncbox);
-  std::vector<double> stats(4 * (size_t)nn);
-  std::vector<double> qmin((size_t)nbox), q((size_t)ncbox);
-  std::vector<int> small, large;
-  for (int n = 0; n < nn; ++n) {
-    const float* yy = y.data() + (size_t)n * kMlpOut;
-    LPComp* c = comp.data() + (size_t)n * kLPComp;
-    double lmax = -1e300;
-    for (int j = 0; j < kLPComp; ++j) lmax = std::max(lmax, (double)yy[6 * j]);
-    double se = 0.0;
-    for (int j = 0; j < kLPComp; ++j) se += std::exp((double)yy[6 * j] - lmax);
-    double cum = 0.0;
-    for (int j = 0; j < kLPComp; ++j) {
-      const double pi = std::exp((double)yy[6 * j] - lmax) / se;
-      cum += pi;
-      const double sw = std::exp(dclamp((double)yy[6 * j + 3], -9.0, 3.0));
-      const double sk = std::exp(dclamp((double)yy[6 * j + 4], -9.0, 3.0));
-      const double r = 0.95 * std::tanh((double)yy[6 * j + 5]);
-      c[j].cpi = j + 1 == kLPComp ? 1.0 : cum;
-      c[j].mw = yy[6 * j + 1];
-      c[j].mk = yy[6 * j + 2];
-      c[j].l11 = sw;
-      c[j].l21 = r * sk;
-      c[j].l22 = sk * std::sqrt(1.0 - r * r);
-      c[j].iw = 1.0 / sw;
-      c[j].ik = 1.0 / sk;
-      c[j].r = r;
-      c[j].om = 1.0 - r * r;
-      c[j].lognorm = std::log(pi) - std::log(2.0 * kPi * sw * sk * std::sqrt(c[j].om));
-      c[j].pad = 0.0;
-    }
-    auto qcomp = [&](int j, double w, double kap) {
-      const double zw = (w - c[j].mw) * c[j].iw, zk = (kap - c[j].mk) * c[j].ik;
-      const double qq = (zw * zw - 2.0 * c[j].r * zw * zk + zk * zk) / c[j].om;
-      return std::exp(c[j].lognorm - 0.5 * qq);
-    };
-    for (int b = 0; b < nbox; ++b) {
-      const int iw = b / bk, ik = b % bk;
-      const double w0 = (double)iw / bw, w1 = (double)(iw + 1) / bw;
-      const double k0 = (double)ik / bk, k1 = (double)(ik + 1) / bk;
-      double qm = 0.0;
-      for (int j = 0; j < kLPComp; ++j)
-        qm += std::min(std::min(qcomp(j, w0, k0), qcomp(j, w0, k1)), std::min(qcomp(j, w1, k0), qcomp(j, w1, k1)));
-      qmin[(size_t)b] = qm * (1.0 - 1e-9);
-    }
-    const double* env = p->kb_env_h.data() + (size_t)n * nbox;
-    auto f = [&](double M) {
-      double tot = M;
-      for (int cb = 0; cb < ncbox; ++cb) {
-        const int cw = cb / bkc, ck = cb % bkc;
-        double mx = 0.0;
-        for (int a = 0; a < fw; ++a)
-          for (int bb = 0; bb < fk; ++bb) {
-            const int j = (cw * fw + a) * bk + (ck * fk + bb);
-            mx = std::max(mx, env[j] - M * qmin[(size_t)j]);
-          }
-        tot += area_c * mx;
-      }
-      return tot;
-    };
-    double fine = 0.0, coarse = f(0.0);
-    double hi = 0.0;
-    for (int j = 0; j < nbox; ++j) {
-      fine += area_f * env[j];
-      if (qmin[(size_t)j] > 0.0) hi = std::max(hi, env[j] / qmin[(size_t)j]);
-    }
-    // convex in M: coarse log scan then golden section around the best point
-    double bestM = 0.0, bestf = coarse;
-    if (hi > 0.0 && coarse > 0.0) {
-      const double lo = std::max(1e-6 * coarse, 1e-300);
-      const int ns = 64;
-      int bi = -1;
-      std::vector<double> grid(ns + 1);
-      for (int i = 0; i <= ns; ++i) {
-        grid[(size_t)i] = lo * std::pow(std::max(hi, 2.0 * lo) / lo, (double)i / ns);
-        const double v = f(grid[(size_t)i]);
-        if (v < bestf) { bestf = v; bestM = grid[(size_t)i]; bi = i; }
-      }
-      if (bi >= 0) {
-        double a = grid[(size_t)std::max(0, bi - 1)], b = grid[(size_t)std::min(ns, bi + 1)];
-        for (int it = 0; it < 60; ++it) {
-          const double m1 = a + 0.381966 * (b - a), m2 = b - 0.381966 * (b - a);
-          if (f(m1) < f(m2)) b = m2; else a = m1;
-        }
-        const double mm = 0.5 * (a + b), fm = f(mm);
-        if (fm < bestf) { bestf = fm; bestM = mm; }
-      }
-    }
-    // an interval with no useful mixture keeps M tiny: the patches then carry
-    // almost all of the proposal mass (the plain coarse envelope)
-    if (!(bestM > 0.0)) bestM = std::max(1e-6 * coarse, 1e-300);
-    double zsum = 0.0;
-    double* pt = patch.data() + (size_t)n * ncbox;
-    for (int cb = 0; cb < ncbox; ++cb) {
-      const int cw = cb / bkc, ck = cb % bkc;
-      double mx = 0.0;
-      for (int a = 0; a < fw; ++a)
-        for (int bb = 0; bb < fk; ++bb) {
-          const int j = (cw * fw + a) * bk + (ck * fk + bb);
-          mx = std::max(mx, env[j] / bestM - qmin[(size_t)j]);
-        }
-      pt[cb] = mx * (1.0 + 1e-12);
-      zsum += area_c * pt[cb];
-    }
-    mz[2 * (size_t)n] = (coarse > 0.0) ? bestM : 0.0;
-    mz[2 * (size_t)n + 1] = 1.0 + zsum;
-    stats[4 * (size_t)n] = bestM;
-    stats[4 * (size_t)n + 1] = 1.0 + zsum;
-    stats[4 * (size_t)n + 2] = coarse;
-    stats[4 * (size_t)n + 3] = fine;
-    // alias table over the patch masses
-    double* pr = pprob.data() + (size_t)n * ncbox;
-    int* al = palias.data() + (size_t)n * ncbox;
-    double tot = 0.0;
-    for (int cb = 0; cb < ncbox; ++cb) tot += pt[cb];
-    if (!(tot > 0.0)) {
-      for (int cb = 0; cb < ncbox; ++cb) { pr[cb] = 1.0; al[cb] = cb; }
-      continue;
-    }
-    small.clear(); large.clear();
-    for (int cb = 0; cb < ncbox; ++cb) {
-      q[(size_t)cb] = pt[cb] * ncbox / tot;
-      (q[(size_t)cb] < 1.0 ? small : large).push_back(cb);
-    }
-    while (!small.empty() && !large.empty()) {
-      const int sj = small.back(); small.pop_back();
-      const int lj = large.back(); large.pop_back();
-      pr[sj] = q[(size_t)sj];
-      al[sj] = lj;
-      q[(size_t)lj] = (q[(size_t)lj] + q[(size_t)sj]) - 1.0;
-      (q[(size_t)lj] < 1.0 ? small : large).push_back(lj);
-    }
-    for (int j : large) { pr[j] = 1.0; al[j] = j; }
-    for (int j : small) { pr[j] = 1.0; al[j] = j; }
-  }
-  p->lp_bufs.clear();
-  auto nb = [p]() { p->lp_bufs.emplace_back(new DevBuf()); return p->lp_bufs.back().get(); };
-  DevBuf* bc = nb(); bc->upload(comp.data(), comp.size());
-  DevBuf* bm = nb(); bm->upload(mz.data(), mz.size());
-  DevBuf* bp = nb(); bp->upload(patch.data(), patch.size());
-  DevBuf* bq = nb(); bq->upload(pprob.data(), pprob.size());
-  DevBuf* ba = nb(); ba->upload(palias.data(), palias.size());
-  LPView v{};
-  v.nn = nn; v.bwc = bwc; v.bkc = bkc; v.ncbox = ncbox;
-  v.comp = bc->as<LPComp>();
-  v.mz = bm->as<double2>();
-  v.patch = bp->as<double>();
-  v.pprob = bq->as<double>();
-  v.palias = ba->as<int>();
-  p->lp = v;
-  p->lp_stats_h = std::move(stats);
-  p->use_lp = true;
-  (void)k;
+  std::vector<KBMix> mix;
+  mix.reserve((size_t)nn);
+  for (int i = 0; i < nn; ++i) mix.emplace_back(y.data() + (size_t)i * kMlpOut);
+  build_kb(p, nn, emin, emax, bw, bk, maxtries, &mix);
 }
 
 // ---------------------------------------------------- batched evaluation ---
@@ -1743,11 +1727,7 @@ void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat,
       p->q_idx.alloc(need);
     }
     if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
-    if (single->use_lp)
-      launch_lp_sample_iso(single->sab, single->kb, single->lp, ekin, n, repeat, p->seed, stream, oE, oMu,
-                           p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
-                           p->q_count.as<unsigned long long>(), single->kb_maxtries, st);
-    else if (single->use_kb)
+    if (single->use_kb)
       launch_kb_sample_iso(single->sab, single->kb, ekin, n, repeat, p->seed, stream, oE, oMu,
                            p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
                            p->q_count.as<unsigned long long>(), single->kb_maxtries, st);
@@ -2358,7 +2338,7 @@ int ncamd_sab_use_envelope_sampler(ncamd_process_t h, int nintervals, double emi
                                    int nbox_k, int maxtries) {
   NCAMD_TRY
   ensure_device();
-  build_kb(extract(h), nintervals, emin, emax, nbox_w, nbox_k, maxtries);
+  build_kb(extract(h), nintervals, emin, emax, nbox_w, nbox_k, maxtries, nullptr);
   return 0;
   NCAMD_CATCH
   return -1;
@@ -2368,53 +2348,47 @@ int ncamd_sab_use_plain_sampler(ncamd_process_t h) {
   Proc* p = extract(h);
   if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
   p->use_kb = false;
-  p->use_lp = false;
   return 0;
   NCAMD_CATCH
   return -1;
 }
 int ncamd_sab_use_learned_sampler(ncamd_process_t h, ncamd_mlp_t model, int nintervals, double emin, double emax,
-                                  int nbox_w, int nbox_k, int coarse_w, int coarse_k, int maxtries) {
+                                  int nbox_w, int nbox_k, int maxtries) {
   NCAMD_TRY
   ensure_device();
-  build_lp(extract(h), extract_mlp(model), nintervals, emin, emax, nbox_w, nbox_k, coarse_w, coarse_k, maxtries);
+  build_learned(extract(h), extract_mlp(model), nintervals, emin, emax, nbox_w, nbox_k, maxtries);
   return 0;
   NCAMD_CATCH
   return -1;
 }
-int ncamd_sab_learned_stats(ncamd_process_t h, double* out) {
+int ncamd_sab_envelope_stats(ncamd_process_t h, double* out) {
   NCAMD_TRY
   Proc* p = extract(h);
-  if (p->kind != kKindSAB || p->lp_stats_h.empty()) raise("BadInput", "no learned sampler tables");
-  std::copy(p->lp_stats_h.begin(), p->lp_stats_h.end(), out);
-  return (int)(p->lp_stats_h.size() / 4);
+  if (p->kind != kKindSAB || p->kb_stats_h.empty()) raise("BadInput", "no envelope sampler tables");
+  std::copy(p->kb_stats_h.begin(), p->kb_stats_h.end(), out);
+  return (int)(p->kb_stats_h.size() / 4);
   NCAMD_CATCH
   return -1;
 }
-int64_t ncamd_dev_sab_sample_alphabeta_learned(ncamd_process_t h, const double* d_ekin, unsigned long n,
-                                               double* d_a, double* d_b, void* stream) {
+int ncamd_sab_envelope_squeeze(ncamd_process_t h, int on) {
   NCAMD_TRY
   Proc* p = extract(h);
-  if (p->kind != kKindSAB || p->lp_stats_h.empty()) raise("BadInput", "no learned sampler tables");
-  if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
-  hipStream_t st = as_stream(stream);
-  launch_zero_u64(p->q_count.as<unsigned long long>(), st);
-  launch_lp_sample_ab(p->sab, p->kb, p->lp, d_ekin, (int64_t)n, p->seed, next_stream(p), d_a, d_b,
-                      p->q_count.as<unsigned long long>(), p->kb_maxtries, st);
-  NCAMD_HIP(hipGetLastError());
-  unsigned long long nf = 0;
-  NCAMD_HIP(hipMemcpyAsync(&nf, p->q_count.p, sizeof(nf), hipMemcpyDeviceToHost, st));
-  NCAMD_HIP(hipStreamSynchronize(st));
-  return (int64_t)nf;
+  if (p->kind != kKindSAB || p->kb_stats_h.empty()) raise("BadInput", "no envelope sampler tables");
+  p->kb.squeeze = on ? 1 : 0;
+  return 0;
   NCAMD_CATCH
   return -1;
 }
-int ncamd_sab_envelope_tables(ncamd_process_t h, double* node_e, double* env) {
+int ncamd_sab_envelope_tables(ncamd_process_t h, double* node_e, double* boxes) {
   NCAMD_TRY
   Proc* p = extract(h);
   if (p->kind != kKindSAB || p->kb_node_h.empty()) raise("BadInput", "no envelope sampler tables");
   std::copy(p->kb_node_h.begin(), p->kb_node_h.end(), node_e);
-  std::copy(p->kb_env_h.begin(), p->kb_env_h.end(), env);
+  for (size_t i = 0; i < p->kb_box_h.size(); ++i) {
+    const KBBox& b = p->kb_box_h[i];
+    double* o = boxes + 6 * i;
+    o[0] = b.w0; o[1] = b.w1; o[2] = b.k0; o[3] = b.k1; o[4] = b.env; o[5] = b.tlow;
+  }
   return 0;
   NCAMD_CATCH
   return -1;

@@ -87,19 +87,13 @@ void launch_sab_sample_iso(const SABView&, const double* ekin, int64_t n, int64_
 void launch_sab_guides(const SABView&, unsigned short* cdf_guide, unsigned short* row_guide, double* row_scale,
                        hipStream_t);
 
-// kinematic-box envelope sampler of S(alpha,beta) (nc_kb.hip)
-void launch_kb_build(const SABView&, const KBView&, double* cmax, double* env, hipStream_t);
+// box-envelope sampler of S(alpha,beta) (nc_kb.hip)
+void launch_kb_bounds(const SABView&, const double4* node_par, int nn, int nbox, int64_t total, KBBox* box,
+                      hipStream_t);
 void launch_kb_sample_iso(const SABView&, const KBView&, const double* ekin, int64_t n, int64_t repeat,
                           uint64_t seed, uint32_t stream, double* oE, double* oMu, unsigned long long* nerr,
                           int64_t* queue, unsigned long long* qcount, int maxtries, hipStream_t);
 void launch_kb_sample_ab(const SABView&, const KBView&, const double* ekin, int64_t n, uint64_t seed,
-                         uint32_t stream, double* oa, double* ob, unsigned long long* nfallback, int maxtries,
-                         hipStream_t);
-void launch_lp_sample_iso(const SABView&, const KBView&, const LPView&, const double* ekin, int64_t n,
-                          int64_t repeat, uint64_t seed, uint32_t stream, double* oE, double* oMu,
-                          unsigned long long* nerr, int64_t* queue, unsigned long long* qcount, int maxtries,
-                          hipStream_t);
-void launch_lp_sample_ab(const SABView&, const KBView&, const LPView&, const double* ekin, int64_t n, uint64_t seed,
                          uint32_t stream, double* oa, double* ob, unsigned long long* nfallback, int maxtries,
                          hipStream_t);
 // helpers shared with the S(alpha,beta) kernels (nc_kernels.hip)

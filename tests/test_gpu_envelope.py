@@ -1,9 +1,12 @@
-"""Exact envelope sampler of S(alpha,beta) (kinematic-box rejection) on the
-GPU.  Its samples follow the interpolated table density exactly, so they are
-checked against (1) the envelope property itself (the stored box bounds
-dominate the target on a fine grid of every box, evaluated here with numpy),
-(2) the exact beta marginal of the interpolated table obtained by numerical
-integration, and (3) the plain sampler of the CPU checker by two-sample KS.
+"""Exact envelope sampler of S(alpha,beta) (box rejection in kinematic
+coordinates) on the GPU, on the uniform grid and on the grid placed by the
+learned model.  Its samples follow the interpolated table density exactly, so
+they are checked against (1) the bounds themselves (every box's upper bound
+dominates the target and its lower bound stays below it on a fine grid of
+the box, evaluated here with numpy), (2) the exact beta marginal of the
+interpolated table obtained by numerical integration, and (3) the plain
+sampler of the CPU checker by two-sample KS.  The squeeze (acceptance below
+the lower bound without evaluating the table) must not change any sample.
 Neutrons the envelope sampler hands back fall back to the plain sampler on
 their own Philox stream, so they must equal the plain GPU sampler exactly."""
 import math
@@ -51,37 +54,67 @@ def _interp_s(a, b, S, al, be):
     return np.where(inside, v, 0.0)
 
 
+def _check_bounds(g, d, a, b, S, intervals, nsample=200, seed=1):
+    """every sampled box: lower bound <= t <= upper bound on a 24 x 24 grid
+    of the box; returns per interval the ratio target mass / envelope mass"""
+    node, boxes = g.envelope_tables()
+    kT = KB * d["temperature"]
+    rng = np.random.default_rng(seed)
+    eff = {}
+    for n in intervals:
+        eps = node[n + 1] / kT
+        wmax = math.sqrt(b[-1] + eps)
+        jf = 8 * math.sqrt(eps) * wmax ** 3
+        bx = boxes[n]
+        for j in rng.choice(len(bx), nsample, replace=False):
+            w0, w1, k0, k1, hi, lo = bx[j]
+            W, K = np.meshgrid(w0 + (w1 - w0) * np.linspace(0, 1, 24), k0 + (k1 - k0) * np.linspace(0, 1, 24),
+                               indexing="ij")
+            r = W * wmax
+            t = _interp_s(a, b, S, r * r + eps - 2 * math.sqrt(eps) * r * (1 - 2 * K), r * r - eps) * jf * W * W
+            assert np.all(t <= hi * (1 + 1e-12)), (n, j)
+            assert np.all(t >= lo * (1 - 1e-12)), (n, j)
+        # target mass on a fine midpoint grid of the unit square vs envelope mass
+        W, K = np.meshgrid((np.arange(512) + 0.5) / 512, (np.arange(256) + 0.5) / 256, indexing="ij")
+        r = W * wmax
+        t = _interp_s(a, b, S, r * r + eps - 2 * math.sqrt(eps) * r * (1 - 2 * K), r * r - eps) * jf * W * W
+        area = (bx[:, 1] - bx[:, 0]) * (bx[:, 3] - bx[:, 2])
+        assert abs(area.sum() - 1.0) < 1e-6
+        eff[n] = (t.mean() / np.sum(bx[:, 4] * area), np.sum(bx[:, 5] * area) / np.sum(bx[:, 4] * area))
+    return eff
+
+
 def test_envelope_bounds_target(env):
     C, core = env
     d, a, b, S = _table()
     g = core.create_sabscatter(d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"], d["sigma_bound"])
-    nn, bw, bk = 64, 32, 16
-    g.use_envelope_sampler(nintervals=nn, emin=1e-3, emax=1.0, nbox_w=bw, nbox_k=bk)
-    node, envv = g.envelope_tables()
-    kT = KB * d["temperature"]
-    rng = np.random.default_rng(1)
-    for n in (0, 13, 40, nn - 1):
-        eps = node[n + 1] / kT
-        wmax = math.sqrt(b[-1] + eps)
-        for box in rng.choice(bw * bk, 200, replace=False):
-            iw, ik = divmod(int(box), bk)
-            w = (iw + np.linspace(0, 1, 24)) / bw
-            k = (ik + np.linspace(0, 1, 24)) / bk
-            W, K = np.meshgrid(w, k, indexing="ij")
-            r = W * wmax
-            be = r * r - eps
-            al = r * r + eps - 2 * math.sqrt(eps) * r * (1 - 2 * K)
-            t = _interp_s(a, b, S, al, be) * 8 * math.sqrt(eps) * wmax ** 3 * W * W
-            assert np.all(t <= envv[n, box] * (1 + 1e-12)), (n, box)
-        # the envelope is not loose on average (coarse 32 x 16 boxes here)
-        w = (np.arange(256) + 0.5) / 256
-        k = (np.arange(128) + 0.5) / 128
-        W, K = np.meshgrid(w, k, indexing="ij")
-        r = W * wmax
-        t = _interp_s(a, b, S, r * r + eps - 2 * math.sqrt(eps) * r * (1 - 2 * K), r * r - eps) * \
-            8 * math.sqrt(eps) * wmax ** 3 * W * W
-        print("interval", n, "envelope efficiency", t.mean() / envv[n].mean())
-        assert t.mean() / envv[n].mean() > 0.3
+    nn = 64
+    g.use_envelope_sampler(nintervals=nn, emin=1e-3, emax=1.0, nbox_w=32, nbox_k=16)
+    eff = _check_bounds(g, d, a, b, S, (0, 13, 40, nn - 1))
+    for n, (acc, sq) in eff.items():
+        # the bounds are not loose on average (coarse 32 x 16 boxes here)
+        print("interval", n, "acceptance", acc, "squeeze fraction", sq)
+        assert acc > 0.45 and sq > 0.2
+    st = g.envelope_stats()
+    assert np.all(st[:, 2] == 1.0) and np.array_equal(st[:, 0], st[:, 1])
+
+
+def test_envelope_squeeze_changes_nothing(env):
+    C, core = env
+    d, a, b, S = _table()
+    g = core.create_sabscatter(d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"], d["sigma_bound"])
+    g.use_envelope_sampler()
+    e = synth.energies_logu(1000000, 1e-3, 1.0, seed=8)
+    de = C.DeviceArray.from_numpy(e)
+    st = C.Stream()
+    out = []
+    for on in (1, 0):
+        g.set_envelope_squeeze(on)
+        g.setRandomSeed(12)
+        da, db = C.DeviceArray(e.size), C.DeviceArray(e.size)
+        g.dev_sample_alphabeta_envelope(de.ptr, e.size, da.ptr, db.ptr, stream=st.ptr)
+        out.append((da.numpy(), db.numpy()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
 def _beta_cdf_exact(a, b, S, E, kT, sub=16):
@@ -183,15 +216,16 @@ def test_learned_sampler_distribution_and_gain(env, oracle, trained):
     args = (d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"], d["sigma_bound"])
     g = core.create_sabscatter(*args)
     g.use_learned_sampler(trained, nintervals=256, emin=1e-3, emax=1.0)
-    st_ = g.learned_stats()
-    M, Z, coarse, fine = st_.T
-    gain = coarse / (M * Z)
-    print("proposal mass with model / without (coarse boxes): median gain %.3f min %.3f max %.3f; "
-          "fine envelope / learned: median %.3f" % (np.median(gain), gain.min(), gain.max(), np.median(fine / (M * Z))))
-    # the model never makes the proposal worse than the coarse envelope it is
-    # combined with: where no scale improves on it, the scale is set to 1e-6
-    # of the coarse mass, which leaves M*Z within 1e-6 of the coarse mass
-    assert np.all(gain >= 1.0 - 2e-6)
+    st_ = g.envelope_stats()
+    mass, uni, lam, sq = st_.T
+    gain = uni / mass
+    print("envelope mass uniform grid / learned grid: median %.3f min %.3f max %.3f; uniform blend weights %s; "
+          "squeeze fraction median %.3f" % (np.median(gain), gain.min(), gain.max(),
+                                           np.unique(lam, return_counts=True), np.median(sq / mass)))
+    # the learned grid is never worse than the uniform grid it is chosen against
+    assert np.all(mass <= uni)
+    eff = _check_bounds(g, d, a, b, S, (0, 100, 255), nsample=100)
+    print("learned grid acceptance / squeeze fraction", eff)
     o = oracle.sab(*args)
     kT = KB * d["temperature"]
     st = C.Stream()
@@ -201,7 +235,7 @@ def test_learned_sampler_distribution_and_gain(env, oracle, trained):
         de = C.DeviceArray.from_numpy(e)
         da, db = C.DeviceArray(n), C.DeviceArray(n)
         g.setRandomSeed(4)
-        nfb = g.dev_sample_alphabeta_learned(de.ptr, n, da.ptr, db.ptr, stream=st.ptr)
+        nfb = g.dev_sample_alphabeta_envelope(de.ptr, n, da.ptr, db.ptr, stream=st.ptr)
         al, be = da.numpy(), db.numpy()
         assert nfb < 1e-3 * n
         bq, F = _beta_cdf_exact(a, b, S, E, kT)
@@ -213,6 +247,17 @@ def test_learned_sampler_distribution_and_gain(env, oracle, trained):
               f"KS vs plain: beta {ks_b:.2e} alpha {ks_a:.2e}")
         assert ks_exact < 1.6 / math.sqrt(n) + 2e-4
         assert ks_b < 2.5 / math.sqrt(n) + 3e-4 and ks_a < 2.5 / math.sqrt(n) + 3e-4
+    # squeeze on and off give the same samples on the learned grid too
+    e = synth.energies_logu(500000, 1e-3, 1.0, seed=9)
+    de = C.DeviceArray.from_numpy(e)
+    res = []
+    for on in (1, 0):
+        g.set_envelope_squeeze(on)
+        g.setRandomSeed(13)
+        da, db = C.DeviceArray(e.size), C.DeviceArray(e.size)
+        g.dev_sample_alphabeta_envelope(de.ptr, e.size, da.ptr, db.ptr, stream=st.ptr)
+        res.append((da.numpy(), db.numpy()))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
 
 
 def test_learned_sampler_iso_fallback_identity(env, trained):
