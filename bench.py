@@ -511,13 +511,13 @@ def run_sab(args):
     model = core.MLPProposal(1e-3, 1.0, seed=1)
     loss = model.train_sab(plain, steps=args.train_steps, batch=8192, lr=3e-3, seed=1)
     learned = core.create_sabscatter(*targs)
-    learned.use_learned_sampler(model, nintervals=512, emin=0.999e-3, emax=1.001)
+    learned.use_learned_sampler(model, nintervals=256, emin=0.999e-3, emax=1.001, nbox_w=128, nbox_k=32)
     t_setup = time.perf_counter() - t0
     mass, uni, lam, sq = learned.envelope_stats().T
     r_learned = measure(learned)
     # the same sampler on the uniform grid: what the network gains
     envelope = core.create_sabscatter(*targs)
-    envelope.use_envelope_sampler(nintervals=512, emin=0.999e-3, emax=1.001)
+    envelope.use_envelope_sampler(nintervals=256, emin=0.999e-3, emax=1.001, nbox_w=128, nbox_k=32)
     emass, _, _, esq = envelope.envelope_stats().T
     r_env = measure(envelope)
 
@@ -563,7 +563,7 @@ def run_sab(args):
                    "ks_vs_reference": ks, "ks_samples_per_energy": args.ks_samples},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_kb_sample_iso", "kernel_ms": r_learned["ms"],
+                     "kernel": "k_kb_sample<0>", "kernel_ms": r_learned["ms"],
                      "plain_kernel": "k_sab_sample_iso_main", "plain_kernel_ms": r_plain["ms"],
                      "bytes_per_sample": BYTES_PER_SAMPLE},
         "cpu_baseline": cpu,

@@ -301,12 +301,12 @@ class SABScatter(Scatter):
         C.call("ncamd_dev_sab_sample_alphabeta", self._h, d_ekin, n, d_a, d_b, stream)
 
     # exact envelope sampler (box rejection in kinematic coordinates) as the sampling mode
-    def use_envelope_sampler(self, nintervals=512, emin=0.0, emax=0.0, nbox_w=64, nbox_k=32, maxtries=64):
+    def use_envelope_sampler(self, nintervals=256, emin=0.0, emax=0.0, nbox_w=128, nbox_k=32, maxtries=64):
         C.call("ncamd_sab_use_envelope_sampler", self._h, int(nintervals), float(emin), float(emax), int(nbox_w),
                int(nbox_k), int(maxtries))
         self._kb = (int(nintervals), int(nbox_w) * int(nbox_k))
 
-    def use_learned_sampler(self, model, nintervals=512, emin=0.0, emax=0.0, nbox_w=64, nbox_k=32, maxtries=64):
+    def use_learned_sampler(self, model, nintervals=256, emin=0.0, emax=0.0, nbox_w=128, nbox_k=32, maxtries=64):
         """the envelope sampler on box grids placed by the trained model"""
         C.call("ncamd_sab_use_learned_sampler", self._h, model.handle, int(nintervals), float(emin), float(emax),
                int(nbox_w), int(nbox_k), int(maxtries))

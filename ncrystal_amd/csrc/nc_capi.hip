@@ -1382,7 +1382,7 @@ void kb_grid_boxes(const KBMix* mix, double lam, int bw, int bk, KBBox* out) {
       KBBox& b = out[(size_t)iw * bk + ik];
       b.w0 = we[(size_t)iw]; b.w1 = we[(size_t)iw + 1];
       b.k0 = ke[(size_t)ik]; b.k1 = ke[(size_t)ik + 1];
-      b.env = 0.0; b.tlow = 0.0;
+      b.env = 0.0f; b.tlow = 0.0f; b.arange = 0; b.brange = 0;
     }
   }
 }
@@ -1401,8 +1401,10 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
   if (nn < 1 || nn > 1 << 16 || bw < 1 || bk < 1 || (int64_t)bw * bk > 1 << 16 || maxtries < 1)
     raise("BadInput", "envelope sampler: invalid grid sizes");
   if (!(p->sab_alpha_h.front() > 0.0)) raise("BadInput", "envelope sampler: alpha grid must be positive");
-  const double lams_learned[] = {1.0, 0.6, 0.35, 0.2, 0.1};
-  const int ncand = mix ? 5 : 1;
+  if (p->sab_alpha_h.size() > 65536 || p->sab_beta_h.size() > 65536)
+    raise("BadInput", "envelope sampler: alpha and beta grids of at most 65536 points");
+  const double lams_learned[] = {1.0, 0.6, 0.45, 0.35, 0.28, 0.2, 0.1};
+  const int ncand = mix ? 7 : 1;
   if ((int64_t)ncand * nn * bw * bk > (int64_t)1 << 26) raise("BadInput", "envelope sampler: too many boxes");
   if (mix && (int)mix->size() != nn) raise("LogicError", "one mixture per interval expected");
   p->use_kb = false;
@@ -1425,41 +1427,6 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
   DevBuf* bp = kb_newbuf(p); bp->upload(par.data(), par.size());
   k.node_e = bn->as<double>();
   k.node_par = bp->as<double4>();
-  // alpha cell search: IEEE-bit buckets (a few per grid point, at most 2^20)
-  {
-    const auto& a = p->sab_alpha_h;
-    auto bits = [](double x) { unsigned long long u; std::memcpy(&u, &x, 8); return u; };
-    auto fromb = [](unsigned long long u) { double x; std::memcpy(&x, &u, 8); return x; };
-    int shift = 52, m;
-    long long base = 0, nbkt = 1;
-    for (m = 20; m >= 0; --m) {
-      shift = 52 - m;
-      base = (long long)(bits(a.front()) >> shift);
-      nbkt = (long long)(bits(a.back()) >> shift) - base + 1;
-      if (nbkt <= std::max<long long>(64, 8LL * (long long)a.size()) && nbkt <= (1 << 20)) break;
-    }
-    if (m < 0) { shift = 52; base = (long long)(bits(a.front()) >> shift);
-                 nbkt = (long long)(bits(a.back()) >> shift) - base + 1; }
-    std::vector<int> tab((size_t)nbkt + 1);
-    for (long long b = 0; b <= nbkt; ++b)
-      tab[(size_t)b] = (int)(std::upper_bound(a.begin(), a.end(), fromb((unsigned long long)(base + b) << shift)) -
-                             a.begin());
-    tab[(size_t)nbkt] = (int)a.size();
-    DevBuf* bt = kb_newbuf(p); bt->upload(tab.data(), tab.size());
-    k.a_bucket = bt->as<int>(); k.a_base = base; k.a_shift = shift; k.a_nbkt = (int)nbkt;
-  }
-  // beta cell search: equal-width buckets, one bucket of margin on each side
-  {
-    const auto& b = p->sab_beta_h;
-    const int ng = (int)std::min<long long>(1 << 20, std::max<long long>(64, 4LL * (long long)b.size()));
-    const double scale = ng / (b.back() - b.front());
-    std::vector<int> g((size_t)ng + 1);
-    for (int i = 0; i < ng; ++i)
-      g[(size_t)i] = (int)(std::upper_bound(b.begin(), b.end(), b.front() + (double)i / scale) - b.begin());
-    g[(size_t)ng] = (int)b.size();
-    DevBuf* bg = kb_newbuf(p); bg->upload(g.data(), g.size());
-    k.b_guide = bg->as<int>(); k.b_ng = ng; k.b_scale = scale;
-  }
   // candidate grids (candidate-major, then interval, then box) and their
   // bounds on the GPU
   const size_t nper = (size_t)nn * k.nbox;
@@ -1520,7 +1487,7 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
     KBAlias* al = alias.data() + (size_t)i * k.nbox;
     double tot = 0.0;
     for (int j = 0; j < k.nbox; ++j) tot += bx[j].env * area(bx[j]);
-    for (int j = 0; j < k.nbox; ++j) al[j] = KBAlias{1.0, j, 0};
+    for (int j = 0; j < k.nbox; ++j) al[j] = KBAlias{1.0, 1.0, 0.0, j, 0};
     if (!(tot > 0.0) || !std::isfinite(tot)) continue;  // nothing to sample: every proposal is rejected -> fallback
     small.clear(); large.clear();
     for (int j = 0; j < k.nbox; ++j) {
@@ -1534,6 +1501,10 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
       al[sj].alias = lj;
       q[(size_t)lj] = (q[(size_t)lj] + q[(size_t)sj]) - 1.0;
       (q[(size_t)lj] < 1.0 ? small : large).push_back(lj);
+    }
+    for (int j = 0; j < k.nbox; ++j) {
+      al[j].inv_p = al[j].prob > 0.0 ? 1.0 / al[j].prob : 0.0;
+      al[j].inv_q = al[j].prob < 1.0 ? 1.0 / (1.0 - al[j].prob) : 0.0;
     }
   }
   DevBuf* bb = kb_newbuf(p); bb->upload(boxes.data(), boxes.size());
@@ -2400,9 +2371,13 @@ int64_t ncamd_dev_sab_sample_alphabeta_envelope(ncamd_process_t h, const double*
   if (p->kind != kKindSAB || p->kb_node_h.empty()) raise("BadInput", "no envelope sampler tables");
   if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
   hipStream_t st = as_stream(stream);
-  launch_zero_u64(p->q_count.as<unsigned long long>(), st);
+  const size_t need = sizeof(int64_t) * (size_t)std::max<unsigned long>(n, 1);
+  if (p->q_idx.bytes < need) {
+    NCAMD_HIP(hipStreamSynchronize(st));  // the old queue may still be in use on this stream
+    p->q_idx.alloc(need);
+  }
   launch_kb_sample_ab(p->sab, p->kb, d_ekin, (int64_t)n, p->seed, next_stream(p), d_a, d_b,
-                      p->q_count.as<unsigned long long>(), p->kb_maxtries, st);
+                      p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), p->kb_maxtries, st);
   NCAMD_HIP(hipGetLastError());
   unsigned long long nf = 0;
   NCAMD_HIP(hipMemcpyAsync(&nf, p->q_count.p, sizeof(nf), hipMemcpyDeviceToHost, st));

@@ -69,6 +69,21 @@ struct Rng {
     const double r2 = (double)(x & 0x7FF) * 0x1.0p-64;
     return (1.0 - r1) - r2;
   }
+  // one Philox block as four 32-bit words; drops a cached half block, so
+  // that every caller consumes whole blocks in step
+  NCD void gen4(uint32_t& w0, uint32_t& w1, uint32_t& w2, uint32_t& w3) {
+    uint32_t x0 = c0, x1 = c1, x2 = c2, x3 = c3, q0 = k0, q1 = k1;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+      if (r) { q0 += 0x9E3779B9u; q1 += 0xBB67AE85u; }
+      const uint32_t lo0 = 0xD2511F53u * x0, hi0 = __umulhi(0xD2511F53u, x0);
+      const uint32_t lo1 = 0xCD9E8D57u * x2, hi1 = __umulhi(0xCD9E8D57u, x2);
+      x0 = hi1 ^ x1 ^ q0; x1 = lo1; x2 = hi0 ^ x3 ^ q1; x3 = lo0;
+    }
+    ++c0;
+    nleft = 0;
+    w0 = x0; w1 = x1; w2 = x2; w3 = x3;
+  }
   NCD double gen() {
     if (nleft) { nleft = 0; return to01(b1); }
     uint32_t x0 = c0, x1 = c1, x2 = c2, x3 = c3, q0 = k0, q1 = k1;

@@ -25,6 +25,13 @@
 // tlow <= t on the box, u * env <= tlow already decides acceptance without
 // evaluating the table (squeeze); the decision is the same either way.  The
 // accepted points follow t exactly.
+//
+// Random numbers per proposal: one Philox block of the neutron's stream.
+// Words 0-1 form a 53-bit uniform that picks the alias slot (u * nbox) and,
+// through the residual of that pick (uniform and independent of the box
+// chosen), the position in w; word 2 gives the position in kappa and word 3
+// the acceptance test, as 32-bit uniforms (j + 1/2) / 2^32.  Every lane of a
+// wave thus computes exactly one block per proposal.
 
 #pragma once
 #include "nc_dev.hh"
@@ -32,15 +39,19 @@
 namespace ncamd {
 
 // one box: its geometry (float edges shared with the neighbouring boxes, so
-// the boxes tile the square exactly) and the bounds of t over it
+// the boxes tile the square exactly), the bounds of t over it (rounded
+// outwards to float) and the ranges of table cells its (alpha, beta) range
+// covers (first | last << 16), inside which the cell searches run
 struct KBBox {
   float w0, w1, k0, k1;
-  double env, tlow;
+  float env, tlow;
+  uint32_t arange, brange;
 };
+// alias slot: acceptance of the slot's own box with the reciprocals that turn
+// the pick's residual into a uniform number, and the box taken otherwise
 struct KBAlias {
-  double prob;  // acceptance of the slot's own box
-  int alias;    // box taken otherwise
-  int pad;
+  double prob, inv_p, inv_q;
+  int alias, pad;
 };
 
 struct KBView {
@@ -52,31 +63,7 @@ struct KBView {
   const double4* node_par;         // [nn]: (eps, sqrt(eps), wmax, 8 sqrt(eps) wmax^3)
   const KBAlias* alias;            // [nn*nbox] alias slots
   const KBBox* box;                // [nn*nbox] boxes, strip-major
-  // cell searches
-  const int* a_bucket;             // IEEE-bit buckets of alpha: upper_bound(alpha, lower edge)
-  long long a_base;
-  int a_shift, a_nbkt;
-  const int* b_guide;              // uniform buckets of beta: upper_bound(beta, beta0 + g/b_scale)
-  int b_ng;
-  double b_scale;
 };
-
-// cell of the table holding (alpha, beta); both inside the grid
-NCD int kb_alpha_cell(const SABView& s, const KBView& k, double a) {
-  if (!(a < s.alpha[s.na - 1])) return s.na - 2;
-  const long long b = (long long)((unsigned long long)__double_as_longlong(a) >> k.a_shift) - k.a_base;
-  const int lo = k.a_bucket[b < 0 ? 0 : b], hi = k.a_bucket[b + 1];
-  const int up = upper_bound_d(s.alpha, lo, hi, a);
-  return up - 1 < 0 ? 0 : (up - 1 > s.na - 2 ? s.na - 2 : up - 1);
-}
-NCD int kb_beta_cell(const SABView& s, const KBView& k, double b) {
-  long long g = (long long)((b - s.beta[0]) * k.b_scale);
-  g = g < 0 ? 0 : (g >= k.b_ng ? k.b_ng - 1 : g);
-  const int lo = k.b_guide[g > 0 ? g - 1 : 0];
-  const int hi = k.b_guide[g + 2 <= k.b_ng ? g + 2 : k.b_ng];
-  const int up = upper_bound_d(s.beta, lo, hi, b);
-  return up - 1 < 0 ? 0 : (up - 1 > s.nb - 2 ? s.nb - 2 : up - 1);
-}
 
 // interpolate_loglin_fallbacklinlin_fast (NCSABUtils.hh:176-195)
 NCD double kb_loglin(double a, double fa, double b, double fb, double x, double lfa, double lfb) {
@@ -91,10 +78,18 @@ NCD double kb_loglin(double a, double fa, double b, double fb, double x, double 
   return lin ? (fb + (fa - fb) * sx) : exp(lfb + (lfa - lfb) * sx);
 }
 
-// S(alpha,beta) inside the table grid (SABCellEval, alpha first)
-NCD double kb_eval_s(const SABView& s, const KBView& k, double a, double b) {
-  const int ia = kb_alpha_cell(s, k, a);
-  const int ib = kb_beta_cell(s, k, b);
+// cell holding x among the cells [lo, hi] of a grid: largest i in [lo, hi]
+// with g[i] <= x (lo if there is none), by binary search
+NCD int kb_cell_in(const double* __restrict__ g, uint32_t range, double x) {
+  const int lo = (int)(range & 0xFFFFu), hi = (int)(range >> 16);
+  return upper_bound_d(g, lo + 1, hi + 1, x) - 1;
+}
+
+// S(alpha,beta) inside the table grid (SABCellEval, alpha first), with the
+// cell searches restricted to the box's cell ranges
+NCD double kb_eval_s(const SABView& s, double a, double b, uint32_t arange, uint32_t brange) {
+  const int ia = kb_cell_in(s.alpha, arange, a);
+  const int ib = kb_cell_in(s.beta, brange, b);
   const double a0 = s.alpha[ia], a1 = s.alpha[ia + 1];
   const double b0 = s.beta[ib], b1 = s.beta[ib + 1];
   const double2* r0 = s.sl + (size_t)ib * s.na + ia;
@@ -105,54 +100,68 @@ NCD double kb_eval_s(const SABView& s, const KBView& k, double a, double b) {
   return s0 + (s1 - s0) * ((b - b0) / (b1 - b0));
 }
 
-enum : int { kKBAccepted = 0, kKBFallback = 1 };
-
-// One neutron: energy interval lookup, then up to maxtries proposals.
-// Returns kKBFallback when E is outside the covered range or no proposal was
-// accepted (the caller then uses the plain sampler from the first draw).
-NCD int kb_sample_ab(const SABView& s, const KBView& k, double E, Rng& rng, double& a_out, double& b_out,
-                     int maxtries) {
-  if (!(E >= k.elo && E < k.ehi)) return kKBFallback;
-  int n = (int)(log(E / k.elo) * k.lr_inv);
+// Energy interval of E, -1 outside [elo, ehi).  The FP32 logarithm gives the
+// index to within one; the edge comparisons make it exact.
+NCD int kb_interval(const KBView& k, double E) {
+  if (!(E >= k.elo && E < k.ehi)) return -1;
+  int n = (int)(__logf((float)(E / k.elo)) * (float)k.lr_inv);
   n = n < 0 ? 0 : (n > k.nn - 1 ? k.nn - 1 : n);
-  if (E < k.node_e[n]) --n;
-  else if (!(E < k.node_e[n + 1])) ++n;
-  if (n < 0 || n >= k.nn) return kKBFallback;
+  while (n > 0 && E < k.node_e[n]) --n;
+  while (n < k.nn - 1 && !(E < k.node_e[n + 1])) ++n;
+  return n;
+}
+
+// per-neutron constants of the proposal loop
+struct KBNeutron {
+  const KBAlias* al;
+  const KBBox* bx;
+  double eps, seps, wmax, jfac, ediv;
+};
+
+NCD void kb_neutron(const SABView& s, const KBView& k, int n, double E, KBNeutron& q) {
   const double4 par = k.node_par[n];
-  const double eps = par.x, seps = par.y, wmax = par.z, jfac = par.w;
-  const double ediv = E / s.kT;
-  const double sed = sqrt(ediv);
-  const KBAlias* al = k.alias + (size_t)n * k.nbox;
-  const KBBox* bx = k.box + (size_t)n * k.nbox;
-  const double amin = s.alpha[0], amax = s.alpha[s.na - 1];
-  const double bmin = s.beta[0], bmax = s.beta[s.nb - 1];
-  for (int t = 0; t < maxtries; ++t) {
-    // box from the alias table, then a uniform point inside it
-    const double u0 = rng.gen() * k.nbox;
-    int slot = (int)u0;
-    slot = slot >= k.nbox ? k.nbox - 1 : slot;
-    const KBAlias as = al[slot];
-    const int box = (as.prob < u0 - (double)slot) ? as.alias : slot;
-    const KBBox b = bx[box];
-    const double w0 = b.w0, w1 = b.w1, k0 = b.k0, k1 = b.k1;
-    const double w = w0 + (w1 - w0) * rng.gen();
-    const double kap = k0 + (k1 - k0) * rng.gen();
-    const double ue = rng.gen() * b.env;
-    const double r = w * wmax;
-    const double beta = r * r - eps;
-    const double alpha = r * r + eps - 2.0 * seps * r * (1.0 - 2.0 * kap);
-    // kinematics at the neutron's own energy and the table range
-    if (!(beta > -ediv) || beta < bmin || beta > bmax || alpha < amin || alpha > amax) continue;
-    const double re = sqrt(ediv + beta);
-    const double alo = (re - sed) * (re - sed), ahi = (re + sed) * (re + sed);
-    if (alpha < alo || alpha > ahi) continue;
-    if ((k.squeeze && ue <= b.tlow) || ue <= kb_eval_s(s, k, alpha, beta) * (jfac * w * w)) {
-      a_out = alpha;
-      b_out = beta;
-      return kKBAccepted;
-    }
+  q.eps = par.x; q.seps = par.y; q.wmax = par.z; q.jfac = par.w;
+  q.ediv = E / s.kT;
+  q.al = k.alias + (size_t)n * k.nbox;
+  q.bx = k.box + (size_t)n * k.nbox;
+}
+
+// One proposal; true if accepted, with (alpha, beta).
+NCD bool kb_propose(const SABView& s, const KBView& k, const KBNeutron& q, Rng& rng, double& a_out,
+                    double& b_out) {
+  uint32_t r0, r1, r2, r3;
+  rng.gen4(r0, r1, r2, r3);
+  // alias slot; the residual of the pick, rescaled, is the position in w
+  const double u = Rng::to01(((uint64_t)r1 << 32) | r0) * k.nbox;
+  int slot = (int)u;
+  slot = slot >= k.nbox ? k.nbox - 1 : slot;
+  const double f = u - (double)slot;
+  const KBAlias as = q.al[slot];
+  int box;
+  double uw;
+  if (f < as.prob) { box = slot; uw = f * as.inv_p; }
+  else { box = as.alias; uw = (f - as.prob) * as.inv_q; }
+  const KBBox b = q.bx[box];
+  const double w0 = b.w0, w1 = b.w1, k0 = b.k0;
+  const double w = w0 + (w1 - w0) * uw;
+  const double kap = k0 + ((double)b.k1 - k0) * (((double)r2 + 0.5) * 0x1.0p-32);
+  const double ue = (((double)r3 + 0.5) * 0x1.0p-32) * (double)b.env;
+  const double r = w * q.wmax;
+  const double beta = r * r - q.eps;
+  const double alpha = r * r + q.eps - 2.0 * q.seps * r * (1.0 - 2.0 * kap);
+  // kinematics at the neutron's own energy: beta > -E/kT and
+  // (alpha - 2E/kT - beta)^2 <= 4 (E/kT) (E/kT + beta), i.e. alpha between
+  // the kinematic limits (sqrt(E/kT + beta) -+ sqrt(E/kT))^2; table range
+  const double ef = q.ediv + beta;
+  const double d = alpha - q.ediv - ef;
+  if (!(ef > 0.0) || d * d > 4.0 * q.ediv * ef) return false;
+  if (beta < s.beta[0] || beta > s.beta[s.nb - 1] || alpha < s.alpha[0] || alpha > s.alpha[s.na - 1]) return false;
+  if ((k.squeeze && ue <= (double)b.tlow) || ue <= kb_eval_s(s, alpha, beta, b.arange, b.brange) * (q.jfac * w * w)) {
+    a_out = alpha;
+    b_out = beta;
+    return true;
   }
-  return kKBFallback;
+  return false;
 }
 
 }  // namespace ncamd

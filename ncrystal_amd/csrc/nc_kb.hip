@@ -81,7 +81,8 @@ NCD void kb_subbox_bounds(const SABView& s, double eps, double seps, double wmax
 // Bounds of t = S * J over each box (boxes of every interval, possibly of
 // several candidate grids: interval = (index / nbox) % nn).  The box is split
 // into kKBSub x kKBSub sub-boxes; J = jfac w^2 is bounded by its values at
-// the sub-box's w edges.
+// the sub-box's w edges.  Also records the table cell of the lower corner of
+// the box's (alpha, beta) range as the start of the sampler's cell searches.
 constexpr int kKBSub = 4;
 __global__ void k_kb_bounds(SABView s, const double4* __restrict__ node_par, int nn, int nbox, int64_t total,
                             KBBox* __restrict__ box) {
@@ -105,57 +106,132 @@ __global__ void k_kb_bounds(SABView s, const double4* __restrict__ node_par, int
       tlow = dmin(tlow, smin * (jfac * sw0 * sw0));
     }
   }
-  b.env = env * (1.0 + 1e-9);
-  b.tlow = tlow * (1.0 - 1e-9);
+  b.env = __double2float_ru(env * (1.0 + 1e-9));
+  b.tlow = __double2float_rd(tlow * (1.0 - 1e-9));
+  // cells covered by the box's (alpha, beta) range (with the margins of the
+  // sampler's rounding); the envelope sampler requires grids of at most
+  // 65536 points
+  double alo = __builtin_huge_val(), ahi = -__builtin_huge_val();
+  const double r0 = w0 * wmax, r1 = w1 * wmax;
+  for (int c = 0; c < 2; ++c) {
+    const double f = 2.0 * seps * (1.0 - 2.0 * (c ? k1 : k0));
+    const double a_r0 = r0 * r0 + eps - f * r0, a_r1 = r1 * r1 + eps - f * r1;
+    ahi = dmax(ahi, dmax(a_r0, a_r1));
+    alo = dmin(alo, dmin(a_r0, a_r1));
+    const double rs = 0.5 * f;
+    if (rs > r0 && rs < r1) alo = dmin(alo, eps - 0.25 * f * f);
+  }
+  double blo = r0 * r0 - eps, bhi = r1 * r1 - eps;
+  const double ma = 1e-9 * (1.0 + fabs(alo) + fabs(ahi)) + 1e-9 * eps;
+  const double mb = 1e-9 * (1.0 + fabs(blo) + fabs(bhi)) + 1e-9 * eps;
+  alo -= ma; ahi += ma; blo -= mb; bhi += mb;
+  const uint32_t ia0 = (uint32_t)kb_cell_clamped(s.alpha, s.na, alo);
+  const uint32_t ia1 = (uint32_t)kb_cell_clamped(s.alpha, s.na, ahi);
+  const uint32_t ib0 = (uint32_t)kb_cell_clamped(s.beta, s.nb, blo);
+  const uint32_t ib1 = (uint32_t)kb_cell_clamped(s.beta, s.nb, bhi);
+  b.arange = ia0 | (ia1 << 16);
+  b.brange = ib0 | (ib1 << 16);
   box[t] = b;
 }
 
-// S(alpha,beta)-only sampling with the envelope sampler; neutrons outside
-// its energy range or without an accepted proposal go to the queue of the
+// Sampling: one lane per neutron at a time, one proposal per lane per loop
+// step, so that lanes whose neutron needs more proposals do not hold up the
+// others.  A wave owns chunks of kKBChunk consecutive neutrons (chunk c of
+// the wave w: c = w, w + nwaves, ...); lanes that finish a neutron take the
+// next ones of the chunk in lane order, so that a step's loads and stores
+// fall on neighbouring entries.  Neutrons outside the sampler's energy range
+// or without an accepted proposal after maxtries go to the queue of the
 // plain-sampler kernel, which restarts their Philox stream from draw 0.
-__global__ void __launch_bounds__(256) k_kb_sample_iso(SABView s, KBView k, const double* __restrict__ ekin,
-                                                       int64_t n, int64_t total, uint64_t seed, uint32_t stream,
-                                                       double* __restrict__ oE, double* __restrict__ oMu,
-                                                       unsigned long long* __restrict__ nerr,
-                                                       int64_t* __restrict__ queue,
-                                                       unsigned long long* __restrict__ qcount, int maxtries) {
-  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; kk < total; kk += gstride) {
-    const int64_t i = (total == n) ? kk : (kk % n);
-    const double E = ekin[i];
-    Rng rng;
-    rng.init(seed, stream, (uint64_t)kk);
-    double a, b;
-    if (kb_sample_ab(s, k, E, rng, a, b, maxtries) != kKBAccepted) {
-      const unsigned long long slot = atomicAdd(qcount, 1ull);
-      queue[slot] = kk;
-      continue;
+// Mode 0: outputs (E', mu); mode 1: raw (alpha, beta).
+constexpr int64_t kKBChunk = 1024;
+template <int Mode>
+__global__ void __launch_bounds__(256) k_kb_sample(SABView s, KBView k, const double* __restrict__ ekin, int64_t n,
+                                                   int64_t total, uint64_t seed, uint32_t stream,
+                                                   double* __restrict__ o1, double* __restrict__ o2,
+                                                   unsigned long long* __restrict__ nerr,
+                                                   int64_t* __restrict__ queue,
+                                                   unsigned long long* __restrict__ qcount, int maxtries) {
+  const int lane = (int)(threadIdx.x & 63);
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  // wave-uniform chunk state
+  int64_t chunk = wave;
+  int64_t next = chunk * kKBChunk;
+  int64_t cend = next + kKBChunk < total ? next + kKBChunk : total;
+  // lane state
+  bool active = false;
+  int64_t kk = 0;
+  int tries = 0;
+  double E = 0.0;
+  KBNeutron q{};
+  Rng rng;
+  rng.init(seed, stream, 0);
+  while (true) {
+    // idle lanes take the next neutrons of the chunk, moving on to the
+    // wave's next chunk when this one is used up
+    uint64_t idle = __ballot(!active);
+    while (idle && next < total) {
+      const int64_t take = (int64_t)__popcll(idle) < cend - next ? (int64_t)__popcll(idle) : cend - next;
+      if (!active) {
+        const int64_t rank = (int64_t)__popcll(idle & below);
+        if (rank < take) {
+          kk = next + rank;
+          E = ekin[(total == n) ? kk : (kk % n)];
+          const int iv = kb_interval(k, E);
+          if (iv < 0) {
+            queue[atomicAdd(qcount, 1ull)] = kk;
+          } else {
+            kb_neutron(s, k, iv, E, q);
+            rng.init(seed, stream, (uint64_t)kk);
+            tries = 0;
+            active = true;
+          }
+        }
+      }
+      next += take;
+      if (next >= cend) {
+        chunk += nwaves;
+        next = chunk * kKBChunk;
+        if (next > total) next = total;
+        cend = next + kKBChunk < total ? next + kKBChunk : total;
+      }
+      idle = __ballot(!active);
     }
-    double Ef, mu;
-    const int rc = sab_ab_to_emu(s, rng, E, a, b, Ef, mu);
-    if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
-    oE[kk] = Ef;
-    oMu[kk] = mu;
+    if (!__ballot(active)) break;
+    if (active) {
+      double a, b;
+      if (kb_propose(s, k, q, rng, a, b)) {
+        if constexpr (Mode == 0) {
+          double Ef, mu;
+          if (sab_ab_to_emu(s, rng, E, a, b, Ef, mu)) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
+          o1[kk] = Ef;
+          o2[kk] = mu;
+        } else {
+          o1[kk] = a;
+          o2[kk] = b;
+        }
+        active = false;
+      } else if (++tries >= maxtries) {
+        queue[atomicAdd(qcount, 1ull)] = kk;
+        active = false;
+      }
+    }
   }
 }
 
-// raw (alpha, beta) of the envelope sampler (statistics tests); fallback
-// neutrons get the plain sampler's values
-__global__ void __launch_bounds__(256) k_kb_sample_ab(SABView s, KBView k, const double* __restrict__ ekin,
-                                                      int64_t n, uint64_t seed, uint32_t stream,
-                                                      double* __restrict__ oa, double* __restrict__ ob,
-                                                      unsigned long long* __restrict__ nfallback, int maxtries) {
-  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; kk < n; kk += gstride) {
+// raw (alpha, beta) of the plain sampler for the neutrons the envelope
+// sampler handed back (from the first draw of their stream)
+__global__ void k_kb_queue_ab(SABView s, const double* __restrict__ ekin, uint64_t seed, uint32_t stream,
+                              double* __restrict__ oa, double* __restrict__ ob, const int64_t* __restrict__ queue,
+                              const unsigned long long* __restrict__ qcount) {
+  const int64_t nq = (int64_t)*qcount;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nq; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t kk = queue[j];
     Rng rng;
     rng.init(seed, stream, (uint64_t)kk);
     double a = -1.0, b = 0.0;
-    if (kb_sample_ab(s, k, ekin[kk], rng, a, b, maxtries) != kKBAccepted) {
-      atomicAdd(nfallback, 1ull);
-      Rng r2;
-      r2.init(seed, stream, (uint64_t)kk);
-      if (sab_sample_ab<true>(s, ekin[kk], r2, a, b)) { a = -1.0; b = 0.0; }
-    }
+    if (sab_sample_ab<true>(s, ekin[kk], rng, a, b)) { a = -1.0; b = 0.0; }
     oa[kk] = a;
     ob[kk] = b;
   }
@@ -176,23 +252,48 @@ void launch_kb_bounds(const SABView& s, const double4* node_par, int nn, int nbo
                      total, box);
 }
 
+// persistent grid: as many blocks as fit on the device at once (the chunks
+// are handed out statically), fewer for small batches
+template <int Mode>
+static unsigned kb_sample_grid(int64_t total) {
+  static int per_cu = 0, cus = 0;
+  if (!per_cu) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_kb_sample<Mode>, 256, 0) != hipSuccess || nb < 1) nb = 2;
+    per_cu = nb;
+  }
+  const int64_t chunks = (total + kKBChunk - 1) / kKBChunk;
+  int64_t g = (chunks + 3) / 4;  // 4 waves per block, one chunk per wave at least
+  const int64_t gmax = (int64_t)per_cu * cus;
+  if (g > gmax) g = gmax;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
 void launch_kb_sample_iso(const SABView& s, const KBView& k, const double* ekin, int64_t n, int64_t repeat,
                           uint64_t seed, uint32_t stream, double* oE, double* oMu, unsigned long long* nerr,
                           int64_t* queue, unsigned long long* qcount, int maxtries, hipStream_t st) {
   const int64_t total = n * repeat;
   if (total <= 0) return;
   launch_zero_u64(qcount, st);
-  hipLaunchKernelGGL(k_kb_sample_iso, dim3(kb_grid(total)), dim3(256), 0, st, s, k, ekin, n, total, seed, stream, oE,
-                     oMu, nerr, queue, qcount, maxtries);
+  hipLaunchKernelGGL(k_kb_sample<0>, dim3(kb_sample_grid<0>(total)), dim3(256), 0, st, s, k, ekin, n, total, seed,
+                     stream, oE, oMu, nerr, queue, qcount, maxtries);
   launch_sab_queue(s, ekin, n, total, seed, stream, oE, oMu, nerr, queue, qcount, st);
 }
 
 void launch_kb_sample_ab(const SABView& s, const KBView& k, const double* ekin, int64_t n, uint64_t seed,
-                         uint32_t stream, double* oa, double* ob, unsigned long long* nfallback, int maxtries,
-                         hipStream_t st) {
+                         uint32_t stream, double* oa, double* ob, int64_t* queue, unsigned long long* qcount,
+                         int maxtries, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_kb_sample_ab, dim3(kb_grid(n)), dim3(256), 0, st, s, k, ekin, n, seed, stream, oa, ob,
-                     nfallback, maxtries);
+  launch_zero_u64(qcount, st);
+  hipLaunchKernelGGL(k_kb_sample<1>, dim3(kb_sample_grid<1>(n)), dim3(256), 0, st, s, k, ekin, n, n, seed, stream, oa,
+                     ob, (unsigned long long*)nullptr, queue, qcount, maxtries);
+  hipLaunchKernelGGL(k_kb_queue_ab, dim3(kb_grid(n)), dim3(256), 0, st, s, ekin, seed, stream, oa, ob, queue,
+                     qcount);
 }
 
 }  // namespace ncamd

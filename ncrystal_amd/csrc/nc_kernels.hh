@@ -94,8 +94,8 @@ void launch_kb_sample_iso(const SABView&, const KBView&, const double* ekin, int
                           uint64_t seed, uint32_t stream, double* oE, double* oMu, unsigned long long* nerr,
                           int64_t* queue, unsigned long long* qcount, int maxtries, hipStream_t);
 void launch_kb_sample_ab(const SABView&, const KBView&, const double* ekin, int64_t n, uint64_t seed,
-                         uint32_t stream, double* oa, double* ob, unsigned long long* nfallback, int maxtries,
-                         hipStream_t);
+                         uint32_t stream, double* oa, double* ob, int64_t* queue, unsigned long long* qcount,
+                         int maxtries, hipStream_t);
 // helpers shared with the S(alpha,beta) kernels (nc_kernels.hip)
 void launch_zero_u64(unsigned long long* p, hipStream_t);
 // drains a queue of neutron indices with the plain sampler (from draw 0)
