@@ -1478,16 +1478,22 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
   }
   cand.clear();
   cand.shrink_to_fit();
-  // alias tables (Walker/Vose) per interval over the box masses env x area
-  std::vector<KBAlias> alias(nper);
+  // alias tables (Walker/Vose) per interval over the box masses env x area,
+  // stored with the boxes in one record per slot
+  std::vector<KBSlot> alias(nper);
   std::vector<double> q((size_t)k.nbox);
   std::vector<int> small, large;
   for (int i = 0; i < nn; ++i) {
     const KBBox* bx = boxes.data() + (size_t)i * k.nbox;
-    KBAlias* al = alias.data() + (size_t)i * k.nbox;
+    KBSlot* al = alias.data() + (size_t)i * k.nbox;
     double tot = 0.0;
     for (int j = 0; j < k.nbox; ++j) tot += bx[j].env * area(bx[j]);
-    for (int j = 0; j < k.nbox; ++j) al[j] = KBAlias{1.0, 1.0, 0.0, j, 0};
+    for (int j = 0; j < k.nbox; ++j) {
+      al[j] = KBSlot{};
+      al[j].prob = 1.0; al[j].inv_p = 1.0; al[j].inv_q = 0.0;
+      al[j].box = bx[j];
+      al[j].alias = j;
+    }
     if (!(tot > 0.0) || !std::isfinite(tot)) continue;  // nothing to sample: every proposal is rejected -> fallback
     small.clear(); large.clear();
     for (int j = 0; j < k.nbox; ++j) {
@@ -1507,10 +1513,8 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
       al[j].inv_q = al[j].prob < 1.0 ? 1.0 / (1.0 - al[j].prob) : 0.0;
     }
   }
-  DevBuf* bb = kb_newbuf(p); bb->upload(boxes.data(), boxes.size());
   DevBuf* ba = kb_newbuf(p); ba->upload(alias.data(), alias.size());
-  k.box = bb->as<KBBox>();
-  k.alias = ba->as<KBAlias>();
+  k.slot = ba->as<KBSlot>();
   p->kb = k;
   p->kb_maxtries = maxtries;
   p->kb_node_h = node;

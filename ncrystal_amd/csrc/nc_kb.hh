@@ -47,12 +47,16 @@ struct KBBox {
   float env, tlow;
   uint32_t arange, brange;
 };
-// alias slot: acceptance of the slot's own box with the reciprocals that turn
-// the pick's residual into a uniform number, and the box taken otherwise
-struct KBAlias {
+// alias slot j with box j stored in it (one 64-byte record): acceptance of
+// the slot's own box with the reciprocals that turn the pick's residual into
+// a uniform number, the box taken otherwise, and box j itself.  A proposal
+// reads one record, and a second one only when the alias is taken.
+struct alignas(64) KBSlot {
   double prob, inv_p, inv_q;
+  KBBox box;
   int alias, pad;
 };
+static_assert(sizeof(KBSlot) == 64, "one slot record per 64 bytes");
 
 struct KBView {
   int nn;                          // energy intervals
@@ -61,8 +65,7 @@ struct KBView {
   double elo, ehi, lr_inv;         // interval index guess: floor(log(E/elo) * lr_inv)
   const double* node_e;            // [nn+1] interval edges
   const double4* node_par;         // [nn]: (eps, sqrt(eps), wmax, 8 sqrt(eps) wmax^3)
-  const KBAlias* alias;            // [nn*nbox] alias slots
-  const KBBox* box;                // [nn*nbox] boxes, strip-major
+  const KBSlot* slot;              // [nn*nbox] alias slots with their boxes, strip-major
 };
 
 // interpolate_loglin_fallbacklinlin_fast (NCSABUtils.hh:176-195)
@@ -113,8 +116,7 @@ NCD int kb_interval(const KBView& k, double E) {
 
 // per-neutron constants of the proposal loop
 struct KBNeutron {
-  const KBAlias* al;
-  const KBBox* bx;
+  const KBSlot* sl;
   double eps, seps, wmax, jfac, ediv;
 };
 
@@ -122,8 +124,7 @@ NCD void kb_neutron(const SABView& s, const KBView& k, int n, double E, KBNeutro
   const double4 par = k.node_par[n];
   q.eps = par.x; q.seps = par.y; q.wmax = par.z; q.jfac = par.w;
   q.ediv = E / s.kT;
-  q.al = k.alias + (size_t)n * k.nbox;
-  q.bx = k.box + (size_t)n * k.nbox;
+  q.sl = k.slot + (size_t)n * k.nbox;
 }
 
 // One proposal; true if accepted, with (alpha, beta).
@@ -136,12 +137,17 @@ NCD bool kb_propose(const SABView& s, const KBView& k, const KBNeutron& q, Rng& 
   int slot = (int)u;
   slot = slot >= k.nbox ? k.nbox - 1 : slot;
   const double f = u - (double)slot;
-  const KBAlias as = q.al[slot];
-  int box;
+  const KBSlot& rec = q.sl[slot];
+  const double prob = rec.prob;
   double uw;
-  if (f < as.prob) { box = slot; uw = f * as.inv_p; }
-  else { box = as.alias; uw = (f - as.prob) * as.inv_q; }
-  const KBBox b = q.bx[box];
+  KBBox b;
+  if (f < prob) {
+    uw = f * rec.inv_p;
+    b = rec.box;
+  } else {
+    uw = (f - prob) * rec.inv_q;
+    b = q.sl[rec.alias].box;
+  }
   const double w0 = b.w0, w1 = b.w1, k0 = b.k0;
   const double w = w0 + (w1 - w0) * uw;
   const double kap = k0 + ((double)b.k1 - k0) * (((double)r2 + 0.5) * 0x1.0p-32);
