@@ -406,7 +406,8 @@ NCD void scb_walk(const SCBView& v, ScbLane& L, int n_select, double target) {
 #pragma unroll
     for (int q = 0; q < kScbGroup; ++q) {
       const float4 r = v.nrec32[jb + q];
-      const float d32 = r.x * fx + r.y * fy + r.z * fz;
+      // contracted FP32 arithmetic: the band covers its rounding
+      const float d32 = __fmaf_rn(r.z, fz, __fmaf_rn(r.y, fy, r.x * fx));
       const float s32 = fw * r.w;
       cand |= (fabsf(fabsf(d32) - s32) < band32 ? 1u : 0u) << q;
     }
