@@ -39,6 +39,12 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# HBM traffic of one launch of the learned sampler on this workload (256
+# intervals x 128 x 32 boxes): 20.86 GB fetched + 2.21 GB written, from the
+# FETCH_SIZE / WRITE_SIZE counter passes scaled by a plain 8e8-byte copy
+# (profiles/k_kb_sample_sab200x800_learned_grid.md); counters cannot be read
+# inside this process, so the figure is carried here
+LEARNED_TRAFFIC_GB = 23.07
 BYTES_PER_SAMPLE = 24  # E in (8) + E' out (8) + mu out (8)
 
 
@@ -562,7 +568,8 @@ def run_sab(args):
                    "train_steps": args.train_steps, "train_and_setup_s": t_setup, "train_final_loss": loss,
                    "ks_vs_reference": ks, "ks_samples_per_energy": args.ks_samples},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": LEARNED_TRAFFIC_GB,
+                     "traffic_unit": "GB per launch (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy)",
                      "kernel": "k_kb_sample<0>", "kernel_ms": r_learned["ms"],
                      "plain_kernel": "k_sab_sample_iso_main", "plain_kernel_ms": r_plain["ms"],
                      "bytes_per_sample": BYTES_PER_SAMPLE},
