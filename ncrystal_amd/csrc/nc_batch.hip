@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(256) k_batch_persistent(const BatchMat* __rest
         Rng rng;
         rng.init(rngs[tile.mat].seed, (uint32_t)rngs[tile.mat].stream, (uint64_t)k);
         double Ef, mu;
-        const int rc = comp_sample_iso(m.view, rng, E, Ef, mu);
+        const int rc = comp_sample_iso<kCompGeneral>(m.view, rng, E, Ef, mu);
         if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
         o1[m.offset + k] = Ef;
         o2[m.offset + k] = mu;

@@ -56,7 +56,22 @@ NCD double comp_xs(const CompView& c, double E, double* commul) {
   return tot;
 }
 
+// Kernel variants by the kinds a composition holds, so that a kernel carries
+// only the samplers it can reach (the S(alpha,beta) sampler's registers
+// would otherwise limit the occupancy of PowderBragg / ElInc materials).
+enum : int { kCompGeneral = 0, kCompNoSAB = 1, kCompPBOnly = 2 };
+
+template <int Variant>
 NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, double& mu) {
+  if constexpr (Variant == kCompPBOnly) {
+    // a single PowderBragg component: no pick (NCProcImpl.cc:379-389 draws
+    // nothing for one component)
+    if (!(E >= c.domain_lo && E <= c.domain_hi)) { Ef = E; mu = 1.0; return kOK; }
+    Ef = E;
+    mu = pb_sample_mu(c.pb, rng, E);
+    return kOK;
+  }
+  constexpr bool WithSAB = Variant == kCompGeneral;
   if (!(E >= c.domain_lo && E <= c.domain_hi)) { Ef = E; mu = 1.0; return kOK; }
   double commul[kMaxComp];
   const double tot = comp_xs<false>(c, E, commul);
@@ -65,9 +80,18 @@ NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, doubl
   switch (c.kind[ic]) {
     case kKindPB: Ef = E; mu = pb_sample_mu(c.pb, rng, E); return kOK;
     case kKindElInc: Ef = E; mu = elinc_sample_mu(c.ei, rng, E); return kOK;
-    case kKindSAB: return sab_sample_iso(c.sab, rng, E, Ef, mu);
+    case kKindSAB:
+      if constexpr (WithSAB) return sab_sample_iso(c.sab, rng, E, Ef, mu);
+      else return kErrBadSampler;
     default: return kErrBadSampler;
   }
+}
+
+NCHD int comp_variant(const CompView& c) {
+  if (c.n == 1 && c.kind[0] == kKindPB) return kCompPBOnly;
+  for (int i = 0; i < c.n; ++i)
+    if (c.kind[i] == kKindSAB) return kCompGeneral;
+  return kCompNoSAB;
 }
 
 // per-neutron kernels (nc_kernels.hip)

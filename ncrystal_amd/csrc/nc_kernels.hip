@@ -19,6 +19,7 @@ __global__ void __launch_bounds__(256) k_xs_iso(CompView c, const double* __rest
   }
 }
 
+template <int Variant>
 __global__ void __launch_bounds__(256) k_sample_iso(CompView c, const double* __restrict__ ekin,
                                                     int64_t n, int64_t total, uint64_t seed,
                                                     uint32_t stream, double* __restrict__ oE,
@@ -30,13 +31,14 @@ __global__ void __launch_bounds__(256) k_sample_iso(CompView c, const double* __
     Rng rng;
     rng.init(seed, stream, (uint64_t)k);
     double Ef, mu;
-    const int rc = comp_sample_iso(c, rng, ekin[i], Ef, mu);
+    const int rc = comp_sample_iso<Variant>(c, rng, ekin[i], Ef, mu);
     if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
     oE[k] = Ef;
     oMu[k] = mu;
   }
 }
 
+template <int Variant>
 __global__ void __launch_bounds__(256) k_sample_dirs(CompView c, const double* __restrict__ ekin,
                                                      const double* __restrict__ dirs, int64_t n,
                                                      uint64_t seed, uint32_t stream,
@@ -47,7 +49,7 @@ __global__ void __launch_bounds__(256) k_sample_dirs(CompView c, const double* _
     Rng rng;
     rng.init(seed, stream, (uint64_t)k);
     double Ef, mu;
-    const int rc = comp_sample_iso(c, rng, ekin[k], Ef, mu);
+    const int rc = comp_sample_iso<Variant>(c, rng, ekin[k], Ef, mu);
     double ox = 0.0, oy = 0.0, oz = 0.0;
     if (rc) {
       Ef = -1.0;
@@ -194,14 +196,36 @@ void launch_sample_iso(const CompView& c, const double* ekin, int64_t n, int64_t
                        uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, hipStream_t st) {
   const int64_t total = n * repeat;
   if (total <= 0) return;
-  hipLaunchKernelGGL(k_sample_iso, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin, n, total, seed,
-                     stream, oE, oMu, nerr);
+  switch (comp_variant(c)) {
+    case kCompPBOnly:
+      hipLaunchKernelGGL(k_sample_iso<kCompPBOnly>, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin, n, total,
+                         seed, stream, oE, oMu, nerr);
+      break;
+    case kCompNoSAB:
+      hipLaunchKernelGGL(k_sample_iso<kCompNoSAB>, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin, n, total,
+                         seed, stream, oE, oMu, nerr);
+      break;
+    default:
+      hipLaunchKernelGGL(k_sample_iso<kCompGeneral>, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin, n,
+                         total, seed, stream, oE, oMu, nerr);
+  }
 }
 void launch_sample_dirs(const CompView& c, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
                         uint32_t stream, double* oE, double* oD, unsigned long long* nerr, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_sample_dirs, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, dirs, n, seed, stream,
-                     oE, oD, nerr);
+  switch (comp_variant(c)) {
+    case kCompPBOnly:
+      hipLaunchKernelGGL(k_sample_dirs<kCompPBOnly>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, dirs, n,
+                         seed, stream, oE, oD, nerr);
+      break;
+    case kCompNoSAB:
+      hipLaunchKernelGGL(k_sample_dirs<kCompNoSAB>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, dirs, n,
+                         seed, stream, oE, oD, nerr);
+      break;
+    default:
+      hipLaunchKernelGGL(k_sample_dirs<kCompGeneral>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, dirs, n,
+                         seed, stream, oE, oD, nerr);
+  }
 }
 void launch_sab_ab(const SABView& s, const double* ekin, int64_t n, uint64_t seed, uint32_t stream, double* oa,
                    double* ob, unsigned long long* nerr, hipStream_t st) {
