@@ -1193,6 +1193,7 @@ struct Batch {
   std::vector<double> cost;  // estimated relative cost per neutron
   int64_t total = 0;
   DevBuf d_mats, d_rngs, d_tiles, d_next, d_err;
+  DevBuf d_sab, d_tcount, d_highe, d_hcount;  // staged sampling (BatchWork)
   std::vector<BatchRng> h_rngs;
   int tile = 4096, nblocks = 1024, ordering = 1;
   int ntiles = 0;
@@ -1240,11 +1241,13 @@ void batch_build_tiles(Batch* b) {
   for (int m : order)
     for (int64_t k = 0; k < b->counts[m]; k += b->tile)
       tiles.push_back(BatchTile{m, k, std::min<int64_t>(b->counts[m], k + b->tile)});
-  if (tiles.size() > (size_t)INT32_MAX) raise("BadInput", "too many tiles");
+  // the high-energy queue packs (tile << 40 | neutron)
+  if (tiles.size() >= ((size_t)1 << 23)) raise("BadInput", "too many tiles (use a larger tile size)");
   // the captured graph refers to the old tile list
   if (b->gexec) { NCAMD_HIP(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
   b->ntiles = (int)tiles.size();
   b->d_tiles.upload(tiles.data(), tiles.size());
+  b->d_tcount.ensure(sizeof(int) * std::max<size_t>(tiles.size(), 1));
   b->tiles_valid = true;
 }
 
@@ -1268,9 +1271,14 @@ void batch_run(Batch* b, int mode, const double* in, double* o1, double* o2, hip
     if (b->gexec) { NCAMD_HIP(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
     hipGraph_t g = nullptr;
     NCAMD_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    BatchWork w;
+    w.next = b->d_next.as<unsigned>();
+    w.sab = b->d_sab.as<int64_t>();
+    w.tcount = b->d_tcount.as<int>();
+    w.highe = b->d_highe.as<int64_t>();
+    w.hcount = b->d_hcount.as<unsigned long long>();
     launch_batch(b->d_mats.as<BatchMat>(), b->d_rngs.as<BatchRng>(), b->d_tiles.as<BatchTile>(), b->ntiles,
-                 b->nblocks, mode, in, o1, o2, b->d_next.as<unsigned>(), b->d_err.as<unsigned long long>(),
-                 b->cap_stream);
+                 b->nblocks, mode, in, o1, o2, w, b->d_err.as<unsigned long long>(), b->cap_stream);
     const hipError_t e1 = hipGetLastError();
     const hipError_t e2 = hipStreamEndCapture(b->cap_stream, &g);
     NCAMD_HIP(e1);
@@ -1701,7 +1709,7 @@ void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat,
       NCAMD_HIP(hipStreamSynchronize(st));  // the old queue may still be in use on this stream
       p->q_idx.alloc(need);
     }
-    if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
+    if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
     if (single->use_kb)
       launch_kb_sample_iso(single->sab, single->kb, ekin, n, repeat, p->seed, stream, oE, oMu,
                            p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
@@ -1711,8 +1719,15 @@ void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat,
                             p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
                             p->q_count.as<unsigned long long>(), st);
   } else {
+    // two queues of up to n * repeat neutrons for the staged composition kernels
+    const size_t need = 2 * sizeof(int64_t) * (size_t)std::max<int64_t>(n * repeat, 1);
+    if (p->q_idx.bytes < need) {
+      NCAMD_HIP(hipStreamSynchronize(st));  // the old queue may still be in use on this stream
+      p->q_idx.alloc(need);
+    }
+    if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
     launch_sample_iso(make_view(p), ekin, n, repeat, p->seed, stream, oE, oMu, p->d_err.as<unsigned long long>(),
-                      st);
+                      p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), st);
   }
 }
 
@@ -1726,7 +1741,7 @@ ScbWork scb_work(Proc* p, int64_t n, hipStream_t st) {
     p->scb_perm.ensure(sizeof(int64_t) * m);
     p->scb_keys.ensure(sizeof(unsigned) * m);
   }
-  if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
+  if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
   if (!p->scb_hist.p) {
     p->scb_hist.alloc(sizeof(unsigned) * kScbBuckets);
     NCAMD_HIP(hipMemset(p->scb_hist.p, 0, sizeof(unsigned) * kScbBuckets));
@@ -2207,7 +2222,10 @@ ncamd_batch_t ncamd_batch_create(int nmat, const ncamd_process_t* procs, const i
   b->h_rngs.resize((size_t)nmat);
   b->d_mats.upload(hm.data(), hm.size());
   b->d_rngs.alloc(sizeof(BatchRng) * (size_t)nmat);
-  b->d_next.alloc(sizeof(unsigned));
+  b->d_next.alloc(2 * sizeof(unsigned));
+  b->d_sab.alloc(sizeof(int64_t) * (size_t)std::max<int64_t>(b->total, 1));
+  b->d_highe.alloc(sizeof(int64_t) * (size_t)std::max<int64_t>(b->total, 1));
+  b->d_hcount.alloc(sizeof(unsigned long long));
   b->d_err.alloc(sizeof(unsigned long long));
   NCAMD_HIP(hipMemset(b->d_err.p, 0, sizeof(unsigned long long)));
   out.internal = b.release();
@@ -2373,7 +2391,7 @@ int64_t ncamd_dev_sab_sample_alphabeta_envelope(ncamd_process_t h, const double*
   NCAMD_TRY
   Proc* p = extract(h);
   if (p->kind != kKindSAB || p->kb_node_h.empty()) raise("BadInput", "no envelope sampler tables");
-  if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
+  if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
   hipStream_t st = as_stream(stream);
   const size_t need = sizeof(int64_t) * (size_t)std::max<unsigned long>(n, 1);
   if (p->q_idx.bytes < need) {
@@ -2481,7 +2499,7 @@ double ncamd_mlp_train_sab(ncamd_mlp_t h, ncamd_process_t sab, int steps, int ba
   if (p->kind != kKindSAB) raise("BadInput", "training needs an S(alpha,beta) process");
   if (steps < 1 || batch < 64 || !(lr > 0.0)) raise("BadInput", "invalid training parameters");
   m->ensure(batch);
-  if (!p->q_count.p) p->q_count.alloc(sizeof(unsigned long long));
+  if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
   const double emin = std::exp(m->lelo), emax = std::exp(m->lelo + 1.0 / m->linv);
   const double bmax = p->sab_beta_h.back();
   double acc = 0.0;

@@ -87,6 +87,25 @@ NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, doubl
   }
 }
 
+// First stage of a composition with an S(alpha,beta) part: component cross
+// sections, the pick, and PowderBragg / ElInc sampling in place.  Returns
+// kCompToSAB when the S(alpha,beta) component is picked; its sampling then
+// continues the neutron's stream after the pick, and the S(alpha,beta) stage
+// replays the pick's draw (one draw when there are several components).
+constexpr int kCompToSAB = 1;
+NCD int comp_pick_sample(const CompView& c, Rng& rng, double E, double& Ef, double& mu) {
+  if (!(E >= c.domain_lo && E <= c.domain_hi)) { Ef = E; mu = 1.0; return kOK; }
+  double commul[kMaxComp];
+  comp_xs<false>(c, E, commul);
+  const int ic = pick_idx(rng, commul, c.n);
+  switch (c.kind[ic]) {
+    case kKindPB: Ef = E; mu = pb_sample_mu(c.pb, rng, E); return kOK;
+    case kKindElInc: Ef = E; mu = elinc_sample_mu(c.ei, rng, E); return kOK;
+    case kKindSAB: return kCompToSAB;
+    default: return kErrBadSampler;
+  }
+}
+
 NCHD int comp_variant(const CompView& c) {
   if (c.n == 1 && c.kind[0] == kKindPB) return kCompPBOnly;
   for (int i = 0; i < c.n; ++i)
@@ -97,8 +116,11 @@ NCHD int comp_variant(const CompView& c) {
 // per-neutron kernels (nc_kernels.hip)
 void launch_xs_iso(const CompView&, const double* ekin, int64_t n, int64_t repeat, double* out, bool fp32,
                    hipStream_t);
+// queues: [2 x n*repeat] neutron indices, qcount: [2] counters (used by
+// compositions with an S(alpha,beta) part)
 void launch_sample_iso(const CompView&, const double* ekin, int64_t n, int64_t repeat, uint64_t seed,
-                       uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, hipStream_t);
+                       uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, int64_t* queues,
+                       unsigned long long* qcount, hipStream_t);
 void launch_sample_dirs(const CompView&, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
                         uint32_t stream, double* oE, double* oD, unsigned long long* nerr, hipStream_t);
 void launch_sab_ab(const SABView&, const double* ekin, int64_t n, uint64_t seed, uint32_t stream, double* oa,
@@ -155,10 +177,18 @@ struct BatchTile {
   int mat;
   int64_t begin, end;  // neutron range within the material
 };
+// work buffers of the staged sampling launch
+struct BatchWork {
+  unsigned* next;              // [2] tile counters of the two persistent stages
+  int64_t* sab;                // [total] per tile, its neutrons that picked S(alpha,beta)
+  int* tcount;                 // [ntiles] their number per tile
+  int64_t* highe;              // [total] (tile << 40 | neutron) above the table's Emax
+  unsigned long long* hcount;  // [1]
+};
 // mode 0: cross sections into o1; mode 1: isotropic sampling into (o1=E', o2=mu)
 void launch_batch(const BatchMat* mats, const BatchRng* rngs, const BatchTile* tiles, int ntiles, int nblocks,
-                  int mode, const double* ekin, double* o1, double* o2, unsigned* next, unsigned long long* nerr,
-                  hipStream_t);
+                  int mode, const double* ekin, double* o1, double* o2, const BatchWork& w,
+                  unsigned long long* nerr, hipStream_t);
 
 // single-crystal Bragg, oriented (nc_scb.hip): counting sort of the batch by
 // scan length, main kernel, Romberg queue kernel

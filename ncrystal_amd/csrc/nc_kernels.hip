@@ -62,6 +62,66 @@ __global__ void __launch_bounds__(256) k_sample_dirs(CompView c, const double* _
   }
 }
 
+// Compositions with an S(alpha,beta) part, in stages so that no kernel
+// carries every sampler: k_comp_pick samples PowderBragg / ElInc picks in
+// place and queues the neutrons that pick S(alpha,beta); k_comp_sab<false>
+// samples those below the table's Emax and queues the rest for
+// k_comp_sab<true> (free-gas extender).  The S(alpha,beta) stages restart
+// the neutron's Philox stream and replay the pick's draw (`skip`), so every
+// neutron gets what one kernel doing all of it would give.
+__global__ void __launch_bounds__(256) k_comp_pick(CompView c, const double* __restrict__ ekin, int64_t n,
+                                                   int64_t total, uint64_t seed, uint32_t stream,
+                                                   double* __restrict__ oE, double* __restrict__ oMu,
+                                                   unsigned long long* __restrict__ nerr, int64_t* __restrict__ sq,
+                                                   unsigned long long* __restrict__ sqc) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gstride) {
+    const int64_t i = (total == n) ? k : (k % n);
+    Rng rng;
+    rng.init(seed, stream, (uint64_t)k);
+    double Ef, mu;
+    const int rc = comp_pick_sample(c, rng, ekin[i], Ef, mu);
+    if (rc == kCompToSAB) {
+      sq[atomicAdd(sqc, 1ull)] = k;
+      continue;
+    }
+    if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
+    oE[k] = Ef;
+    oMu[k] = mu;
+  }
+}
+
+template <bool HighE>
+__global__ void __launch_bounds__(256) k_comp_sab(SABView s, const double* __restrict__ ekin, int64_t n,
+                                                  int64_t total, uint64_t seed, uint32_t stream, int skip,
+                                                  double* __restrict__ oE, double* __restrict__ oMu,
+                                                  unsigned long long* __restrict__ nerr,
+                                                  const int64_t* __restrict__ inq,
+                                                  const unsigned long long* __restrict__ inqc,
+                                                  int64_t* __restrict__ hq, unsigned long long* __restrict__ hqc) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t nq = (int64_t)*inqc;
+  const double emax = s.egrid[s.ne - 1];
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gstride) {
+    const int64_t k = inq[q];
+    const double E = ekin[(total == n) ? k : (k % n)];
+    if constexpr (!HighE) {
+      if (E >= emax) {
+        hq[atomicAdd(hqc, 1ull)] = k;
+        continue;
+      }
+    }
+    Rng rng;
+    rng.init(seed, stream, (uint64_t)k);
+    if (skip) rng.gen();
+    double Ef, mu;
+    const int rc = sab_sample_iso<HighE>(s, rng, E, Ef, mu);
+    if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
+    oE[k] = Ef;
+    oMu[k] = mu;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_sab_ab(SABView s, const double* __restrict__ ekin, int64_t n,
                                                 uint64_t seed, uint32_t stream, double* __restrict__ oa,
                                                 double* __restrict__ ob,
@@ -193,7 +253,8 @@ void launch_xs_iso(const CompView& c, const double* ekin, int64_t n, int64_t rep
   else hipLaunchKernelGGL(k_xs_iso<false>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, n, repeat, out);
 }
 void launch_sample_iso(const CompView& c, const double* ekin, int64_t n, int64_t repeat, uint64_t seed,
-                       uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, hipStream_t st) {
+                       uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, int64_t* queues,
+                       unsigned long long* qcount, hipStream_t st) {
   const int64_t total = n * repeat;
   if (total <= 0) return;
   switch (comp_variant(c)) {
@@ -205,9 +266,23 @@ void launch_sample_iso(const CompView& c, const double* ekin, int64_t n, int64_t
       hipLaunchKernelGGL(k_sample_iso<kCompNoSAB>, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin, n, total,
                          seed, stream, oE, oMu, nerr);
       break;
-    default:
-      hipLaunchKernelGGL(k_sample_iso<kCompGeneral>, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin, n,
-                         total, seed, stream, oE, oMu, nerr);
+    default: {
+      int sab = -1;
+      for (int j = 0; j < c.n; ++j)
+        if (c.kind[j] == kKindSAB) sab = j;
+      int64_t* sq = queues;
+      int64_t* hq = queues + total;
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount);
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount + 1);
+      hipLaunchKernelGGL(k_comp_pick, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin, n, total, seed, stream,
+                         oE, oMu, nerr, sq, qcount);
+      if (sab < 0) break;  // no S(alpha,beta) part: nothing queued
+      const int skip = c.n > 1 ? 1 : 0;
+      hipLaunchKernelGGL(k_comp_sab<false>, dim3(grid_for(total, 256)), dim3(256), 0, st, c.sab, ekin, n, total,
+                         seed, stream, skip, oE, oMu, nerr, sq, qcount, hq, qcount + 1);
+      hipLaunchKernelGGL(k_comp_sab<true>, dim3(grid_for(total, 256)), dim3(256), 0, st, c.sab, ekin, n, total,
+                         seed, stream, skip, oE, oMu, nerr, hq, qcount + 1, nullptr, nullptr);
+    }
   }
 }
 void launch_sample_dirs(const CompView& c, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
