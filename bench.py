@@ -294,7 +294,7 @@ def run_powderbragg(args):
                    "neutrons_per_step": n, "families": int(len(dsp)), "planes": nplanes,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_sample_iso",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_sample_iso<kCompPBOnly>",
                      "kernel_ms": ts, "xs_fp64_ms": t64, "xs_fp32_ms": t32,
                      "xs_fp64_GBs": 16 * n / (t64 * 1e-3) / 1e9, "xs_fp32_GBs": 16 * n / (t32 * 1e-3) / 1e9},
         "cpu_baseline": cpu,
@@ -407,7 +407,8 @@ def run_batch(args):
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline_batch(mats, args.cpu_seconds, threads)
-    # dominant kernel: the persistent launch; E in, xs out (16 B) + E in, E' and mu out (24 B)
+    # the whole step (cross-section launch + the three sampling stages, timed together):
+    # E in, xs out (16 B) + E in, E' and mu out (24 B) per neutron
     achieved = 40.0 * tot / (ms * 1e-3) / 1e9
     nplanes = [len(m["pb"][1]) for m in mats]
     grids = [m["sab"]["sab"].size for m in mats]
@@ -436,7 +437,7 @@ def run_batch(args):
                                                       (tot * max(1, args.steps // 2) / loop_wall),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_batch_persistent",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_batch_xs + k_batch_pick + k_batch_sab + k_batch_highe",
                      "kernel_ms": ms, "bytes_per_neutron": 40},
         "cpu_baseline": cpu,
     }
