@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(kBatchBlock) k_batch_sab(const BatchMat* __res
     for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
       const int64_t k = sab[m.offset + tile.begin + j];
       const double E = ekin[m.offset + k];
-      if (E >= emax) {
+      if (!(E < emax)) {
         highe[atomicAdd(hcount, 1ull)] = ((int64_t)t << 40) | k;
         continue;
       }

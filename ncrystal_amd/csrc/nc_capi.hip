@@ -300,6 +300,7 @@ CompView make_view(Proc* p) {
     }
     if (dlo < dhi) { lo = std::min(lo, dlo); hi = std::max(hi, dhi); }
   }
+  v.use_domain = p->kind == kKindNone ? 1 : 0;
   v.domain_lo = lo;
   v.domain_hi = hi;
   return v;
@@ -1717,7 +1718,7 @@ void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat,
     else
       launch_sab_sample_iso(single->sab, ekin, n, repeat, p->seed, stream, oE, oMu,
                             p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
-                            p->q_count.as<unsigned long long>(), st);
+                            p->q_count.as<unsigned long long>(), p != single ? 1 : 0, st);
   } else {
     // two queues of up to n * repeat neutrons for the staged composition kernels
     const size_t need = 2 * sizeof(int64_t) * (size_t)std::max<int64_t>(n * repeat, 1);

@@ -15,6 +15,10 @@ struct CompView {
   int n;
   int kind[kMaxComp];
   double scale[kMaxComp];
+  // compositions answer 0 xs / no scattering outside the union of their
+  // components' energy domains (NCProcImpl.cc:344-389); a single process is
+  // evaluated as is (use_domain = 0)
+  int use_domain;
   double domain_lo, domain_hi;
   PBView pb;
   ElIncView ei;
@@ -24,7 +28,9 @@ struct CompView {
 // Composition semantics: ncrystal_core/src/interfaces/NCProcImpl.cc:166-204
 // (cumulative component xs), :353-362 (xs), :379-389 (component pick +
 // sampling).
-NCD double comp_domain_lo(const CompView& c) { return c.domain_lo; }
+NCD bool comp_outside(const CompView& c, double E) {
+  return c.use_domain && !(E >= c.domain_lo && E <= c.domain_hi);
+}
 
 template <bool FP32PB>
 NCD double pb_xs_sel(const PBView& pb, double E) {
@@ -40,7 +46,7 @@ NCD double pb_xs_sel(const PBView& pb, double E) {
 
 template <bool FP32PB>
 NCD double comp_xs(const CompView& c, double E, double* commul) {
-  if (!(E >= c.domain_lo && E <= c.domain_hi)) return 0.0;
+  if (comp_outside(c, E)) return 0.0;
   double tot = 0.0;
   for (int i = 0; i < c.n; ++i) {
     double xs = 0.0;
@@ -66,13 +72,13 @@ NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, doubl
   if constexpr (Variant == kCompPBOnly) {
     // a single PowderBragg component: no pick (NCProcImpl.cc:379-389 draws
     // nothing for one component)
-    if (!(E >= c.domain_lo && E <= c.domain_hi)) { Ef = E; mu = 1.0; return kOK; }
+    if (comp_outside(c, E)) { Ef = E; mu = 1.0; return kOK; }
     Ef = E;
     mu = pb_sample_mu(c.pb, rng, E);
     return kOK;
   }
   constexpr bool WithSAB = Variant == kCompGeneral;
-  if (!(E >= c.domain_lo && E <= c.domain_hi)) { Ef = E; mu = 1.0; return kOK; }
+  if (comp_outside(c, E)) { Ef = E; mu = 1.0; return kOK; }
   double commul[kMaxComp];
   const double tot = comp_xs<false>(c, E, commul);
   (void)tot;
@@ -94,7 +100,7 @@ NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, doubl
 // replays the pick's draw (one draw when there are several components).
 constexpr int kCompToSAB = 1;
 NCD int comp_pick_sample(const CompView& c, Rng& rng, double E, double& Ef, double& mu) {
-  if (!(E >= c.domain_lo && E <= c.domain_hi)) { Ef = E; mu = 1.0; return kOK; }
+  if (comp_outside(c, E)) { Ef = E; mu = 1.0; return kOK; }
   double commul[kMaxComp];
   comp_xs<false>(c, E, commul);
   const int ic = pick_idx(rng, commul, c.n);
@@ -129,7 +135,7 @@ void launch_fill_loguniform(double* out, int64_t n, double lemin, double lemax, 
 // S(alpha,beta)-only sampling: main kernel + high-energy queue kernel
 void launch_sab_sample_iso(const SABView&, const double* ekin, int64_t n, int64_t repeat, uint64_t seed,
                            uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, int64_t* queue,
-                           unsigned long long* qcount, hipStream_t);
+                           unsigned long long* qcount, int domain_check, hipStream_t);
 void launch_sab_guides(const SABView&, unsigned short* cdf_guide, unsigned short* row_guide, double* row_scale,
                        hipStream_t);
 

@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(256) k_comp_sab(SABView s, const double* __res
     const int64_t k = inq[q];
     const double E = ekin[(total == n) ? k : (k % n)];
     if constexpr (!HighE) {
-      if (E >= emax) {
+      if (!(E < emax)) {
         hq[atomicAdd(hqc, 1ull)] = k;
         continue;
       }
@@ -148,16 +148,19 @@ __global__ void __launch_bounds__(256) k_sab_sample_iso_main(SABView s, const do
                                                              double* __restrict__ oMu,
                                                              unsigned long long* __restrict__ nerr,
                                                              int64_t* __restrict__ queue,
-                                                             unsigned long long* __restrict__ qcount) {
+                                                             unsigned long long* __restrict__ qcount,
+                                                             int domain_check) {
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
   const double emax = s.egrid[s.ne - 1];
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gstride) {
     const int64_t i = (total == n) ? k : (k % n);
     const double E = ekin[i];
     double Ef, mu;
-    if (!(E >= 0.0 && E <= __builtin_huge_val())) {
+    if (domain_check && !(E >= 0.0 && E <= __builtin_huge_val())) {
+      // a composition outside its energy domain does not scatter
       Ef = E; mu = 1.0;
-    } else if (E >= emax) {
+    } else if (!(E < emax)) {
+      // the free-gas kernel, and the sampler's own handling of non-finite E
       const unsigned long long slot = atomicAdd(qcount, 1ull);
       queue[slot] = k;
       continue;
@@ -310,13 +313,13 @@ void launch_sab_ab(const SABView& s, const double* ekin, int64_t n, uint64_t see
 
 void launch_sab_sample_iso(const SABView& s, const double* ekin, int64_t n, int64_t repeat, uint64_t seed,
                            uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, int64_t* queue,
-                           unsigned long long* qcount, hipStream_t st) {
+                           unsigned long long* qcount, int domain_check, hipStream_t st) {
   const int64_t total = n * repeat;
   if (total <= 0) return;
   // kernel instead of a memset node, so that callers may capture this in a graph
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount);
   hipLaunchKernelGGL(k_sab_sample_iso_main, dim3(grid_for(total, 256)), dim3(256), 0, st, s, ekin, n, total, seed,
-                     stream, oE, oMu, nerr, queue, qcount);
+                     stream, oE, oMu, nerr, queue, qcount, domain_check);
   hipLaunchKernelGGL(k_sab_sample_iso_highe, dim3(grid_for(total, 256)), dim3(256), 0, st, s, ekin, n, total, seed,
                      stream, oE, oMu, nerr, queue, qcount);
 }
