@@ -137,3 +137,28 @@ def test_batch_empty_and_single_material(amd, oracle):
     batch.dev_crossSection(de.ptr, xs.ptr, stream=st.ptr)
     st.sync()
     assert np.array_equal(xs.numpy(), g.crossSectionIsotropic(e))
+
+
+def test_scbragg_special_inputs(amd, oracle):
+    """special energies and direction vectors that are not unit length (the
+    reference normalises them) against the checker"""
+    from test_gpu_scbragg import _fcc
+    C, core, _ = amd
+    g, o = _fcc(core, oracle)
+    rng = np.random.default_rng(12)
+    e = np.concatenate([SPECIAL, synth.energies_logu(3000, 1e-3, 1.0, seed=8)])
+    dirs = synth.isotropic_dirs(len(e), seed=9) * rng.uniform(0.5, 4.0, size=(len(e), 1))
+    assert _same(g.crossSection(e, dirs), o.xs_dir(e, dirs))
+    st = C.Stream()
+    de, dd = C.DeviceArray.from_numpy(e), C.DeviceArray.from_numpy(dirs.ravel())
+    oe, od = C.DeviceArray(len(e)), C.DeviceArray(3 * len(e))
+    g.setRandomSeed(77)
+    g.dev_sampleScatter(de.ptr, dd.ptr, len(e), oe.ptr, od.ptr, stream=st.ptr)
+    nerr_g = g.dev_check_errors(stream=st.ptr)
+    re_, rd, nerr_o = o.sample_dir(e, dirs, seed=77)
+    assert nerr_g == nerr_o
+    ge, gd = oe.numpy(), od.numpy().reshape(-1, 3)
+    n = len(SPECIAL)
+    assert _same(ge[:n], re_[:n]) and _same(gd[:n].ravel(), rd[:n].ravel())
+    frac = np.mean(np.max(np.abs(gd[n:] - rd[n:]), axis=1) > 1e-9)
+    assert frac < 5e-3
