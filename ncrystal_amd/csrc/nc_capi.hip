@@ -1697,6 +1697,17 @@ unsigned long long read_reset_err(Proc* p, hipStream_t st) {
 
 // isotropic sampling dispatch: S(alpha,beta)-only handles take the split
 // main/high-energy kernels, everything else the composition kernel
+// two queues of up to `total` neutrons (and their counters) for the staged
+// composition kernels
+void comp_queues(Proc* p, int64_t total, hipStream_t st) {
+  const size_t need = 2 * sizeof(int64_t) * (size_t)std::max<int64_t>(total, 1);
+  if (p->q_idx.bytes < need) {
+    NCAMD_HIP(hipStreamSynchronize(st));  // the old queue may still be in use on this stream
+    p->q_idx.alloc(need);
+  }
+  if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
+}
+
 void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat, double* oE, double* oMu,
                          hipStream_t st) {
   const uint32_t stream = next_stream(p);
@@ -1720,13 +1731,7 @@ void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat,
                             p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
                             p->q_count.as<unsigned long long>(), p != single ? 1 : 0, st);
   } else {
-    // two queues of up to n * repeat neutrons for the staged composition kernels
-    const size_t need = 2 * sizeof(int64_t) * (size_t)std::max<int64_t>(n * repeat, 1);
-    if (p->q_idx.bytes < need) {
-      NCAMD_HIP(hipStreamSynchronize(st));  // the old queue may still be in use on this stream
-      p->q_idx.alloc(need);
-    }
-    if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
+    comp_queues(p, n * repeat, st);
     launch_sample_iso(make_view(p), ekin, n, repeat, p->seed, stream, oE, oMu, p->d_err.as<unsigned long long>(),
                       p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), st);
   }
@@ -2077,8 +2082,10 @@ void ncamd_samplescatter_many_dirs(ncamd_process_t h, const double* ekin, const 
     launch_scb_sample(p->scb, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->seed, next_stream(p),
                       p->st_out1.as<double>(), p->st_out2.as<double>(), w, 0);
   } else {
+    comp_queues(p, (int64_t)n, 0);
     launch_sample_dirs(v, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->seed, next_stream(p),
-                       p->st_out1.as<double>(), p->st_out2.as<double>(), p->d_err.as<unsigned long long>(), 0);
+                       p->st_out1.as<double>(), p->st_out2.as<double>(), p->d_err.as<unsigned long long>(),
+                       p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), 0);
   }
   NCAMD_HIP(hipGetLastError());
   if (n) {
@@ -2168,8 +2175,10 @@ void ncamd_dev_samplescatter_many_dirs(ncamd_process_t h, const double* d_ekin, 
     launch_scb_sample(p->scb, d_ekin, d_dirs, (int64_t)n, p->seed, next_stream(p), d_oE, d_oD, w,
                       as_stream(stream));
   } else {
+    comp_queues(p, (int64_t)n, as_stream(stream));
     launch_sample_dirs(make_view(p), d_ekin, d_dirs, (int64_t)n, p->seed, next_stream(p), d_oE, d_oD,
-                       p->d_err.as<unsigned long long>(), as_stream(stream));
+                       p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
+                       p->q_count.as<unsigned long long>(), as_stream(stream));
   }
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH

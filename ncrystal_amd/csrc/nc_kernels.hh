@@ -63,32 +63,29 @@ NCD double comp_xs(const CompView& c, double E, double* commul) {
 }
 
 // Kernel variants by the kinds a composition holds, so that a kernel carries
-// only the samplers it can reach (the S(alpha,beta) sampler's registers
-// would otherwise limit the occupancy of PowderBragg / ElInc materials).
+// only the samplers it can reach: PowderBragg only; no S(alpha,beta) part
+// (one kernel); with an S(alpha,beta) part (the staged kernels below, since
+// the S(alpha,beta) sampler's registers would hold every other sampler at
+// one wave per SIMD).
 enum : int { kCompGeneral = 0, kCompNoSAB = 1, kCompPBOnly = 2 };
 
 template <int Variant>
 NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, double& mu) {
+  static_assert(Variant == kCompNoSAB || Variant == kCompPBOnly, "compositions with S(alpha,beta) are staged");
+  if (comp_outside(c, E)) { Ef = E; mu = 1.0; return kOK; }
   if constexpr (Variant == kCompPBOnly) {
     // a single PowderBragg component: no pick (NCProcImpl.cc:379-389 draws
     // nothing for one component)
-    if (comp_outside(c, E)) { Ef = E; mu = 1.0; return kOK; }
     Ef = E;
     mu = pb_sample_mu(c.pb, rng, E);
     return kOK;
   }
-  constexpr bool WithSAB = Variant == kCompGeneral;
-  if (comp_outside(c, E)) { Ef = E; mu = 1.0; return kOK; }
   double commul[kMaxComp];
-  const double tot = comp_xs<false>(c, E, commul);
-  (void)tot;
+  comp_xs<false>(c, E, commul);
   const int ic = pick_idx(rng, commul, c.n);
   switch (c.kind[ic]) {
     case kKindPB: Ef = E; mu = pb_sample_mu(c.pb, rng, E); return kOK;
     case kKindElInc: Ef = E; mu = elinc_sample_mu(c.ei, rng, E); return kOK;
-    case kKindSAB:
-      if constexpr (WithSAB) return sab_sample_iso(c.sab, rng, E, Ef, mu);
-      else return kErrBadSampler;
     default: return kErrBadSampler;
   }
 }
@@ -128,7 +125,8 @@ void launch_sample_iso(const CompView&, const double* ekin, int64_t n, int64_t r
                        uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, int64_t* queues,
                        unsigned long long* qcount, hipStream_t);
 void launch_sample_dirs(const CompView&, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
-                        uint32_t stream, double* oE, double* oD, unsigned long long* nerr, hipStream_t);
+                        uint32_t stream, double* oE, double* oD, unsigned long long* nerr, int64_t* queues,
+                        unsigned long long* qcount, hipStream_t);
 void launch_sab_ab(const SABView&, const double* ekin, int64_t n, uint64_t seed, uint32_t stream, double* oa,
                    double* ob, unsigned long long* nerr, hipStream_t);
 void launch_fill_loguniform(double* out, int64_t n, double lemin, double lemax, uint64_t seed, hipStream_t);

@@ -69,11 +69,30 @@ __global__ void __launch_bounds__(256) k_sample_dirs(CompView c, const double* _
 // k_comp_sab<true> (free-gas extender).  The S(alpha,beta) stages restart
 // the neutron's Philox stream and replay the pick's draw (`skip`), so every
 // neutron gets what one kernel doing all of it would give.
-__global__ void __launch_bounds__(256) k_comp_pick(CompView c, const double* __restrict__ ekin, int64_t n,
-                                                   int64_t total, uint64_t seed, uint32_t stream,
-                                                   double* __restrict__ oE, double* __restrict__ oMu,
-                                                   unsigned long long* __restrict__ nerr, int64_t* __restrict__ sq,
-                                                   unsigned long long* __restrict__ sqc) {
+// outputs of one neutron: (E', mu), or (E', direction) for the oriented call
+// (random azimuth about the incoming direction, NCProcImpl.cc:29-37)
+template <bool Dirs>
+NCD void comp_store(int64_t k, int rc, double Ef, double mu, Rng& rng, const double* __restrict__ dirs,
+                    double* __restrict__ oE, double* __restrict__ o2, unsigned long long* __restrict__ nerr) {
+  if constexpr (Dirs) {
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    if (rc) { Ef = -1.0; atomicAdd(nerr, 1ull); }
+    else rand_dir_given_mu(rng, mu, dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2], ox, oy, oz);
+    oE[k] = Ef;
+    o2[3 * k] = ox; o2[3 * k + 1] = oy; o2[3 * k + 2] = oz;
+  } else {
+    if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
+    oE[k] = Ef;
+    o2[k] = mu;
+  }
+}
+
+template <bool Dirs>
+__global__ void __launch_bounds__(256) k_comp_pick(CompView c, const double* __restrict__ ekin,
+                                                   const double* __restrict__ dirs, int64_t n, int64_t total,
+                                                   uint64_t seed, uint32_t stream, double* __restrict__ oE,
+                                                   double* __restrict__ o2, unsigned long long* __restrict__ nerr,
+                                                   int64_t* __restrict__ sq, unsigned long long* __restrict__ sqc) {
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gstride) {
     const int64_t i = (total == n) ? k : (k % n);
@@ -85,17 +104,15 @@ __global__ void __launch_bounds__(256) k_comp_pick(CompView c, const double* __r
       sq[atomicAdd(sqc, 1ull)] = k;
       continue;
     }
-    if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
-    oE[k] = Ef;
-    oMu[k] = mu;
+    comp_store<Dirs>(k, rc, Ef, mu, rng, dirs, oE, o2, nerr);
   }
 }
 
-template <bool HighE>
-__global__ void __launch_bounds__(256) k_comp_sab(SABView s, const double* __restrict__ ekin, int64_t n,
-                                                  int64_t total, uint64_t seed, uint32_t stream, int skip,
-                                                  double* __restrict__ oE, double* __restrict__ oMu,
-                                                  unsigned long long* __restrict__ nerr,
+template <bool HighE, bool Dirs>
+__global__ void __launch_bounds__(256) k_comp_sab(SABView s, const double* __restrict__ ekin,
+                                                  const double* __restrict__ dirs, int64_t n, int64_t total,
+                                                  uint64_t seed, uint32_t stream, int skip, double* __restrict__ oE,
+                                                  double* __restrict__ o2, unsigned long long* __restrict__ nerr,
                                                   const int64_t* __restrict__ inq,
                                                   const unsigned long long* __restrict__ inqc,
                                                   int64_t* __restrict__ hq, unsigned long long* __restrict__ hqc) {
@@ -116,9 +133,7 @@ __global__ void __launch_bounds__(256) k_comp_sab(SABView s, const double* __res
     if (skip) rng.gen();
     double Ef, mu;
     const int rc = sab_sample_iso<HighE>(s, rng, E, Ef, mu);
-    if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
-    oE[k] = Ef;
-    oMu[k] = mu;
+    comp_store<Dirs>(k, rc, Ef, mu, rng, dirs, oE, o2, nerr);
   }
 }
 
@@ -277,19 +292,22 @@ void launch_sample_iso(const CompView& c, const double* ekin, int64_t n, int64_t
       int64_t* hq = queues + total;
       hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount);
       hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount + 1);
-      hipLaunchKernelGGL(k_comp_pick, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin, n, total, seed, stream,
-                         oE, oMu, nerr, sq, qcount);
+      hipLaunchKernelGGL(k_comp_pick<false>, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin,
+                         (const double*)nullptr, n, total, seed, stream, oE, oMu, nerr, sq, qcount);
       if (sab < 0) break;  // no S(alpha,beta) part: nothing queued
       const int skip = c.n > 1 ? 1 : 0;
-      hipLaunchKernelGGL(k_comp_sab<false>, dim3(grid_for(total, 256)), dim3(256), 0, st, c.sab, ekin, n, total,
-                         seed, stream, skip, oE, oMu, nerr, sq, qcount, hq, qcount + 1);
-      hipLaunchKernelGGL(k_comp_sab<true>, dim3(grid_for(total, 256)), dim3(256), 0, st, c.sab, ekin, n, total,
-                         seed, stream, skip, oE, oMu, nerr, hq, qcount + 1, nullptr, nullptr);
+      hipLaunchKernelGGL((k_comp_sab<false, false>), dim3(grid_for(total, 256)), dim3(256), 0, st, c.sab, ekin,
+                         (const double*)nullptr, n, total, seed, stream, skip, oE, oMu, nerr, sq, qcount, hq,
+                         qcount + 1);
+      hipLaunchKernelGGL((k_comp_sab<true, false>), dim3(grid_for(total, 256)), dim3(256), 0, st, c.sab, ekin,
+                         (const double*)nullptr, n, total, seed, stream, skip, oE, oMu, nerr, hq, qcount + 1,
+                         nullptr, nullptr);
     }
   }
 }
 void launch_sample_dirs(const CompView& c, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
-                        uint32_t stream, double* oE, double* oD, unsigned long long* nerr, hipStream_t st) {
+                        uint32_t stream, double* oE, double* oD, unsigned long long* nerr, int64_t* queues,
+                        unsigned long long* qcount, hipStream_t st) {
   if (n <= 0) return;
   switch (comp_variant(c)) {
     case kCompPBOnly:
@@ -300,9 +318,22 @@ void launch_sample_dirs(const CompView& c, const double* ekin, const double* dir
       hipLaunchKernelGGL(k_sample_dirs<kCompNoSAB>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, dirs, n,
                          seed, stream, oE, oD, nerr);
       break;
-    default:
-      hipLaunchKernelGGL(k_sample_dirs<kCompGeneral>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, dirs, n,
-                         seed, stream, oE, oD, nerr);
+    default: {
+      int64_t* sq = queues;
+      int64_t* hq = queues + n;
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount);
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount + 1);
+      hipLaunchKernelGGL(k_comp_pick<true>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, dirs, n, n, seed,
+                         stream, oE, oD, nerr, sq, qcount);
+      bool has_sab = false;
+      for (int j = 0; j < c.n; ++j) has_sab = has_sab || c.kind[j] == kKindSAB;
+      if (!has_sab) break;
+      const int skip = c.n > 1 ? 1 : 0;
+      hipLaunchKernelGGL((k_comp_sab<false, true>), dim3(grid_for(n, 256)), dim3(256), 0, st, c.sab, ekin, dirs, n,
+                         n, seed, stream, skip, oE, oD, nerr, sq, qcount, hq, qcount + 1);
+      hipLaunchKernelGGL((k_comp_sab<true, true>), dim3(grid_for(n, 256)), dim3(256), 0, st, c.sab, ekin, dirs, n,
+                         n, seed, stream, skip, oE, oD, nerr, hq, qcount + 1, nullptr, nullptr);
+    }
   }
 }
 void launch_sab_ab(const SABView& s, const double* ekin, int64_t n, uint64_t seed, uint32_t stream, double* oa,
