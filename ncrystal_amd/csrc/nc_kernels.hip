@@ -9,12 +9,19 @@
 
 namespace ncamd {
 
-template <bool FP32PB>
+// PBOnly: a single PowderBragg component (the same sum as comp_xs: 0 + scale * xs)
+template <bool FP32PB, bool PBOnly>
 __global__ void __launch_bounds__(256) k_xs_iso(CompView c, const double* __restrict__ ekin,
                                                 int64_t n, int64_t repeat, double* __restrict__ out) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const double xs = comp_xs<FP32PB>(c, ekin[i], nullptr);
+    const double E = ekin[i];
+    double xs;
+    if constexpr (PBOnly) {
+      xs = comp_outside(c, E) ? 0.0 : 0.0 + c.scale[0] * (pb_active(c.pb, E) ? pb_xs_sel<FP32PB>(c.pb, E) : 0.0);
+    } else {
+      xs = comp_xs<FP32PB>(c, E, nullptr);
+    }
     for (int64_t r = 0; r < repeat; ++r) out[r * n + i] = xs;
   }
 }
@@ -267,8 +274,14 @@ static inline unsigned grid_for(int64_t n, int bs) {
 void launch_xs_iso(const CompView& c, const double* ekin, int64_t n, int64_t repeat, double* out,
                    bool fp32, hipStream_t st) {
   if (n <= 0 || repeat <= 0) return;
-  if (fp32) hipLaunchKernelGGL(k_xs_iso<true>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, n, repeat, out);
-  else hipLaunchKernelGGL(k_xs_iso<false>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, n, repeat, out);
+  const dim3 g(grid_for(n, 256)), b(256);
+  if (comp_variant(c) == kCompPBOnly) {
+    if (fp32) hipLaunchKernelGGL((k_xs_iso<true, true>), g, b, 0, st, c, ekin, n, repeat, out);
+    else hipLaunchKernelGGL((k_xs_iso<false, true>), g, b, 0, st, c, ekin, n, repeat, out);
+  } else {
+    if (fp32) hipLaunchKernelGGL((k_xs_iso<true, false>), g, b, 0, st, c, ekin, n, repeat, out);
+    else hipLaunchKernelGGL((k_xs_iso<false, false>), g, b, 0, st, c, ekin, n, repeat, out);
+  }
 }
 void launch_sample_iso(const CompView& c, const double* ekin, int64_t n, int64_t repeat, uint64_t seed,
                        uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, int64_t* queues,
