@@ -222,8 +222,11 @@ def test_learned_sampler_distribution_and_gain(env, oracle, trained):
     print("envelope mass uniform grid / learned grid: median %.3f min %.3f max %.3f; uniform blend weights %s; "
           "squeeze fraction median %.3f" % (np.median(gain), gain.min(), gain.max(),
                                            np.unique(lam, return_counts=True), np.median(sq / mass)))
-    # the learned grid is never worse than the uniform grid it is chosen against
+    # the learned grid is never worse than the uniform grid it is chosen
+    # against, and it gains over the method without the network (measured
+    # median 1.065 on 128 x 32 boxes)
     assert np.all(mass <= uni)
+    assert np.median(gain) > 1.03
     eff = _check_bounds(g, d, a, b, S, (0, 100, 255), nsample=100)
     print("learned grid acceptance / squeeze fraction", eff)
     o = oracle.sab(*args)
