@@ -153,8 +153,10 @@ int ncamd_scbragg_tables(ncamd_process_t, double* inv2d, double* xsfact, int64_t
  * Switches an S(alpha,beta) process to rejection sampling in kinematic
  * coordinates (w = sqrt(beta+eps)/wmax, kappa = position between the
  * kinematic alpha limits) with a box envelope per energy interval:
- * nintervals log-spaced intervals over [emin, emax) (0 = the process's
- * energy grid range), nbox_w strips in w with nbox_k boxes each, at most
+ * nintervals log-spaced intervals over [emin, emax), clipped to the
+ * process's energy grid (0 = its whole range; outside the grid the reference
+ * does not sample the table density itself, so the plain sampler is used
+ * there), nbox_w strips in w with nbox_k boxes each, at most
  * maxtries proposals per neutron.  Every box carries exact upper and lower
  * bounds of the target; proposals below the lower bound are accepted
  * without evaluating the table (squeeze), with the same decisions as the

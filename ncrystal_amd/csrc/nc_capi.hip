@@ -1404,8 +1404,11 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
               const std::vector<KBMix>* mix) {
   if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
   const SABView& s = p->sab;
-  if (emax <= 0.0) emax = p->sab_egrid_h.back();
-  if (emin <= 0.0) emin = p->sab_egrid_h.front();
+  // only inside the table's energy grid does the reference sample the table
+  // density itself; above it mixes in the free-gas extender, below it
+  // applies its low-energy rule, and there the plain sampler takes over
+  if (emax <= 0.0 || emax > p->sab_egrid_h.back()) emax = p->sab_egrid_h.back();
+  if (emin <= 0.0 || emin < p->sab_egrid_h.front()) emin = p->sab_egrid_h.front();
   if (!(emin > 0.0 && emax > emin)) raise("BadInput", "envelope sampler: need 0 < emin < emax");
   if (nn < 1 || nn > 1 << 16 || bw < 1 || bk < 1 || (int64_t)bw * bk > 1 << 16 || maxtries < 1)
     raise("BadInput", "envelope sampler: invalid grid sizes");
@@ -1662,8 +1665,9 @@ Mlp* create_mlp(uint64_t seed, double emin, double emax) {
 // interval's upper energy gives the quantiles at which the box edges go.
 void build_learned(Proc* p, Mlp* m, int nn, double emin, double emax, int bw, int bk, int maxtries) {
   if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
-  if (emax <= 0.0) emax = p->sab_egrid_h.back();
-  if (emin <= 0.0) emin = p->sab_egrid_h.front();
+  // the same energy range as the envelope sampler: inside the table's grid
+  if (emax <= 0.0 || emax > p->sab_egrid_h.back()) emax = p->sab_egrid_h.back();
+  if (emin <= 0.0 || emin < p->sab_egrid_h.front()) emin = p->sab_egrid_h.front();
   if (!(emin > 0.0 && emax > emin)) raise("BadInput", "learned sampler: need 0 < emin < emax");
   if (nn < 1 || nn > 1 << 16) raise("BadInput", "learned sampler: invalid number of intervals");
   const double lr = std::log(emax / emin) / nn;

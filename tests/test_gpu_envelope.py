@@ -280,3 +280,26 @@ def test_learned_sampler_iso_fallback_identity(env, trained):
     inr = ~out
     assert stats.ks_2samp(pm[inr], lm[inr]).statistic < 0.01
     assert stats.ks_2samp(pe[inr], le[inr]).statistic < 0.01
+
+
+def test_envelope_range_clipped_to_table_grid(env):
+    """outside the table's energy grid the reference does not sample the
+    table density (free-gas extender above, low-energy rule below): the
+    envelope sampler's range is clipped to the grid and those neutrons get
+    exactly the plain sampler's values"""
+    C, core = env
+    d, a, b, S = _table()
+    args = (d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"], d["sigma_bound"])
+    plain = core.create_sabscatter(*args)
+    kbs = core.create_sabscatter(*args)
+    kbs.use_envelope_sampler(nintervals=64, emin=1e-9, emax=100.0)
+    eg, _ = kbs.egrid()
+    node, _ = kbs.envelope_tables()
+    assert node[0] == eg[0] and node[-1] == eg[-1]
+    e = np.concatenate([synth.energies_logu(100000, 1e-7, eg[0], seed=3),
+                        synth.energies_logu(100000, eg[-1], 20.0, seed=4)])
+    plain.setRandomSeed(5)
+    kbs.setRandomSeed(5)
+    pe, pm = plain.sampleScatterIsotropic(e)
+    ke, km = kbs.sampleScatterIsotropic(e)
+    assert np.array_equal(pe, ke) and np.array_equal(pm, km)
