@@ -211,7 +211,7 @@ struct Proc {
   // Romberg queue of SCBragg)
   DevBuf q_idx, q_count;
   // SCBragg batch ordering
-  DevBuf scb_keys, scb_perm, scb_hist, scb_cursor;
+  DevBuf scb_keys, scb_perm, scb_hist, scb_cursor, scb_cnt, scb_tot, scb_sel;
   // component data
   PBView pb{};
   ElIncView ei{};
@@ -1121,14 +1121,16 @@ Proc* create_scb(double v0n, int64_t nfam, const double* dsp, const double* fsq,
   // per demi-normal scan records (normal, rounded 1/(2d) of its family) and family index
   std::vector<double> nrec(4 * (size_t)(nplanes + kScbPad), 0.0);
   std::vector<float> nrec32(4 * (size_t)(nplanes + kScbPad), 0.0f);
-  std::vector<int> famidx((size_t)nplanes);
+  // (family cross-section factor, family index) per demi-normal
+  std::vector<double> nfx(2 * (size_t)(nplanes + kScbPad), 0.0);
   for (int f = 0; f < nf; ++f)
     for (int64_t j = off[f]; j < off[f + 1]; ++j) {
       nrec[4 * j] = normals[3 * j];
       nrec[4 * j + 1] = normals[3 * j + 1];
       nrec[4 * j + 2] = normals[3 * j + 2];
       nrec[4 * j + 3] = inv2d_r[f];
-      famidx[j] = f;
+      nfx[2 * j] = xsf[f];
+      nfx[2 * j + 1] = (double)f;
     }
   for (size_t j = 0; j < nrec.size(); ++j) nrec32[j] = (float)nrec[j];
   for (int64_t j = nplanes; j < nplanes + kScbPad; ++j) {  // padding records never pass the pre-filter
@@ -1143,7 +1145,7 @@ Proc* create_scb(double v0n, int64_t nfam, const double* dsp, const double* fsq,
   DevBuf* b_off = p->newbuf(); b_off->upload(off32.data(), off32.size());
   DevBuf* b_nrec = p->newbuf(); b_nrec->upload(nrec.data(), nrec.size());
   DevBuf* b_nrec32 = p->newbuf(); b_nrec32->upload(nrec32.data(), nrec32.size());
-  DevBuf* b_fam = p->newbuf(); b_fam->upload(famidx.data(), famidx.size());
+  DevBuf* b_nfx = p->newbuf(); b_nfx->upload(nfx.data(), nfx.size());
   DevBuf* b_lt1 = p->newbuf(); b_lt1->upload(lt1.data(), lt1.size());
   DevBuf* b_lt2 = p->newbuf(); b_lt2->upload(lt2.data(), lt2.size());
   SCBView& v = p->scb;
@@ -1154,7 +1156,7 @@ Proc* create_scb(double v0n, int64_t nfam, const double* dsp, const double* fsq,
   v.off = b_off->as<int>();
   v.nrec = b_nrec->as<double4>();
   v.nrec32 = b_nrec32->as<float4>();
-  v.famidx = b_fam->as<int>();
+  v.nfx = b_nfx->as<double2>();
   // |sin(phi) - sin(theta)| <= |phi - theta| < truncation angle for every
   // contributing normal; the margins cover the rounding of cos(truncation)
   // (~1e-14) and of the FP32 records and products of the pre-filter (< 3e-6)
@@ -1745,11 +1747,15 @@ void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat,
 // for n neutrons
 ScbWork scb_work(Proc* p, int64_t n, hipStream_t st) {
   const size_t m = (size_t)std::max<int64_t>(n, 1);
-  if (p->q_idx.bytes < sizeof(int64_t) * m || p->scb_keys.bytes < sizeof(unsigned) * m) {
+  if (p->q_idx.bytes < sizeof(int64_t) * m || p->scb_keys.bytes < sizeof(unsigned) * m ||
+      p->scb_sel.bytes < sizeof(double4) * m) {
     NCAMD_HIP(hipStreamSynchronize(st));  // the old buffers may still be in use on this stream
     p->q_idx.ensure(sizeof(int64_t) * m);
     p->scb_perm.ensure(sizeof(int64_t) * m);
     p->scb_keys.ensure(sizeof(unsigned) * m);
+    p->scb_cnt.ensure(sizeof(int) * m);
+    p->scb_tot.ensure(sizeof(double) * m);
+    p->scb_sel.ensure(sizeof(double4) * m);
   }
   if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
   if (!p->scb_hist.p) {
@@ -1764,6 +1770,9 @@ ScbWork scb_work(Proc* p, int64_t n, hipStream_t st) {
   w.cursor = p->scb_cursor.as<unsigned long long>();
   w.queue = p->q_idx.as<int64_t>();
   w.qcount = p->q_count.as<unsigned long long>();
+  w.cnt = p->scb_cnt.as<int>();
+  w.tot = p->scb_tot.as<double>();
+  w.sel = p->scb_sel.as<double4>();
   return w;
 }
 

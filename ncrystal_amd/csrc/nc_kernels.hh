@@ -204,6 +204,9 @@ struct ScbWork {
   unsigned long long* cursor;  // [kScbBuckets]
   int64_t* queue;              // [n] neutrons for the Romberg kernels
   unsigned long long* qcount;  // [1]
+  int* cnt;                    // [n] sampling: contributions at each sorted position
+  double* tot;                 // [n] sampling: their cumulative total
+  double4* sel;                // [n] sampling: selected normal and rounded 1/(2d)
 };
 void launch_scb_xs(const SCBView&, const double* ekin, const double* dirs, int64_t n, double* out,
                    const ScbWork&, hipStream_t);

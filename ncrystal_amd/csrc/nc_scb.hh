@@ -48,7 +48,7 @@ struct SCBView {
   const double4* __restrict__ nrec;     // [nnormals + kScbPad]
   // the same records in FP32 for the pre-filter of the scan
   const float4* __restrict__ nrec32;    // [nnormals + kScbPad]
-  const int* __restrict__ famidx;       // [nnormals] family of each demi-normal
+  const double2* __restrict__ nfx;      // [nnormals + kScbPad] (family xsf, family index) of each demi-normal
   double threshold;                     // ekin below which nothing reflects
   double cta, k1, k2, numacc;           // GaussOnSphere: cos(truncation), fast-path bounds, Romberg accuracy
   float band32;                         // pre-filter half-width: truncation angle + rounding margins
@@ -352,6 +352,100 @@ NCD int wave_max_i(int x) {
   }
   return __builtin_amdgcn_readfirstlane(x);
 }
+NCD double wave_sum_d(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+NCD double wave_min_d(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = dmin(x, __shfl_xor(x, o, 64));
+  return x;
+}
+NCD float wave_min_f(float x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fminf(x, __shfl_xor(x, o, 64));
+  return x;
+}
+NCD float wave_max_f(float x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, 64));
+  return x;
+}
+NCD float readlane_f(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
+
+
+// Cross sections of the anti-normal side (xs0) and normal side (xs1) of
+// demi-normal record r (family cross-section factor xsf) for a neutron with
+// rounded wavelength w (wl3 = w^3) and unit direction u; 0 where a side does
+// not reflect (NCSCBragg.cc:263-300 with GaussMos::calcCrossSections).  The
+// reference evaluates Q and cos(theta) lazily once per family; both depend
+// on the family alone (sin(theta) = w / 2d), so evaluating them per
+// demi-normal gives the same values.  Returns false, without AllowSlow, if a
+// side needs the Romberg circle integral.
+//
+// A demi-normal usually reflects on one side only, and lanes of a wave
+// differ in which: the sides that pass the cone test are evaluated by one
+// loop, so that a wave runs the circle integral once per candidate rather
+// than once per side.
+template <bool AllowSlow>
+NCD bool scb_candidate(const SCBView& v, const double4& r, double xsf, double w, double wl3, double ux, double uy,
+                       double uz, double& xs0, double& xs1) {
+  xs0 = 0.0;
+  xs1 = 0.0;
+  const double cta = v.cta;
+  const double dot = r.x * ux + r.y * uy + r.z * uz;
+  const double sinpt = w * r.w;
+  const double cptsq = 1 - sinpt * sinpt;
+  const double sdotcptsq = (1.0 - dot * dot) * cptsq;
+  const double ds = dot * sinpt;
+  const double A0 = dmax(0.0, cta - fabs(ds));
+  if (sdotcptsq <= A0 * A0) return true;
+  const double A1 = dmax(0.0, cta - ds), A2 = dmax(0.0, cta + ds);
+  unsigned sides = (sdotcptsq > A1 * A1 ? 1u : 0u) | (sdotcptsq > A2 * A2 ? 2u : 0u);
+  if (!sides) return true;
+  const double cospt = sqrt(cptsq);
+  const double tmp2 = cospt * sinpt;
+  const double Qprime = tmp2 > 0 ? wl3 / tmp2 : ((sinpt > 0.5 && xsf) ? -2.0 : 0.0);
+  const double Q = Qprime > 0. ? Qprime * xsf : -1.0;
+  if (!(Q > 0.)) {
+    const double xs = Qprime ? __builtin_huge_val() : 0.0;
+    if (sides & 1u) xs0 = xs;
+    if (sides & 2u) xs1 = xs;
+    return true;
+  }
+#pragma nounroll
+  while (sides) {
+    const int side = __builtin_ctz(sides);
+    sides &= sides - 1u;
+    const double cc = dclamp(side == 0 ? dot : -dot, -1.0, 1.0);
+    const double sc2 = sqrt(1.0 - cc * cc);
+    double ci;
+    if (!gos_circle_fast_ok(v, cc, sc2, sinpt, cospt, ci)) {
+      if constexpr (AllowSlow) ci = gos_circle_slow(v, cc, sc2, sinpt, cospt);
+      else return false;
+    }
+    if (side == 0) xs0 = Q * ci;
+    else xs1 = Q * ci;
+  }
+  return true;
+}
+
+// Candidate lists: a wave keeps, per lane, the demi-normals that passed the
+// pre-filter in LDS (entry t of lane l at list[t * 64 + l], as an offset
+// from the list's base index) and evaluates them afterwards in lock step,
+// lane by lane in scan order.  Evaluating each candidate as the scan meets
+// it would run the evaluation once per group in which any lane of the wave
+// has a candidate, i.e. for nearly every group.
+constexpr int kScbListCap = 64;                      // entries per lane
+constexpr int kScbListWave = kScbListCap * 64;       // uint16 entries per wave
+constexpr int kScbListSpan = 65536;                  // index range of one list
+// accumulation of the contributions in scan order (GaussMos::calcCrossSections
+// cumulative per family on top of the previous families' total)
+struct ScbAcc {
+  int cur_f;
+  double commul_last, xsoffset, xssum;
+};
 
 // One lane's state during a scan of the reflection families.
 struct ScbLane {
@@ -367,126 +461,199 @@ struct ScbLane {
   double sel_inv2d_r;
 };
 
+// Evaluates the first n entries of the lane's list (indices base + entry) in
+// order and accumulates them; all lanes of the wave call this together.
+// Select=false: count and cumulative total.  Select=true: stop at the
+// contribution picked by pickRandIdxByWeight for (n_select, target).
+template <bool Select, bool AllowSlow>
+NCD void scb_eval_list(const SCBView& v, ScbLane& L, ScbAcc& a, bool& alive, const uint16_t* __restrict__ list,
+                       int base, int n, double w, double wl3, int n_select, double target) {
+  const int lane = (int)__lane_id();
+  const int tmax = wave_max_i(alive ? n : 0);
+  for (int t = 0; t < tmax; ++t) {
+    if (!(alive && t < n)) continue;
+    const int jj = base + (int)list[t * 64 + lane];
+    const double4 r = v.nrec[jj];
+    const double2 fx = v.nfx[jj];
+    const int f = (int)fx.y;
+    if (f != a.cur_f) {
+      a.cur_f = f;
+      a.xsoffset = a.commul_last;
+      a.xssum = 0.0;
+    }
+    double xs2[2];
+    if (!scb_candidate<AllowSlow>(v, r, fx.x, w, wl3, L.u.x, L.u.y, L.u.z, xs2[0], xs2[1])) {
+      L.slow = true;
+      alive = false;
+      continue;
+    }
+    for (int side = 0; side < 2; ++side) {
+      const double xs = xs2[side];
+      if (!xs) continue;
+      const double cm = a.xsoffset + (a.xssum += xs);
+      a.commul_last = cm;
+      if constexpr (Select) {
+        const bool hit = n_select < 5 ? (cm > target) : !(cm < target);
+        if (hit || L.count == n_select - 1) {
+          L.found = true;
+          L.sel_inv2d_r = r.w;
+          L.sel_normal = side == 0 ? V3{-r.x, -r.y, -r.z} : V3{r.x, r.y, r.z};
+          alive = false;
+          break;
+        }
+      }
+      L.count++;
+    }
+  }
+}
+
+// FP32 pre-filter of demi-normal record r (FP32 normal, rounded 1/(2d)) for
+// a lane with FP32 direction f and wavelength fw: contracted FP32 arithmetic,
+// the band covers its rounding
+NCD bool scb_prefilter(float rx, float ry, float rz, float rw, float fx, float fy, float fz, float fw,
+                       float band32) {
+  const float d32 = __fmaf_rn(rz, fz, __fmaf_rn(ry, fy, rx * fx));
+  const float s32 = fw * rw;
+  return fabsf(fabsf(d32) - s32) < band32;
+}
+
+// wave-level filter applies while the directions of the wave's lanes lie
+// within this chord of their mean direction (beyond it too few demi-normals
+// are ruled out to pay for the second test)
+constexpr double kScbCoarseChord = 0.2;
+
 // Scan of all (family, demi-normal, side) contributions in the reference's
 // order (families by decreasing d, normals in family order, anti-normal side
-// first).  Select=false: count and cumulative total.  Select=true: find the
-// contribution picked by pickRandIdxByWeight for (n_select, target).
+// first), NCSCBragg.cc:263-300.  Select as for scb_eval_list.
 //
-// The demi-normals are scanned as one flat list in groups of kScbGroup with a
-// wave-uniform index (scalar loads of FP32 records).  A normal can only
-// contribute if the angle phi between the neutron direction and its plane is
-// within the truncation angle of the Bragg angle theta; since
-// |sin(phi) - sin(theta)| <= |phi - theta|, the FP32 test
+// A normal can only contribute if the angle phi between the neutron
+// direction and its plane is within the truncation angle of the Bragg angle
+// theta; since |sin(phi) - sin(theta)| <= |phi - theta|, the FP32 test
 // | |n.u| - sin(theta) | < band32 (band32 = truncation angle plus margins
 // for FP32 rounding, about 1e-5) is a superset of the reference's cone test.
-// Candidates are then processed in order with the FP64 record and the
-// reference's test and arithmetic unchanged.  Per-family state (cumulative
-// offset, lazily computed Q) is (re)initialised at the first candidate of
-// each family, which leaves every value as the reference's family loop
-// computes it.  All lanes of the wavefront must call this together.
+// Candidates go to the lane's list and are evaluated with the FP64 record
+// and the reference's test and arithmetic unchanged whenever a list could
+// overflow and at the end.  Per-family state (cumulative offset) is
+// (re)initialised at the first candidate of each family, which leaves every
+// value as the reference's family loop computes it.
+//
+// Two scans give the same lists:
+//  - coarse: when the wave's directions lie within a chord rho of their mean
+//    m, lane t first tests demi-normal jb + t against the whole wave:
+//    |n.u - n.m| <= |u - m| <= rho for every lane, and sin(theta) lies
+//    between fw_lo/(2d) and fw_hi/(2d) (FP32 products are monotone in fw),
+//    so a normal with | |n.m| - s | >= rho + band32 for all s in that range
+//    (plus 1e-5 for the FP32 dot products) fails every lane's pre-filter.
+//    The normals left are broadcast (readlane) and pre-filtered per lane.
+//  - dense: every lane pre-filters every normal, in groups of kScbGroup with
+//    a wave-uniform index (scalar loads of the FP32 records).
+// `complete` is set (wave-uniform) when the list held every candidate of the
+// scan, so that a second pass can evaluate it without scanning again.  All
+// lanes of the wavefront must call this together.
 template <bool Select, bool AllowSlow>
-NCD void scb_walk(const SCBView& v, ScbLane& L, int n_select, double target) {
+NCD void scb_walk(const SCBView& v, ScbLane& L, int n_select, double target, uint16_t* __restrict__ list,
+                  int& nlist, bool& complete) {
   L.count = 0;
   L.total = 0.0;
   L.slow = false;
   L.found = false;
   bool alive = L.jlim > 0;
   const int jmax = wave_max_i(L.jlim);
-  const double cta = v.cta;
+  const int lane = (int)__lane_id();
   const double w = gm_round(L.wl);
   const double wl3 = w * w * w;
-  const double ux = L.u.x, uy = L.u.y, uz = L.u.z;
-  const float fx = (float)ux, fy = (float)uy, fz = (float)uz, fw = (float)w;
+  const float fx = (float)L.u.x, fy = (float)L.u.y, fz = (float)L.u.z, fw = (float)w;
   const float band32 = v.band32;
-  double commul_last = 0.0;
-  int cur_f = -1;
-  double xsf = 0.0, Q = -1.0, Qprime = -1.0, cospt = -1.0, xsoffset = 0.0, xssum = 0.0;
-  for (int jb = 0; jb < jmax; jb += kScbGroup) {
-    unsigned cand = 0u;
+  ScbAcc a{-1, 0.0, 0.0, 0.0};
+  int n = 0, base = 0;
+  complete = true;
+  // bounds of the wave: mean direction, chord, wavelength range
+  const double mx0 = wave_sum_d(alive ? L.u.x : 0.0), my0 = wave_sum_d(alive ? L.u.y : 0.0),
+               mz0 = wave_sum_d(alive ? L.u.z : 0.0);
+  const double mn = sqrt(mx0 * mx0 + my0 * my0 + mz0 * mz0);
+  const double mx = mx0 / mn, my = my0 / mn, mz = mz0 / mn;
+  const double cmin = wave_min_d(alive ? L.u.x * mx + L.u.y * my + L.u.z * mz : 1.0);
+  const double rho = sqrt(dmax(0.0, 2.0 - 2.0 * cmin));
+  // (a NaN in the bounds, e.g. from directions that cancel, selects the dense scan)
+  if (jmax > 0 && mn > 0.0 && rho < kScbCoarseChord) {
+    const float R = (float)rho + band32 + 1e-5f;
+    const float gx = (float)mx, gy = (float)my, gz = (float)mz;
+    const float fwlo = wave_min_f(alive ? fw : __builtin_huge_valf());
+    const float fwhi = wave_max_f(alive ? fw : -__builtin_huge_valf());
+    for (int jb = 0; jb < jmax; jb += 64) {
+      const int jl = jb + lane;
+      float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+      bool pass = false;
+      if (jl < jmax) {
+        r = v.nrec32[jl];
+        const float D = fabsf(__fmaf_rn(r.z, gz, __fmaf_rn(r.y, gy, r.x * gx)));
+        pass = (fwlo * r.w < D + R) && (fwhi * r.w > D - R);
+      }
+      uint64_t M = __ballot(pass);
+      if (!M) continue;
+      if (__any(n + __popcll(M) > kScbListCap) || jb + 64 - base > kScbListSpan) {
+        scb_eval_list<Select, AllowSlow>(v, L, a, alive, list, base, n, w, wl3, n_select, target);
+        n = 0;
+        base = jb;
+        complete = false;
+        if (!__any(alive)) break;
+      }
+      while (M) {
+        const int t = __builtin_ctzll(M);
+        M &= M - 1ull;
+        const float rx = readlane_f(r.x, t), ry = readlane_f(r.y, t), rz = readlane_f(r.z, t),
+                    rw = readlane_f(r.w, t);
+        const int j = jb + t;
+        if (alive && j < L.jlim && scb_prefilter(rx, ry, rz, rw, fx, fy, fz, fw, band32)) {
+          list[n * 64 + lane] = (uint16_t)(j - base);
+          ++n;
+        }
+      }
+    }
+  } else {
+    for (int jb = 0; jb < jmax; jb += kScbGroup) {
+      if (__any(n > kScbListCap - kScbGroup) || jb + kScbGroup - base > kScbListSpan) {
+        scb_eval_list<Select, AllowSlow>(v, L, a, alive, list, base, n, w, wl3, n_select, target);
+        n = 0;
+        base = jb;
+        complete = false;
+        if (!__any(alive)) break;
+      }
+      unsigned cand = 0u;
 #pragma unroll
-    for (int q = 0; q < kScbGroup; ++q) {
-      const float4 r = v.nrec32[jb + q];
-      // contracted FP32 arithmetic: the band covers its rounding
-      const float d32 = __fmaf_rn(r.z, fz, __fmaf_rn(r.y, fy, r.x * fx));
-      const float s32 = fw * r.w;
-      cand |= (fabsf(fabsf(d32) - s32) < band32 ? 1u : 0u) << q;
-    }
-    const int left = L.jlim - jb;
-    if (left < kScbGroup) cand &= left > 0 ? (1u << left) - 1u : 0u;
-    if (!alive) cand = 0u;
-    if (!__any(cand != 0u)) continue;
-    while (cand) {
-      const int q = __builtin_ctz(cand);
-      cand &= cand - 1u;
-      const int j = jb + q;
-      const double4 r = v.nrec[j];
-      const double dot = r.x * ux + r.y * uy + r.z * uz;
-      const double sinpt = w * r.w;
-      const int f = v.famidx[j];
-      if (f != cur_f) {
-        cur_f = f;
-        xsf = v.xsf[f];
-        Q = -1.0; Qprime = -1.0; cospt = -1.0;
-        xsoffset = commul_last;
-        xssum = 0.0;
+      for (int q = 0; q < kScbGroup; ++q) {
+        const float4 r = v.nrec32[jb + q];
+        cand |= (scb_prefilter(r.x, r.y, r.z, r.w, fx, fy, fz, fw, band32) ? 1u : 0u) << q;
       }
-      const double cptsq = 1 - sinpt * sinpt;
-      const double sdotcptsq = (1.0 - dot * dot) * cptsq;
-      const double ds = dot * sinpt;
-      const double A0 = dmax(0.0, cta - fabs(ds));
-      if (sdotcptsq <= A0 * A0) continue;
-      for (int side = 0; side < 2; ++side) {
-        const double Aside = side == 0 ? dmax(0.0, cta - ds) : dmax(0.0, cta + ds);
-        if (!(sdotcptsq > Aside * Aside)) continue;
-        const double cc = dclamp(side == 0 ? dot : -dot, -1.0, 1.0);
-        if (!(Q > 0.)) {
-          if (Qprime == -1) {
-            cospt = sqrt(cptsq);
-            const double tmp2 = cospt * sinpt;
-            if (tmp2 > 0) Qprime = wl3 / tmp2;
-            else Qprime = (sinpt > 0.5 && xsf) ? -2.0 : 0.0;
-          }
-          if (Qprime > 0.) Q = Qprime * xsf;
-        }
-        double xs;
-        if (Q > 0.) {
-          const double sc2 = sqrt(1.0 - cc * cc);
-          double ci;
-          if (!gos_circle_fast_ok(v, cc, sc2, sinpt, cospt, ci)) {
-            if constexpr (AllowSlow) {
-              ci = gos_circle_slow(v, cc, sc2, sinpt, cospt);
-            } else {
-              L.slow = true;
-              alive = false;
-              break;
-            }
-          }
-          xs = Q * ci;
-        } else {
-          xs = Qprime ? __builtin_huge_val() : 0.0;
-        }
-        if (xs) {
-          const double cm = xsoffset + (xssum += xs);
-          commul_last = cm;
-          if constexpr (Select) {
-            const bool hit = n_select < 5 ? (cm > target) : !(cm < target);
-            if (hit || L.count == n_select - 1) {
-              L.found = true;
-              L.sel_inv2d_r = r.w;
-              const V3 nrm{r.x, r.y, r.z};
-              L.sel_normal = side == 0 ? V3{-nrm.x, -nrm.y, -nrm.z} : nrm;
-              alive = false;
-              break;
-            }
-          }
-          L.count++;
-        }
+      const int left = L.jlim - jb;
+      if (left < kScbGroup) cand &= left > 0 ? (1u << left) - 1u : 0u;
+      if (!alive) cand = 0u;
+      while (cand) {
+        const int q = __builtin_ctz(cand);
+        cand &= cand - 1u;
+        list[n * 64 + lane] = (uint16_t)(jb + q - base);
+        ++n;
       }
-      if (!alive) break;
     }
-    if (!__any(alive)) break;
   }
-  L.total = commul_last;
+  scb_eval_list<Select, AllowSlow>(v, L, a, alive, list, base, n, w, wl3, n_select, target);
+  nlist = n;
+  L.total = a.commul_last;
+}
+
+// Second pass over a complete list (scb_walk set `complete`): the same
+// contributions as scb_walk<Select> without scanning again.
+template <bool AllowSlow>
+NCD void scb_select_list(const SCBView& v, ScbLane& L, int n_select, double target, const uint16_t* __restrict__ list,
+                         int nlist) {
+  L.count = 0;
+  L.slow = false;
+  L.found = false;
+  bool alive = L.jlim > 0;
+  const double w = gm_round(L.wl);
+  const double wl3 = w * w * w;
+  ScbAcc a{-1, 0.0, 0.0, 0.0};
+  scb_eval_list<true, AllowSlow>(v, L, a, alive, list, 0, alive ? nlist : 0, w, wl3, n_select, target);
 }
 
 // per-lane set-up of a neutron (NCSCBragg.cc:233-259): returns false when

@@ -28,14 +28,16 @@ namespace {
 template <bool AllowSlow>
 NCD void scb_xs_one(const SCBView& v, bool valid, int64_t i, const double* __restrict__ ekin,
                     const double* __restrict__ dirs, double* __restrict__ out, int64_t* __restrict__ queue,
-                    unsigned long long* __restrict__ qcount) {
+                    unsigned long long* __restrict__ qcount, uint16_t* __restrict__ list) {
   ScbLane L;
   L.wl = 1.0;
   L.u = V3{0.0, 0.0, 1.0};
   L.jlim = 0;
   bool can = false;
   if (valid) can = scb_lane_setup(v, ekin[i], dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2], L);
-  scb_walk<false, AllowSlow>(v, L, 0, 0.0);
+  int nlist;
+  bool complete;
+  scb_walk<false, AllowSlow>(v, L, 0, 0.0, list, nlist, complete);
   if (!valid) return;
   if (!AllowSlow && L.slow) {
     const unsigned long long slot = atomicAdd(qcount, 1ull);
@@ -49,7 +51,7 @@ template <bool AllowSlow>
 NCD void scb_sample_one(const SCBView& v, bool valid, int64_t i, const double* __restrict__ ekin,
                         const double* __restrict__ dirs, uint64_t seed, uint32_t stream, double* __restrict__ oE,
                         double* __restrict__ oD, int64_t* __restrict__ queue,
-                        unsigned long long* __restrict__ qcount) {
+                        unsigned long long* __restrict__ qcount, uint16_t* __restrict__ list) {
   double E = 0.0;
   V3 din{0.0, 0.0, 1.0};
   ScbLane L;
@@ -63,7 +65,9 @@ NCD void scb_sample_one(const SCBView& v, bool valid, int64_t i, const double* _
     can = scb_lane_setup(v, E, din.x, din.y, din.z, L);
   }
   // pass 1: number of contributions and their cumulative total
-  scb_walk<false, AllowSlow>(v, L, 0, 0.0);
+  int nlist;
+  bool complete;
+  scb_walk<false, AllowSlow>(v, L, 0, 0.0, list, nlist, complete);
   bool queued = false;
   if (valid && !AllowSlow && L.slow) {
     const unsigned long long slot = atomicAdd(qcount, 1ull);
@@ -75,10 +79,18 @@ NCD void scb_sample_one(const SCBView& v, bool valid, int64_t i, const double* _
   rng.init(seed, stream, (uint64_t)i);
   double target = 0.0;
   if (scatter && L.count > 1) target = L.total * rng.gen();
-  // pass 2: the contribution whose cumulative value passes the target
+  // pass 2: the contribution whose cumulative value passes the target, from
+  // the candidate list of pass 1 when it held them all (wave-uniform), else
+  // by a new scan
   ScbLane S = L;
   if (!scatter) S.jlim = 0;
-  scb_walk<true, AllowSlow>(v, S, L.count, target);
+  if (complete) {
+    scb_select_list<AllowSlow>(v, S, L.count, target, list, nlist);
+  } else {
+    int nl2;
+    bool c2;
+    scb_walk<true, AllowSlow>(v, S, L.count, target, list, nl2, c2);
+  }
   if (!valid || queued) return;
   V3 o = din;
   if (scatter && S.found) o = gm_genscat(v, rng, S.sel_normal, S.sel_inv2d_r, L.wl, L.u);
@@ -175,36 +187,143 @@ __global__ void __launch_bounds__(256) k_scb_xs_main(SCBView v, const double* __
                                                      const int64_t* __restrict__ perm, double* __restrict__ out,
                                                      int64_t* __restrict__ queue,
                                                      unsigned long long* __restrict__ qcount) {
+  __shared__ uint16_t lists[4 * kScbListWave];
+  uint16_t* list = lists + (threadIdx.x >> 6) * kScbListWave;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     const int64_t k = base + threadIdx.x;
     const bool valid = k < n;
-    scb_xs_one<false>(v, valid, valid ? perm[k] : 0, ekin, dirs, out, queue, qcount);
+    scb_xs_one<false>(v, valid, valid ? perm[k] : 0, ekin, dirs, out, queue, qcount, list);
   }
 }
 __global__ void __launch_bounds__(256) k_scb_xs_slow(SCBView v, const double* __restrict__ ekin,
                                                      const double* __restrict__ dirs, double* __restrict__ out,
                                                      const int64_t* __restrict__ queue,
                                                      const unsigned long long* __restrict__ qcount) {
+  __shared__ uint16_t lists[4 * kScbListWave];
+  uint16_t* list = lists + (threadIdx.x >> 6) * kScbListWave;
   const int64_t nq = (int64_t)*qcount;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nq; base += stride) {
     const int64_t q = base + threadIdx.x;
     const bool valid = q < nq;
-    scb_xs_one<true>(v, valid, valid ? queue[q] : 0, ekin, dirs, out, nullptr, nullptr);
+    scb_xs_one<true>(v, valid, valid ? queue[q] : 0, ekin, dirs, out, nullptr, nullptr, list);
   }
 }
-__global__ void __launch_bounds__(256) k_scb_sample_main(SCBView v, const double* __restrict__ ekin,
-                                                         const double* __restrict__ dirs, int64_t n,
-                                                         const int64_t* __restrict__ perm, uint64_t seed,
-                                                         uint32_t stream, double* __restrict__ oE,
-                                                         double* __restrict__ oD, int64_t* __restrict__ queue,
-                                                         unsigned long long* __restrict__ qcount) {
+// Sampling in three kernels over the sorted batch (fewer registers each
+// than one kernel doing it all): the first counts the contributions and
+// their total for sorted position k (cnt[k], tot[k]; 0 for a neutron that
+// does not scatter, whose output it writes, or that goes to the Romberg
+// queue); the second draws the target and repeats the scan until the
+// selected contribution (sel[k]: normal of the reflecting side, rounded
+// 1/(2d); 0 if none); the third generates the scattered direction, with the
+// neutron's Philox stream continued after the target draw.
+__global__ void __launch_bounds__(256) k_scb_sample_count(SCBView v, const double* __restrict__ ekin,
+                                                          const double* __restrict__ dirs, int64_t n,
+                                                          const int64_t* __restrict__ perm, double* __restrict__ oE,
+                                                          double* __restrict__ oD, int* __restrict__ cnt,
+                                                          double* __restrict__ tot, int64_t* __restrict__ queue,
+                                                          unsigned long long* __restrict__ qcount) {
+  __shared__ uint16_t lists[4 * kScbListWave];
+  uint16_t* list = lists + (threadIdx.x >> 6) * kScbListWave;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     const int64_t k = base + threadIdx.x;
     const bool valid = k < n;
-    scb_sample_one<false>(v, valid, valid ? perm[k] : 0, ekin, dirs, seed, stream, oE, oD, queue, qcount);
+    const int64_t i = valid ? perm[k] : 0;
+    double E = 0.0;
+    V3 din{0.0, 0.0, 1.0};
+    ScbLane L;
+    L.wl = 1.0;
+    L.u = din;
+    L.jlim = 0;
+    bool can = false;
+    if (valid) {
+      E = ekin[i];
+      din = V3{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+      can = scb_lane_setup(v, E, din.x, din.y, din.z, L);
+    }
+    int nlist;
+    bool complete;
+    scb_walk<false, false>(v, L, 0, 0.0, list, nlist, complete);
+    if (!valid) continue;
+    if (L.slow) {
+      const unsigned long long slot = atomicAdd(qcount, 1ull);
+      queue[slot] = i;
+      cnt[k] = 0;
+      continue;
+    }
+    const bool scatter = can && L.count > 0 && L.total > 0.0;
+    cnt[k] = scatter ? L.count : 0;
+    tot[k] = L.total;
+    if (!scatter) {
+      oE[i] = E;
+      oD[3 * i] = din.x;
+      oD[3 * i + 1] = din.y;
+      oD[3 * i + 2] = din.z;
+    }
+  }
+}
+__global__ void __launch_bounds__(256) k_scb_sample_pick(SCBView v, const double* __restrict__ ekin,
+                                                         const double* __restrict__ dirs, int64_t n,
+                                                         const int64_t* __restrict__ perm, uint64_t seed,
+                                                         uint32_t stream, const int* __restrict__ cnt,
+                                                         const double* __restrict__ tot, double4* __restrict__ sel) {
+  __shared__ uint16_t lists[4 * kScbListWave];
+  uint16_t* list = lists + (threadIdx.x >> 6) * kScbListWave;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const int64_t k = base + threadIdx.x;
+    const int c = k < n ? cnt[k] : 0;
+    const int64_t i = c > 0 ? perm[k] : 0;
+    double E = 0.0;
+    V3 din{0.0, 0.0, 1.0};
+    ScbLane L;
+    L.wl = 1.0;
+    L.u = din;
+    L.jlim = 0;
+    if (c > 0) {
+      E = ekin[i];
+      din = V3{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+      scb_lane_setup(v, E, din.x, din.y, din.z, L);
+    }
+    Rng rng;
+    rng.init(seed, stream, (uint64_t)i);
+    double target = 0.0;
+    if (c > 1) target = tot[k] * rng.gen();
+    int nl;
+    bool complete;
+    scb_walk<true, false>(v, L, c, target, list, nl, complete);
+    if (c <= 0) continue;
+    sel[k] = L.found ? make_double4(L.sel_normal.x, L.sel_normal.y, L.sel_normal.z, L.sel_inv2d_r)
+                     : make_double4(0.0, 0.0, 0.0, 0.0);
+  }
+}
+__global__ void __launch_bounds__(256) k_scb_sample_scatter(SCBView v, const double* __restrict__ ekin,
+                                                            const double* __restrict__ dirs, int64_t n,
+                                                            const int64_t* __restrict__ perm, uint64_t seed,
+                                                            uint32_t stream, double* __restrict__ oE,
+                                                            double* __restrict__ oD, const int* __restrict__ cnt,
+                                                            const double4* __restrict__ sel) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const int c = cnt[k];
+    if (c <= 0) continue;
+    const int64_t i = perm[k];
+    const double E = ekin[i];
+    const V3 din{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+    ScbLane L;
+    scb_lane_setup(v, E, din.x, din.y, din.z, L);
+    Rng rng;
+    rng.init(seed, stream, (uint64_t)i);
+    if (c > 1) (void)rng.gen();  // the target draw of the selection
+    const double4 s4 = sel[k];
+    V3 o = din;
+    if (s4.w != 0.0) o = gm_genscat(v, rng, V3{s4.x, s4.y, s4.z}, s4.w, L.wl, L.u);
+    oE[i] = E;
+    oD[3 * i] = o.x;
+    oD[3 * i + 1] = o.y;
+    oD[3 * i + 2] = o.z;
   }
 }
 __global__ void __launch_bounds__(256) k_scb_sample_slow(SCBView v, const double* __restrict__ ekin,
@@ -212,12 +331,14 @@ __global__ void __launch_bounds__(256) k_scb_sample_slow(SCBView v, const double
                                                          uint32_t stream, double* __restrict__ oE,
                                                          double* __restrict__ oD, const int64_t* __restrict__ queue,
                                                          const unsigned long long* __restrict__ qcount) {
+  __shared__ uint16_t lists[4 * kScbListWave];
+  uint16_t* list = lists + (threadIdx.x >> 6) * kScbListWave;
   const int64_t nq = (int64_t)*qcount;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nq; base += stride) {
     const int64_t q = base + threadIdx.x;
     const bool valid = q < nq;
-    scb_sample_one<true>(v, valid, valid ? queue[q] : 0, ekin, dirs, seed, stream, oE, oD, nullptr, nullptr);
+    scb_sample_one<true>(v, valid, valid ? queue[q] : 0, ekin, dirs, seed, stream, oE, oD, nullptr, nullptr, list);
   }
 }
 
@@ -252,8 +373,12 @@ void launch_scb_sample(const SCBView& v, const double* ekin, const double* dirs,
                        uint32_t stream, double* oE, double* oD, const ScbWork& w, hipStream_t st) {
   if (n <= 0) return;
   scb_order(v, ekin, dirs, n, w, st);
-  hipLaunchKernelGGL(k_scb_sample_main, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, w.perm, seed, stream,
-                     oE, oD, w.queue, w.qcount);
+  hipLaunchKernelGGL(k_scb_sample_count, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, w.perm, oE, oD,
+                     w.cnt, w.tot, w.queue, w.qcount);
+  hipLaunchKernelGGL(k_scb_sample_pick, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, w.perm, seed, stream,
+                     w.cnt, w.tot, w.sel);
+  hipLaunchKernelGGL(k_scb_sample_scatter, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, w.perm, seed,
+                     stream, oE, oD, w.cnt, w.sel);
   hipLaunchKernelGGL(k_scb_sample_slow, dim3(256), dim3(256), 0, st, v, ekin, dirs, seed, stream, oE, oD, w.queue,
                      w.qcount);
 }
