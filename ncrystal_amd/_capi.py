@@ -94,6 +94,11 @@ def lib():
                                                                                  ctypes.c_double, ctypes.c_int64,
                                                                                  _ip, _ip, _dp, _dp, _i64p,
                                                                                  ctypes.c_int64, _i64p, _ip, _i64p]),
+        "ncamd_hkl_spacegroup_members": (ctypes.c_int64, [ctypes.c_int] + [ctypes.c_double] * 6 +
+                                         [ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int64, _ip, _ip, _dp, _dp, _i64p, ctypes.c_int64, _i64p, _ip,
+                                          _i64p]),
+        "ncamd_reci_lattice": (ctypes.c_int, [ctypes.c_double] * 6 + [_dp]),
         "ncamd_create_scbragg": (Handle, [ctypes.c_double, ctypes.c_int64, _dp, _dp, _i64p, _ip, _dp, _dp,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "ncamd_scbragg_info": (ctypes.c_int, [Handle, _i64p, _i64p, _dp, _dp]),

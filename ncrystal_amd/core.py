@@ -196,6 +196,53 @@ def hkl_orthogonal_members(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tol
                 moff[:n + 1].copy(), mhkl[:3 * nm].reshape(nm, 3).copy())
 
 
+def hkl_spacegroup_members(spacegroup, cell, species, dcutoff, fsquarecut=1e-5):
+    """hkl families of a crystal with a space group (NCrystal's
+    calculateHKLPlanesWithSymEqRefl, ncrystal_core/src/extd_utils/NCFillHKL.cc:545-709):
+    cell = (a, b, c, alpha, beta, gamma) in Aa and degrees, species as for
+    hkl_orthogonal.  Returns (hkl, mult, dspacing, fsquared, mem_off[n+1],
+    mem_hkl[m,3]); the members of a family are its equivalent reflections, one
+    per +-pair."""
+    natoms = np.array([len(s[2]) for s in species], dtype=np.int32)
+    pos = C.as_f64(np.concatenate([np.asarray(s[2], dtype=np.float64).reshape(-1, 3) for s in species]).ravel())
+    csl = C.as_f64([s[0] for s in species])
+    msd = C.as_f64([s[1] for s in species])
+    a, b, c, al, be, ga = (float(x) for x in cell)
+    maxout, maxmem = 4096, 65536
+    req, mreq = ctypes.c_int64(), ctypes.c_int64()
+    while True:
+        hkl = np.empty(3 * maxout, dtype=np.int32)
+        mult = np.empty(maxout, dtype=np.int32)
+        dsp = np.empty(maxout)
+        fsq = np.empty(maxout)
+        moff = np.empty(maxout + 1, dtype=np.int64)
+        mhkl = np.empty(3 * maxmem, dtype=np.int32)
+        n = C.call("ncamd_hkl_spacegroup_members", int(spacegroup), a, b, c, al, be, ga, len(species),
+                   C.iptr(natoms), C.dptr(pos), C.dptr(csl), C.dptr(msd), dcutoff, fsquarecut, maxout, C.iptr(hkl),
+                   C.iptr(mult), C.dptr(dsp), C.dptr(fsq), ctypes.byref(req), maxmem, C.i64ptr(moff),
+                   C.iptr(mhkl), ctypes.byref(mreq))
+        if n == -1:
+            maxout = max(maxout, int(req.value))
+            maxmem = max(maxmem, int(mreq.value))
+            continue
+        if n == -2:
+            raise C.NCCalcError(f"Combinatorics too great to reach dcutoff = {dcutoff} Aa")
+        if n < 0:
+            raise C.NCAMDError("hkl enumeration failed")
+        nm = int(moff[n])
+        return (hkl[:3 * n].reshape(n, 3).copy(), mult[:n].copy(), dsp[:n].copy(), fsq[:n].copy(),
+                moff[:n + 1].copy(), mhkl[:3 * nm].reshape(nm, 3).copy())
+
+
+def reci_lattice(cell):
+    """Reciprocal lattice (row-major 3x3) of cell = (a, b, c, alpha, beta,
+    gamma) as ncrystal_core/src/utils/NCLatticeUtils.cc:95-149 builds it
+    (90/90/90, 90/90/120 and 90/beta/90 cells)."""
+    out = np.empty(9)
+    C.call("ncamd_reci_lattice", *[float(x) for x in cell], C.dptr(out))
+    return out
+
+
 def reci_lattice_orthogonal(a, b, c):
     """Reciprocal lattice of a cell with 90-degree angles, row-major 3x3
     (diag(2pi/a, 2pi/b, 2pi/c), ncrystal_core/src/utils/NCLatticeUtils.cc:95-117)."""

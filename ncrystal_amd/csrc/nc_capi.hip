@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <unordered_set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -730,6 +731,90 @@ struct Fam {
   std::vector<HKL3> members;
 };
 
+// species with a non-zero coherent scattering length (NCFillHKL.cc:175-186)
+// and the per-species early-abort thresholds of the Debye-Waller factors
+// (:203-209), uploaded for the |F|^2 kernels; false if there are none
+struct HKLSpecies {
+  DevBuf pos, csl, msd, thr, off;
+};
+bool hkl_species(HKLBuild& P, HKLSpecies& S, int nspecies, const int* natoms, const double* positions,
+                 const double* csl, const double* msd, double dcutoff, double fsquarecut) {
+  if (fsquarecut >= 0.0) fsquarecut = std::max(fsquarecut, 1.0e-300);
+  P.fsquarecut = fsquarecut;
+  std::vector<double> vpos, vcsl, vmsd, vthr;
+  std::vector<int> off{0};
+  const double* pp = positions;
+  for (int i = 0; i < nspecies; ++i) {
+    if (fabs(csl[i]) > 0.0) {
+      vpos.insert(vpos.end(), pp, pp + 3 * natoms[i]);
+      vcsl.push_back(csl[i]);
+      vmsd.push_back(msd[i]);
+      vthr.push_back((fsquarecut < 0.01 && fsquarecut > 1.0e-300) ? std::log(fabs(csl[i]) / fsquarecut)
+                                                                  : __builtin_huge_val());
+      off.push_back(off.back() + natoms[i]);
+    }
+    pp += 3 * natoms[i];
+  }
+  const int ns = (int)vcsl.size();
+  if (ns > 32) raise("BadInput", "at most 32 species with non-zero coherent scattering length");
+  if (ns == 0) return false;
+  // ksq and d intervals (NCFillHKL.cc:220-224)
+  const double DMIN = 2.2250738585072014e-308, DMAX = 1.7976931348623157e308;
+  P.ksq_lo = std::min(std::max((39.4784176043574344753379639995046045412547976 * (1.0 - 1e-14)) /
+                                   (__builtin_huge_val() * __builtin_huge_val()), DMIN), DMAX);
+  P.ksq_hi = std::min(std::max((39.4784176043574344753379639995046045412547976 * (1.0 + 1e-14)) /
+                                   (dcutoff * dcutoff), DMIN), DMAX);
+  P.dcut_lo = std::min(std::max(dcutoff, DMIN), DMAX);
+  P.dcut_hi = DMAX;
+  S.pos.upload(vpos.data(), vpos.size());
+  S.csl.upload(vcsl.data(), vcsl.size());
+  S.msd.upload(vmsd.data(), vmsd.size());
+  S.thr.upload(vthr.data(), vthr.size());
+  S.off.upload(off.data(), off.size());
+  P.nspecies = ns;
+  P.pos = S.pos.as<double>(); P.csl = S.csl.as<double>(); P.msd = S.msd.as<double>();
+  P.thr = S.thr.as<double>(); P.pos_off = S.off.as<int>();
+  return true;
+}
+
+// InfoBuilder's family order (NCInfoBuilder.cc:98-117, std::stable_sort at
+// :452) and the output arrays; members (first = representative) if asked
+int64_t hkl_output(std::vector<Fam>& fams, int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp,
+                   double* out_fsq, int64_t* nrequired, int64_t maxmem, int64_t* out_mem_off, int* out_mem_hkl,
+                   int64_t* nmem_required) {
+  auto before = [](const Fam& rh, const Fam& lh) {
+    if (fabs(lh.dsp - rh.dsp) > 1e-6) return lh.dsp < rh.dsp;
+    const double lfm = lh.fsq * lh.mult, rfm = rh.fsq * rh.mult;
+    if (fabs(lfm - rfm) > 1e-6) return lfm < rfm;
+    if (lh.mult != rh.mult) return lh.mult < rh.mult;
+    return hkl_less(lh.rep, rh.rep);
+  };
+  std::stable_sort(fams.begin(), fams.end(), before);
+  const int64_t nf = (int64_t)fams.size();
+  if (nrequired) *nrequired = nf;
+  if (nf > maxout) return -1;
+  int64_t nmem = 0;
+  for (const auto& f : fams) nmem += (int64_t)f.members.size();
+  if (nmem_required) *nmem_required = nmem;
+  if (out_mem_off && nmem > maxmem) return -1;
+  for (int64_t i = 0; i < nf; ++i) {
+    out_hkl[3 * i] = fams[i].rep.h; out_hkl[3 * i + 1] = fams[i].rep.k; out_hkl[3 * i + 2] = fams[i].rep.l;
+    out_mult[i] = fams[i].mult; out_dsp[i] = fams[i].dsp; out_fsq[i] = fams[i].fsq;
+  }
+  if (out_mem_off) {
+    int64_t m = 0;
+    for (int64_t i = 0; i < nf; ++i) {
+      out_mem_off[i] = m;
+      for (const auto& x : fams[i].members) {
+        out_mem_hkl[3 * m] = x.h; out_mem_hkl[3 * m + 1] = x.k; out_mem_hkl[3 * m + 2] = x.l;
+        ++m;
+      }
+    }
+    out_mem_off[nf] = m;
+  }
+  return nf;
+}
+
 int64_t hkl_orthogonal(double a, double b, double c, int nspecies, const int* natoms, const double* positions,
                        const double* csl, const double* msd, double dcutoff, double fsquarecut, double merge_tol,
                        int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq,
@@ -742,44 +827,9 @@ int64_t hkl_orthogonal(double a, double b, double c, int nspecies, const int* na
   HKLBuild P{};
   P.rl0 = k2Pi / a; P.rl1 = k2Pi / b; P.rl2 = k2Pi / c;
   P.max_h = floorhkl(a * invd); P.max_k = floorhkl(b * invd); P.max_l = floorhkl(c * invd);
-  if (fsquarecut >= 0.0) fsquarecut = std::max(fsquarecut, 1.0e-300);
-  P.fsquarecut = fsquarecut;
-  std::vector<double> vpos, vcsl, vmsd, vthr;
-  std::vector<int> off{0};
-  {
-    const double* pp = positions;
-    for (int i = 0; i < nspecies; ++i) {
-      if (fabs(csl[i]) > 0.0) {
-        vpos.insert(vpos.end(), pp, pp + 3 * natoms[i]);
-        vcsl.push_back(csl[i]);
-        vmsd.push_back(msd[i]);
-        vthr.push_back((fsquarecut < 0.01 && fsquarecut > 1.0e-300) ? std::log(fabs(csl[i]) / fsquarecut)
-                                                                    : __builtin_huge_val());
-        off.push_back(off.back() + natoms[i]);
-      }
-      pp += 3 * natoms[i];
-    }
-  }
-  const int ns = (int)vcsl.size();
-  if (ns > 32) raise("BadInput", "at most 32 species with non-zero coherent scattering length");
+  HKLSpecies S;
   if (nrequired) *nrequired = 0;
-  if (ns == 0) return 0;
-  const double DMIN = 2.2250738585072014e-308, DMAX = 1.7976931348623157e308;
-  P.ksq_lo = std::min(std::max((39.4784176043574344753379639995046045412547976 * (1.0 - 1e-14)) /
-                                   (__builtin_huge_val() * __builtin_huge_val()), DMIN), DMAX);
-  P.ksq_hi = std::min(std::max((39.4784176043574344753379639995046045412547976 * (1.0 + 1e-14)) /
-                                   (dcutoff * dcutoff), DMIN), DMAX);
-  P.dcut_lo = std::min(std::max(dcutoff, DMIN), DMAX);
-  P.dcut_hi = DMAX;
-  DevBuf dpos, dcsl, dmsd, dthr, doff;
-  dpos.upload(vpos.data(), vpos.size());
-  dcsl.upload(vcsl.data(), vcsl.size());
-  dmsd.upload(vmsd.data(), vmsd.size());
-  dthr.upload(vthr.data(), vthr.size());
-  doff.upload(off.data(), off.size());
-  P.nspecies = ns;
-  P.pos = dpos.as<double>(); P.csl = dcsl.as<double>(); P.msd = dmsd.as<double>();
-  P.thr = dthr.as<double>(); P.pos_off = doff.as<int>();
+  if (!hkl_species(P, S, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut)) return 0;
   const int64_t nl = 2 * (int64_t)P.max_l + 1, nkf = 2 * (int64_t)P.max_k + 1;
   const int64_t total = ((int64_t)P.max_k + 1) * nl + (int64_t)P.max_h * nkf * nl;
   const int64_t chunk = std::min<int64_t>(total, (int64_t)1 << 23);
@@ -839,37 +889,172 @@ int64_t hkl_orthogonal(double a, double b, double c, int nspecies, const int* na
     std::sort(f.members.begin(), f.members.end(), hkl_less);
     f.rep = f.members.front();
   }
-  auto before = [](const Fam& rh, const Fam& lh) {
-    if (fabs(lh.dsp - rh.dsp) > 1e-6) return lh.dsp < rh.dsp;
-    const double lfm = lh.fsq * lh.mult, rfm = rh.fsq * rh.mult;
-    if (fabs(lfm - rfm) > 1e-6) return lfm < rfm;
-    if (lh.mult != rh.mult) return lh.mult < rh.mult;
-    return hkl_less(lh.rep, rh.rep);
+  return hkl_output(fams, maxout, out_hkl, out_mult, out_dsp, out_fsq, nrequired, maxmem, out_mem_off,
+                    out_mem_hkl, nmem_required);
+}
+
+// ---- space group: reflections grouped by symmetry equivalence
+// (ncrystal_core/src/extd_utils/NCFillHKL.cc:545-709).
+// Equivalent reflections by Laue class, the rule sets of
+// ncrystal_core/src/phys_utils/NCEqRefl.cc:25-150: each entry is stored as
+// the smaller (HKL ordering) of the pair (x, -x)
+// (ncrystal_core/include/NCrystal/internal/phys_utils/NCEqRefl.hh:125-132).
+HKL3 eq_norm(int h, int k, int l) {
+  const HKL3 x{h, k, l}, xm{-h, -k, -l};
+  return hkl_less(xm, x) ? xm : x;
+}
+int eqrefl_raw(int sg, int h, int k, int l, HKL3* o) {
+  int n = 0;
+  auto add = [&](int x, int y, int z) { o[n++] = eq_norm(x, y, z); };
+  if (sg < 3) {  // triclinic
+    add(h, k, l);
+  } else if (sg < 16) {  // monoclinic
+    add(h, k, l); add(h, -k, l);
+  } else if (sg < 75) {  // orthorhombic
+    add(h, k, l); add(h, -k, -l); add(h, -k, l); add(h, k, -l);
+  } else if (sg < 89) {  // tetragonal 4/m
+    add(h, k, l); add(k, -h, -l); add(h, k, -l); add(k, -h, l);
+  } else if (sg < 143) {  // tetragonal 4/mmm
+    add(h, k, l); add(k, h, l); add(k, -h, -l); add(h, k, -l);
+    add(k, -h, l); add(h, -k, -l); add(k, h, -l); add(h, -k, l);
+  } else if (sg < 149) {  // trigonal -3
+    add(h, k, l); add(h + k, -h, -l); add(k, -h - k, l);
+  } else if (sg < 168) {  // trigonal -3m
+    add(h, k, l); add(h + k, -h, -l); add(k, -h - k, l); add(k, h, -l); add(h + k, -k, l); add(h, -h - k, -l);
+  } else if (sg < 177) {  // hexagonal 6/m
+    add(h, k, l); add(h, k, -l); add(k, -h - k, -l); add(k, -h - k, l); add(h + k, -h, l); add(h + k, -h, -l);
+  } else if (sg < 195) {  // hexagonal 6/mmm
+    add(h, k, l); add(k, -h - k, -l); add(h + k, -h, -l); add(h, k, -l); add(k, -h - k, l);
+    add(h + k, -h, l); add(k, h, l); add(h + k, -k, l); add(h, -h - k, l);
+    add(k, h, -l); add(h + k, -k, -l); add(h, -h - k, -l);
+  } else if (sg < 207) {  // cubic m-3
+    add(h, k, l); add(h, -k, -l); add(h, -k, l); add(h, k, -l);
+    add(k, l, h); add(k, -l, -h); add(k, -l, h); add(k, l, -h);
+    add(l, h, k); add(l, -h, -k); add(l, -h, k); add(l, h, -k);
+  } else {  // cubic m-3m
+    add(h, k, l); add(h, -k, -l); add(h, -k, l); add(h, k, -l);
+    add(k, l, h); add(k, -l, -h); add(k, -l, h); add(k, l, -h);
+    add(l, h, k); add(l, -h, -k); add(l, -h, k); add(l, h, -k);
+    add(k, h, l); add(k, -h, -l); add(k, -h, l); add(k, h, -l);
+    add(l, k, h); add(l, -k, -h); add(l, -k, h); add(l, k, -h);
+    add(h, l, k); add(h, -l, -k); add(h, -l, k); add(h, l, -k);
+  }
+  return n;
+}
+
+// reciprocal lattice (row-major) of a cell given in degrees, the closed
+// forms of ncrystal_core/src/utils/NCLatticeUtils.cc:105-144 (all angles 90;
+// alpha = beta = 90 and gamma = 120; alpha = gamma = 90); false otherwise
+bool reci_lattice(double a, double b, double c, double alpha_deg, double beta_deg, double gamma_deg, double* m) {
+  constexpr double kDeg = 0.0174532925199432957692369076848861271344287189;
+  const double al = alpha_deg * kDeg, be = beta_deg * kDeg, ga = gamma_deg * kDeg;
+  const double tol = 1e-14;
+  const bool a90 = fabs(al - kPiHalf) < tol, b90 = fabs(be - kPiHalf) < tol, g90 = fabs(ga - kPiHalf) < tol;
+  const bool g120 = fabs(ga - kPi * (2 / 3.)) < tol;
+  for (int i = 0; i < 9; ++i) m[i] = 0.0;
+  if (a90 && b90 && g90) {
+    m[0] = k2Pi / a; m[4] = k2Pi / b; m[8] = k2Pi / c;
+  } else if (a90 && b90 && g120) {
+    constexpr double k2PiDivSqrt3 = 3.62759872846843570118815651528431146456813249618553;
+    constexpr double k4PiDivSqrt3 = 7.25519745693687140237631303056862292913626499237106;
+    m[0] = k2Pi / a; m[3] = k2PiDivSqrt3 / a; m[4] = k4PiDivSqrt3 / b; m[8] = k2Pi / c;
+  } else if (a90 && g90) {
+    const double sb = std::sin(be), tb = std::tan(be);
+    const double cotb = 1.0 / tb;
+    m[0] = k2Pi / a; m[4] = k2Pi / b; m[6] = -cotb * k2Pi / a; m[8] = k2Pi / (c * sb);
+  } else {
+    return false;
+  }
+  return true;
+}
+
+int64_t hkl_spacegroup(int sg, double a, double b, double c, double alpha, double beta, double gamma, int nspecies,
+                       const int* natoms, const double* positions, const double* csl, const double* msd,
+                       double dcutoff, double fsquarecut, int64_t maxout, int* out_hkl, int* out_mult,
+                       double* out_dsp, double* out_fsq, int64_t* nrequired, int64_t maxmem,
+                       int64_t* out_mem_off, int* out_mem_hkl, int64_t* nmem_required) {
+  ensure_device();
+  if (sg < 1 || sg > 230) raise("BadInput", "Space group number is not in the range 1 to 230");
+  if (!(dcutoff > 0.0)) raise("BadInput", "dcutoff must be positive");
+  if (!(a > 0 && b > 0 && c > 0)) raise("BadInput", "lattice parameters must be positive");
+  HKLMatrix rm;
+  if (!reci_lattice(a, b, c, alpha, beta, gamma, rm.m))
+    raise("BadInput", "unsupported cell: the angles must be 90/90/90, 90/90/120 or 90/beta/90");
+  // |h| <= |G.a|/(2 pi) <= a/dcutoff: a range that covers the reference's
+  // estimate (NCLatticeUtils.cc:177-242), so the same reflections are met in
+  // the same order
+  const double invd = 1.0 / dcutoff;
+  const int max_h = floorhkl(a * invd), max_k = floorhkl(b * invd), max_l = floorhkl(c * invd);
+  HKLBuild P{};
+  HKLSpecies S;
+  if (nrequired) *nrequired = 0;
+  if (!hkl_species(P, S, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut)) return 0;
+  // first occurrence of each symmetry representative in the loop order of
+  // the reference (h >= 0, then k, then l), (0,0,0) excluded; the ksq
+  // preselection is applied here so that only candidates go to the GPU
+  std::unordered_set<uint64_t> seen;
+  auto pack = [](const HKL3& x) {
+    return ((uint64_t)(uint32_t)(x.h + (1 << 20)) << 42) | ((uint64_t)(uint32_t)(x.k + (1 << 20)) << 21) |
+           (uint64_t)(uint32_t)(x.l + (1 << 20));
   };
-  std::stable_sort(fams.begin(), fams.end(), before);
-  const int64_t nf = (int64_t)fams.size();
-  if (nrequired) *nrequired = nf;
-  if (nf > maxout) return -1;
-  int64_t nmem = 0;
-  for (const auto& f : fams) nmem += (int64_t)f.members.size();
-  if (nmem_required) *nmem_required = nmem;
-  if (out_mem_off && nmem > maxmem) return -1;
-  for (int64_t i = 0; i < nf; ++i) {
-    out_hkl[3 * i] = fams[i].rep.h; out_hkl[3 * i + 1] = fams[i].rep.k; out_hkl[3 * i + 2] = fams[i].rep.l;
-    out_mult[i] = fams[i].mult; out_dsp[i] = fams[i].dsp; out_fsq[i] = fams[i].fsq;
-  }
-  if (out_mem_off) {
-    int64_t m = 0;
-    for (int64_t i = 0; i < nf; ++i) {
-      out_mem_off[i] = m;
-      for (const auto& x : fams[i].members) {
-        out_mem_hkl[3 * m] = x.h; out_mem_hkl[3 * m + 1] = x.k; out_mem_hkl[3 * m + 2] = x.l;
-        ++m;
+  auto rep_of = [&](int h, int k, int l) {
+    HKL3 o[24];
+    const int n = eqrefl_raw(sg, h, k, l, o);
+    int lo = 0;
+    for (int i = 1; i < n; ++i)
+      if (hkl_less(o[i], o[lo])) lo = i;
+    return o[lo];
+  };
+  seen.insert(pack(rep_of(0, 0, 0)));
+  std::vector<int> keys;
+  for (int h = 0; h <= max_h; ++h)
+    for (int k = (h ? -max_k : 0); k <= max_k; ++k)
+      for (int l = -max_l; l <= max_l; ++l) {
+        const HKL3 key = rep_of(h, k, l);
+        if (!seen.insert(pack(key)).second) continue;
+        const double dh = (double)key.h, dk = (double)key.k, dl = (double)key.l;
+        const double wx = rm.m[0] * dh + rm.m[1] * dk + rm.m[2] * dl;
+        const double wy = rm.m[3] * dh + rm.m[4] * dk + rm.m[5] * dl;
+        const double wz = rm.m[6] * dh + rm.m[7] * dk + rm.m[8] * dl;
+        const double ksq = wx * wx + wy * wy + wz * wz;
+        if (!(P.ksq_lo <= ksq && ksq <= P.ksq_hi)) continue;
+        keys.push_back(key.h); keys.push_back(key.k); keys.push_back(key.l);
       }
+  const int64_t nk = (int64_t)keys.size() / 3;
+  std::vector<Fam> fams;
+  if (nk > 0) {
+    DevBuf dK, dF, dD, dV;
+    dK.upload(keys.data(), keys.size());
+    dF.alloc(sizeof(double) * nk);
+    dD.alloc(sizeof(double) * nk);
+    dV.alloc(nk);
+    launch_hkl_eval_keys(P, rm, dK.as<int>(), nk, dF.as<double>(), dD.as<double>(), dV.as<unsigned char>(), 0);
+    NCAMD_HIP(hipGetLastError());
+    std::vector<double> hF(nk), hD(nk);
+    std::vector<unsigned char> hV(nk);
+    NCAMD_HIP(hipMemcpy(hV.data(), dV.p, nk, hipMemcpyDeviceToHost));
+    NCAMD_HIP(hipMemcpy(hF.data(), dF.p, sizeof(double) * nk, hipMemcpyDeviceToHost));
+    NCAMD_HIP(hipMemcpy(hD.data(), dD.p, sizeof(double) * nk, hipMemcpyDeviceToHost));
+    for (int64_t t = 0; t < nk; ++t) {
+      if (!hV[t]) continue;
+      if (fams.size() > 1000000) return -2;
+      HKL3 o[24];
+      const int n = eqrefl_raw(sg, keys[3 * t], keys[3 * t + 1], keys[3 * t + 2], o);
+      std::sort(o, o + n, hkl_less);
+      Fam f;
+      for (int i = 0; i < n; ++i)
+        if (f.members.empty() || !(o[i].h == f.members.back().h && o[i].k == f.members.back().k &&
+                                   o[i].l == f.members.back().l))
+          f.members.push_back(o[i]);
+      f.rep = f.members.front();
+      f.mult = 2 * (int)f.members.size();
+      f.fsq = hF[t];
+      f.dsp = hD[t];
+      fams.push_back(std::move(f));
     }
-    out_mem_off[nf] = m;
   }
-  return nf;
+  return hkl_output(fams, maxout, out_hkl, out_mult, out_dsp, out_fsq, nrequired, maxmem, out_mem_off,
+                    out_mem_hkl, nmem_required);
 }
 
 // ------------------------------------------------- single-crystal Bragg ---
@@ -1854,6 +2039,31 @@ int64_t ncamd_hkl_orthogonal_members(double a, double b, double c, int nspecies,
                         nmem_required);
   NCAMD_CATCH
   return -3;
+}
+
+int64_t ncamd_hkl_spacegroup_members(int spacegroup, double a, double b, double c, double alpha, double beta,
+                                     double gamma, int nspecies, const int* natoms, const double* positions,
+                                     const double* csl, const double* msd, double dcutoff, double fsquarecut,
+                                     int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq,
+                                     int64_t* nrequired, int64_t maxmem, int64_t* out_mem_off, int* out_mem_hkl,
+                                     int64_t* nmem_required) {
+  NCAMD_TRY
+  if (!out_mem_off != !out_mem_hkl) raise("BadInput", "give both member output arrays or neither");
+  return hkl_spacegroup(spacegroup, a, b, c, alpha, beta, gamma, nspecies, natoms, positions, csl, msd, dcutoff,
+                        fsquarecut, maxout, out_hkl, out_mult, out_dsp, out_fsq, nrequired, maxmem, out_mem_off,
+                        out_mem_hkl, nmem_required);
+  NCAMD_CATCH
+  return -3;
+}
+
+int ncamd_reci_lattice(double a, double b, double c, double alpha, double beta, double gamma, double* out9) {
+  NCAMD_TRY
+  if (!(a > 0 && b > 0 && c > 0)) raise("BadInput", "lattice parameters must be positive");
+  if (!reci_lattice(a, b, c, alpha, beta, gamma, out9))
+    raise("BadInput", "unsupported cell: the angles must be 90/90/90, 90/90/120 or 90/beta/90");
+  return 0;
+  NCAMD_CATCH
+  return -1;
 }
 
 ncamd_process_t ncamd_create_scbragg(double v0n, int64_t nfam, const double* dsp, const double* fsq,

@@ -249,5 +249,12 @@ struct HKLBuild {
 // writes fsq/dsp and a validity flag per point
 void launch_hkl_eval(const HKLBuild&, int64_t first, int64_t count, double* fsq, double* dsp,
                      unsigned char* valid, hipStream_t);
+// space-group path: evaluates the listed representatives (hkl[3*count])
+// with the reciprocal lattice rm (row-major 3x3)
+struct HKLMatrix {
+  double m[9];
+};
+void launch_hkl_eval_keys(const HKLBuild&, const HKLMatrix& rm, const int* hkl, int64_t count, double* fsq,
+                          double* dsp, unsigned char* valid, hipStream_t);
 
 }  // namespace ncamd

@@ -605,6 +605,258 @@ int orc_hkl_orthogonal_members(double a, double b, double c, int nspecies, const
   return result;
 }
 
+/* ---- hkl enumeration with a space group (NCFillHKL.cc:545-709) ---- */
+/* Equivalent reflections by Laue class, NCEqRefl.cc:25-150: each rule set
+   lists, for (h,k,l), the reflections equivalent to it, each stored in the
+   form of the pair (x, -x) that is smaller under the HKL ordering
+   (NCEqRefl.hh:125-132). */
+static hkl3 eq_norm(int h, int k, int l) {
+  hkl3 a = {h, k, l}, am = {-h, -k, -l};
+  return hkl_less(am, a) ? am : a;
+}
+static int eqrefl_raw(int sg, int h, int k, int l, hkl3* o) {
+  int n = 0;
+#define EQ(x, y, z) (o[n++] = eq_norm((x), (y), (z)))
+  if (sg < 3) {                    /* triclinic 1-2 */
+    EQ(h, k, l);
+  } else if (sg < 16) {            /* monoclinic 3-15 */
+    EQ(h, k, l); EQ(h, -k, l);
+  } else if (sg < 75) {            /* orthorhombic 16-74 */
+    EQ(h, k, l); EQ(h, -k, -l); EQ(h, -k, l); EQ(h, k, -l);
+  } else if (sg < 89) {            /* tetragonal 75-88 */
+    EQ(h, k, l); EQ(k, -h, -l); EQ(h, k, -l); EQ(k, -h, l);
+  } else if (sg < 143) {           /* tetragonal 89-142 */
+    EQ(h, k, l); EQ(k, h, l); EQ(k, -h, -l); EQ(h, k, -l);
+    EQ(k, -h, l); EQ(h, -k, -l); EQ(k, h, -l); EQ(h, -k, l);
+  } else if (sg < 149) {           /* trigonal 143-148 */
+    EQ(h, k, l); EQ(h + k, -h, -l); EQ(k, -h - k, l);
+  } else if (sg < 168) {           /* trigonal 149-167 */
+    EQ(h, k, l); EQ(h + k, -h, -l); EQ(k, -h - k, l); EQ(k, h, -l); EQ(h + k, -k, l); EQ(h, -h - k, -l);
+  } else if (sg < 177) {           /* hexagonal 168-176 */
+    EQ(h, k, l); EQ(h, k, -l); EQ(k, -h - k, -l); EQ(k, -h - k, l); EQ(h + k, -h, l); EQ(h + k, -h, -l);
+  } else if (sg < 195) {           /* hexagonal 177-194 */
+    EQ(h, k, l); EQ(k, -h - k, -l); EQ(h + k, -h, -l); EQ(h, k, -l); EQ(k, -h - k, l);
+    EQ(h + k, -h, l); EQ(k, h, l); EQ(h + k, -k, l); EQ(h, -h - k, l);
+    EQ(k, h, -l); EQ(h + k, -k, -l); EQ(h, -h - k, -l);
+  } else if (sg < 207) {           /* cubic 195-206 */
+    EQ(h, k, l); EQ(h, -k, -l); EQ(h, -k, l); EQ(h, k, -l);
+    EQ(k, l, h); EQ(k, -l, -h); EQ(k, -l, h); EQ(k, l, -h);
+    EQ(l, h, k); EQ(l, -h, -k); EQ(l, -h, k); EQ(l, h, -k);
+  } else {                         /* cubic 207-230 */
+    EQ(h, k, l); EQ(h, -k, -l); EQ(h, -k, l); EQ(h, k, -l);
+    EQ(k, l, h); EQ(k, -l, -h); EQ(k, -l, h); EQ(k, l, -h);
+    EQ(l, h, k); EQ(l, -h, -k); EQ(l, -h, k); EQ(l, h, -k);
+    EQ(k, h, l); EQ(k, -h, -l); EQ(k, -h, l); EQ(k, h, -l);
+    EQ(l, k, h); EQ(l, -k, -h); EQ(l, -k, h); EQ(l, k, -h);
+    EQ(h, l, k); EQ(h, -l, -k); EQ(h, -l, k); EQ(h, l, -k);
+  }
+#undef EQ
+  return n;
+}
+static int hkl_eq(hkl3 a, hkl3 b) { return a.h == b.h && a.k == b.k && a.l == b.l; }
+/* getEquivalentReflectionsRepresentativeValue: the smallest entry */
+static hkl3 eqrefl_rep(int sg, int h, int k, int l) {
+  hkl3 o[24];
+  const int n = eqrefl_raw(sg, h, k, l, o);
+  int lo = 0;
+  for (int i = 1; i < n; ++i)
+    if (hkl_less(o[i], o[lo])) lo = i;
+  return o[lo];
+}
+/* getEquivalentReflections: sorted, duplicates removed */
+static int eqrefl_list(int sg, hkl3 x, hkl3* o) {
+  const int n = eqrefl_raw(sg, x.h, x.k, x.l, o);
+  qsort(o, (size_t)n, sizeof(hkl3), cmp_hkl_qsort);
+  int m = 0;
+  for (int i = 0; i < n; ++i)
+    if (m == 0 || !hkl_eq(o[i], o[m - 1])) o[m++] = o[i];
+  return m;
+}
+
+/* reciprocal lattice of a cell (angles in degrees), the closed forms of
+   NCLatticeUtils.cc:105-144: all angles 90, alpha = beta = 90 with gamma =
+   120, and alpha = gamma = 90 (monoclinic, b unique); row-major.  Returns -3
+   for any other cell (the reference inverts the general lattice matrix). */
+int orc_reci_lattice(double a, double b, double c, double alpha_deg, double beta_deg, double gamma_deg,
+                     double* m) {
+  const double kdeg = 0.0174532925199432957692369076848861271344287189;
+  const double al = alpha_deg * kdeg, be = beta_deg * kdeg, ga = gamma_deg * kdeg;
+  const double tol = 1e-14;
+  const int a90 = fabs(al - K_PIHALF) < tol, b90 = fabs(be - K_PIHALF) < tol, g90 = fabs(ga - K_PIHALF) < tol;
+  const int g120 = fabs(ga - K_PI * (2 / 3.)) < tol;
+  for (int i = 0; i < 9; ++i) m[i] = 0.0;
+  if (a90 && b90 && g90) {
+    m[0] = K_2PI / a; m[4] = K_2PI / b; m[8] = K_2PI / c;
+  } else if (a90 && b90 && g120) {
+    const double k2ps3 = 3.62759872846843570118815651528431146456813249618553; /* 2 pi / sqrt(3) */
+    const double k4ps3 = 7.25519745693687140237631303056862292913626499237106; /* 4 pi / sqrt(3) */
+    m[0] = K_2PI / a; m[3] = k2ps3 / a; m[4] = k4ps3 / b; m[8] = K_2PI / c;
+  } else if (a90 && g90) {
+    const double sb = sin(be), tb = tan(be);
+    const double cotb = 1.0 / tb;
+    m[0] = K_2PI / a; m[4] = K_2PI / b; m[6] = -cotb * K_2PI / a; m[8] = K_2PI / (c * sb);
+  } else {
+    return -3;
+  }
+  return 0;
+}
+
+typedef struct { uint64_t* keys; size_t cap, n; } hset_t;
+static int hset_insert(hset_t* s, hkl3 x) { /* 1 if newly inserted */
+  const uint64_t key = (((uint64_t)(x.h + (1 << 20))) << 42) | (((uint64_t)(x.k + (1 << 20))) << 21) |
+                       (uint64_t)(x.l + (1 << 20));
+  if ((s->n + 1) * 2 > s->cap) {
+    size_t ncap = s->cap ? s->cap * 2 : 4096;
+    uint64_t* nk = (uint64_t*)calloc(ncap, sizeof(uint64_t));
+    for (size_t i = 0; i < s->cap; ++i)
+      if (s->keys[i]) {
+        size_t j = (size_t)((s->keys[i] * 0x9E3779B97F4A7C15ull) >> 20) & (ncap - 1);
+        while (nk[j]) j = (j + 1) & (ncap - 1);
+        nk[j] = s->keys[i];
+      }
+    free(s->keys); s->keys = nk; s->cap = ncap;
+  }
+  size_t j = (size_t)((key * 0x9E3779B97F4A7C15ull) >> 20) & (s->cap - 1);
+  while (s->keys[j]) {
+    if (s->keys[j] == key) return 0;
+    j = (j + 1) & (s->cap - 1);
+  }
+  s->keys[j] = key;
+  s->n++;
+  return 1;
+}
+
+int orc_hkl_spacegroup_members(int sg, double a, double b, double c, double alpha_deg, double beta_deg,
+                               double gamma_deg, int nspecies, const int* natoms, const double* positions,
+                               const double* csl, const double* msd, double dcutoff, double fsquarecut,
+                               int maxout, int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq,
+                               int64_t* out_mem_off, int* out_mem_hkl, int64_t maxmem) {
+  if (sg < 1 || sg > 230) return -4;
+  double m[9];
+  if (orc_reci_lattice(a, b, c, alpha_deg, beta_deg, gamma_deg, m)) return -3;
+  /* loop bounds: |h| <= |G.a|/(2 pi) <= a/dcutoff, which covers the range the
+     reference estimates (NCLatticeUtils.cc:177-242), so the same reflections
+     are met in the same order */
+  const double invd = 1.0 / dcutoff;
+  const int max_h = floorhkl(a * invd), max_k = floorhkl(b * invd), max_l = floorhkl(c * invd);
+  if (fsquarecut >= 0.0) fsquarecut = dmax(fsquarecut, 1.0e-300);
+  int ns = 0;
+  const double** pos = (const double**)malloc(sizeof(double*) * (size_t)(nspecies + 1));
+  int* np = (int*)malloc(sizeof(int) * (size_t)(nspecies + 1));
+  double* vcsl = (double*)malloc(sizeof(double) * (size_t)(nspecies + 1));
+  double* vmsd = (double*)malloc(sizeof(double) * (size_t)(nspecies + 1));
+  double* thr = (double*)malloc(sizeof(double) * (size_t)(nspecies + 1));
+  double* fac = (double*)malloc(sizeof(double) * (size_t)(nspecies + 1));
+  {
+    const double* p = positions;
+    for (int i = 0; i < nspecies; ++i) {
+      if (fabs(csl[i]) > 0.0) {
+        pos[ns] = p; np[ns] = natoms[i]; vcsl[ns] = csl[i]; vmsd[ns] = msd[i];
+        if (fsquarecut < 0.01 && fsquarecut > 1.0e-300) thr[ns] = log(fabs(csl[i]) / fsquarecut);
+        else thr[ns] = INFINITY;
+        ++ns;
+      }
+      p += 3 * natoms[i];
+    }
+  }
+  const double ksq_lo = dclamp((K_4PISQ * (1.0 - 1e-14)) / (INFINITY * INFINITY), DBL_MIN, DBL_MAX);
+  const double ksq_hi = dclamp((K_4PISQ * (1.0 + 1e-14)) / (dcutoff * dcutoff), DBL_MIN, DBL_MAX);
+  const double dcut_lo = dclamp(dcutoff, DBL_MIN, DBL_MAX), dcut_hi = DBL_MAX;
+  hset_t seen = {0};
+  hset_insert(&seen, eqrefl_rep(sg, 0, 0, 0));
+  int nf = 0, capf = 0, err = 0;
+  fam_t* fams = NULL;
+  for (int h = 0; h <= max_h && !err && ns; ++h) {
+    for (int k = (h ? -max_k : 0); k <= max_k && !err; ++k) {
+      for (int l = -max_l; l <= max_l; ++l) {
+        const hkl3 key = eqrefl_rep(sg, h, k, l);
+        if (!hset_insert(&seen, key)) continue;
+        const double dh = (double)key.h, dk = (double)key.k, dl = (double)key.l;
+        const double wx = m[0] * dh + m[1] * dk + m[2] * dl;
+        const double wy = m[3] * dh + m[4] * dk + m[5] * dl;
+        const double wz = m[6] * dh + m[7] * dk + m[8] * dl;
+        const double ksq = wx * wx + wy * wy + wz * wz;
+        if (!in_interval(ksq_lo, ksq_hi, ksq)) continue;
+        const double kk2 = 0.5 * ksq;
+        double upper = 0.0;
+        for (int i = 0; i < ns; ++i) {
+          const double w = kk2 * vmsd[i];
+          if (w > thr[i]) { fac[i] = 0.0; continue; }
+          const double f = vcsl[i] * exp(-w);
+          fac[i] = f;
+          upper += np[i] * fabs(f);
+        }
+        if (upper * upper * 2.0 < fsquarecut) continue;
+        ssum_t re = {0, 0}, im = {0, 0};
+        for (int i = 0; i < ns; ++i) {
+          const double f = fac[i];
+          if (!f) continue;
+          ssum_t cs = {0, 0}, sn = {0, 0};
+          const double* p = pos[i];
+          for (int j = 0; j < np[i]; ++j) {
+            const double ph = dh * p[3 * j] + dk * p[3 * j + 1] + dl * p[3 * j + 2];
+            double sv, cv;
+            orc_sincos_2pix(ph, &sv, &cv);
+            ss_add(&cs, cv); ss_add(&sn, sv);
+          }
+          ss_add(&re, ss_sum(&cs) * f);
+          ss_add(&im, ss_sum(&sn) * f);
+        }
+        const double r = ss_sum(&re), im2 = ss_sum(&im);
+        const double fsq = r * r + im2 * im2;
+        if (fsq < fsquarecut) continue;
+        const double dsp = K_2PI / sqrt(ksq);
+        if (!in_interval(dcut_lo, dcut_hi, dsp)) continue;
+        if (nf > 1000000) { err = -2; break; }
+        if (nf == capf) { capf = capf ? capf * 2 : 1024; fams = (fam_t*)realloc(fams, sizeof(fam_t) * (size_t)capf); }
+        fam_t* f = &fams[nf++];
+        f->members = (hkl3*)malloc(sizeof(hkl3) * 24);
+        f->nmem = eqrefl_list(sg, key, f->members);
+        f->cap = 24;
+        f->rep = f->members[0];
+        f->mult = 2 * f->nmem;
+        f->fsq = fsq; f->dsp = dsp;
+      }
+    }
+  }
+  int result = err;
+  if (!err) {
+    stable_sort_fams(fams, nf);
+    if (nf > maxout) {
+      result = -1;
+    } else {
+      for (int i = 0; i < nf; ++i) {
+        out_hkl[3 * i] = fams[i].rep.h; out_hkl[3 * i + 1] = fams[i].rep.k; out_hkl[3 * i + 2] = fams[i].rep.l;
+        out_mult[i] = fams[i].mult; out_dsp[i] = fams[i].dsp; out_fsq[i] = fams[i].fsq;
+      }
+      result = nf;
+      if (out_mem_off) {
+        int64_t tot = 0;
+        for (int i = 0; i < nf; ++i) tot += fams[i].nmem;
+        if (tot > maxmem) {
+          result = -1;
+        } else {
+          int64_t o = 0;
+          for (int i = 0; i < nf; ++i) {
+            out_mem_off[i] = o;
+            for (int j = 0; j < fams[i].nmem; ++j, ++o) {
+              out_mem_hkl[3 * o] = fams[i].members[j].h;
+              out_mem_hkl[3 * o + 1] = fams[i].members[j].k;
+              out_mem_hkl[3 * o + 2] = fams[i].members[j].l;
+            }
+          }
+          out_mem_off[nf] = o;
+        }
+      }
+    }
+  }
+  for (int i = 0; i < nf; ++i) free(fams[i].members);
+  free(fams);
+  free(seen.keys);
+  free(pos); free(np); free(vcsl); free(vmsd); free(thr); free(fac);
+  return result;
+}
+
 /* ------------------------------------------------------------------ */
 /* process objects */
 enum { P_PB = 1, P_ELINC = 2, P_SAB = 3, P_COMP = 4, P_SCB = 5 };

@@ -54,6 +54,22 @@ int orc_hkl_orthogonal_members(double a, double b, double c,
                                double* out_dsp, double* out_fsq,
                                int64_t* out_mem_off, int* out_mem_hkl, int64_t maxmem);
 
+/* ---- hkl enumeration with a space group (1..230): reflections grouped by
+   symmetry equivalence, NCFillHKL.cc:545-709 with NCEqRefl.cc:25-150.  Cells
+   with all angles 90, alpha = beta = 90 and gamma = 120, or alpha = gamma =
+   90 (angles in degrees).  Returns the number of families (sorted like the
+   reference's InfoBuilder; members = the equivalent reflections, one of each
+   (x, -x) pair), -1 if maxout/maxmem are too small, -2 if the 1e6-family
+   guard fires, -3 for an unsupported cell, -4 for a bad space group. */
+int orc_hkl_spacegroup_members(int sg, double a, double b, double c, double alpha_deg, double beta_deg,
+                               double gamma_deg, int nspecies, const int* natoms, const double* positions,
+                               const double* csl, const double* msd, double dcutoff, double fsquarecut,
+                               int maxout, int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq,
+                               int64_t* out_mem_off, int* out_mem_hkl, int64_t maxmem);
+/* reciprocal lattice (row-major 3x3) of such a cell; -3 for other cells */
+int orc_reci_lattice(double a, double b, double c, double alpha_deg, double beta_deg, double gamma_deg,
+                     double* m);
+
 /* the hkl family sort alone (test hook): perm[i] = input index of the i-th
    family in the reference's order */
 void orc_debug_sort_fams(int n, const double* dsp, const double* fsq, const int* mult, const int* hkl,

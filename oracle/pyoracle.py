@@ -63,6 +63,13 @@ def lib():
                                                                           _dp, _dp, _i64p, _ip, ctypes.c_int64]
         L.orc_hkl_orthogonal_members.restype = ctypes.c_int
         L.orc_debug_sort_fams.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip, _ip]
+        L.orc_hkl_spacegroup_members.argtypes = ([ctypes.c_int] + [ctypes.c_double] * 6 +
+                                                 [ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_double,
+                                                  ctypes.c_double, ctypes.c_int, _ip, _ip, _dp, _dp, _i64p, _ip,
+                                                  ctypes.c_int64])
+        L.orc_hkl_spacegroup_members.restype = ctypes.c_int
+        L.orc_reci_lattice.argtypes = [ctypes.c_double] * 6 + [_dp]
+        L.orc_reci_lattice.restype = ctypes.c_int
         L.orc_create_scbragg.argtypes = [_dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _i64p, _ip, _dp,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_create_scbragg.restype = _vp
@@ -125,6 +132,37 @@ def alpha_limits(ediv, beta):
     b = ctypes.c_double()
     lib().orc_alpha_limits(ediv, beta, ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value
+
+
+def hkl_spacegroup_members(spacegroup, cell, species, dcutoff, fsquarecut=1e-5, maxout=2000000, maxmem=8000000):
+    """cell = (a, b, c, alpha, beta, gamma), angles in degrees; species as for
+    hkl_orthogonal.  Returns (hkl, mult, dsp, fsq, mem_off, mem_hkl)."""
+    natoms = np.array([len(s[2]) for s in species], dtype=np.int32)
+    pos = np.ascontiguousarray(np.concatenate([np.asarray(s[2], dtype=np.float64).reshape(-1, 3)
+                                               for s in species]).ravel())
+    csl = np.array([s[0] for s in species], dtype=np.float64)
+    msd = np.array([s[1] for s in species], dtype=np.float64)
+    hkl = np.empty(3 * maxout, dtype=np.int32)
+    mult = np.empty(maxout, dtype=np.int32)
+    dsp = np.empty(maxout)
+    fsq = np.empty(maxout)
+    moff = np.empty(maxout + 1, dtype=np.int64)
+    mhkl = np.empty(3 * maxmem, dtype=np.int32)
+    n = lib().orc_hkl_spacegroup_members(int(spacegroup), *[float(x) for x in cell], len(species), _i(natoms),
+                                         _d(pos), _d(csl), _d(msd), dcutoff, fsquarecut, maxout, _i(hkl), _i(mult),
+                                         _d(dsp), _d(fsq), moff.ctypes.data_as(_i64p), _i(mhkl), maxmem)
+    if n < 0:
+        raise RuntimeError(f"oracle space-group hkl enumeration failed ({n})")
+    nm = int(moff[n])
+    return (hkl[:3 * n].reshape(n, 3).copy(), mult[:n].copy(), dsp[:n].copy(), fsq[:n].copy(),
+            moff[:n + 1].copy(), mhkl[:3 * nm].reshape(nm, 3).copy())
+
+
+def reci_lattice(cell):
+    m = np.empty(9)
+    if lib().orc_reci_lattice(*[float(x) for x in cell], _d(m)):
+        raise RuntimeError("unsupported cell")
+    return m
 
 
 def hkl_orthogonal(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tolerance=1e-6, maxout=2000000):

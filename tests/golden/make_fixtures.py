@@ -7,6 +7,8 @@ Sources:
   tests/scripts/nosg.log                 Al hkl families, no space group, dcutoff 0.4
   tests/src/app_debyemsd/main.cc,test.log  sqrt(MSD) table (Debye model)
   tests/src/app_sabutils/main.cc:88-135   mpmath known answers of the alpha-bin integral
+  tests/scripts/ncmat2hkl.log + data/*.ncmat  hkl tables of crystals with a space
+                                         group (10 significant digits) with their inputs
 """
 import json
 import pathlib
@@ -82,7 +84,87 @@ def sabutils():
     return len(pairs)
 
 
+def _frac(t):
+    if "/" in t:
+        num, den = t.split("/")
+        return float(num) / float(den)
+    return float(t)
+
+
+def _ncmat_positions(path):
+    """@ATOMPOSITIONS of an NCMAT file: {element: [[x, y, z], ...]} in file order"""
+    out, on = {}, False
+    for raw in path.read_text().splitlines():
+        line = raw.split("#")[0].strip()
+        if not line:
+            continue
+        if line.startswith("@"):
+            on = line.startswith("@ATOMPOSITIONS")
+            continue
+        if on:
+            t = line.split()
+            out.setdefault(t[0], []).append([_frac(x) for x in t[1:4]])
+    return out
+
+
+def spacegroup_hkl():
+    """laz tables of tests/scripts/ncmat2hkl.log: per material the header
+    values (space group, cell, temperature, effective Debye temperature),
+    the per-element coherent scattering lengths and masses of the log's info
+    dump, the NCMAT file's atom positions and the table (h k l mult d F2)."""
+    lines = (REF / "tests/scripts/ncmat2hkl.log").read_text().splitlines()
+    heads = [i for i, l in enumerate(lines) if l.startswith("====> Conversion of stdlib::") and " to laz " in l]
+    mats = {}
+    for i0 in heads:
+        m = re.match(r"====> Conversion of stdlib::(\S+?\.ncmat)(?:;dcutoff=([0-9.]+))? to laz", lines[i0])
+        fname, dcut = m.group(1), m.group(2)
+        if dcut is None:
+            continue  # automatic dcutoff: not a fixed input
+        hdr, table, atoms, snipped, tail = {}, [], {}, 0, []
+        j = i0 + 1
+        while not lines[j].startswith("====> NCrystal dump of converted laz"):
+            l = lines[j]
+            mh = re.match(r"^(?:STDOUT|res|OUTFILE)># (\w+) ([-0-9.eE+]+)", l)
+            if mh and mh.group(1) in ("temperature", "spacegroup", "lattice_a", "lattice_b", "lattice_c",
+                                      "lattice_aa", "lattice_bb", "lattice_cc", "debye_temperature",
+                                      "average_mass"):
+                hdr[mh.group(1)] = float(mh.group(2))
+            mt = re.match(r"^(?:STDOUT|res|OUTFILE)>\s+(-?\d+)\s+(-?\d+)\s+(-?\d+)\s+(\d+)\s+([0-9.eE+-]+)\s+([0-9.eE+-]+)\s*$", l)
+            if mt:
+                row = [int(mt.group(1)), int(mt.group(2)), int(mt.group(3)), int(mt.group(4)),
+                       float(mt.group(5)), float(mt.group(6))]
+                (tail if snipped else table).append(row)
+            ms = re.match(r"^(?:STDOUT|res|OUTFILE)>\s+<<<SNIPPED (\d+) LINES", l)
+            if ms:
+                snipped = int(ms.group(1))
+            j += 1
+        while not lines[j].startswith("Atom data:"):
+            j += 1
+        j += 1
+        while lines[j].startswith("   "):
+            ma = re.match(r"^\s+(\w+)\s+= \w+\(cohSL=([-0-9.eE+]+)fm .* mass=([0-9.eE+-]+)u", lines[j])
+            atoms[ma.group(1)] = dict(csl_fm=float(ma.group(2)), mass=float(ma.group(3)))
+            j += 1
+        pos = _ncmat_positions(REF / "data" / fname)
+        species = [dict(element=e, csl_fm=atoms[e]["csl_fm"], mass=atoms[e]["mass"], positions=p)
+                   for e, p in pos.items()]
+        mats[fname] = dict(spacegroup=int(hdr["spacegroup"]), dcutoff=float(dcut),
+                           cell=[hdr["lattice_a"], hdr["lattice_b"], hdr["lattice_c"], hdr["lattice_aa"],
+                                 hdr["lattice_bb"], hdr["lattice_cc"]],
+                           temperature=hdr["temperature"], debye_temperature=hdr["debye_temperature"],
+                           average_mass=hdr["average_mass"], species=species, table=table,
+                           snipped=snipped, tail=tail)
+    (OUT / "hkl_spacegroup.json").write_text(json.dumps(dict(
+        source="tests/scripts/ncmat2hkl.log (laz tables at 10 significant digits, header values, info-dump "
+               "atom data) with atom positions from data/<file>.ncmat; F2 in barn, d in Aa, csl in fm. The "
+               "reference's MSDs come from each element's VDOS; debye_temperature is the effective, "
+               "sigma-weighted value the log prints. The log shows long tables as head rows ('table'), "
+               "a count of snipped rows and tail rows ('tail').", materials=mats), indent=1))
+    return {k: len(v["table"]) + v["snipped"] + len(v["tail"]) for k, v in mats.items()}
+
+
 if __name__ == "__main__":
+    print("space-group hkl tables:", spacegroup_hkl())
     print("nosg families:", nosg())
     print("debye msd values:", debyemsd())
     print("alpha integral known answers:", sabutils())

@@ -1,0 +1,105 @@
+"""hkl enumeration with a space group (NCrystal's calculateHKLPlanesWithSymEqRefl,
+ncrystal_core/src/extd_utils/NCFillHKL.cc:545-709, equivalent reflections of
+ncrystal_core/src/phys_utils/NCEqRefl.cc:25-150) in the CPU checker, pinned by
+
+* the reference's own tables in tests/scripts/ncmat2hkl.log (committed as
+  tests/golden/hkl_spacegroup.json): family count, hkl and multiplicity exact,
+  d-spacings to the 10 printed digits, for cubic (225), monoclinic (15,
+  beta = 122 degrees), orthorhombic (62), tetragonal (129), trigonal (167)
+  and hexagonal (186) crystals; |F|^2 to the printed digits for Al, whose
+  mean-squared displacement the log's effective Debye temperature fixes
+  (the multi-element materials' per-element MSDs come from VDOS curves the
+  log does not give, so their |F|^2 is not compared);
+* brute force for every Laue class: with one atom per cell no reflection is
+  extinct, so the families must partition the half-space of (h,k,l) with
+  d >= dcutoff into orbits of equal d, each member listed once.
+"""
+import json
+import math
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+FIX = json.loads((GOLDEN / "hkl_spacegroup.json").read_text())["materials"]
+
+
+def _species(oracle, m):
+    single = len(m["species"]) == 1
+    return [(s["csl_fm"] / 10.0,
+             oracle.debye_msd(m["debye_temperature"], m["temperature"], m["average_mass"] if single else s["mass"]),
+             s["positions"]) for s in m["species"]]
+
+
+SUPPORTED = [k for k, m in FIX.items() if not (m["cell"][3] != 90 and m["cell"][5] != 90)]
+
+
+@pytest.mark.parametrize("name", SUPPORTED)
+def test_reference_tables(oracle, name):
+    m = FIX[name]
+    hkl, mult, dsp, fsq, moff, mhkl = oracle.hkl_spacegroup_members(m["spacegroup"], m["cell"], _species(oracle, m),
+                                                                     m["dcutoff"])
+    head = np.array(m["table"])
+    tail = np.array(m["tail"]).reshape(-1, 6)
+    assert len(dsp) == len(head) + m["snipped"] + len(tail)
+    idx = list(range(len(head))) + list(range(len(dsp) - len(tail), len(dsp)))
+    rows = np.vstack([head, tail])
+    assert np.array_equal(hkl[idx], rows[:, :3].astype(int))
+    assert np.array_equal(mult[idx], rows[:, 3].astype(int))
+    assert np.max(np.abs(dsp[idx] - rows[:, 4]) / rows[:, 4]) < 1e-9
+    if len(m["species"]) == 1:
+        assert np.max(np.abs(fsq[idx] - rows[:, 5]) / rows[:, 5]) < 1e-9
+    # members: mult/2 distinct reflections per family, representative first
+    assert np.array_equal(np.diff(moff), mult // 2)
+    assert np.array_equal(mhkl[moff[:-1]], hkl)
+
+
+def test_general_cell_is_refused(oracle):
+    m = FIX["CaSiO3_sg2_Wollastonite.ncmat"]
+    with pytest.raises(RuntimeError, match="-3"):
+        oracle.hkl_spacegroup_members(m["spacegroup"], m["cell"], _species(oracle, m), m["dcutoff"])
+    with pytest.raises(RuntimeError, match="-4"):
+        oracle.hkl_spacegroup_members(231, (4.0, 4.0, 4.0, 90, 90, 90), [(0.5, 0.001, [[0, 0, 0]])], 1.0)
+
+
+# (space group, cell) per Laue class; cells with the metric the class needs
+LAUE = [
+    (1, (3.1, 3.7, 4.3, 90, 90, 90)),
+    (14, (3.1, 3.7, 4.3, 90, 104.5, 90)),
+    (62, (3.1, 3.7, 4.3, 90, 90, 90)),
+    (80, (3.3, 3.3, 4.1, 90, 90, 90)),
+    (129, (3.3, 3.3, 4.1, 90, 90, 90)),
+    (146, (3.2, 3.2, 4.4, 90, 90, 120)),
+    (166, (3.2, 3.2, 4.4, 90, 90, 120)),
+    (173, (3.2, 3.2, 4.4, 90, 90, 120)),
+    (194, (3.2, 3.2, 4.4, 90, 90, 120)),
+    (200, (3.6, 3.6, 3.6, 90, 90, 90)),
+    (225, (3.6, 3.6, 3.6, 90, 90, 90)),
+]
+ORDER = {1: 1, 14: 2, 62: 4, 80: 4, 129: 8, 146: 3, 166: 6, 173: 6, 194: 12, 200: 12, 225: 24}
+
+
+@pytest.mark.parametrize("sg,cell", LAUE)
+def test_orbits_partition_the_reflections(oracle, sg, cell):
+    dcut = 0.9
+    hkl, mult, dsp, fsq, moff, mhkl = oracle.hkl_spacegroup_members(sg, cell, [(0.5, 1e-4, [[0.0, 0.0, 0.0]])],
+                                                                     dcut, fsquarecut=1e-12)
+    m = oracle.reci_lattice(cell).reshape(3, 3)
+    # every member has its family's d, and no reflection is listed twice
+    dmem = 2 * math.pi / np.linalg.norm(mhkl @ m.T, axis=1)
+    fam = np.repeat(np.arange(len(dsp)), np.diff(moff))
+    assert np.max(np.abs(dmem - dsp[fam]) / dsp[fam]) < 1e-12
+    key = {tuple(x) for x in mhkl}
+    assert len(key) == len(mhkl)
+    # the (h,k,l) half-space (one of each +-pair) with d >= dcutoff, brute force
+    n = 12
+    g = np.array([(h, k, l) for h in range(-n, n + 1) for k in range(-n, n + 1) for l in range(-n, n + 1)
+                  if (h, k, l) > (0, 0, 0)])
+    dd = 2 * math.pi / np.linalg.norm(g @ m.T, axis=1)
+    inside = {tuple(x) for x in g[dd >= dcut]}
+    # members are stored as the form of (x, -x) whose first non-zero index is positive
+    assert all(tuple(x) > (0, 0, 0) for x in mhkl)
+    assert key == inside
+    # orbit sizes divide the order of the Laue group (as +-pairs)
+    assert all(ORDER[sg] % (k // 2) == 0 for k in mult)
