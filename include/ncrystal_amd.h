@@ -107,7 +107,25 @@ int ncamd_reci_lattice(double a, double b, double c, double alpha, double beta, 
  * (ncrystal_core/src/phys_utils/NCDebyeMSD.cc:27-36). */
 double ncamd_debye_msd(double debye_temp, double temperature, double mass_amu);
 
-/* PowderBragg from (d-spacing, |F|^2*multiplicity) pairs; replaces
+/* FreeGas scatter process: replaces FreeGas(Temperature, AtomMass, SigmaFree)
+ * (ncrystal_core/src/freegas/NCFreeGas.cc:25-77): cross section of
+ * FreeGasXSProvider (ncrystal_core/src/phys_utils/NCFreeGasUtils.cc:28-120),
+ * isotropic scattering by FreeGasSampler::sampleDeltaEMu
+ * (ncrystal_core/include/NCrystal/internal/phys_utils/NCFreeGasUtils.hh:146-160).
+ * temperature [K], target mass [amu], free scattering cross section [barn]
+ * (NCrystal's SigmaBound::free: sigma_bound * (A/(1+A))^2).  Evaluated on
+ * its own, not inside compositions or many-material batches.  Sampling
+ * refuses energies that are not finite and >= 0 (poisoned outputs, counted
+ * as errors). */
+ncamd_process_t ncamd_create_freegas(double temperature, double mass_amu, double sigma_free);
+
+/* 1/v absorption process: replaces AbsOOV(SigmaAbsorption)
+ * (ncrystal_core/src/absoov/NCAbsOOV.cc:30-44): xs = sigma_abs
+ * sqrt(E_2200m/s / E), infinite at E = 0.  Cross-section calls only; the
+ * sampling calls raise BadInput. */
+ncamd_process_t ncamd_create_absoov(double sigma_abs);
+
+/* PowderBragg from (d-spacing, |F|^2*multiplicity) pairs; replaces/* PowderBragg from (d-spacing, |F|^2*multiplicity) pairs; replaces
  * PowderBragg(double v0_times_natoms, VectDFM&&)
  * (ncrystal_core/src/powderbragg/NCPowderBragg.cc:117-120,68-110).
  * fp32_tables != 0 additionally builds FP32 Kahan-compensated cumulative sums

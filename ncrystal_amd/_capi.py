@@ -99,6 +99,8 @@ def lib():
                                           ctypes.c_int64, _ip, _ip, _dp, _dp, _i64p, ctypes.c_int64, _i64p, _ip,
                                           _i64p]),
         "ncamd_reci_lattice": (ctypes.c_int, [ctypes.c_double] * 6 + [_dp]),
+        "ncamd_create_freegas": (Handle, [ctypes.c_double] * 3),
+        "ncamd_create_absoov": (Handle, [ctypes.c_double]),
         "ncamd_create_scbragg": (Handle, [ctypes.c_double, ctypes.c_int64, _dp, _dp, _i64p, _ip, _dp, _dp,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "ncamd_scbragg_info": (ctypes.c_int, [Handle, _i64p, _i64p, _dp, _dp]),

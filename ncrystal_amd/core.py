@@ -315,6 +315,19 @@ def create_elincscatter(msd, bound_incoh_xs, scale):
     return Scatter(h, name="ElIncScatter")
 
 
+def create_freegas(temperature, mass_amu, sigma_free):
+    """FreeGas scatter process (NCrystal's FreeGas(Temperature, AtomMass,
+    SigmaFree)); sigma_free in barn."""
+    h = C.call("ncamd_create_freegas", float(temperature), float(mass_amu), float(sigma_free))
+    return Scatter(h, name="FreeGas")
+
+
+def create_absoov(sigma_abs):
+    """1/v absorption process (NCrystal's AbsOOV); cross sections only."""
+    h = C.call("ncamd_create_absoov", float(sigma_abs))
+    return Process(h, name="AbsOOV")
+
+
 class SABScatter(Scatter):
     def egrid(self):
         n = C.call("ncamd_sab_egrid", self._h, None, None, 0)

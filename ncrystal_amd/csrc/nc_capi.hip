@@ -200,7 +200,7 @@ constexpr uint32_t kMagic = 0x6e63616dU;
 struct Proc {
   uint32_t magic = kMagic;
   int refcount = 1;
-  int kind = kKindNone;  // kKindPB, kKindElInc, kKindSAB or 0 for composition
+  int kind = kKindNone;  // kKind* of a single process, or kKindNone for a composition
   uint64_t seed = 0;
   uint64_t callctr = 0;
   // owned device memory
@@ -217,6 +217,8 @@ struct Proc {
   PBView pb{};
   ElIncView ei{};
   SABView sab{};
+  FGView fg{};         // FreeGas process
+  double abs_c = 0.0;  // 1/v absorption: xs = abs_c / sqrt(E)
   double pb_v0n = 0.0;
   std::vector<double> pb_e2d_h, pb_fdm_h;
   std::vector<float> pb_fdm32_h;
@@ -297,6 +299,8 @@ CompView make_view(Proc* p) {
       case kKindElInc: v.ei = c->ei; break;
       case kKindSAB: v.sab = c->sab; break;
       case kKindSCB: raise("BadInput", "oriented process: use the calls with neutron directions");
+      case kKindFG:
+      case kKindAbs: raise("BadInput", "FreeGas and absorption processes are evaluated on their own");
       default: raise("LogicError", "bad component kind");
     }
     if (dlo < dhi) { lo = std::min(lo, dlo); hi = std::max(hi, dhi); }
@@ -1899,8 +1903,48 @@ void comp_queues(Proc* p, int64_t total, hipStream_t st) {
   if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
 }
 
+void check_fp32(Proc* p) {
+  CompView v = make_view(p);
+  for (int i = 0; i < v.n; ++i)
+    if (v.kind[i] == kKindPB && !v.pb.fdm32)
+      raise("BadInput", "process was created without FP32 PowderBragg tables");
+}
+
+// cross sections of any isotropic process or composition (the direction
+// does not enter)
+void xs_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat, double* out, bool fp32, hipStream_t st) {
+  if (p->kind == kKindFG || p->kind == kKindAbs) {
+    if (fp32) raise("BadInput", "process was created without FP32 PowderBragg tables");
+    launch_simple_xs(p->kind, p->fg, p->abs_c, ekin, n, repeat, out, st);
+    return;
+  }
+  if (fp32) check_fp32(p);
+  launch_xs_iso(make_view(p), ekin, n, repeat, out, fp32, st);
+}
+
+// scattering with new directions for an isotropic process or composition
+void sample_dirs_dispatch(Proc* p, const double* ekin, const double* dirs, int64_t n, double* oE, double* oD,
+                          hipStream_t st) {
+  if (p->kind == kKindAbs) raise("BadInput", "an absorption process does not scatter");
+  if (p->kind == kKindFG) {
+    launch_fg_sample(p->fg, ekin, dirs, n, 1, p->seed, next_stream(p), oE, oD, p->d_err.as<unsigned long long>(),
+                     st);
+    return;
+  }
+  const CompView v = make_view(p);
+  comp_queues(p, n, st);
+  launch_sample_dirs(v, ekin, dirs, n, p->seed, next_stream(p), oE, oD, p->d_err.as<unsigned long long>(),
+                     p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), st);
+}
+
 void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat, double* oE, double* oMu,
                          hipStream_t st) {
+  if (p->kind == kKindAbs) raise("BadInput", "an absorption process does not scatter");
+  if (p->kind == kKindFG) {
+    launch_fg_sample(p->fg, ekin, nullptr, n, repeat, p->seed, next_stream(p), oE, oMu,
+                     p->d_err.as<unsigned long long>(), st);
+    return;
+  }
   const uint32_t stream = next_stream(p);
   Proc* single = p->kind == kKindSAB ? p : nullptr;
   if (!single && p->kind == kKindNone && p->comps.size() == 1 && p->comps[0]->kind == kKindSAB &&
@@ -1961,12 +2005,6 @@ ScbWork scb_work(Proc* p, int64_t n, hipStream_t st) {
   return w;
 }
 
-void check_fp32(Proc* p) {
-  CompView v = make_view(p);
-  for (int i = 0; i < v.n; ++i)
-    if (v.kind[i] == kKindPB && !v.pb.fdm32)
-      raise("BadInput", "process was created without FP32 PowderBragg tables");
-}
 
 }  // namespace
 }  // namespace ncamd
@@ -2110,6 +2148,45 @@ ncamd_process_t ncamd_create_elincscatter(int n, const double* msd, const double
   NCAMD_TRY return wrap(create_elinc(n, msd, bixs, scale)); NCAMD_CATCH
   return wrap(nullptr);
 }
+ncamd_process_t ncamd_create_freegas(double temperature, double mass_amu, double sigma_free) {
+  NCAMD_TRY
+  ensure_device();
+  if (!(temperature > 0.0) || !std::isfinite(temperature)) raise("BadInput", "temperature must be > 0");
+  if (!(mass_amu > 0.0) || !std::isfinite(mass_amu)) raise("BadInput", "target mass must be > 0");
+  if (!(sigma_free > 0.0) || !std::isfinite(sigma_free)) raise("BadInput", "free cross section must be > 0");
+  std::unique_ptr<Proc> p(new Proc());
+  p->kind = kKindFG;
+  const double kT = kBoltzmann * temperature;
+  p->fg.T = temperature;
+  p->fg.mass = mass_amu;
+  p->fg.sigma_free = sigma_free;
+  p->fg.ca = (mass_amu / kNeutronMassAmu) / kT;  // NCFreeGasUtils.cc:45-53: a^2 = ca E
+  std::vector<double> et = erfc_table();
+  DevBuf* bet = p->newbuf();
+  bet->upload(et.data(), et.size());
+  p->fg.erfc_tab = bet->as<double>();
+  init_errbuf(p.get());
+  return wrap(p.release());
+  NCAMD_CATCH
+  return wrap(nullptr);
+}
+
+ncamd_process_t ncamd_create_absoov(double sigma_abs) {
+  NCAMD_TRY
+  ensure_device();
+  if (!(sigma_abs >= 0.0) || !std::isfinite(sigma_abs)) raise("BadInput", "absorption cross section must be >= 0");
+  std::unique_ptr<Proc> p(new Proc());
+  p->kind = kKindAbs;
+  // const_ekin_2200m_s (ncrystal_core/include/NCrystal/internal/utils/NCMath.hh:45-46)
+  const double mevc2 = kNeutronMassAmu * kDalton2eVc2 / (kSpeedOfLightAas * kSpeedOfLightAas);
+  const double ekin2200 = 0.5 * mevc2 * 2200.0e10 * 2200.0e10;
+  p->abs_c = std::sqrt(ekin2200) * sigma_abs;
+  init_errbuf(p.get());
+  return wrap(p.release());
+  NCAMD_CATCH
+  return wrap(nullptr);
+}
+
 ncamd_process_t ncamd_create_sabscatter(int na, int nb, const double* alpha, const double* beta, const double* sab,
                                         double T, double mass, double sigma_bound, double suggested_emax,
                                         int n_egrid, const double* egrid) {
@@ -2130,6 +2207,8 @@ ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, co
     if (c->kind == kKindNone) raise("BadInput", "nested compositions are not supported");
     if (c->kind == kKindSCB)
       raise("BadInput", "an SCBragg process is evaluated on its own (oriented), not inside a composition");
+    if (c->kind == kKindFG || c->kind == kKindAbs)
+      raise("BadInput", "FreeGas and absorption processes are evaluated on their own, not inside a composition");
     if (seen[c->kind]++) raise("BadInput", "at most one process of each kind per composition");
     if (!(scales[i] >= 0.0) || std::isinf(scales[i])) raise("BadInput", "component scales must be finite and >= 0");
     // null processes and zero scales are dropped (NCProcImpl.cc:263-283)
@@ -2232,12 +2311,10 @@ static void xs_many_impl(ncamd_process_t h, const double* ekin, unsigned long n,
   NCAMD_TRY
   ensure_device();
   Proc* p = extract(h);
-  if (fp32) check_fp32(p);
-  const CompView v = make_view(p);
   p->st_in.ensure(sizeof(double) * n);
   p->st_out1.ensure(sizeof(double) * n * repeat);
   if (n) NCAMD_HIP(hipMemcpy(p->st_in.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
-  launch_xs_iso(v, p->st_in.as<double>(), (int64_t)n, (int64_t)repeat, p->st_out1.as<double>(), fp32, 0);
+  xs_dispatch(p, p->st_in.as<double>(), (int64_t)n, (int64_t)repeat, p->st_out1.as<double>(), fp32, 0);
   NCAMD_HIP(hipGetLastError());
   if (n * repeat) NCAMD_HIP(hipMemcpy(out, p->st_out1.p, sizeof(double) * n * repeat, hipMemcpyDeviceToHost));
   ok = true;
@@ -2260,13 +2337,11 @@ void ncamd_samplescatterisotropic_many(ncamd_process_t h, const double* ekin, un
   NCAMD_TRY
   ensure_device();
   Proc* p = extract(h);
-  const CompView v = make_view(p);
   const size_t tot = (size_t)n * repeat;
   p->st_in.ensure(sizeof(double) * n);
   p->st_out1.ensure(sizeof(double) * tot);
   p->st_out2.ensure(sizeof(double) * tot);
   if (n) NCAMD_HIP(hipMemcpy(p->st_in.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
-  (void)v;
   sample_iso_dispatch(p, p->st_in.as<double>(), (int64_t)n, (int64_t)repeat, p->st_out1.as<double>(),
                       p->st_out2.as<double>(), 0);
   NCAMD_HIP(hipGetLastError());
@@ -2290,8 +2365,6 @@ void ncamd_samplescatter_many_dirs(ncamd_process_t h, const double* ekin, const 
   ensure_device();
   Proc* p = extract(h);
   const bool scb = p->kind == kKindSCB;
-  CompView v{};
-  if (!scb) v = make_view(p);
   p->st_in.ensure(sizeof(double) * n);
   p->st_in2.ensure(sizeof(double) * 3 * n);
   p->st_out1.ensure(sizeof(double) * n);
@@ -2305,10 +2378,8 @@ void ncamd_samplescatter_many_dirs(ncamd_process_t h, const double* ekin, const 
     launch_scb_sample(p->scb, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->seed, next_stream(p),
                       p->st_out1.as<double>(), p->st_out2.as<double>(), w, 0);
   } else {
-    comp_queues(p, (int64_t)n, 0);
-    launch_sample_dirs(v, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->seed, next_stream(p),
-                       p->st_out1.as<double>(), p->st_out2.as<double>(), p->d_err.as<unsigned long long>(),
-                       p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), 0);
+    sample_dirs_dispatch(p, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->st_out1.as<double>(),
+                         p->st_out2.as<double>(), 0);
   }
   NCAMD_HIP(hipGetLastError());
   if (n) {
@@ -2368,7 +2439,7 @@ void ncamd_dev_crosssection_nonoriented_many(ncamd_process_t h, const double* d_
                                              unsigned long repeat, double* d_out, void* stream) {
   NCAMD_TRY
   Proc* p = extract(h);
-  launch_xs_iso(make_view(p), d_ekin, (int64_t)n, (int64_t)repeat, d_out, false, as_stream(stream));
+  xs_dispatch(p, d_ekin, (int64_t)n, (int64_t)repeat, d_out, false, as_stream(stream));
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH
 }
@@ -2376,8 +2447,7 @@ void ncamd_dev_crosssection_nonoriented_many_fp32(ncamd_process_t h, const doubl
                                                   unsigned long repeat, double* d_out, void* stream) {
   NCAMD_TRY
   Proc* p = extract(h);
-  check_fp32(p);
-  launch_xs_iso(make_view(p), d_ekin, (int64_t)n, (int64_t)repeat, d_out, true, as_stream(stream));
+  xs_dispatch(p, d_ekin, (int64_t)n, (int64_t)repeat, d_out, true, as_stream(stream));
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH
 }
@@ -2398,10 +2468,7 @@ void ncamd_dev_samplescatter_many_dirs(ncamd_process_t h, const double* d_ekin, 
     launch_scb_sample(p->scb, d_ekin, d_dirs, (int64_t)n, p->seed, next_stream(p), d_oE, d_oD, w,
                       as_stream(stream));
   } else {
-    comp_queues(p, (int64_t)n, as_stream(stream));
-    launch_sample_dirs(make_view(p), d_ekin, d_dirs, (int64_t)n, p->seed, next_stream(p), d_oE, d_oD,
-                       p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
-                       p->q_count.as<unsigned long long>(), as_stream(stream));
+    sample_dirs_dispatch(p, d_ekin, d_dirs, (int64_t)n, d_oE, d_oD, as_stream(stream));
   }
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH
@@ -2414,7 +2481,7 @@ void ncamd_dev_crosssection_many_dirs(ncamd_process_t h, const double* d_ekin, c
     const ScbWork w = scb_work(p, (int64_t)n, as_stream(stream));
     launch_scb_xs(p->scb, d_ekin, d_dirs, (int64_t)n, d_out, w, as_stream(stream));
   } else {
-    launch_xs_iso(make_view(p), d_ekin, (int64_t)n, 1, d_out, false, as_stream(stream));
+    xs_dispatch(p, d_ekin, (int64_t)n, 1, d_out, false, as_stream(stream));
   }
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH

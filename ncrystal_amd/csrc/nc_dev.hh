@@ -26,6 +26,8 @@ namespace ncamd {
 
 constexpr double kBoltzmann = 8.6173303e-5;
 constexpr double kNeutronMassAmu = 1.00866491588;
+constexpr double kDalton2eVc2 = 931494095.17;       // [eV/c^2]
+constexpr double kSpeedOfLightAas = 299792458e10;   // [Aa/s]
 constexpr double kPi = 3.1415926535897932384626433832795028841971694;
 constexpr double k2Pi = 6.2831853071795864769252867665590057683943388;
 constexpr double kPiHalf = 1.5707963267948966192313216916397514420985847;

@@ -356,6 +356,49 @@ struct FGSampler {
   }
 };
 
+// FreeGas process (ncrystal_core/src/freegas/NCFreeGas.cc:25-77): temperature,
+// target mass, free cross section, a^2 = ca E, and the erfc bound table
+struct FGView {
+  double T, mass, sigma_free, ca;
+  const double* erfc_tab;
+};
+// FreeGasXSProvider::crossSection (NCFreeGasUtils.hh:122-125)
+NCD double fg_xs(const FGView& g, double E) { return g.sigma_free * freegas_xs_shape(g.ca * E); }
+// AbsOOV::crossSectionIsotropic (ncrystal_core/src/absoov/NCAbsOOV.cc:39-43)
+NCD double absoov_xs(double c, double E) {
+  const double sE = sqrt(E);
+  return sE ? c / sE : __builtin_huge_val();
+}
+
+// FreeGasSampler::sampleDeltaEMu (NCFreeGasUtils.hh:146-160) with
+// convertAlphaBetaToDeltaEMu (NCKinUtils.cc:25-58) at the sampler's clamped
+// energy c kT, then FreeGas::sampleScatterIsotropic's E' = max(0, E + dE).
+// Energies that are not finite and >= 0 are refused (the rejection loops
+// need a finite c).
+NCD int fg_sample_demu(const FGView& g, Rng& rng, double E, double& Ef, double& mu) {
+  if (!(E >= 0.0 && E < __builtin_huge_val())) return kErrKinematics;
+  const FGSampler f(E, g.T, g.mass);
+  const double b = f.sample_beta(rng, g.erfc_tab);
+  double dE, m;
+  if (b <= -f.c || mu_iso_at_beta(b, f.c)) {
+    dE = b * f.kT;
+    m = rng.gen() * 2.0 - 1.0;
+  } else {
+    const double a = f.sample_alpha(b, rng);
+    const double ek = f.c * f.kT;
+    dE = b * f.kT;
+    const double ekf = ek + dE;
+    const double denom = 2.0 * sqrt(ek * ekf);
+    if (!denom) return kErrKinematics;
+    StableSum sm;
+    sm.add(ek); sm.add(ekf); sm.add(-a * f.kT);
+    m = dclamp(sm.sum() / denom, -1.0, 1.0);
+  }
+  Ef = dmax(0.0, E + dE);
+  mu = m;
+  return kOK;
+}
+
 NCD void freegas_sample_alpha_beta(double E, double T, double mass, const double* tab, Rng& rng,
                                    double& alpha, double& beta) {
   const FGSampler f(E, T, mass);

@@ -7,7 +7,7 @@
 
 namespace ncamd {
 
-enum : int { kKindNone = 0, kKindPB = 1, kKindElInc = 2, kKindSAB = 3, kKindSCB = 4 };
+enum : int { kKindNone = 0, kKindPB = 1, kKindElInc = 2, kKindSAB = 3, kKindSCB = 4, kKindFG = 5, kKindAbs = 6 };
 constexpr int kMaxComp = 4;
 
 // Device view of a process composition (passed by value as kernel argument).
@@ -231,6 +231,14 @@ void launch_sab_analyse(const SABBuildBufs&, const double* egrid, int ne, double
                         double* sfirst, double* scdf, double2* sxy, AInfo* sinfo, hipStream_t);
 // free-gas xs at n energies (sigma_free, ca)
 void launch_fg_xs(double sigma_free, double ca, const double* e, int n, double* out, hipStream_t);
+
+// FreeGas and 1/v absorption processes (nc_kernels.hip): cross sections
+// over the repeat layout; FreeGas sampling, with new directions when dirs
+// is given (oX = 3 doubles per neutron) or mu otherwise
+void launch_simple_xs(int kind, const FGView& g, double abs_c, const double* ekin, int64_t n, int64_t repeat,
+                      double* out, hipStream_t);
+void launch_fg_sample(const FGView& g, const double* ekin, const double* dirs, int64_t n, int64_t repeat,
+                      uint64_t seed, uint32_t stream, double* oE, double* oX, unsigned long long* nerr, hipStream_t);
 
 // hkl |F|^2 evaluation (nc_hkl.hip)
 struct HKLBuild {

@@ -69,6 +69,44 @@ __global__ void __launch_bounds__(256) k_sample_dirs(CompView c, const double* _
   }
 }
 
+// FreeGas (kKindFG) and 1/v absorption (kKindAbs) processes: cross section
+// over the repeat layout, and FreeGas sampling (isotropic, or with a new
+// direction from mu), one Philox stream per output slot.
+__global__ void __launch_bounds__(256) k_simple_xs(int kind, FGView g, double abs_c, const double* __restrict__ ekin,
+                                                   int64_t n, int64_t total, double* __restrict__ out) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gstride) {
+    const double E = ekin[(total == n) ? k : (k % n)];
+    out[k] = kind == kKindFG ? fg_xs(g, E) : absoov_xs(abs_c, E);
+  }
+}
+template <bool Dirs>
+__global__ void __launch_bounds__(256) k_fg_sample(FGView g, const double* __restrict__ ekin,
+                                                   const double* __restrict__ dirs, int64_t n, int64_t total,
+                                                   uint64_t seed, uint32_t stream, double* __restrict__ oE,
+                                                   double* __restrict__ oX, unsigned long long* __restrict__ nerr) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gstride) {
+    const int64_t i = (total == n) ? k : (k % n);
+    Rng rng;
+    rng.init(seed, stream, (uint64_t)k);
+    double Ef, mu;
+    const int rc = fg_sample_demu(g, rng, ekin[i], Ef, mu);
+    if (rc) atomicAdd(nerr, 1ull);
+    if constexpr (Dirs) {
+      double ox = 0.0, oy = 0.0, oz = 0.0;
+      if (rc) Ef = -1.0;
+      else rand_dir_given_mu(rng, mu, dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2], ox, oy, oz);
+      oE[k] = Ef;
+      oX[3 * k] = ox; oX[3 * k + 1] = oy; oX[3 * k + 2] = oz;
+    } else {
+      if (rc) { Ef = -1.0; mu = -999.0; }
+      oE[k] = Ef;
+      oX[k] = mu;
+    }
+  }
+}
+
 // Compositions with an S(alpha,beta) part, in stages so that no kernel
 // carries every sampler: k_comp_pick samples PowderBragg / ElInc picks in
 // place and queues the neutrons that pick S(alpha,beta); k_comp_sab<false>
@@ -270,6 +308,26 @@ static inline unsigned grid_for(int64_t n, int bs) {
   if (g < 1) g = 1;
   return (unsigned)g;
 }
+
+void launch_simple_xs(int kind, const FGView& g, double abs_c, const double* ekin, int64_t n, int64_t repeat,
+                      double* out, hipStream_t st) {
+  const int64_t total = n * repeat;
+  if (total <= 0) return;
+  hipLaunchKernelGGL(k_simple_xs, dim3(grid_for(total, 256)), dim3(256), 0, st, kind, g, abs_c, ekin, n, total, out);
+}
+void launch_fg_sample(const FGView& g, const double* ekin, const double* dirs, int64_t n, int64_t repeat,
+                      uint64_t seed, uint32_t stream, double* oE, double* oX, unsigned long long* nerr,
+                      hipStream_t st) {
+  const int64_t total = n * repeat;
+  if (total <= 0) return;
+  if (dirs)
+    hipLaunchKernelGGL(k_fg_sample<true>, dim3(grid_for(total, 256)), dim3(256), 0, st, g, ekin, dirs, n, total, seed,
+                       stream, oE, oX, nerr);
+  else
+    hipLaunchKernelGGL(k_fg_sample<false>, dim3(grid_for(total, 256)), dim3(256), 0, st, g, ekin, dirs, n, total,
+                       seed, stream, oE, oX, nerr);
+}
+
 
 void launch_xs_iso(const CompView& c, const double* ekin, int64_t n, int64_t repeat, double* out,
                    bool fp32, hipStream_t st) {

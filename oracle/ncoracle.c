@@ -859,7 +859,7 @@ int orc_hkl_spacegroup_members(int sg, double a, double b, double c, double alph
 
 /* ------------------------------------------------------------------ */
 /* process objects */
-enum { P_PB = 1, P_ELINC = 2, P_SAB = 3, P_COMP = 4, P_SCB = 5 };
+enum { P_PB = 1, P_ELINC = 2, P_SAB = 3, P_COMP = 4, P_SCB = 5, P_FG = 6, P_ABS = 7 };
 
 typedef struct {
   int n; double* e2d; double* fdm; double threshold;
@@ -894,11 +894,15 @@ typedef struct {
 } sab_t;
 
 typedef struct { int n; orc_proc** comps; double* scales; } comp_t;
+/* FreeGas process (ncrystal_core/src/freegas/NCFreeGas.cc:65-77) */
+typedef struct { double T, kT, mass, sigma_free, ca; } fg_t;
+/* AbsOOV: 1/v absorption (ncrystal_core/src/absoov/NCAbsOOV.cc:30-44) */
+typedef struct { double c; } abs_t;
 
 struct scb_s;
 struct orc_proc {
   int type;
-  union { pb_t pb; elinc_t ei; sab_t sab; comp_t comp; struct scb_s* scb; } u;
+  union { pb_t pb; elinc_t ei; sab_t sab; comp_t comp; struct scb_s* scb; fg_t fg; abs_t ab; } u;
 };
 
 /* ------------------------------------------------------------------ */
@@ -1744,8 +1748,12 @@ static int comp_in_domain(const orc_proc* p, double E) {
   }
   return (lo < hi) && (E >= lo) && (E <= hi);
 }
+double orc_freegas_xs_shape(double a_squared);
+static int fg_sample_demu(const fg_t* g, rng_t* rng, double E, double* Ef, double* mu);
 static double proc_xs(const orc_proc* p, double E) {
   switch (p->type) {
+    case P_FG: return p->u.fg.sigma_free * orc_freegas_xs_shape(p->u.fg.ca * E);
+    case P_ABS: { const double sE = sqrt(E); return sE ? p->u.ab.c / sE : INFINITY; }
     case P_PB: return pb_xs(&p->u.pb, E);
     case P_ELINC: return elinc_xs_commul(&p->u.ei, E, NULL);
     case P_SAB: return sab_xs(&p->u.sab, E);
@@ -1764,6 +1772,8 @@ static double proc_xs(const orc_proc* p, double E) {
 }
 static int proc_sample_iso(const orc_proc* p, rng_t* rng, double E, double* Ef, double* mu) {
   switch (p->type) {
+    case P_FG: return fg_sample_demu(&p->u.fg, rng, E, Ef, mu);
+    case P_ABS: return -8;
     case P_PB: *Ef = E; *mu = pb_sample_mu(&p->u.pb, rng, E); return 0;
     case P_ELINC: *Ef = E; *mu = elinc_sample_mu(&p->u.ei, rng, E); return 0;
     case P_SAB: return sab_sample_iso(&p->u.sab, rng, E, Ef, mu);
@@ -2293,6 +2303,57 @@ static int orc_fg_sample_alpha_beta(double E, double T, double mass, rng_t* rng,
   *alpha = fgs_sample_alpha(&f, b, rng);
   *beta = b;
   return 0;
+}
+
+/* FreeGasSampler::sampleDeltaEMu (NCFreeGasUtils.hh:146-160) with
+   convertAlphaBetaToDeltaEMu (ncrystal_core/src/phys_utils/NCKinUtils.cc:25-58)
+   at the sampler's clamped energy c*kT; FreeGas::sampleScatterIsotropic
+   (NCFreeGas.cc:70-77): E' = max(0, E + dE). */
+static int fg_sample_demu(const fg_t* g, rng_t* rng, double E, double* Ef, double* mu) {
+  /* energies that are not finite and >= 0 are refused (the rejection loops
+     need a finite E/kT) */
+  if (!(E >= 0.0 && E < INFINITY)) return -5;
+  pthread_once(&eb_once, eb_init);
+  fgs_t f; fgs_init(&f, E, g->T, g->mass);
+  const double b = fgs_sample_beta(&f, rng);
+  double dE, m;
+  if (b <= -f.c || mu_iso_at_beta(b, f.c)) {
+    dE = b * f.kT;
+    m = rng_gen(rng) * 2.0 - 1.0;
+  } else {
+    const double a = fgs_sample_alpha(&f, b, rng);
+    const double ek = f.c * f.kT;
+    dE = b * f.kT;
+    const double ekf = ek + dE;
+    const double denom = 2.0 * sqrt(ek * ekf);
+    if (!denom) return -7;
+    ssum_t sm = {0, 0};
+    ss_add(&sm, ek); ss_add(&sm, ekf); ss_add(&sm, -a * f.kT);
+    m = dclamp(ss_sum(&sm) / denom, -1.0, 1.0);
+  }
+  *Ef = dmax(0.0, E + dE);
+  *mu = m;
+  return 0;
+}
+orc_proc* orc_create_freegas(double temperature, double mass_amu, double sigma_free) {
+  if (!(temperature > 0.0) || !(mass_amu > 0.0) || !(sigma_free > 0.0)) return NULL;
+  orc_proc* p = (orc_proc*)calloc(1, sizeof(orc_proc));
+  p->type = P_FG;
+  p->u.fg.T = temperature;
+  p->u.fg.kT = K_BOLTZMANN * temperature;
+  p->u.fg.mass = mass_amu;
+  p->u.fg.sigma_free = sigma_free;
+  p->u.fg.ca = (mass_amu / K_NEUTRON_MASS_AMU) / p->u.fg.kT;
+  return p;
+}
+orc_proc* orc_create_absoov(double sigma_abs) {
+  orc_proc* p = (orc_proc*)calloc(1, sizeof(orc_proc));
+  p->type = P_ABS;
+  /* const_ekin_2200m_s (NCMath.hh:45-46): 0.5 m_n v^2 at 2200 m/s [eV] */
+  const double mevc2 = K_NEUTRON_MASS_AMU * K_DALTON2EVC2 / (K_C_AAS * K_C_AAS);
+  const double ekin2200 = 0.5 * mevc2 * 2200.0e10 * 2200.0e10;
+  p->u.ab.c = sqrt(ekin2200) * sigma_abs;
+  return p;
 }
 
 /* ================================================================== */

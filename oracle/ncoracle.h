@@ -77,6 +77,15 @@ void orc_debug_sort_fams(int n, const double* dsp, const double* fsq, const int*
 
 /* ---- processes ---- */
 typedef struct orc_proc orc_proc;
+/* FreeGas scatter process (ncrystal_core/src/freegas/NCFreeGas.cc:25-77):
+   temperature [K], target mass [amu], free scattering cross section [barn];
+   xs and isotropic sampling through the generic orc_xs_many /
+   orc_sample_iso_many.  NULL on invalid input. */
+orc_proc* orc_create_freegas(double temperature, double mass_amu, double sigma_free);
+/* AbsOOV 1/v absorption (ncrystal_core/src/absoov/NCAbsOOV.cc:30-44):
+   xs = sqrt(E_2200)*sigma_abs/sqrt(E) (inf at E = 0); xs only. */
+orc_proc* orc_create_absoov(double sigma_abs);
+
 orc_proc* orc_create_powderbragg(double v0_times_natoms, const double* dsp,
                                  const double* fsq_x_mult, int n);
 int orc_pb_ntables(const orc_proc*);

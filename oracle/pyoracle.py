@@ -69,6 +69,10 @@ def lib():
                                                   ctypes.c_int64])
         L.orc_hkl_spacegroup_members.restype = ctypes.c_int
         L.orc_reci_lattice.argtypes = [ctypes.c_double] * 6 + [_dp]
+        L.orc_create_freegas.argtypes = [ctypes.c_double] * 3
+        L.orc_create_freegas.restype = _vp
+        L.orc_create_absoov.argtypes = [ctypes.c_double]
+        L.orc_create_absoov.restype = _vp
         L.orc_reci_lattice.restype = ctypes.c_int
         L.orc_create_scbragg.argtypes = [_dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _i64p, _ip, _dp,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double]
@@ -219,6 +223,14 @@ class Proc:
         nerr = lib().orc_sample_dir_many(self.ptr, seed, stream, idx_offset, _d(e), _d(d), e.size,
                                          _d(oe), _d(od), nthreads)
         return oe, od, nerr
+
+
+def freegas(temperature, mass_amu, sigma_free):
+    return Proc(lib().orc_create_freegas(temperature, mass_amu, sigma_free))
+
+
+def absoov(sigma_abs):
+    return Proc(lib().orc_create_absoov(sigma_abs))
 
 
 def powderbragg(v0_times_natoms, dsp, fsq_x_mult):
