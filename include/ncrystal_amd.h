@@ -125,7 +125,7 @@ ncamd_process_t ncamd_create_freegas(double temperature, double mass_amu, double
  * sampling calls raise BadInput. */
 ncamd_process_t ncamd_create_absoov(double sigma_abs);
 
-/* PowderBragg from (d-spacing, |F|^2*multiplicity) pairs; replaces/* PowderBragg from (d-spacing, |F|^2*multiplicity) pairs; replaces
+/* PowderBragg from (d-spacing, |F|^2*multiplicity) pairs; replaces
  * PowderBragg(double v0_times_natoms, VectDFM&&)
  * (ncrystal_core/src/powderbragg/NCPowderBragg.cc:117-120,68-110).
  * fp32_tables != 0 additionally builds FP32 Kahan-compensated cumulative sums
@@ -151,7 +151,14 @@ ncamd_process_t ncamd_create_sabscatter(int nalpha, int nbeta, const double* alp
 
 /* Composition of processes with scales, replaces ProcComposition
  * (ncrystal_core/src/interfaces/NCProcImpl.cc:110-389).  At most one
- * process of each kind; order is kept (it fixes the component pick). */
+ * process of each kind; order is kept (it fixes the component pick).  With
+ * an SCBragg component the composition is oriented (as NCrystal's
+ * single-crystal materials: SCBragg for d >= sccutoff, PowderBragg below it,
+ * ElInc, S(alpha,beta)): only the calls with neutron directions apply, the
+ * isotropic components scatter through randDirectionGivenScatterMu
+ * (NCProcImpl.cc:29-37), and the sampling call waits for the number of
+ * neutrons that pick the SCBragg component before sampling them.  FreeGas
+ * and absorption processes are not accepted. */
 ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, const double* scales);
 
 /* read back tables (for tests and tools) */

@@ -219,6 +219,10 @@ struct Proc {
   SABView sab{};
   FGView fg{};         // FreeGas process
   double abs_c = 0.0;  // 1/v absorption: xs = abs_c / sqrt(E)
+  // oriented composition (one SCBragg component): its cross sections per
+  // neutron, the neutrons that pick it, their compacted inputs
+  bool oriented_comp = false;
+  DevBuf oc_ext, oc_q, oc_qc, oc_cE, oc_cD, oc_idmap;
   double pb_v0n = 0.0;
   std::vector<double> pb_e2d_h, pb_fdm_h;
   std::vector<float> pb_fdm32_h;
@@ -298,7 +302,12 @@ CompView make_view(Proc* p) {
         break;
       case kKindElInc: v.ei = c->ei; break;
       case kKindSAB: v.sab = c->sab; break;
-      case kKindSCB: raise("BadInput", "oriented process: use the calls with neutron directions");
+      case kKindSCB:
+        if (p->kind != kKindNone) raise("BadInput", "oriented process: use the calls with neutron directions");
+        v.has_scb = 1;
+        v.scb_threshold = c->scb.threshold;
+        dlo = c->scb.threshold;
+        break;
       case kKindFG:
       case kKindAbs: raise("BadInput", "FreeGas and absorption processes are evaluated on their own");
       default: raise("LogicError", "bad component kind");
@@ -1910,9 +1919,67 @@ void check_fp32(Proc* p) {
       raise("BadInput", "process was created without FP32 PowderBragg tables");
 }
 
+ScbWork scb_work(Proc* p, int64_t n, hipStream_t st);
+Proc* comp_scb(Proc* p) {
+  for (Proc* c : p->comps)
+    if (c->kind == kKindSCB) return c;
+  raise("LogicError", "oriented composition without an SCBragg component");
+  return nullptr;
+}
+void oc_buffers(Proc* p, int64_t n, hipStream_t st) {
+  const size_t m = (size_t)std::max<int64_t>(n, 1);
+  if (p->oc_ext.bytes < sizeof(double) * m) {
+    NCAMD_HIP(hipStreamSynchronize(st));  // the old buffers may still be in use on this stream
+    p->oc_ext.ensure(sizeof(double) * m);
+    p->oc_q.ensure(sizeof(int64_t) * m);
+    p->oc_cE.ensure(sizeof(double) * m);
+    p->oc_cD.ensure(sizeof(double) * 3 * m);
+    p->oc_idmap.ensure(sizeof(int64_t) * m);
+  }
+  if (!p->oc_qc.p) p->oc_qc.alloc(sizeof(unsigned long long));
+}
+// oriented composition (NCProcImpl.cc:340-389 with the SCBragg component's
+// oriented calls): the SCBragg cross sections of every neutron first, then
+// the composition kernels with them in the SCBragg slot
+void oriented_xs(Proc* p, const double* ekin, const double* dirs, int64_t n, double* out, hipStream_t st) {
+  if (n <= 0) return;
+  Proc* c = comp_scb(p);
+  oc_buffers(p, n, st);
+  const ScbWork w = scb_work(c, n, st);
+  launch_scb_xs(c->scb, ekin, dirs, n, p->oc_ext.as<double>(), w, st);
+  launch_xs_ext(make_view(p), ekin, p->oc_ext.as<double>(), n, out, st);
+}
+// the pick and the isotropic components' sampling as for any composition;
+// the neutrons that pick the SCBragg component are gathered and sampled by
+// the SCBragg kernels on their own Philox streams after the pick's draw
+// (the host waits for their number)
+void oriented_sample(Proc* p, const double* ekin, const double* dirs, int64_t n, double* oE, double* oD,
+                     hipStream_t st) {
+  const uint32_t stream = next_stream(p);
+  if (n <= 0) return;
+  Proc* c = comp_scb(p);
+  oc_buffers(p, n, st);
+  const ScbWork w = scb_work(c, n, st);
+  launch_scb_xs(c->scb, ekin, dirs, n, p->oc_ext.as<double>(), w, st);
+  const CompView v = make_view(p);
+  comp_queues(p, n, st);
+  launch_sample_dirs(v, ekin, dirs, n, p->seed, stream, oE, oD, p->d_err.as<unsigned long long>(),
+                     p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), st, p->oc_ext.as<double>(),
+                     p->oc_q.as<int64_t>(), p->oc_qc.as<unsigned long long>());
+  unsigned long long m = 0;
+  NCAMD_HIP(hipMemcpyAsync(&m, p->oc_qc.p, sizeof(m), hipMemcpyDeviceToHost, st));
+  NCAMD_HIP(hipStreamSynchronize(st));
+  if (!m) return;
+  launch_scb_gather(p->oc_q.as<int64_t>(), p->oc_qc.as<unsigned long long>(), n, ekin, dirs, p->oc_cE.as<double>(),
+                    p->oc_cD.as<double>(), p->oc_idmap.as<int64_t>(), st);
+  launch_scb_sample(c->scb, p->oc_cE.as<double>(), p->oc_cD.as<double>(), (int64_t)m, p->seed, stream, oE, oD, w, st,
+                    p->oc_idmap.as<int64_t>(), v.n > 1 ? 1 : 0);
+}
+
 // cross sections of any isotropic process or composition (the direction
 // does not enter)
 void xs_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat, double* out, bool fp32, hipStream_t st) {
+  if (p->oriented_comp) raise("BadInput", "oriented process: use the calls with neutron directions");
   if (p->kind == kKindFG || p->kind == kKindAbs) {
     if (fp32) raise("BadInput", "process was created without FP32 PowderBragg tables");
     launch_simple_xs(p->kind, p->fg, p->abs_c, ekin, n, repeat, out, st);
@@ -1925,6 +1992,10 @@ void xs_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat, double*
 // scattering with new directions for an isotropic process or composition
 void sample_dirs_dispatch(Proc* p, const double* ekin, const double* dirs, int64_t n, double* oE, double* oD,
                           hipStream_t st) {
+  if (p->oriented_comp) {
+    oriented_sample(p, ekin, dirs, n, oE, oD, st);
+    return;
+  }
   if (p->kind == kKindAbs) raise("BadInput", "an absorption process does not scatter");
   if (p->kind == kKindFG) {
     launch_fg_sample(p->fg, ekin, dirs, n, 1, p->seed, next_stream(p), oE, oD, p->d_err.as<unsigned long long>(),
@@ -1939,6 +2010,7 @@ void sample_dirs_dispatch(Proc* p, const double* ekin, const double* dirs, int64
 
 void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat, double* oE, double* oMu,
                          hipStream_t st) {
+  if (p->oriented_comp) raise("BadInput", "oriented process: use the calls with neutron directions");
   if (p->kind == kKindAbs) raise("BadInput", "an absorption process does not scatter");
   if (p->kind == kKindFG) {
     launch_fg_sample(p->fg, ekin, nullptr, n, repeat, p->seed, next_stream(p), oE, oMu,
@@ -2045,7 +2117,7 @@ int ncamd_valid(ncamd_process_t h) {
   return (p && p->magic == kMagic) ? 1 : 0;
 }
 int ncamd_isoriented(ncamd_process_t h) {
-  NCAMD_TRY return extract(h)->kind == kKindSCB ? 1 : 0; NCAMD_CATCH
+  NCAMD_TRY return (extract(h)->kind == kKindSCB || extract(h)->oriented_comp) ? 1 : 0; NCAMD_CATCH
   return -1;
 }
 
@@ -2201,12 +2273,11 @@ ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, co
   if (n < 1 || n > kMaxComp) raise("BadInput", "composition needs 1..4 components");
   std::unique_ptr<Proc> p(new Proc());
   p->kind = kKindNone;
-  int seen[4] = {0, 0, 0, 0};
+  int seen[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = 0; i < n; ++i) {
     Proc* c = extract(procs[i]);
     if (c->kind == kKindNone) raise("BadInput", "nested compositions are not supported");
-    if (c->kind == kKindSCB)
-      raise("BadInput", "an SCBragg process is evaluated on its own (oriented), not inside a composition");
+    if (c->kind == kKindSCB) p->oriented_comp = true;  // oriented composition: calls with directions only
     if (c->kind == kKindFG || c->kind == kKindAbs)
       raise("BadInput", "FreeGas and absorption processes are evaluated on their own, not inside a composition");
     if (seen[c->kind]++) raise("BadInput", "at most one process of each kind per composition");
@@ -2407,9 +2478,9 @@ void ncamd_samplescatter_many(ncamd_process_t h, double ekin, const double (*dir
 
 void ncamd_crosssection_many_dirs(ncamd_process_t h, const double* ekin, const double* dirs, unsigned long n,
                                   double* out) {
-  bool scb = false;
-  NCAMD_TRY scb = extract(h)->kind == kKindSCB; NCAMD_CATCH
-  if (!scb) {
+  bool scb = false, oc = false;
+  NCAMD_TRY scb = extract(h)->kind == kKindSCB; oc = extract(h)->oriented_comp; NCAMD_CATCH
+  if (!scb && !oc) {
     xs_many_impl(h, ekin, n, 1, out, false);  // isotropic materials: the direction does not enter
     return;
   }
@@ -2424,8 +2495,12 @@ void ncamd_crosssection_many_dirs(ncamd_process_t h, const double* ekin, const d
     NCAMD_HIP(hipMemcpy(p->st_in.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
     NCAMD_HIP(hipMemcpy(p->st_in2.p, dirs, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
   }
-  const ScbWork w = scb_work(p, (int64_t)n, 0);
-  launch_scb_xs(p->scb, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->st_out1.as<double>(), w, 0);
+  if (p->oriented_comp) {
+    oriented_xs(p, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->st_out1.as<double>(), 0);
+  } else {
+    const ScbWork w = scb_work(p, (int64_t)n, 0);
+    launch_scb_xs(p->scb, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->st_out1.as<double>(), w, 0);
+  }
   NCAMD_HIP(hipGetLastError());
   if (n) NCAMD_HIP(hipMemcpy(out, p->st_out1.p, sizeof(double) * n, hipMemcpyDeviceToHost));
   ok = true;
@@ -2480,6 +2555,8 @@ void ncamd_dev_crosssection_many_dirs(ncamd_process_t h, const double* d_ekin, c
   if (p->kind == kKindSCB) {
     const ScbWork w = scb_work(p, (int64_t)n, as_stream(stream));
     launch_scb_xs(p->scb, d_ekin, d_dirs, (int64_t)n, d_out, w, as_stream(stream));
+  } else if (p->oriented_comp) {
+    oriented_xs(p, d_ekin, d_dirs, (int64_t)n, d_out, as_stream(stream));
   } else {
     xs_dispatch(p, d_ekin, (int64_t)n, 1, d_out, false, as_stream(stream));
   }
@@ -2508,6 +2585,7 @@ ncamd_batch_t ncamd_batch_create(int nmat, const ncamd_process_t* procs, const i
   for (int i = 0; i < nmat; ++i) {
     Proc* p = extract(procs[i]);
     if (counts[i] < 0) raise("BadInput", "negative neutron count");
+    if (p->oriented_comp) raise("BadInput", "oriented compositions are not supported in batches");
     const CompView v = make_view(p);  // raises for oriented processes
     p->refcount++;
     b->mats.push_back(p);

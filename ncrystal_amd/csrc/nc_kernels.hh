@@ -23,6 +23,8 @@ struct CompView {
   PBView pb;
   ElIncView ei;
   SABView sab;
+  double scb_threshold;  // domain of an SCBragg component (oriented compositions)
+  int has_scb;           // 1: oriented composition with an SCBragg component
 };
 
 // Composition semantics: ncrystal_core/src/interfaces/NCProcImpl.cc:166-204
@@ -44,8 +46,10 @@ NCD double pb_xs_sel(const PBView& pb, double E) {
   }
 }
 
+// ext: the cross section of an SCBragg component for this neutron and
+// direction (oriented compositions; computed by the SCBragg kernels)
 template <bool FP32PB>
-NCD double comp_xs(const CompView& c, double E, double* commul) {
+NCD double comp_xs(const CompView& c, double E, double* commul, double ext = 0.0) {
   if (comp_outside(c, E)) return 0.0;
   double tot = 0.0;
   for (int i = 0; i < c.n; ++i) {
@@ -54,6 +58,7 @@ NCD double comp_xs(const CompView& c, double E, double* commul) {
       case kKindPB: xs = pb_active(c.pb, E) ? pb_xs_sel<FP32PB>(c.pb, E) : 0.0; break;
       case kKindElInc: xs = elinc_xs(c.ei, E, nullptr); break;
       case kKindSAB: xs = sab_xs(c.sab, E); break;
+      case kKindSCB: xs = E >= c.scb_threshold ? ext : 0.0; break;
       default: break;
     }
     tot += c.scale[i] * xs;
@@ -95,16 +100,21 @@ NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, doubl
 // kCompToSAB when the S(alpha,beta) component is picked; its sampling then
 // continues the neutron's stream after the pick, and the S(alpha,beta) stage
 // replays the pick's draw (one draw when there are several components).
+// kCompToSCB likewise for the SCBragg component of an oriented composition
+// (its cross section for the neutron given as ext).
 constexpr int kCompToSAB = 1;
-NCD int comp_pick_sample(const CompView& c, Rng& rng, double E, double& Ef, double& mu) {
+constexpr int kCompToSCB = 2;
+NCD int comp_pick_sample(const CompView& c, Rng& rng, double E, double& Ef, double& mu, double ext = 0.0) {
   if (comp_outside(c, E)) { Ef = E; mu = 1.0; return kOK; }
   double commul[kMaxComp];
-  comp_xs<false>(c, E, commul);
+  const double tot = comp_xs<false>(c, E, commul, ext);
+  if (!(tot > 0.0)) { Ef = E; mu = 1.0; return kOK; }  // no component scatters (NCProcImpl.cc:379-389)
   const int ic = pick_idx(rng, commul, c.n);
   switch (c.kind[ic]) {
     case kKindPB: Ef = E; mu = pb_sample_mu(c.pb, rng, E); return kOK;
     case kKindElInc: Ef = E; mu = elinc_sample_mu(c.ei, rng, E); return kOK;
     case kKindSAB: return kCompToSAB;
+    case kKindSCB: return kCompToSCB;
     default: return kErrBadSampler;
   }
 }
@@ -112,7 +122,7 @@ NCD int comp_pick_sample(const CompView& c, Rng& rng, double E, double& Ef, doub
 NCHD int comp_variant(const CompView& c) {
   if (c.n == 1 && c.kind[0] == kKindPB) return kCompPBOnly;
   for (int i = 0; i < c.n; ++i)
-    if (c.kind[i] == kKindSAB) return kCompGeneral;
+    if (c.kind[i] == kKindSAB || c.kind[i] == kKindSCB) return kCompGeneral;
   return kCompNoSAB;
 }
 
@@ -124,9 +134,15 @@ void launch_xs_iso(const CompView&, const double* ekin, int64_t n, int64_t repea
 void launch_sample_iso(const CompView&, const double* ekin, int64_t n, int64_t repeat, uint64_t seed,
                        uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, int64_t* queues,
                        unsigned long long* qcount, hipStream_t);
+// oriented compositions with an SCBragg component: ext[n] its cross
+// sections; the neutrons that pick it are listed in cq (count *cqc, cleared
+// here) for the SCBragg sampling kernels
 void launch_sample_dirs(const CompView&, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
                         uint32_t stream, double* oE, double* oD, unsigned long long* nerr, int64_t* queues,
-                        unsigned long long* qcount, hipStream_t);
+                        unsigned long long* qcount, hipStream_t, const double* ext = nullptr,
+                        int64_t* cq = nullptr, unsigned long long* cqc = nullptr);
+// cross sections of an oriented composition given its SCBragg component's
+void launch_xs_ext(const CompView&, const double* ekin, const double* ext, int64_t n, double* out, hipStream_t);
 void launch_sab_ab(const SABView&, const double* ekin, int64_t n, uint64_t seed, uint32_t stream, double* oa,
                    double* ob, unsigned long long* nerr, hipStream_t);
 void launch_fill_loguniform(double* out, int64_t n, double lemin, double lemax, uint64_t seed, hipStream_t);
@@ -210,8 +226,16 @@ struct ScbWork {
 };
 void launch_scb_xs(const SCBView&, const double* ekin, const double* dirs, int64_t n, double* out,
                    const ScbWork&, hipStream_t);
+// idmap (optional): neutron i of the (compact) inputs writes slot idmap[i]
+// of oE/oD and draws from Philox index idmap[i], after skipping `skip`
+// draws (an oriented composition's pick)
 void launch_scb_sample(const SCBView&, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
-                       uint32_t stream, double* oE, double* oD, const ScbWork&, hipStream_t);
+                       uint32_t stream, double* oE, double* oD, const ScbWork&, hipStream_t,
+                       const int64_t* idmap = nullptr, int skip = 0);
+// compacts the neutrons listed in q (count *qc <= nmax): energies, directions
+// and original indices
+void launch_scb_gather(const int64_t* q, const unsigned long long* qc, int64_t nmax, const double* ekin,
+                       const double* dirs, double* cE, double* cD, int64_t* idmap, hipStream_t);
 
 // S(alpha,beta) table construction (nc_sab_build.hip)
 struct SABBuildBufs {

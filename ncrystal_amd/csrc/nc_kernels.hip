@@ -137,16 +137,22 @@ __global__ void __launch_bounds__(256) k_comp_pick(CompView c, const double* __r
                                                    const double* __restrict__ dirs, int64_t n, int64_t total,
                                                    uint64_t seed, uint32_t stream, double* __restrict__ oE,
                                                    double* __restrict__ o2, unsigned long long* __restrict__ nerr,
-                                                   int64_t* __restrict__ sq, unsigned long long* __restrict__ sqc) {
+                                                   int64_t* __restrict__ sq, unsigned long long* __restrict__ sqc,
+                                                   const double* __restrict__ ext, int64_t* __restrict__ cq,
+                                                   unsigned long long* __restrict__ cqc) {
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gstride) {
     const int64_t i = (total == n) ? k : (k % n);
     Rng rng;
     rng.init(seed, stream, (uint64_t)k);
     double Ef, mu;
-    const int rc = comp_pick_sample(c, rng, ekin[i], Ef, mu);
+    const int rc = comp_pick_sample(c, rng, ekin[i], Ef, mu, ext ? ext[k] : 0.0);
     if (rc == kCompToSAB) {
       sq[atomicAdd(sqc, 1ull)] = k;
+      continue;
+    }
+    if (rc == kCompToSCB) {
+      cq[atomicAdd(cqc, 1ull)] = k;
       continue;
     }
     comp_store<Dirs>(k, rc, Ef, mu, rng, dirs, oE, o2, nerr);
@@ -364,7 +370,8 @@ void launch_sample_iso(const CompView& c, const double* ekin, int64_t n, int64_t
       hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount);
       hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount + 1);
       hipLaunchKernelGGL(k_comp_pick<false>, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin,
-                         (const double*)nullptr, n, total, seed, stream, oE, oMu, nerr, sq, qcount);
+                         (const double*)nullptr, n, total, seed, stream, oE, oMu, nerr, sq, qcount,
+                         (const double*)nullptr, (int64_t*)nullptr, (unsigned long long*)nullptr);
       if (sab < 0) break;  // no S(alpha,beta) part: nothing queued
       const int skip = c.n > 1 ? 1 : 0;
       hipLaunchKernelGGL((k_comp_sab<false, false>), dim3(grid_for(total, 256)), dim3(256), 0, st, c.sab, ekin,
@@ -376,9 +383,23 @@ void launch_sample_iso(const CompView& c, const double* ekin, int64_t n, int64_t
     }
   }
 }
+__global__ void __launch_bounds__(256) k_xs_ext(CompView c, const double* __restrict__ ekin,
+                                                const double* __restrict__ ext, int64_t n, double* __restrict__ out) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gstride)
+    out[k] = comp_xs<false>(c, ekin[k], nullptr, ext[k]);
+}
+void launch_xs_ext(const CompView& c, const double* ekin, const double* ext, int64_t n, double* out,
+                   hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_xs_ext, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, ext, n, out);
+}
+
 void launch_sample_dirs(const CompView& c, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
                         uint32_t stream, double* oE, double* oD, unsigned long long* nerr, int64_t* queues,
-                        unsigned long long* qcount, hipStream_t st) {
+                        unsigned long long* qcount, hipStream_t st, const double* ext, int64_t* cq,
+                        unsigned long long* cqc) {
+  if (cqc) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, cqc);
   if (n <= 0) return;
   switch (comp_variant(c)) {
     case kCompPBOnly:
@@ -395,7 +416,7 @@ void launch_sample_dirs(const CompView& c, const double* ekin, const double* dir
       hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount);
       hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount + 1);
       hipLaunchKernelGGL(k_comp_pick<true>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, dirs, n, n, seed,
-                         stream, oE, oD, nerr, sq, qcount);
+                         stream, oE, oD, nerr, sq, qcount, ext, cq, cqc);
       bool has_sab = false;
       for (int j = 0; j < c.n; ++j) has_sab = has_sab || c.kind[j] == kKindSAB;
       if (!has_sab) break;

@@ -51,7 +51,8 @@ template <bool AllowSlow>
 NCD void scb_sample_one(const SCBView& v, bool valid, int64_t i, const double* __restrict__ ekin,
                         const double* __restrict__ dirs, uint64_t seed, uint32_t stream, double* __restrict__ oE,
                         double* __restrict__ oD, int64_t* __restrict__ queue,
-                        unsigned long long* __restrict__ qcount, uint16_t* __restrict__ list) {
+                        unsigned long long* __restrict__ qcount, uint16_t* __restrict__ list,
+                        const int64_t* __restrict__ idmap, int skip) {
   double E = 0.0;
   V3 din{0.0, 0.0, 1.0};
   ScbLane L;
@@ -75,8 +76,10 @@ NCD void scb_sample_one(const SCBView& v, bool valid, int64_t i, const double* _
     queued = true;
   }
   const bool scatter = valid && can && !queued && L.count > 0 && L.total > 0.0;
+  const int64_t o = idmap && valid ? idmap[i] : i;  // output slot and Philox index
   Rng rng;
-  rng.init(seed, stream, (uint64_t)i);
+  rng.init(seed, stream, (uint64_t)o);
+  for (int q = 0; q < skip; ++q) (void)rng.gen();
   double target = 0.0;
   if (scatter && L.count > 1) target = L.total * rng.gen();
   // pass 2: the contribution whose cumulative value passes the target, from
@@ -92,12 +95,12 @@ NCD void scb_sample_one(const SCBView& v, bool valid, int64_t i, const double* _
     scb_walk<true, AllowSlow>(v, S, L.count, target, list, nl2, c2);
   }
   if (!valid || queued) return;
-  V3 o = din;
-  if (scatter && S.found) o = gm_genscat(v, rng, S.sel_normal, S.sel_inv2d_r, L.wl, L.u);
-  oE[i] = E;
-  oD[3 * i] = o.x;
-  oD[3 * i + 1] = o.y;
-  oD[3 * i + 2] = o.z;
+  V3 d = din;
+  if (scatter && S.found) d = gm_genscat(v, rng, S.sel_normal, S.sel_inv2d_r, L.wl, L.u);
+  oE[o] = E;
+  oD[3 * o] = d.x;
+  oD[3 * o + 1] = d.y;
+  oD[3 * o + 2] = d.z;
 }
 
 }  // namespace
@@ -223,7 +226,8 @@ __global__ void __launch_bounds__(256) k_scb_sample_count(SCBView v, const doubl
                                                           const int64_t* __restrict__ perm, double* __restrict__ oE,
                                                           double* __restrict__ oD, int* __restrict__ cnt,
                                                           double* __restrict__ tot, int64_t* __restrict__ queue,
-                                                          unsigned long long* __restrict__ qcount) {
+                                                          unsigned long long* __restrict__ qcount,
+                                                          const int64_t* __restrict__ idmap) {
   __shared__ uint16_t lists[4 * kScbListWave];
   uint16_t* list = lists + (threadIdx.x >> 6) * kScbListWave;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -257,10 +261,11 @@ __global__ void __launch_bounds__(256) k_scb_sample_count(SCBView v, const doubl
     cnt[k] = scatter ? L.count : 0;
     tot[k] = L.total;
     if (!scatter) {
-      oE[i] = E;
-      oD[3 * i] = din.x;
-      oD[3 * i + 1] = din.y;
-      oD[3 * i + 2] = din.z;
+      const int64_t o = idmap ? idmap[i] : i;
+      oE[o] = E;
+      oD[3 * o] = din.x;
+      oD[3 * o + 1] = din.y;
+      oD[3 * o + 2] = din.z;
     }
   }
 }
@@ -268,7 +273,8 @@ __global__ void __launch_bounds__(256) k_scb_sample_pick(SCBView v, const double
                                                          const double* __restrict__ dirs, int64_t n,
                                                          const int64_t* __restrict__ perm, uint64_t seed,
                                                          uint32_t stream, const int* __restrict__ cnt,
-                                                         const double* __restrict__ tot, double4* __restrict__ sel) {
+                                                         const double* __restrict__ tot, double4* __restrict__ sel,
+                                                         const int64_t* __restrict__ idmap, int skip) {
   __shared__ uint16_t lists[4 * kScbListWave];
   uint16_t* list = lists + (threadIdx.x >> 6) * kScbListWave;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -288,7 +294,8 @@ __global__ void __launch_bounds__(256) k_scb_sample_pick(SCBView v, const double
       scb_lane_setup(v, E, din.x, din.y, din.z, L);
     }
     Rng rng;
-    rng.init(seed, stream, (uint64_t)i);
+    rng.init(seed, stream, (uint64_t)(idmap && c > 0 ? idmap[i] : i));
+    for (int q = 0; q < skip; ++q) (void)rng.gen();
     double target = 0.0;
     if (c > 1) target = tot[k] * rng.gen();
     int nl;
@@ -304,7 +311,8 @@ __global__ void __launch_bounds__(256) k_scb_sample_scatter(SCBView v, const dou
                                                             const int64_t* __restrict__ perm, uint64_t seed,
                                                             uint32_t stream, double* __restrict__ oE,
                                                             double* __restrict__ oD, const int* __restrict__ cnt,
-                                                            const double4* __restrict__ sel) {
+                                                            const double4* __restrict__ sel,
+                                                            const int64_t* __restrict__ idmap, int skip) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     const int c = cnt[k];
@@ -314,23 +322,26 @@ __global__ void __launch_bounds__(256) k_scb_sample_scatter(SCBView v, const dou
     const V3 din{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
     ScbLane L;
     scb_lane_setup(v, E, din.x, din.y, din.z, L);
+    const int64_t oi = idmap ? idmap[i] : i;
     Rng rng;
-    rng.init(seed, stream, (uint64_t)i);
+    rng.init(seed, stream, (uint64_t)oi);
+    for (int q = 0; q < skip; ++q) (void)rng.gen();
     if (c > 1) (void)rng.gen();  // the target draw of the selection
     const double4 s4 = sel[k];
     V3 o = din;
     if (s4.w != 0.0) o = gm_genscat(v, rng, V3{s4.x, s4.y, s4.z}, s4.w, L.wl, L.u);
-    oE[i] = E;
-    oD[3 * i] = o.x;
-    oD[3 * i + 1] = o.y;
-    oD[3 * i + 2] = o.z;
+    oE[oi] = E;
+    oD[3 * oi] = o.x;
+    oD[3 * oi + 1] = o.y;
+    oD[3 * oi + 2] = o.z;
   }
 }
 __global__ void __launch_bounds__(256) k_scb_sample_slow(SCBView v, const double* __restrict__ ekin,
                                                          const double* __restrict__ dirs, uint64_t seed,
                                                          uint32_t stream, double* __restrict__ oE,
                                                          double* __restrict__ oD, const int64_t* __restrict__ queue,
-                                                         const unsigned long long* __restrict__ qcount) {
+                                                         const unsigned long long* __restrict__ qcount,
+                                                         const int64_t* __restrict__ idmap, int skip) {
   __shared__ uint16_t lists[4 * kScbListWave];
   uint16_t* list = lists + (threadIdx.x >> 6) * kScbListWave;
   const int64_t nq = (int64_t)*qcount;
@@ -338,7 +349,8 @@ __global__ void __launch_bounds__(256) k_scb_sample_slow(SCBView v, const double
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nq; base += stride) {
     const int64_t q = base + threadIdx.x;
     const bool valid = q < nq;
-    scb_sample_one<true>(v, valid, valid ? queue[q] : 0, ekin, dirs, seed, stream, oE, oD, nullptr, nullptr, list);
+    scb_sample_one<true>(v, valid, valid ? queue[q] : 0, ekin, dirs, seed, stream, oE, oD, nullptr, nullptr, list,
+                         idmap, skip);
   }
 }
 
@@ -369,18 +381,43 @@ void launch_scb_xs(const SCBView& v, const double* ekin, const double* dirs, int
   hipLaunchKernelGGL(k_scb_xs_slow, dim3(256), dim3(256), 0, st, v, ekin, dirs, out, w.queue, w.qcount);
 }
 
+// gathers the neutrons listed in q (count *qc, at most nmax) into compact
+// arrays, with their original indices in idmap
+__global__ void __launch_bounds__(256) k_scb_gather(const int64_t* __restrict__ q,
+                                                    const unsigned long long* __restrict__ qc,
+                                                    const double* __restrict__ ekin, const double* __restrict__ dirs,
+                                                    double* __restrict__ cE, double* __restrict__ cD,
+                                                    int64_t* __restrict__ idmap) {
+  const int64_t m = (int64_t)*qc;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < m; t += stride) {
+    const int64_t k = q[t];
+    cE[t] = ekin[k];
+    cD[3 * t] = dirs[3 * k];
+    cD[3 * t + 1] = dirs[3 * k + 1];
+    cD[3 * t + 2] = dirs[3 * k + 2];
+    idmap[t] = k;
+  }
+}
+void launch_scb_gather(const int64_t* q, const unsigned long long* qc, int64_t nmax, const double* ekin,
+                       const double* dirs, double* cE, double* cD, int64_t* idmap, hipStream_t st) {
+  if (nmax <= 0) return;
+  hipLaunchKernelGGL(k_scb_gather, dim3(scb_grid(nmax)), dim3(256), 0, st, q, qc, ekin, dirs, cE, cD, idmap);
+}
+
 void launch_scb_sample(const SCBView& v, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
-                       uint32_t stream, double* oE, double* oD, const ScbWork& w, hipStream_t st) {
+                       uint32_t stream, double* oE, double* oD, const ScbWork& w, hipStream_t st,
+                       const int64_t* idmap, int skip) {
   if (n <= 0) return;
   scb_order(v, ekin, dirs, n, w, st);
   hipLaunchKernelGGL(k_scb_sample_count, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, w.perm, oE, oD,
-                     w.cnt, w.tot, w.queue, w.qcount);
+                     w.cnt, w.tot, w.queue, w.qcount, idmap);
   hipLaunchKernelGGL(k_scb_sample_pick, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, w.perm, seed, stream,
-                     w.cnt, w.tot, w.sel);
+                     w.cnt, w.tot, w.sel, idmap, skip);
   hipLaunchKernelGGL(k_scb_sample_scatter, dim3(scb_grid(n)), dim3(256), 0, st, v, ekin, dirs, n, w.perm, seed,
-                     stream, oE, oD, w.cnt, w.sel);
+                     stream, oE, oD, w.cnt, w.sel, idmap, skip);
   hipLaunchKernelGGL(k_scb_sample_slow, dim3(256), dim3(256), 0, st, v, ekin, dirs, seed, stream, oE, oD, w.queue,
-                     w.qcount);
+                     w.qcount, idmap, skip);
 }
 
 }  // namespace ncamd

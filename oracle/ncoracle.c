@@ -1732,8 +1732,10 @@ orc_proc* orc_create_composition(int n, orc_proc* const* comps, const double* sc
 static double proc_xs(const orc_proc* p, double E);
 static int proc_sample_iso(const orc_proc* p, rng_t* rng, double E, double* Ef, double* mu);
 /* component domain: PowderBragg [threshold,inf), others [0,inf) */
+static double scb_threshold_p(const orc_proc* p);
 static int comp_domain_ok(const orc_proc* p, double E) {
   if (p->type == P_PB) return p->u.pb.n > 0 && E >= p->u.pb.threshold;
+  if (p->type == P_SCB) return E >= scb_threshold_p(p); /* SCBragg::domain (NCSCBragg.cc:290-293) */
   return 1;
 }
 /* composition energy domain: union of component domains
@@ -1744,6 +1746,7 @@ static int comp_in_domain(const orc_proc* p, double E) {
     const orc_proc* c = p->u.comp.comps[i];
     double dlo = 0.0, dhi = INFINITY;
     if (c->type == P_PB) dlo = c->u.pb.threshold;
+    if (c->type == P_SCB) dlo = scb_threshold_p(c);
     if (dlo < dhi) { lo = dmin(lo, dlo); hi = dmax(hi, dhi); }
   }
   return (lo < hi) && (E >= lo) && (E <= hi);
@@ -1849,17 +1852,71 @@ typedef struct {
 } job_t;
 static double scb_xs_p(const orc_proc* p, double E, const double* d);
 static void scb_sample_p(const orc_proc* p, rng_t* rng, double E, const double* d, double* od);
+/* compositions with an SCBragg component (oriented, NCProcImpl.cc:340-389
+   with the component's oriented crossSection/sampleScatter): component
+   cross sections in order, the pick, then the picked component's sampling
+   with the same stream (isotropic ones through randDirectionGivenScatterMu) */
+static int comp_has_scb(const orc_proc* p) {
+  if (p->type != P_COMP) return 0;
+  for (int i = 0; i < p->u.comp.n; ++i)
+    if (p->u.comp.comps[i]->type == P_SCB) return 1;
+  return 0;
+}
+static double comp_xs_dir(const orc_proc* p, double E, const double* d, double* cm) {
+  if (!comp_in_domain(p, E)) return 0.0;
+  double tot = 0.0;
+  for (int i = 0; i < p->u.comp.n; ++i) {
+    const orc_proc* c = p->u.comp.comps[i];
+    double xs = 0.0;
+    if (comp_domain_ok(c, E)) xs = c->type == P_SCB ? scb_xs_p(c, E, d) : proc_xs(c, E);
+    tot += p->u.comp.scales[i] * xs;
+    if (cm) cm[i] = tot;
+  }
+  return tot;
+}
+static int comp_sample_dir(const orc_proc* p, rng_t* rng, double E, const double* d, double* Ef, double* od) {
+  double cm[16];
+  int ic = -1;
+  if (comp_in_domain(p, E)) {
+    const double tot = comp_xs_dir(p, E, d, cm);
+    if (tot > 0.0) ic = (int)pick_idx(rng, cm, (size_t)p->u.comp.n);
+  }
+  if (ic >= 0 && p->u.comp.comps[ic]->type == P_SCB) {
+    *Ef = E;
+    scb_sample_p(p->u.comp.comps[ic], rng, E, d, od);
+    return 0;
+  }
+  double mu = 1.0;
+  *Ef = E;
+  if (ic >= 0) {
+    const int rc = proc_sample_iso(p->u.comp.comps[ic], rng, E, Ef, &mu);
+    if (rc) return rc;
+  }
+  rand_dir_given_mu(rng, mu, d, od);
+  return 0;
+}
 static void* job_run(void* arg) {
   job_t* j = (job_t*)arg;
   for (int64_t k = j->lo; k < j->hi; ++k) {
     if (j->mode == 0) {
-      j->o1[k] = j->p->type == P_SCB ? scb_xs_p(j->p, j->ekin[k], j->dirs + 3 * k) : proc_xs(j->p, j->ekin[k]);
+      j->o1[k] = j->p->type == P_SCB ? scb_xs_p(j->p, j->ekin[k], j->dirs + 3 * k)
+                 : comp_has_scb(j->p) ? comp_xs_dir(j->p, j->ekin[k], j->dirs + 3 * k, NULL)
+                                      : proc_xs(j->p, j->ekin[k]);
       continue;
     }
     if (j->mode == 3 && j->p->type == P_SCB) {
       rng_t rng; rng_init(&rng, j->seed, j->stream, j->idxoff + (uint64_t)k);
       j->o1[k] = j->ekin[k];
       scb_sample_p(j->p, &rng, j->ekin[k], j->dirs + 3 * k, j->o2 + 3 * k);
+      continue;
+    }
+    if (j->mode == 3 && comp_has_scb(j->p)) {
+      rng_t rng; rng_init(&rng, j->seed, j->stream, j->idxoff + (uint64_t)k);
+      double Ef, od[3];
+      if (comp_sample_dir(j->p, &rng, j->ekin[k], j->dirs + 3 * k, &Ef, od)) {
+        Ef = -1.0; od[0] = od[1] = od[2] = 0.0; j->nerr++;
+      }
+      j->o1[k] = Ef; j->o2[3 * k] = od[0]; j->o2[3 * k + 1] = od[1]; j->o2[3 * k + 2] = od[2];
       continue;
     }
     int64_t i = k % j->n;
@@ -3000,6 +3057,7 @@ void orc_scb_tables(const orc_proc* p, double* inv2d, double* xsfact, int64_t* o
 
 struct scb_s { scb_t s; };
 static double scb_xs_p(const orc_proc* p, double E, const double* d) { return scb_xs(&p->u.scb->s, E, d); }
+static double scb_threshold_p(const orc_proc* p) { return p->u.scb->s.threshold; }
 static void scb_sample_p(const orc_proc* p, rng_t* rng, double E, const double* d, double* od) {
   scb_sample(&p->u.scb->s, rng, E, d, od);
 }

@@ -215,6 +215,15 @@ class Proc:
                                          _d(oe), _d(om), nthreads)
         return oe, om, nerr
 
+    def xs_dir(self, ekin, dirs, nthreads=1):
+        """cross sections with neutron directions (SCBragg and compositions
+        holding it; the direction does not enter for isotropic processes)"""
+        e = np.ascontiguousarray(np.atleast_1d(np.asarray(ekin, dtype=np.float64)))
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dirs, dtype=np.float64), (e.size, 3)))
+        out = np.empty(e.size)
+        lib().orc_xs_dir_many(self.ptr, _d(e), _d(d), e.size, _d(out), nthreads)
+        return out
+
     def sample_dir(self, ekin, dirs, seed=1, stream=0, idx_offset=0, nthreads=1):
         e = np.ascontiguousarray(np.atleast_1d(np.asarray(ekin, dtype=np.float64)))
         d = np.ascontiguousarray(np.asarray(dirs, dtype=np.float64).reshape(-1, 3))
@@ -335,13 +344,6 @@ def hkl_orthogonal_members(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tol
 
 
 class SCB(Proc):
-    def xs_dir(self, ekin, dirs, nthreads=1):
-        e = np.ascontiguousarray(np.atleast_1d(np.asarray(ekin, dtype=np.float64)))
-        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dirs, dtype=np.float64), (e.size, 3)))
-        out = np.empty(e.size)
-        lib().orc_xs_dir_many(self.ptr, _d(e), _d(d), e.size, _d(out), nthreads)
-        return out
-
     def info(self):
         nf, nn, thr = ctypes.c_int(), ctypes.c_int64(), ctypes.c_double()
         gd = np.empty(15)

@@ -1,0 +1,106 @@
+"""Oriented compositions (an SCBragg component with isotropic ones, as
+NCrystal builds single-crystal materials: SCBragg for d >= sccutoff,
+PowderBragg for the rest, ElInc, S(alpha,beta)) through the C ABI on the GPU
+against the CPU checker: cross sections with directions to 1e-12, sampled
+energies and directions on the same Philox streams, with and without an
+S(alpha,beta) part and with a single component; the calls without
+directions are refused."""
+import math
+
+import numpy as np
+import pytest
+
+from ncrystal_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+SCCUTOFF = 0.4
+
+
+@pytest.fixture(scope="module")
+def amd():
+    from ncrystal_amd import _capi, core
+    if _capi.lib().ncamd_device_count() < 1:
+        pytest.fail("no GPU visible to libncrystal_amd")
+    return _capi, core
+
+
+def _subset(fams, keep):
+    hkl, mult, dsp, fsq, moff, mh = fams
+    idx = np.nonzero(keep)[0]
+    parts = [mh[moff[i]:moff[i + 1]] for i in idx]
+    noff = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.int64)
+    return hkl[idx], mult[idx], dsp[idx], fsq[idx], noff, np.concatenate(parts).astype(np.int32)
+
+
+def _material(core, oracle, with_sab=True, only_scb=False):
+    cr = synth.fcc_powder(dcutoff=0.25)
+    msd = oracle.debye_msd(410.0, 293.0, 26.98)
+    sp = [(cr.species[0].csl, msd, cr.species[0].positions)]
+    fams = oracle.hkl_orthogonal_members(cr.a, cr.b, cr.c, sp, cr.dcutoff)
+    v0n = cr.volume * cr.n_atoms
+    R = synth.random_rotation(11)
+    mos = 40 * math.pi / (180 * 60)
+    s_hkl, s_mult, s_dsp, s_fsq, s_moff, s_mh = _subset(fams, fams[2] >= SCCUTOFF)
+    L = core.reci_lattice_orthogonal(cr.a, cr.b, cr.c)
+    g = [core.create_scbragg(v0n, s_dsp, s_fsq, s_moff, s_mh, L, R, mos)]
+    o = [oracle.scbragg(cr.a, cr.b, cr.c, v0n, s_dsp, s_fsq, s_moff, s_mh, R, mos)]
+    if only_scb:
+        return core.create_composition(g, [2.0]), oracle.composition(o, [2.0])
+    low = fams[2] < SCCUTOFF
+    g.append(core.create_powderbragg(v0n, fams[2][low], fams[3][low] * fams[1][low]))
+    o.append(oracle.powderbragg(v0n, fams[2][low], fams[3][low] * fams[1][low]))
+    g.insert(0, core.create_elincscatter([msd], [0.0082], [1.0]))
+    o.insert(0, oracle.elinc([msd], [0.0082], [1.0]))
+    if with_sab:
+        t = synth.small_sab_table(mass_amu=26.98)
+        args = (t["alpha"], t["beta"], t["sab"], t["temperature"], t["mass"], t["sigma_bound"])
+        g.append(core.create_sabscatter(*args))
+        o.append(oracle.sab(*args))
+    sc = [1.0] * len(g)
+    return core.create_composition(g, sc), oracle.composition(o, sc)
+
+
+def _rel(a, b):
+    same = a == b
+    den = np.maximum(np.abs(a), np.abs(b))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        r = np.where(den > 0, np.abs(a - b) / np.where(den > 0, den, 1.0), 0.0)
+    return np.where(same, 0.0, r)
+
+
+@pytest.mark.parametrize("with_sab,only_scb", [(True, False), (False, False), (False, True)])
+def test_oriented_composition_matches_checker(amd, oracle, with_sab, only_scb):
+    C, core = amd
+    g, o = _material(core, oracle, with_sab, only_scb)
+    assert C.call("ncamd_isoriented", g._h) == 1
+    n = 40000
+    wl = synth.wavelengths_uniform(n, 0.5, 6.0, seed=4)
+    e = 0.081804209605330899 / wl ** 2
+    dirs = synth.cone_dirs(n, synth.random_unit_vector(2), 5 * math.pi / 180, seed=6)
+    xg = g.crossSection(e, dirs)
+    xo = o.xs_dir(e, dirs, nthreads=8)
+    assert np.max(_rel(xg, xo)) <= 1e-12
+    g.setRandomSeed(21)
+    ge, gd = g.sampleScatter(e, dirs)
+    oe, od, nerr = o.sample_dir(e, dirs, seed=21, nthreads=8)
+    assert nerr == 0
+    diff = np.mean((np.abs(ge - oe) > 1e-9 * np.abs(oe)) | (np.max(np.abs(gd - od), axis=1) > 1e-9))
+    assert diff < 5e-3, diff
+    assert np.abs(np.linalg.norm(gd, axis=1) - 1).max() < 1e-12
+    # a second call continues the streams
+    ge2, gd2 = g.sampleScatter(e, dirs)
+    oe2, od2, _ = o.sample_dir(e, dirs, seed=21, stream=1, nthreads=8)
+    diff2 = np.mean((np.abs(ge2 - oe2) > 1e-9 * np.abs(oe2)) | (np.max(np.abs(gd2 - od2), axis=1) > 1e-9))
+    assert diff2 < 5e-3
+
+
+def test_oriented_composition_refuses_isotropic_calls(amd, oracle):
+    C, core = amd
+    g, _ = _material(core, oracle, with_sab=False)
+    with pytest.raises(C.NCBadInput):
+        g.crossSectionIsotropic(np.array([0.025]))
+    with pytest.raises(C.NCBadInput):
+        g.sampleScatterIsotropic(np.array([0.025]))
+    with pytest.raises(C.NCBadInput):
+        core.Batch([g], [10])

@@ -204,8 +204,11 @@ def test_scbragg_interface(core, oracle):
     assert C.call("ncamd_isoriented", g.handle) == 1
     with pytest.raises(C.NCAMDError):
         g.crossSectionIsotropic([0.01])
-    with pytest.raises(C.NCAMDError):
-        core.create_composition([g], [1.0])
+    # a composition holding it is oriented as well
+    comp = core.create_composition([g], [1.0])
+    assert C.call("ncamd_isoriented", comp.handle) == 1
+    with pytest.raises(C.NCBadInput):
+        comp.crossSectionIsotropic([0.01])
     # repeated single (E, dir) form and the device-pointer form
     d = (0.3, -0.5, 0.81)
     g.setRandomSeed(3)
