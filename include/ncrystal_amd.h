@@ -150,8 +150,17 @@ ncamd_process_t ncamd_create_sabscatter(int nalpha, int nbeta, const double* alp
                                         int n_egrid, const double* egrid);
 
 /* Composition of processes with scales, replaces ProcComposition
- * (ncrystal_core/src/interfaces/NCProcImpl.cc:110-389).  At most one
- * process of each kind; order is kept (it fixes the component pick).  With
+ * (ncrystal_core/src/interfaces/NCProcImpl.cc:110-389).  Components are
+ * added as ProcComposition::addComponent does (NCProcImpl.cc:263-338):
+ * zero scales and PowderBragg processes without planes are dropped, a nested
+ * composition adds its components with multiplied scales, a process already
+ * present adds to its scale, and PowderBragg + PowderBragg or ElInc + ElInc
+ * are merged into one process at scale 1 (createMerged:
+ * ncrystal_core/src/powderbragg/NCPowderBragg.cc:218-331,
+ * ncrystal_core/src/phys_utils/NCElIncXS.cc:204-233), so a multi-phase
+ * material is the composition of its phases.  After merging: at most 4
+ * components and at most one S(alpha,beta) and one SCBragg process; order
+ * is kept (it fixes the component pick).  With
  * an SCBragg component the composition is oriented (as NCrystal's
  * single-crystal materials: SCBragg for d >= sccutoff, PowderBragg below it,
  * ElInc, S(alpha,beta)): only the calls with neutron directions apply, the
@@ -160,6 +169,13 @@ ncamd_process_t ncamd_create_sabscatter(int nalpha, int nbeta, const double* alp
  * neutrons that pick the SCBragg component before sampling them.  FreeGas
  * and absorption processes are not accepted. */
 ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, const double* scales);
+
+/* Components of a composition after merging, replaces
+ * ProcComposition::components() (ncrystal_core/include/NCrystal/interfaces/
+ * NCProcImpl.hh:328,473).  Returns the number of components; with
+ * 0 <= i < that number, *component receives a new reference to component i
+ * (release it with ncamd_unref) and *scale its scale. */
+int ncamd_composition_component(ncamd_process_t h, int i, ncamd_process_t* component, double* scale);
 
 /* read back tables (for tests and tools) */
 int64_t ncamd_powderbragg_tables(ncamd_process_t, double* e2d, double* fdm, float* fdm32, int64_t maxn);

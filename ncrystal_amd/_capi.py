@@ -111,6 +111,7 @@ def lib():
                                              ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                              _dp]),
         "ncamd_create_composition": (Handle, [ctypes.c_int, ctypes.POINTER(Handle), _dp]),
+        "ncamd_composition_component": (ctypes.c_int, [Handle, ctypes.c_int, ctypes.POINTER(Handle), _dp]),
         "ncamd_powderbragg_tables": (ctypes.c_int64, [Handle, _dp, _dp, _fp, ctypes.c_int64]),
         "ncamd_sab_egrid": (ctypes.c_int, [Handle, _dp, _dp, ctypes.c_int]),
         "ncamd_sab_derived": (ctypes.c_int, [Handle, _dp, _dp]),

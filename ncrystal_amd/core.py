@@ -407,10 +407,26 @@ def create_sabscatter(alpha, beta, sab, temperature, mass_amu, sigma_bound, sugg
 
 
 def create_composition(procs, scales):
+    """ProcComposition of the processes (NCProcImpl.cc:255-338): nested
+    compositions are flattened, repeated processes add their scales,
+    PowderBragg and ElInc components of several phases are merged."""
     arr = (C.Handle * len(procs))(*[p.handle for p in procs])
     sc = C.as_f64(scales)
     h = C.call("ncamd_create_composition", len(procs), arr, C.dptr(sc))
     return Scatter(h, name="ProcComposition", keep=tuple(procs))
+
+
+def composition_components(comp):
+    """[(component, scale)] of a composition after merging
+    (ProcComposition::components(), NCProcImpl.hh:328)."""
+    n = C.call("ncamd_composition_component", comp.handle, -1, None, None)
+    out = []
+    for i in range(n):
+        h = C.Handle()
+        s = np.zeros(1)
+        C.call("ncamd_composition_component", comp.handle, i, ctypes.byref(h), C.dptr(s))
+        out.append((Scatter(h, name="component", keep=(comp,)), float(s[0])))
+    return out
 
 
 class Batch:

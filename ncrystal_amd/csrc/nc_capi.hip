@@ -378,6 +378,9 @@ void pb_guides(Proc* p, const std::vector<double>& e2d, const std::vector<double
   p->pb.c_scale = scale;
 }
 
+void pb_upload(Proc* p, const std::vector<double>& e2d, const std::vector<double>& fdm,
+               const std::vector<double>& contrib, bool fp32);
+
 Proc* create_pb(double v0n, const double* dsp, const double* fsqm, int64_t n, bool fp32) {
   ensure_device();
   if (!(v0n > 0)) raise("BadInput", "v0_times_natoms is not a positive number.");
@@ -414,6 +417,15 @@ Proc* create_pb(double v0n, const double* dsp, const double* fsqm, int64_t n, bo
     }
   }
   if (fdm.empty() || fdm.back() <= 0.0) { fdm.clear(); e2d.clear(); contrib.clear(); }
+  pb_upload(p.get(), e2d, fdm, contrib, fp32);
+  return p.release();
+}
+
+// Device tables of a PowderBragg process from its host tables: 2dE ascending,
+// cumulative fdm, and the per-plane contributions the FP32 cumulative is
+// summed from.
+void pb_upload(Proc* p, const std::vector<double>& e2d, const std::vector<double>& fdm,
+               const std::vector<double>& contrib, bool fp32) {
   p->pb.n = (int)fdm.size();
   p->pb.threshold = fdm.empty() ? __builtin_huge_val() : e2d.front();
   DevBuf* be = p->newbuf();
@@ -423,7 +435,7 @@ Proc* create_pb(double v0n, const double* dsp, const double* fsqm, int64_t n, bo
   p->pb.e2d = be->as<double>();
   p->pb.fdm = bf->as<double>();
   p->pb.fdm32 = nullptr;
-  if (!fdm.empty()) pb_guides(p.get(), e2d, fdm);
+  if (!fdm.empty()) pb_guides(p, e2d, fdm);
   p->pb_e2d_h = e2d;
   p->pb_fdm_h = fdm;
   if (fp32 && !fdm.empty()) {
@@ -439,8 +451,7 @@ Proc* create_pb(double v0n, const double* dsp, const double* fsqm, int64_t n, bo
     p->pb.fdm32 = b32->as<float>();
     p->has_fp32 = true;
   }
-  init_errbuf(p.get());
-  return p.release();
+  init_errbuf(p);
 }
 
 // ------------------------------------------------------------------ ElInc ---
@@ -458,6 +469,116 @@ Proc* create_elinc(int n, const double* msd, const double* bixs, const double* s
   }
   init_errbuf(p.get());
   return p.release();
+}
+
+// ------------------------------------------------------- merged processes ---
+// A composition absorbs the scales of two processes of one kind into a single
+// process where the kind allows it (ProcImpl::Process::createMerged).
+
+// PowderBragg: ncrystal_core/src/powderbragg/NCPowderBragg.cc:218-331.  The
+// 2dE lists are walked in ascending order; points whose d-spacings differ by
+// less than 1e-11 Aa become one point at the mean 2dE; the cumulative fdm is
+// rebuilt with a stable sum of the scaled per-point differences.
+Proc* merge_pb(const Proc* a, double sa, const Proc* b, double sb) {
+  const auto& a_e = a->pb_e2d_h;
+  const auto& b_e = b->pb_e2d_h;
+  std::vector<double> e2d, fdm, contrib;
+  const bool fp32 = a->has_fp32 && b->has_fp32;
+  auto diff = [](const std::vector<double>& c, size_t i) { return i ? c[i] - c[i - 1] : c[0]; };
+  auto dsp = [](double e) { return 0.5 * (e ? std::sqrt(kWl2Ekin / e) : __builtin_huge_val()); };
+  if (a_e.empty() || b_e.empty()) {
+    const Proc* o = a_e.empty() ? b : a;
+    const double so = a_e.empty() ? sb : sa;
+    e2d = o->pb_e2d_h;
+    for (size_t i = 0; i < e2d.size(); ++i) {
+      fdm.push_back(o->pb_fdm_h[i] * so);
+      contrib.push_back(so * diff(o->pb_fdm_h, i));
+    }
+  } else {
+    StableSum s;
+    size_t i = 0, j = 0;
+    auto take = [&](const Proc* o, double so, size_t k) {
+      const double c = so * diff(o->pb_fdm_h, k);
+      e2d.push_back(o->pb_e2d_h[k]);
+      s.add(c);
+      fdm.push_back(s.sum());
+      contrib.push_back(c);
+    };
+    while (i < a_e.size() && j < b_e.size()) {
+      if (std::fabs(dsp(a_e[i]) - dsp(b_e[j])) < 1e-11) {
+        const double ca = sa * diff(a->pb_fdm_h, i), cb = sb * diff(b->pb_fdm_h, j);
+        e2d.push_back(0.5 * (a_e[i] + b_e[j]));
+        s.add(ca);
+        s.add(cb);
+        fdm.push_back(s.sum());
+        contrib.push_back(ca + cb);
+        ++i, ++j;
+      } else if (a_e[i] < b_e[j]) {
+        take(a, sa, i++);
+      } else {
+        take(b, sb, j++);
+      }
+    }
+    while (i < a_e.size()) take(a, sa, i++);
+    while (j < b_e.size()) take(b, sb, j++);
+  }
+  std::unique_ptr<Proc> p(new Proc());
+  p->kind = kKindPB;
+  pb_upload(p.get(), e2d, fdm, contrib, fp32);
+  return p.release();
+}
+
+// ElIncScatter: ncrystal_core/src/phys_utils/NCElIncXS.cc:204-233 and
+// NCElIncScatter.cc:211-223.  The (msd, scaled strength) pairs of both lists,
+// zero strengths dropped, are sorted, and neighbours whose msd agree within
+// |a-b| <= 0.5e-15 (|a|+|b|) + 1e-6 (NCMath.hh:538-546) are summed.
+Proc* merge_elinc(const Proc* a, double sa, const Proc* b, double sb) {
+  std::vector<std::pair<double, double>> t;
+  for (int i = 0; i < a->ei.n; ++i)
+    if (a->ei.bixs[i] * sa != 0.0) t.emplace_back(a->ei.msd[i], a->ei.bixs[i] * sa);
+  for (int i = 0; i < b->ei.n; ++i)
+    if (b->ei.bixs[i] * sb != 0.0) t.emplace_back(b->ei.msd[i], b->ei.bixs[i] * sb);
+  std::sort(t.begin(), t.end());
+  std::vector<std::pair<double, double>> r;
+  for (const auto& e : t) {
+    const double x = r.empty() ? 0.0 : r.back().first;
+    const bool same = !r.empty() && std::fabs(x - e.first) <= 0.5e-15 * (std::fabs(x) + std::fabs(e.first)) + 1e-6;
+    if (same) r.back().second += e.second;
+    else r.push_back(e);
+  }
+  if (r.empty()) raise("BadInput", "merged ElIncScatter has no element with a nonzero cross section");
+  if ((int)r.size() > kMaxElInc) raise("BadInput", "merged ElIncScatter exceeds 16 elements");
+  std::unique_ptr<Proc> p(new Proc());
+  p->kind = kKindElInc;
+  p->ei.n = (int)r.size();
+  for (size_t i = 0; i < r.size(); ++i) {
+    p->ei.msd[i] = r[i].first;
+    p->ei.bixs[i] = r[i].second;
+  }
+  init_errbuf(p.get());
+  return p.release();
+}
+
+// 1/v absorption: ncrystal_core/src/absoov/NCAbsOOV.cc:60-70.
+Proc* merge_abs(const Proc* a, double sa, const Proc* b, double sb) {
+  std::unique_ptr<Proc> p(new Proc());
+  p->kind = kKindAbs;
+  p->abs_c = a->abs_c * sa + b->abs_c * sb;
+  init_errbuf(p.get());
+  return p.release();
+}
+
+// The merged process of a and b (scales sa, sb), or null when the kinds do
+// not merge: different kinds, and the kinds without a createMerged
+// (S(alpha,beta) of distinct tables, SCBragg, FreeGas).
+Proc* try_merge(const Proc* a, double sa, const Proc* b, double sb) {
+  if (a->kind != b->kind) return nullptr;
+  switch (a->kind) {
+    case kKindPB: return merge_pb(a, sa, b, sb);
+    case kKindElInc: return merge_elinc(a, sa, b, sb);
+    case kKindAbs: return merge_abs(a, sa, b, sb);
+    default: return nullptr;
+  }
 }
 
 // ------------------------------------------------------------------- SAB ---
@@ -2267,32 +2388,73 @@ ncamd_process_t ncamd_create_sabscatter(int na, int nb, const double* alpha, con
   NCAMD_CATCH
   return wrap(nullptr);
 }
+// ProcComposition::addComponent (ncrystal_core/src/interfaces/NCProcImpl.cc:
+// 263-331): zero scales and null processes are dropped, a nested composition
+// adds its components with multiplied scales, a process already present adds
+// to its scale, a process that merges with a present one replaces it by the
+// merged process at scale 1, anything else is appended.
+void comp_add(Proc* p, Proc* c, double scale) {
+  if (!(scale >= 0.0) || std::isinf(scale)) raise("BadInput", "component scales must be finite and >= 0");
+  if (c == p) raise("BadInput", "a composition cannot contain itself");
+  if (scale == 0.0 || (c->kind == kKindPB && c->pb.n == 0)) return;
+  if (c->kind == kKindNone) {
+    for (size_t i = 0; i < c->comps.size(); ++i) comp_add(p, c->comps[i], c->scales[i] * scale);
+    return;
+  }
+  if (c->kind == kKindFG || c->kind == kKindAbs)
+    raise("BadInput", "FreeGas and absorption processes are evaluated on their own, not inside a composition");
+  for (size_t i = 0; i < p->comps.size(); ++i) {
+    Proc* e = p->comps[i];
+    if (e == c) {
+      p->scales[i] += scale;
+      return;
+    }
+    if (Proc* m = try_merge(e, p->scales[i], c, scale)) {
+      if (--e->refcount == 0) delete e;
+      p->comps[i] = m;
+      p->scales[i] = 1.0;
+      return;
+    }
+  }
+  c->refcount++;
+  p->comps.push_back(c);
+  p->scales.push_back(scale);
+}
+
 ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, const double* scales) {
   NCAMD_TRY
   ensure_device();
-  if (n < 1 || n > kMaxComp) raise("BadInput", "composition needs 1..4 components");
+  if (n < 1) raise("BadInput", "composition needs at least one component");
   std::unique_ptr<Proc> p(new Proc());
   p->kind = kKindNone;
-  int seen[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int i = 0; i < n; ++i) {
-    Proc* c = extract(procs[i]);
-    if (c->kind == kKindNone) raise("BadInput", "nested compositions are not supported");
-    if (c->kind == kKindSCB) p->oriented_comp = true;  // oriented composition: calls with directions only
-    if (c->kind == kKindFG || c->kind == kKindAbs)
-      raise("BadInput", "FreeGas and absorption processes are evaluated on their own, not inside a composition");
-    if (seen[c->kind]++) raise("BadInput", "at most one process of each kind per composition");
-    if (!(scales[i] >= 0.0) || std::isinf(scales[i])) raise("BadInput", "component scales must be finite and >= 0");
-    // null processes and zero scales are dropped (NCProcImpl.cc:263-283)
-    if (scales[i] == 0.0 || (c->kind == kKindPB && c->pb.n == 0)) continue;
-    c->refcount++;
-    p->comps.push_back(c);
-    p->scales.push_back(scales[i]);
-  }
+  for (int i = 0; i < n; ++i) comp_add(p.get(), extract(procs[i]), scales[i]);
   if (p->comps.empty()) raise("BadInput", "composition has no non-null component");
+  if (p->comps.size() > (size_t)kMaxComp) raise("BadInput", "composition has more than 4 components after merging");
+  int seen[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (Proc* c : p->comps) {
+    if (c->kind == kKindSCB) p->oriented_comp = true;  // oriented composition: calls with directions only
+    if (seen[c->kind]++) raise("BadInput", "at most one S(alpha,beta) and one SCBragg process per composition");
+  }
   init_errbuf(p.get());
   return wrap(p.release());
   NCAMD_CATCH
   return wrap(nullptr);
+}
+
+int ncamd_composition_component(ncamd_process_t h, int i, ncamd_process_t* component, double* scale) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (p->kind != kKindNone) raise("BadInput", "not a composition");
+  const int n = (int)p->comps.size();
+  if (i >= 0 && i < n) {
+    if (!component || !scale) raise("BadInput", "null output pointer");
+    p->comps[i]->refcount++;
+    *component = wrap(p->comps[i]);
+    *scale = p->scales[i];
+  }
+  return n;
+  NCAMD_CATCH
+  return -1;
 }
 
 int64_t ncamd_powderbragg_tables(ncamd_process_t h, double* e2d, double* fdm, float* fdm32, int64_t maxn) {

@@ -58,3 +58,38 @@ class Material:
             comps.append(sb)
             self.components["inelas"] = sb
         self.scatter = core.create_composition(comps, [1.0] * len(comps))
+
+
+def _stable_sum(values):
+    # Neumaier summation (StableSum, ncrystal_core/include/NCrystal/internal/utils/NCMath.hh:194)
+    s = c = 0.0
+    for x in values:
+        t = s + x
+        c += (s - t) + x if abs(s) >= abs(x) else (x - t) + s
+        s = t
+    return s + c
+
+
+def phase_weights(phases):
+    """Scales of the phases of a multi-phase material: volume fraction times
+    number density, normalised by their sum, in ascending order of weight
+    (ncrystal_core/src/factories/NCFactImpl.cc:500-540; phases of equal
+    weight keep the given order).  phases: [(volume fraction, Material)].
+    Returns [(weight, Material)]."""
+    w = [frac * m.crystal.n_atoms / m.crystal.volume for frac, m in phases]
+    tot = _stable_sum(w)
+    if not tot > 0.0:
+        raise ValueError("phases have no atoms")
+    out = [(x / tot, m) for x, (_, m) in zip(w, phases)]
+    return sorted(out, key=lambda e: e[0])
+
+
+class MultiPhase:
+    """Scatter of a multi-phase material: the composition of the phases'
+    scatters with their weights (NCFactImpl.cc:615-642).  The composition
+    flattens the phases and merges their PowderBragg and ElInc components."""
+
+    def __init__(self, phases):
+        self.phases = phase_weights(phases)
+        self.scatter = core.create_composition([m.scatter for _, m in self.phases],
+                                               [w for w, _ in self.phases])

@@ -893,7 +893,7 @@ typedef struct {
   double k1, k2; /* sampler high-E constants */
 } sab_t;
 
-typedef struct { int n; orc_proc** comps; double* scales; } comp_t;
+typedef struct { int n, cap; orc_proc** comps; double* scales; unsigned char* owned; } comp_t;
 /* FreeGas process (ncrystal_core/src/freegas/NCFreeGas.cc:65-77) */
 typedef struct { double T, kT, mass, sigma_free, ca; } fg_t;
 /* AbsOOV: 1/v absorption (ncrystal_core/src/absoov/NCAbsOOV.cc:30-44) */
@@ -1714,20 +1714,141 @@ static int sab_sample_iso(const sab_t* s, rng_t* rng, double E, double* Ef, doub
 }
 
 /* ------------------------------------------------------------------ */
-/* Composition: ncrystal_core/src/interfaces/NCProcImpl.cc:166-204,353-389 */
+/* Merged processes (ProcImpl::Process::createMerged).
+   PowderBragg: ncrystal_core/src/powderbragg/NCPowderBragg.cc:218-331 -- walk
+   both 2dE lists upwards; d-spacings within 1e-11 give one point at the mean
+   2dE; the cumulative is a stable sum of the scaled per-point differences. */
+static double pb_fdm_at(const pb_t* b, int i) { return i ? b->fdm[i] - b->fdm[i - 1] : b->fdm[0]; }
+static double e2d_dsp(double e) { return 0.5 * (e ? sqrt(K_WL2EKIN / e) : INFINITY); }
+static orc_proc* merge_pb(const pb_t* a, double sa, const pb_t* b, double sb) {
+  orc_proc* p = (orc_proc*)calloc(1, sizeof(orc_proc));
+  p->type = P_PB;
+  pb_t* r = &p->u.pb;
+  int cap = a->n + b->n;
+  r->e2d = (double*)malloc(sizeof(double) * (size_t)(cap > 0 ? cap : 1));
+  r->fdm = (double*)malloc(sizeof(double) * (size_t)(cap > 0 ? cap : 1));
+  int m = 0;
+  if (a->n == 0 || b->n == 0) {
+    const pb_t* o = a->n ? a : b;
+    double so = a->n ? sa : sb;
+    for (int i = 0; i < o->n; ++i) { r->e2d[i] = o->e2d[i]; r->fdm[i] = o->fdm[i] * so; }
+    m = o->n;
+  } else {
+    ssum_t s = {0, 0};
+    int i = 0, j = 0;
+    while (i < a->n || j < b->n) {
+      if (i < a->n && j < b->n && fabs(e2d_dsp(a->e2d[i]) - e2d_dsp(b->e2d[j])) < 1e-11) {
+        r->e2d[m] = 0.5 * (a->e2d[i] + b->e2d[j]);
+        ss_add(&s, sa * pb_fdm_at(a, i));
+        ss_add(&s, sb * pb_fdm_at(b, j));
+        ++i; ++j;
+      } else if (j >= b->n || (i < a->n && a->e2d[i] < b->e2d[j])) {
+        r->e2d[m] = a->e2d[i];
+        ss_add(&s, sa * pb_fdm_at(a, i));
+        ++i;
+      } else {
+        r->e2d[m] = b->e2d[j];
+        ss_add(&s, sb * pb_fdm_at(b, j));
+        ++j;
+      }
+      r->fdm[m++] = ss_sum(&s);
+    }
+  }
+  r->n = m;
+  r->threshold = m ? r->e2d[0] : INFINITY;
+  return p;
+}
+/* ElIncScatter: ncrystal_core/src/phys_utils/NCElIncXS.cc:204-233 -- scaled
+   (msd, strength) pairs of both lists without zero strengths, sorted by
+   (msd, strength), neighbours with floateq(msd, msd', 1e-15) (rtol 1e-15,
+   atol 1e-6: NCMath.hh:216,538-546) summed. */
+static int cmp_pair_asc(const void* x, const void* y) { return -cmp_pair_desc(x, y); }
+static orc_proc* merge_elinc(const elinc_t* a, double sa, const elinc_t* b, double sb) {
+  int n = a->n + b->n, m = 0;
+  pairdd* t = (pairdd*)malloc(sizeof(pairdd) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < a->n; ++i) if (a->bixs[i] * sa != 0.0) { t[m].d = a->msd[i]; t[m].f = a->bixs[i] * sa; ++m; }
+  for (int i = 0; i < b->n; ++i) if (b->bixs[i] * sb != 0.0) { t[m].d = b->msd[i]; t[m].f = b->bixs[i] * sb; ++m; }
+  qsort(t, (size_t)m, sizeof(pairdd), cmp_pair_asc);
+  orc_proc* p = (orc_proc*)calloc(1, sizeof(orc_proc));
+  p->type = P_ELINC;
+  p->u.ei.msd = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+  p->u.ei.bixs = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+  int k = 0;
+  for (int i = 0; i < m; ++i) {
+    if (k && fabs(p->u.ei.msd[k - 1] - t[i].d) <= 0.5e-15 * (fabs(p->u.ei.msd[k - 1]) + fabs(t[i].d)) + 1e-6) {
+      p->u.ei.bixs[k - 1] += t[i].f;
+    } else {
+      p->u.ei.msd[k] = t[i].d; p->u.ei.bixs[k] = t[i].f; ++k;
+    }
+  }
+  p->u.ei.n = k;
+  free(t);
+  return p;
+}
+static orc_proc* try_merge(const orc_proc* a, double sa, const orc_proc* b, double sb) {
+  if (a->type != b->type) return NULL;
+  switch (a->type) {
+    case P_PB: return merge_pb(&a->u.pb, sa, &b->u.pb, sb);
+    case P_ELINC: return merge_elinc(&a->u.ei, sa, &b->u.ei, sb);
+    case P_ABS: {  /* ncrystal_core/src/absoov/NCAbsOOV.cc:60-70 */
+      orc_proc* p = (orc_proc*)calloc(1, sizeof(orc_proc));
+      p->type = P_ABS;
+      p->u.ab.c = a->u.ab.c * sa + b->u.ab.c * sb;
+      return p;
+    }
+  }
+  return NULL;
+}
+
+/* ------------------------------------------------------------------ */
+/* Composition: ncrystal_core/src/interfaces/NCProcImpl.cc:166-204,353-389;
+   components are added as ProcComposition::addComponent does
+   (NCProcImpl.cc:263-338): zero scales and null processes dropped, nested
+   compositions flattened with multiplied scales, a process already present
+   adds to its scale, one that merges with a present one replaces it by the
+   merged process at scale 1, else appended. */
+void orc_free(orc_proc* p);
+static void comp_add(comp_t* c, orc_proc* q, double scale) {
+  if (scale == 0.0 || (q->type == P_PB && q->u.pb.n == 0)) return;
+  if (q->type == P_COMP) {
+    for (int i = 0; i < q->u.comp.n; ++i) comp_add(c, q->u.comp.comps[i], q->u.comp.scales[i] * scale);
+    return;
+  }
+  for (int i = 0; i < c->n; ++i) {
+    if (c->comps[i] == q) { c->scales[i] += scale; return; }
+    orc_proc* m = try_merge(c->comps[i], c->scales[i], q, scale);
+    if (m) {
+      if (c->owned[i]) orc_free(c->comps[i]);
+      c->comps[i] = m; c->scales[i] = 1.0; c->owned[i] = 1;
+      return;
+    }
+  }
+  if (c->n == c->cap) {
+    c->cap = c->cap ? 2 * c->cap : 4;
+    c->comps = (orc_proc**)realloc(c->comps, sizeof(orc_proc*) * (size_t)c->cap);
+    c->scales = (double*)realloc(c->scales, sizeof(double) * (size_t)c->cap);
+    c->owned = (unsigned char*)realloc(c->owned, (size_t)c->cap);
+  }
+  c->comps[c->n] = q; c->scales[c->n] = scale; c->owned[c->n] = 0; ++c->n;
+}
 orc_proc* orc_create_composition(int n, orc_proc* const* comps, const double* scales) {
-  /* null processes and zero scales are dropped (NCProcImpl.cc:263-283) */
   orc_proc* p = (orc_proc*)calloc(1, sizeof(orc_proc));
   p->type = P_COMP;
-  p->u.comp.comps = (orc_proc**)malloc(sizeof(orc_proc*) * (size_t)(n > 0 ? n : 1));
-  p->u.comp.scales = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
-  int m = 0;
-  for (int i = 0; i < n; ++i) {
-    if (scales[i] == 0.0 || (comps[i]->type == P_PB && comps[i]->u.pb.n == 0)) continue;
-    p->u.comp.comps[m] = comps[i]; p->u.comp.scales[m] = scales[i]; ++m;
-  }
-  p->u.comp.n = m;
+  for (int i = 0; i < n; ++i) comp_add(&p->u.comp, comps[i], scales[i]);
   return p;
+}
+int orc_comp_ncomponents(const orc_proc* p) { return p->type == P_COMP ? p->u.comp.n : -1; }
+/* process type (P_*) and scale of component i */
+int orc_comp_component(const orc_proc* p, int i, double* scale) {
+  const orc_proc* c = p->u.comp.comps[i];
+  *scale = p->u.comp.scales[i];
+  return c->type;
+}
+const orc_proc* orc_comp_component_ptr(const orc_proc* p, int i) { return p->u.comp.comps[i]; }
+int orc_elinc_n(const orc_proc* p) { return p->type == P_ELINC ? p->u.ei.n : -1; }
+void orc_elinc_data(const orc_proc* p, double* msd, double* bixs) {
+  memcpy(msd, p->u.ei.msd, sizeof(double) * (size_t)p->u.ei.n);
+  memcpy(bixs, p->u.ei.bixs, sizeof(double) * (size_t)p->u.ei.n);
 }
 static double proc_xs(const orc_proc* p, double E);
 static int proc_sample_iso(const orc_proc* p, rng_t* rng, double E, double* Ef, double* mu);
@@ -1813,7 +1934,10 @@ void orc_free(orc_proc* p) {
       free(s->sab); free(s->logsab); free(s->cumul);
       break;
     }
-    case P_COMP: free(p->u.comp.comps); free(p->u.comp.scales); break;
+    case P_COMP:
+      for (int i = 0; i < p->u.comp.n; ++i) if (p->u.comp.owned[i]) orc_free(p->u.comp.comps[i]);
+      free(p->u.comp.comps); free(p->u.comp.scales); free(p->u.comp.owned);
+      break;
     case P_SCB: scb_free(p); break;
   }
   free(p);

@@ -112,6 +112,11 @@ void orc_sab_derived(const orc_proc*, double* logsab, double* cumul);
 double orc_sab_analyse_xs(const orc_proc*, double ekin);
 
 orc_proc* orc_create_composition(int n, orc_proc* const* comps, const double* scales);
+int orc_comp_ncomponents(const orc_proc*);
+int orc_comp_component(const orc_proc*, int i, double* scale);
+const orc_proc* orc_comp_component_ptr(const orc_proc*, int i);
+int orc_elinc_n(const orc_proc*);
+void orc_elinc_data(const orc_proc*, double* msd, double* bixs);
 void orc_free(orc_proc*);
 
 /* batched evaluation (nthreads<=1: single thread) */

@@ -99,6 +99,12 @@ def lib():
         L.orc_create_composition.argtypes = [ctypes.c_int, ctypes.POINTER(_vp), _dp]
         L.orc_create_composition.restype = _vp
         L.orc_free.argtypes = [_vp]
+        L.orc_comp_ncomponents.argtypes = [_vp]
+        L.orc_comp_component.argtypes = [_vp, ctypes.c_int, _dp]
+        L.orc_comp_component_ptr.argtypes = [_vp, ctypes.c_int]
+        L.orc_comp_component_ptr.restype = _vp
+        L.orc_elinc_n.argtypes = [_vp]
+        L.orc_elinc_data.argtypes = [_vp, _dp, _dp]
         L.orc_xs_many.argtypes = [_vp, _dp, ctypes.c_int64, _dp, ctypes.c_int]
         L.orc_sample_iso_many.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _dp,
                                           ctypes.c_int64, ctypes.c_int64, _dp, _dp, ctypes.c_int]
@@ -250,11 +256,38 @@ def powderbragg(v0_times_natoms, dsp, fsq_x_mult):
 
 
 def pb_tables(p):
-    n = lib().orc_pb_ntables(p.ptr)
+    ptr = p if isinstance(p, int) else p.ptr
+    n = lib().orc_pb_ntables(ptr)
     e2d = np.empty(n)
     fdm = np.empty(n)
-    lib().orc_pb_tables(p.ptr, _d(e2d), _d(fdm))
+    lib().orc_pb_tables(ptr, _d(e2d), _d(fdm))
     return e2d, fdm
+
+
+# process types of the checker's components
+PROC_TYPES = {1: "PowderBragg", 2: "ElInc", 3: "SAB", 4: "Composition", 5: "SCBragg", 6: "FreeGas", 7: "AbsOOV"}
+
+
+def components(p):
+    """[(type name, scale, data)] of a composition after merging: data is
+    (e2d, cumulative fdm) for PowderBragg, (msd, scaled bixs) for ElInc, the
+    raw component pointer otherwise."""
+    out = []
+    for i in range(lib().orc_comp_ncomponents(p.ptr)):
+        sc = np.zeros(1)
+        t = lib().orc_comp_component(p.ptr, i, _d(sc))
+        ptr = lib().orc_comp_component_ptr(p.ptr, i)
+        if t == 1:
+            data = pb_tables(ptr)
+        elif t == 2:
+            n = lib().orc_elinc_n(ptr)
+            msd, bixs = np.empty(n), np.empty(n)
+            lib().orc_elinc_data(ptr, _d(msd), _d(bixs))
+            data = (msd, bixs)
+        else:
+            data = ptr
+        out.append((PROC_TYPES[t], float(sc[0]), data))
+    return out
 
 
 def elinc(msd, bixs, scale):
