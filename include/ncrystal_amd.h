@@ -113,8 +113,10 @@ double ncamd_debye_msd(double debye_temp, double temperature, double mass_amu);
  * isotropic scattering by FreeGasSampler::sampleDeltaEMu
  * (ncrystal_core/include/NCrystal/internal/phys_utils/NCFreeGasUtils.hh:146-160).
  * temperature [K], target mass [amu], free scattering cross section [barn]
- * (NCrystal's SigmaBound::free: sigma_bound * (A/(1+A))^2).  Evaluated on
- * its own, not inside compositions or many-material batches.  Sampling
+ * (NCrystal's SigmaBound::free: sigma_bound * (A/(1+A))^2).  On its own or
+ * as a component of a composition (gas mixtures: one FreeGas process per
+ * element, with PowderBragg and ElInc components at most); not in
+ * many-material batches.  Sampling
  * refuses energies that are not finite and >= 0 (poisoned outputs, counted
  * as errors). */
 ncamd_process_t ncamd_create_freegas(double temperature, double mass_amu, double sigma_free);
@@ -122,7 +124,9 @@ ncamd_process_t ncamd_create_freegas(double temperature, double mass_amu, double
 /* 1/v absorption process: replaces AbsOOV(SigmaAbsorption)
  * (ncrystal_core/src/absoov/NCAbsOOV.cc:30-44): xs = sigma_abs
  * sqrt(E_2200m/s / E), infinite at E = 0.  Cross-section calls only; the
- * sampling calls raise BadInput. */
+ * sampling calls raise BadInput.  Compositions of absorption processes merge
+ * them into one (c = s1 c1 + s2 c2, NCAbsOOV.cc:60-70) and hold no
+ * scattering process. */
 ncamd_process_t ncamd_create_absoov(double sigma_abs);
 
 /* PowderBragg from (d-spacing, |F|^2*multiplicity) pairs; replaces
@@ -167,7 +171,8 @@ ncamd_process_t ncamd_create_sabscatter(int nalpha, int nbeta, const double* alp
  * isotropic components scatter through randDirectionGivenScatterMu
  * (NCProcImpl.cc:29-37), and the sampling call waits for the number of
  * neutrons that pick the SCBragg component before sampling them.  FreeGas
- * and absorption processes are not accepted. */
+ * components combine with PowderBragg and ElInc components only; a
+ * composition holds either scattering or absorption processes. */
 ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, const double* scales);
 
 /* Components of a composition after merging, replaces

@@ -309,7 +309,12 @@ CompView make_view(Proc* p) {
         dlo = c->scb.threshold;
         break;
       case kKindFG:
-      case kKindAbs: raise("BadInput", "FreeGas and absorption processes are evaluated on their own");
+      case kKindAbs:
+        // a single FreeGas or absorption process has its own kernels
+        if (p->kind != kKindNone) raise("LogicError", "FreeGas and absorption processes are evaluated on their own");
+        if (c->kind == kKindFG) v.fg[i] = c->fg;
+        else v.absc[i] = c->abs_c;
+        break;
       default: raise("LogicError", "bad component kind");
     }
     if (dlo < dhi) { lo = std::min(lo, dlo); hi = std::max(hi, dhi); }
@@ -2111,13 +2116,18 @@ void xs_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat, double*
 }
 
 // scattering with new directions for an isotropic process or composition
+// an absorption process, or a composition of absorption processes
+bool is_absorption(const Proc* p) {
+  return p->kind == kKindAbs || (p->kind == kKindNone && !p->comps.empty() && p->comps[0]->kind == kKindAbs);
+}
+
 void sample_dirs_dispatch(Proc* p, const double* ekin, const double* dirs, int64_t n, double* oE, double* oD,
                           hipStream_t st) {
   if (p->oriented_comp) {
     oriented_sample(p, ekin, dirs, n, oE, oD, st);
     return;
   }
-  if (p->kind == kKindAbs) raise("BadInput", "an absorption process does not scatter");
+  if (is_absorption(p)) raise("BadInput", "an absorption process does not scatter");
   if (p->kind == kKindFG) {
     launch_fg_sample(p->fg, ekin, dirs, n, 1, p->seed, next_stream(p), oE, oD, p->d_err.as<unsigned long long>(),
                      st);
@@ -2132,7 +2142,7 @@ void sample_dirs_dispatch(Proc* p, const double* ekin, const double* dirs, int64
 void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat, double* oE, double* oMu,
                          hipStream_t st) {
   if (p->oriented_comp) raise("BadInput", "oriented process: use the calls with neutron directions");
-  if (p->kind == kKindAbs) raise("BadInput", "an absorption process does not scatter");
+  if (is_absorption(p)) raise("BadInput", "an absorption process does not scatter");
   if (p->kind == kKindFG) {
     launch_fg_sample(p->fg, ekin, nullptr, n, repeat, p->seed, next_stream(p), oE, oMu,
                      p->d_err.as<unsigned long long>(), st);
@@ -2401,8 +2411,6 @@ void comp_add(Proc* p, Proc* c, double scale) {
     for (size_t i = 0; i < c->comps.size(); ++i) comp_add(p, c->comps[i], c->scales[i] * scale);
     return;
   }
-  if (c->kind == kKindFG || c->kind == kKindAbs)
-    raise("BadInput", "FreeGas and absorption processes are evaluated on their own, not inside a composition");
   for (size_t i = 0; i < p->comps.size(); ++i) {
     Proc* e = p->comps[i];
     if (e == c) {
@@ -2433,8 +2441,14 @@ ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, co
   int seen[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (Proc* c : p->comps) {
     if (c->kind == kKindSCB) p->oriented_comp = true;  // oriented composition: calls with directions only
-    if (seen[c->kind]++) raise("BadInput", "at most one S(alpha,beta) and one SCBragg process per composition");
+    if (c->kind != kKindFG && seen[c->kind]) raise("BadInput", "at most one S(alpha,beta) and one SCBragg process per composition");
+    seen[c->kind]++;
   }
+  // one process type per composition (NCProcImpl.cc:269-272)
+  if (seen[kKindAbs] && seen[kKindAbs] != (int)p->comps.size())
+    raise("BadInput", "a composition holds either scattering or absorption processes");
+  if (seen[kKindFG] && (seen[kKindSAB] || seen[kKindSCB]))
+    raise("BadInput", "FreeGas components combine with PowderBragg and ElInc components only");
   init_errbuf(p.get());
   return wrap(p.release());
   NCAMD_CATCH
@@ -2748,6 +2762,10 @@ ncamd_batch_t ncamd_batch_create(int nmat, const ncamd_process_t* procs, const i
     Proc* p = extract(procs[i]);
     if (counts[i] < 0) raise("BadInput", "negative neutron count");
     if (p->oriented_comp) raise("BadInput", "oriented compositions are not supported in batches");
+    if (p->kind == kKindNone)
+      for (Proc* c : p->comps)
+        if (c->kind == kKindFG || c->kind == kKindAbs)
+          raise("BadInput", "compositions with FreeGas or absorption components are not supported in batches");
     const CompView v = make_view(p);  // raises for oriented processes
     p->refcount++;
     b->mats.push_back(p);

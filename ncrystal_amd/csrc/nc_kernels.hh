@@ -25,6 +25,10 @@ struct CompView {
   SABView sab;
   double scb_threshold;  // domain of an SCBragg component (oriented compositions)
   int has_scb;           // 1: oriented composition with an SCBragg component
+  // FreeGas components (gas mixtures: one per element) and the 1/v
+  // absorption constant of an absorption component, by component index
+  FGView fg[kMaxComp];
+  double absc[kMaxComp];
 };
 
 // Composition semantics: ncrystal_core/src/interfaces/NCProcImpl.cc:166-204
@@ -47,8 +51,10 @@ NCD double pb_xs_sel(const PBView& pb, double E) {
 }
 
 // ext: the cross section of an SCBragg component for this neutron and
-// direction (oriented compositions; computed by the SCBragg kernels)
-template <bool FP32PB>
+// direction (oriented compositions; computed by the SCBragg kernels).
+// Gas: the composition may hold FreeGas or absorption components (only the
+// kernels of such compositions carry their code).
+template <bool FP32PB, bool Gas = false>
 NCD double comp_xs(const CompView& c, double E, double* commul, double ext = 0.0) {
   if (comp_outside(c, E)) return 0.0;
   double tot = 0.0;
@@ -59,6 +65,12 @@ NCD double comp_xs(const CompView& c, double E, double* commul, double ext = 0.0
       case kKindElInc: xs = elinc_xs(c.ei, E, nullptr); break;
       case kKindSAB: xs = sab_xs(c.sab, E); break;
       case kKindSCB: xs = E >= c.scb_threshold ? ext : 0.0; break;
+      case kKindFG:
+        if constexpr (Gas) xs = fg_xs(c.fg[i], E);
+        break;
+      case kKindAbs:
+        if constexpr (Gas) xs = absoov_xs(c.absc[i], E);
+        break;
       default: break;
     }
     tot += c.scale[i] * xs;
@@ -71,12 +83,14 @@ NCD double comp_xs(const CompView& c, double E, double* commul, double ext = 0.0
 // only the samplers it can reach: PowderBragg only; no S(alpha,beta) part
 // (one kernel); with an S(alpha,beta) part (the staged kernels below, since
 // the S(alpha,beta) sampler's registers would hold every other sampler at
-// one wave per SIMD).
-enum : int { kCompGeneral = 0, kCompNoSAB = 1, kCompPBOnly = 2 };
+// one wave per SIMD); with FreeGas components (gas mixtures, whose kernel
+// alone carries the free-gas sampler).
+enum : int { kCompGeneral = 0, kCompNoSAB = 1, kCompPBOnly = 2, kCompFG = 3 };
 
 template <int Variant>
 NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, double& mu) {
-  static_assert(Variant == kCompNoSAB || Variant == kCompPBOnly, "compositions with S(alpha,beta) are staged");
+  static_assert(Variant == kCompNoSAB || Variant == kCompPBOnly || Variant == kCompFG,
+                "compositions with S(alpha,beta) are staged");
   if (comp_outside(c, E)) { Ef = E; mu = 1.0; return kOK; }
   if constexpr (Variant == kCompPBOnly) {
     // a single PowderBragg component: no pick (NCProcImpl.cc:379-389 draws
@@ -86,11 +100,14 @@ NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, doubl
     return kOK;
   }
   double commul[kMaxComp];
-  comp_xs<false>(c, E, commul);
+  comp_xs<false, Variant == kCompFG>(c, E, commul);
   const int ic = pick_idx(rng, commul, c.n);
   switch (c.kind[ic]) {
     case kKindPB: Ef = E; mu = pb_sample_mu(c.pb, rng, E); return kOK;
     case kKindElInc: Ef = E; mu = elinc_sample_mu(c.ei, rng, E); return kOK;
+    case kKindFG:
+      if constexpr (Variant == kCompFG) return fg_sample_demu(c.fg[ic], rng, E, Ef, mu);
+      return kErrBadSampler;
     default: return kErrBadSampler;
   }
 }
@@ -121,6 +138,8 @@ NCD int comp_pick_sample(const CompView& c, Rng& rng, double E, double& Ef, doub
 
 NCHD int comp_variant(const CompView& c) {
   if (c.n == 1 && c.kind[0] == kKindPB) return kCompPBOnly;
+  for (int i = 0; i < c.n; ++i)  // never with S(alpha,beta) or SCBragg (refused at creation)
+    if (c.kind[i] == kKindFG || c.kind[i] == kKindAbs) return kCompFG;
   for (int i = 0; i < c.n; ++i)
     if (c.kind[i] == kKindSAB || c.kind[i] == kKindSCB) return kCompGeneral;
   return kCompNoSAB;

@@ -9,18 +9,19 @@
 
 namespace ncamd {
 
-// PBOnly: a single PowderBragg component (the same sum as comp_xs: 0 + scale * xs)
-template <bool FP32PB, bool PBOnly>
+// Variant kCompPBOnly: a single PowderBragg component (the same sum as
+// comp_xs: 0 + scale * xs); kCompFG: FreeGas / absorption components
+template <bool FP32PB, int Variant>
 __global__ void __launch_bounds__(256) k_xs_iso(CompView c, const double* __restrict__ ekin,
                                                 int64_t n, int64_t repeat, double* __restrict__ out) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const double E = ekin[i];
     double xs;
-    if constexpr (PBOnly) {
+    if constexpr (Variant == kCompPBOnly) {
       xs = comp_outside(c, E) ? 0.0 : 0.0 + c.scale[0] * (pb_active(c.pb, E) ? pb_xs_sel<FP32PB>(c.pb, E) : 0.0);
     } else {
-      xs = comp_xs<FP32PB>(c, E, nullptr);
+      xs = comp_xs<FP32PB, Variant == kCompFG>(c, E, nullptr);
     }
     for (int64_t r = 0; r < repeat; ++r) out[r * n + i] = xs;
   }
@@ -339,12 +340,16 @@ void launch_xs_iso(const CompView& c, const double* ekin, int64_t n, int64_t rep
                    bool fp32, hipStream_t st) {
   if (n <= 0 || repeat <= 0) return;
   const dim3 g(grid_for(n, 256)), b(256);
-  if (comp_variant(c) == kCompPBOnly) {
-    if (fp32) hipLaunchKernelGGL((k_xs_iso<true, true>), g, b, 0, st, c, ekin, n, repeat, out);
-    else hipLaunchKernelGGL((k_xs_iso<false, true>), g, b, 0, st, c, ekin, n, repeat, out);
+  const int v = comp_variant(c);
+  if (v == kCompPBOnly) {
+    if (fp32) hipLaunchKernelGGL((k_xs_iso<true, kCompPBOnly>), g, b, 0, st, c, ekin, n, repeat, out);
+    else hipLaunchKernelGGL((k_xs_iso<false, kCompPBOnly>), g, b, 0, st, c, ekin, n, repeat, out);
+  } else if (v == kCompFG) {
+    if (fp32) hipLaunchKernelGGL((k_xs_iso<true, kCompFG>), g, b, 0, st, c, ekin, n, repeat, out);
+    else hipLaunchKernelGGL((k_xs_iso<false, kCompFG>), g, b, 0, st, c, ekin, n, repeat, out);
   } else {
-    if (fp32) hipLaunchKernelGGL((k_xs_iso<true, false>), g, b, 0, st, c, ekin, n, repeat, out);
-    else hipLaunchKernelGGL((k_xs_iso<false, false>), g, b, 0, st, c, ekin, n, repeat, out);
+    if (fp32) hipLaunchKernelGGL((k_xs_iso<true, kCompNoSAB>), g, b, 0, st, c, ekin, n, repeat, out);
+    else hipLaunchKernelGGL((k_xs_iso<false, kCompNoSAB>), g, b, 0, st, c, ekin, n, repeat, out);
   }
 }
 void launch_sample_iso(const CompView& c, const double* ekin, int64_t n, int64_t repeat, uint64_t seed,
@@ -359,6 +364,10 @@ void launch_sample_iso(const CompView& c, const double* ekin, int64_t n, int64_t
       break;
     case kCompNoSAB:
       hipLaunchKernelGGL(k_sample_iso<kCompNoSAB>, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin, n, total,
+                         seed, stream, oE, oMu, nerr);
+      break;
+    case kCompFG:
+      hipLaunchKernelGGL(k_sample_iso<kCompFG>, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin, n, total,
                          seed, stream, oE, oMu, nerr);
       break;
     default: {
@@ -408,6 +417,10 @@ void launch_sample_dirs(const CompView& c, const double* ekin, const double* dir
       break;
     case kCompNoSAB:
       hipLaunchKernelGGL(k_sample_dirs<kCompNoSAB>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, dirs, n,
+                         seed, stream, oE, oD, nerr);
+      break;
+    case kCompFG:
+      hipLaunchKernelGGL(k_sample_dirs<kCompFG>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, dirs, n,
                          seed, stream, oE, oD, nerr);
       break;
     default: {

@@ -91,13 +91,18 @@ def test_absoov(amd, oracle):
         C.call("ncamd_dev_samplescatterisotropic_many", a._h, d.ptr, len(e), 1, d1.ptr, d2.ptr, st.ptr)
 
 
-def test_not_in_compositions(amd):
+def test_composition_rules(amd):
     C, core = amd
     g = core.create_freegas(293.15, 4.0026, 0.7596)
     a = core.create_absoov(0.231)
     e = core.create_elincscatter([0.01], [1.0], [1.0])
-    for x in (g, a):
+    # scattering and absorption processes do not mix (NCProcImpl.cc:269-272)
+    for x in (e, g):
         with pytest.raises(C.NCBadInput):
-            core.create_composition([e, x], [1.0, 1.0])
+            core.create_composition([x, a], [1.0, 1.0])
+    t = synth.small_sab_table(mass_amu=4.0026)
+    sab = core.create_sabscatter(t["alpha"], t["beta"], t["sab"], t["temperature"], t["mass"], t["sigma_bound"])
+    with pytest.raises(C.NCBadInput):
+        core.create_composition([g, sab], [1.0, 1.0])
     with pytest.raises(C.NCBadInput):
         a.crossSectionIsotropic_fp32(np.array([0.025]))
