@@ -125,7 +125,11 @@ NCD int comp_pick_sample(const CompView& c, Rng& rng, double E, double& Ef, doub
   if (comp_outside(c, E)) { Ef = E; mu = 1.0; return kOK; }
   double commul[kMaxComp];
   const double tot = comp_xs<false>(c, E, commul, ext);
-  if (!(tot > 0.0)) { Ef = E; mu = 1.0; return kOK; }  // no component scatters (NCProcImpl.cc:379-389)
+  // No component scatters this neutron (an SCBragg component off every
+  // reflection, the others outside their domains): the reference
+  // (NCProcImpl.cc:363-377) still picks the last component, whose sampling
+  // then leaves the neutron as it is; the pick is skipped here.
+  if (!(tot > 0.0)) { Ef = E; mu = 1.0; return kOK; }
   const int ic = pick_idx(rng, commul, c.n);
   switch (c.kind[ic]) {
     case kKindPB: Ef = E; mu = pb_sample_mu(c.pb, rng, E); return kOK;
