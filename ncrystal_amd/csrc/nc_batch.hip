@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(kBatchBlock) k_batch_sab(const BatchMat* __res
   }
 }
 
-__global__ void __launch_bounds__(kBatchBlock) k_batch_highe(const BatchMat* __restrict__ mats,
+__global__ void __launch_bounds__(kBatchBlock, 2) k_batch_highe(const BatchMat* __restrict__ mats,
                                                              const BatchRng* __restrict__ rngs,
                                                              const BatchTile* __restrict__ tiles,
                                                              const double* __restrict__ ekin, double* __restrict__ o1,
