@@ -383,6 +383,9 @@ void ncamd_batch_set_tuning(ncamd_batch_t, int tile, int nblocks, int ordering);
 /* times the candidate tunings on these inputs (outputs are scratch, RNG
  * states unchanged), keeps the fastest and returns its time [ms] */
 double ncamd_dev_batch_autotune(ncamd_batch_t, const double* d_ekin, double* d_o1, double* d_o2, void* stream);
+/* how often the graph of a mode (0 cross sections, 1 sampling) was captured
+ * and how often it was replayed unchanged */
+void ncamd_batch_graph_stats(ncamd_batch_t, int mode, int64_t* captures, int64_t* replays);
 
 /* ---------- device memory, streams and events (plumbing for callers that
  * do not bring their own HIP runtime objects; no counterpart in NCrystal) ---- */

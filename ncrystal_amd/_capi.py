@@ -161,6 +161,7 @@ def lib():
         "ncamd_dev_batch_check_errors": (ctypes.c_int64, [BatchHandle, _vp]),
         "ncamd_batch_get_tuning": (None, [BatchHandle, _ip, _ip, _ip]),
         "ncamd_batch_set_tuning": (None, [BatchHandle, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "ncamd_batch_graph_stats": (None, [BatchHandle, ctypes.c_int, _i64p, _i64p]),
         "ncamd_dev_batch_autotune": (ctypes.c_double, [BatchHandle, _vp, _vp, _vp, _vp]),
         "ncamd_dev_malloc": (_vp, [ctypes.c_uint64]),
         "ncamd_dev_free": (None, [_vp]),

@@ -470,6 +470,12 @@ class Batch:
         C.call("ncamd_batch_get_tuning", self._h, ctypes.byref(t), ctypes.byref(b), ctypes.byref(o))
         return dict(tile=t.value, nblocks=b.value, ordering=o.value)
 
+    def graph_stats(self, mode):
+        """(captures, replays) of the graph of mode 0 (xs) or 1 (sampling)"""
+        c, r = ctypes.c_int64(), ctypes.c_int64()
+        C.call("ncamd_batch_graph_stats", self._h, int(mode), ctypes.byref(c), ctypes.byref(r))
+        return c.value, r.value
+
     def set_tuning(self, tile, nblocks, ordering):
         C.call("ncamd_batch_set_tuning", self._h, int(tile), int(nblocks), int(ordering))
 

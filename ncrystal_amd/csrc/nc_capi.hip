@@ -601,19 +601,6 @@ std::vector<double> geomspace(double start, double stop, unsigned num) {
   return v;
 }
 
-std::vector<double> erfc_table() {
-  std::vector<double> t(kErfcTabLen);
-  const int nedges = kErfcTabLen - 2;
-  t[0] = 2.0;
-  const double start = -2.0, stop = 9.0;
-  const unsigned nm1 = (unsigned)nedges - 1;
-  const double interval = (stop - start) / nm1;
-  for (unsigned i = 0; i < nm1; ++i) t[1 + i] = erfc(start + i * interval);
-  t[1 + nm1] = erfc(stop);
-  t[kErfcTabLen - 1] = 0.0;
-  return t;
-}
-
 Proc* create_sab(int na, int nb, const double* alpha, const double* beta, const double* sab, double T,
                  double mass, double sigma_bound, double suggested_emax, int n_egrid, const double* egrid_in) {
   ensure_device();
@@ -638,6 +625,7 @@ Proc* create_sab(int na, int nb, const double* alpha, const double* beta, const 
     const double r = mass / (kNeutronMassAmu + mass);
     s.sigma_free = (r * r) * sigma_bound;
     const double A = (1.0 / kNeutronMassAmu) * mass;
+    s.fg_A = A;
     s.fg_ca = A / s.kT;
   }
   s.egrid_margin = 1.05;
@@ -776,8 +764,6 @@ Proc* create_sab(int na, int nb, const double* alpha, const double* beta, const 
     s.k1 = xs.back() * emax;
     s.k2 = extxs * emax;
   }
-  std::vector<double> et = erfc_table();
-  DevBuf* bet = p->newbuf(); bet->upload(et.data(), et.size());
   // egrid IEEE-bit bucket table: buckets narrower than half the smallest
   // relative egrid spacing so a bracket holds at most ~2 grid points
   {
@@ -819,7 +805,6 @@ Proc* create_sab(int na, int nb, const double* alpha, const double* beta, const 
   s.scdf = bcdf->as<double>();
   s.sxy = bxy->as<double2>();
   s.sinfo = binf->as<AInfo>();
-  s.erfc_tab = bet->as<double>();
   {
     DevBuf* bcg = p->newbuf(); bcg->alloc(sizeof(unsigned short) * (size_t)ne * (kCdfGuide + 1));
     DevBuf* brg = p->newbuf(); brg->alloc(sizeof(unsigned short) * (size_t)nb * (kRowGuide + 1));
@@ -1525,14 +1510,24 @@ struct Batch {
   int tile = 4096, nblocks = 1024, ordering = 1;
   int ntiles = 0;
   bool tiles_valid = false;
-  // captured graph of the last (mode, pointers, tuning) combination
+  // one captured graph per mode (0 cross sections, 1 sampling), each kept
+  // with the pointers and tuning it was captured for, so that callers that
+  // alternate the two calls replay both graphs
+  struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    int tile = 0, nblocks = 0, ordering = -1;
+    const void *in = nullptr, *o1 = nullptr, *o2 = nullptr;
+    long long captures = 0, replays = 0;
+  };
+  GraphSlot graphs[2];
   hipStream_t cap_stream = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  int g_mode = -1, g_tile = 0, g_nblocks = 0, g_ordering = -1;
-  const void *g_in = nullptr, *g_o1 = nullptr, *g_o2 = nullptr;
+  void drop_graphs() {
+    for (GraphSlot& g : graphs)
+      if (g.exec) { (void)hipGraphExecDestroy(g.exec); g.exec = nullptr; }
+  }
   ~Batch() {
     magic = 0;
-    if (gexec) (void)hipGraphExecDestroy(gexec);
+    drop_graphs();
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (auto* m : mats)
       if (m && --m->refcount == 0) delete m;
@@ -1570,8 +1565,8 @@ void batch_build_tiles(Batch* b) {
       tiles.push_back(BatchTile{m, k, std::min<int64_t>(b->counts[m], k + b->tile)});
   // the high-energy queue packs (tile << 40 | neutron)
   if (tiles.size() >= ((size_t)1 << 23)) raise("BadInput", "too many tiles (use a larger tile size)");
-  // the captured graph refers to the old tile list
-  if (b->gexec) { NCAMD_HIP(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
+  // captured graphs refer to the old tile list
+  b->drop_graphs();
   b->ntiles = (int)tiles.size();
   b->d_tiles.upload(tiles.data(), tiles.size());
   b->d_tcount.ensure(sizeof(int) * std::max<size_t>(tiles.size(), 1));
@@ -1587,15 +1582,16 @@ void batch_upload_rngs(Batch* b, bool advance, hipStream_t st) {
                            hipMemcpyHostToDevice, st));
 }
 
-// launch through a captured graph (re-captured when the mode, the pointers or
+// launch through the mode's captured graph (re-captured when its pointers or
 // the tuning change)
 void batch_run(Batch* b, int mode, const double* in, double* o1, double* o2, hipStream_t st) {
   if (!b->tiles_valid) batch_build_tiles(b);
   if (!b->cap_stream) NCAMD_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
-  const bool same = b->gexec && b->g_mode == mode && b->g_in == in && b->g_o1 == o1 && b->g_o2 == o2 &&
-                    b->g_tile == b->tile && b->g_nblocks == b->nblocks && b->g_ordering == b->ordering;
+  Batch::GraphSlot& gs = b->graphs[mode ? 1 : 0];
+  const bool same = gs.exec && gs.in == in && gs.o1 == o1 && gs.o2 == o2 && gs.tile == b->tile &&
+                    gs.nblocks == b->nblocks && gs.ordering == b->ordering;
   if (!same) {
-    if (b->gexec) { NCAMD_HIP(hipGraphExecDestroy(b->gexec)); b->gexec = nullptr; }
+    if (gs.exec) { NCAMD_HIP(hipGraphExecDestroy(gs.exec)); gs.exec = nullptr; }
     hipGraph_t g = nullptr;
     NCAMD_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
     BatchWork w;
@@ -1610,15 +1606,17 @@ void batch_run(Batch* b, int mode, const double* in, double* o1, double* o2, hip
     const hipError_t e2 = hipStreamEndCapture(b->cap_stream, &g);
     NCAMD_HIP(e1);
     NCAMD_HIP(e2);
-    const hipError_t e3 = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    const hipError_t e3 = hipGraphInstantiate(&gs.exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     NCAMD_HIP(e3);
-    b->g_mode = mode; b->g_in = in; b->g_o1 = o1; b->g_o2 = o2;
-    b->g_tile = b->tile; b->g_nblocks = b->nblocks; b->g_ordering = b->ordering;
+    gs.in = in; gs.o1 = o1; gs.o2 = o2;
+    gs.tile = b->tile; gs.nblocks = b->nblocks; gs.ordering = b->ordering;
+    ++gs.captures;
+  } else {
+    ++gs.replays;
   }
-  NCAMD_HIP(hipGraphLaunch(b->gexec, st));
+  NCAMD_HIP(hipGraphLaunch(gs.exec, st));
 }
-
 
 // ------------------------------------------- S(alpha,beta) envelope sampler ---
 // Tables of the box-envelope sampler (device side in nc_kb.hh): interval
@@ -2360,14 +2358,10 @@ ncamd_process_t ncamd_create_freegas(double temperature, double mass_amu, double
   std::unique_ptr<Proc> p(new Proc());
   p->kind = kKindFG;
   const double kT = kBoltzmann * temperature;
-  p->fg.T = temperature;
-  p->fg.mass = mass_amu;
+  p->fg.kT = kT;
+  p->fg.A = (1.0 / kNeutronMassAmu) * mass_amu;
   p->fg.sigma_free = sigma_free;
   p->fg.ca = (mass_amu / kNeutronMassAmu) / kT;  // NCFreeGasUtils.cc:45-53: a^2 = ca E
-  std::vector<double> et = erfc_table();
-  DevBuf* bet = p->newbuf();
-  bet->upload(et.data(), et.size());
-  p->fg.erfc_tab = bet->as<double>();
   init_errbuf(p.get());
   return wrap(p.release());
   NCAMD_CATCH
@@ -2762,6 +2756,8 @@ ncamd_batch_t ncamd_batch_create(int nmat, const ncamd_process_t* procs, const i
     Proc* p = extract(procs[i]);
     if (counts[i] < 0) raise("BadInput", "negative neutron count");
     if (p->oriented_comp) raise("BadInput", "oriented compositions are not supported in batches");
+    if (p->kind == kKindFG || p->kind == kKindAbs)
+      raise("BadInput", "FreeGas and absorption processes are not supported in batches");
     if (p->kind == kKindNone)
       for (Proc* c : p->comps)
         if (c->kind == kKindFG || c->kind == kKindAbs)
@@ -2804,6 +2800,14 @@ void ncamd_batch_get_tuning(ncamd_batch_t h, int* tile, int* nblocks, int* order
   NCAMD_TRY
   Batch* b = extract_batch(h);
   *tile = b->tile; *nblocks = b->nblocks; *ordering = b->ordering;
+  NCAMD_CATCH
+}
+void ncamd_batch_graph_stats(ncamd_batch_t h, int mode, int64_t* captures, int64_t* replays) {
+  NCAMD_TRY
+  Batch* b = extract_batch(h);
+  if (mode != 0 && mode != 1) raise("BadInput", "mode must be 0 (cross sections) or 1 (sampling)");
+  *captures = b->graphs[mode].captures;
+  *replays = b->graphs[mode].replays;
   NCAMD_CATCH
 }
 void ncamd_batch_set_tuning(ncamd_batch_t h, int tile, int nblocks, int ordering) {

@@ -13,7 +13,9 @@
 //    ncrystal_core/src/utils/NCPointwiseDist.cc:76-107
 //  - kinematics: ncrystal_core/include/NCrystal/internal/phys_utils/NCKinUtils.hh:64-121,
 //    ncrystal_core/src/phys_utils/NCKinUtils.cc:25-58
-//  - free gas: ncrystal_core/src/phys_utils/NCFreeGasUtils.cc:64-930
+//  - free gas: cross section and scattering function of
+//    ncrystal_core/src/phys_utils/NCFreeGasUtils.cc:64-84 (sampled as a
+//    two-body collision, nc_freegas.hh)
 //  - RNG to (0,1]: ncrystal_core/include/NCrystal/core/NCDefs.hh:1308-1329
 #pragma once
 #include <hip/hip_runtime.h>
@@ -307,17 +309,16 @@ NCD double sample_loglin_fast(double a, double fa, double b, double fb, double r
   return fa ? b - x : a + x;
 }
 
-// free-gas xs shape (NCFreeGasUtils.cc:64-84)
+// Free-gas cross-section shape sigma/sigma_free as a function of
+// a^2 = A E / kT (NCFreeGasUtils.cc:64-84 evaluates the same function):
+// (1 + 1/(2 a^2)) erf(a) + exp(-a^2) / (a sqrt(pi)).  Both terms are
+// positive, so the closed form keeps full relative precision from a -> 0
+// (where it tends to 2/(a sqrt(pi))) to a -> inf (1 + 1/(2 a^2)).
 NCHD double freegas_xs_shape(double a2) {
-  if (a2 > 36.0) return 1.0 + 0.5 / a2;
   const double a = sqrt(a2);
-  if (a < 0.1) {
-    if (a == 0.0) return __builtin_huge_val();
-    constexpr double c1 = 2.0 / 3.0, c2 = 1.0 / 15.0, c3 = 1.0 / 105.0, c4 = 1.0 / 756.0, c5 = 1.0 / 5940.0;
-    return kInvSqrtPi * (2.0 / a + a * (c1 - a2 * (c2 - a2 * (c3 - a2 * (c4 - a2 * c5)))));
-  }
+  if (a == 0.0) return __builtin_huge_val();
   const double inva = 1.0 / a;
-  return (1.0 + 0.5 * inva * inva) * erf(a) + kInvSqrtPi * exp(-a2) * inva;
+  return (1.0 + 0.5 / a2) * erf(a) + kInvSqrtPi * exp(-a2) * inva;
 }
 
 // ---------------------------------------------------------------------------
@@ -345,8 +346,7 @@ struct SABView {
   const double* scdf;     // [ne*(nb+1)]
   const double2* sxy;     // [ne*(nb+1)] (x, normalised y)
   const AInfo* sinfo;     // [ne*nb]
-  const double* erfc_tab; // [1103] erfc lookup for the free-gas sampler
-  double kT, T, mass, sigma_free, fg_ca, k_ext, k1, k2;
+  double kT, T, mass, fg_A, sigma_free, fg_ca, k_ext, k1, k2;  // fg_A: mass / neutron mass
   double egrid_margin;
   // Search acceleration.  Every table stores, per bucket, the search result
   // at the bucket's lower edge, so the bracket [table[b], table[b+1]] always
@@ -536,15 +536,18 @@ NCD int sab_sample_highe(const SABView& s, double E, Rng& rng, double& a_out, do
     if (rng.gen() < P_extra) { a_out = -1.0; b_out = 0.0; return kOK; }
   }
   const double emax_div_kt = emax / s.kT;
-  while (true) {
-    double al, be;
-    freegas_sample_alpha_beta(E, s.T, s.mass, s.erfc_tab, rng, al, be);
+  for (int it = 0; it < 10000; ++it) {  // a discard happens with probability < 0.95
+    FGDraw d;
+    const int rc = fg_collide(E / s.kT, s.fg_A, rng, d);
+    if (rc != kOK) return rc;
+    const double al = d.alpha, be = d.beta;
     if (be <= -emax_div_kt) { a_out = al; b_out = be; return kOK; }
     const Lims l = alpha_limits(emax_div_kt, be);
     if (!in_interval(l.lo, l.hi, al)) { a_out = al; b_out = be; return kOK; }
     if (P_discard && rng.gen() < P_discard) continue;
     a_out = -1.0; b_out = 0.0; return kOK;
   }
+  return kErrSamplerLoop;
 }
 
 // SABSampler::sampleAlphaBeta (NCSABSampler.cc:158-227)

@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(256) k_comp_pick(CompView c, const double* __r
 }
 
 template <bool HighE, bool Dirs>
-__global__ void __launch_bounds__(256, 2) k_comp_sab(SABView s, const double* __restrict__ ekin,
+__global__ void __launch_bounds__(256) k_comp_sab(SABView s, const double* __restrict__ ekin,
                                                   const double* __restrict__ dirs, int64_t n, int64_t total,
                                                   uint64_t seed, uint32_t stream, int skip, double* __restrict__ oE,
                                                   double* __restrict__ o2, unsigned long long* __restrict__ nerr,
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(256) k_sab_sample_iso_main(SABView s, const do
   }
 }
 
-__global__ void __launch_bounds__(256, 2) k_sab_sample_iso_highe(SABView s, const double* __restrict__ ekin,
+__global__ void __launch_bounds__(256) k_sab_sample_iso_highe(SABView s, const double* __restrict__ ekin,
                                                               int64_t n, int64_t total, uint64_t seed,
                                                               uint32_t stream, double* __restrict__ oE,
                                                               double* __restrict__ oMu,

@@ -1060,7 +1060,7 @@ static double elinc_mu_mono(rng_t* rng, double E, double msd) {
 }
 static double elinc_sample_mu(const elinc_t* ei, rng_t* rng, double E) {
   if (ei->n == 1) return elinc_mu_mono(rng, E, ei->msd[0]);
-  double cm[64];
+  double cm[64] = {0};
   double* commul = ei->n <= 64 ? cm : (double*)malloc(sizeof(double) * (size_t)ei->n);
   elinc_xs_commul(ei, E, commul);
   size_t i = pick_idx(rng, commul, (size_t)ei->n);
@@ -1237,6 +1237,15 @@ static tb_t tailed_breakdown(const double* ag, int na, const double* sab, const 
   return tb;
 }
 
+/* createTailedBreakdown on one caller slice: out = (xs_front, xs_middle,
+   xs_back); the middle-bin indices go to imid[0..1] */
+void orc_tailed_breakdown(const double* ag, int na, const double* sab, const double* lsab, const double* cumul,
+                          double alow, double aupp, unsigned ilo, unsigned iup, double* out, unsigned* imid) {
+  tb_t tb = tailed_breakdown(ag, na, sab, lsab, cumul, alow, aupp, ilo, iup);
+  out[0] = tb.xs_front; out[1] = tb.xs_middle; out[2] = tb.xs_back;
+  imid[0] = tb.imiddle_low; imid[1] = tb.imiddle_upp;
+}
+
 /* PointwiseDist ctor + percentileWithIndex:
    ncrystal_core/src/utils/NCPointwiseDist.cc:32-107 */
 static int pwdist_init(int n, double* x, double* y, double* cdf) {
@@ -1280,6 +1289,36 @@ static double pwdist_percentile(int n, const double* x, const double* y, const d
   *idx_out = (unsigned)(i - 1);
   return dclamp(x[i - 1] + zdx, x[i - 1], x[i]);
 }
+
+/* PointwiseDist (NCPointwiseDist.cc:32-107) on caller arrays: percentiles
+   of the piecewise-linear density through (x[i], y[i]); idx receives the
+   index of the lower edge of the bin. Returns 0, or -1 for invalid input. */
+int orc_pwdist_percentiles(int n, const double* x, const double* y, int m, const double* p, double* out,
+                           unsigned* idx) {
+  if (n < 2) return -1;
+  double* xx = (double*)malloc(sizeof(double) * (size_t)n);
+  double* yy = (double*)malloc(sizeof(double) * (size_t)n);
+  double* cdf = (double*)malloc(sizeof(double) * (size_t)n);
+  memcpy(xx, x, sizeof(double) * (size_t)n);
+  memcpy(yy, y, sizeof(double) * (size_t)n);
+  int rc = pwdist_init(n, xx, yy, cdf);
+  if (rc == 0)
+    for (int k = 0; k < m; ++k) out[k] = pwdist_percentile(n, xx, yy, cdf, p[k], &idx[k]);
+  free(xx); free(yy); free(cdf);
+  return rc ? -1 : 0;
+}
+
+/* Planted procedure changes for the tests that show the Alg1 pins have the
+   power to see them (test infrastructure; 0 = the reference's procedure):
+   1 = no first-bin rule (no 4/3 widening of the first beta bin, and beta in
+       that bin taken from the piecewise-linear percentile with alpha from the
+       first row),
+   2 = egrid margin 1.0 instead of 1.05 when picking the sampler,
+   3 = alpha from the nearer of the two beta rows instead of interpolating
+       the two rows' quantiles.
+   Process-wide; set before creating the table. */
+static int g_alg1_variant = 0;
+void orc_set_alg1_variant(int v) { g_alg1_variant = v; }
 
 /* analyseEnergyPoint: ncrystal_core/src/sab/NCSABIntegrator.cc:352-559 */
 static double sab_analyse(const sab_t* s, double ekin, int doSampler, sampE_t* out) {
@@ -1352,6 +1391,7 @@ static double sab_analyse(const sab_t* s, double ekin, int doSampler, sampE_t* o
         inf->back.alpha = aupp;
       }
     }
+    if (next_kin && g_alg1_variant == 1) next_kin = 0;
     if (next_kin) {
       next_kin = 0;
       double db_real = (beta - prev_b);
@@ -1569,6 +1609,15 @@ static double alg1_sample_alpha(const sab_t* s, const sampE_t* S, size_t ibeta, 
   }
 }
 
+/* alpha of beta row ibeta of the sampler at egrid point ie at percentile
+   u (SABSamplerAtE_Alg1::sampleAlpha); NaN for rows outside the sampler */
+void orc_sab_sample_alpha(const orc_proc* p, int ie, int ibeta, int m, const double* u, double* out) {
+  const sab_t* s = &p->u.sab;
+  const sampE_t* S = &s->samplers[ie];
+  const int ok = S->kind == 1 && ibeta >= S->ibeta_low && ibeta - S->ibeta_low < S->ninfos;
+  for (int k = 0; k < m; ++k) out[k] = ok ? alg1_sample_alpha(s, S, (size_t)ibeta, u[k]) : NAN;
+}
+
 /* SABSamplerAtE_Alg1::sampleAlphaBeta: NCSABSamplerModels.cc:48-155.
    Returns 0 on success, -1 when the 100-try loop is exhausted.
    Deviation (documented): when ibetaSampled==0 without a kinematic endpoint and
@@ -1581,6 +1630,14 @@ static int alg1_sample(const sab_t* s, const sampE_t* S, double ediv, rng_t* rng
   while (--it) {
     unsigned ibs;
     double beta = pwdist_percentile(S->nvals, S->bx, S->by, S->bcdf, rng_gen(rng), &ibs);
+    if (ibs == 0 && S->firstbin <= 0.0 && g_alg1_variant == 1) {
+      if (beta <= dmax(-ediv, bg[0])) continue;
+      double alphaval = alg1_sample_alpha(s, S, (size_t)S->ibeta_low, rng_gen(rng));
+      double l1, l2;
+      alpha_limits(ediv, beta, &l1, &l2);
+      if (in_interval(l1, l2, alphaval)) { *a_out = alphaval; *b_out = beta; return 0; }
+      continue;
+    }
     if (ibs == 0 && S->firstbin <= 0.0) {
       const double b0 = S->firstbin;
       const double b1 = S->bx[1];
@@ -1611,6 +1668,7 @@ static int alg1_sample(const sab_t* s, const sampE_t* S, double ediv, rng_t* rng
     double bh = bg[ibeta];
     double ah = alg1_sample_alpha(s, S, ibeta, rp);
     double alpha = al + (ah - al) * (beta - bl) / (bh - bl);
+    if (g_alg1_variant == 3) alpha = (beta - bl) < (bh - beta) ? al : ah;
     double l1, l2;
     alpha_limits(ediv, beta, &l1, &l2);
     if (in_interval(l1, l2, alpha)) { *a_out = alpha; *b_out = beta; return 0; }
@@ -1637,7 +1695,8 @@ static int sab_sample_ab(const sab_t* s, double E, rng_t* rng, double* a_out, do
     isamp = 0;
     ultra = (E < ultra_e);
   } else {
-    while (up + 1 != (size_t)s->ne && E * 1.05 > s->egrid[up]) ++up;
+    const double margin = g_alg1_variant == 2 ? 1.0 : 1.05;
+    while (up + 1 != (size_t)s->ne && E * margin > s->egrid[up]) ++up;
     isamp = up;
   }
   const sampE_t* S = &s->samplers[isamp];
@@ -1667,6 +1726,84 @@ static int sab_sample_ab(const sab_t* s, double E, rng_t* rng, double* a_out, do
    randExpInterval).  Implemented in ncoracle_freegas.c. */
 static int orc_fg_sample_alpha_beta(double E, double T, double mass, rng_t* rng, double* alpha, double* beta);
 
+/* Free-gas draws as a two-body collision: the procedure of the device
+   (ncrystal_amd/csrc/nc_freegas.hh), restated operation for operation so the
+   checker's streams stay in step with the device's.  The target velocity is
+   drawn from the Maxwellian weighted by the relative speed (envelope
+   (x + y) x^2 exp(-x^2) in reduced speeds x = sqrt(A)|V|, y = sqrt(A)|v|,
+   split into x^3 e^{-x^2} and y x^2 e^{-x^2}; kept with probability
+   |v - V|/(|v| + |V|)), then an isotropic direction of the relative velocity
+   in the centre-of-mass frame; alpha = |v' - v|^2 and beta = |v'|^2 - |v|^2
+   in units of kT.  Its distribution is the free-gas scattering function of
+   NCFreeGasUtils.hh:36-61: tests pin it by closed forms and against the
+   restatement of the reference's own sampler (orc_fg_sample_alpha_beta),
+   which orc_set_freegas_procedure(1) selects instead. */
+static int g_fg_procedure = 0;
+void orc_set_freegas_procedure(int which) { g_fg_procedure = which; }
+typedef struct { double alpha, beta, mu; } fgdraw_t;
+static int fg_collide(double c, double A, rng_t* rng, fgdraw_t* o) {
+  const double u = sqrt(c);
+  const double sA = sqrt(A);
+  const double y = sA * u;
+  const double p_x3 = 2.0 / (2.0 + 1.772453850905516027298167483341145182797549456 * y);
+  double x, s, rel;
+  int tries = 0;
+  while (1) {
+    if (++tries > 1000) return -2;
+    double x2;
+    if (rng_gen(rng) < p_x3) {
+      const double r1 = rng_gen(rng);
+      const double r2 = rng_gen(rng);
+      x2 = -log(r1 * r2);
+    } else {
+      const double r1 = rng_gen(rng);
+      const double r2 = rng_gen(rng);
+      const double cq = cos(K_PIHALF * rng_gen(rng));
+      x2 = -log(r1) - log(r2) * (cq * cq);
+    }
+    x = sqrt(x2);
+    s = rng_gen(rng);
+    const double d = x - y;
+    rel = sqrt(d * d + 4.0 * x * y * s);
+    if (rng_gen(rng) * (x + y) < rel) break;
+  }
+  const double iA = 1.0 / sA;
+  const double w = x * iA;
+  const double rn = rel * iA;
+  const double wx = w * (2.0 * sqrt(s * (1.0 - s)));
+  const double rx = -wx;
+  const double rz = (u - w) + 2.0 * w * s;
+  const double q = A / (1.0 + A);
+  const double g = q * rn;
+  const double irn = 1.0 / rn;
+  const double c_r = u * rz * irn - g;
+  const double c_e = u * wx * irn;
+  const double t = rng_gen(rng);
+  const double sinchi = 2.0 * sqrt(t * (1.0 - t));
+  const double cphi = cos(K_2PI * rng_gen(rng));
+  o->alpha = 4.0 * g * g * t;
+  o->beta = 2.0 * g * (sinchi * cphi * c_e - 2.0 * t * c_r);
+  const double coschi = 1.0 - 2.0 * t;
+  const double cx = q * wx, cz = u - q * rz;
+  const double ox = coschi * rx + sinchi * cphi * rz;
+  const double oz = coschi * rz - sinchi * cphi * rx;
+  const double vx = cx + g * ox * irn;
+  const double vz = cz + g * oz * irn;
+  const double gs = g * sinchi;
+  const double vy2 = gs * gs * (1.0 - cphi * cphi);
+  const double nrm = sqrt(vx * vx + vy2 + vz * vz);
+  o->mu = nrm > 0.0 ? dclamp(vz / nrm, -1.0, 1.0) : 1.0;
+  return 0;
+}
+static int fg_alpha_beta(double E, double kT, double T, double mass, rng_t* rng, double* al, double* be) {
+  if (g_fg_procedure == 1) return orc_fg_sample_alpha_beta(E, T, mass, rng, al, be);
+  fgdraw_t d;
+  int rc = fg_collide(E / kT, (1.0 / K_NEUTRON_MASS_AMU) * mass, rng, &d);
+  if (rc) return rc;
+  *al = d.alpha; *be = d.beta;
+  return 0;
+}
+
 /* SABSampler::sampleHighE: ncrystal_core/src/sab/NCSABSampler.cc:59-156.
    Sets *a = -1 when the table must be sampled at Emax. */
 static int sab_sample_highe(const sab_t* s, double E, rng_t* rng, double* a, double* b) {
@@ -1682,9 +1819,9 @@ static int sab_sample_highe(const sab_t* s, double E, rng_t* rng, double* a, dou
     if (rng_gen(rng) < P_extra) { *a = -1.0; *b = 0.0; return 0; }
   }
   const double emax_div_kt = emax / s->kT;
-  while (1) {
+  for (int it = 0; it < 10000; ++it) { /* a discard happens with probability < 0.95 */
     double al, be;
-    int rc = orc_fg_sample_alpha_beta(E, s->T, s->mass, rng, &al, &be);
+    int rc = fg_alpha_beta(E, s->kT, s->T, s->mass, rng, &al, &be);
     if (rc) return rc;
     if (be <= -emax_div_kt) { *a = al; *b = be; return 0; }
     double l1, l2;
@@ -1693,6 +1830,7 @@ static int sab_sample_highe(const sab_t* s, double E, rng_t* rng, double* a, dou
     if (P_discard && rng_gen(rng) < P_discard) continue;
     *a = -1.0; *b = 0.0; return 0;
   }
+  return -2;
 }
 
 /* SABSampler::sampleDeltaEMu NCSABSampler.cc:229-236 + SABScatter
@@ -2494,6 +2632,14 @@ static int fg_sample_demu(const fg_t* g, rng_t* rng, double E, double* Ef, doubl
   /* energies that are not finite and >= 0 are refused (the rejection loops
      need a finite E/kT) */
   if (!(E >= 0.0 && E < INFINITY)) return -5;
+  if (g_fg_procedure != 1) {
+    fgdraw_t d;
+    int rc = fg_collide(E / g->kT, (1.0 / K_NEUTRON_MASS_AMU) * g->mass, rng, &d);
+    if (rc) return rc;
+    *Ef = dmax(0.0, E + d.beta * g->kT);
+    *mu = d.mu;
+    return 0;
+  }
   pthread_once(&eb_once, eb_init);
   fgs_t f; fgs_init(&f, E, g->T, g->mass);
   const double b = fgs_sample_beta(&f, rng);

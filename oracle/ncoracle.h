@@ -82,6 +82,12 @@ typedef struct orc_proc orc_proc;
    xs and isotropic sampling through the generic orc_xs_many /
    orc_sample_iso_many.  NULL on invalid input. */
 orc_proc* orc_create_freegas(double temperature, double mass_amu, double sigma_free);
+/* Procedure of every free-gas draw (FreeGas processes and the S(alpha,beta)
+   high-energy extender): 0 (default) = two-body collision with a Maxwellian
+   target, the device's procedure; 1 = restatement of the reference's
+   FreeGasSampler (NCFreeGasUtils.cc:492-930).  Both sample the same
+   distribution; process-wide, set it before sampling. */
+void orc_set_freegas_procedure(int which);
 /* AbsOOV 1/v absorption (ncrystal_core/src/absoov/NCAbsOOV.cc:30-44):
    xs = sqrt(E_2200)*sigma_abs/sqrt(E) (inf at E = 0); xs only. */
 orc_proc* orc_create_absoov(double sigma_abs);
@@ -128,6 +134,22 @@ int64_t orc_sample_iso_many(const orc_proc*, uint64_t seed, uint32_t stream,
                             int64_t repeat, double* out_ekin, double* out_mu,
                             int nthreads);
 /* sample (alpha,beta) from the SAB sampler only (no conversion) */
+/* Planted changes of the Alg1 procedure, for tests that show the sampler
+   pins detect them: 0 = none (the reference's procedure), 1 = no first-bin
+   rule, 2 = egrid margin 1.0, 3 = nearer beta row instead of quantile
+   interpolation.  Process-wide; set before creating the table. */
+void orc_set_alg1_variant(int variant);
+/* createTailedBreakdown (NCSABUtils.cc:538-633) on one alpha slice with
+   cumulative integrals `cumul`: out = (xs_front, xs_middle, xs_back) */
+void orc_tailed_breakdown(const double* ag, int na, const double* sab, const double* lsab, const double* cumul,
+                          double alow, double aupp, unsigned ilo, unsigned iup, double* out, unsigned* imid);
+/* SABSamplerAtE_Alg1::sampleAlpha (NCSABSamplerModels.cc:157-234) of beta
+   row ibeta of the sampler at egrid point ie, at the m percentiles u */
+void orc_sab_sample_alpha(const orc_proc* p, int ie, int ibeta, int m, const double* u, double* out);
+/* PointwiseDist percentiles (NCPointwiseDist.cc:32-107) of the piecewise
+   linear density through (x, y) at the m values p; 0 or -1 (bad input). */
+int orc_pwdist_percentiles(int n, const double* x, const double* y, int m, const double* p, double* out,
+                           unsigned* idx);
 int64_t orc_sab_sample_ab_many(const orc_proc*, uint64_t seed, uint32_t stream,
                                const double* ekin, int64_t n, double* out_alpha,
                                double* out_beta, int nthreads);

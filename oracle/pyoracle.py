@@ -71,6 +71,8 @@ def lib():
         L.orc_reci_lattice.argtypes = [ctypes.c_double] * 6 + [_dp]
         L.orc_create_freegas.argtypes = [ctypes.c_double] * 3
         L.orc_create_freegas.restype = _vp
+        L.orc_set_freegas_procedure.argtypes = [ctypes.c_int]
+        L.orc_set_freegas_procedure.restype = None
         L.orc_create_absoov.argtypes = [ctypes.c_double]
         L.orc_create_absoov.restype = _vp
         L.orc_reci_lattice.restype = ctypes.c_int
@@ -112,6 +114,16 @@ def lib():
         L.orc_sab_sample_ab_many.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint32, _dp, ctypes.c_int64,
                                              _dp, _dp, ctypes.c_int]
         L.orc_sab_sample_ab_many.restype = ctypes.c_int64
+        L.orc_set_alg1_variant.argtypes = [ctypes.c_int]
+        L.orc_set_alg1_variant.restype = None
+        L.orc_pwdist_percentiles.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp,
+                                             ctypes.POINTER(ctypes.c_uint)]
+        L.orc_pwdist_percentiles.restype = ctypes.c_int
+        L.orc_tailed_breakdown.argtypes = [_dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_uint, ctypes.c_uint, _dp, ctypes.POINTER(ctypes.c_uint)]
+        L.orc_tailed_breakdown.restype = None
+        L.orc_sab_sample_alpha.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_sab_sample_alpha.restype = None
         L.orc_sample_dir_many.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _dp, _dp,
                                           ctypes.c_int64, _dp, _dp, ctypes.c_int]
         L.orc_sample_dir_many.restype = ctypes.c_int64
@@ -244,6 +256,64 @@ def freegas(temperature, mass_amu, sigma_free):
     return Proc(lib().orc_create_freegas(temperature, mass_amu, sigma_free))
 
 
+def pwdist_percentiles(x, y, p):
+    """PointwiseDist percentiles and bin indices of the piecewise-linear
+    density through (x, y)"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    out = np.empty(p.size)
+    idx = np.empty(p.size, dtype=np.uint32)
+    rc = lib().orc_pwdist_percentiles(x.size, _d(x), _d(y), p.size, _d(p), _d(out),
+                                      idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)))
+    if rc:
+        raise ValueError("invalid pointwise distribution")
+    return out, idx
+
+
+def tailed_breakdown(alpha, sab, logsab, cumul, alow, aupp, ilo, iup):
+    """createTailedBreakdown on one slice: (xs_front, xs_middle, xs_back)"""
+    a, s, ls, c = (np.ascontiguousarray(v, dtype=np.float64) for v in (alpha, sab, logsab, cumul))
+    out = np.empty(3)
+    imid = (ctypes.c_uint * 2)()
+    lib().orc_tailed_breakdown(_d(a), a.size, _d(s), _d(ls), _d(c), alow, aupp, ilo, iup, _d(out), imid)
+    return out
+
+
+class alg1_variant:
+    """Context manager planting a change in the checker's Alg1 procedure
+    (1 no first-bin rule, 2 egrid margin 1.0, 3 nearer beta row); tables must
+    be created inside the context."""
+
+    def __init__(self, variant):
+        self.variant = int(variant)
+
+    def __enter__(self):
+        lib().orc_set_alg1_variant(self.variant)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_alg1_variant(0)
+        return False
+
+
+class freegas_procedure:
+    """Context manager selecting the checker's free-gas procedure:
+    "collision" (default; the device's two-body collision draw) or
+    "reference" (restatement of NCFreeGasUtils.cc:492-930)."""
+
+    def __init__(self, which):
+        self.which = {"collision": 0, "reference": 1}[which]
+
+    def __enter__(self):
+        lib().orc_set_freegas_procedure(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_freegas_procedure(0)
+        return False
+
+
 def absoov(sigma_abs):
     return Proc(lib().orc_create_absoov(sigma_abs))
 
@@ -327,6 +397,13 @@ class SAB(Proc):
         cu = np.empty(na * nb)
         lib().orc_sab_derived(self.ptr, _d(ls), _d(cu))
         return ls.reshape(nb, na), cu.reshape(nb, na)
+
+    def sample_alpha(self, ie, ibeta, u):
+        """the Alg1 alpha sampler of beta row ibeta at egrid point ie"""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.empty(u.size)
+        lib().orc_sab_sample_alpha(self.ptr, ie, ibeta, u.size, _d(u), _d(out))
+        return out
 
     def sample_ab(self, ekin, seed=1, stream=0, nthreads=1):
         e = np.ascontiguousarray(np.atleast_1d(np.asarray(ekin, dtype=np.float64)))

@@ -9,6 +9,10 @@ Sources:
   tests/src/app_sabutils/main.cc:88-135   mpmath known answers of the alpha-bin integral
   tests/scripts/ncmat2hkl.log + data/*.ncmat  hkl tables of crystals with a space
                                          group (10 significant digits) with their inputs
+  tests/src/app_pointwisedist/main.cc,test.log  percentiles of a uniform density
+                                         given as a piecewise-linear one
+  tests/src/app_sabsliceinteg/main.cc,test.log  tailed breakdowns of S = exp(alpha)
+                                         over alpha ranges (front/middle/back integrals)
 """
 import json
 import pathlib
@@ -163,7 +167,48 @@ def spacegroup_hkl():
     return {k: len(v["table"]) + v["snipped"] + len(v["tail"]) for k, v in mats.items()}
 
 
+def pointwisedist():
+    # main.cc: x = {0,1,2,3,4}, weights all 4, 117 percentiles r = linspace(0,1,117)
+    rows = []
+    for l in (REF / "tests/src/app_pointwisedist/test.log").read_text().splitlines():
+        m = re.match(r"percentile\( ([0-9.eE+-]+) \) = ([0-9.eE+-]+)$", l)
+        if m:
+            rows.append([float(m.group(1)), float(m.group(2))])
+    (OUT / "pointwisedist_uniform.json").write_text(json.dumps(dict(
+        source="tests/src/app_pointwisedist/main.cc (inputs) and test.log (printed %g values)",
+        x=[0, 1, 2, 3, 4], w=[4, 4, 4, 4, 4], npoints=117, printed=rows), indent=1))
+    return len(rows)
+
+
+def sabsliceinteg():
+    # main.cc: alpha grid 0.1..2.0 step 0.1, S = exp(alpha); checkXS(a0, a1) cases in order
+    ranges = [(0.1, 2.0), (0.2, 1.8), (0.1123, 1.8), (0.112, 1.8123), (0.123, 1.8123), (0.15, 1.85),
+              (0.1234, 0.1567), (0.20000001, 0.2567), (0.2, 0.2567), (0.2, 0.5567), (0.2, 1.4),
+              (100.0, 101.0), (1e-10, 1e-9), (0.05, 1.123), (0.05, 2.3), (1.05, 2.3)]
+    cases = []
+    lines = (REF / "tests/src/app_sabsliceinteg/test.log").read_text().splitlines()
+    idx = None
+    for l in lines:
+        m = re.match(r"makeTailedBreakdown DBG: aidx_low=(\d+), aidx_upp=(\d+)", l)
+        if m:
+            idx = [int(m.group(1)), int(m.group(2))]
+            continue
+        m = re.match(r" checkXS\((\d+)\): xs_front = (\S+) xs_middle = (\S+) xs_back = (\S+) xs_sum = (\S+)"
+                     r" expected = (\S+) reldiff<1e-15\? = (\w+)", l)
+        if m:
+            k = int(m.group(1))
+            cases.append(dict(alow=ranges[k - 1][0], aupp=ranges[k - 1][1], aidx=idx,
+                              front=float(m.group(2)), middle=float(m.group(3)), back=float(m.group(4)),
+                              total=float(m.group(5)), expected=float(m.group(6)), ok=m.group(7) == "yes"))
+    (OUT / "sabsliceinteg.json").write_text(json.dumps(dict(
+        source="tests/src/app_sabsliceinteg/main.cc:128-177 (inputs) and test.log (printed values, 6 digits)",
+        alpha=[round(0.1 * i, 1) for i in range(1, 21)], cases=cases), indent=1))
+    return len(cases)
+
+
 if __name__ == "__main__":
+    print("pointwisedist percentiles:", pointwisedist())
+    print("sabsliceinteg cases:", sabsliceinteg())
     print("space-group hkl tables:", spacegroup_hkl())
     print("nosg families:", nosg())
     print("debye msd values:", debyemsd())

@@ -13,6 +13,11 @@ S(alpha,beta) high-energy extender shares) pinned by closed forms:
   quadrature over beta) and the conditional CDF of alpha given beta, both
   compared with the samples by Kolmogorov-Smirnov tests;
 * 1/v absorption: sigma_abs sqrt(E_2200 / E).
+
+The sampling checks run on both free-gas procedures of the checker: the
+two-body collision draw that the device performs (the default) and the
+restatement of the reference's FreeGasSampler; a two-sample test compares
+the two directly.
 """
 import math
 
@@ -50,15 +55,24 @@ def _alpha_lims(c, beta):
     return (math.sqrt(c) - r) ** 2, (math.sqrt(c) + r) ** 2
 
 
-@pytest.mark.parametrize("T,M,E", [(293.15, 1.00794, 0.025), (293.15, 4.0026, 0.001), (293.15, 4.0026, 0.5),
-                                   (100.0, 39.95, 0.05), (600.0, 12.011, 2.0)])
-def test_freegas_sampling_distribution(oracle, T, M, E):
+def _sample(oracle, procedure, T, M, E, n, seed=7):
+    g = oracle.freegas(T, M, 1.0)
+    with oracle.freegas_procedure(procedure):
+        return g.sample_iso(np.full(n, E), seed=seed, nthreads=8)
+
+
+CASES = [(293.15, 1.00794, 0.025), (293.15, 4.0026, 0.001), (293.15, 4.0026, 0.5),
+         (100.0, 39.95, 0.05), (600.0, 12.011, 2.0), (293.15, 26.98, 1e-6)]
+
+
+@pytest.mark.parametrize("procedure", ["collision", "reference"])
+@pytest.mark.parametrize("T,M,E", CASES)
+def test_freegas_sampling_distribution(oracle, procedure, T, M, E):
     n = 200000
     kT = KB * T
     A = M / MN
     c = E / kT
-    g = oracle.freegas(T, M, 1.0)
-    ef, mu, nerr = g.sample_iso(np.full(n, E), seed=7, nthreads=8)
+    ef, mu, nerr = _sample(oracle, procedure, T, M, E, n)
     assert nerr == 0 and np.all(np.abs(mu) <= 1.0) and np.all(ef >= 0.0)
     beta = (ef - E) / kT
     alpha = (E + ef - 2 * mu * np.sqrt(E * ef)) / kT
@@ -83,13 +97,13 @@ def test_freegas_sampling_distribution(oracle, T, M, E):
     assert d_beta < lim and d_alpha < lim, (d_beta, d_alpha, lim)
 
 
-def test_freegas_sampler_fails_a_wrong_mass(oracle):
+@pytest.mark.parametrize("procedure", ["collision", "reference"])
+def test_freegas_sampler_fails_a_wrong_mass(oracle, procedure):
     """the check above has the power to see a 10% change of the target mass"""
     n = 200000
     T, M, E = 293.15, 4.0026, 0.025
     kT, c = KB * T, 0.025 / (KB * 293.15)
-    g = oracle.freegas(T, M * 1.1, 1.0)
-    ef, mu, _ = g.sample_iso(np.full(n, E), seed=7, nthreads=8)
+    ef, mu, _ = _sample(oracle, procedure, T, M * 1.1, E, n)
     beta = (ef - E) / kT
     alpha = (E + ef - 2 * mu * np.sqrt(E * ef)) / kT
     A = M / MN
@@ -97,6 +111,19 @@ def test_freegas_sampler_fails_a_wrong_mass(oracle):
     den = _I(np.sqrt(hi / A), beta) - _I(np.sqrt(lo / A), beta)
     f = (_I(np.sqrt(np.clip(alpha, lo, hi) / A), beta) - _I(np.sqrt(lo / A), beta)) / den
     assert stats.kstest(f, "uniform").statistic > 2 * 1.8 / math.sqrt(n)
+
+
+@pytest.mark.parametrize("T,M,E", CASES + [(293.15, 26.98, 50.0), (20.0, 2.014, 1e-4)])
+def test_freegas_procedures_agree(oracle, T, M, E):
+    """collision draw and the reference's sampler: two-sample KS on E' and mu
+    (different streams, 3e5 draws each)"""
+    n = 300000
+    e1, m1, n1 = _sample(oracle, "collision", T, M, E, n, seed=3)
+    e2, m2, n2 = _sample(oracle, "reference", T, M, E, n, seed=4)
+    assert n1 == 0 and n2 == 0
+    lim = 1.95 * math.sqrt(2.0 / n)  # about 0.1% significance
+    assert stats.ks_2samp(e1, e2).statistic < lim
+    assert stats.ks_2samp(m1, m2).statistic < lim
 
 
 def test_absoov_closed_form(oracle):

@@ -88,15 +88,25 @@ def test_batch_matches_per_material_calls(env):
     batch.dev_sampleScatterIsotropic(de.ptr, oe.ptr, om.ptr, stream=st.ptr)
     st.sync()
     assert np.array_equal(oe.numpy(), ref_e) and np.array_equal(om.numpy(), ref_m)
-    # graph replays with eager work in between give the same result every time
+    # alternating cross-section and sampling calls replay one graph per mode,
+    # with eager work in between, and give the same result every time
+    c0, r0 = batch.graph_stats(0)
+    c1, r1 = batch.graph_stats(1)
     for rep in range(3):
         for p, (sd, cc) in zip(procs, states):
             p.setRNGState(sd, cc - 1)
+        batch.dev_crossSection(de.ptr, xs.ptr, stream=st.ptr)
         batch.dev_sampleScatterIsotropic(de.ptr, oe.ptr, om.ptr, stream=st.ptr)
         st.sync()
+        assert np.array_equal(xs.numpy(), ref_xs), rep
         assert np.array_equal(om.numpy(), ref_m), rep
         C.call("ncamd_dev_fill_loguniform", xs.ptr, tot, 1e-3, 1.0, rep, st.ptr)
         st.sync()
+    c0b, r0b = batch.graph_stats(0)
+    c1b, r1b = batch.graph_stats(1)
+    print("graph captures/replays: xs", (c0, r0), "->", (c0b, r0b), "sampling", (c1, r1), "->", (c1b, r1b))
+    assert c1b == c1 and r1b == r1 + 3  # sampling graph replayed each time
+    assert c0b <= c0 + 1 and r0b >= r0 + 2  # xs graph captured at most once more (new tuning), then replayed
 
 
 def test_batch_rejects_oriented_and_bad_input(env):

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from ncrystal_amd import synth
+from conftest import assert_streams_match
 
 pytestmark = pytest.mark.gpu
 
@@ -85,14 +86,14 @@ def test_oriented_composition_matches_checker(amd, oracle, with_sab, only_scb):
     ge, gd = g.sampleScatter(e, dirs)
     oe, od, nerr = o.sample_dir(e, dirs, seed=21, nthreads=8)
     assert nerr == 0
-    diff = np.mean((np.abs(ge - oe) > 1e-9 * np.abs(oe)) | (np.max(np.abs(gd - od), axis=1) > 1e-9))
-    assert diff < 5e-3, diff
+    assert_streams_match((np.abs(ge - oe) > 1e-9 * np.abs(oe)) | (np.max(np.abs(gd - od), axis=1) > 1e-9),
+                         "oriented composition")
     assert np.abs(np.linalg.norm(gd, axis=1) - 1).max() < 1e-12
     # a second call continues the streams
     ge2, gd2 = g.sampleScatter(e, dirs)
     oe2, od2, _ = o.sample_dir(e, dirs, seed=21, stream=1, nthreads=8)
-    diff2 = np.mean((np.abs(ge2 - oe2) > 1e-9 * np.abs(oe2)) | (np.max(np.abs(gd2 - od2), axis=1) > 1e-9))
-    assert diff2 < 5e-3
+    assert_streams_match((np.abs(ge2 - oe2) > 1e-9 * np.abs(oe2)) | (np.max(np.abs(gd2 - od2), axis=1) > 1e-9),
+                         "oriented composition, second call")
 
 
 def test_oriented_composition_refuses_isotropic_calls(amd, oracle):
