@@ -1833,6 +1833,19 @@ static int sab_sample_highe(const sab_t* s, double E, rng_t* rng, double* a, dou
   return -2;
 }
 
+/* sampleHighE on its own (E >= Emax): oa = -1 marks a draw that the table
+   at Emax must make; single-threaded, stream (seed, stream, i) per neutron.
+   Returns the number of errors. */
+int64_t orc_sab_sample_highe_many(const orc_proc* p, uint64_t seed, uint32_t stream, const double* ekin,
+                                  int64_t n, double* oa, double* ob) {
+  int64_t nerr = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    rng_t r; rng_init(&r, seed, stream, (uint64_t)i);
+    if (sab_sample_highe(&p->u.sab, ekin[i], &r, &oa[i], &ob[i])) { ++nerr; oa[i] = NAN; ob[i] = NAN; }
+  }
+  return nerr;
+}
+
 /* SABSampler::sampleDeltaEMu NCSABSampler.cc:229-236 + SABScatter
    ncrystal_core/src/sabscatter/NCSABScatter.cc:93-100 */
 static int sab_sample_iso(const sab_t* s, rng_t* rng, double E, double* Ef, double* mu) {

@@ -146,6 +146,10 @@ void orc_tailed_breakdown(const double* ag, int na, const double* sab, const dou
 /* SABSamplerAtE_Alg1::sampleAlpha (NCSABSamplerModels.cc:157-234) of beta
    row ibeta of the sampler at egrid point ie, at the m percentiles u */
 void orc_sab_sample_alpha(const orc_proc* p, int ie, int ibeta, int m, const double* u, double* out);
+/* SABSampler::sampleHighE alone (NCSABSampler.cc:59-156) for E >= Emax:
+   oa = -1 where the table at Emax must draw; returns the error count */
+int64_t orc_sab_sample_highe_many(const orc_proc* p, uint64_t seed, uint32_t stream, const double* ekin,
+                                  int64_t n, double* oa, double* ob);
 /* PointwiseDist percentiles (NCPointwiseDist.cc:32-107) of the piecewise
    linear density through (x, y) at the m values p; 0 or -1 (bad input). */
 int orc_pwdist_percentiles(int n, const double* x, const double* y, int m, const double* p, double* out,

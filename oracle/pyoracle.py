@@ -124,6 +124,8 @@ def lib():
         L.orc_tailed_breakdown.restype = None
         L.orc_sab_sample_alpha.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_sab_sample_alpha.restype = None
+        L.orc_sab_sample_highe_many.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint32, _dp, ctypes.c_int64, _dp, _dp]
+        L.orc_sab_sample_highe_many.restype = ctypes.c_int64
         L.orc_sample_dir_many.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _dp, _dp,
                                           ctypes.c_int64, _dp, _dp, ctypes.c_int]
         L.orc_sample_dir_many.restype = ctypes.c_int64
@@ -404,6 +406,15 @@ class SAB(Proc):
         out = np.empty(u.size)
         lib().orc_sab_sample_alpha(self.ptr, ie, ibeta, u.size, _d(u), _d(out))
         return out
+
+    def sample_highe(self, ekin, seed=1, stream=0):
+        """sampleHighE alone: (alpha, beta, nerr); alpha = -1 marks draws
+        left to the table at Emax"""
+        e = np.ascontiguousarray(np.atleast_1d(np.asarray(ekin, dtype=np.float64)))
+        oa = np.empty(e.size)
+        ob = np.empty(e.size)
+        nerr = lib().orc_sab_sample_highe_many(self.ptr, seed, stream, _d(e), e.size, _d(oa), _d(ob))
+        return oa, ob, nerr
 
     def sample_ab(self, ekin, seed=1, stream=0, nthreads=1):
         e = np.ascontiguousarray(np.atleast_1d(np.asarray(ekin, dtype=np.float64)))

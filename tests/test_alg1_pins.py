@@ -249,7 +249,7 @@ class Procedure:
         """percentile U with alpha(U) = alpha (monotone root solve)"""
         lo = np.zeros_like(alpha)
         hi = np.ones_like(alpha)
-        for _ in range(60):
+        for _ in range(44):
             mid = 0.5 * (lo + hi)
             below = self.alpha_of_u(l, h, t, mid) < alpha
             lo = np.where(below, mid, lo)
@@ -296,6 +296,9 @@ class Procedure:
                 pts.append(seg[1:] if pts else seg)
         xb = np.concatenate(pts)
         g = self.density(xb)
+        # the accepted percentile window on this grid, for conditional_u
+        rest = (xb > self.b1) & (xb > -self.c)
+        self._wgrid = (xb[rest], self.window(xb[rest]))
         cdf = np.concatenate([[0.0], np.cumsum(0.5 * (g[1:] + g[:-1]) * np.diff(xb))])
         return xb, cdf / cdf[-1]
 
@@ -309,18 +312,24 @@ class Procedure:
         f0, f1 = r1.cdf(lo), r1.cdf(hi)
         v[first] = (r1.cdf(alpha[first]) - f0) / (f1 - f0)
         rest = ~first
-        w = self.window(beta[rest])
-        l, h, t = w[0].astype(int), w[1].astype(int), w[2]
-        u = np.empty(rest.sum())
+        br = beta[rest]
+        l, h = self.rows_of(br)
+        t = (br - self.bg[l]) / (self.bg[h] - self.bg[l])
+        u = np.empty(br.size)
         for k in np.unique(h):
             sel = h == k
             u[sel] = self.u_of_alpha(k - 1, k, t[sel], alpha[rest][sel])
-        v[rest] = (u - w[3]) / (w[4] - w[3])
+        # window interpolated from the fine grid of beta_cdf (smooth within
+        # each bin; grid steps < 1e-3 of a bin)
+        xg, wg = self._wgrid
+        ulo = np.interp(br, xg, wg[3])
+        uhi = np.interp(br, xg, wg[4])
+        v[rest] = (u - ulo) / (uhi - ulo)
         return v
 
 
 def _ks_pair(proc, a, b):
-    xb, cdf = proc.beta_cdf()
+    xb, cdf = proc.beta_cdf()  # also tabulates the percentile window
     d_beta = stats.kstest(b, lambda v: np.interp(v, xb, cdf)).statistic
     d_alpha = stats.kstest(proc.conditional_u(a, b), "uniform").statistic
     return d_beta, d_alpha

@@ -228,13 +228,14 @@ def _bruteforce_beta_xs(alpha, srow, alow, aupp):
 
 def test_sab_xs_bruteforce(oracle):
     """Cross section from the integrator vs independent quadrature of the
-    cell-interpolated table: alpha integrals per beta row (log-linear
-    interpolant), trapezoid in beta with the first bin scaled by 4/3
-    (NCSABIntegrator.cc:501-526), xs = sum * sigma / (4 E/kT)."""
+    cell-interpolated table to 1e-12: alpha integrals per beta row (log-linear
+    interpolant, adaptive quadrature at 1e-12 per cell), trapezoid in beta with
+    the first bin scaled by 4/3 (NCSABIntegrator.cc:501-526), xs = sum *
+    sigma / (4 E/kT); the last energy puts -E/kT below the beta grid."""
     d, s = _sab_small(oracle)
     alpha, beta, sab = d["alpha"], d["beta"], d["sab"]
     kT = 8.6173303e-5 * d["temperature"]
-    for e in (0.003, 0.02, 0.09):
+    for e in (0.003, 0.02, 0.09, 0.6):
         ediv = e / kT
         pts = []
         for ib, b in enumerate(beta):
@@ -244,14 +245,18 @@ def test_sab_xs_bruteforce(oracle):
             am = max(0.0, kk + ediv - 2 * math.sqrt(ediv * kk))
             ap = kk + ediv + 2 * math.sqrt(ediv * kk)
             pts.append((b, _bruteforce_beta_xs(alpha, sab[ib], am, ap)))
-        b0 = max(-ediv, beta[0] - 1e-9)
+        if -ediv >= beta[0]:
+            b0 = -ediv
+        else:  # just below the grid (NCSABIntegrator.cc:379-388)
+            b0 = min(beta[0] - (beta[1] - beta[0]) * 1e-6, beta[0] - abs(beta[0]) * 1e-13,
+                     np.nextafter(beta[0], -np.inf))
         bx = [b0] + [p[0] for p in pts]
         by = [0.0] + [p[1] for p in pts]
         if -ediv >= beta[0]:
             bx[0] = b0 - (bx[1] - b0) / 3.0  # 4/3 first-bin rule
         tot = np.trapezoid(by, bx)
         exact = tot * d["sigma_bound"] / (4 * ediv)
-        assert s.analyse_xs(e) == pytest.approx(exact, rel=1e-8), e
+        assert s.analyse_xs(e) == pytest.approx(exact, rel=1e-12), e
 
 
 def test_sab_xs_provider_limits(oracle):
