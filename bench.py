@@ -1,31 +1,35 @@
-"""Benchmark: S(alpha,beta) scatter sampling throughput on one MI355X.
+"""Benchmark: the per-neutron physics hot path on one MI355X.
 
-Default workload (one "step" = one pass of the sampler over one batch):
-  S(alpha,beta) table 200 (alpha) x 800 (beta) FP64 from a three-Gaussian
-  VDOS (centres 0.01-0.12 eV, widths 0.003-0.02 eV, seeded) on 500 points up
-  to 0.15 eV, T = 293 K; batch of 1e8 neutrons with E log-uniform in
-  [1e-3, 1] eV resident in HBM; outputs E' and mu (FP64) in HBM.  Three
-  samplers are timed: the plain (Alg1) sampler, the box-envelope sampler on
-  a uniform grid and the learned sampler (the same envelope sampler on grids
-  placed by the MLP trained on the plain sampler's samples; its training time
-  is reported); `value` is the learned sampler.  KS distances of E' and mu
-  against the CPU checker's plain sampler at fixed energies are reported.
+Default run (`python bench.py`): the headline workload and, in the same JSON
+line under "subresults", every other configuration of the workload set, each
+with its own roofline figure and CPU baseline:
 
---workload powderbragg: PowderBragg of the seeded orthorhombic 24-atom cell
-  at dcutoff 0.1 Aa (~1e5 planes), 1e8 neutrons, E log-uniform 1e-4..1 eV.
+  headline (C3): S(alpha,beta) scatter sampling.  Table 200 (alpha) x 800
+    (beta) FP64 from a three-Gaussian VDOS (centres 0.01-0.12 eV, widths
+    0.003-0.02 eV, seeded) on 500 points up to 0.15 eV, T = 293 K; batch of
+    1e8 neutrons with E log-uniform in [1e-3, 1] eV resident in HBM; outputs
+    E' and mu (FP64) in HBM.  Three samplers are timed: the plain (Alg1)
+    sampler, the box-envelope sampler on a uniform grid and the learned
+    sampler (the same envelope sampler on grids placed by the MLP trained on
+    the plain sampler's samples; its training time is reported); `value` is
+    the learned sampler.  KS distances of E' and mu against the CPU checker's
+    plain sampler at fixed energies are reported.
+  C1: fcc powder (a = 4.05 Aa, b_coh 3.449 fm, T_Debye 410 K, T 293 K,
+    dcutoff 0.5 Aa): ElInc + PowderBragg + S(alpha,beta) from the Debye VDOS;
+    1e6 neutrons, E log-uniform 1e-5..10 eV, isotropic directions; one step =
+    cross sections + oriented scatter sampling.
+  C2: PowderBragg of the seeded orthorhombic 24-atom cell at dcutoff 0.1 Aa
+    (~1e5 planes), 1e8 neutrons, E log-uniform 1e-4..1 eV.
+  C4: 512 materials (ElInc + PowderBragg + S(alpha,beta) each) of uneven
+    sizes in one persistent launch, ~1.1e8 neutrons per step.
+  C5: mosaic single crystal, the seeded orthorhombic cell, dcutoff 0.2 Aa,
+    mosaicity FWHM 30 arcmin, seeded random orientation; 1e7 neutrons with
+    wavelength ~ U(1,6) Aa and directions in a 2-degree cone about a seeded
+    axis; one step = cross sections + scatter sampling.
 
---workload batch: 512 materials (ElInc + PowderBragg + S(alpha,beta) each)
-  of uneven sizes in one persistent launch, ~1.1e8 neutrons per step.
-
---workload scbragg: mosaic single crystal, the seeded orthorhombic cell
-  (a,b,c ~ U(5,15) Aa, 3 species x 8 atoms), dcutoff 0.2 Aa, mosaicity FWHM
-  30 arcmin, seeded random orientation; 1e7 neutrons with wavelength
-  ~ U(1,6) Aa and directions in a 2-degree cone about a seeded axis; one
-  step = cross sections + scatter sampling of the whole batch.
-
-Prints one JSON line (metric, value, roofline of the dominant kernel from
-HIP events on the launch stream, CPU baseline of the CPU checker timed on a
-bounded sample on this host).
+`--workload` runs one of them alone.  Roofline figures come from HIP events
+on the launch stream; the CPU baseline is the CPU checker timed on a bounded
+sample of the same workload on this host's cores.
 """
 import argparse
 import json
@@ -60,11 +64,50 @@ def parse():
                     help="also run one device-to-device copy of the output size (counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["sab", "scbragg", "powderbragg", "batch"], default="sab")
+    ap.add_argument("--workload", choices=["all", "sab", "c1", "scbragg", "powderbragg", "batch"], default="all")
+    ap.add_argument("--sub-cpu-seconds", type=float, default=8.0,
+                    help="CPU-baseline sample length of each sub-result")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--ks-samples", type=int, default=10_000_000,
                     help="samples per energy for the KS check against the CPU reference sampler (0: skip)")
     return ap.parse_args()
+
+
+# the host's share of CPU threads on a one-GPU node: the CPU baseline uses all
+# threads of this process's affinity mask, at most this many
+CPU_SHARE = 16
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_threads(args):
+    if args.cpu_threads:
+        return args.cpu_threads
+    return max(1, min(len(os.sched_getaffinity(0)), CPU_SHARE))
+
+
+def host_info():
+    model = None
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_threads": len(os.sched_getaffinity(0)),
+            "baseline_threads_cap": CPU_SHARE}
+
+
+def library_info():
+    """which native library ran: path and content hash (the pushed build is
+    run as is; nothing is rebuilt on the GPU host)"""
+    import hashlib
+    lib = ROOT / "ncrystal_amd" / "libncrystal_amd.so"
+    h = hashlib.sha256(lib.read_bytes()).hexdigest()[:16] if lib.exists() else None
+    return {"path": str(lib.relative_to(ROOT)), "sha256_16": h}
 
 
 def cpu_baseline(table, batch, seconds, threads):
@@ -141,6 +184,102 @@ def cpu_baseline_scb(args_scb, e, dirs, seconds, threads):
                        f"cross section + sampling), {threads} threads, {dt:.1f} s")
 
 
+def c1_setup():
+    from ncrystal_amd import materials, synth
+    cr = synth.fcc_powder()
+    sab = synth.sab_table_c1()
+    m = materials.Material(cr, sab=sab)
+    return cr, sab, m
+
+
+def cpu_baseline_c1(m, sab, e, dirs, seconds, threads):
+    """CPU checker timed on the C1 composition: cross sections + oriented
+    sampling of the whole 1e6-neutron batch, repeated for about `seconds`."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    from ncrystal_amd import materials
+    pyoracle.build()
+    mm, bb, ss = materials.elinc_inputs(m.crystal, m.msd)
+    v0n, dsp, fm = m.pb_inputs
+    o = pyoracle.composition([pyoracle.elinc(mm, bb, ss), pyoracle.powderbragg(v0n, dsp, fm),
+                              pyoracle.sab(sab["alpha"], sab["beta"], sab["sab"], sab["temperature"], sab["mass"],
+                                           sab["sigma_bound"])], [1.0, 1.0, 1.0])
+    done, k = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        o.xs(e, nthreads=threads)
+        o.sample_dir(e, dirs, seed=5, stream=k, nthreads=threads)
+        done += e.size
+        k += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    return dict(value=done / dt, unit="neutrons/s (xs + oriented sample)", cores=threads, kind="port",
+                sample=f"{done} neutrons ({k} passes over the 1e6-neutron batch), cross section + oriented "
+                       f"sampling, {threads} threads, {dt:.1f} s")
+
+
+def run_c1(args):
+    import numpy as np
+    from ncrystal_amd import _capi as C
+    from ncrystal_amd import synth
+    n = 1_000_000
+    t0 = time.perf_counter()
+    cr, sab, m = c1_setup()
+    t_setup = time.perf_counter() - t0
+    g = m.scatter
+    e = synth.energies_logu(n, 1e-5, 10.0, seed=1)
+    dirs = np.ascontiguousarray(synth.isotropic_dirs(n, seed=1))
+    stream = C.Stream()
+    de, dd = C.DeviceArray.from_numpy(e), C.DeviceArray.from_numpy(dirs.ravel())
+    xs, oe, od = C.DeviceArray(n), C.DeviceArray(n), C.DeviceArray(3 * n)
+    g.setRandomSeed(1)
+    ev = [C.Event() for _ in range(3)]
+    for _ in range(args.warmup):
+        g.dev_crossSectionIsotropic(de.ptr, n, xs.ptr, stream=stream.ptr)
+        g.dev_sampleScatter(de.ptr, dd.ptr, n, oe.ptr, od.ptr, stream=stream.ptr)
+    stream.sync()
+    xs_ms = samp_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ev[0].record(stream)
+        g.dev_crossSectionIsotropic(de.ptr, n, xs.ptr, stream=stream.ptr)
+        ev[1].record(stream)
+        g.dev_sampleScatter(de.ptr, dd.ptr, n, oe.ptr, od.ptr, stream=stream.ptr)
+        ev[2].record(stream)
+        stream.sync()
+        xs_ms += ev[0].elapsed_ms(ev[1])
+        samp_ms += ev[1].elapsed_ms(ev[2])
+    wall = time.perf_counter() - t0
+    xs_ms /= args.steps
+    samp_ms /= args.steps
+    assert g.dev_check_errors(stream=stream.ptr) == 0
+    o = od.numpy().reshape(n, 3)
+    assert np.all(xs.numpy() >= 0.0) and np.abs(np.linalg.norm(o, axis=1) - 1).max() < 1e-12, "invalid output"
+    # E + dir in (32 B), E' + dir out (32 B); xs: E in, xs out (16 B)
+    ach = 64 * n / (samp_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_c1(m, sab, e, dirs, args.cpu_seconds, cpu_threads(args))
+    return {
+        "metric": "fcc powder neutrons/s on 1 MI355X (cross section + oriented scatter sampling)",
+        "value": n * args.steps / wall, "unit": "neutrons/s",
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "dtype": "f64",
+        "config": {"workload": "fcc powder a=4.05 Aa, b_coh 3.449 fm, sigma_inc 0.0082 b, T_Debye 410 K, T 293 K, "
+                               "dcutoff 0.5 Aa: ElInc + PowderBragg + S(alpha,beta) of the Debye VDOS (%dx%d, "
+                               "Emax %.3f eV, free-gas extender above); E log-uniform 1e-5..10 eV, isotropic "
+                               "directions, seed 1" % (sab["alpha"].size, sab["beta"].size,
+                                                       m.components["inelas"].egrid()[0][-1]),
+                   "neutrons_per_step": n, "families": int(len(m.pb_inputs[1])), "setup_s": t_setup},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "oriented sampling call (k_comp_pick + k_comp_sab + high-energy queue)",
+                     "kernel_ms": samp_ms, "bytes_per_neutron": 64, "xs_call_ms": xs_ms,
+                     "xs_GBs": 16 * n / (xs_ms * 1e-3) / 1e9},
+        "cpu_baseline": cpu,
+    }
+
+
 def run_scbragg(args):
     import numpy as np
     from ncrystal_amd import _capi as C
@@ -193,7 +332,7 @@ def run_scbragg(args):
     achieved = flops / (samp_ms * 1e-3) / 1e12
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu_threads(args)
         cpu = cpu_baseline_scb(a, e, dirs, args.cpu_seconds, threads)
     out = {
         "metric": "SCBragg oriented neutrons/s on 1 MI355X (cross section + scatter sampling)",
@@ -214,7 +353,7 @@ def run_scbragg(args):
                      "flops_per_launch": flops, "normals_scanned_per_neutron": scanned / n},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    return out
 
 
 def cpu_baseline_pb(v0n, dsp, fm, seconds, threads):
@@ -285,7 +424,7 @@ def run_powderbragg(args):
     ach = 24 * n / (ts * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu_threads(args)
         cpu = cpu_baseline_pb(v0n, dsp, fsq * mult, args.cpu_seconds, threads)
     out = {
         "metric": "PowderBragg neutrons/s on 1 MI355X (FP64 xs + FP32-table xs + scatter sampling)",
@@ -302,7 +441,7 @@ def run_powderbragg(args):
                      "xs_fp64_GBs": 16 * n / (t64 * 1e-3) / 1e9, "xs_fp32_GBs": 16 * n / (t32 * 1e-3) / 1e9},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    return out
 
 
 def batch_setup(core, mats):
@@ -408,7 +547,7 @@ def run_batch(args):
     value = tot * args.steps / wall
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu_threads(args)
         cpu = cpu_baseline_batch(mats, args.cpu_seconds, threads)
     # the whole step (cross-section launch + the three sampling stages, timed together):
     # E in, xs out (16 B) + E in, E' and mu out (24 B) per neutron
@@ -444,7 +583,7 @@ def run_batch(args):
                      "kernel_ms": ms, "bytes_per_neutron": 40},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    return out
 
 
 def ks_sorted(a_sorted, b):
@@ -516,6 +655,7 @@ def run_sab(args):
 
     plain = core.create_sabscatter(*targs)
     r_plain = measure(plain)
+    progress(f"C3 plain sampler: {r_plain['ms']:.3f} ms per step")
     # learned proposal: trained on the plain sampler's samples with the GPU kernels
     t0 = time.perf_counter()
     model = core.MLPProposal(1e-3, 1.0, seed=1)
@@ -525,22 +665,25 @@ def run_sab(args):
     t_setup = time.perf_counter() - t0
     mass, uni, lam, sq = learned.envelope_stats().T
     r_learned = measure(learned)
+    progress(f"C3 learned sampler: {r_learned['ms']:.3f} ms per step (training and setup {t_setup:.1f} s)")
     # the same sampler on the uniform grid: what the network gains
     envelope = core.create_sabscatter(*targs)
     envelope.use_envelope_sampler(nintervals=256, emin=0.999e-3, emax=1.001, nbox_w=128, nbox_k=32)
     emass, _, _, esq = envelope.envelope_stats().T
     r_env = measure(envelope)
+    progress(f"C3 uniform-grid envelope sampler: {r_env['ms']:.3f} ms per step")
 
     if args.copy_calib:
         # plain copy of 2*8*n bytes read + written: counter calibration for traffic
         C.call("ncamd_memcpy_dtod", oe.ptr, ekin.ptr, 8 * n, stream.ptr)
         stream.sync()
 
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = cpu_threads(args)
     ks = None
     if args.ks_samples > 0:
         ks = ks_check({"plain": plain, "learned": learned, "envelope": envelope}, table,
                       [0.001, 0.01, 0.025, 0.1, 0.5, 1.0], args.ks_samples, threads)
+    progress("C3 KS checks done")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(table, args.cpu_batch, args.cpu_seconds, threads)
@@ -579,21 +722,38 @@ def run_sab(args):
                      "bytes_per_sample": BYTES_PER_SAMPLE},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    return out
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        print(json.dumps({"error": "this build measures one GPU only (a one-GPU node); run with --gpus 1"}))
         sys.exit(2)
-    if args.workload == "scbragg":
-        return run_scbragg(args)
-    if args.workload == "powderbragg":
-        return run_powderbragg(args)
-    if args.workload == "batch":
-        return run_batch(args)
-    return run_sab(args)
+    runners = {"sab": run_sab, "c1": run_c1, "powderbragg": run_powderbragg, "batch": run_batch,
+               "scbragg": run_scbragg}
+    if args.workload != "all":
+        out = runners[args.workload](args)
+        out.update(n_gpus=1, host=host_info(), library=library_info())
+        print(json.dumps(out))
+        return
+    out = run_sab(args)
+    sub_args = argparse.Namespace(**vars(args))
+    sub_args.cpu_seconds = args.sub_cpu_seconds
+    sub = {}
+    for key, name in (("C1", "c1"), ("C2", "powderbragg"), ("C4", "batch"), ("C5", "scbragg")):
+        t0 = time.perf_counter()
+        r = runners[name](sub_args)
+        for k in ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):
+            r.pop(k, None)
+        r["wall_s"] = time.perf_counter() - t0
+        sub[key] = r
+        print(f"[bench] {key} done: {r['value']:.4g} {r['unit']}", file=sys.stderr, flush=True)
+    out["config"]["workload_key"] = "C3"
+    out["subresults"] = sub
+    out["host"] = host_info()
+    out["library"] = library_info()
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

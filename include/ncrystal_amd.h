@@ -34,16 +34,26 @@ int ncamd_error(void);
 const char* ncamd_lasterror(void);
 const char* ncamd_lasterrortype(void);
 void ncamd_clearerror(void);
-void ncamd_sethaltonerror(int);
-void ncamd_setquietonerror(int);
+int ncamd_sethaltonerror(int);   /* returns the old value */
+int ncamd_setquietonerror(int);  /* returns the old value */
+/* called with (type, message) on every error, before printing/halting
+ * (ncrystal.h:1047-1049) */
+void ncamd_seterrhandler(void (*handler)(char* type, char* message));
 
 /* ---------- handle lifetime (ncrystal.h:672-676) ---------- */
 void ncamd_ref(ncamd_process_t);
 void ncamd_unref(ncamd_process_t*);
 int ncamd_refcount(ncamd_process_t);
 int ncamd_valid(ncamd_process_t);
-/* 0 isotropic powder-like process, 1 oriented single crystal */
+/* 0 isotropic powder-like process, 1 oriented single crystal
+ * (ncrystal_isnonoriented of include/ncrystal_dropin.h is its negation, as
+ * the reference's ncrystal.h:752) */
 int ncamd_isoriented(ncamd_process_t);
+/* a process that shares the tables of `h` with its own stream (seed) and
+ * staging memory, for use from another host thread; the source lives while
+ * clones exist, and neither may change its sampler mode meanwhile
+ * (Scatter::clone, ncrystal_core/include/NCrystal/interfaces/NCProc.hh:118-127) */
+ncamd_process_t ncamd_clone(ncamd_process_t h, uint64_t seed);
 
 /* ---------- material compilation (init path) ---------- */
 

@@ -77,8 +77,9 @@ def lib():
         "ncamd_lasterror": (ctypes.c_char_p, []),
         "ncamd_lasterrortype": (ctypes.c_char_p, []),
         "ncamd_clearerror": (None, []),
-        "ncamd_sethaltonerror": (None, [ctypes.c_int]),
-        "ncamd_setquietonerror": (None, [ctypes.c_int]),
+        "ncamd_sethaltonerror": (ctypes.c_int, [ctypes.c_int]),
+        "ncamd_setquietonerror": (ctypes.c_int, [ctypes.c_int]),
+        "ncamd_clone": (Handle, [Handle, ctypes.c_uint64]),
         "ncamd_ref": (None, [Handle]),
         "ncamd_unref": (None, [ctypes.POINTER(Handle)]),
         "ncamd_refcount": (ctypes.c_int, [Handle]),

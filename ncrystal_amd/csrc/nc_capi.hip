@@ -27,6 +27,7 @@
 #include <atomic>
 
 #include "../../include/ncrystal_amd.h"
+#include "../../include/ncrystal_dropin.h"
 #include "nc_kernels.hh"
 #include "nc_math.hh"
 
@@ -37,12 +38,14 @@ namespace {
 int g_quiet = 0, g_halt = 1, g_waserror = 0;
 char g_errmsg[512];
 char g_errtype[64];
+void (*g_errhandler)(char*, char*) = nullptr;  // called with (type, message)
 
 void set_error(const char* msg, const char* etype) noexcept {
   strncpy(g_errmsg, msg, sizeof(g_errmsg) - 1);
   strncpy(g_errtype, etype, sizeof(g_errtype) - 1);
   g_errmsg[sizeof(g_errmsg) - 1] = '\0';
   g_errtype[sizeof(g_errtype) - 1] = '\0';
+  if (g_errhandler) g_errhandler(g_errtype, g_errmsg);
   g_waserror = 1;
   if (!g_quiet) fprintf(stderr, "ncrystal_amd ERROR [%s]: %s\n", g_errtype, g_errmsg);
   if (g_halt) {
@@ -250,6 +253,10 @@ struct Proc {
   // composition
   std::vector<Proc*> comps;
   std::vector<double> scales;
+  // a clone reads the tables of its source, which it keeps alive; the source
+  // counts its live clones (its tables may not be rebuilt meanwhile)
+  Proc* src = nullptr;
+  int nclones = 0;
 
   DevBuf* newbuf() {
     bufs.emplace_back(new DevBuf());
@@ -259,6 +266,10 @@ struct Proc {
     magic = 0;
     for (auto* c : comps) {
       if (c && --c->refcount == 0) delete c;
+    }
+    if (src) {
+      --src->nclones;
+      if (--src->refcount == 0) delete src;
     }
   }
 };
@@ -2113,10 +2124,96 @@ void xs_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat, double*
   launch_xs_iso(make_view(p), ekin, n, repeat, out, fp32, st);
 }
 
-// scattering with new directions for an isotropic process or composition
 // an absorption process, or a composition of absorption processes
 bool is_absorption(const Proc* p) {
   return p->kind == kKindAbs || (p->kind == kKindNone && !p->comps.empty() && p->comps[0]->kind == kKindAbs);
+}
+
+// the sampler mode of an S(alpha,beta) process rebuilds tables that clones
+// read, so it is fixed while clones exist (and on clones)
+void check_tables_mutable(const Proc* p) {
+  if (p->src) raise("BadInput", "a clone cannot change its sampler (change the source before cloning)");
+  if (p->nclones) raise("BadInput", "a process with live clones cannot change its sampler");
+}
+
+// Process::domain of the reference: PowderBragg and SCBragg from their
+// threshold (NCPowderBragg.cc:147-150, NCSCBragg.cc:290-293; a PowderBragg
+// without planes is null: (inf, inf)), the other processes everywhere
+// (NCProcImpl.hh:446); a composition spans the union of its components'
+// non-null domains, (0, 0) when it has none (NCProcImpl.hh:370,
+// NCProcImpl.cc:285-296)
+void proc_domain(const Proc* p, double& lo, double& hi) {
+  const double inf = __builtin_huge_val();
+  switch (p->kind) {
+    case kKindPB:
+      lo = p->pb.n > 0 ? p->pb.threshold : inf;
+      hi = inf;
+      return;
+    case kKindSCB: lo = p->scb.threshold; hi = inf; return;
+    case kKindNone: {
+      lo = 0.0; hi = 0.0;
+      for (const Proc* c : p->comps) {
+        double a, b;
+        proc_domain(c, a, b);
+        if (a >= b) continue;
+        if (lo >= hi) { lo = a; hi = b; }
+        else { lo = std::min(lo, a); hi = std::max(hi, b); }
+      }
+      return;
+    }
+    default: lo = 0.0; hi = inf; return;
+  }
+}
+
+// ProcImpl::Process::name of the reference's classes
+const char* proc_name(const Proc* p) {
+  switch (p->kind) {
+    case kKindPB: return "PowderBragg";
+    case kKindElInc: return "ElIncScatter";
+    case kKindSAB: return "SABScatter";
+    case kKindSCB: return "SCBragg";
+    case kKindFG: return "FreeGas";
+    case kKindAbs: return "AbsOOV";
+    case kKindNone: return "ProcComposition";
+    default: return "unknown";
+  }
+}
+
+// a process reading the tables of `s` with its own stream and staging memory
+Proc* clone_proc(Proc* s, uint64_t seed) {
+  ensure_device();
+  Proc* root = s->src ? s->src : s;  // clones of clones read the root's tables
+  std::unique_ptr<Proc> c(new Proc());
+  c->kind = s->kind;
+  c->seed = seed;
+  c->pb = s->pb; c->ei = s->ei; c->sab = s->sab; c->fg = s->fg; c->abs_c = s->abs_c;
+  c->oriented_comp = s->oriented_comp;
+  c->pb_v0n = s->pb_v0n; c->pb_e2d_h = s->pb_e2d_h; c->pb_fdm_h = s->pb_fdm_h; c->pb_fdm32_h = s->pb_fdm32_h;
+  c->has_fp32 = s->has_fp32;
+  c->sab_na = s->sab_na; c->sab_nb = s->sab_nb; c->sab_alpha_h = s->sab_alpha_h; c->sab_beta_h = s->sab_beta_h;
+  c->use_kb = s->use_kb; c->kb_maxtries = s->kb_maxtries; c->kb = s->kb;
+  c->kb_node_h = s->kb_node_h; c->kb_box_h = s->kb_box_h; c->kb_stats_h = s->kb_stats_h;
+  c->sab_egrid_h = s->sab_egrid_h; c->sab_xs_h = s->sab_xs_h;
+  c->scb = s->scb;
+  c->scb_inv2d_h = s->scb_inv2d_h; c->scb_xsfact_h = s->scb_xsfact_h; c->scb_normals_h = s->scb_normals_h;
+  c->scb_lt1_h = s->scb_lt1_h; c->scb_lt2_h = s->scb_lt2_h; c->scb_off_h = s->scb_off_h;
+  std::copy(s->scb_gos, s->scb_gos + 15, c->scb_gos);
+  c->comps = s->comps;
+  c->scales = s->scales;
+  init_errbuf(c.get());
+  for (Proc* x : c->comps) x->refcount++;
+  c->src = root;
+  root->refcount++;
+  root->nclones++;
+  return c.release();
+}
+
+// seed of a clone: splitmix64 of (source seed, salt)
+uint64_t clone_seed(uint64_t seed, uint64_t salt) {
+  uint64_t z = seed ^ (0x9E3779B97F4A7C15ull * (salt + 1));
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
 }
 
 void sample_dirs_dispatch(Proc* p, const double* ekin, const double* dirs, int64_t n, double* oE, double* oD,
@@ -2224,8 +2321,9 @@ int ncamd_error(void) { return g_waserror; }
 const char* ncamd_lasterror(void) { return g_waserror ? g_errmsg : ""; }
 const char* ncamd_lasterrortype(void) { return g_waserror ? g_errtype : ""; }
 void ncamd_clearerror(void) { g_waserror = 0; g_errmsg[0] = '\0'; g_errtype[0] = '\0'; }
-void ncamd_sethaltonerror(int h) { g_halt = h; }
-void ncamd_setquietonerror(int q) { g_quiet = q; }
+int ncamd_sethaltonerror(int h) { const int o = g_halt; g_halt = h; return o; }
+int ncamd_setquietonerror(int q) { const int o = g_quiet; g_quiet = q; return o; }
+void ncamd_seterrhandler(void (*handler)(char*, char*)) { g_errhandler = handler; }
 
 void ncamd_ref(ncamd_process_t h) {
   NCAMD_TRY extract(h)->refcount++; NCAMD_CATCH
@@ -2550,8 +2648,8 @@ static void xs_many_impl(ncamd_process_t h, const double* ekin, unsigned long n,
                          bool fp32) {
   bool ok = false;
   NCAMD_TRY
+  Proc* p = extract(h);  // an invalid handle is a LogicError before any device check
   ensure_device();
-  Proc* p = extract(h);
   p->st_in.ensure(sizeof(double) * n);
   p->st_out1.ensure(sizeof(double) * n * repeat);
   if (n) NCAMD_HIP(hipMemcpy(p->st_in.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
@@ -2576,8 +2674,8 @@ void ncamd_samplescatterisotropic_many(ncamd_process_t h, const double* ekin, un
                                        double* oE, double* oMu) {
   bool ok = false;
   NCAMD_TRY
+  Proc* p = extract(h);  // an invalid handle is a LogicError before any device check
   ensure_device();
-  Proc* p = extract(h);
   const size_t tot = (size_t)n * repeat;
   p->st_in.ensure(sizeof(double) * n);
   p->st_out1.ensure(sizeof(double) * tot);
@@ -2603,8 +2701,8 @@ void ncamd_samplescatter_many_dirs(ncamd_process_t h, const double* ekin, const 
                                    double* oE, double* oD) {
   bool ok = false;
   NCAMD_TRY
+  Proc* p = extract(h);  // an invalid handle is a LogicError before any device check
   ensure_device();
-  Proc* p = extract(h);
   const bool scb = p->kind == kKindSCB;
   p->st_in.ensure(sizeof(double) * n);
   p->st_in2.ensure(sizeof(double) * 3 * n);
@@ -2656,8 +2754,8 @@ void ncamd_crosssection_many_dirs(ncamd_process_t h, const double* ekin, const d
   }
   bool ok = false;
   NCAMD_TRY
+  Proc* p = extract(h);  // an invalid handle is a LogicError before any device check
   ensure_device();
-  Proc* p = extract(h);
   p->st_in.ensure(sizeof(double) * n);
   p->st_in2.ensure(sizeof(double) * 3 * n);
   p->st_out1.ensure(sizeof(double) * n);
@@ -2893,6 +2991,7 @@ int ncamd_sab_use_envelope_sampler(ncamd_process_t h, int nintervals, double emi
                                    int nbox_k, int maxtries) {
   NCAMD_TRY
   ensure_device();
+  check_tables_mutable(extract(h));
   build_kb(extract(h), nintervals, emin, emax, nbox_w, nbox_k, maxtries, nullptr);
   return 0;
   NCAMD_CATCH
@@ -2901,6 +3000,7 @@ int ncamd_sab_use_envelope_sampler(ncamd_process_t h, int nintervals, double emi
 int ncamd_sab_use_plain_sampler(ncamd_process_t h) {
   NCAMD_TRY
   Proc* p = extract(h);
+  check_tables_mutable(p);
   if (p->kind != kKindSAB) raise("BadInput", "not an S(alpha,beta) process");
   p->use_kb = false;
   return 0;
@@ -2911,6 +3011,7 @@ int ncamd_sab_use_learned_sampler(ncamd_process_t h, ncamd_mlp_t model, int nint
                                   int nbox_w, int nbox_k, int maxtries) {
   NCAMD_TRY
   ensure_device();
+  check_tables_mutable(extract(h));
   build_learned(extract(h), extract_mlp(model), nintervals, emin, emax, nbox_w, nbox_k, maxtries);
   return 0;
   NCAMD_CATCH
@@ -3169,5 +3270,260 @@ void ncamd_dev_fill_loguniform(double* d, uint64_t n, double emin, double emax, 
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH
 }
+
+
+ncamd_process_t ncamd_clone(ncamd_process_t h, uint64_t seed) {
+  NCAMD_TRY return wrap(clone_proc(extract(h), seed)); NCAMD_CATCH
+  return wrap(nullptr);
+}
+
+// =================================================== NCrystal C API names ===
+// The ncrystal.h functions of the per-neutron path (declared in
+// include/ncrystal_dropin.h) on the same process objects and error state.
+
+namespace {
+void*& handle_internal(void* addr) { return *static_cast<void**>(addr); }
+ncamd_process_t as_ncamd(void* internal) {
+  ncamd_process_t h;
+  h.internal = internal;
+  return h;
+}
+Proc* proc_of(void* internal) { return extract(as_ncamd(internal)); }
+thread_local unsigned long t_thread_stream = 0;
+std::atomic<unsigned long> g_thread_streams{0};
+std::atomic<uint64_t> g_clone_counter{0};
+}  // namespace
+
+ncrystal_scatter_t ncamd_to_ncrystal_scatter(ncamd_process_t h) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (is_absorption(p)) raise("BadInput", "an absorption process is not a scatter process");
+  p->refcount++;
+  return ncrystal_scatter_t{p};
+  NCAMD_CATCH
+  return ncrystal_scatter_t{nullptr};
+}
+ncrystal_absorption_t ncamd_to_ncrystal_absorption(ncamd_process_t h) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  if (!is_absorption(p)) raise("BadInput", "a scatter process is not an absorption process");
+  p->refcount++;
+  return ncrystal_absorption_t{p};
+  NCAMD_CATCH
+  return ncrystal_absorption_t{nullptr};
+}
+ncamd_process_t ncamd_from_ncrystal(ncrystal_process_t h) { return as_ncamd(h.internal); }
+
+// version and errors (ncrystal.cc:279-323, ncrystal.h:1228-1234)
+int ncrystal_version(void) { return 4004002; }
+const char* ncrystal_version_str(void) { return "4.4.2"; }
+const char* ncrystal_namespace(void) { return ""; }
+int ncrystal_sethaltonerror(int h) { return ncamd_sethaltonerror(h); }
+int ncrystal_setquietonerror(int q) { return ncamd_setquietonerror(q); }
+int ncrystal_error(void) { return g_waserror; }
+const char* ncrystal_lasterror(void) { return g_waserror ? g_errmsg : nullptr; }
+const char* ncrystal_lasterrortype(void) { return g_waserror ? g_errtype : nullptr; }
+void ncrystal_clearerror(void) { ncamd_clearerror(); }
+void ncrystal_seterrhandler(void (*handler)(char*, char*)) { g_errhandler = handler; }
+
+// reference counting and casts (ncrystal.cc:392-503)
+int ncrystal_refcount(void* addr) {
+  NCAMD_TRY return proc_of(handle_internal(addr))->refcount; NCAMD_CATCH
+  return -999;
+}
+void ncrystal_ref(void* addr) {
+  NCAMD_TRY proc_of(handle_internal(addr))->refcount++; NCAMD_CATCH
+}
+void ncrystal_unref(void* addr) {
+  NCAMD_TRY
+  void*& in = handle_internal(addr);
+  Proc* p = proc_of(in);
+  if (--p->refcount == 0) {
+    delete p;
+    in = nullptr;
+  }
+  NCAMD_CATCH
+}
+int ncrystal_valid(void* addr) {
+  if (!addr) return 0;
+  Proc* p = static_cast<Proc*>(handle_internal(addr));
+  return (p && p->magic == kMagic) ? 1 : 0;
+}
+void ncrystal_invalidate(void* addr) {
+  if (ncrystal_valid(addr)) handle_internal(addr) = nullptr;
+}
+ncrystal_process_t ncrystal_cast_scat2proc(ncrystal_scatter_t s) {
+  NCAMD_TRY
+  if (is_absorption(proc_of(s.internal))) raise("LogicError", "handle is not a scatter handle");
+  return ncrystal_process_t{s.internal};
+  NCAMD_CATCH
+  return ncrystal_process_t{nullptr};
+}
+ncrystal_process_t ncrystal_cast_abs2proc(ncrystal_absorption_t a) {
+  NCAMD_TRY
+  if (!is_absorption(proc_of(a.internal))) raise("LogicError", "handle is not an absorption handle");
+  return ncrystal_process_t{a.internal};
+  NCAMD_CATCH
+  return ncrystal_process_t{nullptr};
+}
+ncrystal_scatter_t ncrystal_cast_proc2scat(ncrystal_process_t p) {
+  NCAMD_TRY return ncrystal_scatter_t{is_absorption(proc_of(p.internal)) ? nullptr : p.internal}; NCAMD_CATCH
+  return ncrystal_scatter_t{nullptr};
+}
+ncrystal_absorption_t ncrystal_cast_proc2abs(ncrystal_process_t p) {
+  NCAMD_TRY return ncrystal_absorption_t{is_absorption(proc_of(p.internal)) ? p.internal : nullptr}; NCAMD_CATCH
+  return ncrystal_absorption_t{nullptr};
+}
+
+// clones (ncrystal.cc:1527-1563)
+ncrystal_scatter_t ncrystal_clone_scatter(ncrystal_scatter_t s) {
+  NCAMD_TRY
+  Proc* p = proc_of(s.internal);
+  if (is_absorption(p)) raise("LogicError", "handle is not a scatter handle");
+  return ncrystal_scatter_t{clone_proc(p, clone_seed(p->seed, 0x100000000ull + g_clone_counter++))};
+  NCAMD_CATCH
+  return ncrystal_scatter_t{nullptr};
+}
+ncrystal_scatter_t ncrystal_clone_scatter_rngbyidx(ncrystal_scatter_t s, unsigned long idx) {
+  NCAMD_TRY
+  Proc* p = proc_of(s.internal);
+  if (is_absorption(p)) raise("LogicError", "handle is not a scatter handle");
+  return ncrystal_scatter_t{clone_proc(p, clone_seed(p->seed, (uint64_t)idx))};
+  NCAMD_CATCH
+  return ncrystal_scatter_t{nullptr};
+}
+ncrystal_scatter_t ncrystal_clone_scatter_rngforcurrentthread(ncrystal_scatter_t s) {
+  if (!t_thread_stream) t_thread_stream = ++g_thread_streams;
+  return ncrystal_clone_scatter_rngbyidx(s, 0x80000000ul + t_thread_stream);
+}
+ncrystal_absorption_t ncrystal_clone_absorption(ncrystal_absorption_t a) {
+  NCAMD_TRY
+  Proc* p = proc_of(a.internal);
+  if (!is_absorption(p)) raise("LogicError", "handle is not an absorption handle");
+  return ncrystal_absorption_t{clone_proc(p, p->seed)};
+  NCAMD_CATCH
+  return ncrystal_absorption_t{nullptr};
+}
+
+// queries (ncrystal.cc:1061-1091)
+const char* ncrystal_name(ncrystal_process_t p) {
+  NCAMD_TRY return proc_name(proc_of(p.internal)); NCAMD_CATCH
+  return nullptr;
+}
+int ncrystal_isnonoriented(ncrystal_process_t p) {
+  NCAMD_TRY
+  const Proc* q = proc_of(p.internal);
+  return (q->kind == kKindSCB || q->oriented_comp) ? 0 : 1;
+  NCAMD_CATCH
+  return 0;
+}
+void ncrystal_domain(ncrystal_process_t p, double* lo, double* hi) {
+  NCAMD_TRY
+  proc_domain(proc_of(p.internal), *lo, *hi);
+  return;
+  NCAMD_CATCH
+  *lo = *hi = -1.0;
+}
+
+// single neutrons: one-element batched calls (ncrystal.cc:1093-1200)
+void ncrystal_crosssection_nonoriented(ncrystal_process_t p, double ekin, double* result) {
+  ncamd_crosssection_nonoriented_many(as_ncamd(p.internal), &ekin, 1, 1, result);
+}
+void ncrystal_crosssection(ncrystal_process_t p, double ekin, const double (*direction)[3], double* result) {
+  ncamd_crosssection_many_dirs(as_ncamd(p.internal), &ekin, &(*direction)[0], 1, result);
+}
+void ncrystal_samplescatterisotropic(ncrystal_scatter_t s, double ekin, double* ekin_final, double* mu) {
+  ncamd_samplescatterisotropic_many(as_ncamd(s.internal), &ekin, 1, 1, ekin_final, mu);
+}
+void ncrystal_samplescatter(ncrystal_scatter_t s, double ekin, const double (*direction)[3], double* ekin_final,
+                            double (*direction_final)[3]) {
+  ncamd_samplescatter_many_dirs(as_ncamd(s.internal), &ekin, &(*direction)[0], 1, ekin_final,
+                                &(*direction_final)[0]);
+}
+void ncrystal_samplescatter_rs(double (*rngfct)(void*), void* state, ncrystal_scatter_t s, double ekin,
+                               const double (*direction)[3], double* ekin_final, double (*direction_final)[3]) {
+  bool ok = false;
+  NCAMD_TRY
+  Proc* p = proc_of(s.internal);
+  if (!rngfct) raise("BadInput", "no random number function");
+  // two numbers of the caller's stream key this neutron's Philox stream
+  const double r1 = rngfct(state), r2 = rngfct(state);
+  const uint64_t k = ((uint64_t)(r1 * 9007199254740992.0) << 11) ^ (uint64_t)(r2 * 9007199254740992.0);
+  const uint64_t seed0 = p->seed, ctr0 = p->callctr;
+  p->seed = k;
+  p->callctr = 0;
+  ncamd_samplescatter_many_dirs(wrap(p), &ekin, &(*direction)[0], 1, ekin_final, &(*direction_final)[0]);
+  p->seed = seed0;
+  p->callctr = ctr0;
+  ok = !g_waserror;
+  NCAMD_CATCH
+  if (!ok) {
+    *ekin_final = -1.0;
+    (*direction_final)[0] = (*direction_final)[1] = (*direction_final)[2] = 0.0;
+  }
+}
+void ncrystal_genscatter_nonoriented(ncrystal_scatter_t s, double ekin, double* angle, double* dekin) {
+  double ef = -1.0, mu = -999.0;
+  ncamd_samplescatterisotropic_many(as_ncamd(s.internal), &ekin, 1, 1, &ef, &mu);
+  if (mu == -999.0 && ef == -1.0) { *angle = *dekin = -999.0; return; }
+  *dekin = ef - ekin;
+  *angle = std::acos(mu);
+}
+
+// batched calls (ncrystal.cc:1107-1330)
+void ncrystal_crosssection_nonoriented_many(ncrystal_process_t p, const double* ekin, unsigned long n,
+                                            unsigned long repeat, double* results) {
+  ncamd_crosssection_nonoriented_many(as_ncamd(p.internal), ekin, n, repeat, results);
+}
+void ncrystal_samplescatterisotropic_many(ncrystal_scatter_t s, const double* ekin, unsigned long n,
+                                          unsigned long repeat, double* results_ekin, double* results_mu) {
+  ncamd_samplescatterisotropic_many(as_ncamd(s.internal), ekin, n, repeat, results_ekin, results_mu);
+}
+void ncrystal_samplescatter_many(ncrystal_scatter_t s, double ekin, const double (*direction)[3],
+                                 unsigned long repeat, double* results_ekin, double* ux, double* uy, double* uz) {
+  ncamd_samplescatter_many(as_ncamd(s.internal), ekin, direction, repeat, results_ekin, ux, uy, uz);
+}
+void ncrystal_genscatter_nonoriented_many(ncrystal_scatter_t s, const double* ekin, unsigned long n,
+                                          unsigned long repeat, double* angle, double* dekin) {
+  // results_dekin receives E' and results_angle mu first, then both are
+  // converted in place
+  ncamd_samplescatterisotropic_many(as_ncamd(s.internal), ekin, n, repeat, dekin, angle);
+  const size_t tot = (size_t)n * repeat;
+  for (size_t i = 0; i < tot; ++i) {
+    if (angle[i] == -999.0 && dekin[i] == -1.0) { angle[i] = dekin[i] = -999.0; continue; }
+    dekin[i] -= ekin[i % n];
+    angle[i] = std::acos(angle[i]);
+  }
+}
+
+// RNG state of a scatter handle: "seed:callcounter"
+int ncrystal_rngsupportsstatemanip_ofscatter(ncrystal_scatter_t s) {
+  NCAMD_TRY proc_of(s.internal); return 1; NCAMD_CATCH
+  return 0;
+}
+char* ncrystal_getrngstate_ofscatter(ncrystal_scatter_t s) {
+  NCAMD_TRY
+  const Proc* p = proc_of(s.internal);
+  char buf[64];
+  snprintf(buf, sizeof(buf), "%llu:%llu", (unsigned long long)p->seed, (unsigned long long)p->callctr);
+  char* out = static_cast<char*>(malloc(strlen(buf) + 1));
+  if (!out) raise("CalcError", "out of memory");
+  strcpy(out, buf);
+  return out;
+  NCAMD_CATCH
+  return nullptr;
+}
+void ncrystal_setrngstate_ofscatter(ncrystal_scatter_t s, const char* state) {
+  NCAMD_TRY
+  Proc* p = proc_of(s.internal);
+  unsigned long long seed = 0, ctr = 0;
+  char tail = 0;
+  if (!state || sscanf(state, "%llu:%llu%c", &seed, &ctr, &tail) != 2)
+    raise("BadInput", "RNG state must read seed:callcounter");
+  p->seed = seed;
+  p->callctr = ctr;
+  NCAMD_CATCH
+}
+void ncrystal_dealloc_string(char* s) { free(s); }
 
 }  // extern "C"

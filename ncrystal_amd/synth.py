@@ -198,6 +198,35 @@ def sab_table_c3(seed=1, temperature=293.0, mass_amu=26.98, nalpha=200, nbeta=80
                 sigma_bound=1.5, vdos=(e, rho))
 
 
+def vdos_debye(debye_temp, npts=400):
+    """Debye VDOS: rho ~ E^2 up to k_B T_Debye, zero above."""
+    ed = K_BOLTZMANN * debye_temp
+    e = np.linspace(0.0, ed, npts)
+    return e, e * e
+
+
+def sab_table_c1(temperature=293.0, debye_temp=410.0, mass_amu=26.98, sigma_bound=None,
+                 nalpha=100, nbeta=240, emax_cover=0.6):
+    """S(alpha,beta) of the fcc powder's Debye-model inelastic scattering:
+    Debye VDOS to k_B 410 K expanded at 293 K on an alpha/beta grid covering
+    energies to `emax_cover` eV (the free-gas extender takes over above the
+    table's energy grid).  sigma_bound defaults to the fcc material's total
+    bound scattering xs 4 pi b_coh^2 + sigma_inc (b_coh 3.449 fm, 0.0082 b)."""
+    if sigma_bound is None:
+        sigma_bound = 4 * np.pi * 0.3449 ** 2 + 0.0082
+    e, rho = vdos_debye(debye_temp)
+    kT = K_BOLTZMANN * temperature
+    amax = 4.6 * emax_cover / kT
+    alpha = np.geomspace(1e-3, amax, nalpha)
+    bneg = -(emax_cover / kT) * 1.02
+    bpos = 15.0
+    u = np.linspace(-1.0, 1.0, nbeta)
+    beta = np.unique(np.where(u < 0, -np.abs(bneg) * np.abs(u) ** 1.5, bpos * np.abs(u) ** 1.5))
+    sab = phonon_expand_sab(e, rho, temperature, mass_amu, alpha, beta)
+    return dict(alpha=alpha, beta=beta, sab=sab, temperature=temperature, mass=mass_amu,
+                sigma_bound=float(sigma_bound), vdos=(e, rho))
+
+
 def small_sab_table(seed=3, temperature=293.0, mass_amu=12.0, nalpha=40, nbeta=60, amax=40.0,
                     bmin=-20.0, bmax=10.0):
     """Small table for fast tests (high-E extender path is reached above Emax)."""

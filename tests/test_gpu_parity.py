@@ -261,9 +261,13 @@ def test_sab_c3_table(amd, oracle):
 
 
 def test_composition_c1(amd, oracle):
+    """the fcc powder workload: ElInc + PowderBragg + S(alpha,beta) of the
+    Debye VDOS (rho ~ E^2 to k_B 410 K, 293 K), 1e6 neutrons with E
+    log-uniform 1e-5..10 eV (about a fifth above the table's Emax, in the
+    free-gas extender)"""
     core, materials = amd
     cr = synth.fcc_powder()
-    sabd = synth.small_sab_table(mass_amu=26.98)
+    sabd = synth.sab_table_c1()
     m = materials.Material(cr, sab=sabd)
     # oracle composition with the same inputs and order (ElInc, Bragg, SAB)
     msd = m.msd
@@ -274,11 +278,13 @@ def test_composition_c1(amd, oracle):
     o_sab = oracle.sab(sabd["alpha"], sabd["beta"], sabd["sab"], sabd["temperature"], sabd["mass"],
                        sabd["sigma_bound"])
     comp = oracle.composition([o_ei, o_pb, o_sab], [1.0, 1.0, 1.0])
-    e = synth.energies_logu(100000, 1e-5, 10.0, seed=1)
-    assert rel_err(m.scatter.crossSectionIsotropic(e), comp.xs(e)).max() <= XS_RTOL
+    e = synth.energies_logu(1000000, 1e-5, 10.0, seed=1)
+    emax = o_sab.egrid()[0][-1]
+    print("C1: fraction above Emax", np.mean(e >= emax))
+    assert rel_err(m.scatter.crossSectionIsotropic(e), comp.xs(e, nthreads=8)).max() <= XS_RTOL
     m.scatter.setRandomSeed(1)
     oe, om = m.scatter.sampleScatterIsotropic(e)
-    re_, rm, nerr = comp.sample_iso(e, seed=1)
+    re_, rm, nerr = comp.sample_iso(e, seed=1, nthreads=8)
     assert nerr == 0
     _compare_samples(oe, om, re_, rm, "composition")
     # oriented interface on an isotropic material: |dir|=1 and mu = dir_in.dir_out
@@ -286,9 +292,58 @@ def test_composition_c1(amd, oracle):
     m.scatter.setRandomSeed(2)
     oe2, od = m.scatter.sampleScatter(e, dirs)
     assert np.allclose(np.linalg.norm(od, axis=1), 1.0, atol=1e-12)
-    re2, rd, nerr = comp.sample_dir(e, dirs, seed=2)
+    re2, rd, nerr = comp.sample_dir(e, dirs, seed=2, nthreads=8)
     assert nerr == 0
     assert_streams_match(np.max(np.abs(od - rd), axis=1) > 1e-9, "oriented composition dirs")
+
+
+def _chi2_2sample(x1, y1, x2, y2, nb=40):
+    """two-sample chi-square on a 2-D histogram with equal-population bins of
+    the pooled sample in x, then in y within each x slice"""
+    xe = np.quantile(np.concatenate([x1, x2]), np.linspace(0, 1, nb + 1))
+    xe[0], xe[-1] = -np.inf, np.inf
+    chi2, dof = 0.0, -1
+    for i in range(nb):
+        s1 = (x1 >= xe[i]) & (x1 < xe[i + 1])
+        s2 = (x2 >= xe[i]) & (x2 < xe[i + 1])
+        ye = np.quantile(np.concatenate([y1[s1], y2[s2]]), np.linspace(0, 1, nb + 1))
+        ye[0], ye[-1] = -np.inf, np.inf
+        h1 = np.histogram(y1[s1], ye)[0].astype(float)
+        h2 = np.histogram(y2[s2], ye)[0].astype(float)
+        k1, k2 = math.sqrt(len(x2) / len(x1)), math.sqrt(len(x1) / len(x2))
+        m = (h1 + h2) > 0
+        chi2 += np.sum((k1 * h1[m] - k2 * h2[m]) ** 2 / (h1[m] + h2[m]))
+        dof += int(m.sum())
+    return chi2, dof
+
+
+@pytest.mark.parametrize("E", [0.001, 0.025, 0.5])
+def test_sab_alphabeta_chi2_c3(amd, oracle, E):
+    """(alpha, beta) of the plain GPU sampler on the 200 x 800 table against
+    the CPU checker's sampler on independent streams: two-sample chi-square
+    of the 2-D histogram (1600 cells) at 1e7 samples each, and KS of each
+    marginal"""
+    from ncrystal_amd import _capi as C
+    core, _ = amd
+    d = synth.sab_table_c3(seed=1)
+    g = core.create_sabscatter(d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"], d["sigma_bound"])
+    o = oracle.sab(d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"], d["sigma_bound"])
+    n = 10_000_000
+    st = C.Stream()
+    de = C.DeviceArray.from_numpy(np.full(n, E))
+    da, db = C.DeviceArray(n), C.DeviceArray(n)
+    g.setRandomSeed(4242)
+    g.dev_sample_alphabeta(de.ptr, n, da.ptr, db.ptr, stream=st.ptr)
+    assert g.dev_check_errors(stream=st.ptr) == 0
+    ga, gb = da.numpy(), db.numpy()
+    oa, ob, nerr = o.sample_ab(np.full(n, E), seed=99, nthreads=8)
+    assert nerr == 0
+    chi2, dof = _chi2_2sample(gb, ga, ob, oa)
+    p = stats.chi2.sf(chi2, dof)
+    print(f"E={E}: chi2 {chi2:.1f} for {dof} dof, p = {p:.3g}")
+    assert p > 1e-4
+    lim = 1.95 * math.sqrt(2.0 / n)
+    assert stats.ks_2samp(ga, oa).statistic < lim and stats.ks_2samp(gb, ob).statistic < lim
 
 
 def test_rng_state_roundtrip(amd, oracle):
