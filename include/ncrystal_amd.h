@@ -163,6 +163,21 @@ ncamd_process_t ncamd_create_sabscatter(int nalpha, int nbeta, const double* alp
                                         double sigma_bound, double suggested_emax,
                                         int n_egrid, const double* egrid);
 
+/* Scattering-kernel data to the standard S(alpha,beta) table, replaces
+ * SABUtils::transformKernelToStdFormat (ncrystal_core/src/sab/NCSABUtils.cc:
+ * 134-325) with validateScatKnlData's checks (NCScatKnlData.cc:26-80):
+ * all-zero edges trimmed (upper alpha columns, rows at beta > 0 from the top
+ * and at beta < 0 from the bottom), a symmetric table (knltype 2, beta grid
+ * from 0) mirrored to negative beta, a scaled table (knltype 1 or 2)
+ * unscaled by exp(-beta/2), and beta grids of fewer than 500 rows thickened
+ * by linear interpolation unless beta_grid_optimised.  knltype 0 is plain
+ * S(alpha,beta).  Call with NULL output arrays to get the sizes
+ * (out_nalpha, out_nbeta), then with arrays of those sizes; the result
+ * feeds ncamd_create_sabscatter.  Returns 0, or -1 on error. */
+int ncamd_sab_std_format(int knltype, int beta_grid_optimised, int nalpha, int nbeta, const double* alpha,
+                         const double* beta, const double* sab, int* out_nalpha, int* out_nbeta, double* out_alpha,
+                         double* out_beta, double* out_sab, int64_t* ntrimmed);
+
 /* Composition of processes with scales, replaces ProcComposition
  * (ncrystal_core/src/interfaces/NCProcImpl.cc:110-389).  Components are
  * added as ProcComposition::addComponent does (NCProcImpl.cc:263-338):

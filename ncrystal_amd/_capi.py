@@ -80,6 +80,8 @@ def lib():
         "ncamd_sethaltonerror": (ctypes.c_int, [ctypes.c_int]),
         "ncamd_setquietonerror": (ctypes.c_int, [ctypes.c_int]),
         "ncamd_clone": (Handle, [Handle, ctypes.c_uint64]),
+        "ncamd_sab_std_format": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                                _ip, _ip, _dp, _dp, _dp, _i64p]),
         "ncamd_ref": (None, [Handle]),
         "ncamd_unref": (None, [ctypes.POINTER(Handle)]),
         "ncamd_refcount": (ctypes.c_int, [Handle]),

@@ -125,6 +125,10 @@ class Scatter(Process):
     def dev_sampleScatterIsotropic(self, d_ekin, n, d_oe, d_omu, stream=0, repeat=1):
         C.call("ncamd_dev_samplescatterisotropic_many", self._h, d_ekin, n, repeat, d_oe, d_omu, stream)
 
+    def dev_sampleScatter(self, d_ekin, d_dirs, n, d_oe, d_od, stream=0):
+        """per-neutron directions in and out (3 doubles per neutron)"""
+        C.call("ncamd_dev_samplescatter_many_dirs", self._h, d_ekin, d_dirs, n, d_oe, d_od, stream)
+
     def dev_check_errors(self, stream=0):
         return C.call("ncamd_dev_check_errors", self._h, stream)
 
@@ -394,6 +398,23 @@ class SABScatter(Scatter):
 
     def dev_sample_alphabeta_envelope(self, d_ekin, n, d_a, d_b, stream=0):
         return C.call("ncamd_dev_sab_sample_alphabeta_envelope", self._h, d_ekin, n, d_a, d_b, stream)
+
+
+def sab_std_format(knltype, alpha, beta, sab, beta_grid_optimised=False):
+    """Scattering-kernel data to the standard S(alpha,beta) table
+    (transformKernelToStdFormat, ncrystal_core/src/sab/NCSABUtils.cc:134-325):
+    knltype 0 S, 1 scaled S, 2 scaled symmetric S (beta grid from 0).
+    Returns dict(alpha, beta, sab[nbeta][nalpha], ntrimmed)."""
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    s = np.ascontiguousarray(sab, dtype=np.float64).reshape(b.size, a.size)
+    na, nb, nt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+    args = (int(knltype), int(bool(beta_grid_optimised)), a.size, b.size, C.dptr(a), C.dptr(b), C.dptr(s))
+    C.call("ncamd_sab_std_format", *args, ctypes.byref(na), ctypes.byref(nb), None, None, None, ctypes.byref(nt))
+    oa, ob, os_ = np.empty(na.value), np.empty(nb.value), np.empty((nb.value, na.value))
+    C.call("ncamd_sab_std_format", *args, ctypes.byref(na), ctypes.byref(nb), C.dptr(oa), C.dptr(ob), C.dptr(os_),
+           ctypes.byref(nt))
+    return dict(alpha=oa, beta=ob, sab=os_, ntrimmed=nt.value)
 
 
 def create_sabscatter(alpha, beta, sab, temperature, mass_amu, sigma_bound, suggested_emax=0.0, egrid=None):

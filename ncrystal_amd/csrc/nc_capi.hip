@@ -612,6 +612,123 @@ std::vector<double> geomspace(double start, double stop, unsigned num) {
   return v;
 }
 
+// transformKernelToStdFormat (ncrystal_core/src/sab/NCSABUtils.cc:134-325)
+// with the checks of validateScatKnlData (NCScatKnlData.cc:26-80): trim
+// all-zero edges (detail_trimZeroEdgesFromKernel, NCSABUtils.cc:30-123), expand
+// a symmetric table to negative beta (expandBetaAndSABToAllBetas,
+// NCSABUtils.cc:331-380), unscale S = S_scaled exp(-beta/2) (in log space
+// where exp(-beta/2) overflows), and thicken beta grids of fewer than
+// min_nbeta rows unless the grid is marked optimised.  Kernel types:
+// 0 S(alpha,beta), 1 scaled S, 2 scaled symmetric S (beta >= 0).
+struct SabStd {
+  std::vector<double> alpha, beta, sab;
+  int64_t ntrimmed = 0;
+};
+SabStd sab_std_format(int type, bool beta_optimised, int na, int nb, const double* a, const double* b,
+                      const double* s, int min_nbeta) {
+  if (type < 0 || type > 2) raise("BadInput", "kernel type must be 0 (S), 1 (scaled S) or 2 (scaled symmetric S)");
+  for (int pass = 0; pass < 2; ++pass) {
+    const int n = pass ? nb : na;
+    const double* g = pass ? b : a;
+    const char* lbl = pass ? "beta" : "alpha";
+    if (n < 5 || n > 65534)
+      raise("BadInput", std::string("invalid ") + lbl + " grid (must have 5 to 65534 entries)");
+    for (int i = 0; i < n; ++i)
+      if (!std::isfinite(g[i]) || (i && !(g[i] > g[i - 1])))
+        raise("BadInput", std::string("invalid ") + lbl + " grid (must consist of sorted, unique, regular numbers)");
+  }
+  if (!(a[0] > 0.0)) raise("BadInput", "non-positive entries in alpha grid");
+  if (type == 2 ? b[0] != 0.0 : !(b[0] < 0.0))
+    raise("BadInput", type == 2 ? "a symmetric table's beta grid must start with 0.0"
+                                : "beta grid must start with a negative entry");
+  for (size_t i = 0; i < (size_t)na * nb; ++i)
+    if (!(s[i] >= 0.0) || std::isinf(s[i])) raise("BadInput", "S values must be finite and non-negative");
+  SabStd t;
+  // trim all-zero edges: upper alpha columns, upper (beta > 0) and lower (beta < 0) rows
+  auto row_zero = [&](int j) {
+    for (int i = 0; i < na; ++i)
+      if (s[(size_t)j * na + i] != 0.0) return false;
+    return true;
+  };
+  auto col_zero = [&](int i) {
+    for (int j = 0; j < nb; ++j)
+      if (s[(size_t)j * na + i] != 0.0) return false;
+    return true;
+  };
+  int ta = 0, tbu = 0, tbl = 0;
+  for (int i = na - 1; i >= 0 && a[i] > 0.0 && col_zero(i); --i) ++ta;
+  for (int j = nb - 1; j >= 0 && b[j] > 0.0 && row_zero(j); --j) ++tbu;
+  for (int j = 0; j < nb && b[j] < 0.0 && row_zero(j); ++j) ++tbl;
+  if (ta >= na) ta = tbu = tbl = 0;
+  t.ntrimmed = ta + tbu + tbl;
+  const int na2 = na - ta;
+  t.alpha.assign(a, a + na2);
+  t.beta.assign(b + tbl, b + (nb - tbu));
+  for (int j = tbl; j < nb - tbu; ++j) t.sab.insert(t.sab.end(), s + (size_t)j * na, s + (size_t)j * na + na2);
+  const size_t nal = t.alpha.size();
+  if (type == 2) {
+    // S(alpha, -beta) = S(alpha, beta); the zero row appears once
+    const size_t nh = t.beta.size(), npos = nh - 1;
+    std::vector<double> fb, fs;
+    fb.reserve(2 * npos + 1);
+    for (size_t k = nh; k-- > 0;) fb.push_back(-t.beta[k]);
+    fb.back() = 0.0;
+    for (size_t k = 1; k < nh; ++k) fb.push_back(t.beta[k]);
+    fs.reserve((2 * npos + 1) * nal);
+    for (size_t k = 0; k < npos; ++k)
+      fs.insert(fs.end(), t.sab.begin() + (npos - k) * nal, t.sab.begin() + (npos - k + 1) * nal);
+    fs.insert(fs.end(), t.sab.begin(), t.sab.end());
+    t.beta.swap(fb);
+    t.sab.swap(fs);
+    type = 1;
+  }
+  if (type == 1) {
+    for (size_t j = 0; j < t.beta.size(); ++j) {
+      const double ea = -0.5 * t.beta[j];
+      double* row = &t.sab[j * nal];
+      if (ea < 700.0) {
+        const double f = std::exp(ea);
+        for (size_t i = 0; i < nal; ++i) row[i] *= f;
+      } else {
+        for (size_t i = 0; i < nal; ++i) {
+          if (row[i] == 0.0) continue;
+          const double c = ea + std::log(row[i]);
+          if (!(c < 700.0)) raise("BadInput", "unscaling S(alpha,beta) overflows (exp of " + std::to_string(c) + ")");
+          row[i] = std::exp(c);
+        }
+      }
+    }
+  }
+  const size_t nbt = t.beta.size();
+  if (!beta_optimised && nbt < (size_t)min_nbeta) {
+    const size_t nextra = (size_t)min_nbeta / nbt;
+    std::vector<double> fb, fs;
+    fb.reserve(1 + (nbt - 1) * (1 + nextra));
+    fs.reserve(fb.capacity() * nal);
+    for (size_t j = 0; j + 1 < nbt; ++j) {
+      const double b0 = t.beta[j], b1 = t.beta[j + 1];
+      const double* r0 = &t.sab[j * nal];
+      const double* r1 = &t.sab[(j + 1) * nal];
+      fb.push_back(b0);
+      fs.insert(fs.end(), r0, r0 + nal);
+      const double db = (b1 - b0) / (double)(nextra + 1);
+      for (size_t e = 0; e < nextra; ++e) {
+        const double bb = b0 + (double)(e + 1) * db;
+        fb.push_back(bb);
+        for (size_t i = 0; i < nal; ++i) {
+          const double k = (bb - b0) / (b1 - b0);
+          fs.push_back(r0[i] * (1.0 - k) + k * r1[i]);
+        }
+      }
+    }
+    fs.insert(fs.end(), t.sab.end() - nal, t.sab.end());
+    fb.push_back(t.beta.back());
+    t.beta.swap(fb);
+    t.sab.swap(fs);
+  }
+  return t;
+}
+
 Proc* create_sab(int na, int nb, const double* alpha, const double* beta, const double* sab, double T,
                  double mass, double sigma_bound, double suggested_emax, int n_egrid, const double* egrid_in) {
   ensure_device();
@@ -2255,14 +2372,14 @@ void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat,
       p->q_idx.alloc(need);
     }
     if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
+    int64_t* q = p->q_idx.as<int64_t>();
+    unsigned long long* qc = p->q_count.as<unsigned long long>();
     if (single->use_kb)
       launch_kb_sample_iso(single->sab, single->kb, ekin, n, repeat, p->seed, stream, oE, oMu,
-                           p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
-                           p->q_count.as<unsigned long long>(), single->kb_maxtries, st);
+                           p->d_err.as<unsigned long long>(), q, qc, single->kb_maxtries, st);
     else
       launch_sab_sample_iso(single->sab, ekin, n, repeat, p->seed, stream, oE, oMu,
-                            p->d_err.as<unsigned long long>(), p->q_idx.as<int64_t>(),
-                            p->q_count.as<unsigned long long>(), p != single ? 1 : 0, st);
+                            p->d_err.as<unsigned long long>(), q, qc, p != single ? 1 : 0, st);
   } else {
     comp_queues(p, n * repeat, st);
     launch_sample_iso(make_view(p), ekin, n, repeat, p->seed, stream, oE, oMu, p->d_err.as<unsigned long long>(),
@@ -3271,6 +3388,22 @@ void ncamd_dev_fill_loguniform(double* d, uint64_t n, double emin, double emax, 
   NCAMD_CATCH
 }
 
+
+int ncamd_sab_std_format(int knltype, int beta_grid_optimised, int nalpha, int nbeta, const double* alpha,
+                         const double* beta, const double* sab, int* out_nalpha, int* out_nbeta, double* out_alpha,
+                         double* out_beta, double* out_sab, int64_t* ntrimmed) {
+  NCAMD_TRY
+  const SabStd t = sab_std_format(knltype, beta_grid_optimised != 0, nalpha, nbeta, alpha, beta, sab, 500);
+  *out_nalpha = (int)t.alpha.size();
+  *out_nbeta = (int)t.beta.size();
+  if (ntrimmed) *ntrimmed = t.ntrimmed;
+  if (out_alpha) std::copy(t.alpha.begin(), t.alpha.end(), out_alpha);
+  if (out_beta) std::copy(t.beta.begin(), t.beta.end(), out_beta);
+  if (out_sab) std::copy(t.sab.begin(), t.sab.end(), out_sab);
+  return 0;
+  NCAMD_CATCH
+  return -1;
+}
 
 ncamd_process_t ncamd_clone(ncamd_process_t h, uint64_t seed) {
   NCAMD_TRY return wrap(clone_proc(extract(h), seed)); NCAMD_CATCH

@@ -1308,6 +1308,117 @@ int orc_pwdist_percentiles(int n, const double* x, const double* y, int m, const
   return rc ? -1 : 0;
 }
 
+/* transformKernelToStdFormat: ncrystal_core/src/sab/NCSABUtils.cc:134-325
+   (trim :30-123, symmetric expansion :331-380, unscaling :158-215,
+   thickening :262-310) with validateScatKnlData (NCScatKnlData.cc:26-80).
+   Output arrays NULL: sizes only.  Returns 0, or -1 for invalid input. */
+int orc_sab_std_format(int type, int beta_opt, int na, int nb, const double* a, const double* b,
+                       const double* s, int* out_na, int* out_nb, double* oa, double* ob, double* os,
+                       int64_t* ntrim) {
+  if (type < 0 || type > 2 || na < 5 || nb < 5 || na > 65534 || nb > 65534) return -1;
+  for (int i = 1; i < na; ++i) if (!(a[i] > a[i - 1])) return -1;
+  for (int i = 1; i < nb; ++i) if (!(b[i] > b[i - 1])) return -1;
+  if (!(a[0] > 0.0)) return -1;
+  if (type == 2 ? b[0] != 0.0 : !(b[0] < 0.0)) return -1;
+  for (size_t i = 0; i < (size_t)na * nb; ++i) if (!(s[i] >= 0.0) || isinf(s[i])) return -1;
+  /* trim */
+  int ta = 0, tbu = 0, tbl = 0;
+  for (int i = na - 1; i >= 0 && a[i] > 0.0; --i) {
+    int z = 1;
+    for (int j = 0; j < nb && z; ++j) z = s[(size_t)j * na + i] == 0.0;
+    if (!z) break;
+    ++ta;
+  }
+  for (int j = nb - 1; j >= 0 && b[j] > 0.0; --j) {
+    int z = 1;
+    for (int i = 0; i < na && z; ++i) z = s[(size_t)j * na + i] == 0.0;
+    if (!z) break;
+    ++tbu;
+  }
+  for (int j = 0; j < nb && b[j] < 0.0; ++j) {
+    int z = 1;
+    for (int i = 0; i < na && z; ++i) z = s[(size_t)j * na + i] == 0.0;
+    if (!z) break;
+    ++tbl;
+  }
+  if (ta >= na) ta = tbu = tbl = 0;
+  const int nal = na - ta, nb1 = nb - tbu - tbl;
+  /* working table after trimming and (if symmetric) mirroring */
+  const int nbs = type == 2 ? 2 * (nb1 - 1) + 1 : nb1;
+  double* wb = (double*)malloc(sizeof(double) * (size_t)nbs);
+  double* ws = (double*)malloc(sizeof(double) * (size_t)nbs * nal);
+  if (type == 2) {
+    const int npos = nb1 - 1;
+    for (int k = 0; k < nb1; ++k) wb[k] = -b[tbl + nb1 - 1 - k];
+    wb[npos] = 0.0;
+    for (int k = 1; k < nb1; ++k) wb[npos + k] = b[tbl + k];
+    for (int k = 0; k < npos; ++k)
+      memcpy(ws + (size_t)k * nal, s + (size_t)(tbl + npos - k) * na, sizeof(double) * nal);
+    for (int k = 0; k < nb1; ++k)
+      memcpy(ws + (size_t)(npos + k) * nal, s + (size_t)(tbl + k) * na, sizeof(double) * nal);
+  } else {
+    for (int k = 0; k < nb1; ++k) {
+      wb[k] = b[tbl + k];
+      memcpy(ws + (size_t)k * nal, s + (size_t)(tbl + k) * na, sizeof(double) * nal);
+    }
+  }
+  if (type >= 1) {
+    for (int j = 0; j < nbs; ++j) {
+      const double ea = -0.5 * wb[j];
+      double* row = ws + (size_t)j * nal;
+      if (ea < 700.0) {
+        const double f = exp(ea);
+        for (int i = 0; i < nal; ++i) row[i] *= f;
+      } else {
+        for (int i = 0; i < nal; ++i) {
+          if (row[i] == 0.0) continue;
+          const double c = ea + log(row[i]);
+          if (!(c < 700.0)) { free(wb); free(ws); return -1; }
+          row[i] = exp(c);
+        }
+      }
+    }
+  }
+  int nbo = nbs;
+  int nextra = 0;
+  if (!beta_opt && nbs < 500) {
+    nextra = 500 / nbs;
+    nbo = 1 + (nbs - 1) * (1 + nextra);
+  }
+  *out_na = nal;
+  *out_nb = nbo;
+  if (ntrim) *ntrim = ta + tbu + tbl;
+  if (oa) memcpy(oa, a, sizeof(double) * nal);
+  if (ob && os) {
+    if (!nextra) {
+      memcpy(ob, wb, sizeof(double) * nbs);
+      memcpy(os, ws, sizeof(double) * (size_t)nbs * nal);
+    } else {
+      int r = 0;
+      for (int j = 0; j + 1 < nbs; ++j) {
+        const double b0 = wb[j], b1 = wb[j + 1];
+        ob[r] = b0;
+        memcpy(os + (size_t)r * nal, ws + (size_t)j * nal, sizeof(double) * nal);
+        ++r;
+        const double db = (b1 - b0) / (double)(nextra + 1);
+        for (int e = 0; e < nextra; ++e) {
+          const double bb = b0 + (double)(e + 1) * db;
+          ob[r] = bb;
+          for (int i = 0; i < nal; ++i) {
+            const double k = (bb - b0) / (b1 - b0);
+            os[(size_t)r * nal + i] = ws[(size_t)j * nal + i] * (1.0 - k) + k * ws[(size_t)(j + 1) * nal + i];
+          }
+          ++r;
+        }
+      }
+      ob[r] = wb[nbs - 1];
+      memcpy(os + (size_t)r * nal, ws + (size_t)(nbs - 1) * nal, sizeof(double) * nal);
+    }
+  }
+  free(wb); free(ws);
+  return 0;
+}
+
 /* Planted procedure changes for the tests that show the Alg1 pins have the
    power to see them (test infrastructure; 0 = the reference's procedure):
    1 = no first-bin rule (no 4/3 widening of the first beta bin, and beta in

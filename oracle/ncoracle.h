@@ -150,6 +150,11 @@ void orc_sab_sample_alpha(const orc_proc* p, int ie, int ibeta, int m, const dou
    oa = -1 where the table at Emax must draw; returns the error count */
 int64_t orc_sab_sample_highe_many(const orc_proc* p, uint64_t seed, uint32_t stream, const double* ekin,
                                   int64_t n, double* oa, double* ob);
+/* transformKernelToStdFormat (NCSABUtils.cc:134-325): kernel type 0 S,
+   1 scaled S, 2 scaled symmetric S; NULL outputs give the sizes only */
+int orc_sab_std_format(int type, int beta_opt, int na, int nb, const double* a, const double* b,
+                       const double* s, int* out_na, int* out_nb, double* oa, double* ob, double* os,
+                       int64_t* ntrim);
 /* PointwiseDist percentiles (NCPointwiseDist.cc:32-107) of the piecewise
    linear density through (x, y) at the m values p; 0 or -1 (bad input). */
 int orc_pwdist_percentiles(int n, const double* x, const double* y, int m, const double* p, double* out,

@@ -126,6 +126,10 @@ def lib():
         L.orc_sab_sample_alpha.restype = None
         L.orc_sab_sample_highe_many.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint32, _dp, ctypes.c_int64, _dp, _dp]
         L.orc_sab_sample_highe_many.restype = ctypes.c_int64
+        L.orc_sab_std_format.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp,
+                                         ctypes.POINTER(ctypes.c_int64)]
+        L.orc_sab_std_format.restype = ctypes.c_int
         L.orc_sample_dir_many.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _dp, _dp,
                                           ctypes.c_int64, _dp, _dp, ctypes.c_int]
         L.orc_sample_dir_many.restype = ctypes.c_int64
@@ -271,6 +275,20 @@ def pwdist_percentiles(x, y, p):
     if rc:
         raise ValueError("invalid pointwise distribution")
     return out, idx
+
+
+def sab_std_format(knltype, alpha, beta, sab, beta_grid_optimised=False):
+    """transformKernelToStdFormat: dict(alpha, beta, sab[nb][na], ntrimmed)"""
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    s = np.ascontiguousarray(sab, dtype=np.float64).reshape(b.size, a.size)
+    na, nb, nt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+    args = (int(knltype), int(bool(beta_grid_optimised)), a.size, b.size, _d(a), _d(b), _d(s))
+    if lib().orc_sab_std_format(*args, ctypes.byref(na), ctypes.byref(nb), None, None, None, ctypes.byref(nt)):
+        raise ValueError("invalid scattering kernel")
+    oa, ob, os_ = np.empty(na.value), np.empty(nb.value), np.empty((nb.value, na.value))
+    lib().orc_sab_std_format(*args, ctypes.byref(na), ctypes.byref(nb), _d(oa), _d(ob), _d(os_), ctypes.byref(nt))
+    return dict(alpha=oa, beta=ob, sab=os_, ntrimmed=nt.value)
 
 
 def tailed_breakdown(alpha, sab, logsab, cumul, alow, aupp, ilo, iup):

@@ -146,7 +146,8 @@ def test_isotropic_material_through_dropin(nc):
     ang, ang_p = _arr(2 * e.size)
     de, de_p = _arr(2 * e.size)
     lib.ncrystal_genscatter_nonoriented_many(s, e.ctypes.data_as(_dblp), e.size, 2, ang_p, de_p)
-    assert np.array_equal(ang, np.arccos(mu)) and np.array_equal(de, ef - np.tile(e, 2))
+    # acos of the C library and numpy's may differ in the last bit
+    assert np.max(np.abs(ang - np.arccos(mu))) <= 8e-16 and np.array_equal(de, ef - np.tile(e, 2))
     # one energy and direction, repeated: unit directions
     n = 4000
     oe, oe_p = _arr(n)
