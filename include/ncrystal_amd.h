@@ -370,6 +370,18 @@ void ncamd_dev_samplescatterisotropic_many(ncamd_process_t, const double* d_ekin
                                            unsigned long n_ekin, unsigned long repeat,
                                            double* d_results_ekin, double* d_results_mu,
                                            void* stream);
+/* Allocation-free form of ncamd_dev_samplescatterisotropic_many (same
+ * arguments, results and RNG stream use).  No counterpart in NCrystal: it is
+ * for callers that cannot allocate device memory during a call (HIP graph
+ * capture, tight serving loops).  Its queues and counters live in a fixed
+ * per-GPU scratch area of the library (32 MiB + 256 B, held until the
+ * process ends), so a call takes at most 2097152 neutrons (n x repeat;
+ * larger calls raise BadInput), and calls that overlap on one GPU (on
+ * different streams or host threads) race on that area. */
+void ncamd_dev_samplescatterisotropic_many_noalloc(ncamd_process_t, const double* d_ekin,
+                                                   unsigned long n_ekin, unsigned long repeat,
+                                                   double* d_results_ekin, double* d_results_mu,
+                                                   void* stream);
 void ncamd_dev_samplescatter_many_dirs(ncamd_process_t, const double* d_ekin,
                                        const double* d_dirs, unsigned long n,
                                        double* d_results_ekin, double* d_results_dirs,

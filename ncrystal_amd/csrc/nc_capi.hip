@@ -2351,8 +2351,10 @@ void sample_dirs_dispatch(Proc* p, const double* ekin, const double* dirs, int64
                      p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), st);
 }
 
+// ext_q / ext_qc: caller-provided queue (2 x n x repeat entries) and two
+// counters; then no device memory is allocated here
 void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat, double* oE, double* oMu,
-                         hipStream_t st) {
+                         hipStream_t st, int64_t* ext_q = nullptr, unsigned long long* ext_qc = nullptr) {
   if (p->oriented_comp) raise("BadInput", "oriented process: use the calls with neutron directions");
   if (is_absorption(p)) raise("BadInput", "an absorption process does not scatter");
   if (p->kind == kKindFG) {
@@ -2366,14 +2368,16 @@ void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat,
       p->scales[0] == 1.0)
     single = p->comps[0];
   if (single) {
-    const size_t need = sizeof(int64_t) * (size_t)std::max<int64_t>(n * repeat, 1);
-    if (p->q_idx.bytes < need) {
-      NCAMD_HIP(hipStreamSynchronize(st));  // the old queue may still be in use on this stream
-      p->q_idx.alloc(need);
+    if (!ext_q) {
+      const size_t need = sizeof(int64_t) * (size_t)std::max<int64_t>(n * repeat, 1);
+      if (p->q_idx.bytes < need) {
+        NCAMD_HIP(hipStreamSynchronize(st));  // the old queue may still be in use on this stream
+        p->q_idx.alloc(need);
+      }
+      if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
     }
-    if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
-    int64_t* q = p->q_idx.as<int64_t>();
-    unsigned long long* qc = p->q_count.as<unsigned long long>();
+    int64_t* q = ext_q ? ext_q : p->q_idx.as<int64_t>();
+    unsigned long long* qc = ext_q ? ext_qc : p->q_count.as<unsigned long long>();
     if (single->use_kb)
       launch_kb_sample_iso(single->sab, single->kb, ekin, n, repeat, p->seed, stream, oE, oMu,
                            p->d_err.as<unsigned long long>(), q, qc, single->kb_maxtries, st);
@@ -2381,10 +2385,41 @@ void sample_iso_dispatch(Proc* p, const double* ekin, int64_t n, int64_t repeat,
       launch_sab_sample_iso(single->sab, ekin, n, repeat, p->seed, stream, oE, oMu,
                             p->d_err.as<unsigned long long>(), q, qc, p != single ? 1 : 0, st);
   } else {
-    comp_queues(p, n * repeat, st);
+    if (!ext_q) comp_queues(p, n * repeat, st);
     launch_sample_iso(make_view(p), ekin, n, repeat, p->seed, stream, oE, oMu, p->d_err.as<unsigned long long>(),
-                      p->q_idx.as<int64_t>(), p->q_count.as<unsigned long long>(), st);
+                      ext_q ? ext_q : p->q_idx.as<int64_t>(),
+                      ext_q ? ext_qc : p->q_count.as<unsigned long long>(), st);
   }
+}
+
+// ------------------------------------------ allocation-free sampling call ---
+// Fixed scratch for ncamd_dev_samplescatterisotropic_many_noalloc: two
+// counters (in the first 256 bytes) and the sampling queues (two int64
+// entries per neutron) for at most kNoAllocMaxNeutrons neutrons per call,
+// 32 MiB + 256 B, held by every GPU the library runs on until the process
+// ends.  Host code uses the address from hipGetSymbolAddress, never the
+// array's name.
+constexpr int64_t kNoAllocMaxNeutrons = int64_t(1) << 21;
+constexpr size_t kNoAllocCounterBytes = 256;
+constexpr size_t kNoAllocBytes = kNoAllocCounterBytes + 2 * sizeof(int64_t) * (size_t)kNoAllocMaxNeutrons;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// the device address of `scratch` on the current device, looked up once per
+// device (with that device current)
+unsigned char* noalloc_scratch() {
+  int dev = 0;
+  NCAMD_HIP(hipGetDevice(&dev));
+  static std::mutex mtx;
+  static std::map<int, unsigned char*> ptrs;
+  std::lock_guard<std::mutex> lk(mtx);
+  auto it = ptrs.find(dev);
+  if (it != ptrs.end()) return it->second;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    raise("HIPError", std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  ptrs[dev] = static_cast<unsigned char*>(ptr);
+  return ptrs[dev];
 }
 
 // work buffers of the SCBragg kernels (batch ordering, Romberg queue), sized
@@ -2916,6 +2951,22 @@ void ncamd_dev_samplescatterisotropic_many(ncamd_process_t h, const double* d_ek
   NCAMD_TRY
   Proc* p = extract(h);
   sample_iso_dispatch(p, d_ekin, (int64_t)n, (int64_t)repeat, d_oE, d_oMu, as_stream(stream));
+  NCAMD_HIP(hipGetLastError());
+  NCAMD_CATCH
+}
+void ncamd_dev_samplescatterisotropic_many_noalloc(ncamd_process_t h, const double* d_ekin, unsigned long n,
+                                                   unsigned long repeat, double* d_oE, double* d_oMu,
+                                                   void* stream) {
+  NCAMD_TRY
+  Proc* p = extract(h);
+  ensure_device();
+  if (repeat && n > (unsigned long)kNoAllocMaxNeutrons / repeat)
+    raise("BadInput", "the allocation-free sampling call takes at most " + std::to_string(kNoAllocMaxNeutrons) +
+                          " neutrons (n x repeat) per call");
+  unsigned char* base = noalloc_scratch();
+  sample_iso_dispatch(p, d_ekin, (int64_t)n, (int64_t)repeat, d_oE, d_oMu, as_stream(stream),
+                      reinterpret_cast<int64_t*>(base + kNoAllocCounterBytes),
+                      reinterpret_cast<unsigned long long*>(base));
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH
 }
