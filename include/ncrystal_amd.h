@@ -125,8 +125,9 @@ double ncamd_debye_msd(double debye_temp, double temperature, double mass_amu);
  * temperature [K], target mass [amu], free scattering cross section [barn]
  * (NCrystal's SigmaBound::free: sigma_bound * (A/(1+A))^2).  On its own or
  * as a component of a composition (gas mixtures: one FreeGas process per
- * element, with PowderBragg and ElInc components at most); not in
- * many-material batches.  Sampling
+ * element; compounds: beside S(alpha,beta) components, one per element with
+ * free-gas dynamics, NCStdScatFact.cc:256-300); not in many-material
+ * batches.  Sampling
  * refuses energies that are not finite and >= 0 (poisoned outputs, counted
  * as errors). */
 ncamd_process_t ncamd_create_freegas(double temperature, double mass_amu, double sigma_free);
@@ -142,8 +143,9 @@ ncamd_process_t ncamd_create_absoov(double sigma_abs);
 /* PowderBragg from (d-spacing, |F|^2*multiplicity) pairs; replaces
  * PowderBragg(double v0_times_natoms, VectDFM&&)
  * (ncrystal_core/src/powderbragg/NCPowderBragg.cc:117-120,68-110).
- * fp32_tables != 0 additionally builds FP32 Kahan-compensated cumulative sums
- * used by ncamd_crosssection_nonoriented_many_fp32. */
+ * fp32_tables != 0 additionally builds FP32 cumulative sums (compensated:
+ * within one float ulp of the FP64 ones), used by
+ * ncamd_crosssection_nonoriented_many_fp32. */
 ncamd_process_t ncamd_create_powderbragg(double v0_times_natoms, const double* dspacings,
                                          const double* fsq_x_mult, int64_t n, int fp32_tables);
 
@@ -187,16 +189,18 @@ int ncamd_sab_std_format(int knltype, int beta_grid_optimised, int nalpha, int n
  * are merged into one process at scale 1 (createMerged:
  * ncrystal_core/src/powderbragg/NCPowderBragg.cc:218-331,
  * ncrystal_core/src/phys_utils/NCElIncXS.cc:204-233), so a multi-phase
- * material is the composition of its phases.  After merging: at most 4
- * components and at most one S(alpha,beta) and one SCBragg process; order
- * is kept (it fixes the component pick).  With
+ * material is the composition of its phases.  After merging: at most 8
+ * components, at most one SCBragg process, and at most 4 S(alpha,beta)
+ * components and FreeGas components beside them (compounds carry one per
+ * element, NCStdScatFact.cc:256-300; each is sampled by its own kernel after
+ * the pick, with a queue of n entries); order is kept (it fixes the
+ * component pick).  With
  * an SCBragg component the composition is oriented (as NCrystal's
  * single-crystal materials: SCBragg for d >= sccutoff, PowderBragg below it,
  * ElInc, S(alpha,beta)): only the calls with neutron directions apply, the
  * isotropic components scatter through randDirectionGivenScatterMu
  * (NCProcImpl.cc:29-37), and the sampling call waits for the number of
- * neutrons that pick the SCBragg component before sampling them.  FreeGas
- * components combine with PowderBragg and ElInc components only; a
+ * neutrons that pick the SCBragg component before sampling them.  A
  * composition holds either scattering or absorption processes. */
 ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, const double* scales);
 

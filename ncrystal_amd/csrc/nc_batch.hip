@@ -74,8 +74,9 @@ __global__ void __launch_bounds__(kBatchBlock) k_batch_pick(const BatchMat* __re
         Rng rng;
         rng.init(r.seed, (uint32_t)r.stream, (uint64_t)k);
         double Ef, mu;
-        const int rc = comp_pick_sample(m.view, rng, ekin[m.offset + k], Ef, mu);
-        if (rc == kCompToSAB) {
+        int slot = 0;  // a batch material has at most one staged component, its S(alpha,beta) one
+        const int rc = comp_pick_sample<false>(m.view, rng, ekin[m.offset + k], Ef, mu, slot);
+        if (rc == kCompToStage) {
           to_sab = true;
         } else {
           if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
@@ -119,7 +120,7 @@ __global__ void __launch_bounds__(kBatchBlock) k_batch_sab(const BatchMat* __res
     if (cnt == 0) continue;
     const BatchRng r = rngs[tile.mat];
     const int skip = m.view.n > 1 ? 1 : 0;
-    const double emax = m.view.sab.egrid[m.view.sab.ne - 1];
+    const double emax = m.view.sab[0].egrid[m.view.sab[0].ne - 1];
     for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
       const int64_t k = sab[m.offset + tile.begin + j];
       const double E = ekin[m.offset + k];
@@ -131,7 +132,7 @@ __global__ void __launch_bounds__(kBatchBlock) k_batch_sab(const BatchMat* __res
       rng.init(r.seed, (uint32_t)r.stream, (uint64_t)k);
       if (skip) rng.gen();
       double Ef, mu;
-      const int rc = sab_sample_iso<false>(m.view.sab, rng, E, Ef, mu);
+      const int rc = sab_sample_iso<false>(m.view.sab[0], rng, E, Ef, mu);
       if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
       o1[m.offset + k] = Ef;
       o2[m.offset + k] = mu;
@@ -159,7 +160,7 @@ __global__ void __launch_bounds__(kBatchBlock, 2) k_batch_highe(const BatchMat* 
     rng.init(r.seed, (uint32_t)r.stream, (uint64_t)k);
     if (m.view.n > 1) rng.gen();
     double Ef, mu;
-    const int rc = sab_sample_iso<true>(m.view.sab, rng, ekin[m.offset + k], Ef, mu);
+    const int rc = sab_sample_iso<true>(m.view.sab[0], rng, ekin[m.offset + k], Ef, mu);
     if (rc) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
     o1[m.offset + k] = Ef;
     o2[m.offset + k] = mu;

@@ -312,7 +312,7 @@ CompView make_view(Proc* p) {
         if (c->pb.n > 0) dlo = c->pb.threshold; else { dlo = __builtin_huge_val(); dhi = __builtin_huge_val(); }
         break;
       case kKindElInc: v.ei = c->ei; break;
-      case kKindSAB: v.sab = c->sab; break;
+      case kKindSAB: break;  // staged below
       case kKindSCB:
         if (p->kind != kKindNone) raise("BadInput", "oriented process: use the calls with neutron directions");
         v.has_scb = 1;
@@ -333,21 +333,77 @@ CompView make_view(Proc* p) {
   v.use_domain = p->kind == kKindNone ? 1 : 0;
   v.domain_lo = lo;
   v.domain_hi = hi;
+  // staged components: every S(alpha,beta) component, and FreeGas components
+  // beside S(alpha,beta) or SCBragg ones (gas mixtures without them sample
+  // free gas in place)
+  bool staged_kernels = false;
+  for (int i = 0; i < v.n; ++i) staged_kernels = staged_kernels || v.kind[i] == kKindSAB || v.kind[i] == kKindSCB;
+  v.nstaged = 0;
+  for (int i = 0; i < v.n; ++i) {
+    v.stage_of[i] = -1;
+    if (v.kind[i] == kKindSAB || (staged_kernels && v.kind[i] == kKindFG)) {
+      if (v.nstaged >= kMaxStaged)
+        raise("BadInput", "at most 4 S(alpha,beta) components (and FreeGas components beside them) per composition");
+      const int sl = v.nstaged++;
+      v.stage_comp[sl] = i;
+      v.stage_of[i] = sl;
+      if (v.kind[i] == kKindSAB) v.sab[sl] = list[i].first->sab;
+    }
+  }
   return v;
 }
 
 // ------------------------------------------------------------ PowderBragg ---
 // init: ncrystal_core/src/powderbragg/NCPowderBragg.cc:68-110 (+ the exact-d
 // merge of :122-145)
-__global__ void k_fp32_kahan_cumul(const double* __restrict__ contrib, int n, float* __restrict__ out) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  float s = 0.0f, c = 0.0f;
-  for (int i = 0; i < n; ++i) {
-    const float x = (float)contrib[i];
-    const float t = s + x;
-    c += fabsf(s) >= fabsf(x) ? (s - t) + x : (x - t) + s;
-    s = t;
-    out[i] = s + c;
+// FP32 cumulative table of the PowderBragg contributions: a compensated
+// prefix sum in float-float (hi, lo) pairs, whose hi + lo carries the FP64
+// sum to about 2^-44 relative, so each output is the FP64 cumulative rounded
+// to float (to within one ulp).  One workgroup of kCumulBlock lanes: each
+// lane sums a contiguous chunk, the chunk totals are scanned with wavefront
+// shuffles and across the waves through LDS, then each lane walks its chunk
+// again from its exclusive prefix.
+struct F2 {
+  float hi, lo;
+};
+// float-float addition: Knuth's TwoSum of the leading words, the trailing
+// words added to its error term, renormalised
+__device__ inline F2 f2_add(F2 a, F2 b) {
+  const float s = a.hi + b.hi;
+  const float bb = s - a.hi;
+  const float e = (a.hi - (s - bb)) + (b.hi - bb);
+  const float t = e + (a.lo + b.lo);
+  const float hi = s + t;
+  return F2{hi, t - (hi - s)};
+}
+__device__ inline F2 f2_of(double x) {
+  const float hi = (float)x;
+  return F2{hi, (float)(x - (double)hi)};
+}
+constexpr int kCumulBlock = 1024;
+__global__ void __launch_bounds__(kCumulBlock) k_fp32_cumul_scan(const double* __restrict__ contrib, int n,
+                                                                 float* __restrict__ out) {
+  __shared__ F2 s_wave[kCumulBlock / 64];
+  const int tid = (int)threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int chunk = (n + kCumulBlock - 1) / kCumulBlock;
+  const int b = tid * chunk, e = min(n, b + chunk);
+  F2 tot{0.0f, 0.0f};
+  for (int i = b; i < e; ++i) tot = f2_add(tot, f2_of(contrib[i]));
+  // inclusive scan of the chunk totals within the wavefront
+  F2 inc = tot;
+  for (int d = 1; d < 64; d <<= 1) {
+    const F2 up{__shfl_up(inc.hi, d, 64), __shfl_up(inc.lo, d, 64)};
+    if (lane >= d) inc = f2_add(up, inc);
+  }
+  if (lane == 63) s_wave[wid] = inc;
+  __syncthreads();
+  F2 pre{0.0f, 0.0f};
+  for (int w = 0; w < wid; ++w) pre = f2_add(pre, s_wave[w]);
+  const F2 left{__shfl_up(inc.hi, 1, 64), __shfl_up(inc.lo, 1, 64)};
+  if (lane > 0) pre = f2_add(pre, left);
+  for (int i = b; i < e; ++i) {
+    pre = f2_add(pre, f2_of(contrib[i]));
+    out[i] = pre.hi + pre.lo;
   }
 }
 
@@ -459,7 +515,7 @@ void pb_upload(Proc* p, const std::vector<double>& e2d, const std::vector<double
     bc->upload(contrib.data(), contrib.size());
     DevBuf* b32 = p->newbuf();
     b32->alloc(sizeof(float) * contrib.size());
-    hipLaunchKernelGGL(k_fp32_kahan_cumul, dim3(1), dim3(64), 0, 0, bc->as<double>(), (int)contrib.size(),
+    hipLaunchKernelGGL(k_fp32_cumul_scan, dim3(1), dim3(kCumulBlock), 0, 0, bc->as<double>(), (int)contrib.size(),
                        b32->as<float>());
     NCAMD_HIP(hipGetLastError());
     p->pb_fdm32_h.resize(contrib.size());
@@ -2155,13 +2211,17 @@ unsigned long long read_reset_err(Proc* p, hipStream_t st) {
 // main/high-energy kernels, everything else the composition kernel
 // two queues of up to `total` neutrons (and their counters) for the staged
 // composition kernels
+// queues of a staged composition: one per staged component and the
+// high-energy queue, total neutrons each; kMaxStaged + 1 counters
 void comp_queues(Proc* p, int64_t total, hipStream_t st) {
-  const size_t need = 2 * sizeof(int64_t) * (size_t)std::max<int64_t>(total, 1);
+  const int nst = make_view(p).nstaged;
+  const size_t need = (size_t)(nst + 1) * sizeof(int64_t) * (size_t)std::max<int64_t>(total, 1);
   if (p->q_idx.bytes < need) {
     NCAMD_HIP(hipStreamSynchronize(st));  // the old queue may still be in use on this stream
     p->q_idx.alloc(need);
   }
-  if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
+  if (p->q_count.bytes < (kMaxStaged + 1) * sizeof(unsigned long long))
+    p->q_count.alloc((kMaxStaged + 1) * sizeof(unsigned long long));
 }
 
 void check_fp32(Proc* p) {
@@ -2681,18 +2741,17 @@ ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, co
   p->kind = kKindNone;
   for (int i = 0; i < n; ++i) comp_add(p.get(), extract(procs[i]), scales[i]);
   if (p->comps.empty()) raise("BadInput", "composition has no non-null component");
-  if (p->comps.size() > (size_t)kMaxComp) raise("BadInput", "composition has more than 4 components after merging");
+  if (p->comps.size() > (size_t)kMaxComp) raise("BadInput", "composition has more than 8 components after merging");
   int seen[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (Proc* c : p->comps) {
     if (c->kind == kKindSCB) p->oriented_comp = true;  // oriented composition: calls with directions only
-    if (c->kind != kKindFG && seen[c->kind]) raise("BadInput", "at most one S(alpha,beta) and one SCBragg process per composition");
+    if (c->kind == kKindSCB && seen[c->kind]) raise("BadInput", "at most one SCBragg process per composition");
     seen[c->kind]++;
   }
   // one process type per composition (NCProcImpl.cc:269-272)
   if (seen[kKindAbs] && seen[kKindAbs] != (int)p->comps.size())
     raise("BadInput", "a composition holds either scattering or absorption processes");
-  if (seen[kKindFG] && (seen[kKindSAB] || seen[kKindSCB]))
-    raise("BadInput", "FreeGas components combine with PowderBragg and ElInc components only");
+  if (!seen[kKindAbs]) (void)make_view(p.get());  // checks the staged-component limit
   init_errbuf(p.get());
   return wrap(p.release());
   NCAMD_CATCH
@@ -2960,9 +3019,14 @@ void ncamd_dev_samplescatterisotropic_many_noalloc(ncamd_process_t h, const doub
   NCAMD_TRY
   Proc* p = extract(h);
   ensure_device();
-  if (repeat && n > (unsigned long)kNoAllocMaxNeutrons / repeat)
+  // queue entries the call needs: n x repeat per staged component plus the
+  // high-energy queue (two queues for a single S(alpha,beta) stage)
+  const uint64_t nq = is_absorption(p) || p->kind == kKindFG ? 2 : (uint64_t)make_view(p).nstaged + 1;
+  const uint64_t cap = 2 * (uint64_t)kNoAllocMaxNeutrons;
+  if (repeat && (n > cap / repeat || (uint64_t)n * repeat > cap / std::max<uint64_t>(nq, 2)))
     raise("BadInput", "the allocation-free sampling call takes at most " + std::to_string(kNoAllocMaxNeutrons) +
-                          " neutrons (n x repeat) per call");
+                          " neutrons (n x repeat) per call, proportionally fewer for compositions with more than "
+                          "one S(alpha,beta) or FreeGas stage");
   unsigned char* base = noalloc_scratch();
   sample_iso_dispatch(p, d_ekin, (int64_t)n, (int64_t)repeat, d_oE, d_oMu, as_stream(stream),
                       reinterpret_cast<int64_t*>(base + kNoAllocCounterBytes),
@@ -3029,6 +3093,8 @@ ncamd_batch_t ncamd_batch_create(int nmat, const ncamd_process_t* procs, const i
         if (c->kind == kKindFG || c->kind == kKindAbs)
           raise("BadInput", "compositions with FreeGas or absorption components are not supported in batches");
     const CompView v = make_view(p);  // raises for oriented processes
+    if (v.nstaged > 1)
+      raise("BadInput", "materials with more than one S(alpha,beta) component are not supported in batches");
     p->refcount++;
     b->mats.push_back(p);
     b->counts.push_back(counts[i]);

@@ -106,6 +106,24 @@ struct Rng {
 };
 
 // ---------------------------------------------------------------------------
+// Position of this lane's entry in a queue that the active lanes with `want`
+// append to: one atomic per wave (the first such lane adds their count) and
+// each lane's rank among them.  Every active lane of the wave must call it
+// with the same counter.  (Written out because the compiler does not merge
+// atomics whose address it sees as a per-lane value.)
+NCD unsigned long long wave_append(unsigned long long* counter, bool want) {
+  const unsigned long long m = __ballot(want);
+  if (!m) return 0ull;
+  const int lane = (int)__lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  unsigned long long base = 0ull;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader, 64);
+  const unsigned long long below = lane ? (m & (~0ull >> (64 - lane))) : 0ull;
+  return base + (unsigned long long)__popcll(below);
+}
+
+// ---------------------------------------------------------------------------
 // searches (std::upper_bound / std::lower_bound semantics)
 NCD int upper_bound_d(const double* __restrict__ v, int lo, int hi, double x) {
   int n = hi - lo;

@@ -8,7 +8,12 @@
 namespace ncamd {
 
 enum : int { kKindNone = 0, kKindPB = 1, kKindElInc = 2, kKindSAB = 3, kKindSCB = 4, kKindFG = 5, kKindAbs = 6 };
-constexpr int kMaxComp = 4;
+constexpr int kMaxComp = 8;
+// components of a composition with an S(alpha,beta) part that are sampled
+// by their own kernels after the pick: every S(alpha,beta) component (one
+// per element of a compound, NCStdScatFact.cc:256-300) and FreeGas
+// components beside them
+constexpr int kMaxStaged = 4;
 
 // Device view of a process composition (passed by value as kernel argument).
 struct CompView {
@@ -22,7 +27,12 @@ struct CompView {
   double domain_lo, domain_hi;
   PBView pb;
   ElIncView ei;
-  SABView sab;
+  // staged components: slot s serves component stage_comp[s]; stage_of[i] is
+  // the slot of component i (-1: sampled in place by the pick)
+  int nstaged;
+  int stage_comp[kMaxStaged];
+  int stage_of[kMaxComp];
+  SABView sab[kMaxStaged];  // the S(alpha,beta) view of each S(alpha,beta) slot
   double scb_threshold;  // domain of an SCBragg component (oriented compositions)
   int has_scb;           // 1: oriented composition with an SCBragg component
   // FreeGas components (gas mixtures: one per element) and the 1/v
@@ -63,7 +73,7 @@ NCD double comp_xs(const CompView& c, double E, double* commul, double ext = 0.0
     switch (c.kind[i]) {
       case kKindPB: xs = pb_active(c.pb, E) ? pb_xs_sel<FP32PB>(c.pb, E) : 0.0; break;
       case kKindElInc: xs = elinc_xs(c.ei, E, nullptr); break;
-      case kKindSAB: xs = sab_xs(c.sab, E); break;
+      case kKindSAB: xs = sab_xs(c.sab[c.stage_of[i]], E); break;
       case kKindSCB: xs = E >= c.scb_threshold ? ext : 0.0; break;
       case kKindFG:
         if constexpr (Gas) xs = fg_xs(c.fg[i], E);
@@ -114,17 +124,22 @@ NCD int comp_sample_iso(const CompView& c, Rng& rng, double E, double& Ef, doubl
 
 // First stage of a composition with an S(alpha,beta) part: component cross
 // sections, the pick, and PowderBragg / ElInc sampling in place.  Returns
-// kCompToSAB when the S(alpha,beta) component is picked; its sampling then
-// continues the neutron's stream after the pick, and the S(alpha,beta) stage
+// kCompToStage with the component's slot when a staged component
+// (S(alpha,beta), or FreeGas beside it) is picked; its sampling then
+// continues the neutron's stream after the pick, and the stage kernel
 // replays the pick's draw (one draw when there are several components).
 // kCompToSCB likewise for the SCBragg component of an oriented composition
 // (its cross section for the neutron given as ext).
-constexpr int kCompToSAB = 1;
+// Gas: the composition holds FreeGas components (their cross sections enter
+// the pick).
+constexpr int kCompToStage = 1;
 constexpr int kCompToSCB = 2;
-NCD int comp_pick_sample(const CompView& c, Rng& rng, double E, double& Ef, double& mu, double ext = 0.0) {
+template <bool Gas>
+NCD int comp_pick_sample(const CompView& c, Rng& rng, double E, double& Ef, double& mu, int& slot,
+                         double ext = 0.0) {
   if (comp_outside(c, E)) { Ef = E; mu = 1.0; return kOK; }
   double commul[kMaxComp];
-  const double tot = comp_xs<false>(c, E, commul, ext);
+  const double tot = comp_xs<false, Gas>(c, E, commul, ext);
   // No component scatters this neutron (an SCBragg component off every
   // reflection, the others outside their domains): the reference
   // (NCProcImpl.cc:363-377) still picks the last component, whose sampling
@@ -134,7 +149,10 @@ NCD int comp_pick_sample(const CompView& c, Rng& rng, double E, double& Ef, doub
   switch (c.kind[ic]) {
     case kKindPB: Ef = E; mu = pb_sample_mu(c.pb, rng, E); return kOK;
     case kKindElInc: Ef = E; mu = elinc_sample_mu(c.ei, rng, E); return kOK;
-    case kKindSAB: return kCompToSAB;
+    case kKindSAB: slot = c.stage_of[ic]; return kCompToStage;
+    case kKindFG:
+      if constexpr (Gas) { slot = c.stage_of[ic]; return kCompToStage; }
+      return kErrBadSampler;
     case kKindSCB: return kCompToSCB;
     default: return kErrBadSampler;
   }
@@ -142,18 +160,24 @@ NCD int comp_pick_sample(const CompView& c, Rng& rng, double E, double& Ef, doub
 
 NCHD int comp_variant(const CompView& c) {
   if (c.n == 1 && c.kind[0] == kKindPB) return kCompPBOnly;
-  for (int i = 0; i < c.n; ++i)  // never with S(alpha,beta) or SCBragg (refused at creation)
-    if (c.kind[i] == kKindFG || c.kind[i] == kKindAbs) return kCompFG;
   for (int i = 0; i < c.n; ++i)
     if (c.kind[i] == kKindSAB || c.kind[i] == kKindSCB) return kCompGeneral;
+  for (int i = 0; i < c.n; ++i)
+    if (c.kind[i] == kKindFG || c.kind[i] == kKindAbs) return kCompFG;
   return kCompNoSAB;
+}
+NCHD bool comp_has_gas(const CompView& c) {
+  for (int i = 0; i < c.n; ++i)
+    if (c.kind[i] == kKindFG || c.kind[i] == kKindAbs) return true;
+  return false;
 }
 
 // per-neutron kernels (nc_kernels.hip)
 void launch_xs_iso(const CompView&, const double* ekin, int64_t n, int64_t repeat, double* out, bool fp32,
                    hipStream_t);
-// queues: [2 x n*repeat] neutron indices, qcount: [2] counters (used by
-// compositions with an S(alpha,beta) part)
+// queues: [(nstaged + 1) x n*repeat] neutron indices (one queue per staged
+// component and the high-energy queue), qcount: [kMaxStaged + 1] counters
+// (used by compositions with an S(alpha,beta) part)
 void launch_sample_iso(const CompView&, const double* ekin, int64_t n, int64_t repeat, uint64_t seed,
                        uint32_t stream, double* oE, double* oMu, unsigned long long* nerr, int64_t* queues,
                        unsigned long long* qcount, hipStream_t);

@@ -21,7 +21,7 @@ __global__ void __launch_bounds__(256) k_xs_iso(CompView c, const double* __rest
     if constexpr (Variant == kCompPBOnly) {
       xs = comp_outside(c, E) ? 0.0 : 0.0 + c.scale[0] * (pb_active(c.pb, E) ? pb_xs_sel<FP32PB>(c.pb, E) : 0.0);
     } else {
-      xs = comp_xs<FP32PB, Variant == kCompFG>(c, E, nullptr);
+      xs = comp_xs<FP32PB, Variant == kCompFG>(c, E, nullptr);  // kCompFG: any composition with gas components
     }
     for (int64_t r = 0; r < repeat; ++r) out[r * n + i] = xs;
   }
@@ -133,7 +133,7 @@ NCD void comp_store(int64_t k, int rc, double Ef, double mu, Rng& rng, const dou
   }
 }
 
-template <bool Dirs>
+template <bool Dirs, bool Gas>
 __global__ void __launch_bounds__(256) k_comp_pick(CompView c, const double* __restrict__ ekin,
                                                    const double* __restrict__ dirs, int64_t n, int64_t total,
                                                    uint64_t seed, uint32_t stream, double* __restrict__ oE,
@@ -147,16 +147,22 @@ __global__ void __launch_bounds__(256) k_comp_pick(CompView c, const double* __r
     Rng rng;
     rng.init(seed, stream, (uint64_t)k);
     double Ef, mu;
-    const int rc = comp_pick_sample(c, rng, ekin[i], Ef, mu, ext ? ext[k] : 0.0);
-    if (rc == kCompToSAB) {
-      sq[atomicAdd(sqc, 1ull)] = k;
-      continue;
+    int slot = 0;
+    const int rc = comp_pick_sample<Gas>(c, rng, ekin[i], Ef, mu, slot, ext ? ext[k] : 0.0);
+    // queue appends with all active lanes converged: one queue per staged
+    // component, and the SCBragg queue
+    const bool staged = rc == kCompToStage;
+    for (int s = 0; s < c.nstaged; ++s) {
+      const bool want = staged && slot == s;
+      const unsigned long long pos = wave_append(&sqc[s], want);
+      if (want) sq[(int64_t)s * total + (int64_t)pos] = k;
     }
-    if (rc == kCompToSCB) {
-      cq[atomicAdd(cqc, 1ull)] = k;
-      continue;
+    const bool to_scb = rc == kCompToSCB;
+    if (cqc) {
+      const unsigned long long pos = wave_append(cqc, to_scb);
+      if (to_scb) cq[pos] = k;
     }
-    comp_store<Dirs>(k, rc, Ef, mu, rng, dirs, oE, o2, nerr);
+    if (!staged && !to_scb) comp_store<Dirs>(k, rc, Ef, mu, rng, dirs, oE, o2, nerr);
   }
 }
 
@@ -185,6 +191,29 @@ __global__ void __launch_bounds__(256) k_comp_sab(SABView s, const double* __res
     if (skip) rng.gen();
     double Ef, mu;
     const int rc = sab_sample_iso<HighE>(s, rng, E, Ef, mu);
+    comp_store<Dirs>(k, rc, Ef, mu, rng, dirs, oE, o2, nerr);
+  }
+}
+
+// stage kernel of a FreeGas component beside S(alpha,beta) components: the
+// queued neutrons restart their stream, replay the pick's draw and sample
+// the free-gas collision
+template <bool Dirs>
+__global__ void __launch_bounds__(256) k_comp_fg(FGView g, const double* __restrict__ ekin,
+                                                 const double* __restrict__ dirs, int64_t n, int64_t total,
+                                                 uint64_t seed, uint32_t stream, int skip, double* __restrict__ oE,
+                                                 double* __restrict__ o2, unsigned long long* __restrict__ nerr,
+                                                 const int64_t* __restrict__ inq,
+                                                 const unsigned long long* __restrict__ inqc) {
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t nq = (int64_t)*inqc;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gstride) {
+    const int64_t k = inq[q];
+    Rng rng;
+    rng.init(seed, stream, (uint64_t)k);
+    if (skip) rng.gen();
+    double Ef, mu;
+    const int rc = fg_sample_demu(g, rng, ekin[(total == n) ? k : (k % n)], Ef, mu);
     comp_store<Dirs>(k, rc, Ef, mu, rng, dirs, oE, o2, nerr);
   }
 }
@@ -293,7 +322,49 @@ __global__ void k_sab_row_guides(SABView s, unsigned short* __restrict__ guide, 
   guide[t] = rs > 0.0 ? (unsigned short)r : (unsigned short)0;
 }
 
+static inline unsigned grid_for(int64_t n, int bs) {
+  int64_t g = (n + bs - 1) / bs;
+  // enough waves for the 256 CUs, bounded so a grid-stride loop covers the rest
+  const int64_t gmax = 256 * 64;
+  if (g > gmax) g = gmax;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+__global__ void k_zero_u64_n(unsigned long long* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// the stage launches of a composition with an S(alpha,beta) part, after the
+// pick kernel filled one queue per staged component (queues + s * total,
+// counter qcount[s]); the high-energy queue after them (counter
+// qcount[kMaxStaged]) is reused by each S(alpha,beta) component in turn
+template <bool Dirs>
+void launch_stages(const CompView& c, const double* ekin, const double* dirs, int64_t n, int64_t total,
+                   uint64_t seed, uint32_t stream, double* oE, double* o2, unsigned long long* nerr,
+                   int64_t* queues, unsigned long long* qcount, hipStream_t st) {
+  const int skip = c.n > 1 ? 1 : 0;
+  int64_t* hq = queues + (int64_t)c.nstaged * total;
+  unsigned long long* hqc = qcount + kMaxStaged;
+  const dim3 g(grid_for(total, 256)), b(256);
+  for (int sl = 0; sl < c.nstaged; ++sl) {
+    const int comp = c.stage_comp[sl];
+    int64_t* q = queues + (int64_t)sl * total;
+    if (c.kind[comp] == kKindSAB) {
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, hqc);
+      hipLaunchKernelGGL((k_comp_sab<false, Dirs>), g, b, 0, st, c.sab[sl], ekin, dirs, n, total, seed, stream, skip,
+                         oE, o2, nerr, q, qcount + sl, hq, hqc);
+      hipLaunchKernelGGL((k_comp_sab<true, Dirs>), g, b, 0, st, c.sab[sl], ekin, dirs, n, total, seed, stream, skip,
+                         oE, o2, nerr, hq, hqc, nullptr, nullptr);
+    } else {
+      hipLaunchKernelGGL(k_comp_fg<Dirs>, g, b, 0, st, c.fg[comp], ekin, dirs, n, total, seed, stream, skip, oE, o2,
+                         nerr, q, qcount + sl);
+    }
+  }
+}
+
 
 // synthetic neutron energies, log-uniform, from a dedicated Philox stream
 __global__ void __launch_bounds__(256) k_fill_loguniform(double* __restrict__ out, int64_t n, double lemin,
@@ -307,14 +378,6 @@ __global__ void __launch_bounds__(256) k_fill_loguniform(double* __restrict__ ou
   }
 }
 
-static inline unsigned grid_for(int64_t n, int bs) {
-  int64_t g = (n + bs - 1) / bs;
-  // enough waves for the 256 CUs, bounded so a grid-stride loop covers the rest
-  const int64_t gmax = 256 * 64;
-  if (g > gmax) g = gmax;
-  if (g < 1) g = 1;
-  return (unsigned)g;
-}
 
 void launch_simple_xs(int kind, const FGView& g, double abs_c, const double* ekin, int64_t n, int64_t repeat,
                       double* out, hipStream_t st) {
@@ -344,7 +407,7 @@ void launch_xs_iso(const CompView& c, const double* ekin, int64_t n, int64_t rep
   if (v == kCompPBOnly) {
     if (fp32) hipLaunchKernelGGL((k_xs_iso<true, kCompPBOnly>), g, b, 0, st, c, ekin, n, repeat, out);
     else hipLaunchKernelGGL((k_xs_iso<false, kCompPBOnly>), g, b, 0, st, c, ekin, n, repeat, out);
-  } else if (v == kCompFG) {
+  } else if (v == kCompFG || comp_has_gas(c)) {
     if (fp32) hipLaunchKernelGGL((k_xs_iso<true, kCompFG>), g, b, 0, st, c, ekin, n, repeat, out);
     else hipLaunchKernelGGL((k_xs_iso<false, kCompFG>), g, b, 0, st, c, ekin, n, repeat, out);
   } else {
@@ -371,24 +434,17 @@ void launch_sample_iso(const CompView& c, const double* ekin, int64_t n, int64_t
                          seed, stream, oE, oMu, nerr);
       break;
     default: {
-      int sab = -1;
-      for (int j = 0; j < c.n; ++j)
-        if (c.kind[j] == kKindSAB) sab = j;
-      int64_t* sq = queues;
-      int64_t* hq = queues + total;
-      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount);
-      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount + 1);
-      hipLaunchKernelGGL(k_comp_pick<false>, dim3(grid_for(total, 256)), dim3(256), 0, st, c, ekin,
-                         (const double*)nullptr, n, total, seed, stream, oE, oMu, nerr, sq, qcount,
-                         (const double*)nullptr, (int64_t*)nullptr, (unsigned long long*)nullptr);
-      if (sab < 0) break;  // no S(alpha,beta) part: nothing queued
-      const int skip = c.n > 1 ? 1 : 0;
-      hipLaunchKernelGGL((k_comp_sab<false, false>), dim3(grid_for(total, 256)), dim3(256), 0, st, c.sab, ekin,
-                         (const double*)nullptr, n, total, seed, stream, skip, oE, oMu, nerr, sq, qcount, hq,
-                         qcount + 1);
-      hipLaunchKernelGGL((k_comp_sab<true, false>), dim3(grid_for(total, 256)), dim3(256), 0, st, c.sab, ekin,
-                         (const double*)nullptr, n, total, seed, stream, skip, oE, oMu, nerr, hq, qcount + 1,
-                         nullptr, nullptr);
+      hipLaunchKernelGGL(k_zero_u64_n, dim3(1), dim3(64), 0, st, qcount, kMaxStaged + 1);
+      const dim3 g(grid_for(total, 256)), b(256);
+      if (comp_has_gas(c))
+        hipLaunchKernelGGL((k_comp_pick<false, true>), g, b, 0, st, c, ekin, (const double*)nullptr, n, total, seed,
+                           stream, oE, oMu, nerr, queues, qcount, (const double*)nullptr, (int64_t*)nullptr,
+                           (unsigned long long*)nullptr);
+      else
+        hipLaunchKernelGGL((k_comp_pick<false, false>), g, b, 0, st, c, ekin, (const double*)nullptr, n, total,
+                           seed, stream, oE, oMu, nerr, queues, qcount, (const double*)nullptr, (int64_t*)nullptr,
+                           (unsigned long long*)nullptr);
+      launch_stages<false>(c, ekin, nullptr, n, total, seed, stream, oE, oMu, nerr, queues, qcount, st);
     }
   }
 }
@@ -396,7 +452,7 @@ __global__ void __launch_bounds__(256) k_xs_ext(CompView c, const double* __rest
                                                 const double* __restrict__ ext, int64_t n, double* __restrict__ out) {
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gstride)
-    out[k] = comp_xs<false>(c, ekin[k], nullptr, ext[k]);
+    out[k] = comp_xs<false, true>(c, ekin[k], nullptr, ext[k]);  // FreeGas components may sit beside SCBragg
 }
 void launch_xs_ext(const CompView& c, const double* ekin, const double* ext, int64_t n, double* out,
                    hipStream_t st) {
@@ -424,20 +480,15 @@ void launch_sample_dirs(const CompView& c, const double* ekin, const double* dir
                          seed, stream, oE, oD, nerr);
       break;
     default: {
-      int64_t* sq = queues;
-      int64_t* hq = queues + n;
-      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount);
-      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, qcount + 1);
-      hipLaunchKernelGGL(k_comp_pick<true>, dim3(grid_for(n, 256)), dim3(256), 0, st, c, ekin, dirs, n, n, seed,
-                         stream, oE, oD, nerr, sq, qcount, ext, cq, cqc);
-      bool has_sab = false;
-      for (int j = 0; j < c.n; ++j) has_sab = has_sab || c.kind[j] == kKindSAB;
-      if (!has_sab) break;
-      const int skip = c.n > 1 ? 1 : 0;
-      hipLaunchKernelGGL((k_comp_sab<false, true>), dim3(grid_for(n, 256)), dim3(256), 0, st, c.sab, ekin, dirs, n,
-                         n, seed, stream, skip, oE, oD, nerr, sq, qcount, hq, qcount + 1);
-      hipLaunchKernelGGL((k_comp_sab<true, true>), dim3(grid_for(n, 256)), dim3(256), 0, st, c.sab, ekin, dirs, n,
-                         n, seed, stream, skip, oE, oD, nerr, hq, qcount + 1, nullptr, nullptr);
+      hipLaunchKernelGGL(k_zero_u64_n, dim3(1), dim3(64), 0, st, qcount, kMaxStaged + 1);
+      const dim3 g(grid_for(n, 256)), b(256);
+      if (comp_has_gas(c))
+        hipLaunchKernelGGL((k_comp_pick<true, true>), g, b, 0, st, c, ekin, dirs, n, n, seed, stream, oE, oD, nerr,
+                           queues, qcount, ext, cq, cqc);
+      else
+        hipLaunchKernelGGL((k_comp_pick<true, false>), g, b, 0, st, c, ekin, dirs, n, n, seed, stream, oE, oD, nerr,
+                           queues, qcount, ext, cq, cqc);
+      launch_stages<true>(c, ekin, dirs, n, n, seed, stream, oE, oD, nerr, queues, qcount, st);
     }
   }
 }

@@ -35,7 +35,18 @@ def elinc_inputs(crystal, msd=None):
 
 
 class Material:
-    def __init__(self, crystal, sab=None, with_elinc=True, with_bragg=True, fp32_tables=True):
+    """Scatter of a one-phase material: ElInc, PowderBragg, then the inelastic
+    components.  sab: one S(alpha,beta) table (dict of alpha, beta, sab,
+    temperature, mass, sigma_bound, optional egrid) at scale 1.  dynamics:
+    the inelastic components per element instead, as the standard scatter
+    factory adds them (NCStdScatFact.cc:256-300): a list of (fraction, entry),
+    entry an S(alpha,beta) table dict or {"freegas": (mass_amu, sigma_free)}
+    at the crystal's temperature, each a component scaled by the element's
+    number fraction."""
+
+    def __init__(self, crystal, sab=None, with_elinc=True, with_bragg=True, fp32_tables=True, dynamics=None):
+        if sab is not None and dynamics is not None:
+            raise ValueError("give either sab or dynamics")
         self.crystal = crystal
         msd = species_msd(crystal)
         self.msd = msd
@@ -52,12 +63,23 @@ class Material:
             pb = core.create_powderbragg(v0n, dsp, fm, fp32_tables=fp32_tables)
             comps.append(pb)
             self.components["bragg"] = pb
+        scales = [1.0] * len(comps)
         if sab is not None:
-            sb = core.create_sabscatter(sab["alpha"], sab["beta"], sab["sab"], sab["temperature"], sab["mass"],
-                                        sab["sigma_bound"], egrid=sab.get("egrid"))
-            comps.append(sb)
-            self.components["inelas"] = sb
-        self.scatter = core.create_composition(comps, [1.0] * len(comps))
+            dynamics = [(1.0, sab)]
+        self.inelastic = []
+        for frac, d in dynamics or []:
+            if "freegas" in d:
+                mass, sigma = d["freegas"]
+                c = core.create_freegas(crystal.temperature, mass, sigma)
+            else:
+                c = core.create_sabscatter(d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"],
+                                           d["sigma_bound"], egrid=d.get("egrid"))
+            comps.append(c)
+            scales.append(float(frac))
+            self.inelastic.append(c)
+        if sab is not None:
+            self.components["inelas"] = self.inelastic[0]
+        self.scatter = core.create_composition(comps, scales)
 
 
 def _stable_sum(values):

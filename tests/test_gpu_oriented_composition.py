@@ -3,7 +3,8 @@ NCrystal builds single-crystal materials: SCBragg for d >= sccutoff,
 PowderBragg for the rest, ElInc, S(alpha,beta)) through the C ABI on the GPU
 against the CPU checker: cross sections with directions to 1e-12, sampled
 energies and directions on the same Philox streams, with and without an
-S(alpha,beta) part and with a single component; the calls without
+S(alpha,beta) part, with a FreeGas component beside them and with a single
+component; the calls without
 directions are refused."""
 import math
 
@@ -34,7 +35,7 @@ def _subset(fams, keep):
     return hkl[idx], mult[idx], dsp[idx], fsq[idx], noff, np.concatenate(parts).astype(np.int32)
 
 
-def _material(core, oracle, with_sab=True, only_scb=False):
+def _material(core, oracle, with_sab=True, only_scb=False, gas=False):
     cr = synth.fcc_powder(dcutoff=0.25)
     msd = oracle.debye_msd(410.0, 293.0, 26.98)
     sp = [(cr.species[0].csl, msd, cr.species[0].positions)]
@@ -58,6 +59,11 @@ def _material(core, oracle, with_sab=True, only_scb=False):
         args = (t["alpha"], t["beta"], t["sab"], t["temperature"], t["mass"], t["sigma_bound"])
         g.append(core.create_sabscatter(*args))
         o.append(oracle.sab(*args))
+    if gas:
+        # a FreeGas component beside SCBragg (and S(alpha,beta)): sampled by
+        # its own stage kernel after the pick
+        g.append(core.create_freegas(293.0, 4.0026, 0.7596))
+        o.append(oracle.freegas(293.0, 4.0026, 0.7596))
     sc = [1.0] * len(g)
     return core.create_composition(g, sc), oracle.composition(o, sc)
 
@@ -70,10 +76,11 @@ def _rel(a, b):
     return np.where(same, 0.0, r)
 
 
-@pytest.mark.parametrize("with_sab,only_scb", [(True, False), (False, False), (False, True)])
-def test_oriented_composition_matches_checker(amd, oracle, with_sab, only_scb):
+@pytest.mark.parametrize("with_sab,only_scb,gas", [(True, False, False), (False, False, False), (False, True, False),
+                                                   (True, False, True), (False, False, True)])
+def test_oriented_composition_matches_checker(amd, oracle, with_sab, only_scb, gas):
     C, core = amd
-    g, o = _material(core, oracle, with_sab, only_scb)
+    g, o = _material(core, oracle, with_sab, only_scb, gas)
     assert C.call("ncamd_isoriented", g._h) == 1
     n = 40000
     wl = synth.wavelengths_uniform(n, 0.5, 6.0, seed=4)

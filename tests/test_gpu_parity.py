@@ -111,6 +111,23 @@ def test_powderbragg_fp32(amd, oracle):
     assert np.array_equal(x32 > 0, nz)
 
 
+@pytest.mark.parametrize("nplanes", [1, 2, 63, 64, 65, 1023, 1024, 1025, 5000, 262151])
+def test_powderbragg_fp32_table(amd, oracle, nplanes):
+    """the FP32 cumulative table is the FP64 cumulative rounded to float, to
+    within one float ulp, at sizes around the scan's lane and wave counts and
+    with contributions spread over ten decades"""
+    core, _ = amd
+    rng = np.random.default_rng(nplanes)
+    d = np.sort(rng.uniform(0.1, 5.0, nplanes))[::-1].copy()
+    fm = np.exp(rng.uniform(np.log(1e-6), np.log(1e4), nplanes))
+    g = core.create_powderbragg(60.0, d, fm, fp32_tables=True)
+    e2d, fdm, f32 = core.powderbragg_tables(g)
+    assert f32.dtype == np.float32 and f32.size == fdm.size
+    ref = fdm.astype(np.float32)
+    ulp = np.spacing(np.abs(ref))
+    assert np.all(np.abs(f32.astype(np.float64) - fdm) <= ulp), np.max(np.abs(f32 - fdm) / ulp)
+
+
 def test_powderbragg_sampling(amd, oracle):
     core, _ = amd
     g, o = _fcc_pb(core, oracle)
@@ -126,7 +143,7 @@ def test_powderbragg_sampling(amd, oracle):
 def test_powderbragg_c2_ortho(amd, oracle):
     # orthorhombic 24-atom cell, dcutoff 0.1 Aa: ~2.6e5 hkl families, ~7.8e4
     # distinct planes; GPU families/tables vs the oracle, xs bit-exact (FP64)
-    # and within 1e-6 (FP32 Kahan tables), sampling identical per neutron
+    # and within 1e-6 (FP32 cumulative tables), sampling identical per neutron
     core, _ = amd
     cr = synth.ortho_random(seed=1, dcutoff=0.1)
     sp = [(s.csl, oracle.debye_msd(s.debye_temp, cr.temperature, s.mass), s.positions) for s in cr.species]

@@ -4,7 +4,9 @@ the C ABI: the same kernels on the same Philox streams, so outputs must be
 identical, on two calls in a row (the second smaller, with repeat), for an
 S(alpha,beta) process reaching its free-gas extender, the same process on its
 envelope sampler, a full composition (ElInc + PowderBragg + S(alpha,beta)),
-a PowderBragg process and a FreeGas process; oversized calls raise BadInput."""
+a PowderBragg process, a FreeGas process and a compound (two S(alpha,beta)
+components and a FreeGas one, four queues); oversized calls raise
+BadInput."""
 import numpy as np
 import pytest
 
@@ -35,7 +37,10 @@ def _processes(core, materials):
     m = materials.Material(synth.fcc_powder(), sab=t)
     pb = m.components["bragg"]
     fg = core.create_freegas(293.15, 4.0026, 0.7596)
-    return {"sab": sab, "envelope": env, "composition": m.scatter, "powderbragg": pb, "freegas": fg}
+    t2 = synth.small_sab_table(seed=5, mass_amu=15.999, amax=60.0)
+    cpd = materials.Material(synth.fcc_powder(), dynamics=[(0.4, t), (0.4, t2), (0.2, {"freegas": (4.0026, 0.7596)})])
+    return {"sab": sab, "envelope": env, "composition": m.scatter, "powderbragg": pb, "freegas": fg,
+            "compound": cpd.scatter}
 
 
 def _run(C, proc, fn, de, n, repeat, st):
