@@ -59,6 +59,18 @@ struct StableSum {
 // Counter-based RNG: Philox4x32-10.  Stream of a neutron = (key=seed,
 // counter=(block, stream, idx_lo, idx_hi)); draw 2j = low 64 bits of block j,
 // draw 2j+1 = high 64 bits.
+
+// One Philox4x32 round.  Each product is formed as one 32x32 -> 64-bit
+// multiply (v_mad_u64_u32), not as separate low and high halves.
+NCD void philox_round(uint32_t& x0, uint32_t& x1, uint32_t& x2, uint32_t& x3, uint32_t q0, uint32_t q1) {
+  const uint64_t p0 = (uint64_t)0xD2511F53u * x0;
+  const uint64_t p1 = (uint64_t)0xCD9E8D57u * x2;
+  x0 = (uint32_t)(p1 >> 32) ^ x1 ^ q0;
+  x1 = (uint32_t)p1;
+  x2 = (uint32_t)(p0 >> 32) ^ x3 ^ q1;
+  x3 = (uint32_t)p0;
+}
+
 struct Rng {
   uint32_t k0, k1, c0, c1, c2, c3;
   uint64_t b1;
@@ -80,9 +92,7 @@ struct Rng {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
       if (r) { q0 += 0x9E3779B9u; q1 += 0xBB67AE85u; }
-      const uint32_t lo0 = 0xD2511F53u * x0, hi0 = __umulhi(0xD2511F53u, x0);
-      const uint32_t lo1 = 0xCD9E8D57u * x2, hi1 = __umulhi(0xCD9E8D57u, x2);
-      x0 = hi1 ^ x1 ^ q0; x1 = lo1; x2 = hi0 ^ x3 ^ q1; x3 = lo0;
+      philox_round(x0, x1, x2, x3, q0, q1);
     }
     ++c0;
     nleft = 0;
@@ -94,9 +104,7 @@ struct Rng {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
       if (r) { q0 += 0x9E3779B9u; q1 += 0xBB67AE85u; }
-      const uint32_t lo0 = 0xD2511F53u * x0, hi0 = __umulhi(0xD2511F53u, x0);
-      const uint32_t lo1 = 0xCD9E8D57u * x2, hi1 = __umulhi(0xCD9E8D57u, x2);
-      x0 = hi1 ^ x1 ^ q0; x1 = lo1; x2 = hi0 ^ x3 ^ q1; x3 = lo0;
+      philox_round(x0, x1, x2, x3, q0, q1);
     }
     ++c0;
     b1 = ((uint64_t)x3 << 32) | x2;
@@ -168,7 +176,7 @@ NCD int pick_idx(Rng& rng, const double* commul, int n) {
 struct PBView {
   const double* e2d;    // [n] 2d-spacing thresholds in energy units, descending d
   const double* fdm;    // [n] cumulative d*|F|^2*mult*c (FP64)
-  const float* fdm32;   // [n] same, FP32 (Kahan-compensated build), may be null
+  const float* fdm32;   // [n] same, FP32 (compensated build), may be null
   int n;
   double threshold;     // e2d[0]
   // search guides (exact: every bracket contains the plain binary-search answer)

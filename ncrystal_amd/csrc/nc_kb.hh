@@ -114,30 +114,22 @@ NCD int kb_interval(const KBView& k, double E) {
   return n;
 }
 
-// per-neutron constants of the proposal loop
-struct KBNeutron {
-  const KBSlot* sl;
-  double eps, seps, wmax, jfac, ediv;
-};
-
-NCD void kb_neutron(const SABView& s, const KBView& k, int n, double E, KBNeutron& q) {
-  const double4 par = k.node_par[n];
-  q.eps = par.x; q.seps = par.y; q.wmax = par.z; q.jfac = par.w;
-  q.ediv = E / s.kT;
-  q.sl = k.slot + (size_t)n * k.nbox;
-}
-
-// One proposal; true if accepted, with (alpha, beta).
-NCD bool kb_propose(const SABView& s, const KBView& k, const KBNeutron& q, Rng& rng, double& a_out,
-                    double& b_out) {
-  uint32_t r0, r1, r2, r3;
-  rng.gen4(r0, r1, r2, r3);
+// One proposal for a neutron of energy E (ediv = E/kT) in interval iv,
+// from one Philox block (r0..r3); true if accepted, with (alpha, beta).  The
+// interval's constants are read here (a small cached table), so that a lane
+// carries only the neutron's index, energy, interval and draw counter
+// between proposals.
+NCD bool kb_propose(const SABView& s, const KBView& k, int iv, double ediv, uint32_t r0, uint32_t r1, uint32_t r2,
+                    uint32_t r3, double& a_out, double& b_out) {
+  const double4 par = k.node_par[iv];
+  const double eps = par.x, seps = par.y, wmax = par.z, jfac = par.w;
+  const KBSlot* sl = k.slot + (size_t)iv * k.nbox;
   // alias slot; the residual of the pick, rescaled, is the position in w
   const double u = Rng::to01(((uint64_t)r1 << 32) | r0) * k.nbox;
   int slot = (int)u;
   slot = slot >= k.nbox ? k.nbox - 1 : slot;
   const double f = u - (double)slot;
-  const KBSlot& rec = q.sl[slot];
+  const KBSlot& rec = sl[slot];
   const double prob = rec.prob;
   double uw;
   KBBox b;
@@ -146,23 +138,23 @@ NCD bool kb_propose(const SABView& s, const KBView& k, const KBNeutron& q, Rng& 
     b = rec.box;
   } else {
     uw = (f - prob) * rec.inv_q;
-    b = q.sl[rec.alias].box;
+    b = sl[rec.alias].box;
   }
   const double w0 = b.w0, w1 = b.w1, k0 = b.k0;
   const double w = w0 + (w1 - w0) * uw;
   const double kap = k0 + ((double)b.k1 - k0) * (((double)r2 + 0.5) * 0x1.0p-32);
   const double ue = (((double)r3 + 0.5) * 0x1.0p-32) * (double)b.env;
-  const double r = w * q.wmax;
-  const double beta = r * r - q.eps;
-  const double alpha = r * r + q.eps - 2.0 * q.seps * r * (1.0 - 2.0 * kap);
+  const double r = w * wmax;
+  const double beta = r * r - eps;
+  const double alpha = r * r + eps - 2.0 * seps * r * (1.0 - 2.0 * kap);
   // kinematics at the neutron's own energy: beta > -E/kT and
   // (alpha - 2E/kT - beta)^2 <= 4 (E/kT) (E/kT + beta), i.e. alpha between
   // the kinematic limits (sqrt(E/kT + beta) -+ sqrt(E/kT))^2; table range
-  const double ef = q.ediv + beta;
-  const double d = alpha - q.ediv - ef;
-  if (!(ef > 0.0) || d * d > 4.0 * q.ediv * ef) return false;
+  const double ef = ediv + beta;
+  const double d = alpha - ediv - ef;
+  if (!(ef > 0.0) || d * d > 4.0 * ediv * ef) return false;
   if (beta < s.beta[0] || beta > s.beta[s.nb - 1] || alpha < s.alpha[0] || alpha > s.alpha[s.na - 1]) return false;
-  if ((k.squeeze && ue <= (double)b.tlow) || ue <= kb_eval_s(s, alpha, beta, b.arange, b.brange) * (q.jfac * w * w)) {
+  if ((k.squeeze && ue <= (double)b.tlow) || ue <= kb_eval_s(s, alpha, beta, b.arange, b.brange) * (jfac * w * w)) {
     a_out = alpha;
     b_out = beta;
     return true;

@@ -159,14 +159,13 @@ __global__ void __launch_bounds__(256) k_kb_sample(SABView s, KBView k, const do
   int64_t chunk = wave;
   int64_t next = chunk * kKBChunk;
   int64_t cend = next + kKBChunk < total ? next + kKBChunk : total;
-  // lane state
+  // lane state: the neutron, its energy (and E/kT), interval and the draw
+  // counter of its Philox stream (key and stream are the kernel's)
   bool active = false;
   int64_t kk = 0;
-  int tries = 0;
-  double E = 0.0;
-  KBNeutron q{};
-  Rng rng;
-  rng.init(seed, stream, 0);
+  int tries = 0, iv = 0;
+  uint32_t draw = 0;
+  double E = 0.0, ediv = 0.0;
   while (true) {
     // idle lanes take the next neutrons of the chunk, moving on to the
     // wave's next chunk when this one is used up
@@ -178,12 +177,12 @@ __global__ void __launch_bounds__(256) k_kb_sample(SABView s, KBView k, const do
         if (rank < take) {
           kk = next + rank;
           E = ekin[(total == n) ? kk : (kk % n)];
-          const int iv = kb_interval(k, E);
+          iv = kb_interval(k, E);
           if (iv < 0) {
             queue[atomicAdd(qcount, 1ull)] = kk;
           } else {
-            kb_neutron(s, k, iv, E, q);
-            rng.init(seed, stream, (uint64_t)kk);
+            ediv = E / s.kT;
+            draw = 0;
             tries = 0;
             active = true;
           }
@@ -200,8 +199,14 @@ __global__ void __launch_bounds__(256) k_kb_sample(SABView s, KBView k, const do
     }
     if (!__ballot(active)) break;
     if (active) {
+      Rng rng;
+      rng.init(seed, stream, (uint64_t)kk);
+      rng.c0 = draw;
+      uint32_t r0, r1, r2, r3;
+      rng.gen4(r0, r1, r2, r3);
+      ++draw;
       double a, b;
-      if (kb_propose(s, k, q, rng, a, b)) {
+      if (kb_propose(s, k, iv, ediv, r0, r1, r2, r3, a, b)) {
         if constexpr (Mode == 0) {
           double Ef, mu;
           if (sab_ab_to_emu(s, rng, E, a, b, Ef, mu)) { Ef = -1.0; mu = -999.0; atomicAdd(nerr, 1ull); }
