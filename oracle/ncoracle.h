@@ -181,6 +181,27 @@ void orc_scb_tables(const orc_proc*, double* inv2d, double* xsfact, int64_t* off
 void orc_xs_dir_many(const orc_proc*, const double* ekin, const double* dirs, int64_t n, double* out,
                      int nthreads);
 
+/* VDOS -> S(alpha,beta) (ncvdos.c).  Errors: -1 / NULL with the message in
+   orc_vdos_lasterror(). */
+const char* orc_vdos_lasterror(void);
+/* regulariseVDOSGrid: two-call (cap < npts returns npts without writing) */
+int orc_vdos_regularise(const double* egrid, int negrid, const double* density, int n, double* emin_out,
+                        double* emax_out, double* dens_out, int cap);
+/* createVDOSDebye: 20 density points */
+void orc_vdos_debye(double debye_temp, double* emin, double* emax, double* density20);
+/* out = {gamma0, msd [Aa^2], effective temperature [K], integral of the given density} */
+int orc_vdos_analyse(double emin, double emax, const double* density, int n, double T, double mass, double* out);
+/* G_order spectrum on lo + i*bw: two-call as above */
+int orc_vdos_gn(double emin, double emax, const double* density, int n, double T, double mass, int order,
+                double* lo, double* bw, double* spec, int cap);
+typedef struct orc_vdos_kernel orc_vdos_kernel;
+orc_vdos_kernel* orc_vdos_kernel_create(double emin, double emax, const double* density, int n, double T,
+                                        double mass, unsigned vdoslux, double target_emax);
+/* info3 = {suggested Emax, msd, gamma0} */
+int orc_vdos_kernel_dims(const orc_vdos_kernel*, int* na, int* nb, int* norders, double* info3);
+void orc_vdos_kernel_copy(const orc_vdos_kernel*, double* alpha, double* beta, double* sab);
+void orc_vdos_kernel_free(orc_vdos_kernel*);
+
 #ifdef __cplusplus
 }
 #endif

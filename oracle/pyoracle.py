@@ -130,6 +130,23 @@ def lib():
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp,
                                          ctypes.POINTER(ctypes.c_int64)]
         L.orc_sab_std_format.restype = ctypes.c_int
+        L.orc_vdos_lasterror.restype = ctypes.c_char_p
+        L.orc_vdos_regularise.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
+        L.orc_vdos_debye.argtypes = [ctypes.c_double, _dp, _dp, _dp]
+        L.orc_vdos_debye.restype = None
+        L.orc_vdos_analyse.argtypes = [ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, _dp]
+        L.orc_vdos_gn.argtypes = [ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
+        L.orc_vdos_kernel_create.argtypes = [ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_uint, ctypes.c_double]
+        L.orc_vdos_kernel_create.restype = _vp
+        L.orc_vdos_kernel_dims.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_int), _dp]
+        L.orc_vdos_kernel_copy.argtypes = [_vp, _dp, _dp, _dp]
+        L.orc_vdos_kernel_copy.restype = None
+        L.orc_vdos_kernel_free.argtypes = [_vp]
+        L.orc_vdos_kernel_free.restype = None
         L.orc_sample_dir_many.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _dp, _dp,
                                           ctypes.c_int64, _dp, _dp, ctypes.c_int]
         L.orc_sample_dir_many.restype = ctypes.c_int64
@@ -289,6 +306,70 @@ def sab_std_format(knltype, alpha, beta, sab, beta_grid_optimised=False):
     oa, ob, os_ = np.empty(na.value), np.empty(nb.value), np.empty((nb.value, na.value))
     lib().orc_sab_std_format(*args, ctypes.byref(na), ctypes.byref(nb), _d(oa), _d(ob), _d(os_), ctypes.byref(nt))
     return dict(alpha=oa, beta=ob, sab=os_, ntrimmed=nt.value)
+
+
+def _vdos_error():
+    msg = lib().orc_vdos_lasterror()
+    return RuntimeError(msg.decode() if msg else "VDOS error")
+
+
+def vdos_regularise(egrid, density):
+    """regulariseVDOSGrid: (emin, emax, density) on the regular grid"""
+    eg = np.ascontiguousarray(egrid, dtype=np.float64)
+    d = np.ascontiguousarray(density, dtype=np.float64)
+    lo, hi = ctypes.c_double(), ctypes.c_double()
+    n = lib().orc_vdos_regularise(_d(eg), eg.size, _d(d), d.size, ctypes.byref(lo), ctypes.byref(hi), None, 0)
+    if n < 0:
+        raise _vdos_error()
+    out = np.empty(n)
+    lib().orc_vdos_regularise(_d(eg), eg.size, _d(d), d.size, ctypes.byref(lo), ctypes.byref(hi), _d(out), n)
+    return lo.value, hi.value, out
+
+
+def vdos_debye(debye_temp):
+    """createVDOSDebye: (emin, emax, density of 20 points)"""
+    lo, hi, d = ctypes.c_double(), ctypes.c_double(), np.empty(20)
+    lib().orc_vdos_debye(float(debye_temp), ctypes.byref(lo), ctypes.byref(hi), _d(d))
+    return lo.value, hi.value, d
+
+
+def vdos_analyse(emin, emax, density, temperature, mass):
+    """dict(gamma0, msd [Aa^2], teff [K], integral) of a regularised VDOS"""
+    d = np.ascontiguousarray(density, dtype=np.float64)
+    out = np.empty(4)
+    if lib().orc_vdos_analyse(emin, emax, _d(d), d.size, temperature, mass, _d(out)):
+        raise _vdos_error()
+    return dict(gamma0=out[0], msd=out[1], teff=out[2], integral=out[3])
+
+
+def vdos_gn(emin, emax, density, temperature, mass, order):
+    """(egrid, spectrum) of the phonon-order-n function G_n"""
+    d = np.ascontiguousarray(density, dtype=np.float64)
+    lo, bw = ctypes.c_double(), ctypes.c_double()
+    args = (emin, emax, _d(d), d.size, temperature, mass, int(order))
+    n = lib().orc_vdos_gn(*args, ctypes.byref(lo), ctypes.byref(bw), None, 0)
+    if n < 0:
+        raise _vdos_error()
+    spec = np.empty(n)
+    lib().orc_vdos_gn(*args, ctypes.byref(lo), ctypes.byref(bw), _d(spec), n)
+    return lo.value + bw.value * np.arange(n), spec
+
+
+def vdos_kernel(emin, emax, density, temperature, mass, vdoslux=3, target_emax=0.0):
+    """createScatteringKernel: dict(alpha, beta, sab[nb][na], suggested_emax,
+    msd, gamma0, norders)"""
+    d = np.ascontiguousarray(density, dtype=np.float64)
+    k = lib().orc_vdos_kernel_create(emin, emax, _d(d), d.size, temperature, mass, int(vdoslux), float(target_emax))
+    if not k:
+        raise _vdos_error()
+    try:
+        na, nb, no, info = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), np.empty(3)
+        lib().orc_vdos_kernel_dims(k, ctypes.byref(na), ctypes.byref(nb), ctypes.byref(no), _d(info))
+        a, b, s_ = np.empty(na.value), np.empty(nb.value), np.empty((nb.value, na.value))
+        lib().orc_vdos_kernel_copy(k, _d(a), _d(b), _d(s_))
+    finally:
+        lib().orc_vdos_kernel_free(k)
+    return dict(alpha=a, beta=b, sab=s_, suggested_emax=info[0], msd=info[1], gamma0=info[2], norders=no.value)
 
 
 def tailed_breakdown(alpha, sab, logsab, cumul, alow, aupp, ilo, iup):
