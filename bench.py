@@ -64,7 +64,8 @@ def parse():
                     help="also run one device-to-device copy of the output size (counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["all", "sab", "c1", "scbragg", "powderbragg", "batch"], default="all")
+    ap.add_argument("--workload", choices=["all", "sab", "c1", "scbragg", "powderbragg", "batch", "vdos"],
+                    default="all")
     ap.add_argument("--sub-cpu-seconds", type=float, default=8.0,
                     help="CPU-baseline sample length of each sub-result")
     ap.add_argument("--train-steps", type=int, default=2000)
@@ -185,10 +186,21 @@ def cpu_baseline_scb(args_scb, e, dirs, seconds, threads):
 
 
 def c1_setup():
-    from ncrystal_amd import materials, synth
+    """the fcc powder: ElInc, PowderBragg and the S(alpha,beta) expanded on the
+    GPU from the Debye VDOS of T_Debye 410 K (createVDOSDebye; NCrystal's
+    default inelastic model for a crystal with a Debye temperature), vdoslux 3"""
+    import math
+    from ncrystal_amd import core, materials, synth
     cr = synth.fcc_powder()
-    sab = synth.sab_table_c1()
-    m = materials.Material(cr, sab=sab)
+    sp = cr.species[0]
+    sigma = 4 * math.pi * sp.csl ** 2 + sp.sigma_inc
+    dyn = {"vdos": sp.debye_temp, "mass": sp.mass, "sigma_bound": sigma, "vdoslux": 3}
+    m = materials.Material(cr, dynamics=[(1.0, dyn)])
+    k = core.vdos_kernel(sp.debye_temp, cr.temperature, sp.mass, vdoslux=3)
+    t = core.sab_std_format(0, k["alpha"], k["beta"], k["sab"], beta_grid_optimised=True)
+    sab = dict(alpha=t["alpha"], beta=t["beta"], sab=t["sab"], temperature=cr.temperature, mass=sp.mass,
+               sigma_bound=sigma, suggested_emax=k["suggested_emax"])
+    m.components["inelas"] = m.inelastic[0]
     return cr, sab, m
 
 
@@ -203,7 +215,8 @@ def cpu_baseline_c1(m, sab, e, dirs, seconds, threads):
     v0n, dsp, fm = m.pb_inputs
     o = pyoracle.composition([pyoracle.elinc(mm, bb, ss), pyoracle.powderbragg(v0n, dsp, fm),
                               pyoracle.sab(sab["alpha"], sab["beta"], sab["sab"], sab["temperature"], sab["mass"],
-                                           sab["sigma_bound"])], [1.0, 1.0, 1.0])
+                                           sab["sigma_bound"], suggested_emax=sab.get("suggested_emax", 0.0))],
+                             [1.0, 1.0, 1.0])
     done, k = 0, 0
     t0 = time.perf_counter()
     while True:
@@ -266,10 +279,10 @@ def run_c1(args):
         "value": n * args.steps / wall, "unit": "neutrons/s",
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "dtype": "f64",
         "config": {"workload": "fcc powder a=4.05 Aa, b_coh 3.449 fm, sigma_inc 0.0082 b, T_Debye 410 K, T 293 K, "
-                               "dcutoff 0.5 Aa: ElInc + PowderBragg + S(alpha,beta) of the Debye VDOS (%dx%d, "
-                               "Emax %.3f eV, free-gas extender above); E log-uniform 1e-5..10 eV, isotropic "
-                               "directions, seed 1" % (sab["alpha"].size, sab["beta"].size,
-                                                       m.components["inelas"].egrid()[0][-1]),
+                               "dcutoff 0.5 Aa: ElInc + PowderBragg + S(alpha,beta) expanded on the GPU from the "
+                               "Debye VDOS (vdoslux 3: %dx%d, Emax %.3f eV, free-gas extender above); E "
+                               "log-uniform 1e-5..10 eV, isotropic directions, seed 1"
+                               % (sab["alpha"].size, sab["beta"].size, m.components["inelas"].egrid()[0][-1]),
                    "neutrons_per_step": n, "families": int(len(m.pb_inputs[1])), "setup_s": t_setup},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": None,
@@ -725,13 +738,66 @@ def run_sab(args):
     return out
 
 
+def run_vdos(args):
+    """VDOS -> S(alpha,beta) expansion of the aluminium VDOS of the data
+    library (tests/golden/vdos_reference.json) at 293.15 K, vdoslux 5 (the
+    largest standard table, 1600 x 3200): whole calls (host chain + GPU
+    operand tables and order sum) per second; roofline of the order sum on
+    the FP64 matrix cores"""
+    import json as _json
+    import numpy as np
+    from ncrystal_amd import core
+    fix = _json.loads((ROOT / "tests" / "golden" / "vdos_reference.json").read_text())
+    v = fix["vdos"]["Al_sg225/Al"]
+    vin = (np.array(v["egrid"]), np.array(v["density"]))
+    T0, mass, lux = 293.15, v["mass"], 5
+    steps = max(1, min(args.steps, 5))
+    for w in range(max(1, min(args.warmup, 2))):
+        core.vdos_kernel(vin, T0 + 1e-9 * (w + 1), mass, vdoslux=lux)
+    # each call at a temperature 1e-12 apart, so that every call expands anew
+    t0 = time.perf_counter()
+    gms = 0.0
+    for i in range(steps):
+        k = core.vdos_kernel(vin, T0 * (1.0 + 1e-12 * i), mass, vdoslux=lux)
+        gms += k["gpu_ms"]
+    wall = time.perf_counter() - t0
+    gms /= steps
+    M, N, K = k["rows_summed"], k["alpha"].size, k["orders_summed"]
+    flops = 2.0 * M * N * ((K + 3) // 4 * 4)
+    ach = flops / (gms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pyoracle
+        pyoracle.build()
+        lo, hi, d = pyoracle.vdos_regularise(*vin)
+        tc = time.perf_counter()
+        pyoracle.vdos_kernel(lo, hi, d, T0, mass, vdoslux=lux)
+        dt = time.perf_counter() - tc
+        cpu = dict(value=1.0 / dt, unit="expansions/s", cores=1, kind="port",
+                   sample=f"one expansion of the same VDOS at vdoslux {lux} by the checker (host chain and "
+                          f"order sum on one core), {dt:.1f} s")
+    return {
+        "metric": "VDOS -> S(alpha,beta) expansions per second on 1 MI355X",
+        "value": steps / wall, "unit": "expansions/s", "steps": steps, "warmup": 1,
+        "ms_per_step": wall / steps * 1e3, "dtype": "f64",
+        "config": {"workload": "aluminium VDOS of the data library, T 293.15 K, vdoslux 5: %dx%d table, %d phonon "
+                               "orders (%d summed)" % (N, k["beta"].size, k["norders"], K),
+                   "gpu_sum_ms": gms},
+        "roofline": {"bound": "mfma", "achieved": ach, "peak": 78.6, "unit": "TFLOP/s", "frac": ach / 78.6,
+                     "traffic": None, "kernel": "k_vdos_gtab + k_vdos_afac + k_vdos_fill (order sum, "
+                     "v_mfma_f64_16x16x4f64)", "kernel_ms": gms, "flops_per_launch": flops},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only (a one-GPU node); run with --gpus 1"}))
         sys.exit(2)
     runners = {"sab": run_sab, "c1": run_c1, "powderbragg": run_powderbragg, "batch": run_batch,
-               "scbragg": run_scbragg}
+               "scbragg": run_scbragg, "vdos": run_vdos}
     if args.workload != "all":
         out = runners[args.workload](args)
         out.update(n_gpus=1, host=host_info(), library=library_info())
@@ -741,7 +807,7 @@ def main():
     sub_args = argparse.Namespace(**vars(args))
     sub_args.cpu_seconds = args.sub_cpu_seconds
     sub = {}
-    for key, name in (("C1", "c1"), ("C2", "powderbragg"), ("C4", "batch"), ("C5", "scbragg")):
+    for key, name in (("C1", "c1"), ("C2", "powderbragg"), ("C4", "batch"), ("C5", "scbragg"), ("F2", "vdos")):
         t0 = time.perf_counter()
         r = runners[name](sub_args)
         for k in ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):

@@ -180,6 +180,55 @@ int ncamd_sab_std_format(int knltype, int beta_grid_optimised, int nalpha, int n
                          const double* beta, const double* sab, int* out_nalpha, int* out_nbeta, double* out_alpha,
                          double* out_beta, double* out_sab, int64_t* ntrimmed);
 
+/* ---- VDOS -> S(alpha,beta) ----
+ * A phonon density of states is given as n_density values on n_egrid == 2
+ * (range [emin, emax], equidistant) or n_egrid == n_density (grid) energies
+ * [eV], as in NCMAT files; n_egrid == 1 with n_density == 0 is the Debye
+ * spectrum of createVDOSDebye (egrid[0] = Debye temperature [K]; 20 points
+ * of E^2 on [E_D/2, E_D], ncrystal_core/src/dyninfoutils/NCDynInfoUtils.cc:
+ * 377-396).  The grid is first regularised as regulariseVDOSGrid does
+ * (ncrystal_core/src/vdos/NCVDOSEval.cc:497-640). */
+
+/* regulariseVDOSGrid: returns the number of points of the regular grid; with
+ * out_density of at least that capacity also writes emin, emax and the
+ * densities.  -1 on error. */
+int ncamd_vdos_regularise(int n_egrid, const double* egrid, int n_density, const double* density, double* emin,
+                          double* emax, double* out_density, int capacity);
+
+/* VDOSEval (NCVDOSEval.cc:147-445) at temperature [K] for an element of mass
+ * [amu]: out4 = {gamma0, mean squared displacement [Aa^2] (the Debye-Waller
+ * MSD of the element), effective temperature [K], integral of the given
+ * density}.  Host computation.  0, or -1 on error. */
+int ncamd_vdos_analyse(int n_egrid, const double* egrid, int n_density, const double* density, double temperature,
+                       double mass_amu, double* out4);
+
+/* createScatteringKernel (ncrystal_core/src/vdos/NCVDOSToScatKnl.cc:787-944)
+ * with the default truncation and thinning of the phonon-order functions:
+ * the orders needed to reach the kinematic curve of target_emax [eV] (0: the
+ * vdoslux default 0.5, 1, 3, 5, 8, 12 eV for vdoslux 0..5), the alpha and
+ * beta grids (nbeta = 100 * 2^vdoslux, nalpha = nbeta/2) and
+ * sab[ibeta*nalpha+ialpha] = sum over orders n of x^n exp(-x)/n! kT
+ * G_n(beta kT), summed on the GPU's FP64 matrix cores.  info (7 values, or
+ * NULL) = {suggested Emax [eV], msd [Aa^2], gamma0, number of orders, orders
+ * summed (all, or whole blocks of at least 16 as the reference sums them),
+ * rows beta <= 0 summed, device time of the sum [ms]}.  Call with NULL arrays for
+ * the sizes, then with arrays of those sizes (the kernel of the last inputs
+ * is kept, so it is built once).  0, or -1 on error (CalcError when the
+ * expansion cannot reach target_emax). */
+int ncamd_vdos_kernel(int n_egrid, const double* egrid, int n_density, const double* density, double temperature,
+                      double mass_amu, int vdoslux, double target_emax, int* nalpha, int* nbeta, double* alpha,
+                      double* beta, double* sab, double* info);
+
+/* SABScatter of the kernel above, as NCrystal's standard scatter factory
+ * creates one per element with VDOS dynamics (NCStdScatFact.cc:256-275,
+ * NCDynInfoUtils.cc:290-340): the kernel in standard form (zero edges
+ * trimmed, its beta grid kept) with its suggested Emax as the top of the
+ * energy grid; sigma_bound [barn] is the element's bound scattering cross
+ * section. */
+ncamd_process_t ncamd_create_sabscatter_from_vdos(int n_egrid, const double* egrid, int n_density,
+                                                  const double* density, double temperature, double mass_amu,
+                                                  double sigma_bound, int vdoslux, double target_emax);
+
 /* Composition of processes with scales, replaces ProcComposition
  * (ncrystal_core/src/interfaces/NCProcImpl.cc:110-389).  Components are
  * added as ProcComposition::addComponent does (NCProcImpl.cc:263-338):

@@ -82,6 +82,14 @@ def lib():
         "ncamd_clone": (Handle, [Handle, ctypes.c_uint64]),
         "ncamd_sab_std_format": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
                                                 _ip, _ip, _dp, _dp, _dp, _i64p]),
+        "ncamd_vdos_regularise": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]),
+        "ncamd_vdos_analyse": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
+                                              _dp]),
+        "ncamd_vdos_kernel": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_double, _ip, _ip, _dp, _dp, _dp, _dp]),
+        "ncamd_create_sabscatter_from_vdos": (Handle, [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_double,
+                                                       ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                                       ctypes.c_double]),
         "ncamd_ref": (None, [Handle]),
         "ncamd_unref": (None, [ctypes.POINTER(Handle)]),
         "ncamd_refcount": (ctypes.c_int, [Handle]),

@@ -427,6 +427,60 @@ def create_sabscatter(alpha, beta, sab, temperature, mass_amu, sigma_bound, sugg
     return SABScatter(h, name="SABScatter")
 
 
+def _vdos_args(vdos):
+    """(n_egrid, egrid, n_density, density) of a VDOS given as (egrid,
+    density) or as a Debye temperature (createVDOSDebye)"""
+    if np.isscalar(vdos):
+        eg = C.as_f64([float(vdos)])
+        return 1, eg, 0, None, eg
+    egrid, density = vdos
+    eg, d = C.as_f64(egrid), C.as_f64(density)
+    return eg.size, eg, d.size, d, (eg, d)
+
+
+def vdos_regularise(vdos):
+    """regulariseVDOSGrid: (emin, emax, density) of the regular grid"""
+    ne, eg, nd, d, keep = _vdos_args(vdos)
+    dp = C.dptr(d) if d is not None else None
+    lo, hi = ctypes.c_double(), ctypes.c_double()
+    n = C.call("ncamd_vdos_regularise", ne, C.dptr(eg), nd, dp, ctypes.byref(lo), ctypes.byref(hi), None, 0)
+    out = np.empty(n)
+    C.call("ncamd_vdos_regularise", ne, C.dptr(eg), nd, dp, ctypes.byref(lo), ctypes.byref(hi), C.dptr(out), n)
+    return lo.value, hi.value, out
+
+
+def vdos_analyse(vdos, temperature, mass_amu):
+    """VDOSEval values: dict(gamma0, msd [Aa^2], teff [K], integral)"""
+    ne, eg, nd, d, keep = _vdos_args(vdos)
+    out = np.empty(4)
+    C.call("ncamd_vdos_analyse", ne, C.dptr(eg), nd, C.dptr(d) if d is not None else None, float(temperature),
+           float(mass_amu), C.dptr(out))
+    return dict(gamma0=out[0], msd=out[1], teff=out[2], integral=out[3])
+
+
+def vdos_kernel(vdos, temperature, mass_amu, vdoslux=3, target_emax=0.0):
+    """createScatteringKernel on the GPU: dict(alpha, beta, sab[nb][na],
+    suggested_emax, msd, gamma0, norders, orders_summed, rows_summed, gpu_ms)"""
+    ne, eg, nd, d, keep = _vdos_args(vdos)
+    dp = C.dptr(d) if d is not None else None
+    args = (ne, C.dptr(eg), nd, dp, float(temperature), float(mass_amu), int(vdoslux), float(target_emax))
+    na, nb, info = ctypes.c_int(), ctypes.c_int(), np.empty(7)
+    C.call("ncamd_vdos_kernel", *args, ctypes.byref(na), ctypes.byref(nb), None, None, None, C.dptr(info))
+    a, b, s = np.empty(na.value), np.empty(nb.value), np.empty((nb.value, na.value))
+    C.call("ncamd_vdos_kernel", *args, ctypes.byref(na), ctypes.byref(nb), C.dptr(a), C.dptr(b), C.dptr(s), None)
+    return dict(alpha=a, beta=b, sab=s, suggested_emax=info[0], msd=info[1], gamma0=info[2], norders=int(info[3]),
+                orders_summed=int(info[4]), rows_summed=int(info[5]), gpu_ms=info[6])
+
+
+def create_sabscatter_from_vdos(vdos, temperature, mass_amu, sigma_bound, vdoslux=3, target_emax=0.0):
+    """SABScatter of an element with VDOS dynamics (the kernel expanded on the
+    GPU, its suggested Emax as the top of the energy grid)"""
+    ne, eg, nd, d, keep = _vdos_args(vdos)
+    h = C.call("ncamd_create_sabscatter_from_vdos", ne, C.dptr(eg), nd, C.dptr(d) if d is not None else None,
+               float(temperature), float(mass_amu), float(sigma_bound), int(vdoslux), float(target_emax))
+    return SABScatter(h, name="SABScatter")
+
+
 def create_composition(procs, scales):
     """ProcComposition of the processes (NCProcImpl.cc:255-338): nested
     compositions are flattened, repeated processes add their scales,

@@ -30,6 +30,7 @@
 #include "../../include/ncrystal_dropin.h"
 #include "nc_kernels.hh"
 #include "nc_math.hh"
+#include "nc_vdos.hh"
 
 namespace ncamd {
 namespace {
@@ -3520,6 +3521,126 @@ int ncamd_sab_std_format(int knltype, int beta_grid_optimised, int nalpha, int n
   return 0;
   NCAMD_CATCH
   return -1;
+}
+
+// ------------------------------------------------ VDOS -> S(alpha,beta) ---
+namespace {
+// the VDOS of the entry points: n_egrid == 2 (range) or == n_density (grid);
+// n_egrid == 1 with n_density == 0 is createVDOSDebye(egrid[0] = Debye
+// temperature [K])
+RegularVdos vdos_input(int n_egrid, const double* egrid, int n_density, const double* density) {
+  if (n_egrid == 1 && n_density == 0) {
+    if (!egrid || !(egrid[0] > 0.0 && egrid[0] < 1e5)) raise("BadInput", "Debye temperature out of range");
+    double lo, hi;
+    std::vector<double> d;
+    vdos_debye(egrid[0], &lo, &hi, d);
+    return vdos_regularise(std::vector<double>{lo, hi}, d);
+  }
+  if (!egrid || !density || n_egrid < 2 || n_density < 1) raise("BadInput", "missing VDOS arrays");
+  return vdos_regularise(std::vector<double>(egrid, egrid + n_egrid), std::vector<double>(density, density + n_density));
+}
+// the last kernel built, so that the two calls of ncamd_vdos_kernel (sizes,
+// then arrays) build it once
+struct VdosKey {
+  std::vector<double> in;
+  bool operator==(const VdosKey& o) const { return in == o.in; }
+};
+std::mutex g_vdos_mtx;
+VdosKey g_vdos_key;
+std::shared_ptr<VdosKernel> g_vdos_last;
+std::shared_ptr<VdosKernel> vdos_kernel_cached(int n_egrid, const double* egrid, int n_density, const double* density,
+                                               double T, double mass, int lux, double target) {
+  VdosKey key;
+  key.in.assign(egrid, egrid + n_egrid);
+  key.in.insert(key.in.end(), density, density + n_density);
+  key.in.insert(key.in.end(), {(double)n_egrid, (double)n_density, T, mass, (double)lux, target});
+  {
+    std::lock_guard<std::mutex> lk(g_vdos_mtx);
+    if (g_vdos_last && g_vdos_key == key) return g_vdos_last;
+  }
+  ensure_device();
+  const RegularVdos v = vdos_input(n_egrid, egrid, n_density, density);
+  if (lux < 0 || lux > 5) raise("BadInput", "vdoslux must be in 0..5");
+  auto k = std::make_shared<VdosKernel>(vdos_kernel(v, T, mass, (unsigned)lux, target, nullptr));
+  std::lock_guard<std::mutex> lk(g_vdos_mtx);
+  g_vdos_key = key;
+  g_vdos_last = k;
+  return k;
+}
+}  // namespace
+
+#define NCAMD_VDOS_CATCH                                              \
+  }                                                                   \
+  catch (const VdosError& e) { set_error(e.what(), e.type.c_str()); } \
+  catch (const Error& e) { set_error(e.what(), e.type.c_str()); }     \
+  catch (const std::exception& e) { set_error(e.what(), "std::exception"); }
+
+int ncamd_vdos_analyse(int n_egrid, const double* egrid, int n_density, const double* density, double temperature,
+                       double mass_amu, double* out4) {
+  NCAMD_TRY
+  const RegularVdos v = vdos_input(n_egrid, egrid, n_density, density);
+  const VdosAnalysis a = vdos_analyse(v, temperature, mass_amu);
+  out4[0] = a.gamma0;
+  out4[1] = a.msd;
+  out4[2] = a.teff;
+  out4[3] = a.integral;
+  return 0;
+  NCAMD_VDOS_CATCH
+  return -1;
+}
+
+int ncamd_vdos_regularise(int n_egrid, const double* egrid, int n_density, const double* density, double* emin,
+                          double* emax, double* out_density, int capacity) {
+  NCAMD_TRY
+  const RegularVdos v = vdos_input(n_egrid, egrid, n_density, density);
+  const int n = (int)v.density.size();
+  if (out_density && capacity >= n) {
+    *emin = v.emin;
+    *emax = v.emax;
+    std::copy(v.density.begin(), v.density.end(), out_density);
+  }
+  return n;
+  NCAMD_VDOS_CATCH
+  return -1;
+}
+
+int ncamd_vdos_kernel(int n_egrid, const double* egrid, int n_density, const double* density, double temperature,
+                      double mass_amu, int vdoslux, double target_emax, int* nalpha, int* nbeta, double* alpha,
+                      double* beta, double* sab, double* info) {
+  NCAMD_TRY
+  const auto k = vdos_kernel_cached(n_egrid, egrid, n_density, density, temperature, mass_amu, vdoslux, target_emax);
+  *nalpha = (int)k->alpha.size();
+  *nbeta = (int)k->beta.size();
+  if (alpha) std::copy(k->alpha.begin(), k->alpha.end(), alpha);
+  if (beta) std::copy(k->beta.begin(), k->beta.end(), beta);
+  if (sab) std::copy(k->sab.begin(), k->sab.end(), sab);
+  if (info) {
+    info[0] = k->suggested_emax;
+    info[1] = k->msd;
+    info[2] = k->gamma0;
+    info[3] = k->norders;
+    info[4] = k->orders_summed;
+    info[5] = k->rows_summed;
+    info[6] = k->gpu_ms;
+  }
+  return 0;
+  NCAMD_VDOS_CATCH
+  return -1;
+}
+
+ncamd_process_t ncamd_create_sabscatter_from_vdos(int n_egrid, const double* egrid, int n_density,
+                                                  const double* density, double temperature, double mass_amu,
+                                                  double sigma_bound, int vdoslux, double target_emax) {
+  NCAMD_TRY
+  const auto k = vdos_kernel_cached(n_egrid, egrid, n_density, density, temperature, mass_amu, vdoslux, target_emax);
+  // ScatKnlData of knltype SAB with an optimised beta grid (no thickening),
+  // suggestedEmax from the expansion (NCVDOSToScatKnl.cc:930-943)
+  const SabStd t = sab_std_format(0, true, (int)k->alpha.size(), (int)k->beta.size(), k->alpha.data(),
+                                  k->beta.data(), k->sab.data(), 500);
+  return wrap(create_sab((int)t.alpha.size(), (int)t.beta.size(), t.alpha.data(), t.beta.data(), t.sab.data(),
+                         temperature, mass_amu, sigma_bound, k->suggested_emax, 0, nullptr));
+  NCAMD_VDOS_CATCH
+  return wrap(nullptr);
 }
 
 ncamd_process_t ncamd_clone(ncamd_process_t h, uint64_t seed) {

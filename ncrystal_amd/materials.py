@@ -40,9 +40,10 @@ class Material:
     temperature, mass, sigma_bound, optional egrid) at scale 1.  dynamics:
     the inelastic components per element instead, as the standard scatter
     factory adds them (NCStdScatFact.cc:256-300): a list of (fraction, entry),
-    entry an S(alpha,beta) table dict or {"freegas": (mass_amu, sigma_free)}
-    at the crystal's temperature, each a component scaled by the element's
-    number fraction."""
+    entry an S(alpha,beta) table dict, {"freegas": (mass_amu, sigma_free)} or
+    {"vdos": (egrid, density) or Debye temperature, "mass": amu,
+    "sigma_bound": barn, "vdoslux": 0..5} at the crystal's temperature, each a
+    component scaled by the element's number fraction."""
 
     def __init__(self, crystal, sab=None, with_elinc=True, with_bragg=True, fp32_tables=True, dynamics=None):
         if sab is not None and dynamics is not None:
@@ -71,6 +72,11 @@ class Material:
             if "freegas" in d:
                 mass, sigma = d["freegas"]
                 c = core.create_freegas(crystal.temperature, mass, sigma)
+            elif "vdos" in d and "sab" not in d:
+                # an S(alpha,beta) expanded from the element's VDOS on the GPU
+                # ((egrid, density) or a Debye temperature)
+                c = core.create_sabscatter_from_vdos(d["vdos"], crystal.temperature, d["mass"], d["sigma_bound"],
+                                                     vdoslux=d.get("vdoslux", 3))
             else:
                 c = core.create_sabscatter(d["alpha"], d["beta"], d["sab"], d["temperature"], d["mass"],
                                            d["sigma_bound"], egrid=d.get("egrid"))

@@ -463,10 +463,15 @@ static double* v_convolve(vconv_t* cv, const double* a1, int n1, const double* a
     cv->logn = logn;
     cv->wc = (double*)malloc(sizeof(double) * (size_t)(n / 2 + 1));
     cv->ws = (double*)malloc(sizeof(double) * (size_t)(n / 2 + 1));
+    /* separate libm cos and sin calls (through volatile pointers, which no
+       compiler fuses into sincos: that routine can differ from them in the
+       last bit, and the truncation edges of the G_n depend on such bits) */
+    double (*volatile fcos)(double) = cos;
+    double (*volatile fsin)(double) = sin;
     for (int j = 0; j <= n / 2; ++j) {
       const double ang = V_2PI * (double)j / (double)n;
-      cv->wc[j] = cos(ang);
-      cv->ws[j] = sin(ang);
+      cv->wc[j] = fcos(ang);
+      cv->ws[j] = fsin(ang);
     }
     cv->b = (double*)malloc(sizeof(double) * (size_t)n * 4);
   }

@@ -15,7 +15,10 @@ Read (as data, nothing is executed):
   * tests/scripts/vdosregularise.log: the regularised grid (emin, emax, point
     count) and first/last density values of every data-library VDOS;
   * tests/scripts/vdos2knl.log: kernel grid sizes and suggested Emax for the
-    Debye VDOS of 400 K.
+    Debye VDOS of 400 K;
+  * tests/scripts/basic.log: aluminium cross sections (PowderBragg at 1-4 Aa,
+    absorption, incoherent elastic + inelastic at 4 Aa) with the cell of
+    data/Al_sg225.ncmat.
 """
 import json
 import pathlib
@@ -198,9 +201,28 @@ def main():
             rec.update(error=err.group(1))
         knl.append(rec)
 
-    used = {r["vdos"] for r in analyse} | {r["vdos"] for r in gn} | {r["vdos"] for r in keep}
+    # tests/scripts/basic.log: aluminium from data/Al_sg225.ncmat at 293.15 K
+    # (PowderBragg at 1-4 Aa, absorption and the "Bkgd" = incoherent elastic +
+    # inelastic at 4 Aa); cell and atom data from that file and NCAtomDB.cc
+    blog = (REF / "tests/scripts/basic.log").read_text()
+    mpb = re.search(r"PowderBragg diffraction cross-section @ \[1, 2, 3, 4\]Aa: \[(.*?)\] barn", blog)
+    mabs = re.search(r"absorption cross-section @ 4.0Aa: (\S+) barn", blog)
+    mbk = re.search(r"Bkgd cross-section @ 4.0Aa: (\S+) barn", blog)
+    mbp = re.search(r"Bkgd\+PowderBragg cross-section @ 4.0Aa: (\S+) barn", blog)
+    mT = re.search(r"Temperature : (\S+)", blog)
+    atom = re.search(r"addNatElem\(13, ([0-9.]+), ([0-9.]+), ([0-9.]+), ([0-9.]+)\);//Al",
+                     (REF / "ncrystal_core/src/atomdb/NCAtomDB.cc").read_text())
+    kal = key_for("Al_sg225", "Al")
+    basic = dict(vdos=kal, temperature=float(mT.group(1)), spacegroup=225, a=4.04958,
+                 positions=[[0, 0.5, 0.5], [0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5]],
+                 mass=float(atom.group(1)), coh_sl=float(atom.group(2)), sigma_inc=float(atom.group(3)),
+                 sigma_abs=float(atom.group(4)), dcutoff=0.1,
+                 powderbragg_1to4=[float(v) for v in mpb.group(1).split(",")], absorption_4=float(mabs.group(1)),
+                 bkgd_4=float(mbk.group(1)), bkgd_plus_bragg_4=float(mbp.group(1)))
+
+    used = {r["vdos"] for r in analyse} | {r["vdos"] for r in gn} | {r["vdos"] for r in keep} | {kal}
     out = dict(vdos={k: v for k, v in vdos.items() if k in used}, analyse=analyse, gn=gn, regularise=keep,
-               kernel=knl)
+               kernel=knl, basic=basic)
     OUT.write_text(json.dumps(out, indent=0))
     print(f"wrote {OUT}: {len(out['vdos'])} densities, {len(analyse)} analyse, {len(gn)} Gn, "
           f"{len(keep)} regularise, {len(knl)} kernel records")

@@ -16,6 +16,9 @@ against what the reference's own tests record for its data library
 * the MSD of a Debye VDOS against the Debye model's closed form
   (ncrystal_core/src/phys_utils/NCDebyeMSD.cc, itself pinned to
   app_debyemsd);
+* tests/scripts/basic.log: the aluminium cross sections that depend on
+  the VDOS (PowderBragg through its MSD, incoherent elastic + inelastic
+  through the expanded kernel), and absorption;
 * the Sjolander fill: S at sampled cells against an independent sum over
   orders with x^n exp(-x)/n! from lgamma and the checker's G_n, and detailed
   balance S(alpha, +beta) = S(alpha, -beta) exp(-beta) on the mirrored rows.
@@ -179,3 +182,34 @@ def test_fill_independent_sum(oracle):
         assert beta[jm] == -beta[j]
         ok = sab[jm] > 1e-300
         assert np.allclose(sab[j, ok], sab[jm, ok] * math.exp(-beta[j]), rtol=1e-13, atol=0)
+
+
+def test_basic_log_aluminium(oracle):
+    """tests/scripts/basic.log, aluminium of data/Al_sg225.ncmat at 293.15 K:
+    the MSD from its VDOS into the Debye-Waller factors of the space-group-225
+    reflections (dcutoff 0.1 Aa) gives the PowderBragg cross sections at
+    1-4 Aa; 1/v absorption at 4 Aa; incoherent elastic plus the inelastic
+    S(alpha,beta) expanded from the same VDOS (vdoslux 3, suggested Emax) at
+    4 Aa gives the "Bkgd" value (6 printed digits each)"""
+    b = FIX["basic"]
+    eg, dens, mass = _vdos(oracle, b["vdos"])
+    assert mass == b["mass"]
+    T = b["temperature"]
+    lo, hi, d = oracle.vdos_regularise(eg, dens)
+    msd = oracle.vdos_analyse(lo, hi, d, T, mass)["msd"]
+    a = b["a"]
+    hkl, mult, dsp, fsq, _, _ = oracle.hkl_spacegroup_members(b["spacegroup"], (a, a, a, 90, 90, 90),
+                                                               [(b["coh_sl"], msd, np.array(b["positions"]))],
+                                                               b["dcutoff"])
+    wl = np.array([1.0, 2.0, 3.0, 4.0])
+    e = 0.081804209605330899 / wl ** 2
+    pb = oracle.powderbragg(a ** 3 * len(b["positions"]), dsp, fsq * mult).xs(e)
+    assert np.allclose(pb, b["powderbragg_1to4"], rtol=5e-6, atol=0)
+    assert abs(oracle.absoov(b["sigma_abs"]).xs(e[3:])[0] / b["absorption_4"] - 1) < 5e-6
+    k = oracle.vdos_kernel(lo, hi, d, T, mass, vdoslux=3)
+    sf = oracle.sab_std_format(0, k["alpha"], k["beta"], k["sab"], beta_grid_optimised=True)
+    sigma = 4 * math.pi * b["coh_sl"] ** 2 + b["sigma_inc"]
+    inel = oracle.sab(sf["alpha"], sf["beta"], sf["sab"], T, mass, sigma, suggested_emax=k["suggested_emax"])
+    bkgd = oracle.elinc([msd], [b["sigma_inc"]], [1.0]).xs(e[3:])[0] + inel.xs(e[3:])[0]
+    assert abs(bkgd / b["bkgd_4"] - 1) < 5e-6, bkgd
+    assert abs((bkgd + pb[3]) / b["bkgd_plus_bragg_4"] - 1) < 5e-6
