@@ -139,6 +139,7 @@ def cpu_baseline(table, batch, seconds, threads):
 
 
 FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
 # FP32 pre-filter of the scan per (neutron, demi-normal): dot product (3 mul + 2 add),
 # sin(theta) = w * 1/(2d) (1 mul), | |dot| - sin(theta) | (1 sub), compare (1)
 SCB_FLOPS_PER_NORMAL = 8
@@ -359,7 +360,8 @@ def run_scbragg(args):
                    "neutrons_per_step": n, "families": int(len(t["inv2d"])),
                    "demi_normals": int(t["off"][-1]), "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "sampling call: k_scb_sample_count + k_scb_sample_pick + k_scb_sample_scatter "
+                     "frac": achieved / FP32_PEAK_TFLOPS, "frac_vs_fp64_peak": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": None, "kernel": "sampling call: k_scb_sample_count + k_scb_sample_pick + k_scb_sample_scatter "
                                                                "+ sort + Romberg queue",
                      "kernel_ms": samp_ms, "xs_call_ms": xs_ms,
                      "xs_achieved_TFLOPs": flops / (xs_ms * 1e-3) / 1e12,
@@ -686,11 +688,6 @@ def run_sab(args):
     r_env = measure(envelope)
     progress(f"C3 uniform-grid envelope sampler: {r_env['ms']:.3f} ms per step")
 
-    if args.copy_calib:
-        # plain copy of 2*8*n bytes read + written: counter calibration for traffic
-        C.call("ncamd_memcpy_dtod", oe.ptr, ekin.ptr, 8 * n, stream.ptr)
-        stream.sync()
-
     threads = cpu_threads(args)
     ks = None
     if args.ks_samples > 0:
@@ -791,6 +788,18 @@ def run_vdos(args):
     }
 
 
+def copy_calibration(nbytes=800_000_000):
+    """one plain device-to-device copy of nbytes (read nbytes, write nbytes):
+    the reference launch that scales the FETCH_SIZE / WRITE_SIZE counters of
+    a profiling pass of this process"""
+    from ncrystal_amd import _capi as C
+    stream = C.Stream()
+    src, dst = C.DeviceArray(nbytes // 8), C.DeviceArray(nbytes // 8)
+    C.call("ncamd_memcpy_dtod", dst.ptr, src.ptr, nbytes, stream.ptr)
+    stream.sync()
+    progress(f"copy calibration: {nbytes} bytes device to device")
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -798,6 +807,8 @@ def main():
         sys.exit(2)
     runners = {"sab": run_sab, "c1": run_c1, "powderbragg": run_powderbragg, "batch": run_batch,
                "scbragg": run_scbragg, "vdos": run_vdos}
+    if args.copy_calib:
+        copy_calibration()
     if args.workload != "all":
         out = runners[args.workload](args)
         out.update(n_gpus=1, host=host_info(), library=library_info())

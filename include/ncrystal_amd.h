@@ -202,6 +202,17 @@ int ncamd_vdos_regularise(int n_egrid, const double* egrid, int n_density, const
 int ncamd_vdos_analyse(int n_egrid, const double* egrid, int n_density, const double* density, double temperature,
                        double mass_amu, double* out4);
 
+/* VDOSGn (ncrystal_core/src/vdos/NCVDOSGn.cc): the spectrum G_n of phonon
+ * order n (1..100000) at temperature [K] for an element of mass [amu], on
+ * the grid lo + i * binwidth [eV].  Returns the number of points; with out of
+ * at least that capacity also writes the values.  G_1 from VDOSEval, each
+ * higher order as the FFT convolution of orders n - n/2 and n/2 with the
+ * reference's truncation and thinning; transforms of 32768 points or more run
+ * on the GPU with the host's arithmetic (bit-identical results), orders of one
+ * generation (2^k+1..2^(k+1)) on host threads.  -1 on error. */
+int ncamd_vdos_gn(int n_egrid, const double* egrid, int n_density, const double* density, double temperature,
+                  double mass_amu, int order, double* lo, double* binwidth, double* out, int capacity);
+
 /* createScatteringKernel (ncrystal_core/src/vdos/NCVDOSToScatKnl.cc:787-944)
  * with the default truncation and thinning of the phonon-order functions:
  * the orders needed to reach the kinematic curve of target_emax [eV] (0: the
@@ -350,11 +361,17 @@ int ncamd_debug_gemm_bf16(const float* A, const float* B, float* C, int M, int N
 /* outputs [n*48] at the given energies */
 int ncamd_mlp_forward(ncamd_mlp_t, const double* ekin, int n, float* out);
 /* mean mixture negative log-likelihood of targets (w, kappa) and its
- * gradient with respect to all parameters [ncamd_mlp_nparams()] */
+ * gradient with respect to all parameters [ncamd_mlp_nparams()], from the
+ * fused training-step kernel without its update */
 double ncamd_mlp_loss_grad(ncamd_mlp_t, const double* ekin, const double* w, const double* kappa, int n,
                            float* grad);
+/* one fused training step (forward, likelihood, backward, Adam with step
+ * size lr) on the given batch; returns its mean loss */
+double ncamd_mlp_train_step(ncamd_mlp_t, const double* ekin, const double* w, const double* kappa, int n,
+                            double lr);
 /* trains on fresh batches drawn by the plain sampler of an S(alpha,beta)
- * process; returns the mean loss over the last 10% of the steps */
+ * process, one fused kernel per step; returns the mean loss over the last
+ * 10% of the steps */
 double ncamd_mlp_train_sab(ncamd_mlp_t, ncamd_process_t sab, int steps, int batch, double lr, uint64_t seed);
 
 /* ---------- learned-proposal sampler of S(alpha,beta) (no counterpart in

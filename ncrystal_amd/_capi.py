@@ -85,6 +85,8 @@ def lib():
         "ncamd_vdos_regularise": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]),
         "ncamd_vdos_analyse": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
                                               _dp]),
+        "ncamd_vdos_gn": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
         "ncamd_vdos_kernel": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double, _ip, _ip, _dp, _dp, _dp, _dp]),
         "ncamd_create_sabscatter_from_vdos": (Handle, [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_double,
@@ -158,6 +160,7 @@ def lib():
         "ncamd_mlp_set_params": (ctypes.c_int, [MlpHandle, _fp]),
         "ncamd_mlp_forward": (ctypes.c_int, [MlpHandle, _dp, ctypes.c_int, _fp]),
         "ncamd_mlp_loss_grad": (ctypes.c_double, [MlpHandle, _dp, _dp, _dp, ctypes.c_int, _fp]),
+        "ncamd_mlp_train_step": (ctypes.c_double, [MlpHandle, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double]),
         "ncamd_mlp_train_sab": (ctypes.c_double, [MlpHandle, Handle, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                   ctypes.c_uint64]),
         "ncamd_debug_gemm_bf16": (ctypes.c_int, [_fp, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -458,6 +458,19 @@ def vdos_analyse(vdos, temperature, mass_amu):
     return dict(gamma0=out[0], msd=out[1], teff=out[2], integral=out[3])
 
 
+def vdos_gn(vdos, temperature, mass_amu, order):
+    """(egrid, spectrum) of the phonon-order-n function G_n of the expansion"""
+    ne, eg, nd, d, keep = _vdos_args(vdos)
+    dp = C.dptr(d) if d is not None else None
+    lo, bw = ctypes.c_double(), ctypes.c_double()
+    args = (ne, C.dptr(eg), nd, dp, float(temperature), float(mass_amu), int(order), ctypes.byref(lo),
+            ctypes.byref(bw))
+    n = C.call("ncamd_vdos_gn", *args, None, 0)
+    spec = np.empty(n)
+    C.call("ncamd_vdos_gn", *args, C.dptr(spec), n)
+    return lo.value + bw.value * np.arange(n), spec
+
+
 def vdos_kernel(vdos, temperature, mass_amu, vdoslux=3, target_emax=0.0):
     """createScatteringKernel on the GPU: dict(alpha, beta, sab[nb][na],
     suggested_emax, msd, gamma0, norders, orders_summed, rows_summed, gpu_ms)"""
@@ -604,6 +617,10 @@ class MLPProposal:
         g = np.empty(self.nparams(), dtype=np.float32)
         loss = C.call("ncamd_mlp_loss_grad", self._h, C.dptr(e), C.dptr(ww), C.dptr(kk), e.size, C.fptr(g))
         return loss, g
+
+    def train_step(self, ekin, w, kappa, lr):
+        e, ww, kk = C.as_f64(ekin), C.as_f64(w), C.as_f64(kappa)
+        return C.call("ncamd_mlp_train_step", self._h, C.dptr(e), C.dptr(ww), C.dptr(kk), e.size, float(lr))
 
     def train_sab(self, sab, steps=2000, batch=8192, lr=3e-3, seed=1):
         return C.call("ncamd_mlp_train_sab", self._h, sab.handle, int(steps), int(batch), float(lr), int(seed))

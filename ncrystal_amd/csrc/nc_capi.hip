@@ -2046,13 +2046,10 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
 
 // --------------------------------------------------------- MLP proposal ---
 // Host side of the MLP (kernels in nc_mlp.hip): parameters, Adam state and
-// activation buffers live on the device; one training step = sample a batch
-// with the plain S(alpha,beta) sampler, forward, mixture NLL, backward, Adam.
+// activation buffers live on the device.  Training draws the samples of a
+// run of steps with the plain S(alpha,beta) sampler in one launch, then runs
+// one fused kernel per step (features, forward, mixture NLL, backward, Adam).
 constexpr uint32_t kMlpMagic = 0x6e636d6cU;
-constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kMlpIn * kMlpHidden, kOffW2 = kOffB1 + kMlpHidden,
-              kOffB2 = kOffW2 + kMlpHidden * kMlpHidden, kOffW3 = kOffB2 + kMlpHidden,
-              kOffB3 = kOffW3 + kMlpHidden * kMlpHidden, kOffW4 = kOffB3 + kMlpHidden,
-              kOffB4 = kOffW4 + kMlpHidden * kMlpOut, kMlpParams = kOffB4 + kMlpOut;
 
 struct Mlp {
   uint32_t magic = kMlpMagic;
@@ -2060,8 +2057,11 @@ struct Mlp {
   int step = 0;
   double lelo = 0.0, linv = 1.0;  // feature scaling: x = (log E - lelo) * linv
   int cap = 0;
-  DevBuf X, T, Z1, H1, Z2, H2, Z3, H3, Y, dY, dA, dB;
-  DevBuf E, a, b;
+  DevBuf X, T, Z1, H1, Z2, H2, Z3, H3, Y;
+  DevBuf E;
+  DevBuf counter;               // finished workgroups of a fused step
+  DevBuf tE, ta, tb, tloss;     // training samples of a run of steps, loss per step
+  size_t tcap = 0, lcap = 0;
   ~Mlp() { magic = 0; }
   void ensure(int n) {
     if (n <= cap) return;
@@ -2071,9 +2071,18 @@ struct Mlp {
     Z1.alloc(f * n * kMlpHidden); H1.alloc(f * n * kMlpHidden);
     Z2.alloc(f * n * kMlpHidden); H2.alloc(f * n * kMlpHidden);
     Z3.alloc(f * n * kMlpHidden); H3.alloc(f * n * kMlpHidden);
-    Y.alloc(f * n * kMlpOut); dY.alloc(f * n * kMlpOut);
-    dA.alloc(f * n * kMlpHidden); dB.alloc(f * n * kMlpHidden);
-    E.alloc(sizeof(double) * n); a.alloc(sizeof(double) * n); b.alloc(sizeof(double) * n);
+    Y.alloc(f * n * kMlpOut);
+    E.alloc(sizeof(double) * n);
+  }
+  void ensure_train(size_t nsamp, size_t steps) {
+    if (nsamp > tcap) {
+      tE.alloc(sizeof(double) * nsamp); ta.alloc(sizeof(double) * nsamp); tb.alloc(sizeof(double) * nsamp);
+      tcap = nsamp;
+    }
+    if (steps > lcap) {
+      tloss.alloc(sizeof(float) * steps);
+      lcap = steps;
+    }
   }
   float* P(int off) { return w.as<float>() + off; }
   float* G(int off) { return g.as<float>() + off; }
@@ -2099,36 +2108,6 @@ void mlp_forward(Mlp* m, int n, hipStream_t st) {
   launch_silu(m->Z3.as<float>(), m->H3.as<float>(), (int64_t)n * H, st);
   launch_gemm_bf16(m->H3.as<float>(), H, false, m->P(kOffW4), kMlpOut, false, m->Y.as<float>(), kMlpOut,
                    m->P(kOffB4), n, kMlpOut, H, false, st);
-}
-
-// loss and gradients (into m->g) for targets m->T
-void mlp_loss_backward(Mlp* m, int n, hipStream_t st) {
-  const int H = kMlpHidden;
-  launch_fill_f32(m->g.as<float>(), kMlpParams, 0.f, st);
-  launch_fill_f32(m->loss.as<float>(), 1, 0.f, st);
-  launch_mix_loss(m->Y.as<float>(), m->T.as<float>(), n, m->dY.as<float>(), m->loss.as<float>(), st);
-  float* dY = m->dY.as<float>();
-  float* dA = m->dA.as<float>();
-  float* dB = m->dB.as<float>();
-  // layer 4
-  launch_gemm_bf16(m->H3.as<float>(), H, true, dY, kMlpOut, false, m->G(kOffW4), kMlpOut, nullptr, H, kMlpOut, n,
-                   true, st);
-  launch_colsum(dY, n, kMlpOut, m->G(kOffB4), st);
-  launch_gemm_bf16(dY, kMlpOut, false, m->P(kOffW4), kMlpOut, true, dA, H, nullptr, n, H, kMlpOut, false, st);
-  launch_silu_back(m->Z3.as<float>(), dA, (int64_t)n * H, st);
-  // layer 3
-  launch_gemm_bf16(m->H2.as<float>(), H, true, dA, H, false, m->G(kOffW3), H, nullptr, H, H, n, true, st);
-  launch_colsum(dA, n, H, m->G(kOffB3), st);
-  launch_gemm_bf16(dA, H, false, m->P(kOffW3), H, true, dB, H, nullptr, n, H, H, false, st);
-  launch_silu_back(m->Z2.as<float>(), dB, (int64_t)n * H, st);
-  // layer 2
-  launch_gemm_bf16(m->H1.as<float>(), H, true, dB, H, false, m->G(kOffW2), H, nullptr, H, H, n, true, st);
-  launch_colsum(dB, n, H, m->G(kOffB2), st);
-  launch_gemm_bf16(dB, H, false, m->P(kOffW2), H, true, dA, H, nullptr, n, H, H, false, st);
-  launch_silu_back(m->Z1.as<float>(), dA, (int64_t)n * H, st);
-  // layer 1
-  launch_gemm_bf16(m->X.as<float>(), kMlpIn, true, dA, H, false, m->G(kOffW1), H, nullptr, kMlpIn, H, n, true, st);
-  launch_colsum(dA, n, H, m->G(kOffB1), st);
 }
 
 Mlp* create_mlp(uint64_t seed, double emin, double emax) {
@@ -2165,6 +2144,9 @@ Mlp* create_mlp(uint64_t seed, double emin, double emax) {
   m->m.upload(z.data(), z.size());
   m->v.upload(z.data(), z.size());
   m->loss.alloc(sizeof(float));
+  m->counter.alloc(sizeof(unsigned));
+  NCAMD_HIP(hipMemset(m->counter.p, 0, sizeof(unsigned)));
+  NCAMD_HIP(hipMemset(m->g.p, 0, sizeof(float) * kMlpParams));
   return m.release();
 }
 
@@ -3366,10 +3348,8 @@ int ncamd_mlp_forward(ncamd_mlp_t h, const double* ekin, int n, float* out) {
   NCAMD_CATCH
   return -1;
 }
-double ncamd_mlp_loss_grad(ncamd_mlp_t h, const double* ekin, const double* w, const double* kappa, int n,
-                           float* grad) {
-  NCAMD_TRY
-  Mlp* m = extract_mlp(h);
+// features of ekin and targets (w, kappa) into m->X, m->T
+void mlp_batch(Mlp* m, const double* ekin, const double* w, const double* kappa, int n) {
   if (n < 1) raise("BadInput", "empty batch");
   m->ensure(n);
   std::vector<float> t(2 * (size_t)n);
@@ -3377,18 +3357,45 @@ double ncamd_mlp_loss_grad(ncamd_mlp_t h, const double* ekin, const double* w, c
   NCAMD_HIP(hipMemcpy(m->E.p, ekin, sizeof(double) * n, hipMemcpyHostToDevice));
   NCAMD_HIP(hipMemcpy(m->T.p, t.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
   launch_mlp_features(m->E.as<double>(), n, m->lelo, m->linv, m->X.as<float>(), 0);
-  mlp_forward(m, n, 0);
-  mlp_loss_backward(m, n, 0);
+}
+// one fused step on the batch in m->X, m->T; returns the mean loss
+float mlp_step_batch(Mlp* m, int n, bool update, double lr) {
+  NCAMD_HIP(hipMemset(m->loss.p, 0, sizeof(float)));
+  if (!update) NCAMD_HIP(hipMemset(m->g.p, 0, sizeof(float) * kMlpParams));
+  if (update) ++m->step;
+  launch_mlp_step(m->w.as<float>(), m->g.as<float>(), m->m.as<float>(), m->v.as<float>(), nullptr, nullptr, nullptr,
+                  m->X.as<float>(), m->T.as<float>(), n, 1.0, 0.0, m->lelo, m->linv, m->loss.as<float>(),
+                  static_cast<unsigned*>(m->counter.p), update, (float)lr, m->step, 0);
   NCAMD_HIP(hipGetLastError());
   float loss = 0.f;
   NCAMD_HIP(hipMemcpy(&loss, m->loss.p, sizeof(float), hipMemcpyDeviceToHost));
+  return loss;
+}
+double ncamd_mlp_loss_grad(ncamd_mlp_t h, const double* ekin, const double* w, const double* kappa, int n,
+                           float* grad) {
+  NCAMD_TRY
+  Mlp* m = extract_mlp(h);
+  mlp_batch(m, ekin, w, kappa, n);
+  const float loss = mlp_step_batch(m, n, false, 0.0);
   if (grad) NCAMD_HIP(hipMemcpy(grad, m->g.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+  NCAMD_HIP(hipMemset(m->g.p, 0, sizeof(float) * kMlpParams));
   return loss;
   NCAMD_CATCH
   return -1.0;
 }
+double ncamd_mlp_train_step(ncamd_mlp_t h, const double* ekin, const double* w, const double* kappa, int n,
+                            double lr) {
+  NCAMD_TRY
+  Mlp* m = extract_mlp(h);
+  if (!(lr > 0.0)) raise("BadInput", "invalid learning rate");
+  mlp_batch(m, ekin, w, kappa, n);
+  return mlp_step_batch(m, n, true, lr);
+  NCAMD_CATCH
+  return -1.0;
+}
 // Trains on fresh batches of the plain S(alpha,beta) sampler (energies
-// log-uniform over the model's range).  Returns the mean loss of the last
+// log-uniform over the model's range), drawn for runs of up to kRun steps in
+// one launch; one fused kernel per step.  Returns the mean loss of the last
 // 10% of the steps.
 double ncamd_mlp_train_sab(ncamd_mlp_t h, ncamd_process_t sab, int steps, int batch, double lr, uint64_t seed) {
   NCAMD_TRY
@@ -3396,33 +3403,35 @@ double ncamd_mlp_train_sab(ncamd_mlp_t h, ncamd_process_t sab, int steps, int ba
   Proc* p = extract(sab);
   if (p->kind != kKindSAB) raise("BadInput", "training needs an S(alpha,beta) process");
   if (steps < 1 || batch < 64 || !(lr > 0.0)) raise("BadInput", "invalid training parameters");
-  m->ensure(batch);
-  if (!p->q_count.p) p->q_count.alloc(2 * sizeof(unsigned long long));
+  constexpr int kRun = 64;
+  const int run = std::min(steps, kRun);
+  m->ensure_train((size_t)run * batch, (size_t)steps);
   const double emin = std::exp(m->lelo), emax = std::exp(m->lelo + 1.0 / m->linv);
   const double bmax = p->sab_beta_h.back();
-  double acc = 0.0;
-  int nacc = 0;
   hipStream_t st = 0;
-  for (int it = 0; it < steps; ++it) {
-    launch_fill_loguniform(m->E.as<double>(), batch, std::log(emin), std::log(emax), seed + 7919ull * it, st);
-    launch_sab_ab(p->sab, m->E.as<double>(), batch, seed ^ 0x5ab5ab5abull, (uint32_t)it, m->a.as<double>(),
-                  m->b.as<double>(), p->d_err.as<unsigned long long>(), st);
-    launch_mlp_targets(m->E.as<double>(), m->a.as<double>(), m->b.as<double>(), batch, p->sab.kT, bmax, m->lelo,
-                       m->linv, m->X.as<float>(), m->T.as<float>(), st);
-    mlp_forward(m, batch, st);
-    mlp_loss_backward(m, batch, st);
-    ++m->step;
-    launch_adam(m->w.as<float>(), m->g.as<float>(), m->m.as<float>(), m->v.as<float>(), kMlpParams, (float)lr,
-                m->step, st);
-    if (it >= steps - std::max(1, steps / 10)) {
-      float l = 0.f;
-      NCAMD_HIP(hipMemcpy(&l, m->loss.p, sizeof(float), hipMemcpyDeviceToHost));
-      acc += l;
-      ++nacc;
+  NCAMD_HIP(hipMemsetAsync(m->tloss.p, 0, sizeof(float) * steps, st));
+  for (int it0 = 0, r = 0; it0 < steps; it0 += run, ++r) {
+    const int k = std::min(run, steps - it0);
+    const int64_t ns = (int64_t)k * batch;
+    launch_fill_loguniform(m->tE.as<double>(), ns, std::log(emin), std::log(emax), seed + 7919ull * r, st);
+    launch_sab_ab(p->sab, m->tE.as<double>(), ns, seed ^ 0x5ab5ab5abull, (uint32_t)r, m->ta.as<double>(),
+                  m->tb.as<double>(), p->d_err.as<unsigned long long>(), st);
+    for (int i = 0; i < k; ++i) {
+      const size_t off = (size_t)i * batch;
+      ++m->step;
+      launch_mlp_step(m->w.as<float>(), m->g.as<float>(), m->m.as<float>(), m->v.as<float>(),
+                      m->tE.as<double>() + off, m->ta.as<double>() + off, m->tb.as<double>() + off, nullptr, nullptr,
+                      batch, p->sab.kT, bmax, m->lelo, m->linv, m->tloss.as<float>() + it0 + i,
+                      static_cast<unsigned*>(m->counter.p), true, (float)lr, m->step, st);
     }
   }
   NCAMD_HIP(hipGetLastError());
+  std::vector<float> losses((size_t)steps);
+  NCAMD_HIP(hipMemcpyAsync(losses.data(), m->tloss.p, sizeof(float) * steps, hipMemcpyDeviceToHost, st));
   NCAMD_HIP(hipStreamSynchronize(st));
+  const int nacc = std::max(1, steps / 10);
+  double acc = 0.0;
+  for (int i = steps - nacc; i < steps; ++i) acc += losses[(size_t)i];
   return acc / nacc;
   NCAMD_CATCH
   return -1.0;
@@ -3585,6 +3594,21 @@ int ncamd_vdos_analyse(int n_egrid, const double* egrid, int n_density, const do
   out4[2] = a.teff;
   out4[3] = a.integral;
   return 0;
+  NCAMD_VDOS_CATCH
+  return -1;
+}
+
+int ncamd_vdos_gn(int n_egrid, const double* egrid, int n_density, const double* density, double temperature,
+                  double mass_amu, int order, double* lo, double* binwidth, double* out, int capacity) {
+  NCAMD_TRY
+  ensure_device();
+  const RegularVdos v = vdos_input(n_egrid, egrid, n_density, density);
+  double l = 0.0, bw = 0.0;
+  const std::vector<double> g = vdos_gn(v, temperature, mass_amu, (unsigned)std::max(order, 0), l, bw, nullptr);
+  if (lo) *lo = l;
+  if (binwidth) *binwidth = bw;
+  if (out && capacity >= (int)g.size()) std::copy(g.begin(), g.end(), out);
+  return (int)g.size();
   NCAMD_VDOS_CATCH
   return -1;
 }

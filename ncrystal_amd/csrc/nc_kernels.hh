@@ -222,14 +222,20 @@ constexpr int kMlpIn = 8, kMlpHidden = 64, kMlpComp = 8, kMlpOut = 6 * kMlpComp;
 void launch_gemm_bf16(const float* A, int lda, bool ta, const float* B, int ldb, bool tb, float* C, int ldc,
                       const float* bias, int M, int N, int K, bool accum, hipStream_t);
 void launch_silu(const float* z, float* h, int64_t n, hipStream_t);
-void launch_silu_back(const float* z, float* d, int64_t n, hipStream_t);
-void launch_fill_f32(float* p, int64_t n, float v, hipStream_t);
-void launch_colsum(const float* D, int M, int N, float* out, hipStream_t);
 void launch_mlp_features(const double* E, int n, double lelo, double linv, float* X, hipStream_t);
-void launch_mix_loss(const float* Y, const float* T, int n, float* dY, float* loss, hipStream_t);
-void launch_adam(float* w, const float* g, float* m, float* v, int n, float lr, int step, hipStream_t);
-void launch_mlp_targets(const double* E, const double* a, const double* b, int n, double kT, double bmax,
-                        double lelo, double linv, float* X, float* T, hipStream_t);
+// parameter offsets of the MLP (FP32 master copy): W1 [in][hidden], b1, W2,
+// b2, W3, b3, W4 [hidden][out], b4
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kMlpIn * kMlpHidden, kOffW2 = kOffB1 + kMlpHidden,
+              kOffB2 = kOffW2 + kMlpHidden * kMlpHidden, kOffW3 = kOffB2 + kMlpHidden,
+              kOffB3 = kOffW3 + kMlpHidden * kMlpHidden, kOffW4 = kOffB3 + kMlpHidden,
+              kOffB4 = kOffW4 + kMlpHidden * kMlpOut, kMlpParams = kOffB4 + kMlpOut;
+// one fused training step over n samples: either (E, alpha, beta) of the
+// plain sampler or precomputed features X and targets T (E null); adds the
+// step's mean loss to *loss, leaves the gradient in grad (update false) or
+// applies Adam step `step` and clears it (update true)
+void launch_mlp_step(float* w, float* grad, float* m, float* v, const double* E, const double* sa, const double* sb,
+                     const float* X, const float* T, int n, double kT, double bmax, double lelo, double linv,
+                     float* loss, unsigned* counter, bool update, float lr, int step, hipStream_t st);
 
 // many materials in one persistent launch (nc_batch.hip)
 struct BatchMat {

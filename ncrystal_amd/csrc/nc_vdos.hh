@@ -39,4 +39,9 @@ struct VdosKernel {
 VdosKernel vdos_kernel(const RegularVdos&, double temperature, double mass_amu, unsigned vdoslux,
                        double target_emax, hipStream_t stream);
 
+// the phonon-order-n spectrum G_n of the expansion (VDOSGn): values on the
+// grid lo + i * binwidth
+std::vector<double> vdos_gn(const RegularVdos&, double temperature, double mass_amu, unsigned order, double& lo,
+                            double& binwidth, hipStream_t stream);
+
 }  // namespace ncamd

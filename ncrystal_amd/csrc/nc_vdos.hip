@@ -19,6 +19,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <exception>
+#include <memory>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -43,6 +47,12 @@ constexpr double kInvE = 0.367879441171442321595523770161460867445811131;
 constexpr double k2Pi = 6.2831853071795864769252867665590057683943388;
 
 [[noreturn]] void fail(const char* type, const std::string& msg) { throw VdosError(type, msg); }
+
+#define VDOS_HIP(call)                                                                          \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) fail("HIPError", std::string(#call) + ": " + hipGetErrorString(e_));  \
+  } while (0)
 
 struct Sum {
   double s = 0.0, c = 0.0;
@@ -355,9 +365,87 @@ struct Eval {
 // y = dt |IFFT(FFT(a1) FFT(a2))| over the next power of two >= n1 + n2 - 1
 // (FastConvolve::convolve, NCFastConvolve.cc:224-268): radix-2, decimation
 // in time, twiddles exp(+-2 pi i j / N) from libm.
+// Device radix-2 FFT for the large convolutions (the low orders, whose
+// spectra have tens of thousands of points).  Every butterfly performs the
+// host loop's multiplications, additions and subtractions in the same order
+// with contraction into fused multiply-adds disabled, on the same twiddles,
+// and the bit-reversal permutation moves values without arithmetic, so the
+// device transform is bit-identical to the host one.  Layout of the work
+// buffer: re1 | im1 | re2 | im2, n values each.
+__global__ void k_fft_load(const double* __restrict__ in, int m1, int m2, int n, int lg, double* __restrict__ buf) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int r = (int)(__builtin_bitreverse32((unsigned)j) >> (32 - lg));
+  buf[r] = j < m1 ? in[j] : 0.0;
+  buf[n + r] = 0.0;
+  buf[2 * n + r] = j < m2 ? in[m1 + j] : 0.0;
+  buf[3 * n + r] = 0.0;
+}
+
+// bit-reversed copy of (re1, im1) into (re2, im2): the inverse transform's
+// permutation
+__global__ void k_fft_perm(double* __restrict__ buf, int n, int lg) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int r = (int)(__builtin_bitreverse32((unsigned)j) >> (32 - lg));
+  buf[2 * n + r] = buf[j];
+  buf[3 * n + r] = buf[n + j];
+}
+
+// one stage (blocks of 2*half points) on array pair blockIdx.y (re at
+// buf + 2*y*n, im after it), or on pair `pair` when gridDim.y is 1
+__global__ void k_fft_stage(double* __restrict__ buf, int n, int lhalf, int step, const double* __restrict__ wc,
+                            const double* __restrict__ ws, int forward, int pair) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (n >> 1)) return;
+  const int y = gridDim.y > 1 ? (int)blockIdx.y : pair;
+  double* re = buf + (size_t)2 * y * n;
+  double* im = re + n;
+  const int half = 1 << lhalf;
+  const int j = t & (half - 1);
+  const int p = ((t >> lhalf) << (lhalf + 1)) + j, q = p + half;
+  const double c = wc[j * step];
+  const double sn = forward ? ws[j * step] : -ws[j * step];
+  const double rq = re[q], iq = im[q], rp = re[p], ip = im[p];
+  const double tr = rq * c - iq * sn;
+  const double ti = rq * sn + iq * c;
+  re[q] = rp - tr;
+  im[q] = ip - ti;
+  re[p] = rp + tr;
+  im[p] = ip + ti;
+}
+
+__global__ void k_fft_mul(double* __restrict__ buf, int n) {
+#pragma clang fp contract(off)
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double a = buf[j], bb = buf[n + j], c = buf[2 * n + j], d = buf[3 * n + j];
+  buf[j] = a * c - bb * d;
+  buf[n + j] = a * d + bb * c;
+}
+
+// FFT convolution of two sampled spectra (the reference's FFT-based
+// convolution as used by NCVDOSGn.cc): zero-padded to a power of two, two
+// forward transforms, pointwise product, inverse transform, magnitude times
+// dt / n.  Transforms of kDevFFTMin points or more run on the device.
 struct Convolver {
+  static constexpr int kDevFFTMin = 1 << 15;
+  hipStream_t st = nullptr;
   int logn = -1;
   std::vector<double> wc, ws, b;
+  // device workspace: twiddles (wc | ws), work buffer, inputs
+  double *d_tw = nullptr, *d_buf = nullptr, *d_in = nullptr;
+  int d_logn = -1;
+  size_t d_in_cap = 0, d_buf_cap = 0;
+  Convolver() = default;
+  Convolver(const Convolver&) = delete;
+  Convolver& operator=(const Convolver&) = delete;
+  ~Convolver() {
+    if (d_tw) (void)hipFree(d_tw);
+    if (d_buf) (void)hipFree(d_buf);
+    if (d_in) (void)hipFree(d_in);
+  }
   void fft(double* re, double* im, int lg, bool forward) const {
     const int n = 1 << lg;
     for (int j = 1, i = 0; j < n; ++j) {
@@ -382,38 +470,95 @@ struct Convolver {
         }
     }
   }
+  void twiddles(int lg) {
+    if (lg == logn) return;
+    const int n = 1 << lg;
+    logn = lg;
+    wc.resize(n / 2 + 1);
+    ws.resize(n / 2 + 1);
+    // separate libm cos and sin calls (through volatile pointers, which no
+    // compiler fuses into sincos: that routine can differ from them in the
+    // last bit, and the truncation edges of the G_n depend on such bits)
+    double (*volatile fcos)(double) = ::cos;
+    double (*volatile fsin)(double) = ::sin;
+    for (int j = 0; j <= n / 2; ++j) {
+      const double ang = k2Pi * (double)j / (double)n;
+      wc[j] = fcos(ang);
+      ws[j] = fsin(ang);
+    }
+  }
+  // transformed product on the device; leaves re, im of the inverse
+  // transform's first m points in b[0, m) and b[n, n + m)
+  void device_product(const std::vector<double>& a1, const std::vector<double>& a2, int lg) {
+    const int n = 1 << lg, half = n / 2;
+    const size_t nin = a1.size() + a2.size();
+    if (nin > d_in_cap) {
+      if (d_in) (void)hipFree(d_in);
+      d_in = nullptr;
+      VDOS_HIP(hipMalloc(&d_in, sizeof(double) * nin));
+      d_in_cap = nin;
+    }
+    if ((size_t)4 * n > d_buf_cap) {
+      if (d_buf) (void)hipFree(d_buf);
+      d_buf = nullptr;
+      VDOS_HIP(hipMalloc(&d_buf, sizeof(double) * 4 * (size_t)n));
+      d_buf_cap = (size_t)4 * n;
+    }
+    if (d_logn != lg) {
+      if (d_tw) (void)hipFree(d_tw);
+      d_tw = nullptr;
+      VDOS_HIP(hipMalloc(&d_tw, sizeof(double) * 2 * (size_t)(half + 1)));
+      VDOS_HIP(hipMemcpyAsync(d_tw, wc.data(), sizeof(double) * (half + 1), hipMemcpyHostToDevice, st));
+      VDOS_HIP(hipMemcpyAsync(d_tw + half + 1, ws.data(), sizeof(double) * (half + 1), hipMemcpyHostToDevice, st));
+      d_logn = lg;
+    }
+    const int m1 = (int)a1.size(), m2 = (int)a2.size();
+    VDOS_HIP(hipMemcpyAsync(d_in, a1.data(), sizeof(double) * m1, hipMemcpyHostToDevice, st));
+    VDOS_HIP(hipMemcpyAsync(d_in + m1, a2.data(), sizeof(double) * m2, hipMemcpyHostToDevice, st));
+    const double* dwc = d_tw;
+    const double* dws = d_tw + half + 1;
+    const unsigned gn = (unsigned)((n + 255) / 256), gh = (unsigned)((half + 255) / 256);
+    hipLaunchKernelGGL(k_fft_load, dim3(gn), dim3(256), 0, st, d_in, m1, m2, n, lg, d_buf);
+    for (int l = 0; l < lg; ++l)
+      hipLaunchKernelGGL(k_fft_stage, dim3(gh, 2), dim3(256), 0, st, d_buf, n, l, n >> (l + 1), dwc, dws, 1, 0);
+    hipLaunchKernelGGL(k_fft_mul, dim3(gn), dim3(256), 0, st, d_buf, n);
+    hipLaunchKernelGGL(k_fft_perm, dim3(gn), dim3(256), 0, st, d_buf, n, lg);
+    for (int l = 0; l < lg; ++l)
+      hipLaunchKernelGGL(k_fft_stage, dim3(gh, 1), dim3(256), 0, st, d_buf, n, l, n >> (l + 1), dwc, dws, 0, 1);
+    VDOS_HIP(hipGetLastError());
+    const int m = m1 + m2 - 1;
+    b.resize((size_t)n * 2);
+    VDOS_HIP(hipMemcpyAsync(b.data(), d_buf + 2 * (size_t)n, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+    VDOS_HIP(hipMemcpyAsync(b.data() + n, d_buf + 3 * (size_t)n, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+    VDOS_HIP(hipStreamSynchronize(st));
+  }
   std::vector<double> convolve(const std::vector<double>& a1, const std::vector<double>& a2, double dt) {
     const int m = (int)(a1.size() + a2.size() - 1);
     int lg = 0;
     while ((1 << lg) < m) ++lg;
     const int n = 1 << lg;
-    if (lg != logn) {
-      logn = lg;
-      wc.resize(n / 2 + 1);
-      ws.resize(n / 2 + 1);
-      // separate libm cos and sin calls (through volatile pointers, which no
-      // compiler fuses into sincos: that routine can differ from them in the
-      // last bit, and the truncation edges of the G_n depend on such bits)
-      double (*volatile fcos)(double) = ::cos;
-      double (*volatile fsin)(double) = ::sin;
-      for (int j = 0; j <= n / 2; ++j) {
-        const double ang = k2Pi * (double)j / (double)n;
-        wc[j] = fcos(ang);
-        ws[j] = fsin(ang);
+    twiddles(lg);
+    double *r1, *i1;
+    if (n >= kDevFFTMin) {
+      device_product(a1, a2, lg);
+      r1 = b.data();
+      i1 = r1 + n;
+    } else {
+      b.assign((size_t)n * 4, 0.0);
+      r1 = b.data();
+      i1 = r1 + n;
+      double *r2 = r1 + 2 * n, *i2 = r1 + 3 * n;
+      std::copy(a1.begin(), a1.end(), r1);
+      std::copy(a2.begin(), a2.end(), r2);
+      fft(r1, i1, lg, true);
+      fft(r2, i2, lg, true);
+      for (int j = 0; j < n; ++j) {
+        const double a = r1[j], bb = i1[j], c = r2[j], d = i2[j];
+        r1[j] = a * c - bb * d;
+        i1[j] = a * d + bb * c;
       }
+      fft(r1, i1, lg, false);
     }
-    b.assign((size_t)n * 4, 0.0);
-    double *r1 = b.data(), *i1 = r1 + n, *r2 = r1 + 2 * n, *i2 = r1 + 3 * n;
-    std::copy(a1.begin(), a1.end(), r1);
-    std::copy(a2.begin(), a2.end(), r2);
-    fft(r1, i1, lg, true);
-    fft(r2, i2, lg, true);
-    for (int j = 0; j < n; ++j) {
-      const double a = r1[j], bb = i1[j], c = r2[j], d = i2[j];
-      r1[j] = a * c - bb * d;
-      i1[j] = a * d + bb * c;
-    }
-    fft(r1, i1, lg, false);
     std::vector<double> y(m);
     const double kf = dt / n;
     for (int j = 0; j < m; ++j) y[j] = std::sqrt(r1[j] * r1[j] + i1[j] * i1[j]) * kf;
@@ -457,6 +602,7 @@ std::vector<double> thin_vec(const std::vector<double>& v, unsigned long tf) {
 struct GnSet {
   std::vector<Gn> g;
   double kT;
+  unsigned cap = 0;
   Convolver cv;
   explicit GnSet(const Eval& ev) : kT(ev.kT) {
     unsigned long nbins = ev.nptsext - 1;
@@ -483,52 +629,94 @@ struct GnSet {
     }
     g.emplace_back(std::move(spec), lower, bw, 1);
   }
+  // order n = G_{n - n/2} convolved with G_{n/2}, thinned as the reference
+  // does (NCVDOSGn.cc): a function of the two lower orders only
+  Gn order_from(unsigned order, Convolver& conv) const {
+    const unsigned o2 = order / 2, o1 = order - o2;
+    const Gn& p1 = g[o1 - 1];
+    const Gn& p2 = g[o2 - 1];
+    unsigned long t1 = p1.thin, t2 = p2.thin;
+    double dt;
+    std::vector<double> tmp;
+    const std::vector<double>* in1 = &p1.s;
+    const std::vector<double>* in2 = &p2.s;
+    if (t1 == t2) {
+      dt = p1.bw;
+    } else {
+      dt = std::max(p1.bw, p2.bw);
+      if (t1 > t2) {
+        tmp = thin_vec(*in2, t1 / t2);
+        in2 = &tmp;
+        t2 *= t1 / t2;
+      } else {
+        tmp = thin_vec(*in1, t2 / t1);
+        in1 = &tmp;
+        t1 *= t2 / t1;
+      }
+    }
+    double start = p1.lo + p2.lo;
+    std::vector<double> y = conv.convolve(*in1, *in2, dt);
+    unsigned long xthin = 1;
+    if (order >= 5u) {
+      const double mx = *std::max_element(y.begin(), y.end());
+      const double cut = 1e-14 * mx;
+      size_t ifront = 0, iback = y.size() - 1;
+      for (; ifront < iback; ++ifront)
+        if (y[ifront] > cut) break;
+      for (; iback > ifront; --iback)
+        if (y[iback] > cut) break;
+      if (iback > ifront) y = std::vector<double>(y.begin() + ifront, y.begin() + iback + 1);
+      start += ifront * dt;
+      if (y.size() > 1000) {
+        while (y.size() > 1000ul * xthin) xthin *= 2;
+        if (xthin >= 8 && order <= 10u) xthin /= 2;
+        y = thin_vec(y, xthin);
+        dt *= xthin;
+      }
+    }
+    return Gn(std::move(y), start, dt, t1 * xthin);
+  }
+  // orders up to target (and, when cap is set, ahead up to cap).  The orders
+  // c+1..2c depend only on orders <= c, so each such generation is computed
+  // on host threads; every order comes from the same operations in the same
+  // sequence as a one-thread growth, so the results do not depend on the
+  // thread count
   void grow(unsigned target) {
     while (g.size() < target) {
-      const unsigned order = (unsigned)g.size() + 1;
-      const unsigned o2 = order / 2, o1 = order - o2;
-      const Gn& p1 = g[o1 - 1];
-      const Gn& p2 = g[o2 - 1];
-      unsigned long t1 = p1.thin, t2 = p2.thin;
-      double dt;
-      std::vector<double> tmp;
-      const std::vector<double>* in1 = &p1.s;
-      const std::vector<double>* in2 = &p2.s;
-      if (t1 == t2) {
-        dt = p1.bw;
-      } else {
-        dt = std::max(p1.bw, p2.bw);
-        if (t1 > t2) {
-          tmp = thin_vec(*in2, t1 / t2);
-          in2 = &tmp;
-          t2 *= t1 / t2;
-        } else {
-          tmp = thin_vec(*in1, t2 / t1);
-          in1 = &tmp;
-          t1 *= t2 / t1;
+      const unsigned c = (unsigned)g.size();
+      const unsigned hi = std::min(2 * c, std::max(target, cap));
+      const unsigned cnt = hi - c;
+      unsigned nthr = std::min<unsigned>(cnt, std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+      if (cnt < 3) nthr = 1;
+      if (nthr == 1) {
+        for (unsigned o = c + 1; o <= hi; ++o) g.push_back(order_from(o, cv));
+        continue;
+      }
+      std::vector<std::unique_ptr<Gn>> res(cnt);
+      std::atomic<unsigned> next{0};
+      std::vector<std::exception_ptr> errs(nthr);
+      auto work = [&](unsigned t) {
+        try {
+          Convolver conv;
+          conv.st = cv.st;
+          for (unsigned i = next++; i < cnt; i = next++) res[i].reset(new Gn(order_from(c + 1 + i, conv)));
+        } catch (...) {
+          errs[t] = std::current_exception();
+        }
+      };
+      std::vector<std::thread> pool;
+      for (unsigned t = 1; t < nthr; ++t) {
+        try {
+          pool.emplace_back(work, t);
+        } catch (...) {
+          break;  // fewer threads: the others take the remaining orders
         }
       }
-      double start = p1.lo + p2.lo;
-      std::vector<double> y = cv.convolve(*in1, *in2, dt);
-      unsigned long xthin = 1;
-      if (order >= 5u) {
-        const double mx = *std::max_element(y.begin(), y.end());
-        const double cut = 1e-14 * mx;
-        size_t ifront = 0, iback = y.size() - 1;
-        for (; ifront < iback; ++ifront)
-          if (y[ifront] > cut) break;
-        for (; iback > ifront; --iback)
-          if (y[iback] > cut) break;
-        if (iback > ifront) y = std::vector<double>(y.begin() + ifront, y.begin() + iback + 1);
-        start += ifront * dt;
-        if (y.size() > 1000) {
-          while (y.size() > 1000ul * xthin) xthin *= 2;
-          if (xthin >= 8 && order <= 10u) xthin /= 2;
-          y = thin_vec(y, xthin);
-          dt *= xthin;
-        }
-      }
-      g.emplace_back(std::move(y), start, dt, t1 * xthin);
+      work(0);
+      for (auto& th : pool) th.join();
+      for (auto& e : errs)
+        if (e) std::rethrow_exception(e);
+      for (auto& r : res) g.push_back(std::move(*r));
     }
   }
 };
@@ -782,11 +970,6 @@ double stirling9(double x) {
 }
 
 // ---------------------------------------------------------------- GPU fill
-#define VDOS_HIP(call)                                                                          \
-  do {                                                                                          \
-    hipError_t e_ = (call);                                                                     \
-    if (e_ != hipSuccess) fail("HIPError", std::string(#call) + ": " + hipGetErrorString(e_));  \
-  } while (0)
 
 struct GnDev {
   long long off;  // first value in the concatenated spectra
@@ -821,19 +1004,25 @@ __global__ void k_vdos_gtab(const GnDev* __restrict__ gd, const double* __restri
 
 // A[k][i] = x^n exp(-x)/n! kT for n = k+1, x = x_i: by recursion
 // f(x,n) = f(x,n-1) x/n below order 16 (half of exp(-x) applied at the end),
-// from Stirling's series for n! above (NCVDOSToScatKnl.cc:149-188)
+// from Stirling's series for n! above (NCVDOSToScatKnl.cc:149-188).  Thread
+// (i, c) fills orders 16c+1..16c+16 of alpha i: chunk 0 runs the recursion,
+// the others are independent per order.
 __global__ void k_vdos_afac(const double* __restrict__ x, const double* __restrict__ emh,
                             const double* __restrict__ lx, const double* __restrict__ ofact,
                             const double* __restrict__ ologn, double kT, int K, int N, int Kp,
                             double* __restrict__ A) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= N) return;
+  const int kb = (int)blockIdx.y * 16;
+  const int ke = kb + 16 < Kp ? kb + 16 : Kp;
   const double xi = x[i], eh = emh[i], li = lx[i];
   double fxn = eh;
-  for (int k = 0; k < K; ++k) {
+  for (int k = kb; k < ke; ++k) {
     const int n = k + 1;
-    double af;
-    if (n < 16) {
+    double af = 0.0;
+    if (k >= K) {
+      af = 0.0;
+    } else if (n < 16) {
       fxn *= xi * (1.0 / n);
       af = fxn * eh * kT;
     } else {
@@ -842,7 +1031,6 @@ __global__ void k_vdos_afac(const double* __restrict__ x, const double* __restri
     }
     A[(size_t)k * N + i] = af;
   }
-  for (int k = K; k < Kp; ++k) A[(size_t)k * N + i] = 0.0;
 }
 
 // S[j][i] = sum_k G[k][j] A[k][i] for the rows beta_j <= 0 on the FP64 matrix
@@ -904,6 +1092,19 @@ VdosAnalysis vdos_analyse(const RegularVdos& v, double temperature, double mass)
   return a;
 }
 
+std::vector<double> vdos_gn(const RegularVdos& v, double temperature, double mass, unsigned order, double& lo,
+                            double& binwidth, hipStream_t st) {
+  if (order < 1 || order > 100000) fail("BadInput", "order must be in 1..100000");
+  const Eval ev(v, temperature, mass);
+  GnSet gs(ev);
+  gs.cv.st = st;
+  gs.grow(order);
+  const Gn& g = gs.g[order - 1];
+  lo = g.lo;
+  binwidth = g.bw;
+  return g.s;
+}
+
 VdosKernel vdos_kernel(const RegularVdos& v, double temperature, double mass, unsigned lux, double target_emax_req,
                        hipStream_t st) {
   if (lux > 5) fail("BadInput", "vdoslux must be in 0..5");
@@ -917,10 +1118,14 @@ VdosKernel vdos_kernel(const RegularVdos& v, double temperature, double mass, un
   double tE = targetEmax * invkT;
   unsigned maxo = 4;
   GnSet gs(ev);
+  gs.cv.st = st;
   gs.grow(maxo);
   unsigned order_limit = 1000;
   if (target_emax_req > 0.0 || lux == 5) order_limit *= 10;
   if (lux == 0) order_limit /= 10;
+  // orders are grown a generation ahead (at most up to the first order past
+  // the limit); the ones beyond the final count are dropped below
+  gs.cap = order_limit + 1;
   const double emax_lowest = target_emax_req > 0.0 ? target_emax_req : 1e-15;
   const double rel = std::pow(10.0, -(3.0 + 2.0 * lux));
   const double x2a = 1.0 / alpha2x(kT, msd);
@@ -935,7 +1140,7 @@ VdosKernel vdos_kernel(const RegularVdos& v, double temperature, double mass, un
   while (true) {
     gs.grow(maxo);
     double a0, a1, b0, b1;
-    ranges((unsigned)gs.g.size(), a0, a1, b0, b1);
+    ranges(maxo, a0, a1, b0, b1);
     if (within_alphaplus(tE, a0, b1)) ++maxo;
     else break;
     if (maxo > order_limit) {
@@ -957,6 +1162,7 @@ VdosKernel vdos_kernel(const RegularVdos& v, double temperature, double mass, un
     }
   }
   gs.grow(maxo);
+  gs.g.erase(gs.g.begin() + maxo, gs.g.end());
   double betaMin = 0.0, alphaMax = 0.0;
   for (unsigned o = 1; o <= maxo; ++o) {
     double a0, a1, b0, b1, ea, eb;
@@ -1038,8 +1244,8 @@ VdosKernel vdos_kernel(const RegularVdos& v, double temperature, double mass, un
   const long long ng = (long long)Kp * M;
   hipLaunchKernelGGL(k_vdos_gtab, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, d_gd.p, d_spec.p, d_beta.p, kT,
                      K, M, Kp, d_G.p);
-  hipLaunchKernelGGL(k_vdos_afac, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, st, d_x.p, d_emh.p, d_lx.p, d_of.p,
-                     d_ol.p, kT, K, na, Kp, d_A.p);
+  hipLaunchKernelGGL(k_vdos_afac, dim3((unsigned)((na + 63) / 64), (unsigned)((Kp + 15) / 16)), dim3(64), 0, st, d_x.p,
+                     d_emh.p, d_lx.p, d_of.p, d_ol.p, kT, K, na, Kp, d_A.p);
   hipLaunchKernelGGL(k_vdos_fill, dim3((unsigned)((na + 31) / 32), (unsigned)((M + 31) / 32)), dim3(256), 0, st, d_G.p,
                      d_A.p, d_expb.p, M, na, Kp, (int)iz, (int)iflip, d_S.p);
   VDOS_HIP(hipGetLastError());

@@ -97,6 +97,27 @@ def test_kernel_matches_checker(amd, oracle, case):
     assert np.all(gs >= 0.0)
 
 
+@pytest.mark.parametrize("name,T,m", [("Al_sg225/Al", 293.15, 26.981538408), ("Polyethylene_CH2/H", 293.15, 1.00794),
+                                      ("debye", 300.0, 27.0)])
+def test_gn_bit_identical(amd, oracle, name, T, m):
+    """G_n of the library (large transforms on the GPU, generations of
+    orders on host threads) equal to the checker's one-thread host chain bit
+    for bit: the truncation and thinning edges of every higher order, and so
+    the number of orders and the grids, follow from these values"""
+    C, core = amd
+    vin = _case(name)
+    if np.isscalar(vin):
+        lo, hi, d = oracle.vdos_debye(vin)
+        lo, hi, d = oracle.vdos_regularise(np.array([lo, hi]), d)
+    else:
+        lo, hi, d = oracle.vdos_regularise(*vin)
+    for order in (1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 16, 17, 33, 40, 100):
+        eg, sg = core.vdos_gn(vin, T, m, order)
+        eo, so = oracle.vdos_gn(lo, hi, d, T, m, order)
+        print(f"{name} order {order}: {sg.size} points")
+        assert np.array_equal(eg, eo) and np.array_equal(sg, so), f"order {order}"
+
+
 def test_sabscatter_from_vdos_matches_checker(amd, oracle):
     C, core = amd
     vin = _case("Al_sg225/Al")
