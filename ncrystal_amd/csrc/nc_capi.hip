@@ -2015,7 +2015,13 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
       al[j].box = bx[j];
       al[j].alias = j;
     }
-    if (!(tot > 0.0) || !std::isfinite(tot)) continue;  // nothing to sample: every proposal is rejected -> fallback
+    if (!(tot > 0.0) || !std::isfinite(tot)) {
+      // nothing to sample (a zero bound would accept every proposal, u * 0
+      // <= 0): NaN bounds fail every comparison, so every proposal is
+      // rejected and the neutrons go to the plain sampler
+      for (int j = 0; j < k.nbox; ++j) al[j].box.env = al[j].box.tlow = __builtin_nanf("");
+      continue;
+    }
     small.clear(); large.clear();
     for (int j = 0; j < k.nbox; ++j) {
       q[(size_t)j] = bx[j].env * area(bx[j]) * k.nbox / tot;
