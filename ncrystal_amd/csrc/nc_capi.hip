@@ -1934,6 +1934,7 @@ void build_kb(Proc* p, int nn, double emin, double emax, int bw, int bk, int max
   k.nn = nn; k.bw = bw; k.bk = bk; k.nbox = bw * bk; k.squeeze = 1;
   const double lr = std::log(emax / emin) / nn;
   k.elo = emin; k.ehi = emax; k.lr_inv = 1.0 / lr;
+  k.inv_elo = 1.0 / emin; k.inv_kT = 1.0 / s.kT;
   std::vector<double> node((size_t)nn + 1);
   for (int i = 0; i <= nn; ++i) node[(size_t)i] = emin * std::exp(lr * i);
   node[0] = emin; node[(size_t)nn] = emax;

@@ -63,21 +63,24 @@ struct KBView {
   int bw, bk, nbox;                // boxes per interval: bw w-strips x bk boxes per strip
   int squeeze;                     // 1: accept on u * env <= tlow without evaluating the table
   double elo, ehi, lr_inv;         // interval index guess: floor(log(E/elo) * lr_inv)
+  double inv_elo, inv_kT;          // 1/elo (for the guess), 1/kT (E/kT of a neutron as E * inv_kT)
   const double* node_e;            // [nn+1] interval edges
   const double4* node_par;         // [nn]: (eps, sqrt(eps), wmax, 8 sqrt(eps) wmax^3)
   const KBSlot* slot;              // [nn*nbox] alias slots with their boxes, strip-major
 };
 
-// interpolate_loglin_fallbacklinlin_fast (NCSABUtils.hh:176-195)
-NCD double kb_loglin(double a, double fa, double b, double fb, double x, double lfa, double lfb) {
-  const double bma = b - a;
+// interpolate_loglin_fallbacklinlin_fast (NCSABUtils.hh:176-195), with the
+// reciprocal of the cell width given (one division per cell, not per
+// interpolation; the envelope's bounds carry float margins far above the
+// rounding this changes)
+NCD double kb_loglin(double a, double fa, double b, double fb, double x, double lfa, double lfb, double inv_bma) {
   const double mid = 0.5 * (b + a);
   const bool lin = (fa * fb == 0.0);
   if (x < mid) {
-    const double r = (x - a) / bma;
+    const double r = (x - a) * inv_bma;
     return lin ? (fa + (fb - fa) * r) : exp(lfa + (lfb - lfa) * r);
   }
-  const double sx = (b - x) / bma;
+  const double sx = (b - x) * inv_bma;
   return lin ? (fb + (fa - fb) * sx) : exp(lfb + (lfa - lfb) * sx);
 }
 
@@ -98,8 +101,9 @@ NCD double kb_eval_s(const SABView& s, double a, double b, uint32_t arange, uint
   const double2* r0 = s.sl + (size_t)ib * s.na + ia;
   const double2* r1 = r0 + s.na;
   const double2 c00 = r0[0], c10 = r0[1], c01 = r1[0], c11 = r1[1];
-  const double s0 = kb_loglin(a0, c00.x, a1, c10.x, a, c00.y, c10.y);
-  const double s1 = kb_loglin(a0, c01.x, a1, c11.x, a, c01.y, c11.y);
+  const double inv_da = 1.0 / (a1 - a0);
+  const double s0 = kb_loglin(a0, c00.x, a1, c10.x, a, c00.y, c10.y, inv_da);
+  const double s1 = kb_loglin(a0, c01.x, a1, c11.x, a, c01.y, c11.y, inv_da);
   return s0 + (s1 - s0) * ((b - b0) / (b1 - b0));
 }
 
@@ -107,7 +111,7 @@ NCD double kb_eval_s(const SABView& s, double a, double b, uint32_t arange, uint
 // index to within one; the edge comparisons make it exact.
 NCD int kb_interval(const KBView& k, double E) {
   if (!(E >= k.elo && E < k.ehi)) return -1;
-  int n = (int)(__logf((float)(E / k.elo)) * (float)k.lr_inv);
+  int n = (int)(__logf((float)(E * k.inv_elo)) * (float)k.lr_inv);
   n = n < 0 ? 0 : (n > k.nn - 1 ? k.nn - 1 : n);
   while (n > 0 && E < k.node_e[n]) --n;
   while (n < k.nn - 1 && !(E < k.node_e[n + 1])) ++n;

@@ -9,7 +9,8 @@ namespace ncamd {
 NCD double kb_row(const SABView& s, int ib, int ia, double a) {
   const double2* r = s.sl + (size_t)ib * s.na + ia;
   const double2 c0 = r[0], c1 = r[1];
-  return kb_loglin(s.alpha[ia], c0.x, s.alpha[ia + 1], c1.x, a, c0.y, c1.y);
+  const double a0 = s.alpha[ia], a1 = s.alpha[ia + 1];
+  return kb_loglin(a0, c0.x, a1, c1.x, a, c0.y, c1.y, 1.0 / (a1 - a0));
 }
 
 // lower index of the cell containing x (clamped into the grid)
@@ -181,7 +182,7 @@ __global__ void __launch_bounds__(256) k_kb_sample(SABView s, KBView k, const do
           if (iv < 0) {
             queue[atomicAdd(qcount, 1ull)] = kk;
           } else {
-            ediv = E / s.kT;
+            ediv = E * k.inv_kT;
             draw = 0;
             tries = 0;
             active = true;
