@@ -26,6 +26,11 @@ with its own roofline figure and CPU baseline:
     mosaicity FWHM 30 arcmin, seeded random orientation; 1e7 neutrons with
     wavelength ~ U(1,6) Aa and directions in a 2-degree cone about a seeded
     axis; one step = cross sections + scatter sampling.
+  F2: VDOS -> S(alpha,beta) expansion of the data library's aluminium VDOS at
+    vdoslux 5; one step = one whole expansion.
+  F3: MiniMC: 1e7 neutrons of a 4 Aa pencil beam into a sphere of Al_sg225
+    one scattering length across (forced collisions, absorption, roulette,
+    theta tally); one step = one whole engine call.
 
 `--workload` runs one of them alone.  Roofline figures come from HIP events
 on the launch stream; the CPU baseline is the CPU checker timed on a bounded
@@ -64,7 +69,7 @@ def parse():
                     help="also run one device-to-device copy of the output size (counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["all", "sab", "c1", "scbragg", "powderbragg", "batch", "vdos"],
+    ap.add_argument("--workload", choices=["all", "sab", "c1", "scbragg", "powderbragg", "batch", "vdos", "minimc"],
                     default="all")
     ap.add_argument("--sub-cpu-seconds", type=float, default=8.0,
                     help="CPU-baseline sample length of each sub-result")
@@ -788,6 +793,98 @@ def run_vdos(args):
     }
 
 
+def al_sg225_inputs():
+    """NCrystal's Al_sg225 (structure, scattering lengths, absorption, VDOS of
+    the data library) from tests/golden/vdos_reference.json"""
+    import json as _json
+    fix = _json.loads((ROOT / "tests" / "golden" / "vdos_reference.json").read_text())
+    return fix["basic"], fix["vdos"][fix["basic"]["vdos"]]
+
+
+def run_minimc(args):
+    """MiniMC: a pencil beam of 4 Aa neutrons into a sphere of NCrystal's
+    Al_sg225 one scattering length across (the reference's mmc_al test
+    geometry), forced-collision engine with absorption and roulette, theta
+    tally with the breakdown by scattering history; source neutrons per
+    second of whole engine calls; roofline of the engine's step kernel"""
+    import numpy as np
+    from ncrystal_amd import core
+    b, v = al_sg225_inputs()
+    T, mass, a = b["temperature"], b["mass"], b["a"]
+    vin = (np.array(v["egrid"]), np.array(v["density"]))
+    sigma_bound = 4 * math.pi * b["coh_sl"] ** 2 + b["sigma_inc"]
+    msd = core.vdos_analyse(vin, T, mass)["msd"]
+    hkl, mult, dsp, fsq, _, _ = core.hkl_spacegroup_members(b["spacegroup"], (a, a, a, 90, 90, 90),
+                                                             [(b["coh_sl"], msd, np.array(b["positions"]))],
+                                                             b["dcutoff"])
+    ei = core.create_elincscatter([msd], [b["sigma_inc"]], [1.0])
+    pb = core.create_powderbragg(a ** 3 * len(b["positions"]), dsp, fsq * mult)
+    inel = core.create_sabscatter_from_vdos(vin, T, mass, sigma_bound, vdoslux=3)
+    scatter = core.create_composition([ei, pb, inel], [1.0, 1.0, 1.0])
+    ab = core.create_absoov(b["sigma_abs"])
+    nd = len(b["positions"]) / a ** 3
+    ekin = 0.081804209605330899 / 4.0 ** 2
+    r = 0.5 / (nd * 100.0 * scatter.crossSectionIsotropic(np.array([ekin]))[0])
+    z = -r * (1 - 1e-13)
+    n = 10_000_000
+    steps = max(1, min(args.steps, 5))
+    for w in range(max(1, min(args.warmup, 2))):
+        core.minimc_sphere(scatter, ab, nd, r, ekin, n, pos=(0.0, 0.0, z), seed=100 + w)
+    t0 = time.perf_counter()
+    ms = nst = nsc = 0.0
+    for i in range(steps):
+        res = core.minimc_sphere(scatter, ab, nd, r, ekin, n, pos=(0.0, 0.0, z), seed=i)
+        ms += res["info"]["step_kernel_ms"]
+        nst += res["info"]["neutron_steps"]
+        nsc += res["info"]["scattered"]
+    wall = time.perf_counter() - t0
+    tot = res["total"]["content"].sum()
+    assert 0.5 * n < tot < n, "tallied weight out of range"
+    # step kernel: reads 80 B of neutron state + 16 B of cross sections per
+    # neutron and step, writes 80 B per neutron passed on to scatter
+    bytes_ = 96.0 * nst + 80.0 * nsc
+    ach = bytes_ / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pyoracle as po
+        import pymmc
+        po.build()
+        threads = cpu_threads(args)
+        lo, hi, d = po.vdos_regularise(*vin)
+        k = po.vdos_kernel(lo, hi, d, T, mass, vdoslux=3)
+        sf = po.sab_std_format(0, k["alpha"], k["beta"], k["sab"], beta_grid_optimised=True)
+        cms = po.vdos_analyse(lo, hi, d, T, mass)["msd"]
+        h2, m2, d2, f2, _, _ = po.hkl_spacegroup_members(b["spacegroup"], (a, a, a, 90, 90, 90),
+                                                         [(b["coh_sl"], cms, np.array(b["positions"]))], b["dcutoff"])
+        cs = po.composition([po.elinc(np.array([cms]), np.array([b["sigma_inc"]]), np.array([1.0])),
+                             po.powderbragg(a ** 3 * len(b["positions"]), d2, f2 * m2),
+                             po.sab(sf["alpha"], sf["beta"], sf["sab"], T, mass, sigma_bound,
+                                    suggested_emax=k["suggested_emax"])], [1.0, 1.0, 1.0])
+        cab = po.absoov(b["sigma_abs"])
+        nc = 200_000
+        tc = time.perf_counter()
+        pymmc.minimc_sphere(cs, cab, nd, r, ekin, nc, pos=(0.0, 0.0, z), seed=1, nthreads=threads)
+        dt = time.perf_counter() - tc
+        cpu = dict(value=nc / dt, unit="neutrons/s", cores=threads, kind="port",
+                   sample=f"{nc} source neutrons of the same simulation by the checker's engine (numpy steps over "
+                          f"the checker's composition, {threads} threads), {dt:.1f} s")
+    return {
+        "metric": "MiniMC source neutrons/s on 1 MI355X (Al_sg225 sphere, 4 Aa pencil beam)",
+        "value": n * steps / wall, "unit": "neutrons/s", "steps": steps, "warmup": 1,
+        "ms_per_step": wall / steps * 1e3, "dtype": "f64",
+        "config": {"workload": "pencil beam of 4 Aa neutrons into a sphere of Al_sg225 (ElInc + PowderBragg + "
+                               "VDOS S(alpha,beta), 1/v absorption) one scattering length across (r = %.4f m), "
+                               "forced collisions with roulette, theta tally with breakdown" % r,
+                   "neutrons_per_step": n, "neutron_steps_per_call": nst / steps,
+                   "engine_steps_per_call": res["info"]["max_steps"], "tallied_weight_fraction": tot / n},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_mmc_step (engine step: transmission, tally, collision point)",
+                     "kernel_ms": ms / steps, "bytes_per_call": bytes_ / steps},
+        "cpu_baseline": cpu,
+    }
+
+
 def copy_calibration(nbytes=800_000_000):
     """one plain device-to-device copy of nbytes (read nbytes, write nbytes):
     the reference launch that scales the FETCH_SIZE / WRITE_SIZE counters of
@@ -806,7 +903,7 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only (a one-GPU node); run with --gpus 1"}))
         sys.exit(2)
     runners = {"sab": run_sab, "c1": run_c1, "powderbragg": run_powderbragg, "batch": run_batch,
-               "scbragg": run_scbragg, "vdos": run_vdos}
+               "scbragg": run_scbragg, "vdos": run_vdos, "minimc": run_minimc}
     if args.copy_calib:
         copy_calibration()
     if args.workload != "all":
@@ -818,7 +915,8 @@ def main():
     sub_args = argparse.Namespace(**vars(args))
     sub_args.cpu_seconds = args.sub_cpu_seconds
     sub = {}
-    for key, name in (("C1", "c1"), ("C2", "powderbragg"), ("C4", "batch"), ("C5", "scbragg"), ("F2", "vdos")):
+    for key, name in (("C1", "c1"), ("C2", "powderbragg"), ("C4", "batch"), ("C5", "scbragg"), ("F2", "vdos"),
+                      ("F3", "minimc")):
         t0 = time.perf_counter()
         r = runners[name](sub_args)
         for k in ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):

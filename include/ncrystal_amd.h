@@ -374,6 +374,33 @@ double ncamd_mlp_train_step(ncamd_mlp_t, const double* ekin, const double* w, co
  * 10% of the steps */
 double ncamd_mlp_train_sab(ncamd_mlp_t, ncamd_process_t sab, int steps, int batch, double lr, uint64_t seed);
 
+/* ---------- MiniMC on the device ----------
+ * A beam of nsrc neutrons through a sphere of radius [m] centred at the
+ * origin, filled with a material of number density numdens [atoms/Aa^3]
+ * whose scattering is `scatter` and absorption `absorption` (either may be a
+ * null handle), simulated with the forced-collision engine of NCrystal's
+ * MiniMC (ncrystal_core/src/minimc/NCMMC_SimEngine.cc:170-449: per step the
+ * unscattered part of each neutron is tallied at the surface, the rest
+ * scatters at a distance drawn from the truncated exponential, absorption by
+ * weight, Russian roulette), sources "constant" (src_kind 0) and "circular"
+ * (1) of NCMMC_Source.cc with src = {ekin [eV], x, y, z [m], ux, uy, uz, r
+ * [m]}, missing neutrons tallied as in the reference unless ignored.
+ * Tallies (NCMMC_StdTallies.cc): 0 theta [deg] (angle to the beam), 1 mu,
+ * 2 nscat, 3 w, binned in nbins (<= 400) over [xmin, xmax), with the
+ * breakdown by scattering history when `breakdown`.  engine = {roulette
+ * survival probability, weight threshold, nscat threshold, nscatlimit (< 0:
+ * none), ignoremiss, include absorption}.  hist[6][2][nbins + 2] receives,
+ * for the total and the categories NOSCAT, SINGLESCAT_ELAS,
+ * SINGLESCAT_INELAS, MULTISCAT_PUREELAS, MULTISCAT_OTHER, the sums of weights
+ * and of squared weights in (underflow, bins, overflow); info[5] = {most
+ * steps of a batch, neutron-steps, steps summed over batches, neutrons
+ * passed on to scatter, device time of the step kernels [ms]}.  Replaces
+ * NCrystal.minimc.run's engine for this geometry (ncrystal_python
+ * minimc.py run(), NCMMC_RunSim.cc).  0, or -1 on error. */
+int ncamd_minimc_sphere(ncamd_process_t scatter, ncamd_process_t absorption, double numdens, double radius,
+                        int src_kind, const double* src, uint64_t nsrc, int tally, int nbins, double xmin,
+                        double xmax, int breakdown, const double* engine, uint64_t seed, double* hist, double* info);
+
 /* ---------- learned-proposal sampler of S(alpha,beta) (no counterpart in
  * NCrystal) ----------
  * The envelope sampler above on grids placed by the model: for each energy

@@ -160,6 +160,9 @@ def lib():
         "ncamd_mlp_set_params": (ctypes.c_int, [MlpHandle, _fp]),
         "ncamd_mlp_forward": (ctypes.c_int, [MlpHandle, _dp, ctypes.c_int, _fp]),
         "ncamd_mlp_loss_grad": (ctypes.c_double, [MlpHandle, _dp, _dp, _dp, ctypes.c_int, _fp]),
+        "ncamd_minimc_sphere": (ctypes.c_int, [Handle, Handle, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp,
+                                               ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_int, _dp, ctypes.c_uint64, _dp, _dp]),
         "ncamd_mlp_train_step": (ctypes.c_double, [MlpHandle, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double]),
         "ncamd_mlp_train_sab": (ctypes.c_double, [MlpHandle, Handle, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                   ctypes.c_uint64]),

@@ -458,6 +458,41 @@ def vdos_analyse(vdos, temperature, mass_amu):
     return dict(gamma0=out[0], msd=out[1], teff=out[2], integral=out[3])
 
 
+MINIMC_TALLIES = {"theta": 0, "mu": 1, "nscat": 2, "w": 3}
+MINIMC_CATEGORIES = ("NOSCAT", "SINGLESCAT_ELAS", "SINGLESCAT_INELAS", "MULTISCAT_PUREELAS", "MULTISCAT_OTHER")
+
+
+def minimc_sphere(scatter, absorption, numdens, radius, ekin, n, source="constant", pos=(0.0, 0.0, 0.0),
+                  direction=(0.0, 0.0, 1.0), beam_radius=0.0, tally="theta", nbins=90, xrange=(0.0, 180.0),
+                  breakdown=True, roulette=(0.1, 1e-2, 2), nscatlimit=None, ignoremiss=False,
+                  include_absorption=True, seed=0):
+    """MiniMC through a sphere on the GPU (ncamd_minimc_sphere).  Returns a
+    dict: "total" and, with breakdown, each scattering-history category, as
+    dict(content, errorsq, underflow, underflow_errorsq, overflow,
+    overflow_errorsq) over nbins bins of xrange; plus "info"."""
+    kinds = {"constant": 0, "circular": 1}
+    src = np.array([ekin, *pos, *direction, beam_radius], dtype=np.float64)
+    eng = np.array([roulette[0], roulette[1], roulette[2], -1 if nscatlimit is None else nscatlimit,
+                    1 if ignoremiss else 0, 1 if include_absorption else 0], dtype=np.float64)
+    hist = np.zeros((6, 2, nbins + 2))
+    info = np.zeros(5)
+    null = C.Handle()
+    C.call("ncamd_minimc_sphere", scatter.handle if scatter is not None else null,
+           absorption.handle if absorption is not None else null, float(numdens), float(radius), kinds[source],
+           C.dptr(src), int(n), MINIMC_TALLIES[tally], int(nbins), float(xrange[0]), float(xrange[1]),
+           1 if breakdown else 0, C.dptr(eng), int(seed), C.dptr(hist), C.dptr(info))
+
+    def h(k):
+        return dict(content=hist[k, 0, 1:-1].copy(), errorsq=hist[k, 1, 1:-1].copy(), underflow=hist[k, 0, 0],
+                    underflow_errorsq=hist[k, 1, 0], overflow=hist[k, 0, -1], overflow_errorsq=hist[k, 1, -1])
+    out = {"total": h(0), "info": dict(max_steps=int(info[0]), neutron_steps=int(info[1]), scattered=int(info[3]),
+                                       step_kernel_ms=float(info[4]))}
+    if breakdown:
+        for i, name in enumerate(MINIMC_CATEGORIES):
+            out[name] = h(1 + i)
+    return out
+
+
 def vdos_gn(vdos, temperature, mass_amu, order):
     """(egrid, spectrum) of the phonon-order-n function G_n of the expansion"""
     ne, eg, nd, d, keep = _vdos_args(vdos)

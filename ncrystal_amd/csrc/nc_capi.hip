@@ -3024,17 +3024,41 @@ void ncamd_dev_samplescatterisotropic_many_noalloc(ncamd_process_t h, const doub
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH
 }
+}  // extern "C"
+
+namespace ncamd {
+namespace {
+// oriented calls of any process on device arrays (direction interleaved per
+// neutron): scattering and cross sections
+void dev_sample_dirs(Proc* p, const double* d_ekin, const double* d_dirs, int64_t n, double* d_oE, double* d_oD,
+                     hipStream_t st) {
+  if (p->kind == kKindSCB) {
+    const ScbWork w = scb_work(p, n, st);
+    launch_scb_sample(p->scb, d_ekin, d_dirs, n, p->seed, next_stream(p), d_oE, d_oD, w, st);
+  } else {
+    sample_dirs_dispatch(p, d_ekin, d_dirs, n, d_oE, d_oD, st);
+  }
+}
+void dev_xs_dirs(Proc* p, const double* d_ekin, const double* d_dirs, int64_t n, double* d_out, hipStream_t st) {
+  if (p->kind == kKindSCB) {
+    const ScbWork w = scb_work(p, n, st);
+    launch_scb_xs(p->scb, d_ekin, d_dirs, n, d_out, w, st);
+  } else if (p->oriented_comp) {
+    oriented_xs(p, d_ekin, d_dirs, n, d_out, st);
+  } else {
+    xs_dispatch(p, d_ekin, n, 1, d_out, false, st);
+  }
+}
+}  // namespace
+}  // namespace ncamd
+
+extern "C" {
+
 void ncamd_dev_samplescatter_many_dirs(ncamd_process_t h, const double* d_ekin, const double* d_dirs,
                                        unsigned long n, double* d_oE, double* d_oD, void* stream) {
   NCAMD_TRY
   Proc* p = extract(h);
-  if (p->kind == kKindSCB) {
-    const ScbWork w = scb_work(p, (int64_t)n, as_stream(stream));
-    launch_scb_sample(p->scb, d_ekin, d_dirs, (int64_t)n, p->seed, next_stream(p), d_oE, d_oD, w,
-                      as_stream(stream));
-  } else {
-    sample_dirs_dispatch(p, d_ekin, d_dirs, (int64_t)n, d_oE, d_oD, as_stream(stream));
-  }
+  dev_sample_dirs(p, d_ekin, d_dirs, (int64_t)n, d_oE, d_oD, as_stream(stream));
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH
 }
@@ -3042,14 +3066,7 @@ void ncamd_dev_crosssection_many_dirs(ncamd_process_t h, const double* d_ekin, c
                                       unsigned long n, double* d_out, void* stream) {
   NCAMD_TRY
   Proc* p = extract(h);
-  if (p->kind == kKindSCB) {
-    const ScbWork w = scb_work(p, (int64_t)n, as_stream(stream));
-    launch_scb_xs(p->scb, d_ekin, d_dirs, (int64_t)n, d_out, w, as_stream(stream));
-  } else if (p->oriented_comp) {
-    oriented_xs(p, d_ekin, d_dirs, (int64_t)n, d_out, as_stream(stream));
-  } else {
-    xs_dispatch(p, d_ekin, (int64_t)n, 1, d_out, false, as_stream(stream));
-  }
+  dev_xs_dirs(p, d_ekin, d_dirs, (int64_t)n, d_out, as_stream(stream));
   NCAMD_HIP(hipGetLastError());
   NCAMD_CATCH
 }
@@ -3927,5 +3944,168 @@ void ncrystal_setrngstate_ofscatter(ncrystal_scatter_t s, const char* state) {
   NCAMD_CATCH
 }
 void ncrystal_dealloc_string(char* s) { free(s); }
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- MiniMC ---
+namespace ncamd {
+namespace {
+struct MmcBufs {
+  DevBuf x, y, z, d, E, w, id, nscat, ninel;
+  void alloc(size_t n) {
+    const size_t f = sizeof(double) * n;
+    x.alloc(f); y.alloc(f); z.alloc(f); d.alloc(3 * f); E.alloc(f); w.alloc(f);
+    id.alloc(sizeof(int64_t) * n); nscat.alloc(sizeof(int) * n); ninel.alloc(sizeof(int) * n);
+  }
+  MmcBasket view() {
+    return MmcBasket{x.as<double>(), y.as<double>(), z.as<double>(), d.as<double>(), E.as<double>(),
+                     w.as<double>(), id.as<int64_t>(), nscat.as<int>(), ninel.as<int>()};
+  }
+};
+
+unsigned long long read_count(DevBuf& c, hipStream_t st) {
+  unsigned long long v = 0;
+  NCAMD_HIP(hipMemcpyAsync(&v, c.p, sizeof(v), hipMemcpyDeviceToHost, st));
+  NCAMD_HIP(hipStreamSynchronize(st));
+  return v;
+}
+
+// The engine loop (NCMMC_SimEngine.cc:170-449, NCMMC_SimMgr.cc:106-130):
+// source neutrons in batches, each batch stepped until no neutron is left
+void minimc_run(Proc* ps, Proc* pa, double numdens, double radius, const MmcSource& src, uint64_t nsrc, MmcTally t,
+                const MmcEngine& e, int ignoremiss, uint64_t seed, double* hist_out, double* info) {
+  const hipStream_t st = 0;
+  const double macro = 100.0 * numdens;  // barn x atoms/Aa^3 -> 1/m
+  const double r2 = radius * radius;
+  const int64_t batch = (int64_t)std::min<uint64_t>(nsrc, (uint64_t)1 << 21);
+  MmcBufs bufs[2];
+  bufs[0].alloc((size_t)batch);
+  bufs[1].alloc((size_t)batch);
+  int cur = 0;  // the step's input arrays; the other set receives the scattered neutrons
+  DevBuf xs_s, xs_a, oE, oD, cnt, hist;
+  xs_s.alloc(sizeof(double) * batch);
+  xs_a.alloc(sizeof(double) * batch);
+  oE.alloc(sizeof(double) * batch);
+  oD.alloc(3 * sizeof(double) * batch);
+  cnt.alloc(sizeof(unsigned long long));
+  const size_t nh = (size_t)kMmcCats * 2 * (t.nbins + 2);
+  hist.alloc(sizeof(double) * nh);
+  NCAMD_HIP(hipMemsetAsync(hist.p, 0, sizeof(double) * nh, st));
+  t.hist = hist.as<double>();
+  const bool scat = ps && macro > 0.0;
+  const bool absn = pa && macro > 0.0;
+  double steps = 0.0, nsteps = 0.0, maxstep = 0.0, nappend = 0.0, step_ms = 0.0;
+  hipEvent_t ev0, ev1;
+  NCAMD_HIP(hipEventCreate(&ev0));
+  NCAMD_HIP(hipEventCreate(&ev1));
+  for (uint64_t base = 0; base < nsrc; base += (uint64_t)batch) {
+    const int64_t nb = (int64_t)std::min<uint64_t>((uint64_t)batch, nsrc - base);
+    NCAMD_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), st));
+    cur = 0;
+    launch_mmc_source(src, r2, t, ignoremiss, seed, (int64_t)base, nb, bufs[cur].view(), cnt.as<unsigned long long>(),
+                      st);
+    int64_t na = (int64_t)read_count(cnt, st);
+    uint32_t step = 0;
+    while (na > 0) {
+      if (step >= 1000000u) raise("CalcError", "MiniMC: neutrons still scattering after 1e6 steps");
+      MmcBasket a = bufs[cur].view(), b = bufs[1 - cur].view();
+      if (scat) dev_xs_dirs(ps, a.E, a.d, na, xs_s.as<double>(), st);
+      else NCAMD_HIP(hipMemsetAsync(xs_s.p, 0, sizeof(double) * na, st));
+      if (absn) dev_xs_dirs(pa, a.E, a.d, na, xs_a.as<double>(), st);
+      NCAMD_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), st));
+      NCAMD_HIP(hipEventRecord(ev0, st));
+      launch_mmc_step(a, na, xs_s.as<double>(), absn ? xs_a.as<double>() : nullptr, macro, r2, t, e, seed, step, b,
+                      cnt.as<unsigned long long>(), st);
+      NCAMD_HIP(hipEventRecord(ev1, st));
+      const int64_t nbn = (int64_t)read_count(cnt, st);
+      float ms = 0.0f;
+      NCAMD_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+      step_ms += ms;
+      nappend += (double)nbn;
+      if (nbn > 0) {
+        dev_sample_dirs(ps, b.E, b.d, nbn, oE.as<double>(), oD.as<double>(), st);
+        launch_mmc_post(b, nbn, oE.as<double>(), oD.as<double>(), st);
+      }
+      NCAMD_HIP(hipGetLastError());
+      nsteps += (double)na;
+      cur = 1 - cur;
+      na = nbn;
+      ++step;
+    }
+    steps += step;
+    maxstep = std::max(maxstep, (double)step);
+  }
+  NCAMD_HIP(hipMemcpyAsync(hist_out, hist.p, sizeof(double) * nh, hipMemcpyDeviceToHost, st));
+  NCAMD_HIP(hipStreamSynchronize(st));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  if (info) {
+    info[0] = maxstep;
+    info[1] = nsteps;
+    info[2] = steps;
+    info[3] = nappend;
+    info[4] = step_ms;
+  }
+}
+}  // namespace
+}  // namespace ncamd
+
+extern "C" {
+
+int ncamd_minimc_sphere(ncamd_process_t scatter, ncamd_process_t absorption, double numdens, double radius,
+                        int src_kind, const double* src, uint64_t nsrc, int tally, int nbins, double xmin,
+                        double xmax, int breakdown, const double* engine, uint64_t seed, double* hist, double* info) {
+  NCAMD_TRY
+  ensure_device();
+  Proc* ps = scatter.internal ? extract(scatter) : nullptr;
+  Proc* pa = absorption.internal ? extract(absorption) : nullptr;
+  if (ps && is_absorption(ps)) raise("BadInput", "MiniMC: the scattering process is an absorption process");
+  if (pa && !is_absorption(pa)) raise("BadInput", "MiniMC: the absorption process is not an absorption process");
+  if (!(numdens >= 0.0) || !std::isfinite(numdens)) raise("BadInput", "MiniMC: invalid number density");
+  if (!(radius > 0.0) || !std::isfinite(radius)) raise("BadInput", "MiniMC: sphere radius must be > 0");
+  if (src_kind != kMmcSrcConstant && src_kind != kMmcSrcCircular) raise("BadInput", "MiniMC: unknown source");
+  if (nsrc < 1) raise("BadInput", "MiniMC: no source neutrons");
+  if (tally < kMmcTallyTheta || tally > kMmcTallyW) raise("BadInput", "MiniMC: unknown tally");
+  if (nbins < 1 || nbins > kMmcMaxBins || !(xmax > xmin)) raise("BadInput", "MiniMC: invalid tally binning");
+  MmcSource s{};
+  s.kind = src_kind;
+  s.ekin = src[0];
+  s.x = src[1]; s.y = src[2]; s.z = src[3];
+  const double un = std::sqrt(src[4] * src[4] + src[5] * src[5] + src[6] * src[6]);
+  if (!(s.ekin > 0.0) || !std::isfinite(s.ekin)) raise("BadInput", "MiniMC: source energy must be > 0");
+  if (!(un > 0.0) || !std::isfinite(un)) raise("BadInput", "MiniMC: invalid source direction");
+  s.ux = src[4] / un; s.uy = src[5] / un; s.uz = src[6] / un;
+  s.r = src_kind == kMmcSrcCircular ? src[7] : 0.0;
+  if (!(s.r >= 0.0) || !std::isfinite(s.r)) raise("BadInput", "MiniMC: invalid beam radius");
+  if (s.r > 0.0) {  // unit vectors spanning the disk normal to the beam
+    double a[3] = {1.0, 0.0, 0.0};
+    if (s.ux > 0.8) { a[0] = 0.0; a[1] = 1.0; }
+    if (a[1] * s.uy > 0.8) { a[1] = 0.0; a[2] = 1.0; }
+    double c[3] = {s.uy * a[2] - s.uz * a[1], s.uz * a[0] - s.ux * a[2], s.ux * a[1] - s.uy * a[0]};
+    const double cn = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    s.ax = c[0] / cn; s.ay = c[1] / cn; s.az = c[2] / cn;
+    double b[3] = {s.uy * s.az - s.uz * s.ay, s.uz * s.ax - s.ux * s.az, s.ux * s.ay - s.uy * s.ax};
+    const double bn = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    s.bx = b[0] / bn; s.by = b[1] / bn; s.bz = b[2] / bn;
+  }
+  MmcTally t{};
+  t.kind = tally; t.nbins = nbins; t.breakdown = breakdown ? 1 : 0;
+  t.xmin = xmin; t.xmax = xmax; t.inv_bw = nbins / (xmax - xmin);
+  t.u0x = s.ux; t.u0y = s.uy; t.u0z = s.uz;
+  MmcEngine e{};
+  e.roul_psurv = engine[0];
+  e.roul_wthr = engine[1];
+  e.roul_nscat = (int)engine[2];
+  e.nscatlimit = engine[3] < 0.0 ? -1 : (int)engine[3];
+  const int ignoremiss = engine[4] != 0.0 ? 1 : 0;
+  const bool include_abs = engine[5] != 0.0;
+  if (!(e.roul_psurv > 1e-20 && e.roul_psurv < 1.0)) raise("BadInput", "MiniMC: roulette survival probability must be in (1e-20, 1)");
+  if (!(e.roul_wthr > 0.0)) raise("BadInput", "MiniMC: roulette weight threshold must be > 0");
+  if (engine[3] > 32000.0) raise("BadInput", "MiniMC: nscatlimit must be in 0..32000");
+  minimc_run(ps, include_abs ? pa : nullptr, numdens, radius, s, nsrc, t, e, ignoremiss, seed, hist, info);
+  return 0;
+  NCAMD_CATCH
+  return -1;
+}
 
 }  // extern "C"

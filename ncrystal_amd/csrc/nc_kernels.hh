@@ -237,6 +237,40 @@ void launch_mlp_step(float* w, float* grad, float* m, float* v, const double* E,
                      const float* X, const float* T, int n, double kT, double bmax, double lelo, double linv,
                      float* loss, unsigned* counter, bool update, float lr, int step, hipStream_t st);
 
+// MiniMC on the device (nc_mmc.hip): neutrons of one step as a structure of
+// arrays (directions interleaved as (ux, uy, uz) per neutron, as the
+// scattering calls take them)
+struct MmcBasket {
+  double *x, *y, *z, *d, *E, *w;
+  int64_t* id;                 // source index (the key of the neutron's engine draws)
+  int *nscat, *ninel;          // scatterings, of which inelastic
+};
+constexpr int kMmcSrcConstant = 0, kMmcSrcCircular = 1;
+struct MmcSource {
+  int kind;
+  double ekin, x, y, z, ux, uy, uz, r;
+  double ax, ay, az, bx, by, bz;  // unit vectors spanning the circular beam's disk
+};
+constexpr int kMmcTallyTheta = 0, kMmcTallyMu = 1, kMmcTallyNscat = 2, kMmcTallyW = 3;
+constexpr int kMmcCats = 6;       // the total and the five scattering-history categories
+constexpr int kMmcMaxBins = 400;
+struct MmcTally {
+  int kind, nbins, breakdown;
+  double xmin, xmax, inv_bw;
+  double u0x, u0y, u0z;        // reference direction (the beam's)
+  double* hist;                // [kMmcCats][sum w, sum w^2][underflow, nbins, overflow]
+};
+struct MmcEngine {
+  double roul_psurv, roul_wthr;
+  int roul_nscat, nscatlimit;  // nscatlimit < 0: none
+};
+void launch_mmc_source(const MmcSource& src, double r2, const MmcTally& t, int ignoremiss, uint64_t seed, int64_t id0,
+                       int64_t n, const MmcBasket& out, unsigned long long* count, hipStream_t);
+void launch_mmc_step(const MmcBasket& in, int64_t n, const double* xs_s, const double* xs_a, double macro, double r2,
+                     const MmcTally& t, const MmcEngine& e, uint64_t seed, uint32_t step, const MmcBasket& out,
+                     unsigned long long* count, hipStream_t);
+void launch_mmc_post(const MmcBasket& b, int64_t n, const double* oE, const double* oD, hipStream_t);
+
 // many materials in one persistent launch (nc_batch.hip)
 struct BatchMat {
   CompView view;

@@ -1,0 +1,112 @@
+"""Shared pieces of the MiniMC tests: NCrystal's Al_sg225 material (structure,
+scattering lengths, absorption and VDOS of the data library, fixture
+tests/golden/vdos_reference.json) built by the library and by the checker,
+the reference's MiniMC histograms (tests/golden/mmcref, copied from the
+reference's tests/data/mmcref) and its histogram compatibility test
+(ncrystal_python/src/NCrystal/hist.py:810-901, Hist1D.check_compat: chi-square
+of the normalised histograms over the bins filled in either, one degree of
+freedom less than those bins, plus the under- and overflow bins, p-value of
+the chi-square distribution)."""
+import json
+import math
+import pathlib
+
+import numpy as np
+from scipy.stats import chi2
+
+GOLD = pathlib.Path(__file__).parent / "golden"
+FIX = json.loads((GOLD / "vdos_reference.json").read_text())
+AL = FIX["basic"]
+AL_VDOS = FIX["vdos"][AL["vdos"]]
+AL_NUMDENS = len(AL["positions"]) / AL["a"] ** 3          # atoms / Aa^3
+AL_SIGMA_BOUND = 4 * math.pi * AL["coh_sl"] ** 2 + AL["sigma_inc"]
+
+
+def wl2ekin(wl):
+    return 0.081804209605330899 / wl ** 2
+
+
+def ref_hist(name):
+    d = json.loads((GOLD / "mmcref" / f"{name}.json").read_text())["bindata"]
+    return dict(content=np.array(d["content"]), errorsq=np.array(d["errorsq"]), underflow=d["underflow"],
+                underflow_errorsq=d["underflow_errorsq"], overflow=d["overflow"],
+                overflow_errorsq=d["overflow_errorsq"], nbins=d["nbins"], xmin=d["xmin"], xmax=d["xmax"])
+
+
+def pvalue(h1, h2):
+    """Hist1D.check_compat(..., force_norm=True, return_pval=True) of two
+    histograms with the same binning and under/overflow bins"""
+    def norm(h):
+        tot = h["content"].sum() + h["underflow"] + h["overflow"]
+        f = 1.0 / tot if tot > 0 else 1.0
+        return (h["content"] * f, np.sqrt(h["errorsq"]) * f, h["underflow"] * f, math.sqrt(h["underflow_errorsq"]) * f,
+                h["overflow"] * f, math.sqrt(h["overflow_errorsq"]) * f)
+    c1, e1, u1, ue1, o1, oe1 = norm(h1)
+    c2, e2, u2, ue2, o2, oe2 = norm(h2)
+    m = (e1 > 0) | (e2 > 0)
+    n = int(m.sum())
+    x2 = float((((c1 - c2) ** 2)[m] / (e1 ** 2 + e2 ** 2)[m]).sum()) if n else 0.0
+    n += 2
+    if u1 > 0 or u2 > 0:
+        x2 += (u1 - u2) ** 2 / (ue1 ** 2 + ue2 ** 2)
+    if o1 > 0 or o2 > 0:
+        x2 += (o1 - o2) ** 2 / (oe1 ** 2 + oe2 ** 2)
+    return float(chi2.sf(x2, max(1, n - 1)))
+
+
+def library_al(temperature=293.15, absorption=True):
+    """(scatter, absorption or None, number density) of Al_sg225 in the
+    library: ElInc + PowderBragg (MSD of the VDOS) + S(alpha,beta) expanded
+    from the VDOS (vdoslux 3), as NCrystal's standard factory composes them"""
+    from ncrystal_amd import core
+    vin = (np.array(AL_VDOS["egrid"]), np.array(AL_VDOS["density"]))
+    msd = core.vdos_analyse(vin, temperature, AL["mass"])["msd"]
+    a = AL["a"]
+    hkl, mult, dsp, fsq, _, _ = core.hkl_spacegroup_members(AL["spacegroup"], (a, a, a, 90, 90, 90),
+                                                             [(AL["coh_sl"], msd, np.array(AL["positions"]))],
+                                                             AL["dcutoff"])
+    pb = core.create_powderbragg(a ** 3 * len(AL["positions"]), dsp, fsq * mult)
+    ei = core.create_elincscatter([msd], [AL["sigma_inc"]], [1.0])
+    inel = core.create_sabscatter_from_vdos(vin, temperature, AL["mass"], AL_SIGMA_BOUND, vdoslux=3)
+    scatter = core.create_composition([ei, pb, inel], [1.0, 1.0, 1.0])
+    ab = core.create_absoov(AL["sigma_abs"]) if absorption else None
+    return scatter, ab, AL_NUMDENS
+
+
+def checker_al(oracle, temperature=293.15, absorption=True):
+    """the same material built by the checker"""
+    lo, hi, d = oracle.vdos_regularise(np.array(AL_VDOS["egrid"]), np.array(AL_VDOS["density"]))
+    msd = oracle.vdos_analyse(lo, hi, d, temperature, AL["mass"])["msd"]
+    a = AL["a"]
+    hkl, mult, dsp, fsq, _, _ = oracle.hkl_spacegroup_members(AL["spacegroup"], (a, a, a, 90, 90, 90),
+                                                               [(AL["coh_sl"], msd, np.array(AL["positions"]))],
+                                                               AL["dcutoff"])
+    pb = oracle.powderbragg(a ** 3 * len(AL["positions"]), dsp, fsq * mult)
+    ei = oracle.elinc(np.array([msd]), np.array([AL["sigma_inc"]]), np.array([1.0]))
+    k = oracle.vdos_kernel(lo, hi, d, temperature, AL["mass"], vdoslux=3)
+    sf = oracle.sab_std_format(0, k["alpha"], k["beta"], k["sab"], beta_grid_optimised=True)
+    inel = oracle.sab(sf["alpha"], sf["beta"], sf["sab"], temperature, AL["mass"], AL_SIGMA_BOUND,
+                      suggested_emax=k["suggested_emax"])
+    scatter = oracle.composition([ei, pb, inel], [1.0, 1.0, 1.0])
+    ab = oracle.absoov(AL["sigma_abs"]) if absorption else None
+    return scatter, ab, AL_NUMDENS
+
+
+# the reference's Al MiniMC cases (tests/scripts/mmc_al.py, mmc_alnoabs.py):
+# (reference histogram, temperature, wavelength [Aa], with absorption, sphere
+# diameter [m] or None for one scattering length at that energy)
+AL_CASES = [
+    ("mmc_al_4Aa", 293.15, 4.0, True, None),
+    ("mmc_al_4Aa700K", 700.0, 4.0, True, None),
+    ("mmc_al_0.8Aa", 293.15, 1.0, True, None),   # the reference runs this key at 1.0 Aa
+    ("mmc_alnoabs", 293.15, 2.5, False, 0.2),
+]
+
+
+def sphere_setup(scatter_xs, ekin, diameter):
+    """(radius, source z) of minimc_unittest_stdsphere: diameter one
+    scattering length unless given, pencil beam started just inside"""
+    if diameter is None:
+        diameter = 1.0 / (AL_NUMDENS * 100.0 * scatter_xs)
+    r = diameter / 2
+    return r, -r * (1 - 1e-13)
