@@ -3990,8 +3990,11 @@ void minimc_run(Proc* ps, Proc* pa, double numdens, double radius, const MmcSour
   cnt.alloc(sizeof(unsigned long long));
   const size_t nh = (size_t)kMmcCats * 2 * (t.nbins + 2);
   hist.alloc(sizeof(double) * nh);
-  NCAMD_HIP(hipMemsetAsync(hist.p, 0, sizeof(double) * nh, st));
+  DevBuf part;
+  part.alloc(sizeof(double) * nh * kMmcBlocks);
+  NCAMD_HIP(hipMemsetAsync(part.p, 0, sizeof(double) * nh * kMmcBlocks, st));
   t.hist = hist.as<double>();
+  t.part = part.as<double>();
   const bool scat = ps && macro > 0.0;
   const bool absn = pa && macro > 0.0;
   double steps = 0.0, nsteps = 0.0, maxstep = 0.0, nappend = 0.0, step_ms = 0.0;
@@ -4035,6 +4038,8 @@ void minimc_run(Proc* ps, Proc* pa, double numdens, double radius, const MmcSour
     steps += step;
     maxstep = std::max(maxstep, (double)step);
   }
+  launch_mmc_reduce(t, st);
+  NCAMD_HIP(hipGetLastError());
   NCAMD_HIP(hipMemcpyAsync(hist_out, hist.p, sizeof(double) * nh, hipMemcpyDeviceToHost, st));
   NCAMD_HIP(hipStreamSynchronize(st));
   (void)hipEventDestroy(ev0);

@@ -114,6 +114,35 @@ struct Rng {
 };
 
 // ---------------------------------------------------------------------------
+// Position of this thread's entry in a queue that the threads of the
+// workgroup with `want` append to: one atomic per workgroup (appends to one
+// counter serialise, so a launch that appends per wave spends most of its
+// time there).  Every thread of the workgroup must call it: it synchronises
+// the workgroup.  lds: NW + 1 workgroup-shared entries, NW = waves per
+// workgroup.
+template <int NW>
+NCD unsigned long long block_append(unsigned long long* counter, bool want, unsigned long long* lds) {
+  const unsigned long long m = __ballot(want);
+  const int lane = (int)__lane_id(), w = (int)(threadIdx.x >> 6);
+  if (lane == 0) lds[w] = (unsigned long long)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (int j = 0; j < NW; ++j) {
+      const unsigned long long c = lds[j];
+      lds[j] = tot;
+      tot += c;
+    }
+    lds[NW] = tot ? atomicAdd(counter, tot) : 0ull;
+  }
+  __syncthreads();
+  const unsigned long long below = lane ? (m & (~0ull >> (64 - lane))) : 0ull;
+  const unsigned long long pos = lds[NW] + lds[w] + (unsigned long long)__popcll(below);
+  __syncthreads();  // lds is free for the next call
+  return pos;
+}
+
+// ---------------------------------------------------------------------------
 // Position of this lane's entry in a queue that the active lanes with `want`
 // append to: one atomic per wave (the first such lane adds their count) and
 // each lane's rank among them.  Every active lane of the wave must call it

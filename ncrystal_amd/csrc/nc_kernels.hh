@@ -259,7 +259,9 @@ struct MmcTally {
   double xmin, xmax, inv_bw;
   double u0x, u0y, u0z;        // reference direction (the beam's)
   double* hist;                // [kMmcCats][sum w, sum w^2][underflow, nbins, overflow]
+  double* part;                // [kMmcBlocks][the same]: per-workgroup partial sums
 };
+constexpr int kMmcBlocks = 1024;  // most workgroups of an engine launch
 struct MmcEngine {
   double roul_psurv, roul_wthr;
   int roul_nscat, nscatlimit;  // nscatlimit < 0: none
@@ -270,6 +272,7 @@ void launch_mmc_step(const MmcBasket& in, int64_t n, const double* xs_s, const d
                      const MmcTally& t, const MmcEngine& e, uint64_t seed, uint32_t step, const MmcBasket& out,
                      unsigned long long* count, hipStream_t);
 void launch_mmc_post(const MmcBasket& b, int64_t n, const double* oE, const double* oD, hipStream_t);
+void launch_mmc_reduce(const MmcTally& t, hipStream_t);  // hist = sum of the partial sums
 
 // many materials in one persistent launch (nc_batch.hip)
 struct BatchMat {

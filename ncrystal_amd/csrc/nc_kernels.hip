@@ -141,28 +141,34 @@ __global__ void __launch_bounds__(256) k_comp_pick(CompView c, const double* __r
                                                    int64_t* __restrict__ sq, unsigned long long* __restrict__ sqc,
                                                    const double* __restrict__ ext, int64_t* __restrict__ cq,
                                                    unsigned long long* __restrict__ cqc) {
+  __shared__ unsigned long long s_app[5];
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gstride) {
-    const int64_t i = (total == n) ? k : (k % n);
+  // workgroup-uniform loop: the queue appends synchronise the workgroup
+  for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x; k0 < total; k0 += gstride) {
+    const int64_t k = k0 + threadIdx.x;
+    const bool valid = k < total;
     Rng rng;
-    rng.init(seed, stream, (uint64_t)k);
-    double Ef, mu;
-    int slot = 0;
-    const int rc = comp_pick_sample<Gas>(c, rng, ekin[i], Ef, mu, slot, ext ? ext[k] : 0.0);
-    // queue appends with all active lanes converged: one queue per staged
-    // component, and the SCBragg queue
-    const bool staged = rc == kCompToStage;
+    double Ef = 0.0, mu = 0.0;
+    int slot = 0, rc = kCompToStage + kCompToSCB + 1;  // neither queue
+    if (valid) {
+      const int64_t i = (total == n) ? k : (k % n);
+      rng.init(seed, stream, (uint64_t)k);
+      rc = comp_pick_sample<Gas>(c, rng, ekin[i], Ef, mu, slot, ext ? ext[k] : 0.0);
+    }
+    // queue appends, one atomic per workgroup and queue: one queue per
+    // staged component, and the SCBragg queue
+    const bool staged = valid && rc == kCompToStage;
     for (int s = 0; s < c.nstaged; ++s) {
       const bool want = staged && slot == s;
-      const unsigned long long pos = wave_append(&sqc[s], want);
+      const unsigned long long pos = block_append<4>(&sqc[s], want, s_app);
       if (want) sq[(int64_t)s * total + (int64_t)pos] = k;
     }
-    const bool to_scb = rc == kCompToSCB;
+    const bool to_scb = valid && rc == kCompToSCB;
     if (cqc) {
-      const unsigned long long pos = wave_append(cqc, to_scb);
+      const unsigned long long pos = block_append<4>(cqc, to_scb, s_app);
       if (to_scb) cq[pos] = k;
     }
-    if (!staged && !to_scb) comp_store<Dirs>(k, rc, Ef, mu, rng, dirs, oE, o2, nerr);
+    if (valid && !staged && !to_scb) comp_store<Dirs>(k, rc, Ef, mu, rng, dirs, oE, o2, nerr);
   }
 }
 

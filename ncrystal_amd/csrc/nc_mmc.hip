@@ -96,19 +96,27 @@ NCD void mmc_hist_clear(MmcLdsHist& L, const MmcTally& t) {
   __syncthreads();
 }
 
+// each workgroup adds its histograms to its own slice of the partial sums
+// (launches on one stream, at most kMmcBlocks workgroups: no two writers of
+// a slice at a time); k_mmc_reduce sums the slices at the end
 NCD void mmc_hist_flush(MmcLdsHist& L, const MmcTally& t) {
   __syncthreads();
   const int m = (t.breakdown ? kMmcCats : 1) * 2 * (t.nbins + 2);
+  double* part = t.part + (size_t)blockIdx.x * kMmcCats * 2 * (t.nbins + 2);
   for (int i = (int)threadIdx.x; i < m; i += (int)blockDim.x)
-    if (L.h[i] != 0.0) atomicAdd(&t.hist[i], L.h[i]);
+    if (L.h[i] != 0.0) part[i] += L.h[i];
 }
 
 // source neutrons id0 .. id0 + n - 1: position and direction, moved to the
 // sphere's surface; those missing it are tallied unless ignored
-__global__ void __launch_bounds__(256) k_mmc_source(MmcSource src, double r2, MmcTally t, int ignoremiss, uint64_t seed,
+constexpr int kMmcThreads = 512;
+constexpr int kMmcWaves = kMmcThreads / 64;
+
+__global__ void __launch_bounds__(kMmcThreads) k_mmc_source(MmcSource src, double r2, MmcTally t, int ignoremiss, uint64_t seed,
                                                     int64_t id0, int64_t n, MmcBasket out,
                                                     unsigned long long* __restrict__ count) {
   __shared__ MmcLdsHist L;
+  __shared__ unsigned long long s_app[kMmcWaves + 1];
   mmc_hist_clear(L, t);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
@@ -133,7 +141,7 @@ __global__ void __launch_bounds__(256) k_mmc_source(MmcSource src, double r2, Mm
         take = true;
       }
     }
-    const unsigned long long pos = wave_append(count, take);
+    const unsigned long long pos = block_append<kMmcWaves>(count, take, s_app);
     if (take) {
       out.x[pos] = x + d * src.ux;
       out.y[pos] = y + d * src.uy;
@@ -152,11 +160,12 @@ __global__ void __launch_bounds__(256) k_mmc_source(MmcSource src, double r2, Mm
 }
 
 // one step of the forced-collision engine for the n neutrons of `in`
-__global__ void __launch_bounds__(256) k_mmc_step(MmcBasket in, int64_t n, const double* __restrict__ xs_s,
+__global__ void __launch_bounds__(kMmcThreads) k_mmc_step(MmcBasket in, int64_t n, const double* __restrict__ xs_s,
                                                   const double* __restrict__ xs_a, double macro, double r2,
                                                   MmcTally t, MmcEngine e, uint64_t seed, uint32_t step,
                                                   MmcBasket out, unsigned long long* __restrict__ count) {
   __shared__ MmcLdsHist L;
+  __shared__ unsigned long long s_app[kMmcWaves + 1];
   mmc_hist_clear(L, t);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
@@ -204,7 +213,7 @@ __global__ void __launch_bounds__(256) k_mmc_step(MmcBasket in, int64_t n, const
         }
       }
     }
-    const unsigned long long pos = wave_append(count, take);
+    const unsigned long long pos = block_append<kMmcWaves>(count, take, s_app);
     if (take) {
       out.x[pos] = x + dsc * ux;
       out.y[pos] = y + dsc * uy;
@@ -237,23 +246,35 @@ __global__ void k_mmc_post(MmcBasket b, int64_t n, const double* __restrict__ oE
   b.d[3 * i + 2] = oD[3 * i + 2];
 }
 
+__global__ void k_mmc_reduce(const double* __restrict__ part, int m, int nblocks, double* __restrict__ hist) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= m) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * m + i];
+  hist[i] = s;
+}
+
 unsigned mmc_grid(int64_t n) {
-  const int64_t g = (n + 255) / 256;
-  return (unsigned)(g < 1 ? 1 : (g > 4096 ? 4096 : g));
+  const int64_t g = (n + kMmcThreads - 1) / kMmcThreads;
+  return (unsigned)(g < 1 ? 1 : (g > kMmcBlocks ? kMmcBlocks : g));
 }
 
 }  // namespace
 
 void launch_mmc_source(const MmcSource& src, double r2, const MmcTally& t, int ignoremiss, uint64_t seed, int64_t id0,
                        int64_t n, const MmcBasket& out, unsigned long long* count, hipStream_t st) {
-  hipLaunchKernelGGL(k_mmc_source, dim3(mmc_grid(n)), dim3(256), 0, st, src, r2, t, ignoremiss, seed, id0, n, out,
-                     count);
+  hipLaunchKernelGGL(k_mmc_source, dim3(mmc_grid(n)), dim3(kMmcThreads), 0, st, src, r2, t, ignoremiss, seed, id0, n,
+                     out, count);
 }
 void launch_mmc_step(const MmcBasket& in, int64_t n, const double* xs_s, const double* xs_a, double macro, double r2,
                      const MmcTally& t, const MmcEngine& e, uint64_t seed, uint32_t step, const MmcBasket& out,
                      unsigned long long* count, hipStream_t st) {
-  hipLaunchKernelGGL(k_mmc_step, dim3(mmc_grid(n)), dim3(256), 0, st, in, n, xs_s, xs_a, macro, r2, t, e, seed, step,
-                     out, count);
+  hipLaunchKernelGGL(k_mmc_step, dim3(mmc_grid(n)), dim3(kMmcThreads), 0, st, in, n, xs_s, xs_a, macro, r2, t, e, seed,
+                     step, out, count);
+}
+void launch_mmc_reduce(const MmcTally& t, hipStream_t st) {
+  const int m = kMmcCats * 2 * (t.nbins + 2);
+  hipLaunchKernelGGL(k_mmc_reduce, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, t.part, m, kMmcBlocks, t.hist);
 }
 void launch_mmc_post(const MmcBasket& b, int64_t n, const double* oE, const double* oD, hipStream_t st) {
   if (n <= 0) return;
