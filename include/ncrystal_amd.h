@@ -386,7 +386,9 @@ double ncamd_mlp_train_sab(ncamd_mlp_t, ncamd_process_t sab, int steps, int batc
  * (1) of NCMMC_Source.cc with src = {ekin [eV], x, y, z [m], ux, uy, uz, r
  * [m]}, missing neutrons tallied as in the reference unless ignored.
  * Tallies (NCMMC_StdTallies.cc): 0 theta [deg] (angle to the beam), 1 mu,
- * 2 nscat, 3 w, binned in nbins (<= 400) over [xmin, xmax), with the
+ * 2 nscat (-1 for neutrons missing the sphere), 3 w (unweighted), 4
+ * nscat_uw (unweighted), 5 e [eV], 6 l [Aa], 7 de = E0 - E [eV], 8 q [1/Aa],
+ * binned in nbins (<= 400) over [xmin, xmax), with the
  * breakdown by scattering history when `breakdown`.  engine = {roulette
  * survival probability, weight threshold, nscat threshold, nscatlimit (< 0:
  * none), ignoremiss, include absorption}.  hist[6][2][nbins + 2] receives,

@@ -458,7 +458,7 @@ def vdos_analyse(vdos, temperature, mass_amu):
     return dict(gamma0=out[0], msd=out[1], teff=out[2], integral=out[3])
 
 
-MINIMC_TALLIES = {"theta": 0, "mu": 1, "nscat": 2, "w": 3}
+MINIMC_TALLIES = {"theta": 0, "mu": 1, "nscat": 2, "w": 3, "nscat_uw": 4, "e": 5, "l": 6, "de": 7, "q": 8}
 MINIMC_CATEGORIES = ("NOSCAT", "SINGLESCAT_ELAS", "SINGLESCAT_INELAS", "MULTISCAT_PUREELAS", "MULTISCAT_OTHER")
 
 

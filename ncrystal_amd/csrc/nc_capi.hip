@@ -4070,7 +4070,7 @@ int ncamd_minimc_sphere(ncamd_process_t scatter, ncamd_process_t absorption, dou
   if (!(radius > 0.0) || !std::isfinite(radius)) raise("BadInput", "MiniMC: sphere radius must be > 0");
   if (src_kind != kMmcSrcConstant && src_kind != kMmcSrcCircular) raise("BadInput", "MiniMC: unknown source");
   if (nsrc < 1) raise("BadInput", "MiniMC: no source neutrons");
-  if (tally < kMmcTallyTheta || tally > kMmcTallyW) raise("BadInput", "MiniMC: unknown tally");
+  if (tally < kMmcTallyTheta || tally > kMmcTallyQ) raise("BadInput", "MiniMC: unknown tally");
   if (nbins < 1 || nbins > kMmcMaxBins || !(xmax > xmin)) raise("BadInput", "MiniMC: invalid tally binning");
   MmcSource s{};
   s.kind = src_kind;
@@ -4097,6 +4097,7 @@ int ncamd_minimc_sphere(ncamd_process_t scatter, ncamd_process_t absorption, dou
   t.kind = tally; t.nbins = nbins; t.breakdown = breakdown ? 1 : 0;
   t.xmin = xmin; t.xmax = xmax; t.inv_bw = nbins / (xmax - xmin);
   t.u0x = s.ux; t.u0y = s.uy; t.u0z = s.uz;
+  t.e0 = s.ekin;
   MmcEngine e{};
   e.roul_psurv = engine[0];
   e.roul_wthr = engine[1];

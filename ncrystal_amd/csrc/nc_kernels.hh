@@ -251,13 +251,17 @@ struct MmcSource {
   double ekin, x, y, z, ux, uy, uz, r;
   double ax, ay, az, bx, by, bz;  // unit vectors spanning the circular beam's disk
 };
-constexpr int kMmcTallyTheta = 0, kMmcTallyMu = 1, kMmcTallyNscat = 2, kMmcTallyW = 3;
+// tallies (NCMMC_StdTallies.cc): theta [deg], mu, nscat, w (unweighted),
+// nscat_uw (unweighted), e [eV], l [Aa], de = E0 - E [eV], q [1/Aa]
+constexpr int kMmcTallyTheta = 0, kMmcTallyMu = 1, kMmcTallyNscat = 2, kMmcTallyW = 3, kMmcTallyNscatUW = 4,
+              kMmcTallyE = 5, kMmcTallyL = 6, kMmcTallyDE = 7, kMmcTallyQ = 8;
 constexpr int kMmcCats = 6;       // the total and the five scattering-history categories
 constexpr int kMmcMaxBins = 400;
 struct MmcTally {
   int kind, nbins, breakdown;
   double xmin, xmax, inv_bw;
   double u0x, u0y, u0z;        // reference direction (the beam's)
+  double e0;                   // reference energy (the beam's) [eV]
   double* hist;                // [kMmcCats][sum w, sum w^2][underflow, nbins, overflow]
   double* part;                // [kMmcBlocks][the same]: per-workgroup partial sums
 };

@@ -52,16 +52,34 @@ NCD int mmc_category(int nscat, int ninel) {
   return 0;
 }
 
-// the tallied quantity of an outgoing neutron
-NCD double mmc_value(const MmcTally& t, double ux, double uy, double uz, double w, int nscat) {
+// wavelength [Aa] of energy [eV] (ekin2wl) and |k|^2 [1/Aa^2] per eV
+constexpr double kMmcWl2Ekin = 0.081804209605330899;
+constexpr double kMmcEkin2Ksq = 39.478417604357434475337963999505 / kMmcWl2Ekin;
+
+// the tallied quantity of an outgoing neutron (NCMMC_StdTallies.cc:328-520,
+// 590-670): angle or cosine to the beam, scatterings (-1: missed the
+// volume), weight, energy, wavelength, energy transfer, momentum transfer
+NCD double mmc_value(const MmcTally& t, double ux, double uy, double uz, double w, int nscat, double E) {
   const double mu0 = ux * t.u0x + uy * t.u0y + uz * t.u0z;
   const double mu = mu0 < -1.0 ? -1.0 : (mu0 > 1.0 ? 1.0 : mu0);
   switch (t.kind) {
     case kMmcTallyTheta: return acos(mu) * kMmcRadToDeg;
     case kMmcTallyMu: return mu;
-    case kMmcTallyNscat: return (double)nscat;
+    case kMmcTallyNscat:
+    case kMmcTallyNscatUW: return (double)nscat;
+    case kMmcTallyE: return E;
+    case kMmcTallyL: return sqrt(kMmcWl2Ekin / (E > 4.9406564584124654e-324 ? E : 4.9406564584124654e-324));
+    case kMmcTallyDE: return t.e0 - E;
+    case kMmcTallyQ: {
+      const double q2 = (E + t.e0 - 2.0 * mu * sqrt(E * t.e0)) * kMmcEkin2Ksq;
+      return q2 > 0.0 ? sqrt(q2) : 0.0;
+    }
     default: return w;
   }
+}
+// the w and nscat_uw tallies count records, the others sum weights
+NCD double mmc_weight(const MmcTally& t, double w) {
+  return (t.kind == kMmcTallyW || t.kind == kMmcTallyNscatUW) ? 1.0 : w;
 }
 
 // per-workgroup histograms in LDS: [1 + 5 categories][sum w, sum w^2][underflow, bins, overflow]
@@ -69,16 +87,20 @@ struct MmcLdsHist {
   double h[kMmcCats * 2 * (kMmcMaxBins + 2)];
 };
 
+// slot of a value among (underflow, bins, overflow) as the reference's
+// histograms bin it (NCHists.hh valueToBin): the upper edge itself goes to
+// the last bin
 NCD int mmc_slot(const MmcTally& t, double v) {
   if (!(v >= t.xmin)) return 0;  // underflow (and NaN)
+  if (v >= t.xmax) return v == t.xmax ? t.nbins : t.nbins + 1;
   const double f = (v - t.xmin) * t.inv_bw;
-  if (!(f < (double)t.nbins)) return t.nbins + 1;
-  int b = (int)f;
-  b = b < 0 ? 0 : (b >= t.nbins ? t.nbins - 1 : b);
-  return b + 1;
+  const int b = f < (double)t.nbins ? (int)f : t.nbins;
+  return 1 + b;
 }
 
+// weighted fills skip weights that are not positive (NCHists.hh fill)
 NCD void mmc_fill(MmcLdsHist& L, const MmcTally& t, double v, double w, int cat) {
+  if (!(w > 0.0)) return;
   const int s = mmc_slot(t, v);
   const int stride = t.nbins + 2;
   atomicAdd(&L.h[s], w);
@@ -136,7 +158,7 @@ __global__ void __launch_bounds__(kMmcThreads) k_mmc_source(MmcSource src, doubl
       }
       d = sphere_entry(r2, x, y, z, src.ux, src.uy, src.uz);
       if (d < 0.0) {
-        if (!ignoremiss) mmc_fill(L, t, mmc_value(t, src.ux, src.uy, src.uz, 1.0, 0), 1.0, 0);
+        if (!ignoremiss) mmc_fill(L, t, mmc_value(t, src.ux, src.uy, src.uz, 1.0, -1, src.ekin), 1.0, 0);
       } else {
         take = true;
       }
@@ -189,7 +211,7 @@ __global__ void __launch_bounds__(kMmcThreads) k_mmc_step(MmcBasket in, int64_t 
       const double ptr = ss > 0.0 ? exp(-ss * D) : 1.0;
       // the part reaching the surface unscattered
       const double wt = w * (sa > 0.0 ? exp(-sa * D) : 1.0) * ptr;
-      mmc_fill(L, t, mmc_value(t, ux, uy, uz, wt, nscat), wt, mmc_category(nscat, ninel));
+      mmc_fill(L, t, mmc_value(t, ux, uy, uz, wt, nscat, E), mmc_weight(t, wt), mmc_category(nscat, ninel));
       // the part scattering inside
       if (w > 0.0 && ss > 0.0) {
         Rng rng;

@@ -19,17 +19,22 @@ cross sections and scattering with directions):
   w exp(-Sigma_a x) (1 - P), after Russian roulette of neutrons with at least
   nscat_threshold scatterings and weight below weight_threshold (survival
   probability p, survivors' weight / p);
-* tallies (NCMMC_StdTallies.cc:223-330, 435-520): theta = acos(u . u0) in
-  degrees, mu, nscat or w, weighted, with the breakdown by scattering
-  history (none, one elastic, one inelastic, several all elastic, several
-  otherwise; a scattering is elastic when the energy is unchanged).
+* tallies (NCMMC_StdTallies.cc:223-330, 328-434, 435-520, 590-670): theta =
+  acos(u . u0) in degrees, mu, nscat (-1 for neutrons missing the volume),
+  e, l = wavelength, de = E0 - E and q = |k_f - k_i|, weighted; w and
+  nscat_uw unweighted (one entry per neutron and step); with the breakdown
+  by scattering history (none, one elastic, one inelastic, several all
+  elastic, several otherwise; a scattering is elastic when the energy is
+  unchanged).
 
 The random numbers come from numpy's PCG64, so a run agrees with the
 reference's and the library's only statistically.
 """
 import numpy as np
 
-TALLIES = ("theta", "mu", "nscat", "w")
+TALLIES = ("theta", "mu", "nscat", "w", "nscat_uw", "e", "l", "de", "q")
+_WL2EKIN = 0.081804209605330899
+_EKIN2KSQ = 4.0 * np.pi ** 2 / _WL2EKIN
 CATEGORIES = ("NOSCAT", "SINGLESCAT_ELAS", "SINGLESCAT_INELAS", "MULTISCAT_PUREELAS", "MULTISCAT_OTHER")
 
 
@@ -63,9 +68,14 @@ class _Hist:
         self.breakdown = breakdown
 
     def fill(self, v, w, cat):
+        # NCHists.hh valueToBin with under/overflow bins (the upper edge goes
+        # to the last bin); weighted fills skip weights that are not positive
+        keep = w > 0.0
+        v, w, cat = v[keep], w[keep], cat[keep]
         f = (v - self.xmin) * (self.nbins / (self.xmax - self.xmin))
-        s = np.where(~(v >= self.xmin), 0, np.where(~(f < self.nbins), self.nbins + 1,
-                                                      np.clip(f.astype(np.int64, copy=False), 0, self.nbins - 1) + 1))
+        inner = 1 + np.minimum(np.floor(np.where(np.isfinite(f), f, 0.0)), self.nbins).astype(np.int64)
+        s = np.where(~(v >= self.xmin), 0,
+                     np.where(v >= self.xmax, np.where(v == self.xmax, self.nbins, self.nbins + 1), inner))
         s = s.astype(np.int64)
         n = self.nbins + 2
         self.h[0, 0] += np.bincount(s, weights=w, minlength=n)
@@ -78,15 +88,27 @@ class _Hist:
                     self.h[1 + k, 1] += np.bincount(s[m], weights=w[m] * w[m], minlength=n)
 
 
-def _value(tally, u, u0, w, nscat):
+def _value(tally, u, u0, w, nscat, E, E0):
     mu = np.clip(u @ u0, -1.0, 1.0)
     if tally == "theta":
         return np.degrees(np.arccos(mu))
     if tally == "mu":
         return mu
-    if tally == "nscat":
+    if tally in ("nscat", "nscat_uw"):
         return nscat.astype(np.float64)
+    if tally == "e":
+        return E
+    if tally == "l":
+        return np.sqrt(_WL2EKIN / np.maximum(E, 5e-324))
+    if tally == "de":
+        return E0 - E
+    if tally == "q":
+        return np.sqrt(np.maximum((E + E0 - 2.0 * mu * np.sqrt(E * E0)) * _EKIN2KSQ, 0.0))
     return w
+
+
+def _fill_weight(tally, w):
+    return np.ones_like(w) if tally in ("w", "nscat_uw") else w
 
 
 def _category(nscat, ninel):
@@ -124,7 +146,7 @@ def minimc_sphere(scatter, absorption, numdens, radius, ekin, n, source="constan
     if miss.any() and not ignoremiss:
         zero = np.zeros(miss.sum(), dtype=np.int64)
         w1 = np.ones(miss.sum())
-        hist.fill(_value(tally, u[miss], u0, w1, zero), w1, zero)
+        hist.fill(_value(tally, u[miss], u0, w1, zero - 1, np.full(len(w1), float(ekin)), float(ekin)), w1, zero)
     keep = ~miss
     p = p[keep] + d[keep, None] * u[keep]
     u = u[keep]
@@ -148,7 +170,7 @@ def minimc_sphere(scatter, absorption, numdens, radius, ekin, n, source="constan
             sa = np.zeros(m)
         ptr = np.where(ss > 0.0, np.exp(-ss * D), 1.0)
         wt = w * np.exp(-sa * D) * ptr
-        hist.fill(_value(tally, u, u0, wt, nscat), wt, _category(nscat, ninel))
+        hist.fill(_value(tally, u, u0, wt, nscat, E, float(ekin)), _fill_weight(tally, wt), _category(nscat, ninel))
         go = (w > 0.0) & (ss > 0.0)
         u_roul = rng.random(m)
         u_dist = 1.0 - rng.random(m)  # (0, 1]

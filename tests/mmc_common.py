@@ -54,6 +54,68 @@ def pvalue(h1, h2):
     return float(chi2.sf(x2, max(1, n - 1)))
 
 
+def _flat(h):
+    return np.concatenate([[h["underflow"]], h["content"], [h["overflow"]]]), \
+        np.concatenate([[h["underflow_errorsq"]], h["errorsq"], [h["overflow_errorsq"]]])
+
+
+def batched(run, n, k=20, key="total"):
+    """k independent runs of n/k source neutrons (run(seed, n) -> MiniMC
+    result): the summed histogram and the overdispersion factor phi (variance
+    of the summed bin contents estimated from the spread of the batches,
+    over the sum of squared weights, averaged over the bins of at least 100
+    effective entries, at least 1).  Entries of one neutron's history are
+    correlated (successive steps fill the same bins of e.g. the energy
+    tally), so the sum of squared weights underestimates the variance of
+    those bins; a bin holding most of the weight (the unscattered energy) is
+    instead nearly deterministic and must not set the factor alone."""
+    runs = [run(1000 + j, n // k)[key] for j in range(k)]
+    C = np.array([_flat(r)[0] for r in runs])
+    W2 = np.array([_flat(r)[1] for r in runs]).sum(axis=0)
+    var = k * C.var(axis=0, ddof=1)
+    tot = C.sum(axis=0)
+    good = (W2 > 0) & (tot * tot >= 100.0 * np.where(W2 > 0, W2, 1.0))
+    phi = max(1.0, float(np.mean(var[good] / W2[good]))) if good.any() else 1.0
+    h = dict(content=tot[1:-1], errorsq=W2[1:-1], underflow=tot[0], underflow_errorsq=W2[0], overflow=tot[-1],
+             overflow_errorsq=W2[-1])
+    return h, phi
+
+
+def merged(h1, h2, min_eff=10.0):
+    """both histograms with adjacent bins merged, left to right, until each
+    merged bin holds at least min_eff effective entries ((sum w)^2 / sum w^2)
+    in h2 (a last short group joins its left neighbour): the chi-square
+    approximation fails for bins of a few weighted entries"""
+    c2, e2 = h2["content"], h2["errorsq"]
+    groups, start, acc_c, acc_e = [], 0, 0.0, 0.0
+    for i in range(len(c2)):
+        acc_c += c2[i]
+        acc_e += e2[i]
+        if acc_e > 0 and acc_c * acc_c / acc_e >= min_eff:
+            groups.append((start, i + 1))
+            start, acc_c, acc_e = i + 1, 0.0, 0.0
+    if start < len(c2):
+        if groups:
+            groups[-1] = (groups[-1][0], len(c2))
+        else:
+            groups.append((start, len(c2)))
+
+    def merge(h):
+        return dict(h, content=np.array([h["content"][a:b].sum() for a, b in groups]),
+                    errorsq=np.array([h["errorsq"][a:b].sum() for a, b in groups]))
+    return merge(h1), merge(h2)
+
+
+def pvalue_phi(h1, h2, phi):
+    """pvalue with both histograms' variances scaled by phi, after merging
+    the bins sparse in the second one"""
+    def scaled(h):
+        return dict(h, errorsq=h["errorsq"] * phi, underflow_errorsq=h["underflow_errorsq"] * phi,
+                    overflow_errorsq=h["overflow_errorsq"] * phi)
+    a, b = merged(h1, h2)
+    return pvalue(scaled(a), scaled(b))
+
+
 def library_al(temperature=293.15, absorption=True):
     """(scatter, absorption or None, number density) of Al_sg225 in the
     library: ElInc + PowderBragg (MSD of the VDOS) + S(alpha,beta) expanded
@@ -110,3 +172,14 @@ def sphere_setup(scatter_xs, ekin, diameter):
         diameter = 1.0 / (AL_NUMDENS * 100.0 * scatter_xs)
     r = diameter / 2
     return r, -r * (1 - 1e-13)
+
+
+# the reference's tally and roulette cases on aluminium (tests/scripts/
+# mmctally.py: 200 K, 2.5 Aa, each tally; mmcroul.py: 2.0 Aa, four roulette
+# settings (survival probability, weight threshold, scattering threshold)):
+# (reference histogram, temperature, wavelength, tally, roulette)
+ROUL = {"def": (0.1, 1e-2, 2), "roul50p": (0.5, 1.0, 0), "roulw": (0.001, 0.001, 0), "roul5": (1e-12, 1.0, 5)}
+TALLY_CASES = ([(f"mmctally_{t}", 200.0, 2.5, t, ROUL["def"])
+                for t in ("de", "e", "l", "mu", "nscat", "nscat_uw", "q", "theta", "w")]
+               + [(f"mmcroul_{t}_{k}", 293.15, 2.0, t, r) for t in ("theta", "nscat", "nscat_uw", "w")
+                  for k, r in ROUL.items()])

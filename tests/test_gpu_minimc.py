@@ -4,7 +4,8 @@ the reference's own tests require) and against the checker's restatement
 (oracle/pymmc.py, itself pinned to the same histograms) with more neutrons,
 category by category; weight conservation without absorption; a circular
 beam wider than the sphere (misses tallied); the single-scattering limit;
-argument errors."""
+every tally and roulette setting of the reference's tally and roulette
+tests; argument errors."""
 import numpy as np
 import pytest
 
@@ -42,6 +43,41 @@ def test_matches_reference_histogram(amd, case):
     assert p >= 1e-3
     parts = sum(res[c]["content"] for c in core.MINIMC_CATEGORIES)
     assert np.allclose(parts, res["total"]["content"], rtol=1e-9, atol=1e-9)
+
+
+@pytest.fixture(scope="module")
+def al_by_temperature(amd):
+    C, core = amd
+    cache = {}
+
+    def get(T):
+        if T not in cache:
+            cache[T] = M.library_al(T, True)
+        return cache[T]
+    return get
+
+
+@pytest.mark.parametrize("case", M.TALLY_CASES, ids=[c[0] for c in M.TALLY_CASES])
+def test_tallies_and_roulette_match_reference(amd, al_by_temperature, case):
+    """every tally at the reference's binning and every roulette setting of
+    the reference's tests (mmctally.py, mmcroul.py), 1e5 neutrons in 20
+    independent batches (variance factor for correlated entries, sparse bins
+    merged; the comparison of tests/test_minimc_checker.py)"""
+    C, core = amd
+    name, T, wl, tally, roul = case
+    scatter, ab, nd = al_by_temperature(T)
+    ekin = M.wl2ekin(wl)
+    r, z = M.sphere_setup(scatter.crossSectionIsotropic(np.array([ekin]))[0], ekin, None)
+    ref = M.ref_hist(name)
+
+    def run(seed, n):
+        return core.minimc_sphere(scatter, ab, nd, r, ekin, n, pos=(0.0, 0.0, z), seed=seed, tally=tally,
+                                  nbins=ref["nbins"], xrange=(ref["xmin"], ref["xmax"]), roulette=roul)
+    h, phi = M.batched(run, 100000)
+    p = M.pvalue_phi(h, ref, phi)
+    print(f"{name}: integral {h['content'].sum():.1f} (reference {ref['content'].sum():.1f}), phi {phi:.2f}, "
+          f"p {p:.3g}")
+    assert p >= 1e-3
 
 
 def test_matches_checker_by_category(amd, oracle):

@@ -52,3 +52,27 @@ def test_transmission_without_scattering(oracle):
     assert res["info"]["max_steps"] == 0
     assert abs(res["total"]["content"][0] - 1000 * w) <= 1e-9 * 1000
     assert abs(res["NOSCAT"]["content"].sum() - 1000 * w) <= 1e-9 * 1000
+
+
+@pytest.mark.parametrize("case", M.TALLY_CASES, ids=[c[0] for c in M.TALLY_CASES])
+def test_checker_tallies_and_roulette(oracle, case):
+    """every tally (at the reference's binning) and roulette setting; the
+    chi-square uses variances corrected for the correlation of a history's
+    entries (factor phi from 20 independent batches), which the reference's
+    sum of squared weights leaves out: two checker runs differ at p ~ 1e-4
+    on the energy tallies without it"""
+    name, T, wl, tally, roul = case
+    scatter, ab, nd = M.checker_al(oracle, T, True)
+    ekin = M.wl2ekin(wl)
+    r, z = M.sphere_setup(scatter.xs(np.array([ekin]))[0], ekin, None)
+    ref = M.ref_hist(name)
+
+    def run(seed, n):
+        return pymmc.minimc_sphere(scatter, ab, nd, r, ekin, n, pos=(0.0, 0.0, z), seed=seed, sample_seed=seed,
+                                   nthreads=8, tally=tally, nbins=ref["nbins"], xrange=(ref["xmin"], ref["xmax"]),
+                                   roulette=roul)
+    h, phi = M.batched(run, 100000)
+    p = M.pvalue_phi(h, ref, phi)
+    print(f"{name}: integral {h['content'].sum():.1f} (reference {ref['content'].sum():.1f}), phi {phi:.2f}, "
+          f"p {p:.3g}")
+    assert p >= 1e-3
