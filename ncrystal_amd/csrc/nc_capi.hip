@@ -3970,37 +3970,61 @@ unsigned long long read_count(DevBuf& c, hipStream_t st) {
   return v;
 }
 
+// engine buffers of one device, kept between calls (grown to the largest
+// batch seen, released when the process ends): two sets of neutron arrays,
+// cross sections, sampled energies and directions, about 210 B per neutron
+// of a batch, and the tally's per-workgroup partial sums
+struct MmcWork {
+  MmcBufs bufs[2];
+  DevBuf xs_s, xs_a, oE, oD, cnt, hist, part;
+  int64_t cap = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
 // The engine loop (NCMMC_SimEngine.cc:170-449, NCMMC_SimMgr.cc:106-130):
-// source neutrons in batches, each batch stepped until no neutron is left
+// source neutrons in batches, each batch stepped until no neutron is left.
+// Calls on one device are serialised (they share that device's buffers).
 void minimc_run(Proc* ps, Proc* pa, double numdens, double radius, const MmcSource& src, uint64_t nsrc, MmcTally t,
                 const MmcEngine& e, int ignoremiss, uint64_t seed, double* hist_out, double* info) {
   const hipStream_t st = 0;
   const double macro = 100.0 * numdens;  // barn x atoms/Aa^3 -> 1/m
   const double r2 = radius * radius;
-  const int64_t batch = (int64_t)std::min<uint64_t>(nsrc, (uint64_t)1 << 21);
-  MmcBufs bufs[2];
-  bufs[0].alloc((size_t)batch);
-  bufs[1].alloc((size_t)batch);
-  int cur = 0;  // the step's input arrays; the other set receives the scattered neutrons
-  DevBuf xs_s, xs_a, oE, oD, cnt, hist;
-  xs_s.alloc(sizeof(double) * batch);
-  xs_a.alloc(sizeof(double) * batch);
-  oE.alloc(sizeof(double) * batch);
-  oD.alloc(3 * sizeof(double) * batch);
-  cnt.alloc(sizeof(unsigned long long));
+  const int64_t batch = (int64_t)std::min<uint64_t>(nsrc, (uint64_t)1 << 24);
+  int dev = 0;
+  NCAMD_HIP(hipGetDevice(&dev));
+  static std::mutex mtx;
+  static std::map<int, MmcWork*>* works = new std::map<int, MmcWork*>();  // never destroyed: no frees at exit
+  std::lock_guard<std::mutex> lk(mtx);
+  MmcWork*& wp = (*works)[dev];
+  if (!wp) wp = new MmcWork();
+  MmcWork& W = *wp;
   const size_t nh = (size_t)kMmcCats * 2 * (t.nbins + 2);
-  hist.alloc(sizeof(double) * nh);
-  DevBuf part;
-  part.alloc(sizeof(double) * nh * kMmcBlocks);
+  if (W.cap < batch) {
+    NCAMD_HIP(hipStreamSynchronize(st));  // a failed earlier call may have left work queued
+    W.cap = 0;
+    W.bufs[0].alloc((size_t)batch);
+    W.bufs[1].alloc((size_t)batch);
+    W.xs_s.alloc(sizeof(double) * batch);
+    W.xs_a.alloc(sizeof(double) * batch);
+    W.oE.alloc(sizeof(double) * batch);
+    W.oD.alloc(3 * sizeof(double) * batch);
+    W.cap = batch;
+  }
+  if (!W.cnt.p) W.cnt.alloc(sizeof(unsigned long long));
+  if (!W.ev0) NCAMD_HIP(hipEventCreate(&W.ev0));
+  if (!W.ev1) NCAMD_HIP(hipEventCreate(&W.ev1));
+  W.hist.ensure(sizeof(double) * (size_t)kMmcCats * 2 * (kMmcMaxBins + 2));
+  W.part.ensure(sizeof(double) * (size_t)kMmcCats * 2 * (kMmcMaxBins + 2) * kMmcBlocks);
+  MmcBufs* bufs = W.bufs;
+  DevBuf &xs_s = W.xs_s, &xs_a = W.xs_a, &oE = W.oE, &oD = W.oD, &cnt = W.cnt, &hist = W.hist, &part = W.part;
+  int cur = 0;  // the step's input arrays; the other set receives the scattered neutrons
   NCAMD_HIP(hipMemsetAsync(part.p, 0, sizeof(double) * nh * kMmcBlocks, st));
   t.hist = hist.as<double>();
   t.part = part.as<double>();
   const bool scat = ps && macro > 0.0;
   const bool absn = pa && macro > 0.0;
   double steps = 0.0, nsteps = 0.0, maxstep = 0.0, nappend = 0.0, step_ms = 0.0;
-  hipEvent_t ev0, ev1;
-  NCAMD_HIP(hipEventCreate(&ev0));
-  NCAMD_HIP(hipEventCreate(&ev1));
+  const hipEvent_t ev0 = W.ev0, ev1 = W.ev1;
   for (uint64_t base = 0; base < nsrc; base += (uint64_t)batch) {
     const int64_t nb = (int64_t)std::min<uint64_t>((uint64_t)batch, nsrc - base);
     NCAMD_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), st));
@@ -4042,8 +4066,6 @@ void minimc_run(Proc* ps, Proc* pa, double numdens, double radius, const MmcSour
   NCAMD_HIP(hipGetLastError());
   NCAMD_HIP(hipMemcpyAsync(hist_out, hist.p, sizeof(double) * nh, hipMemcpyDeviceToHost, st));
   NCAMD_HIP(hipStreamSynchronize(st));
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
   if (info) {
     info[0] = maxstep;
     info[1] = nsteps;

@@ -268,12 +268,28 @@ __global__ void k_mmc_post(MmcBasket b, int64_t n, const double* __restrict__ oE
   b.d[3 * i + 2] = oD[3 * i + 2];
 }
 
-__global__ void k_mmc_reduce(const double* __restrict__ part, int m, int nblocks, double* __restrict__ hist) {
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (i >= m) return;
+// one workgroup per 64 consecutive histogram entries: wave w sums slices
+// w, w + 16, ... (the lanes read consecutive entries of a slice), then the
+// 16 wave sums are added in a fixed order, so the result does not depend on
+// scheduling
+constexpr int kMmcRedWaves = 16;
+__global__ void __launch_bounds__(64 * kMmcRedWaves) k_mmc_reduce(const double* __restrict__ part, int m, int nblocks,
+                                                                  double* __restrict__ hist) {
+  __shared__ double s_sum[kMmcRedWaves][64];
+  const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+  const int i = (int)blockIdx.x * 64 + lane;
   double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * m + i];
-  hist[i] = s;
+  if (i < m) {
+#pragma unroll 8
+    for (int b = wv; b < nblocks; b += kMmcRedWaves) s += part[(size_t)b * m + i];
+  }
+  s_sum[wv][lane] = s;
+  __syncthreads();
+  if (wv == 0 && i < m) {
+    double tot = 0.0;
+    for (int w = 0; w < kMmcRedWaves; ++w) tot += s_sum[w][lane];
+    hist[i] = tot;
+  }
 }
 
 unsigned mmc_grid(int64_t n) {
@@ -296,7 +312,8 @@ void launch_mmc_step(const MmcBasket& in, int64_t n, const double* xs_s, const d
 }
 void launch_mmc_reduce(const MmcTally& t, hipStream_t st) {
   const int m = kMmcCats * 2 * (t.nbins + 2);
-  hipLaunchKernelGGL(k_mmc_reduce, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, t.part, m, kMmcBlocks, t.hist);
+  hipLaunchKernelGGL(k_mmc_reduce, dim3((unsigned)((m + 63) / 64)), dim3(64 * kMmcRedWaves), 0, st, t.part, m,
+                     kMmcBlocks, t.hist);
 }
 void launch_mmc_post(const MmcBasket& b, int64_t n, const double* oE, const double* oD, hipStream_t st) {
   if (n <= 0) return;
