@@ -240,8 +240,9 @@ def hkl_spacegroup_members(spacegroup, cell, species, dcutoff, fsquarecut=1e-5):
 
 def reci_lattice(cell):
     """Reciprocal lattice (row-major 3x3) of cell = (a, b, c, alpha, beta,
-    gamma) as ncrystal_core/src/utils/NCLatticeUtils.cc:95-149 builds it
-    (90/90/90, 90/90/120 and 90/beta/90 cells)."""
+    gamma) as ncrystal_core/src/utils/NCLatticeUtils.cc:95-149 builds it:
+    closed forms for 90/90/90, 90/90/120 and 90/beta/90 cells, 2 pi times the
+    inverse of the reference's lattice matrix for any other cell."""
     out = np.empty(9)
     C.call("ncamd_reci_lattice", *[float(x) for x in cell], C.dptr(out))
     return out

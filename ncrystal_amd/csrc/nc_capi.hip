@@ -1251,9 +1251,50 @@ int eqrefl_raw(int sg, int h, int k, int l, HKL3* o) {
   return n;
 }
 
-// reciprocal lattice (row-major) of a cell given in degrees, the closed
+// lattice matrix (row-major; angles in radians) of a cell outside the
+// closed forms, as ncrystal_core/src/utils/NCLatticeUtils.cc:29-90
+// (getLatticeRot) builds it: rows (a, 0, 0), (b cos g, b sin g, q),
+// (c cos b, q, r), where q = c (cos a - cos b cos g) / sin g stands in both
+// off-diagonal places (1,2) and (2,1) and r = c sqrt(sin^2 b - (q/c)^2); the
+// reference's d-spacings and cell volumes of triclinic and rhombohedral
+// cells follow from this matrix, so it is kept as it is.  Cosines and sines
+// of 90 and 120 degrees are exact.
+void lattice_general(double a, double b, double c, double al, double be, double ga, double* L) {
+  constexpr double kSin120 = 0.86602540378443864676372317075293618347140262690519;
+  const double tol = 1e-14;
+  const bool g90 = fabs(ga - kPiHalf) < tol, g120 = fabs(ga - kPi * (2 / 3.)) < tol;
+  const double cg = g90 ? 0.0 : (g120 ? -0.5 : std::cos(ga));
+  const double sg = g90 ? 1.0 : (g120 ? kSin120 : std::sin(ga));
+  const double ca = fabs(al - kPiHalf) < tol ? 0.0 : std::cos(al);
+  const bool b90 = fabs(be - kPiHalf) < tol;
+  const double cb = b90 ? 0.0 : std::cos(be), sb = b90 ? 1.0 : std::sin(be);
+  const double t = (ca - cb * cg) / sg, q = c * t;
+  const double r = q == 0.0 ? (cb == 0.0 ? c : c * sb) : c * std::sqrt(sb * sb - t * t);
+  const double rows[9] = {a, 0.0, 0.0, b * cg, b * sg, q, c * cb, q, r};
+  for (int i = 0; i < 9; ++i) L[i] = rows[i];
+}
+
+// hkl loop bounds: |h_i| = |row_i . G| / (2 pi) <= |row_i| / dcutoff with the
+// rows of the lattice matrix (lengths a, b, c for the closed-form cells); a
+// range covering the reference's estimate (NCLatticeUtils.cc:177-242, the
+// same rows mapped over points of a sphere, times 1.05), so the same
+// reflections are met in the same order
+void hkl_bounds(double a, double b, double c, double alpha_deg, double beta_deg, double gamma_deg, double dcutoff,
+                int* mx) {
+  constexpr double kDeg = 0.0174532925199432957692369076848861271344287189;
+  double L[9];
+  lattice_general(a, b, c, alpha_deg * kDeg, beta_deg * kDeg, gamma_deg * kDeg, L);
+  for (int i = 0; i < 3; ++i)
+    mx[i] = floorhkl(std::sqrt(L[3 * i] * L[3 * i] + L[3 * i + 1] * L[3 * i + 1] + L[3 * i + 2] * L[3 * i + 2]) /
+                     dcutoff);
+}
+
+// reciprocal lattice (row-major) of a cell given in degrees: the closed
 // forms of ncrystal_core/src/utils/NCLatticeUtils.cc:105-144 (all angles 90;
-// alpha = beta = 90 and gamma = 120; alpha = gamma = 90); false otherwise
+// alpha = beta = 90 and gamma = 120; alpha = gamma = 90), else 2 pi times the
+// inverse of the lattice matrix (NCLatticeUtils.cc:145-148; an adjugate
+// inverse, equal to the reference's Gauss-Jordan one to rounding); false
+// for angles outside (0, 180) degrees or that admit no cell
 bool reci_lattice(double a, double b, double c, double alpha_deg, double beta_deg, double gamma_deg, double* m) {
   constexpr double kDeg = 0.0174532925199432957692369076848861271344287189;
   const double al = alpha_deg * kDeg, be = beta_deg * kDeg, ga = gamma_deg * kDeg;
@@ -1272,7 +1313,15 @@ bool reci_lattice(double a, double b, double c, double alpha_deg, double beta_de
     const double cotb = 1.0 / tb;
     m[0] = k2Pi / a; m[4] = k2Pi / b; m[6] = -cotb * k2Pi / a; m[8] = k2Pi / (c * sb);
   } else {
-    return false;
+    if (!(al > 0.0 && al < kPi && be > 0.0 && be < kPi && ga > 0.0 && ga < kPi)) return false;
+    double L[9];
+    lattice_general(a, b, c, al, be, ga, L);
+    if (!(L[8] > 0.0)) return false;
+    const double adj[9] = {L[4] * L[8] - L[5] * L[7], L[2] * L[7] - L[1] * L[8], L[1] * L[5] - L[2] * L[4],
+                           L[5] * L[6] - L[3] * L[8], L[0] * L[8] - L[2] * L[6], L[2] * L[3] - L[0] * L[5],
+                           L[3] * L[7] - L[4] * L[6], L[1] * L[6] - L[0] * L[7], L[0] * L[4] - L[1] * L[3]};
+    const double f = k2Pi / (L[0] * adj[0] + L[1] * adj[3] + L[2] * adj[6]);
+    for (int i = 0; i < 9; ++i) m[i] = adj[i] * f;
   }
   return true;
 }
@@ -1288,12 +1337,10 @@ int64_t hkl_spacegroup(int sg, double a, double b, double c, double alpha, doubl
   if (!(a > 0 && b > 0 && c > 0)) raise("BadInput", "lattice parameters must be positive");
   HKLMatrix rm;
   if (!reci_lattice(a, b, c, alpha, beta, gamma, rm.m))
-    raise("BadInput", "unsupported cell: the angles must be 90/90/90, 90/90/120 or 90/beta/90");
-  // |h| <= |G.a|/(2 pi) <= a/dcutoff: a range that covers the reference's
-  // estimate (NCLatticeUtils.cc:177-242), so the same reflections are met in
-  // the same order
-  const double invd = 1.0 / dcutoff;
-  const int max_h = floorhkl(a * invd), max_k = floorhkl(b * invd), max_l = floorhkl(c * invd);
+    raise("BadInput", "invalid cell: angles must lie in (0, 180) degrees and admit a cell");
+  int mx[3];
+  hkl_bounds(a, b, c, alpha, beta, gamma, dcutoff, mx);
+  const int max_h = mx[0], max_k = mx[1], max_l = mx[2];
   HKLBuild P{};
   HKLSpecies S;
   if (nrequired) *nrequired = 0;
@@ -2599,7 +2646,7 @@ int ncamd_reci_lattice(double a, double b, double c, double alpha, double beta, 
   NCAMD_TRY
   if (!(a > 0 && b > 0 && c > 0)) raise("BadInput", "lattice parameters must be positive");
   if (!reci_lattice(a, b, c, alpha, beta, gamma, out9))
-    raise("BadInput", "unsupported cell: the angles must be 90/90/90, 90/90/120 or 90/beta/90");
+    raise("BadInput", "invalid cell: angles must lie in (0, 180) degrees and admit a cell");
   return 0;
   NCAMD_CATCH
   return -1;

@@ -673,10 +673,63 @@ static int eqrefl_list(int sg, hkl3 x, hkl3* o) {
   return m;
 }
 
+/* lattice matrix of a cell (angles in radians) as NCLatticeUtils.cc:29-90
+   (getLatticeRot) builds it for cells outside the closed forms: rows
+   (a, 0, 0), (b cos g, b sin g, q), (c cos b, q, r) with
+   q = c (cos a - cos b cos g) / sin g standing in BOTH off-diagonal places
+   (1,2) and (2,1), and r = c sqrt(sin^2 b - (q/c)^2) (c where cos b = q = 0,
+   c sin b where only q = 0); cosines and sines of 90 and 120 degrees exact.
+   The reference's d-spacings and cell volumes of such cells follow from this
+   matrix (Wollastonite: V = 373.764 Aa^3, where the textbook triclinic cell
+   has 397.8), so the checker uses it as it is. */
+static void lattice_general(double a, double b, double c, double al, double be, double ga, double* L) {
+  const double tol = 1e-14;
+  double cg = cos(ga), sg = sin(ga), ca = cos(al), cb = cos(be), sb = sin(be);
+  if (fabs(ga - K_PIHALF) < tol) { cg = 0.0; sg = 1.0; }
+  else if (fabs(ga - K_PI * (2 / 3.)) < tol) { cg = -0.5; sg = 0.86602540378443864676372317075293618347140262690519; }
+  if (fabs(al - K_PIHALF) < tol) ca = 0.0;
+  if (fabs(be - K_PIHALF) < tol) { cb = 0.0; sb = 1.0; }
+  const double q = c * (ca - cb * cg) / sg;
+  L[0] = a; L[1] = 0.0; L[2] = 0.0;
+  L[3] = b * cg; L[4] = b * sg; L[5] = q;
+  L[6] = c * cb; L[7] = q;
+  if (cb == 0.0 && q == 0.0) L[8] = c;
+  else if (q == 0.0) L[8] = c * sb;
+  else {
+    const double t = (ca - cb * cg) / sg;
+    L[8] = c * sqrt(sb * sb - t * t);
+  }
+}
+
+/* 2 pi times the inverse of a 3x3 matrix (adjugate over determinant; the
+   reference's Gauss-Jordan inverse, NCMatrix.cc:82-115, agrees to rounding) */
+static void inv3_2pi(const double* L, double* m) {
+  const double c00 = L[4] * L[8] - L[5] * L[7], c01 = L[5] * L[6] - L[3] * L[8], c02 = L[3] * L[7] - L[4] * L[6];
+  const double det = L[0] * c00 + L[1] * c01 + L[2] * c02;
+  const double f = K_2PI / det;
+  m[0] = c00 * f; m[1] = (L[2] * L[7] - L[1] * L[8]) * f; m[2] = (L[1] * L[5] - L[2] * L[4]) * f;
+  m[3] = c01 * f; m[4] = (L[0] * L[8] - L[2] * L[6]) * f; m[5] = (L[2] * L[3] - L[0] * L[5]) * f;
+  m[6] = c02 * f; m[7] = (L[1] * L[6] - L[0] * L[7]) * f; m[8] = (L[0] * L[4] - L[1] * L[3]) * f;
+}
+
+/* lengths of the rows of the lattice matrix (a, b, c for the closed-form
+   cells): |h_i| = |row_i . G| / (2 pi) <= |row_i| / d bounds the hkl loops,
+   a range that covers the reference's estimate (NCLatticeUtils.cc:177-242,
+   the same rows mapped over points of a sphere, times 1.05) */
+static void lattice_rows(double a, double b, double c, double alpha_deg, double beta_deg, double gamma_deg,
+                         double* len) {
+  const double kdeg = 0.0174532925199432957692369076848861271344287189;
+  double L[9];
+  lattice_general(a, b, c, alpha_deg * kdeg, beta_deg * kdeg, gamma_deg * kdeg, L);
+  for (int i = 0; i < 3; ++i)
+    len[i] = sqrt(L[3 * i] * L[3 * i] + L[3 * i + 1] * L[3 * i + 1] + L[3 * i + 2] * L[3 * i + 2]);
+}
+
 /* reciprocal lattice of a cell (angles in degrees), the closed forms of
    NCLatticeUtils.cc:105-144: all angles 90, alpha = beta = 90 with gamma =
-   120, and alpha = gamma = 90 (monoclinic, b unique); row-major.  Returns -3
-   for any other cell (the reference inverts the general lattice matrix). */
+   120, and alpha = gamma = 90 (monoclinic, b unique); any other cell the
+   general branch (NCLatticeUtils.cc:145-148): 2 pi times the inverse of the
+   lattice matrix above; row-major.  Returns -3 for angles outside (0, 180). */
 int orc_reci_lattice(double a, double b, double c, double alpha_deg, double beta_deg, double gamma_deg,
                      double* m) {
   const double kdeg = 0.0174532925199432957692369076848861271344287189;
@@ -696,7 +749,11 @@ int orc_reci_lattice(double a, double b, double c, double alpha_deg, double beta
     const double cotb = 1.0 / tb;
     m[0] = K_2PI / a; m[4] = K_2PI / b; m[6] = -cotb * K_2PI / a; m[8] = K_2PI / (c * sb);
   } else {
-    return -3;
+    if (!(al > 0.0 && al < K_PI && be > 0.0 && be < K_PI && ga > 0.0 && ga < K_PI)) return -3;
+    double L[9];
+    lattice_general(a, b, c, al, be, ga, L);
+    if (!(L[8] > 0.0)) return -3;  /* angles that admit no cell */
+    inv3_2pi(L, m);
   }
   return 0;
 }
@@ -734,11 +791,12 @@ int orc_hkl_spacegroup_members(int sg, double a, double b, double c, double alph
   if (sg < 1 || sg > 230) return -4;
   double m[9];
   if (orc_reci_lattice(a, b, c, alpha_deg, beta_deg, gamma_deg, m)) return -3;
-  /* loop bounds: |h| <= |G.a|/(2 pi) <= a/dcutoff, which covers the range the
-     reference estimates (NCLatticeUtils.cc:177-242), so the same reflections
-     are met in the same order */
+  /* loop bounds from the lattice rows, covering the range the reference
+     estimates, so the same reflections are met in the same order */
   const double invd = 1.0 / dcutoff;
-  const int max_h = floorhkl(a * invd), max_k = floorhkl(b * invd), max_l = floorhkl(c * invd);
+  double len[3];
+  lattice_rows(a, b, c, alpha_deg, beta_deg, gamma_deg, len);
+  const int max_h = floorhkl(len[0] * invd), max_k = floorhkl(len[1] * invd), max_l = floorhkl(len[2] * invd);
   if (fsquarecut >= 0.0) fsquarecut = dmax(fsquarecut, 1.0e-300);
   int ns = 0;
   const double** pos = (const double**)malloc(sizeof(double*) * (size_t)(nspecies + 1));

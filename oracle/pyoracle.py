@@ -206,7 +206,7 @@ def hkl_spacegroup_members(spacegroup, cell, species, dcutoff, fsquarecut=1e-5, 
 def reci_lattice(cell):
     m = np.empty(9)
     if lib().orc_reci_lattice(*[float(x) for x in cell], _d(m)):
-        raise RuntimeError("unsupported cell")
+        raise RuntimeError("invalid cell")
     return m
 
 

@@ -5,8 +5,11 @@ ncrystal_core/src/phys_utils/NCEqRefl.cc:25-150) in the CPU checker, pinned by
 * the reference's own tables in tests/scripts/ncmat2hkl.log (committed as
   tests/golden/hkl_spacegroup.json): family count, hkl and multiplicity exact,
   d-spacings to the 10 printed digits, for cubic (225), monoclinic (15,
-  beta = 122 degrees), orthorhombic (62), tetragonal (129), trigonal (167)
-  and hexagonal (186) crystals; |F|^2 to the printed digits for Al, whose
+  beta = 122 degrees), orthorhombic (62), tetragonal (129), trigonal (167),
+  hexagonal (186) and triclinic (Wollastonite, 2: the general lattice
+  branch, NCLatticeUtils.cc:29-90 and 145-148, whose d-spacings and cell
+  volume follow from the reference's lattice matrix as it stands) crystals;
+  |F|^2 to the printed digits for Al, whose
   mean-squared displacement the log's effective Debye temperature fixes
   (the multi-element materials' per-element MSDs come from VDOS curves the
   log does not give, so their |F|^2 is not compared);
@@ -32,10 +35,7 @@ def _species(oracle, m):
              s["positions"]) for s in m["species"]]
 
 
-SUPPORTED = [k for k, m in FIX.items() if not (m["cell"][3] != 90 and m["cell"][5] != 90)]
-
-
-@pytest.mark.parametrize("name", SUPPORTED)
+@pytest.mark.parametrize("name", sorted(FIX))
 def test_reference_tables(oracle, name):
     m = FIX[name]
     hkl, mult, dsp, fsq, moff, mhkl = oracle.hkl_spacegroup_members(m["spacegroup"], m["cell"], _species(oracle, m),
@@ -55,17 +55,43 @@ def test_reference_tables(oracle, name):
     assert np.array_equal(mhkl[moff[:-1]], hkl)
 
 
-def test_general_cell_is_refused(oracle):
-    m = FIX["CaSiO3_sg2_Wollastonite.ncmat"]
-    with pytest.raises(RuntimeError, match="-3"):
-        oracle.hkl_spacegroup_members(m["spacegroup"], m["cell"], _species(oracle, m), m["dcutoff"])
+def test_invalid_input_is_refused(oracle):
     with pytest.raises(RuntimeError, match="-4"):
         oracle.hkl_spacegroup_members(231, (4.0, 4.0, 4.0, 90, 90, 90), [(0.5, 0.001, [[0, 0, 0]])], 1.0)
+    with pytest.raises(RuntimeError, match="-3"):   # angles that admit no cell
+        oracle.hkl_spacegroup_members(2, (4.0, 4.0, 4.0, 60, 60, 179), [(0.5, 0.001, [[0, 0, 0]])], 1.0)
+
+
+def test_general_lattice_matrix(oracle):
+    """The general branch against what fixes it independently: the cell
+    volume the reference prints for Wollastonite (ncmat2hkl.log: Vc
+    373.76436342422 Aa^3) is |det| of its lattice matrix = (2 pi)^3 / |det| of
+    the reciprocal one; reciprocal and direct lattice satisfy row_i(L) .
+    column_j(M) = 2 pi delta_ij; a cell a hair away from the closed forms
+    (outside their 1e-14 tolerance) takes the general branch and gives the
+    closed forms' matrices to rounding.  (The reference's matrix has
+    c (cos a - cos b cos g) / sin g in both off-diagonal places, so its second
+    row is longer than b and, for example, a rhombohedral cell's (100), (010)
+    and (001) planes get different d; the reference's tables follow from it
+    and parity keeps it.)"""
+    m = FIX["CaSiO3_sg2_Wollastonite.ncmat"]
+    M = oracle.reci_lattice(m["cell"]).reshape(3, 3)
+    assert abs((2 * math.pi) ** 3 / abs(np.linalg.det(M)) - 373.76436342422) < 1e-9
+    L = 2 * math.pi * np.linalg.inv(M)
+    assert np.allclose(L @ M, 2 * math.pi * np.eye(3), rtol=0, atol=1e-12)
+    a, b, c, al, be, ga = m["cell"]
+    assert abs(L[0, 0] - a) < 1e-12 and abs(L[0, 1]) < 1e-12 and abs(L[0, 2]) < 1e-12
+    assert abs(np.linalg.norm(L[2]) - c) < 1e-12      # the third row keeps the length c
+    for cell in [(3.1, 3.7, 4.3, 90, 90, 90), (3.2, 3.2, 4.4, 90, 90, 120), (3.1, 3.7, 4.3, 90, 104.5, 90)]:
+        near = tuple(x + (1e-9 if i >= 3 else 0.0) for i, x in enumerate(cell))
+        assert np.allclose(oracle.reci_lattice(near), oracle.reci_lattice(cell), rtol=0, atol=1e-9)
 
 
 # (space group, cell) per Laue class; cells with the metric the class needs
 LAUE = [
     (1, (3.1, 3.7, 4.3, 90, 90, 90)),
+    (2, (3.1, 3.7, 4.3, 97.3, 104.1, 88.2)),
+    (2, (3.9, 3.9, 3.9, 76.5, 76.5, 76.5)),
     (14, (3.1, 3.7, 4.3, 90, 104.5, 90)),
     (62, (3.1, 3.7, 4.3, 90, 90, 90)),
     (80, (3.3, 3.3, 4.1, 90, 90, 90)),
@@ -77,7 +103,7 @@ LAUE = [
     (200, (3.6, 3.6, 3.6, 90, 90, 90)),
     (225, (3.6, 3.6, 3.6, 90, 90, 90)),
 ]
-ORDER = {1: 1, 14: 2, 62: 4, 80: 4, 129: 8, 146: 3, 166: 6, 173: 6, 194: 12, 200: 12, 225: 24}
+ORDER = {1: 1, 2: 1, 14: 2, 62: 4, 80: 4, 129: 8, 146: 3, 166: 6, 173: 6, 194: 12, 200: 12, 225: 24}
 
 
 @pytest.mark.parametrize("sg,cell", LAUE)
