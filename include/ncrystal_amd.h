@@ -299,6 +299,21 @@ ncamd_process_t ncamd_create_scbragg(double v0_times_natoms, int64_t nfam, const
                                      const double* fsquared, const int64_t* mem_off, const int* mem_hkl,
                                      const double* reci_lattice, const double* cry2lab,
                                      double mosaicity_fwhm, double prec, double ntrunc);
+/* Layered crystal by one of the reference models, replaces
+ * LCBragg(const Info&, const SCOrientation&, MosaicityFWHM, const LCAxis&,
+ * mode, 0.0, nullptr, prec, ntrunc) for mode != 0
+ * (ncrystal_core/src/lcbragg/NCLCBragg.cc:62-73): the single crystal
+ * (an SCBragg process of the same orientation, mosaicity and precision)
+ * rotated around the layer axis (lab frame; the reference converts the
+ * crystal-frame lcaxis with the crystal-to-lab rotation).  mode > 0:
+ * LCBraggRef (ncrystal_core/src/extd_utils/NCLCRefModels.cc:25-80), cross
+ * sections averaged over N equally spaced rotations phi_i = 2 pi i / N - pi
+ * (N the smallest prime >= mode), scattering over `mode` random rotations
+ * picked by cross section; mode < 0: LCBraggRndmRot (NCLCRefModels.cc:82-171),
+ * -mode random rotations for both.  Mode 0 (the LCHelper model) raises
+ * BadInput.  The process holds a reference to the SCBragg process; oriented
+ * calls only; it cannot be a component of a composition. */
+ncamd_process_t ncamd_create_lcbragg_ref(ncamd_process_t scbragg, const double* lcaxis_lab, int mode);
 /* read-back: family count, demi-normal count, threshold energy [eV] and 15
  * GaussOnSphere parameters (sigma, truncangle, prec, cos/sin truncangle,
  * expfact, norm, k1, k2, numacc, table size, first argument and inverse step

@@ -13,7 +13,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "libncrystal_amd.so"
 BUILD = HERE / "_build"
-SOURCES = ["nc_kernels.hip", "nc_sab_build.hip", "nc_hkl.hip", "nc_scb.hip", "nc_batch.hip", "nc_kb.hip", "nc_mlp.hip", "nc_vdos.hip", "nc_mmc.hip", "nc_capi.hip"]
+SOURCES = ["nc_kernels.hip", "nc_sab_build.hip", "nc_hkl.hip", "nc_scb.hip", "nc_batch.hip", "nc_kb.hip", "nc_mlp.hip", "nc_vdos.hip", "nc_mmc.hip", "nc_lc.hip", "nc_capi.hip"]
 ARCH = os.environ.get("NCAMD_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -299,6 +299,33 @@ def create_scbragg(v0_times_natoms, dspacing, fsquared, mem_off, mem_hkl, reci_l
     return SCBragg(h, name="SCBragg")
 
 
+class LCBragg(Scatter):
+    """Layered crystal by one of NCrystal's reference models
+    (ncrystal_core/src/lcbragg/NCLCBragg.cc:62-73): a single crystal
+    rotated around the layer axis.  Use crossSection / sampleScatter."""
+
+    def __init__(self, h, crystal, name="LCBragg"):
+        super().__init__(h, name=name)
+        self.crystal = crystal
+
+    def dev_crossSection(self, d_ekin, d_dirs, n, d_out, stream=0):
+        C.call("ncamd_dev_crosssection_many_dirs", self._h, d_ekin, d_dirs, n, d_out, stream)
+
+    def dev_sampleScatter(self, d_ekin, d_dirs, n, d_oe, d_od, stream=0):
+        C.call("ncamd_dev_samplescatter_many_dirs", self._h, d_ekin, d_dirs, n, d_oe, d_od, stream)
+
+
+def create_lcbragg_ref(crystal, lcaxis_lab, mode):
+    """LCBragg with lcmode = mode != 0 over an SCBragg process `crystal`
+    (NCLCRefModels.cc): mode > 0 averages cross sections over the smallest
+    prime >= mode of equally spaced rotations around lcaxis_lab and scatters
+    over `mode` random ones picked by cross section; mode < 0 uses -mode
+    random rotations for both."""
+    ax = C.as_f64(np.asarray(lcaxis_lab, dtype=np.float64).ravel())
+    h = C.call("ncamd_create_lcbragg_ref", crystal._h, C.dptr(ax), int(mode))
+    return LCBragg(h, crystal)
+
+
 def create_powderbragg(v0_times_natoms, dspacings, fsq_x_mult, fp32_tables=True):
     d = C.as_f64(dspacings)
     f = C.as_f64(fsq_x_mult)

@@ -251,6 +251,13 @@ struct Proc {
   std::vector<double> scb_inv2d_h, scb_xsfact_h, scb_normals_h, scb_lt1_h, scb_lt2_h;
   std::vector<int64_t> scb_off_h;
   double scb_gos[15] = {};
+  // layered crystal by a reference model: the SCBragg process is comps[0];
+  // lc_mode > 0 averages cross sections over lc_nprime equally spaced
+  // rotations and samples over lc_n random ones, lc_mode < 0 uses lc_n
+  // random rotations for both
+  int lc_mode = 0, lc_n = 0, lc_nprime = 0;
+  LCView lc{};
+  DevBuf lc_cs, lc_E, lc_D, lc_xs, lc_csr, lc_sE, lc_sD, lc_sel, lc_oE, lc_oD;
   // composition
   std::vector<Proc*> comps;
   std::vector<double> scales;
@@ -314,6 +321,7 @@ CompView make_view(Proc* p) {
         break;
       case kKindElInc: v.ei = c->ei; break;
       case kKindSAB: break;  // staged below
+      case kKindLC: raise("BadInput", "oriented process: use the calls with neutron directions");
       case kKindSCB:
         if (p->kind != kKindNone) raise("BadInput", "oriented process: use the calls with neutron directions");
         v.has_scb = 1;
@@ -2364,6 +2372,7 @@ void proc_domain(const Proc* p, double& lo, double& hi) {
       hi = inf;
       return;
     case kKindSCB: lo = p->scb.threshold; hi = inf; return;
+    case kKindLC: proc_domain(p->comps[0], lo, hi); return;
     case kKindNone: {
       lo = 0.0; hi = 0.0;
       for (const Proc* c : p->comps) {
@@ -2386,6 +2395,7 @@ const char* proc_name(const Proc* p) {
     case kKindElInc: return "ElIncScatter";
     case kKindSAB: return "SABScatter";
     case kKindSCB: return "SCBragg";
+    case kKindLC: return "LCBragg";
     case kKindFG: return "FreeGas";
     case kKindAbs: return "AbsOOV";
     case kKindNone: return "ProcComposition";
@@ -2394,6 +2404,7 @@ const char* proc_name(const Proc* p) {
 }
 
 // a process reading the tables of `s` with its own stream and staging memory
+uint64_t clone_seed(uint64_t seed, uint64_t salt);
 Proc* clone_proc(Proc* s, uint64_t seed) {
   ensure_device();
   Proc* root = s->src ? s->src : s;  // clones of clones read the root's tables
@@ -2412,10 +2423,18 @@ Proc* clone_proc(Proc* s, uint64_t seed) {
   c->scb_inv2d_h = s->scb_inv2d_h; c->scb_xsfact_h = s->scb_xsfact_h; c->scb_normals_h = s->scb_normals_h;
   c->scb_lt1_h = s->scb_lt1_h; c->scb_lt2_h = s->scb_lt2_h; c->scb_off_h = s->scb_off_h;
   std::copy(s->scb_gos, s->scb_gos + 15, c->scb_gos);
+  c->lc_mode = s->lc_mode; c->lc_n = s->lc_n; c->lc_nprime = s->lc_nprime; c->lc = s->lc;
   c->comps = s->comps;
   c->scales = s->scales;
   init_errbuf(c.get());
   for (Proc* x : c->comps) x->refcount++;
+  if (s->kind == kKindLC) {  // the crystal's work buffers are per process: the clone gets its own crystal
+    Proc* own = clone_proc(s->comps[0], clone_seed(seed, 1));
+    --c->comps[0]->refcount;
+    c->comps[0] = own;
+    c->lc_cs.alloc(s->lc_cs.bytes);
+    NCAMD_HIP(hipMemcpy(c->lc_cs.p, s->lc_cs.p, s->lc_cs.bytes, hipMemcpyDeviceToDevice));
+  }
   c->src = root;
   root->refcount++;
   root->nclones++;
@@ -2552,6 +2571,74 @@ ScbWork scb_work(Proc* p, int64_t n, hipStream_t st) {
   return w;
 }
 
+// ---- layered crystal by the reference models (NCLCRefModels.cc:25-171)
+constexpr int64_t kLcLanes = int64_t(1) << 23;  // rotated neutrons per pass
+
+// work arrays for `lanes` rotated neutrons and m neutrons
+void lc_buffers(Proc* p, int64_t lanes, int64_t m, hipStream_t st) {
+  const size_t L = (size_t)std::max<int64_t>(lanes, 1), M = (size_t)std::max<int64_t>(m, 1);
+  if (p->lc_E.bytes < sizeof(double) * L || p->lc_sE.bytes < sizeof(double) * M) {
+    NCAMD_HIP(hipStreamSynchronize(st));  // the old buffers may still be in use on this stream
+    p->lc_E.ensure(sizeof(double) * L);
+    p->lc_D.ensure(3 * sizeof(double) * L);
+    p->lc_xs.ensure(sizeof(double) * L);
+    p->lc_csr.ensure(sizeof(double2) * L);
+    p->lc_sE.ensure(sizeof(double) * M);
+    p->lc_sD.ensure(3 * sizeof(double) * M);
+    p->lc_sel.ensure(sizeof(int) * M);
+    p->lc_oE.ensure(sizeof(double) * M);
+    p->lc_oD.ensure(3 * sizeof(double) * M);
+  }
+}
+
+// cross sections (LCBraggRef::crossSection: mean over equally spaced
+// rotations phi_i = 2 pi i / N - pi, N the smallest prime >= nsample;
+// LCBraggRndmRot::crossSection: mean over nsample random rotations, new ones
+// on every call)
+void lc_xs(Proc* p, const double* E, const double* D, int64_t n, double* out, hipStream_t st) {
+  Proc* sc = p->comps[0];
+  const bool fixed = p->lc_mode > 0;
+  const int N = fixed ? p->lc_nprime : p->lc_n;
+  const int64_t chunk = std::max<int64_t>(1, kLcLanes / N);
+  for (int64_t i0 = 0; i0 < n; i0 += chunk) {
+    const int64_t m = std::min(chunk, n - i0);
+    lc_buffers(p, m * N, m, st);
+    launch_lc_expand(E + i0, D + 3 * i0, m, N, p->lc, fixed ? p->lc_cs.as<double2>() : nullptr, p->seed,
+                     fixed ? 0u : next_stream(p), p->lc_E.as<double>(), p->lc_D.as<double>(), p->lc_csr.as<double2>(),
+                     st);
+    const ScbWork w = scb_work(sc, m * N, st);
+    launch_scb_xs(sc->scb, p->lc_E.as<double>(), p->lc_D.as<double>(), m * N, p->lc_xs.as<double>(), w, st);
+    launch_lc_mean(p->lc_xs.as<double>(), m, N, out + i0, st);
+  }
+}
+
+// scattering (LCBraggRef / LCBraggRndmRot::sampleScatter): nsample random
+// rotations, one picked with probability proportional to its cross section,
+// the crystal's scattering of the rotated neutron, the outgoing direction
+// rotated back; neutrons without cross section at every rotation leave
+// unchanged.  The random-rotation model draws new rotations here instead of
+// reusing those of a preceding cross-section call (the same distribution).
+void lc_sample(Proc* p, const double* E, const double* D, int64_t n, double* oE, double* oD, hipStream_t st) {
+  Proc* sc = p->comps[0];
+  const int N = p->lc_n;
+  const int64_t chunk = std::max<int64_t>(1, kLcLanes / N);
+  for (int64_t i0 = 0; i0 < n; i0 += chunk) {
+    const int64_t m = std::min(chunk, n - i0);
+    lc_buffers(p, m * N, m, st);
+    launch_lc_expand(E + i0, D + 3 * i0, m, N, p->lc, nullptr, p->seed, next_stream(p), p->lc_E.as<double>(),
+                     p->lc_D.as<double>(), p->lc_csr.as<double2>(), st);
+    const ScbWork wx = scb_work(sc, m * N, st);
+    launch_scb_xs(sc->scb, p->lc_E.as<double>(), p->lc_D.as<double>(), m * N, p->lc_xs.as<double>(), wx, st);
+    launch_lc_pick(p->lc_xs.as<double>(), p->lc_E.as<double>(), p->lc_D.as<double>(), m, N, p->seed, next_stream(p),
+                   p->lc_sE.as<double>(), p->lc_sD.as<double>(), p->lc_sel.as<int>(), st);
+    const ScbWork ws = scb_work(sc, m, st);
+    launch_scb_sample(sc->scb, p->lc_sE.as<double>(), p->lc_sD.as<double>(), m, p->seed, next_stream(p),
+                      p->lc_oE.as<double>(), p->lc_oD.as<double>(), ws, st);
+    launch_lc_unrotate(p->lc_oD.as<double>(), p->lc_sel.as<int>(), p->lc_csr.as<double2>(), E + i0, D + 3 * i0, m, N,
+                       p->lc, oE + i0, oD + 3 * i0, st);
+  }
+}
+
 
 }  // namespace
 }  // namespace ncamd
@@ -2593,7 +2680,10 @@ int ncamd_valid(ncamd_process_t h) {
   return (p && p->magic == kMagic) ? 1 : 0;
 }
 int ncamd_isoriented(ncamd_process_t h) {
-  NCAMD_TRY return (extract(h)->kind == kKindSCB || extract(h)->oriented_comp) ? 1 : 0; NCAMD_CATCH
+  NCAMD_TRY
+  const Proc* p = extract(h);
+  return (p->kind == kKindSCB || p->kind == kKindLC || p->oriented_comp) ? 1 : 0;
+  NCAMD_CATCH
   return -1;
 }
 
@@ -2650,6 +2740,52 @@ int ncamd_reci_lattice(double a, double b, double c, double alpha, double beta, 
   return 0;
   NCAMD_CATCH
   return -1;
+}
+
+// LCBragg by a reference model (NCLCBragg.cc:62-73, NCLCRefModels.cc:25-171):
+// mode > 0 LCBraggRef with nsample = mode, mode < 0 LCBraggRndmRot with
+// nsample = -mode; the layer axis in the lab frame
+ncamd_process_t ncamd_create_lcbragg_ref(ncamd_process_t scbragg, const double* lcaxis_lab, int mode) {
+  NCAMD_TRY
+  ensure_device();
+  Proc* sc = extract(scbragg);
+  if (sc->kind != kKindSCB) raise("BadInput", "LCBragg: the crystal must be an SCBragg process");
+  if (mode == 0) raise("BadInput", "LCBragg: mode 0 (the layered-crystal helper) is not available; use a reference model (mode != 0)");
+  if (mode > 100000 || mode < -100000) raise("BadInput", "LCBragg: at most 100000 rotations per neutron");
+  const double ax = lcaxis_lab[0], ay = lcaxis_lab[1], az = lcaxis_lab[2];
+  const double norm = std::sqrt(ax * ax + ay * ay + az * az);
+  if (!(norm > 0.0) || !std::isfinite(norm)) raise("BadInput", "LCBragg: invalid layer axis");
+  std::unique_ptr<Proc> p(new Proc());
+  p->kind = kKindLC;
+  p->lc_mode = mode;
+  p->lc_n = mode > 0 ? mode : -mode;
+  int np = p->lc_n;
+  auto prime = [](int k) {
+    if (k < 2) return false;
+    for (int d = 2; (int64_t)d * d <= k; ++d)
+      if (k % d == 0) return false;
+    return true;
+  };
+  while (!prime(np)) ++np;
+  p->lc_nprime = np;
+  p->lc = LCView{ax / norm, ay / norm, az / norm};
+  if (mode > 0) {  // rotations 2 pi i / N - pi of the cross sections
+    std::vector<double> cs(2 * (size_t)np);
+    const double dphi = k2Pi / np;
+    for (int i = 0; i < np; ++i) {
+      const double phi = i * dphi - kPi;
+      cs[2 * i] = std::cos(phi);
+      cs[2 * i + 1] = std::sin(phi);
+    }
+    p->lc_cs.upload(cs.data(), cs.size());
+  }
+  sc->refcount++;
+  p->comps.push_back(sc);
+  p->scales.push_back(1.0);
+  init_errbuf(p.get());
+  return wrap(p.release());
+  NCAMD_CATCH
+  return wrap(nullptr);
 }
 
 ncamd_process_t ncamd_create_scbragg(double v0n, int64_t nfam, const double* dsp, const double* fsq,
@@ -2781,6 +2917,7 @@ ncamd_process_t ncamd_create_composition(int n, const ncamd_process_t* procs, co
   if (p->comps.size() > (size_t)kMaxComp) raise("BadInput", "composition has more than 8 components after merging");
   int seen[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (Proc* c : p->comps) {
+    if (c->kind == kKindLC) raise("BadInput", "an LCBragg process cannot be a component of a composition");
     if (c->kind == kKindSCB) p->oriented_comp = true;  // oriented composition: calls with directions only
     if (c->kind == kKindSCB && seen[c->kind]) raise("BadInput", "at most one SCBragg process per composition");
     seen[c->kind]++;
@@ -2964,6 +3101,9 @@ void ncamd_samplescatter_many_dirs(ncamd_process_t h, const double* ekin, const 
     const ScbWork w = scb_work(p, (int64_t)n, 0);
     launch_scb_sample(p->scb, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->seed, next_stream(p),
                       p->st_out1.as<double>(), p->st_out2.as<double>(), w, 0);
+  } else if (p->kind == kKindLC) {
+    lc_sample(p, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->st_out1.as<double>(),
+              p->st_out2.as<double>(), 0);
   } else {
     sample_dirs_dispatch(p, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->st_out1.as<double>(),
                          p->st_out2.as<double>(), 0);
@@ -2995,7 +3135,8 @@ void ncamd_samplescatter_many(ncamd_process_t h, double ekin, const double (*dir
 void ncamd_crosssection_many_dirs(ncamd_process_t h, const double* ekin, const double* dirs, unsigned long n,
                                   double* out) {
   bool scb = false, oc = false;
-  NCAMD_TRY scb = extract(h)->kind == kKindSCB; oc = extract(h)->oriented_comp; NCAMD_CATCH
+  NCAMD_TRY scb = extract(h)->kind == kKindSCB || extract(h)->kind == kKindLC; oc = extract(h)->oriented_comp;
+  NCAMD_CATCH
   if (!scb && !oc) {
     xs_many_impl(h, ekin, n, 1, out, false);  // isotropic materials: the direction does not enter
     return;
@@ -3013,6 +3154,8 @@ void ncamd_crosssection_many_dirs(ncamd_process_t h, const double* ekin, const d
   }
   if (p->oriented_comp) {
     oriented_xs(p, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->st_out1.as<double>(), 0);
+  } else if (p->kind == kKindLC) {
+    lc_xs(p, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->st_out1.as<double>(), 0);
   } else {
     const ScbWork w = scb_work(p, (int64_t)n, 0);
     launch_scb_xs(p->scb, p->st_in.as<double>(), p->st_in2.as<double>(), (int64_t)n, p->st_out1.as<double>(), w, 0);
@@ -3082,6 +3225,8 @@ void dev_sample_dirs(Proc* p, const double* d_ekin, const double* d_dirs, int64_
   if (p->kind == kKindSCB) {
     const ScbWork w = scb_work(p, n, st);
     launch_scb_sample(p->scb, d_ekin, d_dirs, n, p->seed, next_stream(p), d_oE, d_oD, w, st);
+  } else if (p->kind == kKindLC) {
+    lc_sample(p, d_ekin, d_dirs, n, d_oE, d_oD, st);
   } else {
     sample_dirs_dispatch(p, d_ekin, d_dirs, n, d_oE, d_oD, st);
   }
@@ -3090,6 +3235,8 @@ void dev_xs_dirs(Proc* p, const double* d_ekin, const double* d_dirs, int64_t n,
   if (p->kind == kKindSCB) {
     const ScbWork w = scb_work(p, n, st);
     launch_scb_xs(p->scb, d_ekin, d_dirs, n, d_out, w, st);
+  } else if (p->kind == kKindLC) {
+    lc_xs(p, d_ekin, d_dirs, n, d_out, st);
   } else if (p->oriented_comp) {
     oriented_xs(p, d_ekin, d_dirs, n, d_out, st);
   } else {
@@ -3879,7 +4026,7 @@ const char* ncrystal_name(ncrystal_process_t p) {
 int ncrystal_isnonoriented(ncrystal_process_t p) {
   NCAMD_TRY
   const Proc* q = proc_of(p.internal);
-  return (q->kind == kKindSCB || q->oriented_comp) ? 0 : 1;
+  return (q->kind == kKindSCB || q->kind == kKindLC || q->oriented_comp) ? 0 : 1;
   NCAMD_CATCH
   return 0;
 }

@@ -7,7 +7,8 @@
 
 namespace ncamd {
 
-enum : int { kKindNone = 0, kKindPB = 1, kKindElInc = 2, kKindSAB = 3, kKindSCB = 4, kKindFG = 5, kKindAbs = 6 };
+enum : int { kKindNone = 0, kKindPB = 1, kKindElInc = 2, kKindSAB = 3, kKindSCB = 4, kKindFG = 5, kKindAbs = 6,
+             kKindLC = 7 };
 constexpr int kMaxComp = 8;
 // components of a composition with an S(alpha,beta) part that are sampled
 // by their own kernels after the pick: every S(alpha,beta) component (one
@@ -326,6 +327,21 @@ void launch_scb_xs(const SCBView&, const double* ekin, const double* dirs, int64
 void launch_scb_sample(const SCBView&, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
                        uint32_t stream, double* oE, double* oD, const ScbWork&, hipStream_t,
                        const int64_t* idmap = nullptr, int skip = 0);
+// layered crystal by the reference models (nc_lc.hip): the SCBragg kernels
+// over rotations of each neutron around the layer axis (unit, lab frame)
+struct LCView {
+  double ax, ay, az;
+};
+// n x N rotated neutrons (lane i * N + j): rotation j from the table cs, or
+// random (then its (cos, sin) goes to ocs)
+void launch_lc_expand(const double* ekin, const double* dirs, int64_t n, int N, const LCView&, const double2* cs,
+                      uint64_t seed, uint32_t stream, double* oE, double* oD, double2* ocs, hipStream_t);
+void launch_lc_mean(const double* xs, int64_t n, int N, double* out, hipStream_t);
+void launch_lc_pick(const double* xs, const double* rE, const double* rD, int64_t n, int N, uint64_t seed,
+                    uint32_t stream, double* sE, double* sD, int* sel, hipStream_t);
+void launch_lc_unrotate(const double* scD, const int* sel, const double2* cs, const double* ekin, const double* dirs,
+                        int64_t n, int N, const LCView&, double* oE, double* oD, hipStream_t);
+
 // compacts the neutrons listed in q (count *qc <= nmax): energies, directions
 // and original indices
 void launch_scb_gather(const int64_t* q, const unsigned long long* qc, int64_t nmax, const double* ekin,

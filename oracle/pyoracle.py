@@ -7,6 +7,8 @@ import ctypes
 import os
 import pathlib
 
+import math
+
 import numpy as np
 
 _HERE = pathlib.Path(__file__).resolve().parent
@@ -599,6 +601,80 @@ def scbragg(a, b, c, v0_times_natoms, dsp, fsq, mem_off, mem_hkl, cry2lab, mos_f
     ptr = lib().orc_create_scbragg(_d(L), v0_times_natoms, d.size, _d(d), _d(f), mo.ctypes.data_as(_i64p),
                                    _i(mh), _d(R), mos_fwhm, prec, ntrunc)
     return SCB(ptr, keep=(L,))
+
+
+def _lc_rotate(axis, v, c, s):
+    """Rodrigues' rotation of unit vectors v (n, 3) by (cos, sin) (n,) around
+    the unit axis, NCRotMatrix.hh:172-184: v c + (a x v) s + a ((a . v)(1 - c))"""
+    cr = np.cross(axis, v)
+    dt = v @ axis
+    return v * c[:, None] + cr * s[:, None] + np.outer(dt * (1.0 - c), axis)
+
+
+class LCBraggRef:
+    """The layered-crystal reference models of
+    ncrystal_core/src/extd_utils/NCLCRefModels.cc over a checker SCBragg
+    process: mode > 0 LCBraggRef (cross section: mean over N equally spaced
+    rotations phi_i = 2 pi i / N - pi around the layer axis, N the smallest
+    prime >= mode, :25-52; scattering: `mode` random rotations, one picked by
+    cross section, the crystal's scattering of the rotated neutron rotated
+    back, :54-80), mode < 0 LCBraggRndmRot (-mode random rotations for both,
+    :82-171).  Random numbers from numpy's PCG64."""
+
+    def __init__(self, scb, lcaxis_lab, mode):
+        self.scb = scb
+        a = np.asarray(lcaxis_lab, dtype=np.float64)
+        self.axis = a / np.linalg.norm(a)
+        self.mode = int(mode)
+        self.nsample = abs(self.mode)
+        n = self.nsample
+        while n < 2 or any(n % k == 0 for k in range(2, int(math.isqrt(n)) + 1)):
+            n += 1
+        self.nprime = n
+
+    def _expand(self, ekin, dirs, c, s):
+        """(n*N) energies and rotated directions; c, s of shape (n, N)"""
+        n, N = c.shape
+        u = dirs / np.linalg.norm(dirs, axis=1)[:, None]
+        U = np.repeat(u, N, axis=0)
+        return np.repeat(ekin, N), _lc_rotate(self.axis, U, c.ravel(), s.ravel())
+
+    def xs_dir(self, ekin, dirs, seed=0, nthreads=8):
+        ekin = np.asarray(ekin, dtype=np.float64)
+        dirs = np.asarray(dirs, dtype=np.float64).reshape(-1, 3)
+        n = len(ekin)
+        if self.mode > 0:
+            N = self.nprime
+            phi = np.arange(N) * (2 * math.pi / N) - math.pi
+            c, s = np.tile(np.cos(phi), (n, 1)), np.tile(np.sin(phi), (n, 1))
+        else:
+            N = self.nsample
+            phi = np.random.default_rng(seed).uniform(-math.pi, math.pi, size=(n, N))
+            c, s = np.cos(phi), np.sin(phi)
+        E, D = self._expand(ekin, dirs, c, s)
+        x = self.scb.xs_dir(E, np.ascontiguousarray(D), nthreads=nthreads).reshape(n, N)
+        return np.array([math.fsum(r) for r in x]) / N
+
+    def sample_dir(self, ekin, dirs, seed=0, nthreads=8):
+        ekin = np.asarray(ekin, dtype=np.float64)
+        dirs = np.asarray(dirs, dtype=np.float64).reshape(-1, 3)
+        n, N = len(ekin), self.nsample
+        rng = np.random.default_rng(seed)
+        phi = rng.uniform(-math.pi, math.pi, size=(n, N))
+        c, s = np.cos(phi), np.sin(phi)
+        E, D = self._expand(ekin, dirs, c, s)
+        x = self.scb.xs_dir(E, np.ascontiguousarray(D), nthreads=nthreads).reshape(n, N)
+        cum = np.cumsum(x, axis=1)
+        tot = cum[:, -1]
+        live = tot > 0.0
+        u = rng.random(n) * tot
+        pick = np.minimum((cum <= u[:, None]).sum(axis=1), N - 1)
+        sel = np.arange(n) * N + pick
+        oE, oD, nerr = self.scb.sample_dir(E[sel], np.ascontiguousarray(D[sel]), seed=seed + 1, nthreads=nthreads)
+        oD = np.asarray(oD).reshape(-1, 3)
+        back = _lc_rotate(self.axis, oD, c.ravel()[sel], -s.ravel()[sel])
+        out = np.where(live[:, None], back, dirs)
+        return np.asarray(ekin).copy(), out, nerr
 
 
 def debug_sort_fams(dsp, fsq, mult, hkl):
