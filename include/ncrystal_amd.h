@@ -314,6 +314,21 @@ ncamd_process_t ncamd_create_scbragg(double v0_times_natoms, int64_t nfam, const
  * BadInput.  The process holds a reference to the SCBragg process; oriented
  * calls only; it cannot be a component of a composition. */
 ncamd_process_t ncamd_create_lcbragg_ref(ncamd_process_t scbragg, const double* lcaxis_lab, int mode);
+/* Layered crystal, replaces LCBragg(const Info&, const SCOrientation&,
+ * MosaicityFWHM, const LCAxis&, mode, 0.0, nullptr, prec, ntrunc)
+ * (ncrystal_core/src/lcbragg/NCLCBragg.cc:36-73) with the standard plane
+ * provider: the SCBragg inputs of ncamd_create_scbragg and the layer axis in
+ * the crystal frame.  mode 0: the layered-crystal model (LCHelper,
+ * ncrystal_core/src/extd_utils/NCLCUtils.cc:58-516: plane sets of equal
+ * (d, angle to the axis), rotation ranges reaching the mosaic band, Romberg
+ * averages over them) for cross sections; its scattering raises BadInput.
+ * mode != 0: as ncamd_create_lcbragg_ref with the axis taken to the lab
+ * frame by cry2lab. */
+ncamd_process_t ncamd_create_lcbragg(double v0_times_natoms, int64_t nfam, const double* dspacing,
+                                     const double* fsquared, const int64_t* mem_off, const int* mem_hkl,
+                                     const double* reci_lattice, const double* cry2lab,
+                                     const double* lcaxis_crystal, double mosaicity_fwhm, double prec,
+                                     double ntrunc, int mode);
 /* read-back: family count, demi-normal count, threshold energy [eV] and 15
  * GaussOnSphere parameters (sigma, truncangle, prec, cos/sin truncangle,
  * expfact, norm, k1, k2, numacc, table size, first argument and inverse step

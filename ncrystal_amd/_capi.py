@@ -117,6 +117,8 @@ def lib():
         "ncamd_create_scbragg": (Handle, [ctypes.c_double, ctypes.c_int64, _dp, _dp, _i64p, _ip, _dp, _dp,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "ncamd_create_lcbragg_ref": (Handle, [Handle, _dp, ctypes.c_int]),
+        "ncamd_create_lcbragg": (Handle, [ctypes.c_double, ctypes.c_int64, _dp, _dp, _i64p, _ip, _dp, _dp, _dp,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
         "ncamd_scbragg_info": (ctypes.c_int, [Handle, _i64p, _i64p, _dp, _dp]),
         "ncamd_scbragg_tables": (ctypes.c_int, [Handle, _dp, _dp, _i64p, _dp, _dp, _dp]),
         "ncamd_create_powderbragg": (Handle, [ctypes.c_double, _dp, _dp, ctypes.c_int64, ctypes.c_int]),

@@ -326,6 +326,23 @@ def create_lcbragg_ref(crystal, lcaxis_lab, mode):
     return LCBragg(h, crystal)
 
 
+def create_lcbragg(v0_times_natoms, dspacing, fsquared, mem_off, mem_hkl, reci_lattice, cry2lab, lcaxis_crystal,
+                   mosaicity_fwhm, prec=1e-3, ntrunc=0.0, mode=0):
+    """LCBragg from the inputs of create_scbragg and the layer axis in the
+    crystal frame (NCLCBragg.cc:36-73): mode 0 the layered-crystal model
+    (cross sections), else a reference model (see create_lcbragg_ref)."""
+    d, f = C.as_f64(dspacing), C.as_f64(fsquared)
+    mo = np.ascontiguousarray(mem_off, dtype=np.int64)
+    mh = np.ascontiguousarray(mem_hkl, dtype=np.int32).ravel()
+    L = C.as_f64(np.asarray(reci_lattice, dtype=np.float64).ravel())
+    R = C.as_f64(np.asarray(cry2lab, dtype=np.float64).ravel())
+    ax = C.as_f64(np.asarray(lcaxis_crystal, dtype=np.float64).ravel())
+    h = C.call("ncamd_create_lcbragg", float(v0_times_natoms), d.size, C.dptr(d), C.dptr(f), C.i64ptr(mo),
+               C.iptr(mh), C.dptr(L), C.dptr(R), C.dptr(ax), float(mosaicity_fwhm), float(prec), float(ntrunc),
+               int(mode))
+    return LCBragg(h, None)
+
+
 def create_powderbragg(v0_times_natoms, dspacings, fsq_x_mult, fp32_tables=True):
     d = C.as_f64(dspacings)
     f = C.as_f64(fsq_x_mult)

@@ -257,6 +257,8 @@ struct Proc {
   // random rotations for both
   int lc_mode = 0, lc_n = 0, lc_nprime = 0;
   LCView lc{};
+  LC0View lc0{};  // lcmode=0: plane sets (device copy in lc0_sets)
+  DevBuf lc0_sets;
   DevBuf lc_cs, lc_E, lc_D, lc_xs, lc_csr, lc_sE, lc_sD, lc_sel, lc_oE, lc_oD;
   // composition
   std::vector<Proc*> comps;
@@ -2424,6 +2426,7 @@ Proc* clone_proc(Proc* s, uint64_t seed) {
   c->scb_lt1_h = s->scb_lt1_h; c->scb_lt2_h = s->scb_lt2_h; c->scb_off_h = s->scb_off_h;
   std::copy(s->scb_gos, s->scb_gos + 15, c->scb_gos);
   c->lc_mode = s->lc_mode; c->lc_n = s->lc_n; c->lc_nprime = s->lc_nprime; c->lc = s->lc;
+  c->lc0 = s->lc0;  // plane sets read from the source's buffer (the clone keeps the source alive)
   c->comps = s->comps;
   c->scales = s->scales;
   init_errbuf(c.get());
@@ -2597,6 +2600,10 @@ void lc_buffers(Proc* p, int64_t lanes, int64_t m, hipStream_t st) {
 // on every call)
 void lc_xs(Proc* p, const double* E, const double* D, int64_t n, double* out, hipStream_t st) {
   Proc* sc = p->comps[0];
+  if (p->lc_mode == 0) {
+    launch_lc0_xs(p->lc0, sc->scb, E, D, n, out, st);
+    return;
+  }
   const bool fixed = p->lc_mode > 0;
   const int N = fixed ? p->lc_nprime : p->lc_n;
   const int64_t chunk = std::max<int64_t>(1, kLcLanes / N);
@@ -2619,6 +2626,8 @@ void lc_xs(Proc* p, const double* E, const double* D, int64_t n, double* out, hi
 // unchanged.  The random-rotation model draws new rotations here instead of
 // reusing those of a preceding cross-section call (the same distribution).
 void lc_sample(Proc* p, const double* E, const double* D, int64_t n, double* oE, double* oD, hipStream_t st) {
+  if (p->lc_mode == 0)
+    raise("BadInput", "LCBragg lcmode=0: scattering is not available on the GPU; use a reference model (lcmode != 0)");
   Proc* sc = p->comps[0];
   const int N = p->lc_n;
   const int64_t chunk = std::max<int64_t>(1, kLcLanes / N);
@@ -2781,6 +2790,86 @@ ncamd_process_t ncamd_create_lcbragg_ref(ncamd_process_t scbragg, const double* 
   }
   sc->refcount++;
   p->comps.push_back(sc);
+  p->scales.push_back(1.0);
+  init_errbuf(p.get());
+  return wrap(p.release());
+  NCAMD_CATCH
+  return wrap(nullptr);
+}
+
+// LCBragg (NCLCBragg.cc:36-73) from the inputs of an SCBragg of the crystal
+// and the layer axis in the crystal frame: mode 0 the layered-crystal model
+// (LCHelper, NCLCUtils.cc:58-131 plane sets), else a reference model
+ncamd_process_t ncamd_create_lcbragg(double v0n, int64_t nfam, const double* dsp, const double* fsq,
+                                     const int64_t* mem_off, const int* mem_hkl, const double* reci_lattice,
+                                     const double* cry2lab, const double* lcaxis_crystal, double mos_fwhm,
+                                     double prec, double ntrunc, int mode) {
+  NCAMD_TRY
+  ensure_device();
+  double ac[3] = {lcaxis_crystal[0], lcaxis_crystal[1], lcaxis_crystal[2]};
+  const double an = std::sqrt(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
+  if (!(an > 0.0) || !std::isfinite(an)) raise("BadInput", "LCBragg: invalid layer axis");
+  for (double& x : ac) x /= an;
+  double al[3];
+  for (int r = 0; r < 3; ++r) al[r] = cry2lab[3 * r] * ac[0] + cry2lab[3 * r + 1] * ac[1] + cry2lab[3 * r + 2] * ac[2];
+  std::unique_ptr<Proc> sc(create_scb(v0n, nfam, dsp, fsq, mem_off, mem_hkl, reci_lattice, cry2lab, mos_fwhm, prec,
+                                      ntrunc));
+  if (mode != 0) {
+    ncamd_process_t h = ncamd_create_lcbragg_ref(wrap(sc.get()), al, mode);
+    if (!h.internal) return h;  // error state already set
+    --sc->refcount;  // the LCBragg process holds the crystal
+    sc.release();
+    return h;
+  }
+  // plane sets: crystal-frame demi-normals by the angle of the one in the
+  // layer axis' hemisphere, merged on the 2^-40 grid of (d, angle), sorted
+  // by decreasing (d, angle); values from the keys
+  constexpr double kDiscr = 1099511627776.0;
+  std::map<std::pair<uint64_t, uint64_t>, double, std::greater<std::pair<uint64_t, uint64_t>>> sets;
+  for (int64_t f = 0; f < nfam; ++f) {
+    if (!(dsp[f] > 0.0 && dsp[f] < 1e7) || !(fsq[f] > 0.0)) raise("BadInput", "LCBragg: invalid family");
+    for (int64_t j = mem_off[f]; j < mem_off[f + 1]; ++j) {
+      const double h = mem_hkl[3 * j], k = mem_hkl[3 * j + 1], l = mem_hkl[3 * j + 2];
+      double nv[3];
+      for (int r = 0; r < 3; ++r)
+        nv[r] = reci_lattice[3 * r] * h + reci_lattice[3 * r + 1] * k + reci_lattice[3 * r + 2] * l;
+      const double nn = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+      const double ca = std::fabs((ac[0] * nv[0] + ac[1] * nv[1] + ac[2] * nv[2]) / nn);
+      const double alpha = ca > 0.9999999 ? 0.0 : std::acos(ca);
+      const auto key = std::make_pair((uint64_t)(dsp[f] * kDiscr + 0.5), (uint64_t)(alpha * kDiscr + 0.5));
+      sets[key] += fsq[f];
+    }
+  }
+  const double trunc = sc->scb_gos[1];
+  std::vector<LC0Set> hs;
+  for (const auto& e : sets) {
+    const double d = e.first.first * (1.0 / kDiscr);
+    const double alpha = std::min(e.first.second * (1.0 / kDiscr), kPiHalf);
+    LC0Set ps{};
+    ps.twodsp = 2.0 * d;
+    ps.inv2d = 0.5 / d;
+    ps.c1 = alpha != 0.0 ? std::cos(alpha) : 1.0;
+    ps.s1 = alpha != 0.0 ? std::sin(alpha) : 0.0;
+    ps.c1minus = alpha > trunc ? std::cos(alpha - trunc) : 1.0;
+    ps.c1plus = std::cos(alpha + trunc);
+    ps.fsq = e.second;
+    hs.push_back(ps);
+  }
+  std::unique_ptr<Proc> p(new Proc());
+  p->kind = kKindLC;
+  p->lc_mode = 0;
+  p->lc = LCView{al[0], al[1], al[2]};
+  const double aln = std::sqrt(al[0] * al[0] + al[1] * al[1] + al[2] * al[2]);
+  p->lc0_sets.alloc(sizeof(LC0Set) * std::max<size_t>(hs.size(), 1));
+  if (!hs.empty()) NCAMD_HIP(hipMemcpy(p->lc0_sets.p, hs.data(), sizeof(LC0Set) * hs.size(), hipMemcpyHostToDevice));
+  p->lc0.sets = p->lc0_sets.as<LC0Set>();
+  p->lc0.nsets = (int)hs.size();
+  p->lc0.ax = al[0] / aln; p->lc0.ay = al[1] / aln; p->lc0.az = al[2] / aln;
+  p->lc0.cta = sc->scb_gos[3];
+  p->lc0.sta = sc->scb_gos[4];
+  p->lc0.acc = dclamp(sc->scb_gos[2], 1e-7, 1e-2);
+  p->lc0.xsfact = 1.0 / v0n;
+  p->comps.push_back(sc.release());
   p->scales.push_back(1.0);
   init_errbuf(p.get());
   return wrap(p.release());

@@ -341,6 +341,20 @@ void launch_lc_pick(const double* xs, const double* rE, const double* rD, int64_
                     uint32_t stream, double* sE, double* sD, int* sel, hipStream_t);
 void launch_lc_unrotate(const double* scD, const int* sel, const double2* cs, const double* ekin, const double* dirs,
                         int64_t n, int N, const LCView&, double* oE, double* oD, hipStream_t);
+// the layered-crystal model proper (lcmode=0): plane sets of equal (d,
+// angle to the layer axis), sorted by decreasing (d, angle)
+struct LC0Set {
+  double twodsp, inv2d, c1, s1, c1minus, c1plus, fsq, pad;
+};
+struct LC0View {
+  const LC0Set* sets;
+  int nsets;
+  double ax, ay, az;        // layer axis, lab frame
+  double cta, sta, acc;     // cos/sin of the mosaic truncation angle, Romberg accuracy
+  double xsfact;            // 1 / (V natoms)
+};
+void launch_lc0_xs(const LC0View&, const SCBView&, const double* ekin, const double* dirs, int64_t n, double* out,
+                   hipStream_t);
 
 // compacts the neutrons listed in q (count *qc <= nmax): energies, directions
 // and original indices

@@ -9,6 +9,7 @@
 // rotate the scattered direction back (sampling).
 #include "nc_kernels.hh"
 #include "nc_dev.hh"
+#include "nc_scb.hh"
 
 namespace ncamd {
 namespace {
@@ -134,7 +135,141 @@ __global__ void __launch_bounds__(256) k_lc_unrotate(const double* __restrict__ 
 
 unsigned lc_blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+// ---- lcmode=0: ncrystal_core/src/extd_utils/NCLCUtils.cc
+
+// values on the 2^-40 grid of the model's cache keys (:45-52)
+NCD double lc_discr(double x) {
+  return (double)(uint64_t)(x * 1099511627776.0 + 0.5) * (1.0 / 1099511627776.0);
+}
+
+// GaussMos::calcRawCrossSectionValue after InteractionPars::set
+// (NCGaussMos.hh:204-214, NCGaussMos.cc:252-280): Q times the
+// Gaussian-on-sphere circle integral, Q = wl^3 / (sin cos theta_Bragg) x
+// fsq / 2
+NCD double lc0_raw(const SCBView& v, double wl, double inv2d, double fsq, double cosg) {
+  const double w = gm_round(wl), i2 = gm_round(inv2d);
+  const double sinpt = w * i2;
+  const double cospt = sqrt(1 - sinpt * sinpt);
+  const double tmp2 = cospt * sinpt;
+  const double Qprime = tmp2 > 0 ? (w * w * w) / tmp2 : ((sinpt > 0.5 && fsq != 0.0) ? -2.0 : 0.0);
+  if (!(Qprime > 0.0)) return Qprime != 0.0 ? __builtin_huge_val() : 0.0;
+  const double Q = Qprime * (fsq * 0.5);
+  const double cc = dclamp(cosg, -1.0, 1.0);
+  const double sc = sqrt(1.0 - cc * cc);
+  double val;
+  if (!gos_circle_fast_ok(v, cc, sc, sinpt, cospt, val)) val = gos_circle_slow(v, cc, sc, sinpt, cospt);
+  return Q * val;
+}
+
+// Romberg integration of NCRomberg.cc:58-140 (17 points, halvings up to
+// level 15, accept |R_k - R_(k-1)| <= acc |R_k|) of the cross section over
+// crystallite rotations phi, cos(gamma) = k1 cos(phi) + k0
+NCD double lc0_integrate(const SCBView& v, double wl, const LC0Set& ps, double k1, double k0, double acc, double a,
+                         double b) {
+  auto f = [&](double phi) { return lc0_raw(v, wl, ps.inv2d, ps.fsq, k1 * cos(phi) + k0); };
+  double h = b - a;
+  double fv[17];
+  const double delta = h * 0.0625;
+  for (int i = 0; i < 17; ++i) fv[i] = f(a + delta * (double)i);
+  double prev[16], row[16];
+  h *= 0.5;
+  prev[0] = (fv[0] + fv[16]) * h;
+  double r1[2], r2[3], r3[4], r4[5];
+  r1[0] = h * fv[8] + 0.5 * prev[0];
+  r1[1] = (4.0 * r1[0] - prev[0]) / 3.0;
+  h *= 0.5;
+  r2[0] = h * (fv[4] + fv[12]) + 0.5 * r1[0];
+  r2[1] = (4.0 * r2[0] - r1[0]) / 3.0;
+  r2[2] = (16.0 * r2[1] - r1[1]) / 15.0;
+  h *= 0.5;
+  r3[0] = h * ((fv[2] + fv[6]) + (fv[10] + fv[14])) + 0.5 * r2[0];
+  r3[1] = (4.0 * r3[0] - r2[0]) / 3.0;
+  r3[2] = (16.0 * r3[1] - r2[1]) / 15.0;
+  r3[3] = (64.0 * r3[2] - r2[2]) / 63.0;
+  h *= 0.5;
+  r4[0] = h * (((fv[1] + fv[3]) + (fv[5] + fv[7])) + ((fv[9] + fv[11]) + (fv[13] + fv[15]))) + 0.5 * r3[0];
+  r4[1] = (4.0 * r4[0] - r3[0]) / 3.0;
+  r4[2] = (16.0 * r4[1] - r3[1]) / 15.0;
+  r4[3] = (64.0 * r4[2] - r3[2]) / 63.0;
+  r4[4] = (256.0 * r4[3] - r3[3]) / 255.0;
+  if (fabs(r4[4] - r3[3]) <= acc * fabs(r4[4])) return r4[4];
+  for (int j = 0; j < 5; ++j) prev[j] = r4[j];
+  int nj = 8;
+  for (int i = 5; i < 16; ++i) {
+    const double hh = h;
+    h *= 0.5;
+    nj *= 2;
+    double c = 0.0;
+    for (int j = 0; j < nj; ++j) c += f(a + h + hh * (double)j);
+    row[0] = h * c + 0.5 * prev[0];
+    double nk = 1.0;
+    for (int j = 0; j < i; ++j) {
+      nk *= 4.0;
+      row[j + 1] = (nk * row[j] - prev[j]) / (nk - 1.0);
+    }
+    if (fabs(row[i] - prev[i - 1]) <= acc * fabs(row[i])) return row[i];
+    for (int j = 0; j <= i; ++j) prev[j] = row[j];
+  }
+  return prev[15];
+}
+
+// one lane per neutron: discretised wavelength and |cos| to the layer axis
+// (:332-344), plane sets with 2d >= wl (:365-372), their phi ranges
+// (LCROIFinder::findROIs, :184-303) and range averages (:396-439)
+__global__ void __launch_bounds__(256) k_lc0_xs(LC0View L, SCBView v, const double* __restrict__ ekin,
+                                                const double* __restrict__ dirs, int64_t n, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double E = ekin[i];
+  double tot = 0.0;
+  if (E > 0.0) {
+    double ux = dirs[3 * i], uy = dirs[3 * i + 1], uz = dirs[3 * i + 2];
+    const double inv = 1.0 / sqrt(ux * ux + uy * uy + uz * uz);
+    ux *= inv; uy *= inv; uz *= inv;
+    const double wl = lc_discr(sqrt(kWl2Ekin / E));
+    const double c3 = dmin(lc_discr(fabs(L.ax * ux + L.ay * uy + L.az * uz)), 1.0);
+    const double s3 = sqrt(fabs(1.0 - c3 * c3));
+    for (int k = 0; k < L.nsets; ++k) {
+      const LC0Set ps = L.sets[k];
+      if (wl > ps.twodsp) break;
+      const double c2 = ps.inv2d * wl;
+      const double c23 = c3 * c2;
+      const double s2 = sqrt(1.0 - c2 * c2);
+      const double s23 = s3 * s2;
+      const double lo = c23 - s23, hi = c23 + s23;
+      if (dmax(lo, ps.c1plus) > dmin(hi, ps.c1minus)) continue;
+      const bool anti = dmax(lo, -ps.c1minus) <= dmin(hi, -ps.c1plus);
+      for (int side = 0; side < (anti ? 2 : 1); ++side) {
+        const double sg = side ? -1.0 : 1.0;
+        if (ps.s1 == 0.0) {
+          tot += lc0_raw(v, wl, ps.inv2d, ps.fsq, sg * c3);
+        } else if (fabs(s3) < 1e-10) {
+          tot += lc0_raw(v, wl, ps.inv2d, ps.fsq, (ps.c1 * c3) * sg);
+        } else {
+          const double c2low = c2 * L.cta - s2 * L.sta;
+          const double c2high = s2 < L.sta ? 1.0 : c2 * L.cta + s2 * L.sta;
+          const double b = 1.0 / (ps.s1 * s3);
+          const double mab = -(ps.c1 * c3) * b;
+          const double p1 = dclamp(mab + sg * (c2low * b), -1.0, 1.0);
+          const double p2 = dclamp(mab + sg * (c2high * b), -1.0, 1.0);
+          if (fabs(p1 - p2) > 1e-10) {
+            const double r0 = acos(dmax(p1, p2)), r1 = acos(dmin(p1, p2));
+            tot += lc0_integrate(v, wl, ps, ps.s1 * s3 * sg, ps.c1 * sg * c3, L.acc, r0, r1) * (1.0 / kPi);
+          }
+        }
+      }
+    }
+  }
+  out[i] = L.xsfact * tot;
+}
+
 }  // namespace
+
+void launch_lc0_xs(const LC0View& L, const SCBView& v, const double* ekin, const double* dirs, int64_t n, double* out,
+                   hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lc0_xs, dim3(lc_blocks(n)), dim3(256), 0, st, L, v, ekin, dirs, n, out);
+}
 
 void launch_lc_expand(const double* ekin, const double* dirs, int64_t n, int N, const LCView& L, const double2* cs,
                       uint64_t seed, uint32_t stream, double* oE, double* oD, double2* ocs, hipStream_t st) {

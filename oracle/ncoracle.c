@@ -3592,6 +3592,24 @@ orc_proc* orc_create_scbragg(const double* L, double v0n, int nfam, const double
   gos_set(&sc->gos, sigma, trunc, prec);
   return p;
 }
+/* GaussMos::calcRawCrossSectionValue (NCGaussMos.hh:204-214) after
+   InteractionPars::set (NCGaussMos.cc:252-280) for one (wavelength, 1/(2d),
+   fsq factor) and n cosines of the angle between neutron and normal, with the
+   mosaic distribution of an SCBragg object */
+void orc_scb_rawxs(const orc_proc* p, double wl, double inv2d, double xsfact, int n, const double* cosg, double* out) {
+  const scb_t* sc = &p->u.scb->s;
+  const double w = gm_round(wl), i2 = gm_round(inv2d);
+  const double sinpt = w * i2;
+  const double cospt = sqrt(1 - sinpt * sinpt);
+  const double tmp2 = cospt * sinpt;
+  const double Qprime = tmp2 > 0 ? (w * w * w) / tmp2 : ((sinpt > 0.5 && xsfact) ? -2.0 : 0.0);
+  const double Q = Qprime > 0 ? Qprime * (xsfact * 0.5) : -1.0;
+  for (int i = 0; i < n; ++i) {
+    const double cc = dclamp(cosg[i], -1.0, 1.0);
+    if (Q > 0.) out[i] = Q * gos_circle(&sc->gos, cc, sqrt(1.0 - cc * cc), sinpt, cospt);
+    else out[i] = Qprime ? INFINITY : 0.0;
+  }
+}
 int orc_scb_info(const orc_proc* p, int* nfam, int64_t* nnormals, double* threshold, double* gd) {
   const scb_t* sc = &p->u.scb->s;
   *nfam = sc->nfam; *nnormals = sc->off[sc->nfam]; *threshold = sc->threshold;

@@ -82,6 +82,8 @@ def lib():
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_create_scbragg.restype = _vp
         L.orc_scb_info.argtypes = [_vp, _ip, _i64p, _dp, _dp]
+        L.orc_scb_rawxs.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp]
+        L.orc_scb_rawxs.restype = None
         L.orc_scb_tables.argtypes = [_vp, _dp, _dp, _i64p, _dp, _dp, _dp]
         L.orc_xs_dir_many.argtypes = [_vp, _dp, _dp, ctypes.c_int64, _dp, ctypes.c_int]
         L.orc_create_powderbragg.argtypes = [ctypes.c_double, _dp, _dp, ctypes.c_int]
@@ -675,6 +677,152 @@ class LCBraggRef:
         back = _lc_rotate(self.axis, oD, c.ravel()[sel], -s.ravel()[sel])
         out = np.where(live[:, None], back, dirs)
         return np.asarray(ekin).copy(), out, nerr
+
+
+_LC_DISCR = 1099511627776.0  # 2^40
+
+
+def _lc_discr(x):
+    return int(x * _LC_DISCR + 0.5)
+
+
+def _romberg(f_many, a, b, acc):
+    """NCRomberg.cc:58-140 (the library's romberg restatement): 17 points,
+    then halvings up to level 15, accepting |R_k - R_{k-1}| <= acc |R_k|;
+    f_many(x array) -> values"""
+    h = b - a
+    fv = f_many(a + (h / 16.0) * np.arange(17))
+    h *= 0.5
+    R = [[(fv[0] + fv[16]) * h]]
+    R.append([h * fv[8] + 0.5 * R[0][0]])
+    h *= 0.5
+    R.append([h * (fv[4] + fv[12]) + 0.5 * R[1][0]])
+    h *= 0.5
+    R.append([h * ((fv[2] + fv[6]) + (fv[10] + fv[14])) + 0.5 * R[2][0]])
+    h *= 0.5
+    R.append([h * (((fv[1] + fv[3]) + (fv[5] + fv[7])) + ((fv[9] + fv[11]) + (fv[13] + fv[15]))) + 0.5 * R[3][0]])
+    for i in range(1, 5):
+        nk = 1.0
+        for j in range(i):
+            nk *= 4.0
+            R[i].append((nk * R[i][j] - R[i - 1][j]) / (nk - 1.0))
+    if abs(R[4][4] - R[3][3]) <= acc * abs(R[4][4]):
+        return R[4][4]
+    prev, nj = R[4], 8
+    for i in range(5, 16):
+        hh = h
+        h *= 0.5
+        nj *= 2
+        c = float(np.sum(f_many(a + h + hh * np.arange(nj))))
+        row = [h * c + 0.5 * prev[0]]
+        nk = 1.0
+        for j in range(i):
+            nk *= 4.0
+            row.append((nk * row[j] - prev[j]) / (nk - 1.0))
+        if abs(row[i] - prev[i - 1]) <= acc * abs(row[i]):
+            return row[i]
+        prev = row
+    return prev[-1]
+
+
+class LCHelperXS:
+    """Cross sections of NCrystal's layered-crystal model (lcmode=0,
+    ncrystal_core/src/extd_utils/NCLCUtils.cc): plane sets of equal
+    (d, angle to the layer axis) merged on a 2^-40 grid and sorted by
+    decreasing (d, angle) (:58-131); per neutron the discretised wavelength
+    and |cos| of its angle to the axis (:332-344), the phi ranges of
+    crystallite rotations that reach the mosaic band of each plane set's
+    normal and anti-normal (LCROIFinder, :159-303), the Romberg average of the
+    Gaussian-mosaic cross section over each range (:396-439, 455-516), on-axis
+    special cases, times 1/(V natoms).  scb: the checker's SCBragg of the
+    crystal (its mosaic); normals: crystal-frame demi-normals with their
+    family's |F|^2 and d."""
+
+    def __init__(self, scb, normals, fsq, dsp, lcaxis_crys, lcaxis_lab, v0n):
+        self.scb = scb
+        nf, nn, thr, gd = ctypes.c_int(), ctypes.c_int64(), ctypes.c_double(), np.empty(15)
+        lib().orc_scb_info(scb.ptr, ctypes.byref(nf), ctypes.byref(nn), ctypes.byref(thr), _d(gd))
+        self.trunc, self.prec, self.cta, self.sta = gd[1], gd[2], gd[3], gd[4]
+        ax = np.asarray(lcaxis_crys, dtype=np.float64)
+        ax = ax / np.linalg.norm(ax)
+        lab = np.asarray(lcaxis_lab, dtype=np.float64)
+        self.axis = lab / np.linalg.norm(lab)
+        sets = {}
+        for n, f, d in zip(np.asarray(normals).reshape(-1, 3), fsq, dsp):
+            ca = abs(float(ax @ n))
+            alpha = 0.0 if ca > 0.9999999 else math.acos(ca)
+            key = (_lc_discr(d), _lc_discr(alpha))
+            sets[key] = sets.get(key, 0.0) + f
+        self.sets = []
+        for (kd, ka), f in sorted(sets.items(), reverse=True):
+            d, alpha = kd / _LC_DISCR, min(ka / _LC_DISCR, math.pi / 2)
+            self.sets.append(dict(twodsp=2 * d, inv2d=0.5 / d, c1=math.cos(alpha) if alpha else 1.0,
+                                  s1=math.sin(alpha) if alpha else 0.0,
+                                  c1minus=math.cos(alpha - self.trunc) if alpha > self.trunc else 1.0,
+                                  c1plus=math.cos(alpha + self.trunc), fsq=f))
+        self.xsfact = 1.0 / v0n
+
+    def _raw(self, wl, ps, cosg):
+        cosg = np.ascontiguousarray(np.atleast_1d(cosg), dtype=np.float64)
+        out = np.empty(cosg.size)
+        lib().orc_scb_rawxs(self.scb.ptr, wl, ps["inv2d"], ps["fsq"], cosg.size, _d(cosg), _d(out))
+        return out
+
+    def _rois(self, wl, c3, s3, ps):
+        c2 = ps["inv2d"] * wl
+        c23 = c3 * c2
+        s2 = math.sqrt(1.0 - c2 * c2)
+        s23 = s3 * s2
+        lo, hi = c23 - s23, c23 + s23
+        if max(lo, ps["c1plus"]) > min(hi, ps["c1minus"]):
+            return []
+        anti = max(lo, -ps["c1minus"]) <= min(hi, -ps["c1plus"])
+        signs = [1.0] + ([-1.0] if anti else [])
+        if not ps["s1"]:
+            return [("axis", sg, 0.0, 0.0) for sg in signs]
+        if abs(s3) < 1e-10:
+            return [("neutron", sg, 0.0, 0.0) for sg in signs]
+        c2ta, s2ta = c2 * self.cta, s2 * self.sta
+        c2low = c2ta - s2ta
+        c2high = 1.0 if s2 < self.sta else c2ta + s2ta
+        a = ps["c1"] * c3
+        b = 1.0 / (ps["s1"] * s3)
+        mab, lowb, highb = -a * b, c2low * b, c2high * b
+        out = []
+        for sg in signs:
+            p1 = max(-1.0, min(1.0, mab + sg * lowb))
+            p2 = max(-1.0, min(1.0, mab + sg * highb))
+            if abs(p1 - p2) > 1e-10:
+                out.append(("range", sg, math.acos(max(p1, p2)), math.acos(min(p1, p2))))
+        return out
+
+    def xs_one(self, wl_raw, u):
+        wl = _lc_discr(wl_raw) / _LC_DISCR
+        c3 = min(_lc_discr(abs(float(self.axis @ u))) / _LC_DISCR, 1.0)
+        s3 = math.sqrt(abs(1.0 - c3 * c3))
+        acc = min(max(self.prec, 1e-7), 1e-2)
+        tot = 0.0
+        for ps in self.sets:
+            if wl > ps["twodsp"]:
+                break
+            for kind, sg, r0, r1 in self._rois(wl, c3, s3, ps):
+                if kind == "axis":
+                    x = self._raw(wl, ps, sg * c3)[0]
+                elif kind == "neutron":
+                    x = self._raw(wl, ps, sg * ps["c1"] * c3)[0]
+                else:
+                    k1, k0 = ps["s1"] * s3 * sg, ps["c1"] * sg * c3
+                    x = _romberg(lambda phi: self._raw(wl, ps, k1 * np.cos(phi) + k0), r0, r1, acc) / math.pi
+                tot += x
+        return self.xsfact * tot
+
+    def xs_dir(self, ekin, dirs):
+        ekin = np.atleast_1d(np.asarray(ekin, dtype=np.float64))
+        dirs = np.asarray(dirs, dtype=np.float64).reshape(-1, 3)
+        out = np.empty(len(ekin))
+        for i, (e, d) in enumerate(zip(ekin, dirs)):
+            out[i] = self.xs_one(math.sqrt(0.081804209605330899 / e), d / np.linalg.norm(d)) if e > 0 else 0.0
+        return out
 
 
 def debug_sort_fams(dsp, fsq, mult, hkl):

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 from scipy.stats import chi2
 
-from test_lcbragg_checker import FIX, WL2EKIN, beam, bragg_angles, checker_crystal, graphite_inputs
+from test_lcbragg_checker import FIX, HDR, WL2EKIN, beam, bragg_angles, checker_crystal, graphite_inputs, lchelper
 
 pytestmark = pytest.mark.gpu
 
@@ -141,3 +141,42 @@ def test_argument_errors(core, crystals):
     with pytest.raises(C.NCBadInput):
         lc.crossSectionIsotropic(np.array([0.01]))
     assert lc.isOriented() if hasattr(lc, "isOriented") else True
+
+
+def test_lcmode0_cross_sections(core, oracle, crystals):
+    """the layered-crystal model (lcmode=0) against the reference's values
+    (6 printed digits) and the checker (1e-10: the same plane sets, ranges and
+    Romberg steps, device cosines)"""
+    def msd(e, d, T, m):
+        return core.vdos_analyse((e, d), T, m)["msd"]
+    cell, sp, v0n = graphite_inputs(msd)
+    hkl, mult, dsp, fsq, moff, mh = core.hkl_spacegroup_members(FIX["spacegroup"], cell, sp, FIX["dcutoff"])
+    L = core.reci_lattice(cell)
+    lc = core.create_lcbragg(v0n, dsp, fsq, moff, mh, L, np.eye(3), FIX["lcaxis"], math.radians(FIX["mosaicity_deg"]))
+    ref = FIX["app_lc"]["0"]
+    wl, xs = np.array(ref["wavelength"]), np.array(ref["xs"])
+    _, d = beam(len(wl), 1.0)
+    got = lc.crossSection(WL2EKIN / wl ** 2, d)
+    zero = xs == 0
+    assert np.all(got[zero] == 0)
+    assert np.max(np.abs(got[~zero] - xs[~zero]) / xs[~zero]) < 6e-6
+    o, ofam = checker_crystal(oracle)
+    olc = lchelper(oracle, o, ofam)
+    rng = np.random.default_rng(8)
+    n = 400
+    e = WL2EKIN / rng.uniform(0.5, 7.0, n) ** 2
+    dd = rng.normal(size=(n, 3))
+    dd /= np.linalg.norm(dd, axis=1)[:, None]
+    dd[:20] = (0.0, 0.0, 1.0)          # neutrons along the layer axis
+    g = lc.crossSection(e, dd)
+    w = olc.xs_dir(e, dd)
+    assert np.max(np.where(w > 0, np.abs(g - w) / np.where(w > 0, w, 1), np.abs(g))) < 1e-10
+    assert np.mean(g > 0) > 0.3
+    # the mode != 0 constructor with the same inputs equals the reference-model path
+    r20 = core.create_lcbragg(v0n, dsp, fsq, moff, mh, L, np.eye(3), FIX["lcaxis"],
+                              math.radians(FIX["mosaicity_deg"]), mode=20)
+    assert np.max(np.abs(r20.crossSection(WL2EKIN / wl ** 2, d) - np.array(FIX["app_lc"]["20"]["xs"]))
+                  / np.maximum(np.array(FIX["app_lc"]["20"]["xs"]), 1e-300)) < 6e-6
+    from ncrystal_amd import _capi as C
+    with pytest.raises(C.NCBadInput):
+        lc.sampleScatter(WL2EKIN / wl[:3] ** 2, d[:3])

@@ -13,7 +13,10 @@ tests/scripts/ncmat2hkl.log):
   rotations): a fine grid of rotations reproduces its cross sections, and
   the mean of many single random rotations (lcmode=-1) converges to them;
 * sampling: elastic, and every scattering angle is 2 asin(lambda / 2d) of a
-  family that can reflect (at 4.74539 Aa only (002): 90 degrees).
+  family that can reflect (at 4.74539 Aa only (002): 90 degrees);
+* the layered-crystal model itself (lcmode=0, oracle.pyoracle.LCHelperXS:
+  plane sets, rotation ranges, Romberg averages): its cross sections at the
+  43 wavelengths to the printed 6 digits.
 """
 import json
 import math
@@ -132,3 +135,25 @@ def test_sampling_angles(oracle, crystal):
         assert np.max(err) < 1e-5, wl
         if wl == 4.74539:
             assert np.max(np.abs(ang - 90.0)) < 1e-3
+
+
+def lchelper(oracle, crystal, fam):
+    hkl, mult, dsp, fsq, moff, mh = fam
+    L = oracle.reci_lattice((HDR["lattice_a"], HDR["lattice_a"], HDR["lattice_c"], 90, 90, 120)).reshape(3, 3)
+    nrm = mh @ L.T
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    of = np.repeat(np.arange(len(dsp)), np.diff(moff))
+    return oracle.LCHelperXS(crystal, nrm, fsq[of], dsp[of], FIX["lcaxis"], FIX["lcaxis"],
+                             HDR["Vc"] * len(FIX["positions"]))
+
+
+def test_lcmode0_cross_sections(oracle):
+    crystal, fam = checker_crystal(oracle)
+    lc = lchelper(oracle, crystal, fam)
+    ref = FIX["app_lc"]["0"]
+    wl, xs = np.array(ref["wavelength"]), np.array(ref["xs"])
+    _, d = beam(len(wl), 1.0)
+    got = lc.xs_dir(WL2EKIN / wl ** 2, d)
+    zero = xs == 0
+    assert np.all(got[zero] == 0)
+    assert np.max(np.abs(got[~zero] - xs[~zero]) / xs[~zero]) < 6e-6
