@@ -321,7 +321,8 @@ ncamd_process_t ncamd_create_lcbragg_ref(ncamd_process_t scbragg, const double* 
  * the crystal frame.  mode 0: the layered-crystal model (LCHelper,
  * ncrystal_core/src/extd_utils/NCLCUtils.cc:58-516: plane sets of equal
  * (d, angle to the axis), rotation ranges reaching the mosaic band, Romberg
- * averages over them) for cross sections; its scattering raises BadInput.
+ * averages over them; scattering: a range by its cross section, the rotation
+ * by rejection under the reference's 8-bin envelope, NCLCUtils.cc:518-690).
  * mode != 0: as ncamd_create_lcbragg_ref with the axis taken to the lab
  * frame by cry2lab. */
 ncamd_process_t ncamd_create_lcbragg(double v0_times_natoms, int64_t nfam, const double* dspacing,

@@ -2626,9 +2626,11 @@ void lc_xs(Proc* p, const double* E, const double* D, int64_t n, double* out, hi
 // unchanged.  The random-rotation model draws new rotations here instead of
 // reusing those of a preceding cross-section call (the same distribution).
 void lc_sample(Proc* p, const double* E, const double* D, int64_t n, double* oE, double* oD, hipStream_t st) {
-  if (p->lc_mode == 0)
-    raise("BadInput", "LCBragg lcmode=0: scattering is not available on the GPU; use a reference model (lcmode != 0)");
   Proc* sc = p->comps[0];
+  if (p->lc_mode == 0) {
+    launch_lc0_sample(p->lc0, sc->scb, E, D, n, p->seed, next_stream(p), oE, oD, st);
+    return;
+  }
   const int N = p->lc_n;
   const int64_t chunk = std::max<int64_t>(1, kLcLanes / N);
   for (int64_t i0 = 0; i0 < n; i0 += chunk) {

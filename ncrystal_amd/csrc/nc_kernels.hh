@@ -355,6 +355,8 @@ struct LC0View {
 };
 void launch_lc0_xs(const LC0View&, const SCBView&, const double* ekin, const double* dirs, int64_t n, double* out,
                    hipStream_t);
+void launch_lc0_sample(const LC0View&, const SCBView&, const double* ekin, const double* dirs, int64_t n, uint64_t seed,
+                       uint32_t stream, double* oE, double* oD, hipStream_t);
 
 // compacts the neutrons listed in q (count *qc <= nmax): energies, directions
 // and original indices

@@ -213,9 +213,72 @@ NCD double lc0_integrate(const SCBView& v, double wl, const LC0Set& ps, double k
   return prev[15];
 }
 
-// one lane per neutron: discretised wavelength and |cos| to the layer axis
-// (:332-344), plane sets with 2d >= wl (:365-372), their phi ranges
-// (LCROIFinder::findROIs, :184-303) and range averages (:396-439)
+// a rotation range (ROI) of a plane set: kind 0 normal along the axis, 1
+// neutron along the axis, 2 the range [r0, r1] of phi
+struct LC0Roi {
+  int set, kind;
+  double sg, r0, r1;
+};
+
+// the ROIs of one neutron (LCROIFinder::findROIs, NCLCUtils.cc:184-303) and
+// their cross sections (:396-439) in order; returns the sum, and with
+// target >= 0 stops at the first ROI whose running sum exceeds it (pick)
+NCD double lc0_walk(const LC0View& L, const SCBView& v, double wl, double c3, double s3, double target, LC0Roi& pick) {
+  double tot = 0.0;
+  for (int k = 0; k < L.nsets; ++k) {
+    const LC0Set ps = L.sets[k];
+    if (wl > ps.twodsp) break;
+    const double c2 = ps.inv2d * wl;
+    const double c23 = c3 * c2;
+    const double s2 = sqrt(1.0 - c2 * c2);
+    const double s23 = s3 * s2;
+    const double lo = c23 - s23, hi = c23 + s23;
+    if (dmax(lo, ps.c1plus) > dmin(hi, ps.c1minus)) continue;
+    const bool anti = dmax(lo, -ps.c1minus) <= dmin(hi, -ps.c1plus);
+    for (int side = 0; side < (anti ? 2 : 1); ++side) {
+      const double sg = side ? -1.0 : 1.0;
+      LC0Roi roi{k, 2, sg, 0.0, 0.0};
+      double x;
+      if (ps.s1 == 0.0) {
+        roi.kind = 0;
+        x = lc0_raw(v, wl, ps.inv2d, ps.fsq, sg * c3);
+      } else if (fabs(s3) < 1e-10) {
+        roi.kind = 1;
+        x = lc0_raw(v, wl, ps.inv2d, ps.fsq, (ps.c1 * c3) * sg);
+      } else {
+        const double c2low = c2 * L.cta - s2 * L.sta;
+        const double c2high = s2 < L.sta ? 1.0 : c2 * L.cta + s2 * L.sta;
+        const double b = 1.0 / (ps.s1 * s3);
+        const double mab = -(ps.c1 * c3) * b;
+        const double p1 = dclamp(mab + sg * (c2low * b), -1.0, 1.0);
+        const double p2 = dclamp(mab + sg * (c2high * b), -1.0, 1.0);
+        if (!(fabs(p1 - p2) > 1e-10)) continue;
+        roi.r0 = acos(dmax(p1, p2));
+        roi.r1 = acos(dmin(p1, p2));
+        x = lc0_integrate(v, wl, ps, ps.s1 * s3 * sg, ps.c1 * sg * c3, L.acc, roi.r0, roi.r1) * (1.0 / kPi);
+      }
+      tot += x;
+      if (target >= 0.0 && tot > target) {
+        pick = roi;
+        return tot;
+      }
+      pick = roi;  // the last ROI, should rounding leave the target unreached
+    }
+  }
+  return tot;
+}
+
+// discretised wavelength and |cos| to the layer axis (:332-344)
+NCD void lc0_neutron(const LC0View& L, double E, double& ux, double& uy, double& uz, double& wl, double& c3raw,
+                     double& c3, double& s3) {
+  const double inv = 1.0 / sqrt(ux * ux + uy * uy + uz * uz);
+  ux *= inv; uy *= inv; uz *= inv;
+  wl = lc_discr(sqrt(kWl2Ekin / E));
+  c3raw = L.ax * ux + L.ay * uy + L.az * uz;
+  c3 = dmin(lc_discr(fabs(c3raw)), 1.0);
+  s3 = sqrt(fabs(1.0 - c3 * c3));
+}
+
 __global__ void __launch_bounds__(256) k_lc0_xs(LC0View L, SCBView v, const double* __restrict__ ekin,
                                                 const double* __restrict__ dirs, int64_t n, double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -223,44 +286,89 @@ __global__ void __launch_bounds__(256) k_lc0_xs(LC0View L, SCBView v, const doub
   const double E = ekin[i];
   double tot = 0.0;
   if (E > 0.0) {
-    double ux = dirs[3 * i], uy = dirs[3 * i + 1], uz = dirs[3 * i + 2];
-    const double inv = 1.0 / sqrt(ux * ux + uy * uy + uz * uz);
-    ux *= inv; uy *= inv; uz *= inv;
-    const double wl = lc_discr(sqrt(kWl2Ekin / E));
-    const double c3 = dmin(lc_discr(fabs(L.ax * ux + L.ay * uy + L.az * uz)), 1.0);
-    const double s3 = sqrt(fabs(1.0 - c3 * c3));
-    for (int k = 0; k < L.nsets; ++k) {
-      const LC0Set ps = L.sets[k];
-      if (wl > ps.twodsp) break;
-      const double c2 = ps.inv2d * wl;
-      const double c23 = c3 * c2;
-      const double s2 = sqrt(1.0 - c2 * c2);
-      const double s23 = s3 * s2;
-      const double lo = c23 - s23, hi = c23 + s23;
-      if (dmax(lo, ps.c1plus) > dmin(hi, ps.c1minus)) continue;
-      const bool anti = dmax(lo, -ps.c1minus) <= dmin(hi, -ps.c1plus);
-      for (int side = 0; side < (anti ? 2 : 1); ++side) {
-        const double sg = side ? -1.0 : 1.0;
-        if (ps.s1 == 0.0) {
-          tot += lc0_raw(v, wl, ps.inv2d, ps.fsq, sg * c3);
-        } else if (fabs(s3) < 1e-10) {
-          tot += lc0_raw(v, wl, ps.inv2d, ps.fsq, (ps.c1 * c3) * sg);
-        } else {
-          const double c2low = c2 * L.cta - s2 * L.sta;
-          const double c2high = s2 < L.sta ? 1.0 : c2 * L.cta + s2 * L.sta;
-          const double b = 1.0 / (ps.s1 * s3);
-          const double mab = -(ps.c1 * c3) * b;
-          const double p1 = dclamp(mab + sg * (c2low * b), -1.0, 1.0);
-          const double p2 = dclamp(mab + sg * (c2high * b), -1.0, 1.0);
-          if (fabs(p1 - p2) > 1e-10) {
-            const double r0 = acos(dmax(p1, p2)), r1 = acos(dmin(p1, p2));
-            tot += lc0_integrate(v, wl, ps, ps.s1 * s3 * sg, ps.c1 * sg * c3, L.acc, r0, r1) * (1.0 / kPi);
-          }
-        }
-      }
-    }
+    double ux = dirs[3 * i], uy = dirs[3 * i + 1], uz = dirs[3 * i + 2], wl, c3raw, c3, s3;
+    lc0_neutron(L, E, ux, uy, uz, wl, c3raw, c3, s3);
+    LC0Roi roi;
+    tot = lc0_walk(L, v, wl, c3, s3, -1.0, roi);
   }
   out[i] = L.xsfact * tot;
+}
+
+// scattering (LCHelper::genScatter, NCLCUtils.cc:518-690): a ROI by its
+// cross section; in it a rotation phi by rejection under the reference's
+// envelope (8 bins over the range, each the larger end value times 1.7 plus
+// 2% of the largest, at most 1000 tries), uniform for a neutron along the
+// axis; a random sign of sin(phi); the Gaussian-mosaic scattering of the
+// reversed neutron (-s3, 0, -c3) off that normal in the frame of the axis
+// (LCStdFrame::genScat, :755-772), taken to the lab frame by the neutron and
+// the axis (rotateToFrame) and reversed
+__global__ void __launch_bounds__(256) k_lc0_sample(LC0View L, SCBView v, const double* __restrict__ ekin,
+                                                    const double* __restrict__ dirs, int64_t n, uint64_t seed,
+                                                    uint32_t stream, double* __restrict__ oE, double* __restrict__ oD) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double E = ekin[i];
+  double ox = dirs[3 * i], oy = dirs[3 * i + 1], oz = dirs[3 * i + 2];
+  oE[i] = E;
+  if (E > 0.0) {
+    double ux = ox, uy = oy, uz = oz, wl, c3raw, c3, s3;
+    lc0_neutron(L, E, ux, uy, uz, wl, c3raw, c3, s3);
+    LC0Roi roi;
+    const double tot = lc0_walk(L, v, wl, c3, s3, -1.0, roi);
+    if (tot > 0.0) {
+      Rng r;
+      r.init(seed, stream, (uint64_t)i);
+      lc0_walk(L, v, wl, c3, s3, r.gen() * tot, roi);
+      const LC0Set ps = L.sets[roi.set];
+      const V3 indir_std{-s3, 0.0, -c3};
+      V3 normal{0.0, 0.0, roi.sg};
+      if (roi.kind != 0) {
+        double phi, cosphi;
+        if (roi.kind == 1) {
+          phi = r.gen() * kPi;
+          cosphi = cos(phi);
+        } else {
+          const double k1 = ps.s1 * s3 * roi.sg, k0 = ps.c1 * roi.sg * c3;
+          const double len = roi.r1 - roi.r0;
+          double tmp[9];
+          double mx = 0.0;
+          for (int j = 0; j < 9; ++j) {
+            tmp[j] = lc0_raw(v, wl, ps.inv2d, ps.fsq, k1 * cos(roi.r0 + j * (len * 0.125)) + k0);
+            mx = dmax(mx, tmp[j]);
+          }
+          float data[8];
+          float sum = 0.0f;
+          for (int j = 0; j < 8; ++j) data[j] = (sum += (float)(dmax(tmp[j], tmp[j + 1]) * 1.7 + 0.02 * mx));
+          phi = roi.r0;
+          cosphi = cos(phi);
+          for (int tries = 0; tries < 1000; ++tries) {
+            const float t = data[7] * (float)r.gen();
+            int b = 0;
+            while (b < 7 && data[b] < t) ++b;
+            const double ov = b ? (double)data[b] - (double)data[b - 1] : (double)data[0];
+            phi = roi.r0 + ((b + r.gen()) * 0.125) * len;
+            cosphi = cos(phi);
+            const double xsphi = lc0_raw(v, wl, ps.inv2d, ps.fsq, k1 * cosphi + k0);
+            if (xsphi > ov * r.gen()) break;
+          }
+        }
+        const double sgn = r.gen() < 0.5 ? 1.0 : -1.0;
+        const double sinphi = sgn * sqrt(dmax(0.0, 1.0 - cosphi * cosphi));
+        const double ns1 = roi.sg * ps.s1;
+        normal = V3{ns1 * cosphi, ns1 * sinphi, roi.sg * ps.c1};
+      }
+      const V3 w = gm_genscat(v, r, normal, gm_round(ps.inv2d), wl, indir_std);
+      const double isg = c3raw >= 0.0 ? 1.0 : -1.0;
+      const V3 a{ux, uy, uz}, b{L.ax * isg, L.ay * isg, L.az * isg};
+      const V3 o = rotate_to_frame(s3, c3, a, b, w, r);
+      ox = -o.x;
+      oy = -o.y;
+      oz = -o.z;
+    }
+  }
+  oD[3 * i] = ox;
+  oD[3 * i + 1] = oy;
+  oD[3 * i + 2] = oz;
 }
 
 }  // namespace
@@ -269,6 +377,11 @@ void launch_lc0_xs(const LC0View& L, const SCBView& v, const double* ekin, const
                    hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_lc0_xs, dim3(lc_blocks(n)), dim3(256), 0, st, L, v, ekin, dirs, n, out);
+}
+void launch_lc0_sample(const LC0View& L, const SCBView& v, const double* ekin, const double* dirs, int64_t n,
+                       uint64_t seed, uint32_t stream, double* oE, double* oD, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lc0_sample, dim3(lc_blocks(n)), dim3(256), 0, st, L, v, ekin, dirs, n, seed, stream, oE, oD);
 }
 
 void launch_lc_expand(const double* ekin, const double* dirs, int64_t n, int N, const LCView& L, const double2* cs,

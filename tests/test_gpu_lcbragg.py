@@ -6,7 +6,9 @@ reference's own pyrolytic-graphite cross sections (lcmode=20 to the printed
 the random-rotation model converging to the reference's exact model,
 sampling elastic with every scattering angle a Bragg angle of a family and
 the choice among families distributed as the checker's, neutrons without
-cross section unchanged, clones, argument errors."""
+cross section unchanged, clones, argument errors; the layered-crystal model
+itself (lcmode=0): cross sections against the reference's values and the
+checker, scattering against the reference model over 401 rotations."""
 import math
 
 import numpy as np
@@ -177,6 +179,55 @@ def test_lcmode0_cross_sections(core, oracle, crystals):
                               math.radians(FIX["mosaicity_deg"]), mode=20)
     assert np.max(np.abs(r20.crossSection(WL2EKIN / wl ** 2, d) - np.array(FIX["app_lc"]["20"]["xs"]))
                   / np.maximum(np.array(FIX["app_lc"]["20"]["xs"]), 1e-300)) < 6e-6
-    from ncrystal_amd import _capi as C
-    with pytest.raises(C.NCBadInput):
-        lc.sampleScatter(WL2EKIN / wl[:3] ** 2, d[:3])
+
+
+def _graphite_inputs(core):
+    def msd(e, d, T, m):
+        return core.vdos_analyse((e, d), T, m)["msd"]
+    cell, sp, v0n = graphite_inputs(msd)
+    hkl, mult, dsp, fsq, moff, mh = core.hkl_spacegroup_members(FIX["spacegroup"], cell, sp, FIX["dcutoff"])
+    return v0n, dsp, fsq, moff, mh, core.reci_lattice(cell)
+
+
+def test_lcmode0_sampling(core, crystals):
+    """scattering of the layered-crystal model (lcmode=0) against the
+    reference model over 401 rotations (itself compared with the checker
+    above): elastic, Bragg angles, the same choice among families and the same
+    distribution of the outgoing angle to the layer axis (chi-square), and
+    unchanged neutrons without cross section"""
+    v0n, dsp, fsq, moff, mh, L = _graphite_inputs(core)
+    mos = math.radians(FIX["mosaicity_deg"])
+    lc = core.create_lcbragg(v0n, dsp, fsq, moff, mh, L, np.eye(3), FIX["lcaxis"], mos)
+    ref = core.create_lcbragg(v0n, dsp, fsq, moff, mh, L, np.eye(3), FIX["lcaxis"], mos, mode=401)
+    for wl, direction in ((4.74539, FIX["direction"]), (2.37269, FIX["direction"]), (1.5818, FIX["direction"]),
+                          (2.9, (0.3, 0.5, 0.812404))):
+        n = 30000
+        e = np.full(n, WL2EKIN / wl ** 2)
+        d = np.tile(np.asarray(direction) / np.linalg.norm(direction), (n, 1))
+        lc.setRandomSeed(21)
+        ge, gd = lc.sampleScatter(e, d)
+        assert np.array_equal(ge, e)
+        assert np.allclose(np.linalg.norm(gd, axis=1), 1.0, atol=1e-12)
+        re, rd = ref.sampleScatter(e, d)
+        u = d[0]
+        for a, b in ((gd, rd),):
+            ang_a = np.degrees(np.arccos(np.clip(a @ u, -1, 1)))
+            ang_b = np.degrees(np.arccos(np.clip(b @ u, -1, 1)))
+            allowed = bragg_angles(dsp, wl)
+            ka = np.argmin(np.abs(ang_a[:, None] - allowed[None, :]), axis=1)
+            kb = np.argmin(np.abs(ang_b[:, None] - allowed[None, :]), axis=1)
+            moved = ang_a > 1e-6
+            assert np.max(np.abs(ang_a[moved] - allowed[ka[moved]])) < 1e-5
+            ca = np.bincount(ka, minlength=len(allowed))
+            cb = np.bincount(kb, minlength=len(allowed))
+            m = (ca + cb) > 0
+            p1 = chi2.sf(float(((ca[m] - cb[m]) ** 2 / (ca[m] + cb[m])).sum()), max(1, int(m.sum()) - 1))
+            ha, _ = np.histogram(a[:, 2], bins=20, range=(-1, 1))
+            hb, _ = np.histogram(b[:, 2], bins=20, range=(-1, 1))
+            mm = (ha + hb) > 0
+            p2 = chi2.sf(float(((ha[mm] - hb[mm]) ** 2 / (ha[mm] + hb[mm])).sum()), max(1, int(mm.sum()) - 1))
+            print(f"lcmode 0 vs 401 at {wl} Aa: families p {p1:.3g}, cos to axis p {p2:.3g}")
+            assert p1 > 1e-3 and p2 > 1e-3
+    e, d = beam(50, 7.9)
+    ge, gd = lc.sampleScatter(e, d)
+    assert np.array_equal(ge, e) and np.array_equal(gd, d)
