@@ -31,6 +31,8 @@ with its own roofline figure and CPU baseline:
   F3: MiniMC: 1e7 neutrons of a 4 Aa pencil beam into a sphere of Al_sg225
     one scattering length across (forced collisions, absorption, roulette,
     theta tally); one step = one whole engine call.
+  F4: LCBragg, the layered-crystal model (lcmode=0) of pyrolytic graphite:
+    cross sections and scattering of 2^20 neutrons (0.5-7 Aa, isotropic).
 
 `--workload` runs one of them alone.  Roofline figures come from HIP events
 on the launch stream; the CPU baseline is the CPU checker timed on a bounded
@@ -69,7 +71,7 @@ def parse():
                     help="also run one device-to-device copy of the output size (counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["all", "sab", "c1", "scbragg", "powderbragg", "batch", "vdos", "minimc"],
+    ap.add_argument("--workload", choices=["all", "sab", "c1", "scbragg", "powderbragg", "batch", "vdos", "minimc", "lcbragg"],
                     default="all")
     ap.add_argument("--sub-cpu-seconds", type=float, default=8.0,
                     help="CPU-baseline sample length of each sub-result")
@@ -885,6 +887,98 @@ def run_minimc(args):
     }
 
 
+def run_lcbragg(args):
+    """LCBragg, the layered-crystal model (lcmode=0) of pyrolytic graphite
+    (the reference's app_lc material: sg 194, dcutoff 0.5 Aa, 4 deg mosaicity,
+    layer axis (0,0,1)): cross sections and scattering of 2^20 neutrons with
+    wavelengths uniform in 0.5-7 Aa and isotropic directions, device-resident"""
+    import json as _json
+    import numpy as np
+    from ncrystal_amd import _capi as C, core
+    g = _json.loads((ROOT / "tests" / "golden" / "lc_graphite.json").read_text())
+    v = _json.loads((ROOT / "tests" / "golden" / "vdos_reference.json").read_text())["vdos"][
+        "C_sg194_pyrolytic_graphite/C"]
+    h = g["laz"]["header"]
+    a, c = h["lattice_a"], h["lattice_c"]
+    cell = (a, a, c, 90, 90, 120)
+    msd = core.vdos_analyse((np.array(v["egrid"]), np.array(v["density"])), h["temperature"], h["average_mass"])["msd"]
+    csl = math.sqrt(h["sigma_coh"] / (4 * math.pi))
+    pos = np.array(g["positions"])
+    _, _, dsp, fsq, moff, mh = core.hkl_spacegroup_members(g["spacegroup"], cell, [(csl, msd, pos)], g["dcutoff"])
+    L = core.reci_lattice(cell)
+    lc = core.create_lcbragg(h["Vc"] * len(pos), dsp, fsq, moff, mh, L, np.eye(3), g["lcaxis"],
+                             math.radians(g["mosaicity_deg"]))
+    n = 1 << 20
+    rng = np.random.default_rng(12)
+    wl = rng.uniform(0.5, 7.0, n)
+    e = 0.081804209605330899 / wl ** 2
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    stream = C.Stream()
+    de, dd = C.DeviceArray.from_numpy(e), C.DeviceArray.from_numpy(dirs.ravel())
+    xs, oe, od = C.DeviceArray(n), C.DeviceArray(n), C.DeviceArray(3 * n)
+    lc.setRandomSeed(5)
+    ev = [C.Event() for _ in range(3)]
+    steps = max(1, min(args.steps, 5))
+    for _ in range(max(1, min(args.warmup, 2))):
+        lc.dev_crossSection(de.ptr, dd.ptr, n, xs.ptr, stream=stream.ptr)
+        lc.dev_sampleScatter(de.ptr, dd.ptr, n, oe.ptr, od.ptr, stream=stream.ptr)
+    stream.sync()
+    xs_ms = samp_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev[0].record(stream)
+        lc.dev_crossSection(de.ptr, dd.ptr, n, xs.ptr, stream=stream.ptr)
+        ev[1].record(stream)
+        lc.dev_sampleScatter(de.ptr, dd.ptr, n, oe.ptr, od.ptr, stream=stream.ptr)
+        ev[2].record(stream)
+        stream.sync()
+        xs_ms += ev[0].elapsed_ms(ev[1])
+        samp_ms += ev[1].elapsed_ms(ev[2])
+    wall = time.perf_counter() - t0
+    xs_ms /= steps
+    samp_ms /= steps
+    x = xs.numpy()
+    assert np.all(np.isfinite(x)) and np.mean(x > 0) > 0.3, "LCBragg cross sections out of range"
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pyoracle as po
+        po.build()
+        lo, hi, dv = po.vdos_regularise(np.array(v["egrid"]), np.array(v["density"]))
+        cms = po.vdos_analyse(lo, hi, dv, h["temperature"], h["average_mass"])["msd"]
+        fam = po.hkl_spacegroup_members(g["spacegroup"], cell, [(csl, cms, pos)], g["dcutoff"])
+        ol = po.reci_lattice(cell)
+        osc = po.scbragg(a, a, c, h["Vc"] * len(pos), fam[2], fam[3], fam[4], fam[5], np.eye(3),
+                         math.radians(g["mosaicity_deg"]), reci=ol)
+        nrm = fam[5] @ ol.reshape(3, 3).T
+        nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+        of = np.repeat(np.arange(len(fam[2])), np.diff(fam[4]))
+        ochk = po.LCHelperXS(osc, nrm, fam[3][of], fam[2][of], g["lcaxis"], g["lcaxis"], h["Vc"] * len(pos))
+        m = 0
+        tc = time.perf_counter()
+        while time.perf_counter() - tc < min(args.sub_cpu_seconds, 10.0) and m < n:
+            ochk.xs_dir(e[m:m + 200], dirs[m:m + 200])
+            m += 200
+        dt = time.perf_counter() - tc
+        cpu = dict(value=m / dt, unit="neutrons/s (cross sections)", cores=1, kind="port",
+                   sample=f"{m} neutrons of the same batch through the checker's lcmode=0 cross section "
+                          f"(one thread), {dt:.1f} s")
+    return {
+        "metric": "LCBragg lcmode=0 neutrons/s on 1 MI355X (pyrolytic graphite, cross section + scattering)",
+        "value": n / ((xs_ms + samp_ms) * 1e-3), "unit": "neutrons/s", "steps": steps, "warmup": 1,
+        "ms_per_step": wall / steps * 1e3, "dtype": "f64",
+        "config": {"workload": "pyrolytic graphite layered crystal (sg 194, dcutoff 0.5 Aa, 4 deg mosaicity, "
+                               "layer axis 001), 2^20 neutrons, 0.5-7 Aa, isotropic directions",
+                   "neutrons": n, "xs_ms": xs_ms, "sample_ms": samp_ms,
+                   "xs_neutrons_per_s": n / (xs_ms * 1e-3)},
+        "roofline": {"bound": "fp64-valu", "achieved": None, "peak": None, "unit": None, "frac": None,
+                     "traffic": None, "kernel": "k_lc0_xs (per-neutron Romberg averages over rotation ranges)",
+                     "note": "40 B of HBM traffic per neutron; the FP64 Romberg arithmetic of each range bounds it"},
+        "cpu_baseline": cpu,
+    }
+
+
 def copy_calibration(nbytes=800_000_000):
     """one plain device-to-device copy of nbytes (read nbytes, write nbytes):
     the reference launch that scales the FETCH_SIZE / WRITE_SIZE counters of
@@ -903,7 +997,7 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only (a one-GPU node); run with --gpus 1"}))
         sys.exit(2)
     runners = {"sab": run_sab, "c1": run_c1, "powderbragg": run_powderbragg, "batch": run_batch,
-               "scbragg": run_scbragg, "vdos": run_vdos, "minimc": run_minimc}
+               "scbragg": run_scbragg, "vdos": run_vdos, "minimc": run_minimc, "lcbragg": run_lcbragg}
     if args.copy_calib:
         copy_calibration()
     if args.workload != "all":
@@ -916,7 +1010,7 @@ def main():
     sub_args.cpu_seconds = args.sub_cpu_seconds
     sub = {}
     for key, name in (("C1", "c1"), ("C2", "powderbragg"), ("C4", "batch"), ("C5", "scbragg"), ("F2", "vdos"),
-                      ("F3", "minimc")):
+                      ("F3", "minimc"), ("F4", "lcbragg")):
         t0 = time.perf_counter()
         r = runners[name](sub_args)
         for k in ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):

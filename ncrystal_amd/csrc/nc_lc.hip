@@ -161,46 +161,19 @@ NCD double lc0_raw(const SCBView& v, double wl, double inv2d, double fsq, double
   return Q * val;
 }
 
-// Romberg integration of NCRomberg.cc:58-140 (17 points, halvings up to
-// level 15, accept |R_k - R_(k-1)| <= acc |R_k|) of the cross section over
-// crystallite rotations phi, cos(gamma) = k1 cos(phi) + k0
-NCD double lc0_integrate(const SCBView& v, double wl, const LC0Set& ps, double k1, double k0, double acc, double a,
-                         double b) {
-  auto f = [&](double phi) { return lc0_raw(v, wl, ps.inv2d, ps.fsq, k1 * cos(phi) + k0); };
-  double h = b - a;
-  double fv[17];
-  const double delta = h * 0.0625;
-  for (int i = 0; i < 17; ++i) fv[i] = f(a + delta * (double)i);
+// levels 5 and up of the Romberg integration below (rare at the model's
+// accuracy)
+NCD double lc0_romberg_tail(const SCBView& v, double wl, const LC0Set& ps, double k1, double k0, double acc, double a,
+                            double h, double r40, double r41, double r42, double r43, double r44) {
   double prev[16], row[16];
-  h *= 0.5;
-  prev[0] = (fv[0] + fv[16]) * h;
-  double r1[2], r2[3], r3[4], r4[5];
-  r1[0] = h * fv[8] + 0.5 * prev[0];
-  r1[1] = (4.0 * r1[0] - prev[0]) / 3.0;
-  h *= 0.5;
-  r2[0] = h * (fv[4] + fv[12]) + 0.5 * r1[0];
-  r2[1] = (4.0 * r2[0] - r1[0]) / 3.0;
-  r2[2] = (16.0 * r2[1] - r1[1]) / 15.0;
-  h *= 0.5;
-  r3[0] = h * ((fv[2] + fv[6]) + (fv[10] + fv[14])) + 0.5 * r2[0];
-  r3[1] = (4.0 * r3[0] - r2[0]) / 3.0;
-  r3[2] = (16.0 * r3[1] - r2[1]) / 15.0;
-  r3[3] = (64.0 * r3[2] - r2[2]) / 63.0;
-  h *= 0.5;
-  r4[0] = h * (((fv[1] + fv[3]) + (fv[5] + fv[7])) + ((fv[9] + fv[11]) + (fv[13] + fv[15]))) + 0.5 * r3[0];
-  r4[1] = (4.0 * r4[0] - r3[0]) / 3.0;
-  r4[2] = (16.0 * r4[1] - r3[1]) / 15.0;
-  r4[3] = (64.0 * r4[2] - r3[2]) / 63.0;
-  r4[4] = (256.0 * r4[3] - r3[3]) / 255.0;
-  if (fabs(r4[4] - r3[3]) <= acc * fabs(r4[4])) return r4[4];
-  for (int j = 0; j < 5; ++j) prev[j] = r4[j];
+  prev[0] = r40; prev[1] = r41; prev[2] = r42; prev[3] = r43; prev[4] = r44;
   int nj = 8;
   for (int i = 5; i < 16; ++i) {
     const double hh = h;
     h *= 0.5;
     nj *= 2;
     double c = 0.0;
-    for (int j = 0; j < nj; ++j) c += f(a + h + hh * (double)j);
+    for (int j = 0; j < nj; ++j) c += lc0_raw(v, wl, ps.inv2d, ps.fsq, k1 * cos(a + h + hh * (double)j) + k0);
     row[0] = h * c + 0.5 * prev[0];
     double nk = 1.0;
     for (int j = 0; j < i; ++j) {
@@ -211,6 +184,47 @@ NCD double lc0_integrate(const SCBView& v, double wl, const LC0Set& ps, double k
     for (int j = 0; j <= i; ++j) prev[j] = row[j];
   }
   return prev[15];
+}
+
+// Romberg integration of NCRomberg.cc:58-140 (17 points, halvings up to
+// level 15, accept |R_k - R_(k-1)| <= acc |R_k|) of the cross section over
+// crystallite rotations phi, cos(gamma) = k1 cos(phi) + k0
+NCD double lc0_integrate(const SCBView& v, double wl, const LC0Set& ps, double k1, double k0, double acc, double a,
+                         double b) {
+  double h = b - a;
+  const double delta = h * 0.0625;
+  // the 17 points summed into the groups of the first five levels (ends,
+  // midpoint, quarters, eighths, sixteenths), in one loop
+  double s_end = 0.0, s_mid = 0.0, s_q = 0.0, s_e = 0.0, s_s = 0.0;
+  for (int i = 0; i < 17; ++i) {
+    const double f = lc0_raw(v, wl, ps.inv2d, ps.fsq, k1 * cos(a + delta * (double)i) + k0);
+    if (i == 0 || i == 16) s_end += f;
+    else if (i == 8) s_mid += f;
+    else if (i == 4 || i == 12) s_q += f;
+    else if ((i & 1) == 0) s_e += f;
+    else s_s += f;
+  }
+  h *= 0.5;
+  const double r00 = s_end * h;
+  const double r10 = h * s_mid + 0.5 * r00;
+  const double r11 = (4.0 * r10 - r00) / 3.0;
+  h *= 0.5;
+  const double r20 = h * s_q + 0.5 * r10;
+  const double r21 = (4.0 * r20 - r10) / 3.0;
+  const double r22 = (16.0 * r21 - r11) / 15.0;
+  h *= 0.5;
+  const double r30 = h * s_e + 0.5 * r20;
+  const double r31 = (4.0 * r30 - r20) / 3.0;
+  const double r32 = (16.0 * r31 - r21) / 15.0;
+  const double r33 = (64.0 * r32 - r22) / 63.0;
+  h *= 0.5;
+  const double r40 = h * s_s + 0.5 * r30;
+  const double r41 = (4.0 * r40 - r30) / 3.0;
+  const double r42 = (16.0 * r41 - r31) / 15.0;
+  const double r43 = (64.0 * r42 - r32) / 63.0;
+  const double r44 = (256.0 * r43 - r33) / 255.0;
+  if (fabs(r44 - r33) <= acc * fabs(r44)) return r44;
+  return lc0_romberg_tail(v, wl, ps, k1, k0, acc, a, h, r40, r41, r42, r43, r44);
 }
 
 // a rotation range (ROI) of a plane set: kind 0 normal along the axis, 1
