@@ -311,7 +311,7 @@ ncamd_process_t ncamd_create_scbragg(double v0_times_natoms, int64_t nfam, const
  * (N the smallest prime >= mode), scattering over `mode` random rotations
  * picked by cross section; mode < 0: LCBraggRndmRot (NCLCRefModels.cc:82-171),
  * -mode random rotations for both.  Mode 0 (the LCHelper model) raises
- * BadInput.  The process holds a reference to the SCBragg process; oriented
+ * BadInput here; ncamd_create_lcbragg builds it.  The process holds a reference to the SCBragg process; oriented
  * calls only; it cannot be a component of a composition. */
 ncamd_process_t ncamd_create_lcbragg_ref(ncamd_process_t scbragg, const double* lcaxis_lab, int mode);
 /* Layered crystal, replaces LCBragg(const Info&, const SCOrientation&,
