@@ -85,16 +85,24 @@ int64_t ncamd_hkl_orthogonal_members(double a, double b, double c, int nspecies,
                                      int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp,
                                      double* out_fsq, int64_t* nrequired, int64_t maxmem,
                                      int64_t* out_mem_off, int* out_mem_hkl, int64_t* nmem_required);
+/* the same for any cell (angles in degrees): the reference's enumeration
+ * without symmetry (NCFillHKL.cc:230-450) with the reciprocal lattice of
+ * getReciprocalLatticeRot (NCLatticeUtils.cc:95-149, its general branch for
+ * triclinic and rhombohedral cells); arguments as above after the angles */
+int64_t ncamd_hkl_cell_members(double a, double b, double c, double alpha, double beta, double gamma, int nspecies,
+                               const int* natoms, const double* positions, const double* csl, const double* msd,
+                               double dcutoff, double fsquarecut, double merge_tolerance, int64_t maxout,
+                               int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq, int64_t* nrequired,
+                               int64_t maxmem, int64_t* out_mem_off, int* out_mem_hkl, int64_t* nmem_required);
 
 /* hkl families of a crystal with a space group (1..230): reflections grouped
  * by symmetry equivalence.  Replaces calculateHKLPlanesWithSymEqRefl
  * (ncrystal_core/src/extd_utils/NCFillHKL.cc:545-709, equivalent reflections
  * by ncrystal_core/src/phys_utils/NCEqRefl.cc:25-150) followed by the
- * InfoBuilder sort (ncrystal_core/src/infobld/NCInfoBuilder.cc:452).  Cells
- * (angles in degrees): all 90, alpha = beta = 90 with gamma = 120, or
- * alpha = gamma = 90 -- the cells whose reciprocal lattice the reference
- * builds in closed form (ncrystal_core/src/utils/NCLatticeUtils.cc:105-144);
- * others raise BadInput.  The loop over (h,k,l) and the symmetry bookkeeping
+ * InfoBuilder sort (ncrystal_core/src/infobld/NCInfoBuilder.cc:452).  Any
+ * cell (angles in degrees): closed-form reciprocal lattices for 90/90/90,
+ * 90/90/120 and 90/beta/90 cells, the general branch otherwise
+ * (ncrystal_core/src/utils/NCLatticeUtils.cc:95-149).  The loop over (h,k,l) and the symmetry bookkeeping
  * run on the host, |F|^2 of every representative on the GPU.  Each family's
  * members are its equivalent reflections, one per +-pair, sorted so that the
  * first is the representative hkl, and mult = 2 x their number; the member

@@ -107,6 +107,11 @@ def lib():
                                                                                  ctypes.c_double, ctypes.c_int64,
                                                                                  _ip, _ip, _dp, _dp, _i64p,
                                                                                  ctypes.c_int64, _i64p, _ip, _i64p]),
+        "ncamd_hkl_cell_members": (ctypes.c_int64, [ctypes.c_double] * 6 + [ctypes.c_int, _ip, _dp, _dp, _dp,
+                                                                                 ctypes.c_double, ctypes.c_double,
+                                                                                 ctypes.c_double, ctypes.c_int64,
+                                                                                 _ip, _ip, _dp, _dp, _i64p,
+                                                                                 ctypes.c_int64, _i64p, _ip, _i64p]),
         "ncamd_hkl_spacegroup_members": (ctypes.c_int64, [ctypes.c_int] + [ctypes.c_double] * 6 +
                                          [ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_int64, _ip, _ip, _dp, _dp, _i64p, ctypes.c_int64, _i64p, _ip,

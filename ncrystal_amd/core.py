@@ -167,7 +167,13 @@ def hkl_orthogonal(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tolerance=1
         return hkl[:3 * n].reshape(n, 3).copy(), mult[:n].copy(), dsp[:n].copy(), fsq[:n].copy()
 
 
-def hkl_orthogonal_members(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tolerance=1e-6):
+def hkl_cell_members(cell, species, dcutoff, fsquarecut=1e-5, merge_tolerance=1e-6):
+    """hkl_orthogonal_members for any cell = (a, b, c, alpha, beta, gamma)
+    (enumeration without symmetry over the reference's reciprocal lattice)."""
+    return hkl_orthogonal_members(*cell[:3], species, dcutoff, fsquarecut, merge_tolerance, _angles=tuple(cell[3:]))
+
+
+def hkl_orthogonal_members(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tolerance=1e-6, _angles=None):
     """As hkl_orthogonal, plus the explicit hkl members of every family:
     returns (hkl, mult, dspacing, fsquared, mem_off[n+1], mem_hkl[m,3])."""
     natoms = np.array([len(s[2]) for s in species], dtype=np.int32)
@@ -183,7 +189,9 @@ def hkl_orthogonal_members(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tol
         fsq = np.empty(maxout)
         moff = np.empty(maxout + 1, dtype=np.int64)
         mhkl = np.empty(3 * maxmem, dtype=np.int32)
-        n = C.call("ncamd_hkl_orthogonal_members", a, b, c, len(species), C.iptr(natoms), C.dptr(pos),
+        fn, cellargs = (("ncamd_hkl_orthogonal_members", (a, b, c)) if _angles is None else
+                        ("ncamd_hkl_cell_members", (a, b, c) + tuple(float(x) for x in _angles)))
+        n = C.call(fn, *cellargs, len(species), C.iptr(natoms), C.dptr(pos),
                    C.dptr(csl), C.dptr(msd), dcutoff, fsquarecut, merge_tolerance, maxout, C.iptr(hkl),
                    C.iptr(mult), C.dptr(dsp), C.dptr(fsq), ctypes.byref(req), maxmem, C.i64ptr(moff),
                    C.iptr(mhkl), ctypes.byref(mreq))

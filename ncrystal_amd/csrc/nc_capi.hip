@@ -1134,18 +1134,27 @@ int64_t hkl_output(std::vector<Fam>& fams, int64_t maxout, int* out_hkl, int* ou
   return nf;
 }
 
-int64_t hkl_orthogonal(double a, double b, double c, int nspecies, const int* natoms, const double* positions,
-                       const double* csl, const double* msd, double dcutoff, double fsquarecut, double merge_tol,
-                       int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq,
-                       int64_t* nrequired, int64_t maxmem = 0, int64_t* out_mem_off = nullptr,
-                       int* out_mem_hkl = nullptr, int64_t* nmem_required = nullptr) {
+bool reci_lattice(double a, double b, double c, double alpha_deg, double beta_deg, double gamma_deg, double* m);
+void hkl_bounds(double a, double b, double c, double alpha_deg, double beta_deg, double gamma_deg, double dcutoff,
+                int* mx);
+
+// enumeration without symmetry (NCFillHKL.cc:230-450) for any cell: the
+// reciprocal lattice of NCLatticeUtils.cc:95-149 times (h,k,l), loop bounds
+// from the lattice rows
+int64_t hkl_nosg(double a, double b, double c, double alpha, double beta, double gamma, int nspecies,
+                 const int* natoms, const double* positions, const double* csl, const double* msd, double dcutoff,
+                 double fsquarecut, double merge_tol, int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp,
+                 double* out_fsq, int64_t* nrequired, int64_t maxmem = 0, int64_t* out_mem_off = nullptr,
+                 int* out_mem_hkl = nullptr, int64_t* nmem_required = nullptr) {
   ensure_device();
   if (!(dcutoff > 0.0)) raise("BadInput", "dcutoff must be positive");
   if (!(a > 0 && b > 0 && c > 0)) raise("BadInput", "lattice parameters must be positive");
-  const double invd = 1.0 / dcutoff;
   HKLBuild P{};
-  P.rl0 = k2Pi / a; P.rl1 = k2Pi / b; P.rl2 = k2Pi / c;
-  P.max_h = floorhkl(a * invd); P.max_k = floorhkl(b * invd); P.max_l = floorhkl(c * invd);
+  if (!reci_lattice(a, b, c, alpha, beta, gamma, P.rm))
+    raise("BadInput", "invalid cell: angles must lie in (0, 180) degrees and admit a cell");
+  int mx[3];
+  hkl_bounds(a, b, c, alpha, beta, gamma, dcutoff, mx);
+  P.max_h = mx[0]; P.max_k = mx[1]; P.max_l = mx[2];
   HKLSpecies S;
   if (nrequired) *nrequired = 0;
   if (!hkl_species(P, S, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut)) return 0;
@@ -2708,7 +2717,7 @@ int64_t ncamd_hkl_orthogonal(double a, double b, double c, int nspecies, const i
                              double merge_tolerance, int64_t maxout, int* out_hkl, int* out_mult, double* out_dsp,
                              double* out_fsq, int64_t* nrequired) {
   NCAMD_TRY
-  return hkl_orthogonal(a, b, c, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut, merge_tolerance, maxout,
+  return hkl_nosg(a, b, c, 90.0, 90.0, 90.0, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut, merge_tolerance, maxout,
                         out_hkl, out_mult, out_dsp, out_fsq, nrequired);
   NCAMD_CATCH
   return -3;
@@ -2721,9 +2730,23 @@ int64_t ncamd_hkl_orthogonal_members(double a, double b, double c, int nspecies,
                                      int64_t* nmem_required) {
   NCAMD_TRY
   if (!out_mem_off || !out_mem_hkl) raise("BadInput", "member output arrays are required");
-  return hkl_orthogonal(a, b, c, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut, merge_tolerance, maxout,
-                        out_hkl, out_mult, out_dsp, out_fsq, nrequired, maxmem, out_mem_off, out_mem_hkl,
-                        nmem_required);
+  return hkl_nosg(a, b, c, 90.0, 90.0, 90.0, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut,
+                  merge_tolerance, maxout, out_hkl, out_mult, out_dsp, out_fsq, nrequired, maxmem, out_mem_off,
+                  out_mem_hkl, nmem_required);
+  NCAMD_CATCH
+  return -3;
+}
+
+int64_t ncamd_hkl_cell_members(double a, double b, double c, double alpha, double beta, double gamma, int nspecies,
+                               const int* natoms, const double* positions, const double* csl, const double* msd,
+                               double dcutoff, double fsquarecut, double merge_tolerance, int64_t maxout,
+                               int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq, int64_t* nrequired,
+                               int64_t maxmem, int64_t* out_mem_off, int* out_mem_hkl, int64_t* nmem_required) {
+  NCAMD_TRY
+  if (!out_mem_off || !out_mem_hkl) raise("BadInput", "member output arrays are required");
+  return hkl_nosg(a, b, c, alpha, beta, gamma, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut,
+                  merge_tolerance, maxout, out_hkl, out_mult, out_dsp, out_fsq, nrequired, maxmem, out_mem_off,
+                  out_mem_hkl, nmem_required);
   NCAMD_CATCH
   return -3;
 }

@@ -1,11 +1,11 @@
 // |F(hkl)|^2 and d-spacing of every (h,k,l) in the enumeration domain of a
-// cell with all angles 90 degrees (no space group), or of a list of
+// cell (no space group), or of a list of
 // symmetry representatives (space group), one lane per (h,k,l).
 //
 // Follows ncrystal_core/src/extd_utils/NCFillHKL.cc:299-383 in operation
 // order (so that d-spacings are bit-identical and |F|^2 agrees to the last
-// few ulp of exp): diagonal reciprocal lattice
-// (ncrystal_core/src/utils/NCLatticeUtils.cc:105-117), Debye-Waller factors
+// few ulp of exp): the reciprocal lattice
+// (ncrystal_core/src/utils/NCLatticeUtils.cc:95-149) times (h,k,l), Debye-Waller factors
 // exp(-0.5 ksq msd) with the per-species early-abort thresholds, phases with
 // the polynomial sincos of ncrystal_core/include/NCrystal/internal/utils/NCMath.hh:318-326
 // and ncrystal_core/src/utils/NCMath.cc:96-125, Neumaier sums per species and
@@ -51,9 +51,14 @@ __global__ void __launch_bounds__(256) k_hkl_eval(HKLBuild p, int64_t first, int
   unsigned char ok = 0;
   double fsq = 0.0, dsp = 0.0;
   const double dh = (double)h, dk = (double)k, dl = (double)l;
-  const double wx = p.rl0 * dh + 0.0 * dk + 0.0 * dl;
-  const double wy = 0.0 * dh + p.rl1 * dk + 0.0 * dl;
-  const double wz = 0.0 * dh + 0.0 * dk + p.rl2 * dl;
+  double wx, wy, wz;
+  {
+    // RotMatrix::operator* order without contraction (NCRotMatrix.hh)
+#pragma clang fp contract(off)
+    wx = p.rm[0] * dh + p.rm[1] * dk + p.rm[2] * dl;
+    wy = p.rm[3] * dh + p.rm[4] * dk + p.rm[5] * dl;
+    wz = p.rm[6] * dh + p.rm[7] * dk + p.rm[8] * dl;
+  }
   const double ksq = wx * wx + wy * wy + wz * wz;
   do {
     if (!in_interval(p.ksq_lo, p.ksq_hi, ksq)) break;

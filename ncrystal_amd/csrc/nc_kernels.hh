@@ -392,7 +392,7 @@ void launch_fg_sample(const FGView& g, const double* ekin, const double* dirs, i
 
 // hkl |F|^2 evaluation (nc_hkl.hip)
 struct HKLBuild {
-  double rl0, rl1, rl2;         // diagonal reciprocal lattice 2pi/a, 2pi/b, 2pi/c
+  double rm[9];                 // reciprocal lattice (row-major; diagonal 2pi/a, 2pi/b, 2pi/c for 90/90/90)
   int max_h, max_k, max_l;
   int nspecies;
   const double* pos;            // [3*natoms_total]

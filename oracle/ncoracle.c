@@ -457,14 +457,36 @@ int orc_hkl_orthogonal(double a, double b, double c, int nspecies, const int* na
   return orc_hkl_orthogonal_members(a, b, c, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut,
                                     merge_tol, maxout, out_hkl, out_mult, out_dsp, out_fsq, NULL, NULL, 0);
 }
+int orc_reci_lattice(double a, double b, double c, double alpha_deg, double beta_deg, double gamma_deg,
+                     double* m);
+static void lattice_rows(double a, double b, double c, double alpha_deg, double beta_deg, double gamma_deg,
+                         double* len);
+int orc_hkl_cell_members(double a, double b, double c, double alpha, double beta, double gamma, int nspecies,
+                         const int* natoms, const double* positions, const double* csl, const double* msd,
+                         double dcutoff, double fsquarecut, double merge_tol, int maxout, int* out_hkl,
+                         int* out_mult, double* out_dsp, double* out_fsq, int64_t* out_mem_off, int* out_mem_hkl,
+                         int64_t maxmem);
 int orc_hkl_orthogonal_members(double a, double b, double c, int nspecies, const int* natoms,
                                const double* positions, const double* csl, const double* msd,
                                double dcutoff, double fsquarecut, double merge_tol, int maxout,
                                int* out_hkl, int* out_mult, double* out_dsp, double* out_fsq,
                                int64_t* out_mem_off, int* out_mem_hkl, int64_t maxmem) {
-  const double rl0 = K_2PI / a, rl1 = K_2PI / b, rl2 = K_2PI / c;
+  return orc_hkl_cell_members(a, b, c, 90.0, 90.0, 90.0, nspecies, natoms, positions, csl, msd, dcutoff, fsquarecut,
+                              merge_tol, maxout, out_hkl, out_mult, out_dsp, out_fsq, out_mem_off, out_mem_hkl, maxmem);
+}
+/* any cell (NCFillHKL.cc:230-450 with getReciprocalLatticeRot): the wave
+   vector is the reciprocal lattice times (h,k,l), loop bounds from the
+   lattice rows */
+int orc_hkl_cell_members(double a, double b, double c, double alpha, double beta, double gamma, int nspecies,
+                         const int* natoms, const double* positions, const double* csl, const double* msd,
+                         double dcutoff, double fsquarecut, double merge_tol, int maxout, int* out_hkl,
+                         int* out_mult, double* out_dsp, double* out_fsq, int64_t* out_mem_off, int* out_mem_hkl,
+                         int64_t maxmem) {
+  double rm[9], len[3];
+  if (orc_reci_lattice(a, b, c, alpha, beta, gamma, rm)) return -3;
+  lattice_rows(a, b, c, alpha, beta, gamma, len);
   const double invd = 1.0 / dcutoff;
-  const int max_h = floorhkl(a * invd), max_k = floorhkl(b * invd), max_l = floorhkl(c * invd);
+  const int max_h = floorhkl(len[0] * invd), max_k = floorhkl(len[1] * invd), max_l = floorhkl(len[2] * invd);
   if (fsquarecut >= 0.0) fsquarecut = dmax(fsquarecut, 1.0e-300);
   /* species with bcoh==0 are ignored (NCFillHKL.cc:180-182) */
   int ns = 0;
@@ -497,9 +519,9 @@ int orc_hkl_orthogonal_members(double a, double b, double c, int nspecies, const
   for (int h = 0; h <= max_h && !err; ++h) {
     for (int k = (h ? -max_k : 0); k <= max_k && !err; ++k) {
       for (int l = -max_l; l <= max_l; ++l) {
-        const double wx = rl0 * (double)h + 0.0 * (double)k + 0.0 * (double)l;
-        const double wy = 0.0 * (double)h + rl1 * (double)k + 0.0 * (double)l;
-        const double wz = 0.0 * (double)h + 0.0 * (double)k + rl2 * (double)l;
+        const double wx = rm[0] * (double)h + rm[1] * (double)k + rm[2] * (double)l;
+        const double wy = rm[3] * (double)h + rm[4] * (double)k + rm[5] * (double)l;
+        const double wz = rm[6] * (double)h + rm[7] * (double)k + rm[8] * (double)l;
         const double ksq = wx * wx + wy * wy + wz * wz;
         if (!in_interval(ksq_lo, ksq_hi, ksq)) continue;
         const double kk2 = 0.5 * ksq;

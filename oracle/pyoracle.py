@@ -64,6 +64,10 @@ def lib():
                                                                           ctypes.c_double, ctypes.c_int, _ip, _ip,
                                                                           _dp, _dp, _i64p, _ip, ctypes.c_int64]
         L.orc_hkl_orthogonal_members.restype = ctypes.c_int
+        L.orc_hkl_cell_members.argtypes = [ctypes.c_double] * 6 + [ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_double,
+                                                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, _ip,
+                                                                    _ip, _dp, _dp, _i64p, _ip, ctypes.c_int64]
+        L.orc_hkl_cell_members.restype = ctypes.c_int
         L.orc_debug_sort_fams.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip, _ip]
         L.orc_hkl_spacegroup_members.argtypes = ([ctypes.c_int] + [ctypes.c_double] * 6 +
                                                  [ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_double,
@@ -544,8 +548,13 @@ def composition(procs, scales):
     return p
 
 
+def hkl_cell_members(cell, species, dcutoff, fsquarecut=1e-5, merge_tolerance=1e-6):
+    """enumeration without symmetry for any cell (a, b, c, alpha, beta, gamma)"""
+    return hkl_orthogonal_members(*cell[:3], species, dcutoff, fsquarecut, merge_tolerance, angles=tuple(cell[3:]))
+
+
 def hkl_orthogonal_members(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tolerance=1e-6, maxout=2000000,
-                           maxmem=8000000):
+                           maxmem=8000000, angles=None):
     natoms = np.array([len(s[2]) for s in species], dtype=np.int32)
     pos = np.ascontiguousarray(np.concatenate([np.asarray(s[2], dtype=np.float64).reshape(-1, 3)
                                                for s in species]).ravel())
@@ -557,9 +566,14 @@ def hkl_orthogonal_members(a, b, c, species, dcutoff, fsquarecut=1e-5, merge_tol
     fsq = np.empty(maxout)
     moff = np.empty(maxout + 1, dtype=np.int64)
     mhkl = np.empty(3 * maxmem, dtype=np.int32)
-    n = lib().orc_hkl_orthogonal_members(a, b, c, len(species), _i(natoms), _d(pos), _d(csl), _d(msd), dcutoff,
-                                         fsquarecut, merge_tolerance, maxout, _i(hkl), _i(mult), _d(dsp), _d(fsq),
-                                         moff.ctypes.data_as(_i64p), _i(mhkl), maxmem)
+    if angles is None:
+        n = lib().orc_hkl_orthogonal_members(a, b, c, len(species), _i(natoms), _d(pos), _d(csl), _d(msd), dcutoff,
+                                             fsquarecut, merge_tolerance, maxout, _i(hkl), _i(mult), _d(dsp),
+                                             _d(fsq), moff.ctypes.data_as(_i64p), _i(mhkl), maxmem)
+    else:
+        n = lib().orc_hkl_cell_members(a, b, c, *[float(x) for x in angles], len(species), _i(natoms), _d(pos),
+                                       _d(csl), _d(msd), dcutoff, fsquarecut, merge_tolerance, maxout, _i(hkl),
+                                       _i(mult), _d(dsp), _d(fsq), moff.ctypes.data_as(_i64p), _i(mhkl), maxmem)
     if n < 0:
         raise RuntimeError(f"oracle hkl enumeration failed ({n})")
     nm = int(moff[n])

@@ -92,3 +92,20 @@ def test_scbragg_hexagonal_zno(core, oracle):
     oe, od, nerr = o.sample_dir(e, dirs, seed=3, nthreads=8)
     assert nerr == 0 and np.array_equal(ge, e)
     assert_streams_match(np.max(np.abs(gd - od), axis=1) > 1e-9, "space-group mosaic crystal")
+
+
+@pytest.mark.parametrize("cell", [(3.1, 3.7, 4.3, 97.3, 104.1, 88.2), (3.9, 3.9, 3.9, 76.5, 76.5, 76.5),
+                                  (3.2, 3.2, 4.4, 90, 90, 120)])
+def test_cell_without_symmetry(core, oracle, cell):
+    """the enumeration without symmetry on general cells (and a hexagonal
+    one) through ncamd_hkl_cell_members against the checker: same families
+    and members, d and |F|^2 to rounding"""
+    rng = np.random.default_rng(3)
+    sp = [(0.3 + 0.2 * i, 0.002 * (i + 1), rng.uniform(0, 1, size=(2 + i, 3))) for i in range(3)]
+    g = core.hkl_cell_members(cell, sp, 0.6, fsquarecut=1e-12)
+    o = oracle.hkl_cell_members(cell, sp, 0.6, fsquarecut=1e-12)
+    assert len(g[2]) > 50
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+    assert np.array_equal(g[4], o[4]) and np.array_equal(g[5], o[5])
+    assert np.max(np.abs(g[2] - o[2]) / o[2]) < 1e-14
+    assert np.max(np.abs(g[3] - o[3]) / o[3]) < 1e-13

@@ -129,3 +129,28 @@ def test_orbits_partition_the_reflections(oracle, sg, cell):
     assert key == inside
     # orbit sizes divide the order of the Laue group (as +-pairs)
     assert all(ORDER[sg] % (k // 2) == 0 for k in mult)
+
+
+@pytest.mark.parametrize("cell", [(3.1, 3.7, 4.3, 97.3, 104.1, 88.2), (3.9, 3.9, 3.9, 76.5, 76.5, 76.5)])
+def test_cell_without_symmetry_matches_p1(oracle, cell):
+    """the enumeration without symmetry (NCFillHKL.cc:230-450) on a general
+    triclinic or rhombohedral cell against the space-group path with P1: with
+    atoms at general positions no two reflections other than Friedel pairs
+    share d and |F|^2, so both give the same families (d, |F|^2,
+    multiplicity 2) and members; and the general entry on a 90/90/90 cell
+    reproduces the orthogonal one exactly"""
+    rng = np.random.default_rng(3)
+    sp = [(0.3 + 0.2 * i, 0.002 * (i + 1), rng.uniform(0, 1, size=(2 + i, 3))) for i in range(3)]
+    g = oracle.hkl_cell_members(cell, sp, 0.6, fsquarecut=1e-12)
+    p = oracle.hkl_spacegroup_members(1, cell, sp, 0.6, fsquarecut=1e-12)
+    assert len(g[2]) == len(p[2]) > 50
+    og = np.lexsort((g[3], g[2]))
+    op = np.lexsort((p[3], p[2]))
+    assert np.max(np.abs(g[2][og] - p[2][op]) / p[2][op]) < 1e-12
+    assert np.max(np.abs(g[3][og] - p[3][op]) / p[3][op]) < 1e-9
+    assert np.all(g[1] == 2) and np.all(p[1] == 2)
+    ortho = (3.1, 3.7, 4.3)
+    a = oracle.hkl_cell_members(ortho + (90, 90, 90), sp, 0.6)
+    b = oracle.hkl_orthogonal_members(*ortho, sp, 0.6)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
