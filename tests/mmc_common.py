@@ -61,21 +61,30 @@ def _flat(h):
 
 def batched(run, n, k=20, key="total"):
     """k independent runs of n/k source neutrons (run(seed, n) -> MiniMC
-    result): the summed histogram and the overdispersion factor phi (variance
-    of the summed bin contents estimated from the spread of the batches,
-    over the sum of squared weights, averaged over the bins of at least 100
-    effective entries, at least 1).  Entries of one neutron's history are
-    correlated (successive steps fill the same bins of e.g. the energy
-    tally), so the sum of squared weights underestimates the variance of
+    result): the summed histogram and the overdispersion factor phi per bin
+    (under- and overflow first and last): the variance of a bin's summed
+    content estimated from the spread of the batches over its sum of squared
+    weights, as a running median over the neighbouring bins of at least 100
+    effective entries (within 3 bins), elsewhere the mean over all such bins,
+    at least 1.  Entries of one neutron's history are correlated (successive
+    steps fill the same bins of e.g. the energy tally, more in some bins than
+    in others), so the sum of squared weights underestimates the variance of
     those bins; a bin holding most of the weight (the unscattered energy) is
-    instead nearly deterministic and must not set the factor alone."""
+    instead nearly deterministic and must not set the factor of the others."""
     runs = [run(1000 + j, n // k)[key] for j in range(k)]
     C = np.array([_flat(r)[0] for r in runs])
     W2 = np.array([_flat(r)[1] for r in runs]).sum(axis=0)
     var = k * C.var(axis=0, ddof=1)
     tot = C.sum(axis=0)
     good = (W2 > 0) & (tot * tot >= 100.0 * np.where(W2 > 0, W2, 1.0))
-    phi = max(1.0, float(np.mean(var[good] / W2[good]))) if good.any() else 1.0
+    ratio = np.where(W2 > 0, var / np.where(W2 > 0, W2, 1.0), 0.0)
+    glob = max(1.0, float(np.mean(ratio[good]))) if good.any() else 1.0
+    phi = np.full(len(tot), glob)
+    idx = np.nonzero(good)[0]
+    for i in range(len(tot)):
+        near = idx[np.abs(idx - i) <= 3]
+        if len(near) >= 3:
+            phi[i] = max(1.0, float(np.median(ratio[near])))
     h = dict(content=tot[1:-1], errorsq=W2[1:-1], underflow=tot[0], underflow_errorsq=W2[0], overflow=tot[-1],
              overflow_errorsq=W2[-1])
     return h, phi
@@ -107,13 +116,16 @@ def merged(h1, h2, min_eff=10.0):
 
 
 def pvalue_phi(h1, h2, phi):
-    """pvalue with both histograms' variances scaled by phi, after merging
-    the bins sparse in the second one"""
+    """pvalue with both histograms' variances scaled by phi (a number, or
+    one per bin with under- and overflow first and last), then the bins
+    sparse in the second one merged"""
+    f = np.broadcast_to(np.asarray(phi, dtype=np.float64), (len(h1["content"]) + 2,))
+
     def scaled(h):
-        return dict(h, errorsq=h["errorsq"] * phi, underflow_errorsq=h["underflow_errorsq"] * phi,
-                    overflow_errorsq=h["overflow_errorsq"] * phi)
-    a, b = merged(h1, h2)
-    return pvalue(scaled(a), scaled(b))
+        return dict(h, errorsq=h["errorsq"] * f[1:-1], underflow_errorsq=h["underflow_errorsq"] * f[0],
+                    overflow_errorsq=h["overflow_errorsq"] * f[-1])
+    a, b = merged(scaled(h1), scaled(h2))
+    return pvalue(a, b)
 
 
 def library_al(temperature=293.15, absorption=True):

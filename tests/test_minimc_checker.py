@@ -73,6 +73,6 @@ def test_checker_tallies_and_roulette(oracle, case):
                                    roulette=roul)
     h, phi = M.batched(run, 100000)
     p = M.pvalue_phi(h, ref, phi)
-    print(f"{name}: integral {h['content'].sum():.1f} (reference {ref['content'].sum():.1f}), phi {phi:.2f}, "
+    print(f"{name}: integral {h['content'].sum():.1f} (reference {ref['content'].sum():.1f}), phi {np.mean(phi):.2f}-{np.max(phi):.2f}, "
           f"p {p:.3g}")
     assert p >= 1e-3
