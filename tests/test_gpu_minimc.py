@@ -14,6 +14,11 @@ import pymmc
 
 pytestmark = pytest.mark.gpu
 
+# about 30 statistical comparisons per run: a threshold of 1e-4 keeps the
+# chance of any false alarm near 0.3%, while a 10% error in the material's
+# temperature fails the energy tallies at p < 1e-28
+P_MIN = 1e-4
+
 
 @pytest.fixture(scope="module")
 def amd():
@@ -40,7 +45,7 @@ def test_matches_reference_histogram(amd, case):
     p = M.pvalue(res["total"], ref)
     print(f"{name}: integral {res['total']['content'].sum():.1f} (reference {ref['content'].sum():.1f}), "
           f"p-value {p:.3g}, {res['info']}")
-    assert p >= 1e-3
+    assert p >= P_MIN
     parts = sum(res[c]["content"] for c in core.MINIMC_CATEGORIES)
     assert np.allclose(parts, res["total"]["content"], rtol=1e-9, atol=1e-9)
 
@@ -77,7 +82,7 @@ def test_tallies_and_roulette_match_reference(amd, al_by_temperature, case):
     p = M.pvalue_phi(h, ref, phi)
     print(f"{name}: integral {h['content'].sum():.1f} (reference {ref['content'].sum():.1f}), phi {np.mean(phi):.2f}-{np.max(phi):.2f}, "
           f"p {p:.3g}")
-    assert p >= 1e-3
+    assert p >= P_MIN
 
 
 def test_matches_checker_by_category(amd, oracle):
@@ -90,7 +95,7 @@ def test_matches_checker_by_category(amd, oracle):
     for key in ("total",) + tuple(core.MINIMC_CATEGORIES):
         p = M.pvalue(g[key], o[key])
         print(f"{key}: GPU {g[key]['content'].sum():.1f} checker {o[key]['content'].sum():.1f} p-value {p:.3g}")
-        assert p >= 1e-3, key
+        assert p >= P_MIN, key
         # the weight in each category agrees within 5 standard errors
         sg, so = g[key]["content"].sum(), o[key]["content"].sum()
         err = np.sqrt(g[key]["errorsq"].sum() + o[key]["errorsq"].sum())
@@ -120,7 +125,7 @@ def test_circular_beam_with_misses(amd, oracle):
     p = M.pvalue(g["total"], o["total"])
     miss = n * (1 - 1 / 1.5 ** 2)
     print("p-value", p, "first bin", g["total"]["content"][0], "misses expected", miss)
-    assert p >= 1e-3
+    assert p >= P_MIN
     assert g["total"]["content"][0] > miss
     gi = core.minimc_sphere(scatter, ab, nd, r, ekin, n, seed=9, ignoremiss=True, **kw)
     assert abs(g["total"]["content"][0] - gi["total"]["content"][0] - miss) < 5 * np.sqrt(miss)
@@ -137,7 +142,7 @@ def test_single_scattering_limit(amd, oracle):
     o = pymmc.minimc_sphere(cs, cab, nd, r, ekin, 200000, seed=4, nthreads=8, **kw)
     p = M.pvalue(g["total"], o["total"])
     print("nscat p-value", p)
-    assert p >= 1e-3
+    assert p >= P_MIN
 
 
 def test_argument_errors(amd):

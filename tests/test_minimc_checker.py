@@ -22,7 +22,7 @@ def test_checker_matches_reference_histogram(oracle, case):
     tot = res["total"]["content"].sum()
     print(f"{name}: sphere radius {r:.4f} m, integral {tot:.1f} (reference {ref['content'].sum():.1f}), "
           f"p-value {p:.3g}, steps {res['info']['max_steps']}")
-    assert p >= 1e-3
+    assert p >= 1e-4  # many comparisons per run (family-wise rate near 0.3%)
     # the breakdown adds up to the total
     parts = sum(res[c]["content"] for c in pymmc.CATEGORIES)
     assert np.allclose(parts, res["total"]["content"], rtol=1e-12, atol=1e-12)
@@ -75,4 +75,4 @@ def test_checker_tallies_and_roulette(oracle, case):
     p = M.pvalue_phi(h, ref, phi)
     print(f"{name}: integral {h['content'].sum():.1f} (reference {ref['content'].sum():.1f}), phi {np.mean(phi):.2f}-{np.max(phi):.2f}, "
           f"p {p:.3g}")
-    assert p >= 1e-3
+    assert p >= 1e-4  # many comparisons per run (family-wise rate near 0.3%)
